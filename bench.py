@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark of the numeric multifrontal LDLᵀ on one MI355X: factor + solve to a backward error of 1e-10, mixed
+(FP32 fronts on MFMA + FP64 iterative refinement) against the plain FP64 GPU factor.
+
+Workload: the 500 × 500 quad4 shell plate (1,494,006 DOF), interior nodes jittered ±20 % of the element size, element
+thickness log-uniform in [1, 4] mm, E = 70 GPa, ν = 0.3, clamped edges, 16 load cases of independent N(0, 1) forces on
+the translational DOFs (rotations 0); nested-dissection order and symbolic factor on the host (untimed), the matrix
+and the right-hand sides resident in device memory before the timed region.
+
+A step is one factorisation and one refined solve of the 16 right-hand sides. One JSON line on stdout; progress on
+stderr.
+
+    python bench.py [--steps K] [--warmup W] [--scale 500] [--nrhs 16]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "factor+solve s at 1e-10 backward err: mixed FP32+IR vs FP64 GPU; % MFMA peak"
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
+
+
+def log(message):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(workload, b, threads):
+    """The reference's own FP64 factor (AVX2 kernel family, the family it selects itself) and solve on the host
+    cores, on the same symbolic structure and matrix: oracle/_ref, timed as one factorisation + one solve of the
+    16 right-hand sides. Returns the cpu_baseline object, or None when the compiled reference is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    if _oracle.reference() is None:
+        return None
+    import numpy as np
+    h, sizes, flops, arr = _oracle.ref_symbolic(workload.graph_offsets, workload.adjacency, workload.weights,
+                                                order=workload.symbolic.node_order.copy())
+    try:
+        assert np.array_equal(arr["rows"], workload.symbolic.rows_all), "the reference's structure differs"
+        x = np.array(b, order="F", copy=True)
+        rc, msg, inertia, counts, times, _, _ = _oracle.ref_factor_solve(h, workload.matrix, b=x, kernel=1,
+                                                                         threads=threads)
+    finally:
+        _oracle.reference().ref_sym_free(h)
+    if rc != 0:
+        raise RuntimeError("reference factorisation failed: " + msg)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    total = float(times[0] + times[1])
+    return {"value": round(total, 4), "unit": "s", "cores": threads, "kind": "reference",
+            "factor_s": round(float(times[0]), 4), "solve_s": round(float(times[1]), 4),
+            "gflops_factor": round(flops / times[0] / 1e9, 1), "cpu": model,
+            "hardware_concurrency": int(_oracle.reference().ref_hardware_threads()),
+            "sample": "the whole workload: one FP64 factorisation (SupernodalFactor::factorise, avx2 family, "
+                      f"{threads} threads) and one solve of the {b.shape[1]} right-hand sides (FactorSolve::solve)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=500, help="elements per side of the plate")
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--target", type=float, default=1e-10)
+    ap.add_argument("--max-steps", type=int, default=30, help="refinement steps of the mixed path at most")
+    ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the host's CPU share, at most 16")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the factor of one matrix on one MI355X); "
+              "run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import o_o_solver_amd as oos
+    from o_o_solver_amd import FactorOptions, FactorSolve, NotConvergedError, Precision, SupernodalFactor, models
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU", file=sys.stderr)
+        sys.exit(1)
+    device = torch.device("cuda:0")
+
+    t = time.perf_counter()
+    workload = models.shell_plate_c2(args.scale)
+    log(f"model {workload.name}: {workload.n} DOF ({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    sym = workload.analyse()
+    log(f"order + symbolic: {sym.supernode_count} supernodes, {sym.flop_count:.3e} flops "
+        f"({time.perf_counter() - t:.1f} s)")
+    b = workload.loads("random", nrhs=args.nrhs, seed=20261203)
+    t = time.perf_counter()
+    plan = sym.plan()
+    dm = plan.matrix(workload.matrix)
+    bd = torch.tensor(b, device=device)
+    torch.cuda.synchronize()
+    stats = plan.stats()
+    log(f"plan + upload ({time.perf_counter() - t:.1f} s): {stats}")
+
+    def mixed_step():
+        f = SupernodalFactor.factorise(sym, workload.matrix, FactorOptions(precision=Precision.FP32),
+                                       device_matrix=dm)
+        try:
+            x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=args.max_steps)
+            fallback = False
+        except NotConvergedError:
+            # The FP32 factor is not accurate enough: the FP64 factor decides.
+            del f
+            f = SupernodalFactor.factorise(sym, workload.matrix, device_matrix=dm)
+            x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=4)
+            fallback = True
+        return f.factor_seconds, rep, fallback
+
+    def fp64_step():
+        f = SupernodalFactor.factorise(sym, workload.matrix, device_matrix=dm)
+        x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=4)
+        return f.factor_seconds, rep, False
+
+    def run(step, label, profile):
+        for k in range(args.warmup):
+            fs, rep, fb = step()
+            log(f"{label} warmup {k}: factor {fs * 1e3:.1f} ms, {rep.steps} solves, eta {rep.backward_error:.2e}"
+                f"{' (fp64 fallback)' if fb else ''}")
+        plan.profile(profile)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        results = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = plan.profile_read() if profile else None
+        plan.profile(False)
+        fs = [r[0] for r in results]
+        reps = [r[1] for r in results]
+        log(f"{label}: {elapsed / args.steps * 1e3:.1f} ms/step, factor {np.mean(fs) * 1e3:.1f} ms, "
+            f"eta {max(r.backward_error for r in reps):.2e}, solves {[r.steps for r in reps]}")
+        return elapsed / args.steps, float(np.mean(fs)), reps, any(r[2] for r in results), prof
+
+    fp64_s, fp64_factor, fp64_reps, _, fp64_prof = run(fp64_step, "fp64", True)
+    mixed_s, mixed_factor, mixed_reps, fallback, mixed_prof = run(mixed_step, "mixed", True)
+
+    # The roofline of the kernel that takes the most device time in the headline (mixed) path.
+    prof = mixed_prof
+    dominant = max(prof, key=lambda k: prof[k][0])
+    seconds, launches, work = prof[dominant]
+    if dominant in ("forward_sweep", "backward_sweep", "residual"):
+        achieved = work / seconds / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
+                    "frac": round(achieved / PEAK_HBM, 4), "traffic": None}
+    else:
+        peak = PEAK_FP64 if fallback else PEAK_FP32
+        achieved = work / seconds / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(achieved / peak, 4), "traffic": None}
+    roofline.update({"kernel": dominant, "launches_per_step": launches / args.steps,
+                     "avg_launch_ms": round(seconds / launches * 1e3, 4),
+                     "work_per_launch": work / launches,
+                     "share_of_device_time": round(seconds / sum(v[0] for v in prof.values()), 3)})
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps * 1e3, 3), "launches_per_step": v[1] / args.steps,
+                   "rate": round(v[2] / v[0] / (1e9 if k in ("forward_sweep", "backward_sweep", "residual")
+                                                else 1e12), 2) if v[0] > 0 else None}
+               for k, v in prof.items()}
+
+    baseline = None
+    if args.cpu_baseline == "auto":
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline: the reference's avx2 factor + solve on {threads} threads")
+        baseline = cpu_baseline(workload, b, threads)
+        if baseline:
+            log(f"cpu baseline: {baseline['value']} s ({baseline['gflops_factor']} GFLOP/s factor)")
+
+    flops = sym.flop_count
+    out = {
+        "metric": METRIC,
+        "value": round(mixed_s, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_s * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"shell plate {args.scale}x{args.scale} quad4, {workload.n} DOF, jitter 0.2, "
+                               f"t log-uniform [1,4] mm, clamped; {args.nrhs} N(0,1) load cases",
+                   "nrhs": args.nrhs, "target_backward_error": args.target, "flops": flops,
+                   "supernodes": sym.supernode_count, "levels": stats["levels"]},
+        "fp64": {"seconds": round(fp64_s, 5), "factor_s": round(fp64_factor, 5),
+                 "solve_s": round(float(np.mean([r.seconds for r in fp64_reps])), 5),
+                 "solves": max(r.steps for r in fp64_reps),
+                 "backward_error": max(r.backward_error for r in fp64_reps),
+                 "factor_tflops": round(flops / fp64_factor / 1e12, 2),
+                 "mfma_frac": round(flops / fp64_factor / 1e12 / PEAK_FP64, 4)},
+        "mixed": {"seconds": round(mixed_s, 5), "factor_s": round(mixed_factor, 5),
+                  "solve_s": round(float(np.mean([r.seconds for r in mixed_reps])), 5),
+                  "solves": max(r.steps for r in mixed_reps),
+                  "backward_error": max(r.backward_error for r in mixed_reps),
+                  "fp64_fallback": fallback,
+                  "factor_tflops": round(flops / mixed_factor / 1e12, 2),
+                  "mfma_frac": round(flops / mixed_factor / 1e12 / PEAK_FP32, 4)},
+        "speedup_mixed_vs_fp64": round(fp64_s / mixed_s, 3),
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

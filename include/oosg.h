@@ -1,0 +1,202 @@
+/*
+ * oosg — the numeric phase of the supernodal multifrontal LDLᵀ (factor and solve) on one AMD Instinct MI355X
+ * (gfx950), behind a plain C ABI: plain pointers and sizes, status codes, no exceptions across the boundary.
+ *
+ * It replaces, for a caller of jcdulas/O_o_Solver, the numeric phase behind:
+ *   SupernodalFactor::factorise      include/oos/linalg/supernodal/SupernodalFactor.hpp:176-183
+ *   SupernodalFactor accessors       include/oos/linalg/supernodal/SupernodalFactor.hpp:192-233
+ *   FactorSolve::solve               include/oos/linalg/supernodal/FactorSolve.hpp:77-81
+ *   SymmetricCsc (the input)         include/oos/linalg/sparse/SymmetricCsc.hpp:14-108
+ * The ordering and the symbolic analysis stay on the host (SymbolicFactor::analyse,
+ * include/oos/linalg/symbolic/SymbolicFactor.hpp:47-50): their result is this library's structural input.
+ *
+ * Status codes map onto the reference's exceptions (include/oos/base/errors/): OOSG_INVALID_ARGUMENT is
+ * std::invalid_argument, OOSG_OUT_OF_MEMORY is MemoryBudgetError (raised before any numeric work), OOSG_SINGULAR is
+ * SingularityError, OOSG_INTERNAL is InternalError. oosg_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Ownership: the caller owns every array it passes; the library copies what it keeps. Handles own device memory.
+ * Threading: a plan and the matrices and factors made with it belong to one device and one stream; calls on one plan
+ * are serialised by the caller (distinct plans may be used from distinct threads). A factor is immutable once made.
+ */
+#ifndef OOSG_H
+#define OOSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum oosg_status {
+    OOSG_OK = 0,
+    OOSG_INVALID_ARGUMENT = 1,
+    OOSG_OUT_OF_MEMORY = 2,
+    OOSG_SINGULAR = 3,
+    OOSG_CANCELLED = 4,
+    OOSG_INTERNAL = 5,
+    OOSG_HIP_ERROR = 6,
+    OOSG_NOT_CONVERGED = 7
+} oosg_status;
+
+/* The symbolic factor, factor order (SymbolicFactor, include/oos/linalg/symbolic/SymbolicFactor.hpp:37-125):
+ * supernode s owns the columns [first_column[s], first_column[s] + column_count[s]), its rows below them are
+ * rows[row_start[s] .. row_start[s + 1]) (ascending), parent[s] > s or -1 at a root; supernodes in postorder. */
+typedef struct oosg_symbolic_desc {
+    int32_t n;
+    int32_t supernode_count;
+    const int32_t *first_column;
+    const int32_t *column_count;
+    const int32_t *parent;
+    const int64_t *row_start;
+    const int32_t *rows;
+} oosg_symbolic_desc;
+
+/* A symmetric matrix as its lower triangle by columns, each column's rows ascending from its diagonal, structural
+ * zeros kept (SymmetricCsc, include/oos/linalg/sparse/SymmetricCsc.hpp:14-20). Host pointers. */
+typedef struct oosg_csc_desc {
+    int32_t n;
+    const int64_t *offsets;
+    const int32_t *rows;
+    const double *values;
+} oosg_csc_desc;
+
+typedef enum oosg_pivoting {
+    OOSG_POSITIVE_DEFINITE = 0, /* PivotingMode::PositiveDefinite (SupernodalFactor.hpp:34-42) */
+    OOSG_INDEFINITE = 1         /* PivotingMode::Indefinite: Bunch-Kaufman fronts, delayed pivots */
+} oosg_pivoting;
+
+typedef enum oosg_precision {
+    OOSG_FP64 = 0, /* every front in double precision */
+    OOSG_FP32 = 1  /* fronts in single precision on the diagonally scaled matrix; solves refine in double */
+} oosg_precision;
+
+typedef struct oosg_factor_options {
+    int32_t pivoting;  /* oosg_pivoting */
+    int32_t precision; /* oosg_precision */
+    double max_ratio;  /* PARAM MaxDiagonalRatio (PivotPolicy, PivotDiagnostics.hpp:34-44); 0: 1e7 */
+    int32_t bailout;   /* PARAM SingularityBailOut: < 0 keeps a factor that fails the ratio test */
+} oosg_factor_options;
+
+/* The inertia, pivot counts and ratio test of a factor (SupernodalFactor::inertia, pivotDiagnostics,
+ * SupernodalFactor.hpp:192-205; MultifrontalDriver::finish, src/linalg/supernodal/MultifrontalDriver.cpp:704-820). */
+typedef struct oosg_factor_info {
+    int64_t positive, negative, zero;
+    int64_t delayed_pivots;
+    int64_t two_by_two_pivots;
+    int64_t issue_count;          /* pivots failing the ratio test */
+    int32_t first_issue_equation; /* -1 when none */
+    int32_t first_issue_negative; /* 1: a negative pivot */
+    double largest_ratio;
+    int32_t largest_ratio_equation;
+    int32_t precision;
+    double factor_seconds;        /* device time of the numeric factorisation */
+    double flops;                 /* SymbolicFactor::flopCount() of the structure */
+} oosg_factor_info;
+
+/* What a refined solve did. */
+typedef struct oosg_solve_report {
+    int32_t steps;             /* solves with the factor */
+    int32_t converged;         /* 1: the backward error reached the target */
+    double backward_error;     /* max over columns of ‖b − A·x‖∞ / (‖A‖∞·‖x‖∞ + ‖b‖∞) */
+    double seconds;            /* device time of the solve */
+} oosg_solve_report;
+
+typedef struct oosg_plan oosg_plan;
+typedef struct oosg_matrix oosg_matrix;
+typedef struct oosg_factor oosg_factor;
+
+/* The message of the last failure on this thread. */
+const char *oosg_last_error(void);
+
+/* The plan of the numeric phase for one symbolic factor on `device`: the supernode tree by levels, the relative
+ * indices of the extend-add, the schedule of the kernels and the device memory (panels, update matrices), all
+ * reserved before any numeric work (the memory-budget contract of SupernodalFactor::factorise,
+ * SupernodalFactor.hpp:130-135). hbm_budget_bytes = 0: the device's free memory. A plan serves every factorisation of
+ * matrices of its structure. */
+oosg_status oosg_plan_create(const oosg_symbolic_desc *symbolic, int32_t device, uint64_t hbm_budget_bytes,
+                             oosg_plan **out);
+void oosg_plan_destroy(oosg_plan *plan);
+/* {supernodes, levels, panel values, update values, small fronts, large fronts, kernel launches per factorisation,
+ * MFMA update tiles}: 8 values. */
+oosg_status oosg_plan_stats(const oosg_plan *plan, int64_t *stats);
+
+/* A matrix of the plan's structure copied into device memory (SymmetricCsc::fromCompressedColumns,
+ * SymmetricCsc.hpp:66-72). OOSG_INVALID_ARGUMENT when an entry lies outside the structure of the symbolic factor
+ * (the test of src/linalg/supernodal/FrontAssembly.cpp:22-28). */
+oosg_status oosg_matrix_create(oosg_plan *plan, const oosg_csc_desc *matrix, oosg_matrix **out);
+/* New values for the same structure (the shifted matrices K − σB of an eigenvalue solver). */
+oosg_status oosg_matrix_set_values(oosg_matrix *matrix, const double *values);
+void oosg_matrix_destroy(oosg_matrix *matrix);
+
+/* The numeric factorisation (SupernodalFactor::factorise). OOSG_SINGULAR for a zero or non-finite pivot in positive
+ * definite mode, or when the ratio test fails with bailout >= 0; the message names the equations. */
+oosg_status oosg_factorise(oosg_plan *plan, const oosg_matrix *matrix, const oosg_factor_options *options,
+                           oosg_factor **out);
+void oosg_factor_destroy(oosg_factor *factor);
+oosg_status oosg_factor_info_get(const oosg_factor *factor, oosg_factor_info *info);
+/* The panel of supernode s in the reference's layout (FactorPanel, SupernodalFactor.hpp:110-125): values
+ * (c + r) × e column-major, L below the diagonal, D on it, zero above; rows[k] the equation of row k. Either output
+ * may be null; *eliminated receives e. For tests and diagnostics. */
+oosg_status oosg_factor_panel(const oosg_factor *factor, int32_t supernode, double *values, int32_t *rows,
+                              int32_t *eliminated);
+/* D of each equation (n values). */
+oosg_status oosg_factor_diagonal(const oosg_factor *factor, double *d);
+
+/* X ← A⁻¹·B in place (FactorSolve::solve, FactorSolve.hpp:77-81): b is n × nrhs column-major, leading dimension
+ * ldb >= n, host memory. With an FP32 factor the solve applies the factor once (no refinement). */
+oosg_status oosg_solve(const oosg_factor *factor, double *b, int64_t ldb, int64_t nrhs);
+/* The same on device memory: b_device is n × nrhs column-major (leading dimension ldb) in the plan's device memory. */
+oosg_status oosg_solve_device(const oosg_factor *factor, double *b_device, int64_t ldb, int64_t nrhs);
+/* x = A⁻¹·b by iterative refinement in double precision until the backward error is at most `target` (or
+ * max_steps solves): x_device ← 0, then d = factor⁻¹·r, x += d, r = b − A·x with `matrix` (the matrix factorised).
+ * b_device and x_device are n × nrhs column-major device arrays (leading dimension ld). OOSG_NOT_CONVERGED when the
+ * target is not met (the report says how far it got). */
+oosg_status oosg_solve_refined_device(const oosg_factor *factor, const oosg_matrix *matrix, const double *b_device,
+                                      double *x_device, int64_t ld, int64_t nrhs, double target, int32_t max_steps,
+                                      oosg_solve_report *report);
+/* y = A·x for n × nrhs column-major device arrays (SymmetricCsc::multiply, SymmetricCsc.hpp:86-96). */
+oosg_status oosg_matrix_multiply_device(const oosg_matrix *matrix, const double *x_device, double *y_device,
+                                        int64_t ld, int64_t nrhs);
+
+/* Diagnostic: one 16 × 16 × 4 MFMA product in FP64 and in FP32 (A row-major 16 × 4, B row-major 4 × 16, D row-major
+ * 16 × 16, host arrays), run on the current device: the check of the lane layout the update tiles rely on. */
+oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, const float *af, const float *bf,
+                                  float *df);
+
+/* Per-kernel device times, by HIP events recorded on the plan's stream around every launch while enabled
+ * (enabling resets the sums). Kinds, in order: 0 small fronts (fused assembly and elimination), 1 assembly of large
+ * fronts, 2 diagonal blocks, 3 panel solves (rows below a diagonal block), 4 panel updates (MFMA tiles), 5 update
+ * matrices (MFMA tiles), 6 forward sweeps, 7 backward sweeps, 8 residual products. work[k] is the algorithmic work of
+ * the launches timed: flops for kinds 0-5, bytes for kinds 6-8 (the panels a sweep reads; the matrix and vectors of a
+ * residual). Arrays of 9 values; any may be null. */
+oosg_status oosg_plan_profile(oosg_plan *plan, int32_t enable);
+oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64_t *launches, double *work);
+
+/* Blocks until the plan's stream is idle. */
+oosg_status oosg_plan_synchronize(oosg_plan *plan);
+
+/* ---- host analysis (the reference keeps these on the host too) --------------------------------------------- */
+
+/* Geometric nested dissection of a node graph (adjacency without self loops, both directions) on 3 coordinates per
+ * node: out_order[p] = the node at position p (the role of ConstrainedOrdering::order,
+ * include/oos/linalg/ordering/ConstrainedOrdering.hpp:41-60; this project's own dissection). Returns 0 on success. */
+int oosa_order_dissection(int32_t n, const int64_t *offsets, const int32_t *adjacency, const int32_t *weights,
+                          const double *coords, int32_t leaf_nodes, int32_t *out_order);
+/* The symbolic analysis (SymbolicFactor::analyse, include/oos/linalg/symbolic/SymbolicFactor.hpp:47-50): the same
+ * structure as the reference for the same graph, weights and order. Returns a handle, null on failure. */
+void *oosa_analyse(int32_t n, const int64_t *offsets, const int32_t *adjacency, const int32_t *weights,
+                   const int32_t *order, int32_t trailing, int amalgamate);
+void oosa_free(void *symbolic);
+/* {nodes, DOFs, supernodes, row total, panel values, factor entries, exact entries}, and the flop count. */
+void oosa_sizes(void *symbolic, int64_t *sizes, double *flops);
+void oosa_export(void *symbolic, int32_t *node_order, int32_t *first_column, int32_t *column_count, int32_t *parent,
+                 int64_t *row_start, int32_t *rows);
+void oosa_memory(void *symbolic, int64_t *values);
+const char *oosa_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,461 @@
+"""O_o_Solver's numeric multifrontal LDLᵀ (factor and solve) on an AMD Instinct MI355X.
+
+The host-side mirror of the reference's interface for this path (jcdulas/O_o_Solver, include/oos/linalg):
+
+    SymmetricCsc          include/oos/linalg/sparse/SymmetricCsc.hpp:21-108
+    SymbolicFactor        include/oos/linalg/symbolic/SymbolicFactor.hpp:37-125 (analysis on the host)
+    SupernodalFactor      include/oos/linalg/supernodal/SupernodalFactor.hpp:166-294 (numeric phase on the GPU)
+    FactorSolve           include/oos/linalg/supernodal/FactorSolve.hpp:59-87
+    FactorOptions, PivotingMode, PivotPolicy, Inertia, PivotDiagnostics
+
+Errors mirror the reference's exception taxonomy: ValueError for std::invalid_argument, MemoryBudgetError,
+SingularityError, InternalError; HipError for a failure of the HIP runtime; NotConvergedError when a refined solve
+misses its target. All numeric work runs in liboosg.so (hand-written HIP for gfx950); nothing here computes.
+"""
+
+import ctypes
+import enum
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+__all__ = ["SymmetricCsc", "SymbolicFactor", "SupernodalFactor", "FactorSolve", "FactorOptions", "PivotingMode",
+           "Precision", "PivotPolicy", "Inertia", "PivotDiagnostics", "SolveReport", "Plan", "MemoryBudgetError",
+           "SingularityError", "InternalError", "HipError", "NotConvergedError", "nested_dissection"]
+
+
+class MemoryBudgetError(MemoryError):
+    """The factorisation does not fit the device memory (refused before any numeric work)."""
+
+
+class SingularityError(RuntimeError):
+    """A zero or non-finite pivot in positive definite mode, or a failed ratio test that stops the run."""
+
+
+class InternalError(RuntimeError):
+    pass
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class NotConvergedError(RuntimeError):
+    pass
+
+
+_ERRORS = {1: ValueError, 2: MemoryBudgetError, 3: SingularityError, 4: RuntimeError, 5: InternalError, 6: HipError,
+           7: NotConvergedError}
+
+
+def _check(status):
+    if status != 0:
+        message = _native.lib().oosg_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, RuntimeError)(message)
+
+
+def _ptr(array):
+    return array.ctypes.data_as(ctypes.c_void_p) if array is not None else None
+
+
+class PivotingMode(enum.IntEnum):
+    POSITIVE_DEFINITE = 0
+    INDEFINITE = 1
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32 = 1
+
+
+@dataclass
+class PivotPolicy:
+    """PARAM MaxDiagonalRatio and SingularityBailOut (include/oos/linalg/supernodal/PivotDiagnostics.hpp:34-44)."""
+    max_ratio: float = 1.0e7
+    bailout: int = 0
+
+
+@dataclass
+class FactorOptions:
+    """FactorOptions (SupernodalFactor.hpp:63-97): the pivoting mode and the pivot policy; `precision` chooses the
+    storage of the fronts (FP32 fronts are factorised on the diagonally scaled matrix and refined in FP64)."""
+    pivoting: PivotingMode = PivotingMode.POSITIVE_DEFINITE
+    pivot_policy: PivotPolicy = field(default_factory=PivotPolicy)
+    precision: Precision = Precision.FP64
+
+
+@dataclass(frozen=True)
+class Inertia:
+    positive: int
+    negative: int
+    zero: int
+
+
+@dataclass(frozen=True)
+class PivotDiagnostics:
+    issue_count: int
+    first_issue_equation: int
+    first_issue_negative: bool
+    largest_ratio: float
+    largest_ratio_equation: int
+
+
+@dataclass(frozen=True)
+class SolveReport:
+    steps: int
+    converged: bool
+    backward_error: float
+    seconds: float
+
+
+class SymmetricCsc:
+    """The lower triangle by columns, int64 offsets, int32 rows, float64 values (SymmetricCsc.hpp:14-20)."""
+
+    def __init__(self, n, offsets, rows, values):
+        self.n = int(n)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int32)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        if self.offsets.shape != (self.n + 1,) or self.rows.shape != self.values.shape or \
+                self.offsets[-1] != self.rows.shape[0]:
+            raise ValueError("SymmetricCsc: offsets of n + 1 values and one row per value expected")
+
+    @classmethod
+    def from_compressed_columns(cls, n, offsets, rows, values):
+        return cls(n, offsets, rows, values)
+
+    @property
+    def non_zero_count(self):
+        return int(self.rows.shape[0])
+
+    def desc(self):
+        return _native.CscDesc(self.n, _ptr(self.offsets), _ptr(self.rows), _ptr(self.values))
+
+    def to_dense(self):
+        """The full matrix (tests, small sizes)."""
+        dense = np.zeros((self.n, self.n))
+        cols = np.repeat(np.arange(self.n), np.diff(self.offsets))
+        dense[self.rows, cols] = self.values
+        dense[cols, self.rows] = self.values
+        return dense
+
+
+class SymbolicFactor:
+    """The symbolic factor of a node graph under an order (SymbolicFactor::analyse): the final (postordered) node
+    order, the supernodes, their rows, sizes and flops. Computed on the host by liboosg.so's analysis, which gives
+    the reference's structure for the same graph, weights and order."""
+
+    def __init__(self, n, first_column, column_count, parent, row_start, rows, node_order=None, flop_count=None,
+                 panel_value_count=None, factor_entry_count=None):
+        self.n = int(n)
+        self.first_column = np.ascontiguousarray(first_column, dtype=np.int32)
+        self.column_count = np.ascontiguousarray(column_count, dtype=np.int32)
+        self.parent = np.ascontiguousarray(parent, dtype=np.int32)
+        self.row_start = np.ascontiguousarray(row_start, dtype=np.int64)
+        self.rows_all = np.ascontiguousarray(rows, dtype=np.int32)
+        self.node_order = None if node_order is None else np.ascontiguousarray(node_order, dtype=np.int32)
+        c = self.column_count.astype(np.int64)
+        r = np.diff(self.row_start)
+        self.panel_value_count = int(np.sum(c * (c + r))) if panel_value_count is None else panel_value_count
+        self.factor_entry_count = int(np.sum(c * (c + 1) // 2 + c * r)) if factor_entry_count is None \
+            else factor_entry_count
+        if flop_count is None:
+            flop_count = 0.0
+            for cc, rr in zip(c.tolist(), r.tolist()):
+                m = np.arange(rr, rr + cc, dtype=np.float64)
+                flop_count += float(np.sum(m * (m + 1) + 2 * m))
+        self.flop_count = flop_count
+        self._plans = {}
+
+    @classmethod
+    def analyse(cls, graph_offsets, adjacency, node_weights, order, trailing_node_count=0, amalgamate=True):
+        lib = _native.lib()
+        graph_offsets = np.ascontiguousarray(graph_offsets, dtype=np.int64)
+        adjacency = np.ascontiguousarray(adjacency, dtype=np.int32)
+        node_weights = np.ascontiguousarray(node_weights, dtype=np.int32)
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        nodes = int(graph_offsets.shape[0] - 1)
+        if node_weights.shape != (nodes,) or order.shape != (nodes,):
+            raise ValueError("SymbolicFactor: expected an ordering and one weight per node")
+        handle = lib.oosa_analyse(nodes, _ptr(graph_offsets), _ptr(adjacency), _ptr(node_weights), _ptr(order),
+                                  int(trailing_node_count), 1 if amalgamate else 0)
+        if not handle:
+            raise ValueError(lib.oosa_last_error().decode())
+        try:
+            sizes = np.zeros(7, np.int64)
+            flops = ctypes.c_double()
+            lib.oosa_sizes(handle, _ptr(sizes), ctypes.byref(flops))
+            S = int(sizes[2])
+            node_order = np.zeros(nodes, np.int32)
+            fc, cc, pa = np.zeros(S, np.int32), np.zeros(S, np.int32), np.zeros(S, np.int32)
+            rs, rows = np.zeros(S + 1, np.int64), np.zeros(int(sizes[3]), np.int32)
+            lib.oosa_export(handle, _ptr(node_order), _ptr(fc), _ptr(cc), _ptr(pa), _ptr(rs), _ptr(rows))
+        finally:
+            lib.oosa_free(handle)
+        return cls(int(sizes[1]), fc, cc, pa, rs, rows, node_order=node_order, flop_count=flops.value,
+                   panel_value_count=int(sizes[4]), factor_entry_count=int(sizes[5]))
+
+    @property
+    def supernode_count(self):
+        return int(self.first_column.shape[0])
+
+    @property
+    def dof_count(self):
+        return self.n
+
+    def rows(self, s):
+        return self.rows_all[self.row_start[s]:self.row_start[s + 1]]
+
+    def row_count(self, s):
+        return int(self.row_start[s + 1] - self.row_start[s])
+
+    def desc(self):
+        return _native.SymbolicDesc(self.n, self.supernode_count, _ptr(self.first_column), _ptr(self.column_count),
+                                    _ptr(self.parent), _ptr(self.row_start), _ptr(self.rows_all))
+
+    def plan(self, device=0, hbm_budget_bytes=0):
+        """The device plan of this structure (kept: one per device and budget)."""
+        key = (device, hbm_budget_bytes)
+        if key not in self._plans:
+            self._plans[key] = Plan(self, device, hbm_budget_bytes)
+        return self._plans[key]
+
+
+class Plan:
+    """oosg_plan: levels, relative indices, the kernel schedule and the device memory of one symbolic factor."""
+
+    def __init__(self, symbolic, device=0, hbm_budget_bytes=0):
+        self._lib = _native.lib()
+        self.symbolic = symbolic
+        handle = ctypes.c_void_p()
+        desc = symbolic.desc()
+        _check(self._lib.oosg_plan_create(ctypes.byref(desc), int(device), int(hbm_budget_bytes),
+                                          ctypes.byref(handle)))
+        self.handle = handle
+        self._matrices = {}
+
+    def stats(self):
+        s = np.zeros(8, np.int64)
+        _check(self._lib.oosg_plan_stats(self.handle, _ptr(s)))
+        keys = ["supernodes", "levels", "panel_values", "update_values", "small_fronts", "large_fronts", "launches",
+                "tiles"]
+        return dict(zip(keys, s.tolist()))
+
+    PROFILE_KINDS = ["small_fronts", "assembly", "diagonal_blocks", "panel_solve", "panel_update_tiles",
+                     "update_matrix_tiles", "forward_sweep", "backward_sweep", "residual"]
+
+    def profile(self, enable=True):
+        """Per-kernel device times by HIP events on the plan's stream (enabling resets the sums)."""
+        _check(self._lib.oosg_plan_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        """{kind: (seconds, launches, work)}: work in flops for the factor kernels, in bytes for the sweeps and the
+        residuals."""
+        seconds = np.zeros(9)
+        launches = np.zeros(9, np.int64)
+        work = np.zeros(9)
+        _check(self._lib.oosg_plan_profile_read(self.handle, _ptr(seconds), _ptr(launches), _ptr(work)))
+        return {k: (float(seconds[i]), int(launches[i]), float(work[i])) for i, k in enumerate(self.PROFILE_KINDS)}
+
+    def matrix(self, matrix):
+        """The device copy of `matrix` (kept while `matrix` is alive and unchanged)."""
+        return DeviceMatrix(self, matrix)
+
+    def synchronize(self):
+        _check(self._lib.oosg_plan_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.oosg_plan_destroy(self.handle)
+            self.handle = None
+
+
+class DeviceMatrix:
+    """oosg_matrix: a SymmetricCsc in device memory, with its front positions and its row form for residuals."""
+
+    def __init__(self, plan, matrix):
+        self._lib = _native.lib()
+        self.plan = plan
+        self.matrix = matrix
+        handle = ctypes.c_void_p()
+        desc = matrix.desc()
+        _check(self._lib.oosg_matrix_create(plan.handle, ctypes.byref(desc), ctypes.byref(handle)))
+        self.handle = handle
+
+    def set_values(self, values):
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        if values.shape != self.matrix.values.shape:
+            raise ValueError("DeviceMatrix.set_values: one value per stored entry expected")
+        _check(self._lib.oosg_matrix_set_values(self.handle, _ptr(values)))
+        self.matrix = SymmetricCsc(self.matrix.n, self.matrix.offsets, self.matrix.rows, values)
+
+    def multiply(self, x):
+        """y = A·x for a torch tensor on the plan's device, n × k (column-major storage: x.T contiguous)."""
+        import torch
+        xs = _as_device_columns(x)
+        y = torch.empty_like(xs)
+        _check(self._lib.oosg_matrix_multiply_device(self.handle, xs.data_ptr(), y.data_ptr(), xs.shape[1],
+                                                     xs.shape[0]))
+        return y.T if x.dim() == 2 else y.reshape(-1)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.oosg_matrix_destroy(self.handle)
+            self.handle = None
+
+
+def _as_device_columns(x):
+    """A (k, n) contiguous float64 view of an n × k (or n) tensor: column j of x is row j."""
+    if x.dim() == 1:
+        return x.reshape(1, -1).contiguous().double()
+    return x.T.contiguous().double()
+
+
+class SupernodalFactor:
+    """The numeric factorisation of a SymmetricCsc in the order of its symbolic factor, on the GPU."""
+
+    def __init__(self, handle, plan, device_matrix, options):
+        self._lib = _native.lib()
+        self.handle = handle
+        self.plan = plan
+        self.device_matrix = device_matrix
+        self.options = options
+        info = _native.FactorInfoC()
+        _check(self._lib.oosg_factor_info_get(handle, ctypes.byref(info)))
+        self._info = info
+
+    @classmethod
+    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None):
+        """SupernodalFactor::factorise (SupernodalFactor.hpp:176-183). `matrix` is a SymmetricCsc in the factor's
+        order; `device_matrix` (optional) is its device copy, reused across factorisations."""
+        options = options or FactorOptions()
+        plan = symbolic.plan(device, hbm_budget_bytes)
+        if device_matrix is None:
+            device_matrix = plan.matrix(matrix)
+        opts = _native.FactorOptionsC(int(options.pivoting), int(options.precision),
+                                      float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout))
+        handle = ctypes.c_void_p()
+        lib = _native.lib()
+        _check(lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle)))
+        return cls(handle, plan, device_matrix, options)
+
+    @property
+    def dimension(self):
+        return self.plan.symbolic.n
+
+    @property
+    def inertia(self):
+        return Inertia(int(self._info.positive), int(self._info.negative), int(self._info.zero))
+
+    @property
+    def is_singular(self):
+        return self._info.zero > 0
+
+    @property
+    def delayed_pivot_count(self):
+        return int(self._info.delayed_pivots)
+
+    @property
+    def two_by_two_pivot_count(self):
+        return int(self._info.two_by_two_pivots)
+
+    @property
+    def pivot_diagnostics(self):
+        i = self._info
+        return PivotDiagnostics(int(i.issue_count), int(i.first_issue_equation), bool(i.first_issue_negative),
+                                float(i.largest_ratio), int(i.largest_ratio_equation))
+
+    @property
+    def factor_seconds(self):
+        return float(self._info.factor_seconds)
+
+    @property
+    def flop_count(self):
+        return float(self._info.flops)
+
+    @property
+    def precision(self):
+        return Precision(int(self._info.precision))
+
+    def panel(self, s):
+        """(values (c + r) × e, rows) of supernode s, in the reference's FactorPanel layout."""
+        sym = self.plan.symbolic
+        c = int(sym.column_count[s])
+        m = c + sym.row_count(s)
+        values = np.zeros(m * c)
+        rows = np.zeros(m, np.int32)
+        eliminated = ctypes.c_int32()
+        _check(self._lib.oosg_factor_panel(self.handle, int(s), _ptr(values), _ptr(rows), ctypes.byref(eliminated)))
+        return values.reshape(c, m).T, rows
+
+    def diagonal(self):
+        d = np.zeros(self.dimension)
+        _check(self._lib.oosg_factor_diagonal(self.handle, _ptr(d)))
+        return d
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.oosg_factor_destroy(self.handle)
+            self.handle = None
+
+
+class FactorSolve:
+    """FactorSolve (FactorSolve.hpp:59-87): solves with a factor, in place."""
+
+    def __init__(self, factor):
+        self.factor = factor
+        self._lib = _native.lib()
+
+    def solve(self, b):
+        """X = A⁻¹·B. `b`: a numpy array (n or n × k; solved in place when it is Fortran-ordered float64, a solved
+        copy is returned otherwise) or a torch tensor on the GPU (a solved copy is returned)."""
+        n = self.factor.dimension
+        if hasattr(b, "data_ptr"):
+            cols = _as_device_columns(b)
+            if cols.shape[1] != n:
+                raise ValueError(f"FactorSolve::solve: the right-hand sides have {cols.shape[1]} rows, the factor {n}")
+            _check(self._lib.oosg_solve_device(self.factor.handle, cols.data_ptr(), n, cols.shape[0]))
+            return cols.T if b.dim() == 2 else cols.reshape(-1)
+        arr = np.asarray(b)
+        vector = arr.ndim == 1
+        x = np.array(arr.reshape(n, -1) if not vector else arr.reshape(n, 1), dtype=np.float64, order="F", copy=True)
+        if x.shape[0] != n:
+            raise ValueError(f"FactorSolve::solve: the right-hand sides have {x.shape[0]} rows, the factor {n}")
+        _check(self._lib.oosg_solve(self.factor.handle, _ptr(x), n, x.shape[1]))
+        return x.reshape(-1) if vector else x
+
+    def solve_refined(self, b, target=1e-10, max_steps=10, x=None):
+        """x = A⁻¹·b by refinement in FP64 with the factorised matrix, to the backward error `target`. `b`: torch
+        tensor n × k (k <= 16) on the device. Returns (x, SolveReport); raises NotConvergedError when the target is
+        missed."""
+        import torch
+        bc = _as_device_columns(b)
+        xc = torch.empty_like(bc) if x is None else x
+        report = _native.SolveReportC()
+        status = self._lib.oosg_solve_refined_device(self.factor.handle, self.factor.device_matrix.handle,
+                                                     bc.data_ptr(), xc.data_ptr(), bc.shape[1], bc.shape[0],
+                                                     float(target), int(max_steps), ctypes.byref(report))
+        rep = SolveReport(int(report.steps), bool(report.converged), float(report.backward_error),
+                          float(report.seconds))
+        if status == 7:
+            raise NotConvergedError(f"{_native.lib().oosg_last_error().decode()} ({rep})")
+        _check(status)
+        return (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
+
+
+def nested_dissection(graph_offsets, adjacency, node_weights, coordinates, leaf_nodes=64):
+    """The node order of this project's geometric nested dissection (node at each position)."""
+    lib = _native.lib()
+    graph_offsets = np.ascontiguousarray(graph_offsets, dtype=np.int64)
+    adjacency = np.ascontiguousarray(adjacency, dtype=np.int32)
+    node_weights = np.ascontiguousarray(node_weights, dtype=np.int32)
+    coordinates = np.ascontiguousarray(coordinates, dtype=np.float64).reshape(-1)
+    n = int(graph_offsets.shape[0] - 1)
+    order = np.zeros(n, np.int32)
+    status = lib.oosa_order_dissection(n, _ptr(graph_offsets), _ptr(adjacency), _ptr(node_weights),
+                                       _ptr(coordinates), int(leaf_nodes), _ptr(order))
+    if status != 0:
+        raise ValueError(lib.oosa_last_error().decode())
+    return order

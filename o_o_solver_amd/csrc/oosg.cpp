@@ -1,0 +1,1433 @@
+// Host driver of the numeric multifrontal LDLᵀ on one GPU: the C ABI of include/oosg.h.
+//
+// The reference runs the supernodes of a subtree one after the other on a thread and the top of the tree in waves
+// (MultifrontalDriver::run, src/linalg/supernodal/MultifrontalDriver.cpp:651-702). Here the tree is cut into levels
+// by height (leaves first), and every front of a level is processed by the same few launches: the small fronts by one
+// fused kernel (one workgroup per front), the large ones by an assembly kernel, then panel steps of PANEL_BLOCK
+// columns (diagonal block, rows below it, MFMA trailing update) and one MFMA launch for their update matrices. All
+// launches of a factorisation are known from the symbolic structure alone and are listed once, in the plan.
+
+#include "kernels.h"
+
+#include "../../include/oosg.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace
+{
+
+thread_local std::string g_lastError;
+
+struct Failure
+{
+    oosg_status status;
+    std::string message;
+};
+
+[[noreturn]] void fail(oosg_status status, const std::string& message)
+{
+    throw Failure{status, message};
+}
+
+void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        fail(OOSG_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F> oosg_status guarded(F&& body)
+{
+    try
+    {
+        body();
+        return OOSG_OK;
+    }
+    catch (const Failure& f)
+    {
+        g_lastError = f.message;
+        return f.status;
+    }
+    catch (const std::bad_alloc&)
+    {
+        g_lastError = "host memory exhausted";
+        return OOSG_OUT_OF_MEMORY;
+    }
+    catch (const std::exception& e)
+    {
+        g_lastError = e.what();
+        return OOSG_INTERNAL;
+    }
+}
+
+// Device memory owned by a handle.
+struct DeviceBuffer
+{
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes)
+    {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        reset();
+        ptr = o.ptr;
+        bytes = o.bytes;
+        o.ptr = nullptr;
+        o.bytes = 0;
+        return *this;
+    }
+    ~DeviceBuffer() { reset(); }
+    void reset()
+    {
+        if (ptr != nullptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void allocate(size_t size, const char* what)
+    {
+        reset();
+        if (size == 0)
+            size = 8;
+        check(hipMalloc(&ptr, size), what);
+        bytes = size;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+template <typename T> DeviceBuffer upload(const std::vector<T>& host, hipStream_t stream, const char* what)
+{
+    DeviceBuffer b;
+    b.allocate(host.size() * sizeof(T), what);
+    if (!host.empty())
+        check(hipMemcpyAsync(b.ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream), what);
+    return b;
+}
+
+enum class LaunchKind
+{
+    Small,
+    Assemble,
+    Diagonal,
+    PanelSolve,
+    Tiles
+};
+
+// The kernels as the profile counts them.
+enum ProfileKind
+{
+    PROFILE_SMALL = 0,
+    PROFILE_ASSEMBLE,
+    PROFILE_DIAGONAL,
+    PROFILE_PANEL_SOLVE,
+    PROFILE_PANEL_UPDATE,
+    PROFILE_UPDATE_MATRIX,
+    PROFILE_FORWARD,
+    PROFILE_BACKWARD,
+    PROFILE_RESIDUAL,
+    PROFILE_KINDS
+};
+
+struct Launch
+{
+    LaunchKind kind;
+    int profile = PROFILE_SMALL;
+    int64_t offset = 0; // into the list of its kind
+    int count = 0;
+    int k0 = 0;
+    int maxOrder = 0;
+    double flops = 0.0; // algorithmic flops of the launch
+};
+
+// Per-launch device times by HIP events on the plan's stream, summed by kernel kind.
+struct Profiler
+{
+    bool enabled = false;
+    std::vector<hipEvent_t> events;
+    std::vector<std::pair<int, double>> pending; // (kind, work) of each recorded launch
+    double seconds[PROFILE_KINDS] = {};
+    double work[PROFILE_KINDS] = {}; // algorithmic flops (factor kernels) or bytes (sweeps, residuals)
+    int64_t launches[PROFILE_KINDS] = {};
+
+    ~Profiler()
+    {
+        for (hipEvent_t e : events)
+            (void)hipEventDestroy(e);
+    }
+    void reset()
+    {
+        pending.clear();
+        for (int k = 0; k < PROFILE_KINDS; ++k)
+        {
+            seconds[k] = work[k] = 0.0;
+            launches[k] = 0;
+        }
+    }
+    // Records the event before a launch; returns its slot (or -1 when disabled).
+    int begin(int kind, double launchWork, hipStream_t stream)
+    {
+        if (!enabled)
+            return -1;
+        const size_t slot = pending.size();
+        while (events.size() < 2 * (slot + 1))
+        {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess)
+                return -1;
+            events.push_back(e);
+        }
+        pending.emplace_back(kind, launchWork);
+        (void)hipEventRecord(events[2 * slot], stream);
+        return static_cast<int>(slot);
+    }
+    void end(int slot, hipStream_t stream)
+    {
+        if (slot >= 0)
+            (void)hipEventRecord(events[2 * slot + 1], stream);
+    }
+    // After the stream is synchronised.
+    void collect()
+    {
+        for (size_t k = 0; k < pending.size(); ++k)
+        {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]) != hipSuccess)
+                continue;
+            const int kind = pending[k].first;
+            seconds[kind] += ms * 1e-3;
+            work[kind] += pending[k].second;
+            ++launches[kind];
+        }
+        pending.clear();
+    }
+};
+
+double frontFlops(int64_t c, int64_t r)
+{
+    double f = 0.0;
+    for (int64_t m = r; m < r + c; ++m)
+        f += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
+    return f;
+}
+
+} // namespace
+
+struct oosg_plan
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    int32_t n = 0, S = 0;
+    std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
+    std::vector<int64_t> rowStart, panelOff, updateOff;
+    int32_t levels = 0;
+    int64_t smallFronts = 0, largeFronts = 0;
+    double flops = 0.0;
+
+    std::vector<Launch> schedule;
+    std::vector<int32_t> smallList, stepFronts;
+    std::vector<int2> chunks;
+    std::vector<oosg::AssemblyBlock> blocks;
+    std::vector<oosg::UpdateTile> tiles;
+
+    DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren;
+    DeviceBuffer dSmallList, dStepFronts, dChunks, dBlocks, dTiles, dLevelList;
+    oosg::DevicePlan dp{};
+
+    DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
+    DeviceBuffer panelCache[2]; // panels given back by destroyed factors, per precision
+    DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
+    uint64_t budget = 0;
+    std::vector<double> levelPanelValues; // per level: Σ c·(c + r), the panel values a sweep of the level reads
+    Profiler profiler;
+
+    ~oosg_plan()
+    {
+        if (start)
+            (void)hipEventDestroy(start);
+        if (stop)
+            (void)hipEventDestroy(stop);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+struct oosg_matrix
+{
+    oosg_plan* plan = nullptr;
+    int32_t n = 0;
+    int64_t nnz = 0;
+    std::vector<int64_t> offsets;
+    std::vector<int32_t> rows;
+    std::vector<double> diagonal; // a_jj, 0 when not stored
+    std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
+    double normInf = 0.0;
+    DeviceBuffer dOffsets, dRows, dValues, dPosition, dScale, dCsrOffsets, dCsrCols, dCsrValues;
+    std::vector<double> scale;
+};
+
+struct oosg_factor
+{
+    oosg_plan* plan = nullptr;
+    int32_t precision = OOSG_FP64;
+    DeviceBuffer panels;
+    DeviceBuffer scale; // FP32: the equations' scales of the factorised matrix
+    std::vector<double> diag;
+    oosg_factor_info info{};
+    ~oosg_factor()
+    {
+        if (plan != nullptr && panels.ptr != nullptr && plan->panelCache[precision].ptr == nullptr)
+            plan->panelCache[precision] = std::move(panels);
+    }
+};
+
+namespace
+{
+
+size_t valueBytes(int precision)
+{
+    return precision == OOSG_FP32 ? sizeof(float) : sizeof(double);
+}
+
+void setDevice(const oosg_plan* plan)
+{
+    check(hipSetDevice(plan->device), "hipSetDevice");
+}
+
+// ---- plan --------------------------------------------------------------------------------------------------------
+
+void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
+{
+    if (s.n < 0 || s.supernode_count < 0 || (s.supernode_count > 0 && (!s.first_column || !s.column_count ||
+                                                                        !s.parent || !s.row_start)))
+        fail(OOSG_INVALID_ARGUMENT, "plan: the symbolic factor is incomplete");
+    p.n = s.n;
+    p.S = s.supernode_count;
+    const int32_t S = p.S;
+    p.first.assign(s.first_column, s.first_column + S);
+    p.ncol.assign(s.column_count, s.column_count + S);
+    p.parent.assign(s.parent, s.parent + S);
+    p.rowStart.assign(s.row_start, s.row_start + S + 1);
+    if (p.rowStart[0] != 0)
+        fail(OOSG_INVALID_ARGUMENT, "plan: row_start[0] must be 0");
+    p.rows.assign(s.rows, s.rows + p.rowStart[S]);
+    p.nrow.resize(S);
+    int32_t expected = 0;
+    for (int32_t k = 0; k < S; ++k)
+    {
+        if (p.first[k] != expected || p.ncol[k] < 1)
+            fail(OOSG_INVALID_ARGUMENT, "plan: the supernodes must cover the columns in order, each with >= 1 column");
+        expected += p.ncol[k];
+        if (p.parent[k] != -1 && (p.parent[k] <= k || p.parent[k] >= S))
+            fail(OOSG_INVALID_ARGUMENT, "plan: a parent must follow its child (postorder)");
+        const int64_t r = p.rowStart[k + 1] - p.rowStart[k];
+        if (r < 0 || r > INT32_MAX)
+            fail(OOSG_INVALID_ARGUMENT, "plan: row starts must increase");
+        p.nrow[k] = static_cast<int32_t>(r);
+        int32_t last = p.first[k] + p.ncol[k] - 1;
+        for (int64_t q = p.rowStart[k]; q < p.rowStart[k + 1]; ++q)
+        {
+            if (p.rows[q] <= last || p.rows[q] >= s.n)
+                fail(OOSG_INVALID_ARGUMENT, "plan: the rows of a supernode must ascend below its columns");
+            last = p.rows[q];
+        }
+        if (p.parent[k] == -1 && r > 0)
+            fail(OOSG_INVALID_ARGUMENT, "plan: a root supernode has rows below it");
+    }
+    if (expected != s.n)
+        fail(OOSG_INVALID_ARGUMENT, "plan: the supernodes do not cover the n columns");
+    // Children in increasing order.
+    p.childStart.assign(S + 1, 0);
+    for (int32_t k = 0; k < S; ++k)
+        if (p.parent[k] >= 0)
+            ++p.childStart[p.parent[k] + 1];
+    for (int32_t k = 0; k < S; ++k)
+        p.childStart[k + 1] += p.childStart[k];
+    p.children.resize(p.childStart[S]);
+    {
+        std::vector<int32_t> fill(S, 0);
+        for (int32_t k = 0; k < S; ++k)
+            if (p.parent[k] >= 0)
+                p.children[p.childStart[p.parent[k]] + fill[p.parent[k]]++] = k;
+    }
+    // Relative indices: each row of a child mapped into its parent's front (merge of two ascending lists).
+    p.rel.assign(p.rows.size(), 0);
+    for (int32_t k = 0; k < S; ++k)
+    {
+        const int32_t q = p.parent[k];
+        if (q < 0)
+            continue;
+        const int32_t f = p.first[q], c = p.ncol[q];
+        int64_t cursor = p.rowStart[q];
+        for (int64_t e = p.rowStart[k]; e < p.rowStart[k + 1]; ++e)
+        {
+            const int32_t row = p.rows[e];
+            if (row >= f && row < f + c)
+                p.rel[e] = row - f;
+            else
+            {
+                while (cursor < p.rowStart[q + 1] && p.rows[cursor] < row)
+                    ++cursor;
+                if (cursor >= p.rowStart[q + 1] || p.rows[cursor] != row)
+                    fail(OOSG_INVALID_ARGUMENT, "plan: the rows of a child are not among its parent's columns and rows");
+                p.rel[e] = c + static_cast<int32_t>(cursor - p.rowStart[q]);
+            }
+        }
+    }
+    // Offsets, levels by height, flops.
+    p.panelOff.assign(S + 1, 0);
+    p.updateOff.assign(S + 1, 0);
+    std::vector<int32_t> height(S, 0);
+    for (int32_t k = 0; k < S; ++k)
+    {
+        const int64_t c = p.ncol[k], r = p.nrow[k];
+        p.panelOff[k + 1] = p.panelOff[k] + c * (c + r);
+        p.updateOff[k + 1] = p.updateOff[k] + r * r;
+        for (int64_t m = r; m < r + c; ++m)
+            p.flops += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
+        if (p.parent[k] >= 0)
+            height[p.parent[k]] = std::max(height[p.parent[k]], height[k] + 1);
+    }
+    p.levels = 0;
+    for (int32_t k = 0; k < S; ++k)
+        p.levels = std::max(p.levels, height[k] + 1);
+    p.levelStart.assign(p.levels + 1, 0);
+    for (int32_t k = 0; k < S; ++k)
+        ++p.levelStart[height[k] + 1];
+    for (int32_t l = 0; l < p.levels; ++l)
+        p.levelStart[l + 1] += p.levelStart[l];
+    p.levelList.resize(S);
+    p.levelPanelValues.assign(p.levels, 0.0);
+    {
+        std::vector<int32_t> fill(p.levels, 0);
+        for (int32_t k = 0; k < S; ++k)
+        {
+            p.levelList[p.levelStart[height[k]] + fill[height[k]]++] = k;
+            p.levelPanelValues[height[k]] += static_cast<double>(p.panelOff[k + 1] - p.panelOff[k]);
+        }
+    }
+}
+
+// The algorithmic flops of a tile: 2·k per element it updates (the lower part only on the diagonal).
+double tileFlops(const oosg::UpdateTile& t)
+{
+    double elements = static_cast<double>(t.m) * t.n;
+    if (t.lower)
+        elements -= static_cast<double>(t.n) * (t.n - 1) / 2.0;
+    return 2.0 * t.k * elements;
+}
+
+void buildSchedule(oosg_plan& p)
+{
+    using oosg::PANEL_BLOCK;
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        std::vector<int32_t> large;
+        int maxSmall = 0;
+        const int64_t smallStart = static_cast<int64_t>(p.smallList.size());
+        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+        {
+            const int32_t s = p.levelList[q];
+            const int order = p.ncol[s] + p.nrow[s];
+            if (order <= oosg::SMALL_FRONT_MAX)
+            {
+                p.smallList.push_back(s);
+                maxSmall = std::max(maxSmall, order);
+            }
+            else
+                large.push_back(s);
+        }
+        const int smallCount = static_cast<int>(p.smallList.size() - smallStart);
+        p.smallFronts += smallCount;
+        p.largeFronts += static_cast<int64_t>(large.size());
+        if (smallCount > 0)
+        {
+            Launch launch{LaunchKind::Small, PROFILE_SMALL};
+            launch.offset = smallStart;
+            launch.count = smallCount;
+            launch.maxOrder = maxSmall;
+            for (int64_t q = smallStart; q < static_cast<int64_t>(p.smallList.size()); ++q)
+                launch.flops += frontFlops(p.ncol[p.smallList[q]], p.nrow[p.smallList[q]]);
+            p.schedule.push_back(launch);
+        }
+        if (large.empty())
+            continue;
+        // Assembly blocks.
+        const int64_t blockStart = static_cast<int64_t>(p.blocks.size());
+        int maxC = 0;
+        for (const int32_t s : large)
+        {
+            const int c = p.ncol[s], n = c + p.nrow[s];
+            maxC = std::max(maxC, c);
+            int j0 = 0;
+            while (j0 < n)
+            {
+                const int i0 = j0 < c ? 0 : j0;
+                const int height = n - i0;
+                if (height <= oosg::ASSEMBLY_BLOCK_VALUES)
+                {
+                    int j1 = std::min(n, j0 + std::max(1, oosg::ASSEMBLY_BLOCK_VALUES / height));
+                    if (j0 < c && j1 > c)
+                        j1 = c;
+                    p.blocks.push_back({s, j0, j1, i0, n, 0});
+                    j0 = j1;
+                }
+                else
+                {
+                    for (int i = i0; i < n; i += oosg::ASSEMBLY_BLOCK_VALUES)
+                        p.blocks.push_back({s, j0, j0 + 1, i, std::min(n, i + oosg::ASSEMBLY_BLOCK_VALUES), 0});
+                    j0 += 1;
+                }
+            }
+        }
+        Launch assemble{LaunchKind::Assemble, PROFILE_ASSEMBLE};
+        assemble.offset = blockStart;
+        assemble.count = static_cast<int>(p.blocks.size() - blockStart);
+        p.schedule.push_back(assemble);
+        // Panel steps.
+        for (int k0 = 0; k0 < maxC; k0 += PANEL_BLOCK)
+        {
+            Launch diagonal{LaunchKind::Diagonal, PROFILE_DIAGONAL};
+            diagonal.k0 = k0;
+            diagonal.offset = static_cast<int64_t>(p.stepFronts.size());
+            Launch solve{LaunchKind::PanelSolve, PROFILE_PANEL_SOLVE};
+            solve.k0 = k0;
+            solve.offset = static_cast<int64_t>(p.chunks.size());
+            Launch update{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
+            const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
+            for (const int32_t s : large)
+            {
+                const int c = p.ncol[s], n = c + p.nrow[s];
+                if (c <= k0)
+                    continue;
+                const int kb = std::min(PANEL_BLOCK, c - k0), k1 = k0 + kb;
+                p.stepFronts.push_back(s);
+                diagonal.flops += frontFlops(kb, 0);
+                solve.flops += static_cast<double>(n - k1) * (static_cast<double>(kb) * kb + kb);
+                for (int row = k1; row < n; row += 256)
+                    p.chunks.push_back(make_int2(s, row));
+                for (int J0 = k1; J0 < c; J0 += 64)
+                    for (int I0 = J0; I0 < n; I0 += 64)
+                    {
+                        oosg::UpdateTile t{};
+                        t.cOff = p.panelOff[s] + I0 + int64_t(J0) * n;
+                        t.ldc = n;
+                        t.aOff = p.panelOff[s] + I0 + int64_t(k0) * n;
+                        t.bOff = p.panelOff[s] + J0 + int64_t(k0) * n;
+                        t.dOff = p.panelOff[s] + int64_t(k0) * (n + 1);
+                        t.ld = n;
+                        t.m = std::min(64, n - I0);
+                        t.n = std::min(64, c - J0);
+                        t.k = kb;
+                        t.target = 0;
+                        t.lower = I0 == J0;
+                        p.tiles.push_back(t);
+                        update.flops += tileFlops(t);
+                    }
+            }
+            diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
+            solve.count = static_cast<int>(p.chunks.size() - solve.offset);
+            p.schedule.push_back(diagonal);
+            if (solve.count > 0)
+                p.schedule.push_back(solve);
+            if (static_cast<int64_t>(p.tiles.size()) > tileStart)
+            {
+                update.offset = tileStart;
+                update.count = static_cast<int>(p.tiles.size() - tileStart);
+                p.schedule.push_back(update);
+            }
+        }
+        // The update matrices: C(r × r) −= L_R·D·L_Rᵀ, K = c.
+        const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
+        for (const int32_t s : large)
+        {
+            const int c = p.ncol[s], r = p.nrow[s], n = c + r;
+            for (int J0 = 0; J0 < r; J0 += 64)
+                for (int I0 = J0; I0 < r; I0 += 64)
+                {
+                    oosg::UpdateTile t{};
+                    t.cOff = p.updateOff[s] + I0 + int64_t(J0) * r;
+                    t.ldc = r;
+                    t.aOff = p.panelOff[s] + c + I0;
+                    t.bOff = p.panelOff[s] + c + J0;
+                    t.dOff = p.panelOff[s];
+                    t.ld = n;
+                    t.m = std::min(64, r - I0);
+                    t.n = std::min(64, r - J0);
+                    t.k = c;
+                    t.target = 1;
+                    t.lower = I0 == J0;
+                    p.tiles.push_back(t);
+                }
+        }
+        if (static_cast<int64_t>(p.tiles.size()) > tileStart)
+        {
+            Launch tilesLaunch{LaunchKind::Tiles, PROFILE_UPDATE_MATRIX};
+            tilesLaunch.offset = tileStart;
+            tilesLaunch.count = static_cast<int>(p.tiles.size() - tileStart);
+            for (int64_t q = tileStart; q < static_cast<int64_t>(p.tiles.size()); ++q)
+                tilesLaunch.flops += tileFlops(p.tiles[q]);
+            p.schedule.push_back(tilesLaunch);
+        }
+    }
+}
+
+void uploadPlan(oosg_plan& p)
+{
+    const hipStream_t st = p.stream;
+    p.dFirst = upload(p.first, st, "plan: first columns");
+    p.dNcol = upload(p.ncol, st, "plan: column counts");
+    p.dNrow = upload(p.nrow, st, "plan: row counts");
+    p.dPanelOff = upload(p.panelOff, st, "plan: panel offsets");
+    p.dUpdateOff = upload(p.updateOff, st, "plan: update offsets");
+    p.dRowStart = upload(p.rowStart, st, "plan: row starts");
+    p.dRows = upload(p.rows, st, "plan: rows");
+    p.dRel = upload(p.rel, st, "plan: relative indices");
+    p.dChildStart = upload(p.childStart, st, "plan: child starts");
+    p.dChildren = upload(p.children, st, "plan: children");
+    p.dSmallList = upload(p.smallList, st, "plan: small fronts");
+    p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
+    p.dChunks = upload(p.chunks, st, "plan: panel chunks");
+    p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
+    p.dTiles = upload(p.tiles, st, "plan: update tiles");
+    p.dLevelList = upload(p.levelList, st, "plan: levels");
+    p.dp.first = p.dFirst.as<int32_t>();
+    p.dp.ncol = p.dNcol.as<int32_t>();
+    p.dp.nrow = p.dNrow.as<int32_t>();
+    p.dp.panelOff = p.dPanelOff.as<int64_t>();
+    p.dp.updateOff = p.dUpdateOff.as<int64_t>();
+    p.dp.rowStart = p.dRowStart.as<int64_t>();
+    p.dp.rows = p.dRows.as<int32_t>();
+    p.dp.rel = p.dRel.as<int32_t>();
+    p.dp.childStart = p.dChildStart.as<int32_t>();
+    p.dp.children = p.dChildren.as<int32_t>();
+    p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
+    check(hipStreamSynchronize(st), "plan: upload");
+}
+
+// ---- factorisation -------------------------------------------------------------------------------------------------
+
+template <typename T>
+void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, double* diagOut, int32_t* zeroAt, bool scaled)
+{
+    const hipStream_t st = p.stream;
+    oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
+                         m.dPosition.as<int32_t>(), scaled ? m.dScale.as<double>() : nullptr};
+    T* updates = p.updates.as<T>();
+    const int32_t* smallList = p.dSmallList.as<int32_t>();
+    const int32_t* stepFronts = p.dStepFronts.as<int32_t>();
+    const int2* chunks = p.dChunks.as<int2>();
+    const oosg::AssemblyBlock* blocks = p.dBlocks.as<oosg::AssemblyBlock>();
+    const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
+    for (const Launch& l : p.schedule)
+    {
+        const int slot = p.profiler.begin(l.profile, l.flops, st);
+        switch (l.kind)
+        {
+        case LaunchKind::Small:
+            check(oosg::launchSmallFronts<T>(p.dp, a, panels, updates, smallList + l.offset, l.count, l.maxOrder,
+                                             diagOut, zeroAt, st),
+                  "factor: small fronts");
+            break;
+        case LaunchKind::Assemble:
+            check(oosg::launchAssembly<T>(p.dp, a, panels, updates, blocks + l.offset, l.count, st),
+                  "factor: assembly");
+            break;
+        case LaunchKind::Diagonal:
+            check(oosg::launchDiagonalBlocks<T>(p.dp, panels, stepFronts + l.offset, l.count, l.k0, diagOut, zeroAt,
+                                                st),
+                  "factor: diagonal blocks");
+            break;
+        case LaunchKind::PanelSolve:
+            check(oosg::launchPanelSolve<T>(p.dp, panels, chunks + l.offset, l.count, l.k0, st), "factor: panel solve");
+            break;
+        case LaunchKind::Tiles:
+            check(oosg::launchUpdateTiles<T>(panels, updates, tiles + l.offset, l.count, st), "factor: update tiles");
+            break;
+        }
+        p.profiler.end(slot, st);
+    }
+}
+
+std::string equationList(const std::vector<int32_t>& equations, size_t total)
+{
+    std::string text;
+    const size_t listed = std::min<size_t>(equations.size(), 20);
+    for (size_t k = 0; k < listed; ++k)
+        text += "  equation " + std::to_string(equations[k]) + "\n";
+    if (total > listed)
+        text += "  and " + std::to_string(total - listed) + " more\n";
+    return text;
+}
+
+// The zero-pivot test, the inertia and the ratio test of MultifrontalDriver::finish
+// (src/linalg/supernodal/MultifrontalDriver.cpp:704-820), on the host from the pivots the kernels wrote.
+void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std::vector<int32_t>& zeroAt,
+            const oosg_factor_options& options)
+{
+    // A supernode whose descendant failed is skipped in the reference: only the failures with no failed descendant
+    // are reported (their ancestors' values are meaningless).
+    std::vector<char> failedBelow(p.S, 0);
+    std::vector<int32_t> zeroPivots;
+    size_t zeroCount = 0;
+    for (int32_t s = 0; s < p.S; ++s)
+    {
+        const bool failedHere = zeroAt[s] != INT32_MAX;
+        if (failedHere && !failedBelow[s])
+        {
+            ++zeroCount;
+            if (zeroPivots.size() < 1000)
+                zeroPivots.push_back(p.first[s] + zeroAt[s]);
+        }
+        if ((failedHere || failedBelow[s]) && p.parent[s] >= 0)
+            failedBelow[p.parent[s]] = 1;
+    }
+    if (zeroCount > 0)
+        fail(OOSG_SINGULAR, "the stiffness matrix is singular: " + std::to_string(zeroCount) +
+                                " zero (or not finite) pivots met in the factorisation, a mechanism or an "
+                                "unconstrained DOF at:\n" +
+                                equationList(zeroPivots, zeroCount) +
+                                "constrain these DOFs (single-point constraints) or connect them to the structure");
+    oosg_factor_info& info = f.info;
+    const double maxRatio = options.max_ratio > 0 ? options.max_ratio : 1.0e7;
+    double largest = 0.0;
+    int32_t largestAt = -1;
+    std::vector<int32_t> issues;
+    info.first_issue_equation = -1;
+    for (int32_t j = 0; j < p.n; ++j)
+    {
+        const double pivot = f.diag[j];
+        ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
+        const double ratio = m.diagonal[j] / pivot;
+        if (largestAt < 0 || ratio > largest)
+        {
+            largest = ratio;
+            largestAt = j;
+        }
+        const bool negative = ratio < 0.0 || pivot < 0.0;
+        if (ratio > maxRatio || negative)
+        {
+            if (info.issue_count == 0)
+            {
+                info.first_issue_equation = j;
+                info.first_issue_negative = negative ? 1 : 0;
+            }
+            ++info.issue_count;
+            if (issues.size() < 1000)
+                issues.push_back(j);
+        }
+    }
+    info.largest_ratio = largest;
+    info.largest_ratio_equation = largestAt;
+    if (info.issue_count > 0 && options.bailout >= 0)
+        fail(OOSG_SINGULAR, "the stiffness matrix is nearly singular or not positive definite: " +
+                                std::to_string(info.issue_count) +
+                                " pivots fail the factor diagonal ratio test (ratio of the matrix diagonal to the "
+                                "factor diagonal above " +
+                                std::to_string(maxRatio) + ", or negative):\n" +
+                                equationList(issues, static_cast<size_t>(info.issue_count)));
+}
+
+// ---- solves --------------------------------------------------------------------------------------------------------
+
+void ensure(DeviceBuffer& b, size_t bytes, const char* what)
+{
+    if (b.bytes < bytes)
+        b.allocate(bytes, what);
+}
+
+// X (row-major n × k, k <= 16, in p.solveX) ← factor⁻¹·X.
+template <typename TP> void runSweeps(oosg_plan& p, const TP* panels, int k)
+{
+    const hipStream_t st = p.stream;
+    double* X = p.solveX.as<double>();
+    double* U = p.solveU.as<double>();
+    const int32_t* levels = p.dLevelList.as<int32_t>();
+    // The profile counts bytes as the work of the sweeps (they are bound by HBM): the level's panels, read once.
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        const int slot = p.profiler.begin(PROFILE_FORWARD, p.levelPanelValues[l] * sizeof(TP), st);
+        check(oosg::launchForward<TP>(p.dp, panels, X, U, k, levels + p.levelStart[l],
+                                      p.levelStart[l + 1] - p.levelStart[l], st),
+              "solve: forward sweep");
+        p.profiler.end(slot, st);
+    }
+    for (int32_t l = p.levels - 1; l >= 0; --l)
+    {
+        const int slot = p.profiler.begin(PROFILE_BACKWARD, p.levelPanelValues[l] * sizeof(TP), st);
+        check(oosg::launchBackward<TP>(p.dp, panels, X, k, levels + p.levelStart[l],
+                                       p.levelStart[l + 1] - p.levelStart[l], st),
+              "solve: backward sweep");
+        p.profiler.end(slot, st);
+    }
+}
+
+// b (device, column-major, ld) ← factor⁻¹·b, by blocks of 16 columns.
+void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
+{
+    oosg_plan& p = *f.plan;
+    const int64_t n = p.n;
+    const int64_t width = std::min<int64_t>(nrhs, 16);
+    ensure(p.solveX, static_cast<size_t>(n * width) * sizeof(double), "solve: right-hand sides");
+    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * width) * sizeof(double),
+           "solve: update vectors");
+    const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
+    {
+        const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
+        double* block = b + c0 * ld;
+        check(oosg::launchToRowMajor(block, ld, p.solveX.as<double>(), n, k, scale, p.stream), "solve: layout");
+        if (f.precision == OOSG_FP32)
+            runSweeps<float>(p, f.panels.as<float>(), k);
+        else
+            runSweeps<double>(p, f.panels.as<double>(), k);
+        check(oosg::launchFromRowMajor(p.solveX.as<double>(), block, ld, n, k, scale, 0, p.stream), "solve: layout");
+    }
+}
+
+std::vector<double> columnMax(oosg_plan& p, const double* v, int64_t ld, int64_t n, int k)
+{
+    ensure(p.solveNorms, 16 * sizeof(unsigned long long), "solve: norms");
+    std::vector<unsigned long long> bits(static_cast<size_t>(k), 0);
+    check(hipMemcpyAsync(p.solveNorms.ptr, bits.data(), k * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                         p.stream),
+          "solve: norms");
+    check(oosg::launchColumnMax(v, ld, n, k, p.solveNorms.as<unsigned long long>(), p.stream), "solve: norms");
+    check(hipMemcpyAsync(bits.data(), p.solveNorms.ptr, k * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         p.stream),
+          "solve: norms");
+    check(hipStreamSynchronize(p.stream), "solve: norms");
+    std::vector<double> out(static_cast<size_t>(k));
+    for (int h = 0; h < k; ++h)
+        std::memcpy(&out[h], &bits[h], sizeof(double));
+    return out;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* oosg_last_error(void)
+{
+    return g_lastError.c_str();
+}
+
+oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device, uint64_t hbm_budget_bytes,
+                             oosg_plan** out)
+{
+    return guarded(
+        [&]
+        {
+            if (symbolic == nullptr || out == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            *out = nullptr;
+            auto p = std::make_unique<oosg_plan>();
+            p->device = device;
+            int count = 0;
+            check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+            if (device < 0 || device >= count)
+                fail(OOSG_INVALID_ARGUMENT, "plan: no device " + std::to_string(device));
+            setDevice(p.get());
+            buildStructure(*p, *symbolic);
+            buildSchedule(*p);
+            // The memory of every factorisation of this plan, reserved before any numeric work (MemoryBudgetError).
+            size_t freeBytes = 0, totalBytes = 0;
+            check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
+            p->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : freeBytes;
+            const uint64_t structure = (p->first.size() * 3 + p->rows.size() * 2 + p->children.size() +
+                                        p->smallList.size() + p->stepFronts.size() + p->levelList.size()) *
+                                           4 +
+                                       p->panelOff.size() * 16 + p->chunks.size() * 8 +
+                                       p->blocks.size() * sizeof(oosg::AssemblyBlock) +
+                                       p->tiles.size() * sizeof(oosg::UpdateTile);
+            const uint64_t values = static_cast<uint64_t>(p->updateOff[p->S] + p->panelOff[p->S]) * sizeof(double);
+            if (structure + values > p->budget)
+                fail(OOSG_OUT_OF_MEMORY,
+                     "plan: the factorisation needs " + std::to_string((structure + values) >> 20) +
+                         " MiB of device memory (panels " +
+                         std::to_string((static_cast<uint64_t>(p->panelOff[p->S]) * 8) >> 20) + " MiB, update matrices " +
+                         std::to_string((static_cast<uint64_t>(p->updateOff[p->S]) * 8) >> 20) + " MiB), " +
+                         std::to_string(p->budget >> 20) + " MiB are available");
+            check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+            check(hipEventCreate(&p->start), "hipEventCreate");
+            check(hipEventCreate(&p->stop), "hipEventCreate");
+            uploadPlan(*p);
+            *out = p.release();
+        });
+}
+
+void oosg_plan_destroy(oosg_plan* plan)
+{
+    if (plan == nullptr)
+        return;
+    (void)hipSetDevice(plan->device);
+    if (plan->stream)
+        (void)hipStreamSynchronize(plan->stream);
+    delete plan;
+}
+
+oosg_status oosg_plan_stats(const oosg_plan* plan, int64_t* stats)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || stats == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan stats: null argument");
+            stats[0] = plan->S;
+            stats[1] = plan->levels;
+            stats[2] = plan->panelOff[plan->S];
+            stats[3] = plan->updateOff[plan->S];
+            stats[4] = plan->smallFronts;
+            stats[5] = plan->largeFronts;
+            stats[6] = static_cast<int64_t>(plan->schedule.size());
+            stats[7] = static_cast<int64_t>(plan->tiles.size());
+        });
+}
+
+oosg_status oosg_plan_profile(oosg_plan* plan, int32_t enable)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "profile: null plan");
+            plan->profiler.enabled = enable != 0;
+            plan->profiler.reset();
+        });
+}
+
+oosg_status oosg_plan_profile_read(const oosg_plan* plan, double* seconds, int64_t* launches, double* work)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "profile: null plan");
+            for (int k = 0; k < PROFILE_KINDS; ++k)
+            {
+                if (seconds)
+                    seconds[k] = plan->profiler.seconds[k];
+                if (launches)
+                    launches[k] = plan->profiler.launches[k];
+                if (work)
+                    work[k] = plan->profiler.work[k];
+            }
+        });
+}
+
+oosg_status oosg_plan_synchronize(oosg_plan* plan)
+{
+    return guarded(
+        [&]
+        {
+            setDevice(plan);
+            check(hipStreamSynchronize(plan->stream), "synchronize");
+        });
+}
+
+oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_matrix** out)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || csc == nullptr || out == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            *out = nullptr;
+            if (csc->n != plan->n)
+                fail(OOSG_INVALID_ARGUMENT, "supernodal factorisation: the matrix has " + std::to_string(csc->n) +
+                                                " equations and the symbolic factor " + std::to_string(plan->n) +
+                                                "; factorise the matrix assembled in the order of its symbolic analysis");
+            setDevice(plan);
+            auto m = std::make_unique<oosg_matrix>();
+            m->plan = plan;
+            const int32_t n = csc->n;
+            m->n = n;
+            m->offsets.assign(csc->offsets, csc->offsets + n + 1);
+            m->nnz = m->offsets[n];
+            if (m->offsets[0] != 0)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: offsets[0] must be 0");
+            m->rows.assign(csc->rows, csc->rows + m->nnz);
+            std::vector<double> values(csc->values, csc->values + m->nnz);
+            // Front positions; the structure test of FrontAssembly.cpp:22-28.
+            std::vector<int32_t> position(static_cast<size_t>(m->nnz));
+            m->diagonal.assign(n, 0.0);
+            for (int32_t s = 0; s < plan->S; ++s)
+            {
+                const int32_t f = plan->first[s], c = plan->ncol[s];
+                const int32_t* rws = plan->rows.data() + plan->rowStart[s];
+                const int32_t r = plan->nrow[s];
+                for (int32_t j = f; j < f + c; ++j)
+                {
+                    if (m->offsets[j + 1] < m->offsets[j])
+                        fail(OOSG_INVALID_ARGUMENT, "matrix: column offsets must not decrease");
+                    int32_t previous = -1;
+                    for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                    {
+                        const int32_t row = m->rows[k];
+                        if (row <= previous)
+                            fail(OOSG_INVALID_ARGUMENT, "matrix: the rows of a column must ascend");
+                        previous = row;
+                        int32_t pos = -1;
+                        if (row >= j && row < f + c)
+                            pos = row - f;
+                        else if (row >= f + c)
+                        {
+                            const int32_t* it = std::lower_bound(rws, rws + r, row);
+                            if (it != rws + r && *it == row)
+                                pos = c + static_cast<int32_t>(it - rws);
+                        }
+                        if (pos < 0)
+                            fail(OOSG_INVALID_ARGUMENT,
+                                 "supernodal factorisation: the matrix entry (" + std::to_string(row) + ", " +
+                                     std::to_string(j) + ") lies outside the structure of the symbolic factor");
+                        position[k] = pos;
+                        if (row == j)
+                            m->diagonal[j] = values[k];
+                    }
+                }
+            }
+            // Power-of-two scales for the FP32 factor: s_j² a_jj in [1/2, 2) (no rounding: exact scaling).
+            m->scale.assign(n, 1.0);
+            for (int32_t j = 0; j < n; ++j)
+            {
+                const double a = m->diagonal[j];
+                if (a > 0.0 && std::isfinite(a))
+                {
+                    int e = 0;
+                    (void)std::frexp(a, &e);
+                    m->scale[j] = std::ldexp(1.0, -(e / 2));
+                }
+            }
+            // The full symmetric matrix by rows for the residuals.
+            std::vector<int64_t> rowCount(static_cast<size_t>(n) + 1, 0);
+            for (int32_t j = 0; j < n; ++j)
+                for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                {
+                    ++rowCount[m->rows[k] + 1];
+                    if (m->rows[k] != j)
+                        ++rowCount[j + 1];
+                }
+            for (int32_t i = 0; i < n; ++i)
+                rowCount[i + 1] += rowCount[i];
+            std::vector<int32_t> csrCols(static_cast<size_t>(rowCount[n]));
+            m->csrFromCsc.resize(static_cast<size_t>(rowCount[n]));
+            {
+                std::vector<int64_t> fill(rowCount.begin(), rowCount.end() - 1);
+                // Row i: the columns j < i (entries (i, j) of column j), then its own column from the diagonal.
+                for (int32_t j = 0; j < n; ++j)
+                    for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                    {
+                        const int32_t i = m->rows[k];
+                        if (i != j)
+                        {
+                            csrCols[fill[i]] = j;
+                            m->csrFromCsc[fill[i]++] = k;
+                        }
+                    }
+                for (int32_t j = 0; j < n; ++j)
+                    for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                    {
+                        csrCols[fill[j]] = m->rows[k];
+                        m->csrFromCsc[fill[j]++] = k;
+                    }
+            }
+            std::vector<double> csrValues(m->csrFromCsc.size());
+            for (size_t e = 0; e < csrValues.size(); ++e)
+                csrValues[e] = values[m->csrFromCsc[e]];
+            m->normInf = 0.0;
+            for (int32_t i = 0; i < n; ++i)
+            {
+                double sum = 0.0;
+                for (int64_t e = rowCount[i]; e < rowCount[i + 1]; ++e)
+                    sum += std::fabs(csrValues[e]);
+                m->normInf = std::max(m->normInf, sum);
+            }
+            const hipStream_t st = plan->stream;
+            m->dOffsets = upload(m->offsets, st, "matrix: offsets");
+            m->dRows = upload(m->rows, st, "matrix: rows");
+            m->dValues = upload(values, st, "matrix: values");
+            m->dPosition = upload(position, st, "matrix: positions");
+            m->dScale = upload(m->scale, st, "matrix: scales");
+            m->dCsrOffsets = upload(rowCount, st, "matrix: row offsets");
+            m->dCsrCols = upload(csrCols, st, "matrix: row columns");
+            m->dCsrValues = upload(csrValues, st, "matrix: row values");
+            check(hipStreamSynchronize(st), "matrix: upload");
+            *out = m.release();
+        });
+}
+
+oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
+{
+    return guarded(
+        [&]
+        {
+            if (m == nullptr || values == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            setDevice(m->plan);
+            const hipStream_t st = m->plan->stream;
+            std::vector<double> v(values, values + m->nnz);
+            for (int32_t j = 0; j < m->n; ++j)
+            {
+                m->diagonal[j] = 0.0;
+                if (m->offsets[j] < m->offsets[j + 1] && m->rows[m->offsets[j]] == j)
+                    m->diagonal[j] = v[m->offsets[j]];
+                const double a = m->diagonal[j];
+                m->scale[j] = 1.0;
+                if (a > 0.0 && std::isfinite(a))
+                {
+                    int e = 0;
+                    (void)std::frexp(a, &e);
+                    m->scale[j] = std::ldexp(1.0, -(e / 2));
+                }
+            }
+            std::vector<double> csrValues(m->csrFromCsc.size());
+            for (size_t e = 0; e < csrValues.size(); ++e)
+                csrValues[e] = v[m->csrFromCsc[e]];
+            m->normInf = 0.0;
+            std::vector<double> rowSum(m->n, 0.0);
+            for (int32_t j = 0; j < m->n; ++j)
+                for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                {
+                    rowSum[m->rows[k]] += std::fabs(v[k]);
+                    if (m->rows[k] != j)
+                        rowSum[j] += std::fabs(v[k]);
+                }
+            for (const double s : rowSum)
+                m->normInf = std::max(m->normInf, s);
+            check(hipMemcpyAsync(m->dValues.ptr, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, st),
+                  "matrix: values");
+            check(hipMemcpyAsync(m->dScale.ptr, m->scale.data(), m->scale.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, st),
+                  "matrix: scales");
+            check(hipMemcpyAsync(m->dCsrValues.ptr, csrValues.data(), csrValues.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, st),
+                  "matrix: row values");
+            check(hipStreamSynchronize(st), "matrix: values");
+        });
+}
+
+void oosg_matrix_destroy(oosg_matrix* m)
+{
+    if (m == nullptr)
+        return;
+    (void)hipSetDevice(m->plan->device);
+    delete m;
+}
+
+oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options,
+                           oosg_factor** out)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || matrix == nullptr || out == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: null argument");
+            *out = nullptr;
+            if (matrix->plan != plan)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: the matrix was made for another plan");
+            oosg_factor_options opts{};
+            opts.max_ratio = 1.0e7;
+            if (options != nullptr)
+                opts = *options;
+            if (opts.pivoting != OOSG_POSITIVE_DEFINITE)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: the indefinite mode (Bunch-Kaufman fronts) is not built yet");
+            if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: unknown precision");
+            setDevice(plan);
+            auto f = std::make_unique<oosg_factor>();
+            f->plan = plan;
+            f->precision = opts.precision;
+            const size_t panelBytes = static_cast<size_t>(plan->panelOff[plan->S]) * valueBytes(opts.precision);
+            if (plan->panelCache[opts.precision].ptr != nullptr && plan->panelCache[opts.precision].bytes >= panelBytes)
+                f->panels = std::move(plan->panelCache[opts.precision]);
+            else
+                f->panels.allocate(panelBytes, "factor: panels");
+            DeviceBuffer diag, zero;
+            diag.allocate(static_cast<size_t>(plan->n) * sizeof(double), "factor: pivots");
+            zero.allocate(static_cast<size_t>(std::max(plan->S, 1)) * sizeof(int32_t), "factor: pivot status");
+            const hipStream_t st = plan->stream;
+            check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
+            check(hipEventRecord(plan->start, st), "hipEventRecord");
+            if (opts.precision == OOSG_FP32)
+                runFactor<float>(*plan, *matrix, f->panels.as<float>(), diag.as<double>(), zero.as<int32_t>(), true);
+            else
+                runFactor<double>(*plan, *matrix, f->panels.as<double>(), diag.as<double>(), zero.as<int32_t>(),
+                                  false);
+            check(hipEventRecord(plan->stop, st), "hipEventRecord");
+            std::vector<int32_t> zeroAt(static_cast<size_t>(plan->S));
+            f->diag.resize(static_cast<size_t>(plan->n));
+            check(hipMemcpyAsync(zeroAt.data(), zero.ptr, zeroAt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+                  "factor: status");
+            check(hipMemcpyAsync(f->diag.data(), diag.ptr, f->diag.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 st),
+                  "factor: pivots");
+            check(hipStreamSynchronize(st), "factor");
+            plan->profiler.collect();
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, plan->start, plan->stop), "hipEventElapsedTime");
+            f->info.factor_seconds = ms * 1e-3;
+            f->info.flops = plan->flops;
+            f->info.precision = opts.precision;
+            if (opts.precision == OOSG_FP32)
+            {
+                // The pivots of S·A·S: d_j / s_j² are those of A (same signs: the inertia is kept).
+                f->scale = upload(matrix->scale, st, "factor: scales");
+                check(hipStreamSynchronize(st), "factor: scales");
+                for (int32_t j = 0; j < plan->n; ++j)
+                    f->diag[j] = f->diag[j] / (matrix->scale[j] * matrix->scale[j]);
+            }
+            finish(*plan, *matrix, *f, zeroAt, opts);
+            *out = f.release();
+        });
+}
+
+void oosg_factor_destroy(oosg_factor* f)
+{
+    if (f == nullptr)
+        return;
+    (void)hipSetDevice(f->plan->device);
+    delete f;
+}
+
+oosg_status oosg_factor_info_get(const oosg_factor* f, oosg_factor_info* info)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || info == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "factor info: null argument");
+            *info = f->info;
+        });
+}
+
+oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, int32_t* rows, int32_t* eliminated)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "panel: null factor");
+            const oosg_plan& p = *f->plan;
+            if (s < 0 || s >= p.S)
+                fail(OOSG_INVALID_ARGUMENT, "panel: no supernode " + std::to_string(s));
+            const int32_t c = p.ncol[s], r = p.nrow[s];
+            const size_t count = static_cast<size_t>(c) * (c + r);
+            if (eliminated != nullptr)
+                *eliminated = c;
+            if (rows != nullptr)
+            {
+                for (int32_t k = 0; k < c; ++k)
+                    rows[k] = p.first[s] + k;
+                std::copy(p.rows.begin() + p.rowStart[s], p.rows.begin() + p.rowStart[s + 1], rows + c);
+            }
+            if (values == nullptr)
+                return;
+            setDevice(&p);
+            if (f->precision == OOSG_FP32)
+            {
+                std::vector<float> tmp(count);
+                check(hipMemcpy(tmp.data(), f->panels.as<float>() + p.panelOff[s], count * sizeof(float),
+                                hipMemcpyDeviceToHost),
+                      "panel: copy");
+                std::copy(tmp.begin(), tmp.end(), values);
+            }
+            else
+                check(hipMemcpy(values, f->panels.as<double>() + p.panelOff[s], count * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "panel: copy");
+        });
+}
+
+oosg_status oosg_factor_diagonal(const oosg_factor* f, double* d)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || d == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "diagonal: null argument");
+            std::copy(f->diag.begin(), f->diag.end(), d);
+        });
+}
+
+oosg_status oosg_solve_device(const oosg_factor* f, double* b, int64_t ld, int64_t nrhs)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || (b == nullptr && nrhs > 0))
+                fail(OOSG_INVALID_ARGUMENT, "FactorSolve::solve: null argument");
+            if (ld < std::max<int64_t>(f->plan->n, 1) || nrhs < 0)
+                fail(OOSG_INVALID_ARGUMENT, "FactorSolve::solve: the leading dimension is below the row count");
+            if (nrhs == 0 || f->plan->n == 0)
+                return;
+            setDevice(f->plan);
+            solveDevice(*f, b, ld, nrhs);
+            check(hipStreamSynchronize(f->plan->stream), "solve");
+            f->plan->profiler.collect();
+        });
+}
+
+oosg_status oosg_solve(const oosg_factor* f, double* b, int64_t ld, int64_t nrhs)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || (b == nullptr && nrhs > 0))
+                fail(OOSG_INVALID_ARGUMENT, "FactorSolve::solve: null argument");
+            oosg_plan& p = *f->plan;
+            if (ld < std::max<int64_t>(p.n, 1) || nrhs < 0)
+                fail(OOSG_INVALID_ARGUMENT, "FactorSolve::solve: the leading dimension is below the row count");
+            if (nrhs == 0 || p.n == 0)
+                return;
+            setDevice(&p);
+            DeviceBuffer db;
+            db.allocate(static_cast<size_t>(p.n) * nrhs * sizeof(double), "solve: right-hand sides");
+            check(hipMemcpy2DAsync(db.ptr, p.n * sizeof(double), b, ld * sizeof(double), p.n * sizeof(double), nrhs,
+                                   hipMemcpyHostToDevice, p.stream),
+                  "solve: upload");
+            solveDevice(*f, db.as<double>(), p.n, nrhs);
+            check(hipMemcpy2DAsync(b, ld * sizeof(double), db.ptr, p.n * sizeof(double), p.n * sizeof(double), nrhs,
+                                   hipMemcpyDeviceToHost, p.stream),
+                  "solve: download");
+            check(hipStreamSynchronize(p.stream), "solve");
+            p.profiler.collect();
+        });
+}
+
+oosg_status oosg_matrix_multiply_device(const oosg_matrix* m, const double* x, double* y, int64_t ld, int64_t nrhs)
+{
+    return guarded(
+        [&]
+        {
+            if (m == nullptr || x == nullptr || y == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::multiply: null argument");
+            if (ld < std::max<int64_t>(m->n, 1))
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::multiply: the leading dimension is below the row count");
+            setDevice(m->plan);
+            check(oosg::launchResidual(m->dCsrOffsets.as<int64_t>(), m->dCsrCols.as<int32_t>(),
+                                       m->dCsrValues.as<double>(), x, nullptr, y, ld, m->n, static_cast<int>(nrhs),
+                                       m->plan->stream),
+                  "multiply");
+            check(hipStreamSynchronize(m->plan->stream), "multiply");
+        });
+}
+
+oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m, const double* b, double* x,
+                                      int64_t ld, int64_t nrhs, double target, int32_t max_steps,
+                                      oosg_solve_report* report)
+{
+    oosg_solve_report local{};
+    const oosg_status status = guarded(
+        [&]
+        {
+            if (f == nullptr || m == nullptr || b == nullptr || x == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: null argument");
+            oosg_plan& p = *f->plan;
+            if (m->plan != &p)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: the matrix was made for another plan");
+            if (ld < std::max<int64_t>(p.n, 1) || nrhs < 1 || nrhs > 16)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: 1 to 16 right-hand sides, leading dimension >= n");
+            setDevice(&p);
+            const hipStream_t st = p.stream;
+            const int k = static_cast<int>(nrhs);
+            const int64_t n = p.n;
+            const size_t bytes = static_cast<size_t>(ld) * nrhs * sizeof(double);
+            ensure(p.solveD, bytes, "refined solve: corrections");
+            ensure(p.solveR, bytes, "refined solve: residuals");
+            double* d = p.solveD.as<double>();
+            double* r = p.solveR.as<double>();
+            const std::vector<double> bNorm = columnMax(p, b, ld, n, k);
+            check(hipEventRecord(p.start, st), "hipEventRecord");
+            const auto t0 = std::chrono::steady_clock::now();
+            check(oosg::launchFillDouble(x, ld, n, k, 0.0, st), "refined solve: zero");
+            check(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, st), "refined solve: residual");
+            double previous = INFINITY;
+            for (int32_t step = 1; step <= std::max(1, max_steps); ++step)
+            {
+                check(hipMemcpyAsync(d, r, bytes, hipMemcpyDeviceToDevice, st), "refined solve: correction");
+                solveDevice(*f, d, ld, nrhs);
+                check(oosg::launchAxpyColumns(x, d, ld, n, k, st), "refined solve: update");
+                const double residualBytes = static_cast<double>(m->csrFromCsc.size()) * 12.0 +
+                                             static_cast<double>(n + 1) * 8.0 + 2.0 * static_cast<double>(n) * k * 8.0;
+                const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes, st);
+                check(oosg::launchResidual(m->dCsrOffsets.as<int64_t>(), m->dCsrCols.as<int32_t>(),
+                                           m->dCsrValues.as<double>(), x, b, r, ld, n, k, st),
+                      "refined solve: residual");
+                p.profiler.end(slot, st);
+                const std::vector<double> rNorm = columnMax(p, r, ld, n, k);
+                const std::vector<double> xNorm = columnMax(p, x, ld, n, k);
+                double eta = 0.0;
+                for (int h = 0; h < k; ++h)
+                {
+                    const double denominator = m->normInf * xNorm[h] + bNorm[h];
+                    const double e = denominator > 0 ? rNorm[h] / denominator : rNorm[h];
+                    eta = std::max(eta, e != e ? INFINITY : e);
+                }
+                local.steps = step;
+                local.backward_error = eta;
+                if (eta <= target)
+                {
+                    local.converged = 1;
+                    break;
+                }
+                if (step > 1 && !(eta < 0.5 * previous))
+                    break; // stalled: the factor is not accurate enough for this matrix
+                previous = eta;
+            }
+            check(hipEventRecord(p.stop, st), "hipEventRecord");
+            check(hipStreamSynchronize(st), "refined solve");
+            p.profiler.collect();
+            local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (!local.converged)
+                fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +
+                                             " did not reach " + std::to_string(target) + " in " +
+                                             std::to_string(local.steps) + " steps");
+        });
+    if (report != nullptr)
+        *report = local;
+    return status;
+}
+
+// Diagnostic: one 16 × 16 × 4 MFMA product of each precision (A row-major 16 × 4, B row-major 4 × 16, D row-major
+// 16 × 16), host arrays; checks the lane layout the tile kernel assumes.
+oosg_status oosg_debug_mfma_probe(const double* a, const double* b, double* d, const float* af, const float* bf,
+                                  float* df)
+{
+    return guarded(
+        [&]
+        {
+            DeviceBuffer da, db, dd, fa, fb, fd;
+            da.allocate(64 * 8, "probe");
+            db.allocate(64 * 8, "probe");
+            dd.allocate(256 * 8, "probe");
+            fa.allocate(64 * 4, "probe");
+            fb.allocate(64 * 4, "probe");
+            fd.allocate(256 * 4, "probe");
+            check(hipMemcpy(da.ptr, a, 64 * 8, hipMemcpyHostToDevice), "probe");
+            check(hipMemcpy(db.ptr, b, 64 * 8, hipMemcpyHostToDevice), "probe");
+            check(hipMemcpy(fa.ptr, af, 64 * 4, hipMemcpyHostToDevice), "probe");
+            check(hipMemcpy(fb.ptr, bf, 64 * 4, hipMemcpyHostToDevice), "probe");
+            check(oosg::launchMfmaProbe(da.as<double>(), db.as<double>(), dd.as<double>(), fa.as<float>(),
+                                        fb.as<float>(), fd.as<float>(), nullptr),
+                  "probe");
+            check(hipDeviceSynchronize(), "probe");
+            check(hipMemcpy(d, dd.ptr, 256 * 8, hipMemcpyDeviceToHost), "probe");
+            check(hipMemcpy(df, fd.ptr, 256 * 4, hipMemcpyDeviceToHost), "probe");
+        });
+}
+
+} // extern "C"

@@ -1,0 +1,110 @@
+"""Synthetic finite-element workloads (liboosm.so): the matrices the factor and the solves are measured and tested
+on. The reference assembles its stiffness matrices from its own elements (src/elements, src/assembly), which are
+outside the numeric phase; these models are this project's own (flat MITC4 shells with a drilling penalty, trilinear
+hexahedra), generated from a seed so that a workload is the same on every machine.
+
+A Workload carries the node graph, coordinates and weights, the order (this project's nested dissection, or one
+given), the symbolic factor and the lower CSC in factor order, and builds loads in that order.
+"""
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from . import SymbolicFactor, SymmetricCsc, nested_dissection
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Workload:
+    name: str
+    handle: object
+    node_count: int
+    dof_per_node: int
+    graph_offsets: np.ndarray
+    adjacency: np.ndarray
+    coordinates: np.ndarray
+    weights: np.ndarray
+    symbolic: SymbolicFactor = None
+    matrix: SymmetricCsc = None
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _native.models().oosm_free(self.handle)
+            self.handle = None
+
+    @property
+    def n(self):
+        return self.node_count * self.dof_per_node
+
+    def analyse(self, order=None, amalgamate=True):
+        """The symbolic factor under `order` (default: nested dissection) and the CSC in its order."""
+        if order is None:
+            order = nested_dissection(self.graph_offsets, self.adjacency, self.weights, self.coordinates)
+        self.symbolic = SymbolicFactor.analyse(self.graph_offsets, self.adjacency, self.weights, order,
+                                               amalgamate=amalgamate)
+        self.matrix = self.csc(self.symbolic.node_order)
+        return self.symbolic
+
+    def csc(self, node_order):
+        m = _native.models()
+        sizes = np.zeros(4, np.int64)
+        m.oosm_sizes(self.handle, _ptr(sizes))
+        nnz = int(sizes[3])
+        offsets = np.zeros(self.n + 1, np.int64)
+        rows = np.zeros(nnz, np.int32)
+        values = np.zeros(nnz)
+        node_order = np.ascontiguousarray(node_order, dtype=np.int32)
+        m.oosm_csc(self.handle, _ptr(node_order), _ptr(offsets), _ptr(rows), _ptr(values))
+        return SymmetricCsc(self.n, offsets, rows, values)
+
+    def loads(self, kind="random", nrhs=1, seed=1):
+        """n × nrhs (Fortran order) in the factor's equation order. kind: 'pressure' (uniform pressure, shells),
+        'random' (N(0, 1) forces on the translational DOFs), 'top' (N(0, 1) on the top face's nodes)."""
+        code = {"pressure": 0, "random": 1, "top": 2}[kind]
+        b = np.zeros((nrhs, self.n))
+        _native.models().oosm_loads(self.handle, _ptr(self.symbolic.node_order), code, int(nrhs), int(seed), _ptr(b))
+        return np.asfortranarray(b.T)
+
+
+def _workload(name, handle):
+    m = _native.models()
+    sizes = np.zeros(4, np.int64)
+    m.oosm_sizes(handle, _ptr(sizes))
+    n_nodes, dof = int(sizes[0]), int(sizes[1])
+    offsets = np.zeros(n_nodes + 1, np.int64)
+    adjacency = np.zeros(int(sizes[2]), np.int32)
+    m.oosm_graph(handle, _ptr(offsets), _ptr(adjacency))
+    coords = np.zeros(3 * n_nodes)
+    m.oosm_coords(handle, _ptr(coords))
+    return Workload(name, handle, n_nodes, dof, offsets, adjacency, coords, np.full(n_nodes, dof, np.int32))
+
+
+def plate(nx, ny=None, length=1.0, width=None, jitter=0.0, seed=0, thickness=(2e-3, 2e-3), thickness_mode="constant",
+          E=70e9, nu=0.3, pressure=1e3):
+    """A flat plate of nx × ny quadrilateral shells, all edges clamped (6 DOFs per kept node)."""
+    ny = nx if ny is None else ny
+    width = length if width is None else width
+    mode = {"constant": 0, "log-uniform": 1, "uniform": 2}[thickness_mode]
+    handle = _native.models().oosm_plate(nx, ny, length, width, jitter, seed, thickness[0], thickness[1], mode, E, nu,
+                                         pressure)
+    return _workload(f"plate {nx}x{ny}", handle)
+
+
+def hex_block(nx, ny=None, nz=None, size=(1.0, 1.0, 1.0), seed=0, E_median=200e9, sigma=0.0, nu=0.3):
+    """A block of nx × ny × nz hexahedra, the face z = 0 fixed (3 DOFs per kept node)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    handle = _native.models().oosm_hex_block(nx, ny, nz, size[0], size[1], size[2], seed, E_median, sigma, nu)
+    return _workload(f"hex block {nx}x{ny}x{nz}", handle)
+
+
+def shell_plate_c2(scale=500, seed=20261203):
+    """The shell plate of the headline workload: scale × scale quad4 over 1 m, interior nodes jittered ±20 % of the
+    element size, per-element thickness log-uniform in [1, 4] mm, E = 70 GPa, ν = 0.3, clamped edges."""
+    return plate(scale, jitter=0.2, seed=seed, thickness=(1e-3, 4e-3), thickness_mode="log-uniform")

@@ -1,0 +1,155 @@
+"""ctypes access to the checkers: oracle/liboracle.so (the C restatement) and oracle/_ref/liboos_ref.so (the compiled
+reference, present where it was built). Test infrastructure only."""
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "liboos_ref.so")
+
+vp = ctypes.c_void_p
+i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+class OrcSymbolic(ctypes.Structure):
+    _fields_ = [("n", i32), ("supernode_count", i32), ("first_column", vp), ("column_count", vp), ("parent", vp),
+                ("row_start", vp), ("rows", vp)]
+
+
+class OrcResult(ctypes.Structure):
+    _fields_ = [("status", i32), ("zero_pivot_count", i32), ("first_zero_pivot", i32), ("first_issue", i32),
+                ("first_issue_negative", i32), ("largest_ratio_equation", i32), ("issue_count", i64),
+                ("inertia", i64 * 3), ("largest_ratio", dbl)]
+
+
+_orc = None
+_ref = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        lib = ctypes.CDLL(ORACLE_PATH)
+        lib.orc_ldlt_nopivot.restype = i64
+        lib.orc_ldlt_nopivot.argtypes = [vp, i64, i64, i64]
+        lib.orc_front_nopivot.restype = i64
+        lib.orc_front_nopivot.argtypes = [vp, i64, vp, i64, i64, i64]
+        lib.orc_bunch_kaufman.restype = i64
+        lib.orc_bunch_kaufman.argtypes = [vp, i64, i64, i64, vp, vp]
+        lib.orc_inertia.restype = None
+        lib.orc_inertia.argtypes = [vp, vp, i64, vp]
+        lib.orc_spmv.restype = None
+        lib.orc_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
+        lib.orc_multifrontal_pd.restype = None
+        lib.orc_multifrontal_pd.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, vp, dbl, vp, vp,
+                                            ctypes.POINTER(OrcResult)]
+        lib.orc_solve.restype = None
+        lib.orc_solve.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64]
+        _orc = lib
+    return _orc
+
+
+def reference():
+    """The compiled reference, or None where it was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        lib = ctypes.CDLL(REF_PATH)
+        lib.ref_analyse.restype = vp
+        lib.ref_analyse.argtypes = [i32, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                    ctypes.c_char_p, ctypes.c_int]
+        lib.ref_sym_free.argtypes = [vp]
+        lib.ref_sym_sizes.argtypes = [vp, vp, vp]
+        lib.ref_sym_export.argtypes = [vp] * 7
+        lib.ref_factor_solve.restype = ctypes.c_int
+        lib.ref_factor_solve.argtypes = [vp, i32, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dbl,
+                                         ctypes.c_int, vp, i64, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_spmv.restype = ctypes.c_int
+        lib.ref_spmv.argtypes = [i32, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_dense_front_nopivot.restype = i64
+        lib.ref_dense_front_nopivot.argtypes = [vp, i64, vp, i64, i64, i64]
+        lib.ref_dense_bunch_kaufman.restype = i64
+        lib.ref_dense_bunch_kaufman.argtypes = [vp, i64, i64, i64, vp, vp]
+        lib.ref_hardware_threads.restype = ctypes.c_int
+        _ref = lib
+    return _ref
+
+
+def orc_symbolic(sym):
+    """An OrcSymbolic over a SymbolicFactor's arrays (keep `sym` alive while it is used)."""
+    return OrcSymbolic(sym.n, sym.supernode_count, P(sym.first_column), P(sym.column_count), P(sym.parent),
+                       P(sym.row_start), P(sym.rows_all))
+
+
+def multifrontal(sym, matrix, max_ratio=1e7):
+    """(panels, D per equation, OrcResult) of the oracle's positive definite multifrontal factorisation."""
+    lib = oracle()
+    panels = np.zeros(sym.panel_value_count)
+    diag = np.zeros(sym.n)
+    res = OrcResult()
+    s = orc_symbolic(sym)
+    lib.orc_multifrontal_pd(ctypes.byref(s), P(matrix.offsets), P(matrix.rows), P(matrix.values), max_ratio,
+                            P(panels), P(diag), ctypes.byref(res))
+    return panels, diag, res
+
+
+def oracle_solve(sym, panels, b):
+    x = np.array(b, dtype=np.float64, order="F", copy=True).reshape(sym.n, -1, order="F")
+    s = orc_symbolic(sym)
+    oracle().orc_solve(ctypes.byref(s), P(panels), P(x), sym.n, x.shape[1])
+    return x
+
+
+def oracle_spmv(matrix, x):
+    y = np.zeros(matrix.n)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    oracle().orc_spmv(matrix.n, P(matrix.offsets), P(matrix.rows), P(matrix.values), P(x), P(y))
+    return y
+
+
+def ref_symbolic(graph_offsets, adjacency, weights, coords=None, order=None, method=4, amalgamate=1, threads=1):
+    """The reference's SymbolicFactor::analyse as (handle, sizes, flops, arrays dict); free with ref_sym_free."""
+    lib = reference()
+    n = int(len(graph_offsets) - 1)
+    status = ctypes.c_int()
+    err = ctypes.create_string_buffer(1024)
+    h = lib.ref_analyse(n, P(graph_offsets), P(adjacency), P(weights), P(coords), P(order), method, amalgamate,
+                        threads, ctypes.byref(status), err, 1024)
+    if status.value != 0:
+        raise RuntimeError(err.value.decode())
+    sizes = np.zeros(7, np.int64)
+    flops = dbl()
+    lib.ref_sym_sizes(h, P(sizes), ctypes.byref(flops))
+    S = int(sizes[2])
+    arrays = dict(node_order=np.zeros(n, np.int32), first_column=np.zeros(S, np.int32),
+                  column_count=np.zeros(S, np.int32), parent=np.zeros(S, np.int32),
+                  row_start=np.zeros(S + 1, np.int64), rows=np.zeros(int(sizes[3]), np.int32))
+    lib.ref_sym_export(h, *[P(arrays[k]) for k in ["node_order", "first_column", "column_count", "parent",
+                                                     "row_start", "rows"]])
+    return h, sizes, flops.value, arrays
+
+
+def ref_factor_solve(handle, matrix, b=None, kernel=0, threads=1, pivoting=0, max_ratio=1e7, bailout=0,
+                     panel_values=0):
+    """The reference's factor (+ solve of b in place when given): (status, message, inertia, counts, times, panels,
+    diag)."""
+    lib = reference()
+    inertia = np.zeros(3, np.int64)
+    counts = np.zeros(4, np.int64)
+    times = np.zeros(2)
+    panels = np.zeros(panel_values) if panel_values else None
+    diag = np.zeros(matrix.n)
+    err = ctypes.create_string_buffer(4096)
+    nrhs = 0 if b is None else b.shape[1]
+    rc = lib.ref_factor_solve(handle, matrix.n, P(matrix.offsets), P(matrix.rows), P(matrix.values), kernel, threads,
+                              pivoting, max_ratio, bailout, P(b), nrhs, P(inertia), P(counts), P(times), P(panels),
+                              P(diag), err, 4096)
+    return rc, err.value.decode(errors="replace"), inertia, counts, times, panels, diag

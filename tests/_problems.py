@@ -1,0 +1,69 @@
+"""Small scalar test problems in the form of the reference's checks (checks/linalg/FactorEdgeChecks.cpp:50-104,
+checks/linalg/FactorOrderChecks.cpp:54-114): node graphs of chains and grids, and their matrices assembled in the
+equation order of a symbolic factor."""
+
+import numpy as np
+
+from o_o_solver_amd import SymmetricCsc
+
+
+def chain_graph(count):
+    """A chain of `count` scalar nodes (node v next to v − 1 and v + 1)."""
+    offsets, nb = [0], []
+    for v in range(count):
+        if v > 0:
+            nb.append(v - 1)
+        if v + 1 < count:
+            nb.append(v + 1)
+        offsets.append(len(nb))
+    return (np.array(offsets, np.int64), np.array(nb, np.int32), np.ones(count, np.int32),
+            np.stack([np.arange(count, dtype=float), np.zeros(count), np.zeros(count)], 1))
+
+
+def grid_graph(side):
+    """side × side scalar nodes, each next to its four neighbours (node i + side·j)."""
+    offsets, nb = [0], []
+    for j in range(side):
+        for i in range(side):
+            if j > 0:
+                nb.append(i + side * (j - 1))
+            if i > 0:
+                nb.append(i - 1 + side * j)
+            if i + 1 < side:
+                nb.append(i + 1 + side * j)
+            if j + 1 < side:
+                nb.append(i + side * (j + 1))
+            offsets.append(len(nb))
+    coords = np.array([[i, j, 0.0] for j in range(side) for i in range(side)], float)
+    return np.array(offsets, np.int64), np.array(nb, np.int32), np.ones(side * side, np.int32), coords
+
+
+def graph_matrix(symbolic, graph_offsets, adjacency, diagonal, coupling):
+    """The matrix of a scalar graph in the equations of `symbolic`: diagonal[v] at node v, coupling(a, b) between
+    neighbours a < b; each column's rows ascending."""
+    position = np.empty(len(symbolic.node_order), np.int64)
+    position[symbolic.node_order] = np.arange(len(symbolic.node_order))
+    n = len(position)
+    columns = [[] for _ in range(n)]
+    for a in range(n):
+        ea = position[a]
+        columns[ea].append((ea, float(diagonal[a])))
+        for b in adjacency[graph_offsets[a]:graph_offsets[a + 1]]:
+            if b > a:
+                eb = position[b]
+                columns[min(ea, eb)].append((max(ea, eb), float(coupling(a, b))))
+    offsets, rows, values = [0], [], []
+    for col in columns:
+        col.sort()
+        rows.extend(r for r, _ in col)
+        values.extend(v for _, v in col)
+        offsets.append(len(rows))
+    return SymmetricCsc(n, np.array(offsets, np.int64), np.array(rows, np.int32), np.array(values))
+
+
+def random_spd(n, seed, shift=None):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    a = a @ a.T
+    a += np.eye(n) * (n if shift is None else shift)
+    return a

@@ -1,0 +1,215 @@
+"""Parity of the GPU factor and solve (liboosg.so, HIP on gfx950) with the oracle and the reference's fixtures.
+
+Contract (FP64): inertia exact; D and the panels within 1e-9 of the oracle's, normwise relative per supernode; the
+solution within 1e-8 of the oracle's (relative, max norm) and a backward error below 1e-13. FP32 fronts: refined in
+FP64 to a backward error of 1e-10 or less, the solution within 1e-8 of the FP64 one.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import _oracle
+from _problems import chain_graph, graph_matrix
+import o_o_solver_amd as oos
+from o_o_solver_amd import FactorOptions, FactorSolve, Precision, SupernodalFactor, SymbolicFactor, SymmetricCsc, models
+
+pytestmark = pytest.mark.gpu
+
+
+def backward_error(matrix, x, b):
+    dense = matrix.to_dense()
+    r = b - dense @ x
+    return np.max(np.abs(r), axis=0).max() / (np.abs(dense).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+
+
+def test_mfma_lane_layout(gpu_device):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((16, 4))
+    b = rng.standard_normal((4, 16))
+    d = np.zeros((16, 16))
+    af = a.astype(np.float32)
+    bf = b.astype(np.float32)
+    df = np.zeros((16, 16), np.float32)
+    P = _oracle.P
+    status = oos._native.lib().oosg_debug_mfma_probe(P(np.ascontiguousarray(a)), P(np.ascontiguousarray(b)), P(d),
+                                                     P(np.ascontiguousarray(af)), P(np.ascontiguousarray(bf)), P(df))
+    assert status == 0
+    assert np.allclose(d, a @ b, rtol=1e-14, atol=1e-14)
+    assert np.allclose(df, af @ bf, rtol=1e-5, atol=1e-5)
+
+
+def _panels_close(factor, sym, oracle_panels, tol):
+    offset = 0
+    worst = 0.0
+    for s in range(sym.supernode_count):
+        c = int(sym.column_count[s])
+        m = c + sym.row_count(s)
+        ref = oracle_panels[offset:offset + c * m].reshape(c, m).T
+        offset += c * m
+        got, rows = factor.panel(s)
+        scale = max(np.abs(ref).max(), 1e-300)
+        worst = max(worst, np.abs(got - ref).max() / scale)
+    assert worst <= tol, worst
+
+
+CASES = {
+    "plate 12": lambda: models.plate(12, jitter=0.2, seed=5, thickness=(1e-3, 4e-3), thickness_mode="log-uniform"),
+    "plate 40": lambda: models.plate(40, jitter=0.2, seed=6, thickness=(1e-3, 4e-3), thickness_mode="log-uniform"),
+    "hex 7": lambda: models.hex_block(7, seed=3, sigma=1.5),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_factor_matches_oracle(gpu_device, case):
+    print(f"[gpu] fp64 factor {case}", flush=True)
+    w = CASES[case]()
+    sym = w.analyse()
+    m = w.matrix
+    panels, diag, res = _oracle.multifrontal(sym, m)
+    assert res.status == 0
+    factor = SupernodalFactor.factorise(sym, m)
+    assert (factor.inertia.positive, factor.inertia.negative, factor.inertia.zero) == tuple(res.inertia)
+    d = factor.diagonal()
+    assert np.max(np.abs(d - diag) / np.abs(diag)) <= 1e-9
+    _panels_close(factor, sym, panels, 1e-9)
+    b = w.loads("random", nrhs=3, seed=2)
+    x = FactorSolve(factor).solve(b)
+    xo = _oracle.oracle_solve(sym, panels, b)
+    assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
+    if m.n <= 3000:
+        assert backward_error(m, x, b) <= 1e-13
+
+
+def test_large_path_is_exercised(gpu_device):
+    w = CASES["hex 7"]()
+    sym = w.analyse()
+    stats = sym.plan().stats()
+    assert stats["large_fronts"] > 0 and stats["small_fronts"] > 0
+    assert int(np.max(sym.column_count)) > 64  # several panel steps
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 16, 20])
+def test_right_hand_side_counts(gpu_device, nrhs):
+    w = CASES["plate 12"]()
+    sym = w.analyse()
+    factor = SupernodalFactor.factorise(sym, w.matrix)
+    panels, _, _ = _oracle.multifrontal(sym, w.matrix)
+    b = w.loads("random", nrhs=nrhs, seed=nrhs)
+    x = FactorSolve(factor).solve(b)
+    xo = _oracle.oracle_solve(sym, panels, b)
+    assert x.shape == (w.n, nrhs)
+    assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+def test_device_solve_and_multiply(gpu_device):
+    import torch
+    w = CASES["plate 12"]()
+    sym = w.analyse()
+    factor = SupernodalFactor.factorise(sym, w.matrix)
+    b = w.loads("random", nrhs=2, seed=8)
+    xh = FactorSolve(factor).solve(b)
+    bd = torch.tensor(b, device=gpu_device)
+    xd = FactorSolve(factor).solve(bd)
+    assert np.allclose(xd.cpu().numpy(), xh, rtol=0, atol=1e-14 * np.abs(xh).max())
+    y = factor.device_matrix.multiply(xd)
+    assert np.allclose(y.cpu().numpy(), w.matrix.to_dense() @ xh, rtol=1e-10, atol=1e-10 * np.abs(b).max())
+
+
+def test_golden_plate_matches_reference(gpu_device):
+    g = np.load(__file__.rsplit("/", 1)[0] + "/golden/plate8.npz")
+    sym = SymbolicFactor(int(g["n"]), g["first_column"], g["column_count"], g["parent"], g["row_start"], g["rows"],
+                         node_order=g["node_order"])
+    m = SymmetricCsc(int(g["n"]), g["offsets"], g["csc_rows"], g["values"])
+    factor = SupernodalFactor.factorise(sym, m)
+    assert [factor.inertia.positive, factor.inertia.negative, factor.inertia.zero] == g["inertia"].tolist()
+    assert np.max(np.abs(factor.diagonal() - g["diag"]) / np.abs(g["diag"])) <= 1e-9
+    x = FactorSolve(factor).solve(g["b"])
+    assert np.max(np.abs(x - g["x"])) <= 1e-8 * np.max(np.abs(g["x"]))
+
+
+def test_golden_grid_matches_reference(gpu_device):
+    g = np.load(__file__.rsplit("/", 1)[0] + "/golden/grid20.npz")
+    sym = SymbolicFactor(400, g["first_column"], g["column_count"], g["parent"], g["row_start"], g["rows"],
+                         node_order=g["node_order"])
+    m = SymmetricCsc(400, g["pd_offsets"], g["pd_csc_rows"], g["pd_values"])
+    factor = SupernodalFactor.factorise(sym, m)
+    assert [factor.inertia.positive, factor.inertia.negative, factor.inertia.zero] == g["pd_inertia"].tolist()
+    x = FactorSolve(factor).solve(g["b"])
+    assert np.max(np.abs(x - g["pd_x"])) <= 1e-12 * np.max(np.abs(g["pd_x"]))
+
+
+@pytest.mark.parametrize("last_diagonal,negatives", [(0.0, 1), (-1.0, 1), (2.0, 0)])
+def test_two_by_two_known_answers(gpu_device, last_diagonal, negatives):
+    # checks/linalg/FactorEdgeChecks.cpp:151-189 on the GPU.
+    g = chain_graph(2)
+    sym = SymbolicFactor.analyse(g[0], g[1], g[2], np.arange(2, dtype=np.int32))
+    diag = np.full(2, 2.0)
+    diag[sym.node_order[1]] = last_diagonal
+    m = graph_matrix(sym, g[0], g[1], diag, lambda a, b: 1.0)
+    options = FactorOptions(pivot_policy=oos.PivotPolicy(bailout=-1))
+    factor = SupernodalFactor.factorise(sym, m, options)
+    assert factor.inertia.negative == negatives
+    diagnostics = factor.pivot_diagnostics
+    assert diagnostics.issue_count == negatives
+    if negatives:
+        assert diagnostics.first_issue_equation == 1 and diagnostics.first_issue_negative
+        with pytest.raises(oos.SingularityError):
+            SupernodalFactor.factorise(sym, m)
+
+
+def test_zero_pivot_is_singular(gpu_device):
+    g = chain_graph(3)
+    sym = SymbolicFactor.analyse(g[0], g[1], g[2], np.arange(3, dtype=np.int32))
+    m = graph_matrix(sym, g[0], g[1], np.array([2.0, 0.5, 3.0]), lambda a, b: 1.0)
+    with pytest.raises(oos.SingularityError, match="equation 1"):
+        SupernodalFactor.factorise(sym, m)
+
+
+def test_entry_outside_structure_is_refused(gpu_device):
+    g = chain_graph(4)
+    # Fundamental supernodes {0}, {1}, {2, 3}: column 0 holds row 1 only.
+    sym = SymbolicFactor.analyse(g[0], g[1], g[2], np.arange(4, dtype=np.int32), amalgamate=False)
+    # A coupling between equations 0 and 3, which the chain's structure does not hold.
+    m = SymmetricCsc(4, np.array([0, 3, 5, 7, 8]), np.array([0, 1, 3, 1, 2, 2, 3, 3], np.int32), np.ones(8) * 4)
+    with pytest.raises(ValueError, match="outside the structure"):
+        SupernodalFactor.factorise(sym, m)
+
+
+def test_refactorisation_reuses_the_plan(gpu_device):
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    m = w.matrix
+    first = SupernodalFactor.factorise(sym, m)
+    dm = first.device_matrix
+    del first
+    shifted = m.values.copy()
+    diag_entries = m.offsets[:-1]
+    shifted[diag_entries] *= 1.5
+    dm.set_values(shifted)
+    m2 = SymmetricCsc(m.n, m.offsets, m.rows, shifted)
+    second = SupernodalFactor.factorise(sym, m2, device_matrix=dm)
+    _, diag, res = _oracle.multifrontal(sym, m2)
+    assert np.max(np.abs(second.diagonal() - diag) / np.abs(diag)) <= 1e-9
+
+
+@pytest.mark.parametrize("case", ["plate 40", "hex 7"])
+def test_fp32_fronts_refined_to_fp64_accuracy(gpu_device, case):
+    import torch
+    print(f"[gpu] fp32 refined {case}", flush=True)
+    w = CASES[case]()
+    sym = w.analyse()
+    f64 = SupernodalFactor.factorise(sym, w.matrix)
+    f32 = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=Precision.FP32),
+                                     device_matrix=f64.device_matrix)
+    assert f32.inertia == f64.inertia
+    b = w.loads("random", nrhs=4, seed=4)
+    bd = torch.tensor(b, device=gpu_device)
+    x32, report = FactorSolve(f32).solve_refined(bd, target=1e-10, max_steps=30)
+    assert report.converged and report.backward_error <= 1e-10
+    x64, r64 = FactorSolve(f64).solve_refined(bd, target=1e-10)
+    assert r64.steps <= 2
+    x32 = x32.cpu().numpy()
+    x64 = x64.cpu().numpy()
+    assert np.max(np.abs(x32 - x64)) <= 1e-8 * np.max(np.abs(x64))
