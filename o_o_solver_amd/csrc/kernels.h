@@ -15,6 +15,10 @@ constexpr int PANEL_BLOCK = 64;
 constexpr int SMALL_FRONT_MAX = 96;
 // Elements of a front a workgroup of the assembly kernel holds in LDS.
 constexpr int ASSEMBLY_BLOCK_VALUES = 8192;
+// Supernodes with at least this many rows below their columns spread the products of their sweeps over workgroups.
+constexpr int BIG_ROWS = 256;
+// Supernodes with at least this many columns run the column blocks of their sweeps as steps over many workgroups.
+constexpr int WIDE_COLUMNS = 192;
 
 // The symbolic structure in device memory (read-only during a factorisation or a solve).
 struct DevicePlan
@@ -29,6 +33,7 @@ struct DevicePlan
     const int32_t* rel;         // [row total] position of each row of s in its parent's front
     const int32_t* childStart;  // [S + 1]
     const int32_t* children;    // children in increasing order
+    const int64_t* linvOff;     // [S] offsets of the inverses of the diagonal blocks (PANEL_BLOCK columns each)
 };
 
 // A matrix in device memory: its lower CSC, the front position of each entry (within the front of the supernode
@@ -47,34 +52,44 @@ struct AssemblyBlock
     int32_t supernode, j0, j1, i0, i1, pad;
 };
 
-// One 64 × 64 tile of C −= A·diag(d)·Bᵀ (element offsets into the panels; C in the panels or the update matrices).
+// One 64 × 64 MFMA tile (element offsets): mode 0 C −= A·diag(d)·Bᵀ with A, B, d in the panels and C in the panels
+// (target 0) or the update matrices (target 1); mode 1 C = A·Bᵀ·diag(d)⁻¹ with B an inverse of a diagonal block.
 struct UpdateTile
 {
-    int64_t cOff, aOff, bOff, dOff, ld, ldc;
-    int32_t m, n, k, target, lower, pad;
+    int64_t cOff, aOff, bOff, dOff, ld, ldb, ldc;
+    int32_t m, n, k, target, lower, mode;
 };
 
 template <typename T>
-hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates, const int32_t* list,
-                             int count, int maxOrder, double* diagOut, int32_t* zeroAt, hipStream_t stream);
+hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates, T* linv,
+                             const int32_t* list, int count, int maxOrder, double* diagOut, int32_t* zeroAt,
+                             hipStream_t stream);
 template <typename T> size_t smallFrontBytes(int maxOrder);
 template <typename T>
 hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates,
                           const AssemblyBlock* blocks, int count, hipStream_t stream);
 template <typename T>
-hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, const int32_t* fronts, int frontCount, int k0,
+hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const int32_t* fronts, int frontCount, int k0,
                                 double* diagOut, int32_t* zeroAt, hipStream_t stream);
 template <typename T>
-hipError_t launchPanelSolve(const DevicePlan& p, T* panels, const int2* chunks, int chunkCount, int k0,
-                            hipStream_t stream);
-template <typename T>
-hipError_t launchUpdateTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream);
+hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
+                             hipStream_t stream);
+// One level of the forward sweep: the supernodes of `list`, then the row tiles (supernode, tile) of its big ones.
 template <typename TP>
-hipError_t launchForward(const DevicePlan& p, const TP* panels, double* X, double* U, int k, const int32_t* list,
-                         int count, hipStream_t stream);
+hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                         const int32_t* list, int count, const int2* rowTiles, int tileCount, hipStream_t stream);
+// One level of the backward sweep: the column blocks (supernode, block) of its big supernodes, then all of `list`.
 template <typename TP>
-hipError_t launchBackward(const DevicePlan& p, const TP* panels, double* X, int k, const int32_t* list, int count,
-                          hipStream_t stream);
+hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
+                          int count, const int2* columnBlocks, int blockCount, hipStream_t stream);
+// One step of the sweeps of the wide supernodes of a level (items (supernode, tile or block), −1: finalise a block).
+template <typename TP>
+hipError_t launchForwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                             const int2* items, int count, int step, hipStream_t stream);
+template <typename TP>
+hipError_t launchBackwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                              const int2* items, int count, int step, hipStream_t stream);
+hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream);
 
 hipError_t launchFillInt32(int32_t* data, int64_t count, int32_t value, hipStream_t stream);
 hipError_t launchToRowMajor(const double* src, int64_t ld, double* dst, int64_t n, int k, const double* scale,

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 #include "kernels.h"
 
@@ -50,19 +51,81 @@ __device__ inline int lowerBound(const int32_t* rel, int count, int value)
     return lo;
 }
 
+// f(i, j) for i in [0, rows), j in [0, cols), spread over the workgroup without integer division: rows across the
+// threads when they are many, else rows across the lanes of a wave and columns across the waves.
+template <typename F> __device__ inline void forEach2D(int rows, int cols, F f)
+{
+    if (rows >= int(blockDim.x))
+    {
+        for (int j = 0; j < cols; ++j)
+            for (int i = threadIdx.x; i < rows; i += blockDim.x)
+                f(i, j);
+    }
+    else
+    {
+        const int lane = threadIdx.x & 63, groups = blockDim.x >> 6;
+        for (int j = threadIdx.x >> 6; j < cols; j += groups)
+            for (int i = lane; i < rows; i += 64)
+                f(i, j);
+    }
+}
+
+// The rank-one step of column k on the lower triangle of rows and columns [k + 1, end): F(i, j) −= L(i, k)·W(j, k),
+// W(j, k) = L(j, k)·d_k (L(·, k) in lk), the reference's chain (DenseBlockKernels.cpp:376-390); a column per wave,
+// its rows across the lanes.
+template <typename T> __device__ inline void rankOneStep(T* F, int ld, int k, int end, const T* lk, T pivot)
+{
+    const int lane = threadIdx.x & 63, groups = blockDim.x >> 6;
+    for (int j = k + 1 + (threadIdx.x >> 6); j < end; j += groups)
+    {
+        const T w = lk[j] * pivot;
+        for (int i = j + lane; i < end; i += 64)
+            F[i + j * ld] -= lk[i] * w;
+    }
+}
+
 __global__ void fillInt32Kernel(int32_t* data, int64_t count, int32_t value)
 {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
         data[i] = value;
 }
 
+// ---- unit lower inverses of the diagonal blocks ------------------------------------------------------------------
+//
+// The inverse X = L_JJ⁻¹ of each diagonal block of PANEL_BLOCK columns is accumulated during the elimination itself
+// (Gauss-Jordan on [L | I]): once column k of L is known, X(i, j) −= L(i, k)·X(k, j) for the rows i > k of its block
+// and the columns j <= k, every pair in parallel (X(k, k) = 1). X(i, j), i > j, is kept at F[j + i·ld], in the block's
+// upper triangle, which the elimination leaves free. The inverses turn the panel solves into MFMA products and the
+// in-block triangles of the sweeps into small products.
+
+// The step of column k (local to a block starting at base = q0 + q0·ld, kb columns): `lk` holds L(i, k) at index i.
+template <typename T>
+__device__ inline void inverseStep(T* F, int ld, int q0, int kb, int k, const T* lk)
+{
+    const int lane = threadIdx.x & 63, groups = blockDim.x >> 6;
+    for (int j = q0 + (threadIdx.x >> 6); j <= k; j += groups)
+    {
+        const T xkj = j == k ? T(1) : F[j + k * ld];
+        for (int i = k + 1 + lane; i < q0 + kb; i += 64)
+            F[j + i * ld] -= lk[i] * xkj;
+    }
+}
+
+// Writes the inverse accumulated in the upper triangle of the block at (q0, q0): column-major kb × kb, with its unit
+// diagonal and zeros above.
+template <typename T>
+__device__ inline void writeInverse(const T* F, int ld, int q0, int kb, T* out)
+{
+    forEach2D(kb, kb, [&](int i, int j) { out[i + j * kb] = i == j ? T(1) : (i > j ? F[(q0 + j) + (q0 + i) * ld] : T(0)); });
+}
+
 // ---- small fronts: assembly and elimination fused, the whole front in LDS ------------------------------------------
 
 // One workgroup per front of at most SMALL_FRONT_MAX rows: the front is assembled in LDS (zero, matrix entries,
-// children in order), its c columns are eliminated right-looking (two barriers per column), and the panel and the
-// update matrix are written once.
+// children in order), its c columns are eliminated right-looking (two barriers per column), and the panel, the update
+// matrix and the inverses of the diagonal blocks are written once.
 template <typename T>
-__global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatrix a, T* panels, T* updates,
+__global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatrix a, T* panels, T* updates, T* linv,
                                                         const int32_t* list, double* diagOut, int32_t* zeroAt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -96,12 +159,10 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
         const int rc = p.nrow[ch];
         const int32_t* rel = p.rel + p.rowStart[ch];
         const T* u = updates + p.updateOff[ch];
-        for (int idx = tid; idx < rc * rc; idx += nt)
-        {
-            const int i = idx % rc, jj = idx / rc;
+        forEach2D(rc, rc, [&](int i, int jj) {
             if (i >= jj)
-                F[rel[i] + rel[jj] * n] += u[idx];
-        }
+                F[rel[i] + rel[jj] * n] += u[i + jj * rc];
+        });
     }
     __syncthreads();
 
@@ -114,33 +175,25 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
         for (int i = k + 1 + tid; i < n; i += nt)
             lcol[i] = F[i + k * n] / pivot;
         __syncthreads();
-        const int m = n - k - 1;
-        for (int idx = tid; idx < m * m; idx += nt)
-        {
-            const int i = k + 1 + idx % m, j = k + 1 + idx / m;
-            if (i >= j)
-                F[i + j * n] -= lcol[i] * (lcol[j] * pivot);
-        }
+        rankOneStep(F, n, k, n, lcol, pivot);
         for (int i = k + 1 + tid; i < n; i += nt)
             F[i + k * n] = lcol[i];
+        const int q0 = (k / PANEL_BLOCK) * PANEL_BLOCK;
+        inverseStep(F, n, q0, min(PANEL_BLOCK, c - q0), k, lcol);
         __syncthreads();
     }
 
     T* panel = panels + p.panelOff[s];
-    for (int idx = tid; idx < n * c; idx += nt)
-    {
-        const int i = idx % n, j = idx / n;
-        panel[idx] = i < j ? T(0) : F[idx];
-    }
+    forEach2D(n, c, [&](int i, int j) { panel[i + j * n] = i < j ? T(0) : F[i + j * n]; });
     T* update = updates + p.updateOff[s];
-    for (int idx = tid; idx < r * r; idx += nt)
-    {
-        const int i = idx % r, j = idx / r;
+    forEach2D(r, r, [&](int i, int j) {
         if (i >= j)
-            update[idx] = F[(c + i) + (c + j) * n];
-    }
+            update[i + j * r] = F[(c + i) + (c + j) * n];
+    });
     for (int k = tid; k < c; k += nt)
         diagOut[first + k] = double(F[k + k * n]);
+    for (int q0 = 0; q0 < c; q0 += PANEL_BLOCK)
+        writeInverse(F, n, q0, min(PANEL_BLOCK, c - q0), linv + p.linvOff[s] + q0 * PANEL_BLOCK);
 }
 
 // ---- large fronts --------------------------------------------------------------------------------------------------
@@ -194,118 +247,116 @@ __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < rowsIn * colsIn; idx += nt)
-    {
-        const int il = idx % rowsIn, jl = idx / rowsIn;
+    forEach2D(rowsIn, colsIn, [&](int il, int jl) {
         const int i = blk.i0 + il, j = blk.j0 + jl;
         if (j < c)
-            panels[p.panelOff[s] + i + int64_t(j) * n] = B[idx];
+            panels[p.panelOff[s] + i + int64_t(j) * n] = B[il + jl * rowsIn];
         else if (i >= j)
-            updates[p.updateOff[s] + (i - c) + int64_t(j - c) * r] = B[idx];
-    }
+            updates[p.updateOff[s] + (i - c) + int64_t(j - c) * r] = B[il + jl * rowsIn];
+    });
 }
 
-// The diagonal block [k0, k0 + kb) of a front's panel factorised in place (kb <= PANEL_BLOCK), one workgroup per front.
-template <typename T>
-__global__ __launch_bounds__(256) void diagonalBlockKernel(DevicePlan p, T* panels, const int32_t* list, int k0,
-                                                           double* diagOut, int32_t* zeroAt)
+// The value of lane `lane` (uniform across the wave) on every lane: v_readlane into a scalar register, a few cycles,
+// where a general shuffle (ds_bpermute) waits on LDS latency.
+__device__ inline float broadcastLane(float v, int lane)
 {
-    __shared__ T D[PANEL_BLOCK * (PANEL_BLOCK + 1)];
-    __shared__ T lcol[PANEL_BLOCK];
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ inline double broadcastLane(double v, int lane)
+{
+    const long long bits = __builtin_bit_cast(long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(bits), lane));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(bits >> 32), lane));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// The diagonal block [k0, k0 + kb) of a front's panel factorised in place (kb <= PANEL_BLOCK), one wave per front.
+// Elimination: lane i holds row i of the block in registers, from the current column on (the row shifts by one
+// register per step, so every register index is a constant); each step publishes L(·, K) in LDS (one store per lane)
+// and every lane reads the column back by broadcast reads. Inverse L⁻¹ of the unit lower part: lane i holds column i
+// of the inverse from row K on (shifting likewise), x(K + t) −= L(K + t, K)·x(K) (Gauss-Jordan on [L | I]), L(·, K)
+// read from LDS, each finished entry x(K) written out. Rows beyond kb hold the identity.
+template <typename T>
+__global__ __launch_bounds__(64) void diagonalBlockKernel(DevicePlan p, T* panels, T* linv, const int32_t* list,
+                                                          int k0, double* diagOut, int32_t* zeroAt)
+{
+    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1;
+    __shared__ T Lsh[B * LD]; // Lsh[j·LD + i]: L(i, j) below the diagonal, D on it, 0 above
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], n = c + p.nrow[s];
-    const int kb = min(PANEL_BLOCK, c - k0);
-    const int ld = PANEL_BLOCK + 1;
+    const int kb = min(B, c - k0);
     T* panel = panels + p.panelOff[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < kb * kb; idx += nt)
+    const int i = threadIdx.x;
+    T a[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+        a[j] = (i < kb && j <= i) ? panel[(k0 + i) + int64_t(k0 + j) * n] : T(i == j ? 1 : 0);
+    for (int K = 0; K < B; ++K)
     {
-        const int i = idx % kb, j = idx / kb;
-        D[i + j * ld] = panel[(k0 + i) + int64_t(k0 + j) * n];
-    }
-    __syncthreads();
-    for (int k = 0; k < kb; ++k)
-    {
-        const T pivot = D[k + k * ld];
-        if (tid == 0 && isBadPivot(pivot))
-            atomicMin(&zeroAt[s], k0 + k);
-        for (int i = k + 1 + tid; i < kb; i += nt)
-            lcol[i] = D[i + k * ld] / pivot;
-        __syncthreads();
-        const int m = kb - k - 1;
-        for (int idx = tid; idx < m * m; idx += nt)
+        const T pivot = broadcastLane(a[0], K);
+        if (i == 0 && K < kb && isBadPivot(pivot))
+            atomicMin(&zeroAt[s], k0 + K);
+        const T l = i > K ? a[0] / pivot : T(0);
+        T* column = Lsh + K * LD;
+        column[i] = i > K ? l : (i == K ? a[0] : T(0));
+        __syncthreads(); // one wave: orders the column's stores before its reads
+#pragma unroll
+        for (int j = 1; j < B; ++j)
         {
-            const int i = k + 1 + idx % m, j = k + 1 + idx / m;
-            if (i >= j)
-                D[i + j * ld] -= lcol[i] * (lcol[j] * pivot);
+            T v = a[j];
+            // The index wraps inside the column (never read past it); the row is only used when K + j <= i.
+            const T lj = column[(K + j) & (B - 1)];
+            if (K + j <= i)
+                v -= l * (lj * pivot);
+            a[j - 1] = v;
+            if ((j & 15) == 0)
+                __builtin_amdgcn_sched_barrier(0); // bounds the LDS reads in flight (registers)
         }
-        for (int i = k + 1 + tid; i < kb; i += nt)
-            D[i + k * ld] = lcol[i];
-        __syncthreads();
+        a[B - 1] = T(0);
     }
-    for (int idx = tid; idx < kb * kb; idx += nt)
-    {
-        const int i = idx % kb, j = idx / kb;
-        if (i >= j)
-            panel[(k0 + i) + int64_t(k0 + j) * n] = D[i + j * ld];
-    }
-    for (int k = tid; k < kb; k += nt)
-        diagOut[p.first[s] + k0 + k] = double(D[k + k * ld]);
-}
-
-// The rows below a factorised diagonal block: L(i, blk) = F(i, blk)·L11⁻ᵀ·D⁻¹, one row per thread held in registers
-// (PANEL_BLOCK values), L11 and D broadcast from LDS. A chunk is (front, first row); rows [row0, row0 + 256) ∩ [k0 + kb, n).
-template <typename T>
-__global__ __launch_bounds__(256) void panelSolveKernel(DevicePlan p, T* panels, const int2* chunks, int k0)
-{
-    __shared__ T L[PANEL_BLOCK * PANEL_BLOCK];
-    __shared__ T d[PANEL_BLOCK];
-    const int2 chunk = chunks[blockIdx.x];
-    const int s = chunk.x;
-    const int c = p.ncol[s], n = c + p.nrow[s];
-    const int kb = min(PANEL_BLOCK, c - k0);
-    T* panel = panels + p.panelOff[s];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < PANEL_BLOCK * PANEL_BLOCK; idx += blockDim.x)
-    {
-        const int i = idx % PANEL_BLOCK, j = idx / PANEL_BLOCK;
-        // L stored row-major in LDS: L[i * B + j] = L11(i, j), identity outside the block.
-        T v = T(i == j ? 1 : 0);
-        if (i < kb && j < kb && i > j)
-            v = panel[(k0 + i) + int64_t(k0 + j) * n];
-        L[i * PANEL_BLOCK + j] = v;
-    }
-    for (int k = tid; k < PANEL_BLOCK; k += blockDim.x)
-        d[k] = k < kb ? panel[(k0 + k) + int64_t(k0 + k) * n] : T(1);
     __syncthreads();
-    const int i = chunk.y + tid;
-    if (i >= n)
-        return;
-    T x[PANEL_BLOCK];
+    for (int j = 0; j < kb; ++j)
+        if (i >= j && i < kb)
+            panel[(k0 + i) + int64_t(k0 + j) * n] = Lsh[j * LD + i];
+    if (i < kb)
+        diagOut[p.first[s] + k0 + i] = double(Lsh[i * LD + i]);
+    T* out = linv + p.linvOff[s] + int64_t(k0) * B;
+    T* x = a; // the row registers, free now: x[t] = L⁻¹(K + t, i)
 #pragma unroll
-    for (int k = 0; k < PANEL_BLOCK; ++k)
-        x[k] = k < kb ? panel[i + int64_t(k0 + k) * n] : T(0);
-    // z·L11ᵀ = x, z = L(i, blk)·D.
-#pragma unroll
-    for (int k = 0; k < PANEL_BLOCK; ++k)
+    for (int q = 0; q < B; ++q)
+        x[q] = T(q == i ? 1 : 0);
+    for (int K = 0; K < B; ++K)
     {
-        T z = x[k];
+        const T xk = x[0];
+        if (i < kb && K < kb)
+            out[K + i * kb] = xk;
+        const T* column = Lsh + K * LD;
 #pragma unroll
-        for (int q = 0; q < k; ++q)
-            z -= x[q] * L[k * PANEL_BLOCK + q];
-        x[k] = z;
+        for (int t = 1; t < B; ++t)
+        {
+            T v = x[t];
+            const T lt = column[(K + t) & (B - 1)];
+            if (K + t < B)
+                v -= lt * xk;
+            x[t - 1] = v;
+            if ((t & 15) == 0)
+                __builtin_amdgcn_sched_barrier(0);
+        }
+        x[B - 1] = T(0);
     }
-#pragma unroll
-    for (int k = 0; k < PANEL_BLOCK; ++k)
-        if (k < kb)
-            panel[i + int64_t(k0 + k) * n] = x[k] / d[k];
 }
 
-// MFMA tiles of C −= A·diag(d)·Bᵀ (64 × 64 per workgroup, four waves of 32 × 32, 16 × 16 × 4 MFMA blocks): the
-// trailing update of a panel step and the update matrix of a front. A(i, p) = panels[aOff + i + p·ld],
-// B(j, p) = panels[bOff + j + p·ld], d_p = panels[dOff + p·(ld + 1)].
+// MFMA tiles, 64 × 64 per workgroup, four waves of 32 × 32 in 16 × 16 × 4 blocks:
+//   mode 0  C −= A·diag(d)·Bᵀ  the trailing update of a panel step, and the update matrix of a front;
+//   mode 1  C  = A·Bᵀ·diag(d)⁻¹ with B = L11⁻¹ of the diagonal block: the panel solve (rows below a factorised
+//           diagonal block, L(i, blk) = F(i, blk)·L11⁻ᵀ·D⁻¹), in place (C = A: the tile reads its rows before writing).
+// A(i, p) = panels[aOff + i + p·ld]; mode 0: B(j, p) = panels[bOff + j + p·ld]; mode 1: B(j, p) = linv[bOff + j + p·ldb];
+// d_p = panels[dOff + p·(ld + 1)].
 template <typename T> struct Mfma;
 
+// Operands of a 16 × 16 × 4 block: lane l holds A(l mod 16, l / 16) and B(l / 16, l mod 16); its four results are
+// D(row(l, v), l mod 16), v = 0..3, with the row mapping of each instruction (checked on the device by
+// oosg_debug_mfma_probe, which writes through these same functions).
 template <> struct Mfma<double>
 {
     typedef double4_t Acc;
@@ -313,6 +364,7 @@ template <> struct Mfma<double>
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 template <> struct Mfma<float>
@@ -322,13 +374,15 @@ template <> struct Mfma<float>
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 constexpr int TILE = 64;
 constexpr int TILE_K = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, const UpdateTile* tiles)
+__global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, const T* linv,
+                                                         const UpdateTile* tiles)
 {
     __shared__ T As[TILE_K][TILE + 4];
     __shared__ T Bs[TILE_K][TILE + 4];
@@ -343,12 +397,12 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
         for (int b = 0; b < 2; ++b)
             acc[a][b] = typename Mfma<T>::Acc{0, 0, 0, 0};
     const T* A = panels + t.aOff;
-    const T* Bm = panels + t.bOff;
+    const T* Bm = (t.mode == 1 ? linv : panels) + t.bOff;
     const T* dv = panels + t.dOff;
-    const int64_t ld = t.ld;
+    const int64_t ld = t.ld, ldb = t.mode == 1 ? t.ldb : t.ld;
     for (int p0 = 0; p0 < t.k; p0 += TILE_K)
     {
-        // Stage A and W = B·D: 64 rows × 16 terms each, 4 elements per thread, coalesced along the rows.
+        // Stage A and B (times D in mode 0): 64 rows × 16 terms each, 4 elements per thread, coalesced along the rows.
 #pragma unroll
         for (int q = 0; q < (TILE * TILE_K) / 256; ++q)
         {
@@ -361,7 +415,7 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
                 if (row < t.m)
                     av = A[row + pp * ld];
                 if (row < t.n)
-                    bv = Bm[row + pp * ld] * dv[pp * (ld + 1)];
+                    bv = t.mode == 1 ? Bm[row + pp * ldb] : Bm[row + pp * ld] * dv[pp * (ld + 1)];
             }
             As[kk][row] = av;
             Bs[kk][row] = bv;
@@ -393,25 +447,154 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
 #pragma unroll
             for (int v = 0; v < 4; ++v)
             {
-                const int i = wm + a * 16 + 4 * (lane >> 4) + v;
+                const int i = wm + a * 16 + Mfma<T>::row(lane, v);
                 const int j = wn + b * 16 + (lane & 15);
-                if (i < t.m && j < t.n && (!t.lower || i >= j))
+                if (i >= t.m || j >= t.n || (t.lower && i < j))
+                    continue;
+                if (t.mode == 1)
+                    C[i + int64_t(j) * t.ldc] = acc[a][b][v] / dv[j * (ld + 1)];
+                else
                     C[i + int64_t(j) * t.ldc] -= acc[a][b][v];
             }
 }
 
-// ---- solves --------------------------------------------------------------------------------------------------------
-
-constexpr int SOLVE_BLOCK = 64;
-
-// Forward sweep of a level, one workgroup per supernode, K right-hand sides at most (k actual): y = [X(cols); 0] plus
-// the children's update vectors in child order, y ← L⁻¹·y on the front (unit lower), X(cols) = D⁻¹·y(cols),
-// U(s) = y(rows) for the parent.
-template <typename TP, int K>
-__global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* panels, double* X, double* U, int k,
-                                                     const int32_t* list)
+__global__ void mfmaProbeKernel(const double* a, const double* b, double* d, const float* af, const float* bf,
+                                float* df)
 {
-    __shared__ double ys[SOLVE_BLOCK][K];
+    const int lane = threadIdx.x;
+    double4_t acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(lane & 15) * 4 + (lane >> 4)], b[(lane >> 4) * 16 + (lane & 15)],
+                                               acc, 0, 0, 0);
+    float4_t accf = {0, 0, 0, 0};
+    accf = __builtin_amdgcn_mfma_f32_16x16x4f32(af[(lane & 15) * 4 + (lane >> 4)], bf[(lane >> 4) * 16 + (lane & 15)],
+                                                accf, 0, 0, 0);
+    for (int v = 0; v < 4; ++v)
+    {
+        d[Mfma<double>::row(lane, v) * 16 + (lane & 15)] = acc[v];
+        df[Mfma<float>::row(lane, v) * 16 + (lane & 15)] = accf[v];
+    }
+}
+
+// ---- solves --------------------------------------------------------------------------------------------------------
+//
+// Multifrontal sweeps, level by level, in double precision (panels of either precision). Per supernode s, with
+// y = [X(cols); update vector]:
+//   forward   gather the children's update vectors (child order); by blocks J of 64 columns: y_J ← L_JJ⁻¹·y_J, then
+//             the rows below the block −= L(rows, J)·y_J (MFMA block products); X(cols) keeps L⁻¹y, D⁻¹ is applied
+//             to all of X after the last level;
+//   backward  X(cols) −= L_RFᵀ·X(rows), then by blocks J from the last: y_J −= L(rows after J, J)ᵀ·X(those rows),
+//             X_J ← L_JJ⁻ᵀ·y_J.
+// The products of a supernode with many rows (BIG_ROWS or more) are spread over workgroups: row tiles of L_RF in the
+// forward sweep, column blocks in the backward sweep (each owns its outputs: no reduction across workgroups).
+
+constexpr int SB = 64;       // rows or columns of a staged panel block
+constexpr int SBP = SB + 1;  // its padded leading dimension in LDS
+constexpr int KMAX = 16;     // right-hand sides per sweep
+
+// blk[q·SBP + i] = L(row0 + i, col0 + q), zero outside [0, rows) × [0, cols).
+template <typename TP>
+__device__ inline void stageBlock(double* blk, const TP* L, int64_t ld, int row0, int rows, int col0, int cols)
+{
+    for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x)
+    {
+        const int i = idx & (SB - 1), q = idx >> 6;
+        blk[q * SBP + i] = (i < rows && q < cols) ? double(L[(row0 + i) + int64_t(col0 + q) * ld]) : 0.0;
+    }
+}
+
+// Wave w: acc(i, h) += Σ_q blk(i, q)·v(q, h) for its 16 rows i = 16w + ..., v[q·KMAX + h].
+__device__ inline double4_t rowsTimesVectors(const double* blk, const double* v, double4_t acc)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int kk = 0; kk < SB; kk += 4)
+    {
+        const double a = blk[(kk + (lane >> 4)) * SBP + 16 * w + (lane & 15)];
+        const double b = v[(kk + (lane >> 4)) * KMAX + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Wave w: acc(q, h) += Σ_i blk(i, q)·v(i, h) for its 16 columns q = 16w + ..., v[i·KMAX + h].
+__device__ inline double4_t columnsTimesVectors(const double* blk, const double* v, double4_t acc)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int kk = 0; kk < SB; kk += 4)
+    {
+        const double a = blk[(16 * w + (lane & 15)) * SBP + kk + (lane >> 4)];
+        const double b = v[(kk + (lane >> 4)) * KMAX + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// The output row (within a 64-row tile) of acc[v] on this lane, and its column h.
+__device__ inline int accRow(int v)
+{
+    return 16 * (threadIdx.x >> 6) + ((threadIdx.x & 63) >> 4) + 4 * v;
+}
+__device__ inline int accColumn()
+{
+    return threadIdx.x & 15;
+}
+
+__device__ inline bool isBigSupernode(int r)
+{
+    return r >= BIG_ROWS;
+}
+
+// Wide supernodes run their column blocks as steps of their own (forwardWideKernel, backwardWideKernel).
+__device__ inline bool isWideSupernode(int c)
+{
+    return c >= WIDE_COLUMNS;
+}
+
+// ys ← inv·ys (transpose: invᵀ·ys) for the unit lower inverse of a diagonal block (column-major qb × qb), staged in
+// blk and applied by MFMA; ys is SB × KMAX, zero beyond its rows and columns.
+template <typename TP>
+__device__ inline void applyBlockInverse(const TP* inv, int qb, double* blk, double* ys, bool transpose)
+{
+    stageBlock(blk, inv, qb, 0, qb, 0, qb);
+    __syncthreads();
+    const double4_t zero = {0, 0, 0, 0};
+    const double4_t acc = transpose ? columnsTimesVectors(blk, ys, zero) : rowsTimesVectors(blk, ys, zero);
+    __syncthreads();
+    const int h = accColumn();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        ys[accRow(v) * KMAX + h] = acc[v];
+    __syncthreads();
+}
+
+// ys ← the rows row0 .. row0 + qb of X (k columns), zero padded to SB × KMAX.
+__device__ inline void loadBlockRows(double* ys, const double* X, int64_t row0, int qb, int k)
+{
+    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        ys[idx] = (i < qb && h < k) ? X[(row0 + i) * k + h] : 0.0;
+    }
+}
+
+// The rows row0 .. row0 + qb of X ← ys.
+__device__ inline void storeBlockRows(const double* ys, double* X, int64_t row0, int qb, int k)
+{
+    for (int idx = threadIdx.x; idx < qb * KMAX; idx += blockDim.x)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        if (h < k)
+            X[(row0 + i) * k + h] = ys[idx];
+    }
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                     double* U, int k, const int32_t* list)
+{
+    __shared__ double blk[SB * SBP];
+    __shared__ double ys[SB * KMAX];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     const TP* L = panels + p.panelOff[s];
@@ -426,134 +609,271 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
         const int rc = p.nrow[ch];
         const int32_t* rel = p.rel + p.rowStart[ch];
         const double* uc = U + p.rowStart[ch] * k;
-        for (int idx = tid; idx < rc * k; idx += nt)
+        for (int idx = tid; idx < rc * KMAX; idx += nt)
         {
-            const int i = idx / k, h = idx - i * k;
+            const int i = idx >> 4, h = idx & (KMAX - 1);
+            if (h >= k)
+                continue;
             const int pos = rel[i];
+            const double v = uc[int64_t(i) * k + h];
             if (pos < c)
-                X[int64_t(first + pos) * k + h] += uc[idx];
+                X[int64_t(first + pos) * k + h] += v;
             else
-                u[int64_t(pos - c) * k + h] += uc[idx];
+                u[int64_t(pos - c) * k + h] += v;
         }
     }
     __syncthreads();
-    for (int q0 = 0; q0 < c; q0 += SOLVE_BLOCK)
+    if (isWideSupernode(c))
+        return; // its column blocks are steps of forwardWideKernel
+    const int rowsEnd = isBigSupernode(r) ? c : n;
+    for (int q0 = 0; q0 < c; q0 += SB)
     {
-        const int qb = min(SOLVE_BLOCK, c - q0);
-        for (int idx = tid; idx < qb * k; idx += nt)
+        const int qb = min(SB, c - q0);
+        loadBlockRows(ys, X, first + q0, qb, k);
+        // y_J ← L_JJ⁻¹·y_J.
+        applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, false);
+        storeBlockRows(ys, X, first + q0, qb, k);
+        // The rows below the block, 64 at a time.
+        for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
-            const int i = idx / k, h = idx - i * k;
-            ys[i][h] = X[int64_t(first + q0 + i) * k + h];
-        }
-        __syncthreads();
-        // In-block triangle: the columns one after the other.
-        for (int q = 0; q < qb; ++q)
-        {
-            for (int idx = tid; idx < (qb - q - 1) * k; idx += nt)
+            stageBlock(blk, L, n, i0, min(SB, rowsEnd - i0), q0, qb);
+            __syncthreads();
+            const double4_t acc = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
+            const int h = accColumn();
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
             {
-                const int i = q + 1 + idx / k, h = idx % k;
-                ys[i][h] -= double(L[(q0 + i) + int64_t(q0 + q) * n]) * ys[q][h];
+                const int i = i0 + accRow(v);
+                if (i < rowsEnd && h < k)
+                {
+                    double* y = i < c ? X + int64_t(first + i) * k : u + int64_t(i - c) * k;
+                    y[h] -= acc[v];
+                }
             }
             __syncthreads();
         }
-        for (int idx = tid; idx < qb * k; idx += nt)
-        {
-            const int i = idx / k, h = idx - i * k;
-            X[int64_t(first + q0 + i) * k + h] = ys[i][h];
-        }
-        // The rows below the block: y(i) −= Σ_q L(i, q)·y(q), one row per thread, K sums in registers.
-        for (int i = q0 + qb + tid; i < n; i += nt)
-        {
-            double acc[K];
-#pragma unroll
-            for (int h = 0; h < K; ++h)
-                acc[h] = 0.0;
-            for (int q = 0; q < qb; ++q)
-            {
-                const double l = double(L[i + int64_t(q0 + q) * n]);
-#pragma unroll
-                for (int h = 0; h < K; ++h)
-                    acc[h] += l * ys[q][h];
-            }
-            double* y = i < c ? X + int64_t(first + i) * k : u + int64_t(i - c) * k;
-#pragma unroll
-            for (int h = 0; h < K; ++h)
-                if (h < k)
-                    y[h] -= acc[h];
-        }
         __syncthreads();
-    }
-    for (int idx = tid; idx < c * k; idx += nt)
-    {
-        const int i = idx / k;
-        X[int64_t(first) * k + idx] /= double(L[i + int64_t(i) * n]);
     }
 }
 
-// Backward sweep of a level (top down), one workgroup per supernode: X(cols) −= L_RFᵀ·X(rows), then
-// X(cols) ← L_FF⁻ᵀ·X(cols), by blocks of columns from the last.
-template <typename TP, int K>
-__global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* panels, double* X, int k,
-                                                      const int32_t* list)
+// The update vectors of the big supernodes of a level: U(s)(rows of the tile) −= L_RF(tile, :)·X(cols(s)).
+template <typename TP>
+__global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP* panels, const double* X, double* U,
+                                                         int k, const int2* tiles)
 {
-    __shared__ double ys[SOLVE_BLOCK][K];
-    const int s = list[blockIdx.x];
+    __shared__ double blk[SB * SBP];
+    __shared__ double ys[SB * KMAX];
+    const int2 tile = tiles[blockIdx.x];
+    const int s = tile.x;
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int i0 = c + tile.y * SB;
+    const int rows = min(SB, n - i0);
+    const TP* L = panels + p.panelOff[s];
+    double4_t acc = {0, 0, 0, 0};
+    for (int q0 = 0; q0 < c; q0 += SB)
+    {
+        const int qb = min(SB, c - q0);
+        loadBlockRows(ys, X, first + q0, qb, k);
+        stageBlock(blk, L, n, i0, rows, q0, qb);
+        __syncthreads();
+        acc = rowsTimesVectors(blk, ys, acc);
+        __syncthreads();
+    }
+    double* u = U + p.rowStart[s] * k;
+    const int h = accColumn();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int i = accRow(v);
+        if (i < rows && h < k)
+            u[int64_t(i0 - c + i) * k + h] -= acc[v];
+    }
+}
+
+// X ← D⁻¹·X, X row-major n × k.
+__global__ void diagonalSolveKernel(double* X, const double* d, int64_t n, int k)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    {
+        const double di = d[i];
+        for (int h = 0; h < k; ++h)
+            X[i * k + h] /= di;
+    }
+}
+
+// ys ← the rows rows[i0 − c + i] (or first + i0 + i below c) of X, for i < ib.
+__device__ inline void gatherRows(double* xs, const double* X, const int32_t* rows, int first, int c, int i0, int ib,
+                                  int k)
+{
+    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        double v = 0.0;
+        if (i < ib && h < k)
+        {
+            const int row = i0 + i;
+            v = row < c ? X[int64_t(first + row) * k + h] : X[int64_t(rows[row - c]) * k + h];
+        }
+        xs[idx] = v;
+    }
+}
+
+// X(cols of block J) −= L_RF(:, J)ᵀ·X(rows), for the column blocks of the big and wide supernodes of a level.
+template <typename TP>
+__global__ __launch_bounds__(256) void backwardColumnsKernel(DevicePlan p, const TP* panels, double* X, int k,
+                                                             const int2* blocks)
+{
+    __shared__ double blk[SB * SBP];
+    __shared__ double xs[SB * KMAX];
+    const int2 block = blocks[blockIdx.x];
+    const int s = block.x;
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int q0 = block.y * SB, qb = min(SB, c - q0);
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, waves = blockDim.x >> 6;
-    for (int q0 = ((c - 1) / SOLVE_BLOCK) * SOLVE_BLOCK; q0 >= 0; q0 -= SOLVE_BLOCK)
+    double4_t acc = {0, 0, 0, 0};
+    for (int i0 = c; i0 < n; i0 += SB)
     {
-        const int qb = min(SOLVE_BLOCK, c - q0);
-        // Dot products of the block's columns with the final rows below the block, one column per wave at a time.
-        for (int q = wave; q < qb; q += waves)
-        {
-            double acc[K];
-#pragma unroll
-            for (int h = 0; h < K; ++h)
-                acc[h] = 0.0;
-            const TP* col = L + int64_t(q0 + q) * n;
-            for (int i = q0 + qb + lane; i < n; i += 64)
-            {
-                const double l = double(col[i]);
-                const double* x = i < c ? X + int64_t(first + i) * k : X + int64_t(rows[i - c]) * k;
-#pragma unroll
-                for (int h = 0; h < K; ++h)
-                    if (h < k)
-                        acc[h] += l * x[h];
-            }
-#pragma unroll
-            for (int h = 0; h < K; ++h)
-            {
-                double v = acc[h];
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                    v += __shfl_xor(v, off);
-                acc[h] = v;
-            }
-            if (lane == 0)
-#pragma unroll
-                for (int h = 0; h < K; ++h)
-                    if (h < k)
-                        ys[q][h] = X[int64_t(first + q0 + q) * k + h] - acc[h];
-        }
+        const int ib = min(SB, n - i0);
+        gatherRows(xs, X, rows, first, c, i0, ib, k);
+        stageBlock(blk, L, n, i0, ib, q0, qb);
         __syncthreads();
-        // In-block triangle, from the last column up.
-        for (int q = qb - 1; q >= 0; --q)
+        acc = columnsTimesVectors(blk, xs, acc);
+        __syncthreads();
+    }
+    const int h = accColumn();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int q = accRow(v);
+        if (q < qb && h < k)
+            X[int64_t(first + q0 + q) * k + h] -= acc[v];
+    }
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X, int k,
+                                                      const int32_t* list)
+{
+    __shared__ double blk[SB * SBP];
+    __shared__ double xs[SB * KMAX];
+    __shared__ double ys[SB * KMAX];
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    if (isWideSupernode(c))
+        return; // backwardColumnsKernel and backwardWideKernel did it
+    const TP* L = panels + p.panelOff[s];
+    const int32_t* rows = p.rows + p.rowStart[s];
+    const int h = accColumn();
+    const int lastBlock = ((c - 1) / SB) * SB;
+    for (int q0 = lastBlock; q0 >= 0; q0 -= SB)
+    {
+        const int qb = min(SB, c - q0);
+        // Dot products of the block's columns with the final rows after it: the supernode's own columns after the
+        // block, then the rows of L_RF (unless a big supernode, whose column blocks did them).
+        double4_t acc = {0, 0, 0, 0};
+        const int rowsEnd = isBigSupernode(r) ? c : n;
+        for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
-            for (int idx = tid; idx < q * k; idx += blockDim.x)
-            {
-                const int i = idx / k, h = idx % k;
-                ys[i][h] -= double(L[(q0 + q) + int64_t(q0 + i) * n]) * ys[q][h];
-            }
+            const int ib = min(SB, rowsEnd - i0);
+            gatherRows(xs, X, rows, first, c, i0, ib, k);
+            stageBlock(blk, L, n, i0, ib, q0, qb);
+            __syncthreads();
+            acc = columnsTimesVectors(blk, xs, acc);
             __syncthreads();
         }
-        for (int idx = tid; idx < qb * k; idx += blockDim.x)
+        for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
+            ys[idx] = 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
         {
-            const int i = idx / k, h = idx - i * k;
-            X[int64_t(first + q0 + i) * k + h] = ys[i][h];
+            const int q = accRow(v);
+            if (q < qb && h < k)
+                ys[q * KMAX + h] = X[int64_t(first + q0 + q) * k + h] - acc[v];
         }
         __syncthreads();
+        // X_J ← L_JJ⁻ᵀ·y_J.
+        applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, true);
+        storeBlockRows(ys, X, first + q0, qb, k);
+        __syncthreads();
+    }
+}
+
+// One step J of the forward sweep of the wide supernodes of a level. Item (s, t >= 0): rows [q0 + qb + 64t, +64) of s
+// −= L(rows, J)·z_J with z_J = L_JJ⁻¹·X_J (every item recomputes z_J: X_J is read-only during the step). Item (s, −1):
+// X_{J−1} ← L⁻¹·X_{J−1}, the block the previous step used (no item of this step reads it).
+template <typename TP>
+__global__ __launch_bounds__(256) void forwardWideKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                         double* U, int k, const int2* items, int J)
+{
+    __shared__ double blk[SB * SBP];
+    __shared__ double ys[SB * KMAX];
+    const int2 item = items[blockIdx.x];
+    const int s = item.x;
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int block = item.y < 0 ? J - 1 : J;
+    const int q0 = block * SB, qb = min(SB, c - q0);
+    loadBlockRows(ys, X, first + q0, qb, k);
+    applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, false);
+    if (item.y < 0)
+    {
+        storeBlockRows(ys, X, first + q0, qb, k);
+        return;
+    }
+    const int i0 = q0 + qb + item.y * SB;
+    const int rows = min(SB, n - i0);
+    const TP* L = panels + p.panelOff[s];
+    stageBlock(blk, L, n, i0, rows, q0, qb);
+    __syncthreads();
+    const double4_t acc = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
+    double* u = U + p.rowStart[s] * k;
+    const int h = accColumn();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int i = i0 + accRow(v);
+        if (i < n && h < k)
+        {
+            double* y = i < c ? X + int64_t(first + i) * k : u + int64_t(i - c) * k;
+            y[h] -= acc[v];
+        }
+    }
+}
+
+// One step J of the backward sweep of the wide supernodes of a level (J decreasing). Item (s, I >= 0), I < J:
+// X_I −= L(rows of block J, cols of block I)ᵀ·x_J with x_J = L_JJ⁻ᵀ·X_J (recomputed by every item). Item (s, −1):
+// X_{J+1} ← L⁻ᵀ·X_{J+1}, the block the previous step used.
+template <typename TP>
+__global__ __launch_bounds__(256) void backwardWideKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                          int k, const int2* items, int J)
+{
+    __shared__ double blk[SB * SBP];
+    __shared__ double ys[SB * KMAX];
+    const int2 item = items[blockIdx.x];
+    const int s = item.x;
+    const int c = p.ncol[s], n = c + p.nrow[s], first = p.first[s];
+    const int block = item.y < 0 ? J + 1 : J;
+    const int q0 = block * SB, qb = min(SB, c - q0);
+    loadBlockRows(ys, X, first + q0, qb, k);
+    applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, true);
+    if (item.y < 0)
+    {
+        storeBlockRows(ys, X, first + q0, qb, k);
+        return;
+    }
+    const int c0 = item.y * SB, cb = min(SB, c - c0);
+    const TP* L = panels + p.panelOff[s];
+    stageBlock(blk, L, n, q0, qb, c0, cb);
+    __syncthreads();
+    const double4_t acc = columnsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
+    const int h = accColumn();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int q = accRow(v);
+        if (q < cb && h < k)
+            X[int64_t(first + c0 + q) * k + h] -= acc[v];
     }
 }
 
@@ -645,23 +965,6 @@ __global__ void axpyColumnsKernel(double* x, const double* d, int64_t ld, int64_
     }
 }
 
-__global__ void mfmaProbeKernel(const double* a, const double* b, double* d, const float* af, const float* bf,
-                                float* df)
-{
-    const int lane = threadIdx.x;
-    double4_t acc = {0, 0, 0, 0};
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(lane & 15) * 4 + (lane >> 4)], b[(lane >> 4) * 16 + (lane & 15)],
-                                               acc, 0, 0, 0);
-    float4_t accf = {0, 0, 0, 0};
-    accf = __builtin_amdgcn_mfma_f32_16x16x4f32(af[(lane & 15) * 4 + (lane >> 4)], bf[(lane >> 4) * 16 + (lane & 15)],
-                                                accf, 0, 0, 0);
-    for (int v = 0; v < 4; ++v)
-    {
-        d[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[v];
-        df[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = accf[v];
-    }
-}
-
 // ---- launchers -----------------------------------------------------------------------------------------------------
 
 namespace
@@ -679,8 +982,9 @@ template <typename T> size_t smallFrontBytes(int maxOrder)
 }
 
 template <typename T>
-hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates, const int32_t* list,
-                             int count, int maxOrder, double* diagOut, int32_t* zeroAt, hipStream_t stream)
+hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates, T* linv,
+                             const int32_t* list, int count, int maxOrder, double* diagOut, int32_t* zeroAt,
+                             hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
@@ -689,8 +993,8 @@ hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* pane
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(smallFrontKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, list, diagOut,
-                       zeroAt);
+    hipLaunchKernelGGL(smallFrontKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, linv, list,
+                       diagOut, zeroAt);
     return hipGetLastError();
 }
 
@@ -710,62 +1014,74 @@ hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels,
 }
 
 template <typename T>
-hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, const int32_t* fronts, int frontCount, int k0,
+hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const int32_t* fronts, int frontCount, int k0,
                                 double* diagOut, int32_t* zeroAt, hipStream_t stream)
 {
     if (frontCount == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(diagonalBlockKernel<T>, dim3(frontCount), dim3(256), 0, stream, p, panels, fronts, k0, diagOut,
-                       zeroAt);
+    // One wave per block: its barriers cost next to nothing.
+    hipLaunchKernelGGL(diagonalBlockKernel<T>, dim3(frontCount), dim3(64), 0, stream, p, panels, linv, fronts, k0,
+                       diagOut, zeroAt);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launchPanelSolve(const DevicePlan& p, T* panels, const int2* chunks, int chunkCount, int k0,
-                            hipStream_t stream)
-{
-    if (chunkCount == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(panelSolveKernel<T>, dim3(chunkCount), dim3(256), 0, stream, p, panels, chunks, k0);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launchUpdateTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream)
+hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
+                             hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, tiles);
+    hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, linv, tiles);
     return hipGetLastError();
 }
 
 template <typename TP>
-hipError_t launchForward(const DevicePlan& p, const TP* panels, double* X, double* U, int k, const int32_t* list,
-                         int count, hipStream_t stream)
+hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                         const int32_t* list, int count, const int2* rowTiles, int tileCount, hipStream_t stream)
 {
-    if (count == 0)
-        return hipSuccess;
-    if (k == 1)
-        hipLaunchKernelGGL((forwardKernel<TP, 1>), dim3(count), dim3(256), 0, stream, p, panels, X, U, k, list);
-    else if (k <= 4)
-        hipLaunchKernelGGL((forwardKernel<TP, 4>), dim3(count), dim3(256), 0, stream, p, panels, X, U, k, list);
-    else
-        hipLaunchKernelGGL((forwardKernel<TP, 16>), dim3(count), dim3(256), 0, stream, p, panels, X, U, k, list);
+    if (count > 0)
+        hipLaunchKernelGGL(forwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, list);
+    if (tileCount > 0)
+        hipLaunchKernelGGL(forwardRowsKernel<TP>, dim3(tileCount), dim3(256), 0, stream, p, panels, X, U, k,
+                           rowTiles);
     return hipGetLastError();
 }
 
 template <typename TP>
-hipError_t launchBackward(const DevicePlan& p, const TP* panels, double* X, int k, const int32_t* list, int count,
-                          hipStream_t stream)
+hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
+                          int count, const int2* columnBlocks, int blockCount, hipStream_t stream)
 {
-    if (count == 0)
-        return hipSuccess;
-    if (k == 1)
-        hipLaunchKernelGGL((backwardKernel<TP, 1>), dim3(count), dim3(256), 0, stream, p, panels, X, k, list);
-    else if (k <= 4)
-        hipLaunchKernelGGL((backwardKernel<TP, 4>), dim3(count), dim3(256), 0, stream, p, panels, X, k, list);
-    else
-        hipLaunchKernelGGL((backwardKernel<TP, 16>), dim3(count), dim3(256), 0, stream, p, panels, X, k, list);
+    if (blockCount > 0)
+        hipLaunchKernelGGL(backwardColumnsKernel<TP>, dim3(blockCount), dim3(256), 0, stream, p, panels, X, k,
+                           columnBlocks);
+    if (count > 0)
+        hipLaunchKernelGGL(backwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, k, list);
+    return hipGetLastError();
+}
+
+template <typename TP>
+hipError_t launchForwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                             const int2* items, int count, int step, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(forwardWideKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, items,
+                           step);
+    return hipGetLastError();
+}
+
+template <typename TP>
+hipError_t launchBackwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                              const int2* items, int count, int step, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(backwardWideKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, k, items,
+                           step);
+    return hipGetLastError();
+}
+
+hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream)
+{
+    hipLaunchKernelGGL(diagonalSolveKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, X, d, n, k);
     return hipGetLastError();
 }
 
@@ -825,17 +1141,21 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
 }
 
 #define OOSG_INSTANTIATE(T)                                                                                            \
-    template hipError_t launchSmallFronts<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, const int32_t*, int, int, \
-                                             double*, int32_t*, hipStream_t);                                          \
+    template hipError_t launchSmallFronts<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, T*, const int32_t*, int,  \
+                                             int, double*, int32_t*, hipStream_t);                                     \
     template hipError_t launchAssembly<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, const AssemblyBlock*, int,   \
                                           hipStream_t);                                                                \
-    template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, const int32_t*, int, int, double*, int32_t*,    \
-                                                hipStream_t);                                                          \
-    template hipError_t launchPanelSolve<T>(const DevicePlan&, T*, const int2*, int, int, hipStream_t);               \
-    template hipError_t launchUpdateTiles<T>(T*, T*, const UpdateTile*, int, hipStream_t);                             \
-    template hipError_t launchForward<T>(const DevicePlan&, const T*, double*, double*, int, const int32_t*, int,      \
-                                         hipStream_t);                                                                 \
-    template hipError_t launchBackward<T>(const DevicePlan&, const T*, double*, int, const int32_t*, int, hipStream_t); \
+    template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, T*, const int32_t*, int, int, double*,          \
+                                                int32_t*, hipStream_t);                                                \
+    template hipError_t launchUpdateTiles<T>(T*, T*, const T*, const UpdateTile*, int, hipStream_t);                   \
+    template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \
+                                         int, const int2*, int, hipStream_t);                                          \
+    template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
+                                          const int2*, int, hipStream_t);                                              \
+    template hipError_t launchForwardWide<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \
+                                             const int2*, int, int, hipStream_t);                                      \
+    template hipError_t launchBackwardWide<T>(const DevicePlan&, const T*, const T*, double*, int, const int2*, int,   \
+                                              int, hipStream_t);                                                       \
     template size_t smallFrontBytes<T>(int);
 
 OOSG_INSTANTIATE(double)

@@ -123,7 +123,6 @@ enum class LaunchKind
     Small,
     Assemble,
     Diagonal,
-    PanelSolve,
     Tiles
 };
 
@@ -233,23 +232,36 @@ struct oosg_plan
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
-    std::vector<int64_t> rowStart, panelOff, updateOff;
+    std::vector<int64_t> rowStart, panelOff, updateOff, linvOff;
+    // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward).
+    std::vector<int2> forwardTiles, backwardBlocks;
+    std::vector<int64_t> forwardTileStart, backwardBlockStart;
+    // The steps of the wide supernodes' sweeps: per level, (step, offset, count) into wideItems.
+    struct WideStep
+    {
+        int step;
+        int64_t offset;
+        int count;
+    };
+    std::vector<int2> wideItems;
+    std::vector<std::vector<WideStep>> wideForward, wideBackward;
     int32_t levels = 0;
     int64_t smallFronts = 0, largeFronts = 0;
     double flops = 0.0;
 
     std::vector<Launch> schedule;
     std::vector<int32_t> smallList, stepFronts;
-    std::vector<int2> chunks;
     std::vector<oosg::AssemblyBlock> blocks;
     std::vector<oosg::UpdateTile> tiles;
 
-    DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren;
-    DeviceBuffer dSmallList, dStepFronts, dChunks, dBlocks, dTiles, dLevelList;
+    DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
+        dLinvOff;
+    DeviceBuffer dSmallList, dStepFronts, dBlocks, dTiles, dLevelList, dForwardTiles, dBackwardBlocks, dWideItems;
     oosg::DevicePlan dp{};
 
     DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
     DeviceBuffer panelCache[2]; // panels given back by destroyed factors, per precision
+    DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
     uint64_t budget = 0;
     std::vector<double> levelPanelValues; // per level: Σ c·(c + r), the panel values a sweep of the level reads
@@ -285,6 +297,8 @@ struct oosg_factor
     oosg_plan* plan = nullptr;
     int32_t precision = OOSG_FP64;
     DeviceBuffer panels;
+    DeviceBuffer linv;  // the inverses of the diagonal blocks of PANEL_BLOCK columns (L_JJ⁻¹)
+    DeviceBuffer dDiag; // D as the solves divide by it (FP32: of the scaled matrix)
     DeviceBuffer scale; // FP32: the equations' scales of the factorised matrix
     std::vector<double> diag;
     oosg_factor_info info{};
@@ -292,6 +306,8 @@ struct oosg_factor
     {
         if (plan != nullptr && panels.ptr != nullptr && plan->panelCache[precision].ptr == nullptr)
             plan->panelCache[precision] = std::move(panels);
+        if (plan != nullptr && linv.ptr != nullptr && plan->linvCache[precision].ptr == nullptr)
+            plan->linvCache[precision] = std::move(linv);
     }
 };
 
@@ -391,12 +407,17 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
     // Offsets, levels by height, flops.
     p.panelOff.assign(S + 1, 0);
     p.updateOff.assign(S + 1, 0);
+    p.linvOff.assign(S + 1, 0);
     std::vector<int32_t> height(S, 0);
     for (int32_t k = 0; k < S; ++k)
     {
         const int64_t c = p.ncol[k], r = p.nrow[k];
         p.panelOff[k + 1] = p.panelOff[k] + c * (c + r);
         p.updateOff[k + 1] = p.updateOff[k] + r * r;
+        int64_t inverses = 0;
+        for (int64_t q0 = 0; q0 < c; q0 += oosg::PANEL_BLOCK)
+            inverses += std::min<int64_t>(oosg::PANEL_BLOCK, c - q0) * std::min<int64_t>(oosg::PANEL_BLOCK, c - q0);
+        p.linvOff[k + 1] = p.linvOff[k] + inverses;
         for (int64_t m = r; m < r + c; ++m)
             p.flops += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
         if (p.parent[k] >= 0)
@@ -419,6 +440,69 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
             p.levelList[p.levelStart[height[k]] + fill[height[k]]++] = k;
             p.levelPanelValues[height[k]] += static_cast<double>(p.panelOff[k + 1] - p.panelOff[k]);
         }
+    }
+    // The sweeps' row tiles and column blocks of the big supernodes, level by level.
+    p.forwardTileStart.assign(p.levels + 1, 0);
+    p.backwardBlockStart.assign(p.levels + 1, 0);
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        int maxBlocks = 0;
+        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+        {
+            const int32_t s = p.levelList[q];
+            const bool wide = p.ncol[s] >= oosg::WIDE_COLUMNS, big = p.nrow[s] >= oosg::BIG_ROWS;
+            if (wide)
+                maxBlocks = std::max(maxBlocks, (p.ncol[s] + 63) / 64);
+            if (big && !wide)
+                for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
+                    p.forwardTiles.push_back(make_int2(s, t));
+            if ((big || wide) && p.nrow[s] > 0)
+                for (int32_t b = 0; b * 64 < p.ncol[s]; ++b)
+                    p.backwardBlocks.push_back(make_int2(s, b));
+        }
+        p.forwardTileStart[l + 1] = static_cast<int64_t>(p.forwardTiles.size());
+        p.backwardBlockStart[l + 1] = static_cast<int64_t>(p.backwardBlocks.size());
+        // Wide supernodes: forward steps J = 0 .. maxBlocks (the last only finalises), backward J = maxBlocks − 1 .. −1.
+        std::vector<oosg_plan::WideStep> forward, backward;
+        for (int J = 0; maxBlocks > 0 && J <= maxBlocks; ++J)
+        {
+            const int64_t offset = static_cast<int64_t>(p.wideItems.size());
+            for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+            {
+                const int32_t s = p.levelList[q];
+                const int c = p.ncol[s], n = c + p.nrow[s], blocks = (c + 63) / 64;
+                if (c < oosg::WIDE_COLUMNS)
+                    continue;
+                if (J >= 1 && J <= blocks)
+                    p.wideItems.push_back(make_int2(s, -1));
+                if (J < blocks)
+                {
+                    const int below = std::min(c, (J + 1) * 64);
+                    for (int t = 0; below + t * 64 < n; ++t)
+                        p.wideItems.push_back(make_int2(s, t));
+                }
+            }
+            forward.push_back({J, offset, static_cast<int>(p.wideItems.size() - offset)});
+        }
+        for (int J = maxBlocks - 1; maxBlocks > 0 && J >= -1; --J)
+        {
+            const int64_t offset = static_cast<int64_t>(p.wideItems.size());
+            for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+            {
+                const int32_t s = p.levelList[q];
+                const int blocks = (p.ncol[s] + 63) / 64;
+                if (p.ncol[s] < oosg::WIDE_COLUMNS)
+                    continue;
+                if (J + 1 >= 0 && J + 1 < blocks)
+                    p.wideItems.push_back(make_int2(s, -1));
+                if (J >= 0 && J < blocks)
+                    for (int I = 0; I < J; ++I)
+                        p.wideItems.push_back(make_int2(s, I));
+            }
+            backward.push_back({J, offset, static_cast<int>(p.wideItems.size() - offset)});
+        }
+        p.wideForward.push_back(std::move(forward));
+        p.wideBackward.push_back(std::move(backward));
     }
 }
 
@@ -504,11 +588,8 @@ void buildSchedule(oosg_plan& p)
             Launch diagonal{LaunchKind::Diagonal, PROFILE_DIAGONAL};
             diagonal.k0 = k0;
             diagonal.offset = static_cast<int64_t>(p.stepFronts.size());
-            Launch solve{LaunchKind::PanelSolve, PROFILE_PANEL_SOLVE};
-            solve.k0 = k0;
-            solve.offset = static_cast<int64_t>(p.chunks.size());
-            Launch update{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
-            const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
+            Launch solve{LaunchKind::Tiles, PROFILE_PANEL_SOLVE};
+            solve.offset = static_cast<int64_t>(p.tiles.size());
             for (const int32_t s : large)
             {
                 const int c = p.ncol[s], n = c + p.nrow[s];
@@ -517,9 +598,34 @@ void buildSchedule(oosg_plan& p)
                 const int kb = std::min(PANEL_BLOCK, c - k0), k1 = k0 + kb;
                 p.stepFronts.push_back(s);
                 diagonal.flops += frontFlops(kb, 0);
-                solve.flops += static_cast<double>(n - k1) * (static_cast<double>(kb) * kb + kb);
-                for (int row = k1; row < n; row += 256)
-                    p.chunks.push_back(make_int2(s, row));
+                // L(rows, blk) = F(rows, blk)·L11⁻ᵀ·D⁻¹: one mode-1 tile per 64 rows below the block.
+                for (int row = k1; row < n; row += 64)
+                {
+                    oosg::UpdateTile t{};
+                    t.aOff = p.panelOff[s] + row + int64_t(k0) * n;
+                    t.cOff = t.aOff;
+                    t.bOff = p.linvOff[s] + int64_t(k0) * PANEL_BLOCK;
+                    t.dOff = p.panelOff[s] + int64_t(k0) * (n + 1);
+                    t.ld = n;
+                    t.ldb = kb;
+                    t.ldc = n;
+                    t.m = std::min(64, n - row);
+                    t.n = kb;
+                    t.k = kb;
+                    t.mode = 1;
+                    p.tiles.push_back(t);
+                    solve.flops += 2.0 * t.m * (static_cast<double>(kb) * (kb + 1) / 2.0);
+                }
+            }
+            solve.count = static_cast<int>(p.tiles.size() - solve.offset);
+            Launch update{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
+            const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
+            for (const int32_t s : large)
+            {
+                const int c = p.ncol[s], n = c + p.nrow[s];
+                if (c <= k0)
+                    continue;
+                const int kb = std::min(PANEL_BLOCK, c - k0), k1 = k0 + kb;
                 for (int J0 = k1; J0 < c; J0 += 64)
                     for (int I0 = J0; I0 < n; I0 += 64)
                     {
@@ -540,7 +646,6 @@ void buildSchedule(oosg_plan& p)
                     }
             }
             diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
-            solve.count = static_cast<int>(p.chunks.size() - solve.offset);
             p.schedule.push_back(diagonal);
             if (solve.count > 0)
                 p.schedule.push_back(solve);
@@ -599,9 +704,12 @@ void uploadPlan(oosg_plan& p)
     p.dRel = upload(p.rel, st, "plan: relative indices");
     p.dChildStart = upload(p.childStart, st, "plan: child starts");
     p.dChildren = upload(p.children, st, "plan: children");
+    p.dLinvOff = upload(p.linvOff, st, "plan: inverse offsets");
+    p.dForwardTiles = upload(p.forwardTiles, st, "plan: forward row tiles");
+    p.dBackwardBlocks = upload(p.backwardBlocks, st, "plan: backward column blocks");
+    p.dWideItems = upload(p.wideItems, st, "plan: wide supernode steps");
     p.dSmallList = upload(p.smallList, st, "plan: small fronts");
     p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
-    p.dChunks = upload(p.chunks, st, "plan: panel chunks");
     p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
     p.dTiles = upload(p.tiles, st, "plan: update tiles");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
@@ -615,6 +723,7 @@ void uploadPlan(oosg_plan& p)
     p.dp.rel = p.dRel.as<int32_t>();
     p.dp.childStart = p.dChildStart.as<int32_t>();
     p.dp.children = p.dChildren.as<int32_t>();
+    p.dp.linvOff = p.dLinvOff.as<int64_t>();
     p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
 }
@@ -622,7 +731,7 @@ void uploadPlan(oosg_plan& p)
 // ---- factorisation -------------------------------------------------------------------------------------------------
 
 template <typename T>
-void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, double* diagOut, int32_t* zeroAt, bool scaled)
+void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled)
 {
     const hipStream_t st = p.stream;
     oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
@@ -630,7 +739,6 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, double* diagOut, i
     T* updates = p.updates.as<T>();
     const int32_t* smallList = p.dSmallList.as<int32_t>();
     const int32_t* stepFronts = p.dStepFronts.as<int32_t>();
-    const int2* chunks = p.dChunks.as<int2>();
     const oosg::AssemblyBlock* blocks = p.dBlocks.as<oosg::AssemblyBlock>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
     for (const Launch& l : p.schedule)
@@ -639,8 +747,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, double* diagOut, i
         switch (l.kind)
         {
         case LaunchKind::Small:
-            check(oosg::launchSmallFronts<T>(p.dp, a, panels, updates, smallList + l.offset, l.count, l.maxOrder,
-                                             diagOut, zeroAt, st),
+            check(oosg::launchSmallFronts<T>(p.dp, a, panels, updates, linv, smallList + l.offset, l.count,
+                                             l.maxOrder, diagOut, zeroAt, st),
                   "factor: small fronts");
             break;
         case LaunchKind::Assemble:
@@ -648,15 +756,13 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, double* diagOut, i
                   "factor: assembly");
             break;
         case LaunchKind::Diagonal:
-            check(oosg::launchDiagonalBlocks<T>(p.dp, panels, stepFronts + l.offset, l.count, l.k0, diagOut, zeroAt,
-                                                st),
+            check(oosg::launchDiagonalBlocks<T>(p.dp, panels, linv, stepFronts + l.offset, l.count, l.k0, diagOut,
+                                                zeroAt, st),
                   "factor: diagonal blocks");
             break;
-        case LaunchKind::PanelSolve:
-            check(oosg::launchPanelSolve<T>(p.dp, panels, chunks + l.offset, l.count, l.k0, st), "factor: panel solve");
-            break;
         case LaunchKind::Tiles:
-            check(oosg::launchUpdateTiles<T>(panels, updates, tiles + l.offset, l.count, st), "factor: update tiles");
+            check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
+                  "factor: update tiles");
             break;
         }
         p.profiler.end(slot, st);
@@ -751,26 +857,43 @@ void ensure(DeviceBuffer& b, size_t bytes, const char* what)
 }
 
 // X (row-major n × k, k <= 16, in p.solveX) ← factor⁻¹·X.
-template <typename TP> void runSweeps(oosg_plan& p, const TP* panels, int k)
+template <typename TP>
+void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k)
 {
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
     double* U = p.solveU.as<double>();
     const int32_t* levels = p.dLevelList.as<int32_t>();
+    const int2* forwardTiles = p.dForwardTiles.as<int2>();
+    const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
+    const int2* wide = p.dWideItems.as<int2>();
     // The profile counts bytes as the work of the sweeps (they are bound by HBM): the level's panels, read once.
     for (int32_t l = 0; l < p.levels; ++l)
     {
         const int slot = p.profiler.begin(PROFILE_FORWARD, p.levelPanelValues[l] * sizeof(TP), st);
-        check(oosg::launchForward<TP>(p.dp, panels, X, U, k, levels + p.levelStart[l],
-                                      p.levelStart[l + 1] - p.levelStart[l], st),
+        check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
+                                      p.levelStart[l + 1] - p.levelStart[l], forwardTiles + p.forwardTileStart[l],
+                                      static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]), st),
               "solve: forward sweep");
+        for (const oosg_plan::WideStep& w : p.wideForward[l])
+            check(oosg::launchForwardWide<TP>(p.dp, panels, linv, X, U, k, wide + w.offset, w.count, w.step, st),
+                  "solve: forward sweep (wide)");
         p.profiler.end(slot, st);
     }
+    check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
     for (int32_t l = p.levels - 1; l >= 0; --l)
     {
         const int slot = p.profiler.begin(PROFILE_BACKWARD, p.levelPanelValues[l] * sizeof(TP), st);
-        check(oosg::launchBackward<TP>(p.dp, panels, X, k, levels + p.levelStart[l],
-                                       p.levelStart[l + 1] - p.levelStart[l], st),
+        // The L_RFᵀ products of the big and wide supernodes first (column blocks), then the wide steps, then the rest.
+        const int blocks = static_cast<int>(p.backwardBlockStart[l + 1] - p.backwardBlockStart[l]);
+        check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0, backwardBlocks + p.backwardBlockStart[l],
+                                       blocks, st),
+              "solve: backward sweep");
+        for (const oosg_plan::WideStep& w : p.wideBackward[l])
+            check(oosg::launchBackwardWide<TP>(p.dp, panels, linv, X, k, wide + w.offset, w.count, w.step, st),
+                  "solve: backward sweep (wide)");
+        check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
+                                       p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, st),
               "solve: backward sweep");
         p.profiler.end(slot, st);
     }
@@ -792,9 +915,9 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
         double* block = b + c0 * ld;
         check(oosg::launchToRowMajor(block, ld, p.solveX.as<double>(), n, k, scale, p.stream), "solve: layout");
         if (f.precision == OOSG_FP32)
-            runSweeps<float>(p, f.panels.as<float>(), k);
+            runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k);
         else
-            runSweeps<double>(p, f.panels.as<double>(), k);
+            runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k);
         check(oosg::launchFromRowMajor(p.solveX.as<double>(), block, ld, n, k, scale, 0, p.stream), "solve: layout");
     }
 }
@@ -851,10 +974,12 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
             const uint64_t structure = (p->first.size() * 3 + p->rows.size() * 2 + p->children.size() +
                                         p->smallList.size() + p->stepFronts.size() + p->levelList.size()) *
                                            4 +
-                                       p->panelOff.size() * 16 + p->chunks.size() * 8 +
+                                       p->panelOff.size() * 24 +
+                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->wideItems.size()) * 8 +
                                        p->blocks.size() * sizeof(oosg::AssemblyBlock) +
                                        p->tiles.size() * sizeof(oosg::UpdateTile);
-            const uint64_t values = static_cast<uint64_t>(p->updateOff[p->S] + p->panelOff[p->S]) * sizeof(double);
+            const uint64_t values =
+                static_cast<uint64_t>(p->updateOff[p->S] + p->panelOff[p->S] + p->linvOff[p->S]) * sizeof(double);
             if (structure + values > p->budget)
                 fail(OOSG_OUT_OF_MEMORY,
                      "plan: the factorisation needs " + std::to_string((structure + values) >> 20) +
@@ -1156,17 +1281,24 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
                 f->panels = std::move(plan->panelCache[opts.precision]);
             else
                 f->panels.allocate(panelBytes, "factor: panels");
-            DeviceBuffer diag, zero;
+            const size_t linvBytes = static_cast<size_t>(plan->linvOff[plan->S]) * valueBytes(opts.precision);
+            if (plan->linvCache[opts.precision].ptr != nullptr && plan->linvCache[opts.precision].bytes >= linvBytes)
+                f->linv = std::move(plan->linvCache[opts.precision]);
+            else
+                f->linv.allocate(linvBytes, "factor: inverses of the diagonal blocks");
+            DeviceBuffer& diag = f->dDiag;
+            DeviceBuffer zero;
             diag.allocate(static_cast<size_t>(plan->n) * sizeof(double), "factor: pivots");
             zero.allocate(static_cast<size_t>(std::max(plan->S, 1)) * sizeof(int32_t), "factor: pivot status");
             const hipStream_t st = plan->stream;
             check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
             check(hipEventRecord(plan->start, st), "hipEventRecord");
             if (opts.precision == OOSG_FP32)
-                runFactor<float>(*plan, *matrix, f->panels.as<float>(), diag.as<double>(), zero.as<int32_t>(), true);
+                runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
+                                 zero.as<int32_t>(), true);
             else
-                runFactor<double>(*plan, *matrix, f->panels.as<double>(), diag.as<double>(), zero.as<int32_t>(),
-                                  false);
+                runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(), diag.as<double>(),
+                                  zero.as<int32_t>(), false);
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
             std::vector<int32_t> zeroAt(static_cast<size_t>(plan->S));
             f->diag.resize(static_cast<size_t>(plan->n));
