@@ -76,6 +76,10 @@ typedef struct oosg_factor_options {
     int32_t precision; /* oosg_precision */
     double max_ratio;  /* PARAM MaxDiagonalRatio (PivotPolicy, PivotDiagnostics.hpp:34-44); 0: 1e7 */
     int32_t bailout;   /* PARAM SingularityBailOut: < 0 keeps a factor that fails the ratio test */
+    /* The diagonal the ratio test divides by d_j: n values (host memory, factor order; a negative value keeps the
+     * matrix diagonal of that equation) or null for the matrix diagonal everywhere (FactorOptions::referenceDiagonal,
+     * SupernodalFactor.hpp:75; ratioTestDiagonal, PivotDiagnostics.hpp:108-120). */
+    const double *reference_diagonal;
 } oosg_factor_options;
 
 /* The inertia, pivot counts and ratio test of a factor (SupernodalFactor::inertia, pivotDiagnostics,

@@ -84,6 +84,9 @@ class FactorOptions:
     pivoting: PivotingMode = PivotingMode.POSITIVE_DEFINITE
     pivot_policy: PivotPolicy = field(default_factory=PivotPolicy)
     precision: Precision = Precision.FP64
+    # The diagonal the ratio test divides by the pivots: one value per equation in factor order (a negative value
+    # keeps the matrix diagonal there), or None (FactorOptions::referenceDiagonal, SupernodalFactor.hpp:75).
+    reference_diagonal: object = None
 
 
 @dataclass(frozen=True)
@@ -334,8 +337,17 @@ class SupernodalFactor:
         plan = symbolic.plan(device, hbm_budget_bytes)
         if device_matrix is None:
             device_matrix = plan.matrix(matrix)
+        reference = None
+        if options.reference_diagonal is not None:
+            reference = np.ascontiguousarray(options.reference_diagonal, dtype=np.float64)
+            if reference.shape != (symbolic.n,):
+                # requireReferenceDiagonal (src/linalg/supernodal/PivotDiagnostics.cpp:24-32)
+                raise ValueError(f"factorisation: the reference diagonal of the ratio test has {reference.size} "
+                                 f"values and the matrix {symbolic.n} equations; give one value per equation "
+                                 "(a negative value where the matrix diagonal is kept) or none")
         opts = _native.FactorOptionsC(int(options.pivoting), int(options.precision),
-                                      float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout))
+                                      float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
+                                      _ptr(reference))
         handle = ctypes.c_void_p()
         lib = _native.lib()
         _check(lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle)))

@@ -27,7 +27,8 @@ class CscDesc(ctypes.Structure):
 
 
 class FactorOptionsC(ctypes.Structure):
-    _fields_ = [("pivoting", c_int32), ("precision", c_int32), ("max_ratio", c_double), ("bailout", c_int32)]
+    _fields_ = [("pivoting", c_int32), ("precision", c_int32), ("max_ratio", c_double), ("bailout", c_int32),
+                ("reference_diagonal", c_void_p)]
 
 
 class FactorInfoC(ctypes.Structure):

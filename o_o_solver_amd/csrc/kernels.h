@@ -14,7 +14,7 @@ constexpr int PANEL_BLOCK = 64;
 // Fronts of at most this order are assembled and eliminated in one workgroup, the whole front in LDS.
 constexpr int SMALL_FRONT_MAX = 96;
 // Elements of a front a workgroup of the assembly kernel holds in LDS.
-constexpr int ASSEMBLY_BLOCK_VALUES = 8192;
+constexpr int ASSEMBLY_BLOCK_VALUES = 4096;
 // Supernodes with at least this many rows below their columns spread the products of their sweeps over workgroups.
 constexpr int BIG_ROWS = 256;
 // Supernodes with at least this many columns run the column blocks of their sweeps as steps over many workgroups.
@@ -43,13 +43,22 @@ struct DeviceMatrix
     const int64_t* offsets;
     const int32_t* rows;
     const double* values;
-    const int32_t* position;
+    const int32_t* position; // row of each entry in its front
+    const int32_t* column;   // column of each entry in its front (local to its supernode)
     const double* scale;
 };
 
+// A block of a large front, columns [j0, j1) by rows [i0, i1), and its extend-add items [itemStart, itemEnd).
 struct AssemblyBlock
 {
-    int32_t supernode, j0, j1, i0, i1, pad;
+    int32_t supernode, j0, j1, i0, i1, itemStart, itemEnd, pad;
+};
+
+// The part of a child's update matrix that lands in a block: its columns [q0, q1) and, of each column jj, the rows
+// [max(jj, ia), ib) (positions in the child's row list; the ranges found on the host, once per plan).
+struct AssemblyItem
+{
+    int32_t child, q0, q1, ia, ib, pad;
 };
 
 // One 64 × 64 MFMA tile (element offsets): mode 0 C −= A·diag(d)·Bᵀ with A, B, d in the panels and C in the panels
@@ -67,7 +76,7 @@ hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* pane
 template <typename T> size_t smallFrontBytes(int maxOrder);
 template <typename T>
 hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates,
-                          const AssemblyBlock* blocks, int count, hipStream_t stream);
+                          const AssemblyBlock* blocks, const AssemblyItem* items, int count, hipStream_t stream);
 template <typename T>
 hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const int32_t* fronts, int frontCount, int k0,
                                 double* diagOut, int32_t* zeroAt, hipStream_t stream);
@@ -102,6 +111,27 @@ hipError_t launchColumnMax(const double* v, int64_t ld, int64_t n, int k, unsign
                            hipStream_t stream);
 hipError_t launchFillDouble(double* data, int64_t ld, int64_t n, int k, double value, hipStream_t stream);
 hipError_t launchAxpyColumns(double* x, const double* d, int64_t ld, int64_t n, int k, hipStream_t stream);
+// Per-column coefficients of the Krylov updates (k <= 16), passed by value.
+struct ColumnCoefficients
+{
+    double v[16];
+};
+// Workgroups of the column dot products (the partial sums array holds DOT_GROUPS × 16 values).
+constexpr int DOT_GROUPS = 1024;
+
+// Row-major n × k vectors (k <= 16): out = b − A·x (b null: A·x) with the CSR copy of A; column dot products (into
+// out[0, k), through `partial`); column max |v| (bits into out, which the caller zeroes); x += α p, r −= α q;
+// p = z + β p; dst = src·diag(scale) by rows.
+hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
+                              const double* b, double* out, int64_t n, int k, hipStream_t stream);
+hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, double* partial, double* out,
+                            hipStream_t stream);
+hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream);
+hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, const ColumnCoefficients& alpha,
+                        int64_t n, int k, hipStream_t stream);
+hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
+                             hipStream_t stream);
+hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream);
 hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const float* af, const float* bf, float* df,
                            hipStream_t stream);
 

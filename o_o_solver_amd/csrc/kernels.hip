@@ -84,6 +84,14 @@ template <typename T> __device__ inline void rankOneStep(T* F, int ld, int k, in
     }
 }
 
+// The value as an opaque register: a load whose result feeds a select stays unconditional (the compiler would
+// otherwise move it into a branch and wait for it there), so the loads of a loop issue back to back.
+template <typename T> __device__ inline T pinned(T v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 __global__ void fillInt32Kernel(int32_t* data, int64_t count, int32_t value)
 {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
@@ -138,16 +146,15 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
     for (int idx = tid; idx < n * n; idx += nt)
         F[idx] = T(0);
     __syncthreads();
-    // The matrix entries of the supernode's columns (distinct positions: no two threads meet).
-    for (int j = 0; j < c; ++j)
+    // The matrix entries of the supernode's columns, one range of the CSC (distinct positions: no two threads meet).
     {
-        const int64_t b = a.offsets[first + j], e = a.offsets[first + j + 1];
-        const double sj = a.scale ? a.scale[first + j] : 1.0;
-        for (int64_t k = b + tid; k < e; k += nt)
+        const int64_t kb = a.offsets[first], ke = a.offsets[first + c];
+        for (int64_t k = kb + tid; k < ke; k += nt)
         {
+            const int j = a.column[k];
             double v = a.values[k];
             if (a.scale)
-                v = v * a.scale[a.rows[k]] * sj;
+                v = v * a.scale[a.rows[k]] * a.scale[first + j];
             F[a.position[k] + j * n] += T(v);
         }
     }
@@ -198,11 +205,14 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
 
 // ---- large fronts --------------------------------------------------------------------------------------------------
 
-// Assembly of a block of a front: columns [j0, j1), rows [i0, i1), accumulated in LDS in the reference's order, then
-// written to the panel (columns < c, the rows above the diagonal as zeros) or to the update matrix (lower part).
+// Assembly of a block of a front: columns [j0, j1), rows [i0, i1), accumulated in LDS in the reference's order (zero,
+// the matrix entries, the children in increasing order), then written to the panel (columns < c, the rows above the
+// diagonal as zeros) or to the update matrix (lower part). The matrix entries of the block's columns are one range
+// of the CSC, spread over the threads; a child's columns go to the waves in turn, four at a time, its rows across
+// the lanes, so that each lane has several independent loads in flight.
 template <typename T>
 __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix a, T* panels, T* updates,
-                                                      const AssemblyBlock* blocks)
+                                                      const AssemblyBlock* blocks, const AssemblyItem* items)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T* B = reinterpret_cast<T*>(smem);
@@ -211,48 +221,85 @@ __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     const int rowsIn = blk.i1 - blk.i0, colsIn = blk.j1 - blk.j0;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     for (int idx = tid; idx < rowsIn * colsIn; idx += nt)
         B[idx] = T(0);
     __syncthreads();
-    for (int j = blk.j0; j < min(blk.j1, c); ++j)
+    if (blk.j0 < c)
     {
-        const int64_t b = a.offsets[first + j], e = a.offsets[first + j + 1];
-        const double sj = a.scale ? a.scale[first + j] : 1.0;
-        for (int64_t k = b + tid; k < e; k += nt)
+        const int64_t kb = a.offsets[first + blk.j0], ke = a.offsets[first + min(blk.j1, c)];
+        for (int64_t k = kb + tid; k < ke; k += nt)
         {
             const int pos = a.position[k];
             if (pos < blk.i0 || pos >= blk.i1)
                 continue;
+            const int j = a.column[k];
             double v = a.values[k];
             if (a.scale)
-                v = v * a.scale[a.rows[k]] * sj;
+                v = v * a.scale[a.rows[k]] * a.scale[first + j];
             B[(pos - blk.i0) + (j - blk.j0) * rowsIn] += T(v);
         }
     }
-    for (int q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+    for (int q = blk.itemStart; q < blk.itemEnd; ++q)
     {
         __syncthreads();
-        const int ch = p.children[q];
-        const int rc = p.nrow[ch];
-        const int32_t* rel = p.rel + p.rowStart[ch];
-        const T* u = updates + p.updateOff[ch];
-        const int q0 = lowerBound(rel, rc, blk.j0), q1 = lowerBound(rel, rc, blk.j1);
-        const int ia = lowerBound(rel, rc, blk.i0), ib = lowerBound(rel, rc, blk.i1);
-        for (int jj = q0; jj < q1; ++jj)
+        const AssemblyItem it = items[q];
+        const int rc = p.nrow[it.child];
+        const int32_t* rel = p.rel + p.rowStart[it.child];
+        const T* u = updates + p.updateOff[it.child];
+        for (int jj0 = it.q0 + 4 * wave; jj0 < it.q1; jj0 += 16)
         {
-            const int lo = max(jj, ia);
-            const int jcol = rel[jj] - blk.j0;
-            for (int i = lo + tid; i < ib; i += nt)
-                B[(rel[i] - blk.i0) + jcol * rowsIn] += u[i + int64_t(jj) * rc];
+            // Columns jj0 .. jj0 + 3 of the child (those below q1); every load of the four is issued first.
+            int col[4], lo[4];
+            const T* uc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                const int jj = min(jj0 + t, it.q1 - 1);
+                col[t] = rel[jj];
+                lo[t] = max(jj, it.ia);
+                uc[t] = u + int64_t(jj) * rc;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                col[t] = jj0 + t < it.q1 ? pinned(col[t]) - blk.j0 : -1;
+            for (int i0 = 0;; i0 += 64)
+            {
+                bool any = false;
+                T v[4];
+                int row[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+                    const int i = lo[t] + i0 + lane;
+                    const bool in = col[t] >= 0 && i < it.ib;
+                    any |= col[t] >= 0 && lo[t] + i0 < it.ib;
+                    const int ic = in ? i : lo[t];
+                    v[t] = uc[t][ic];
+                    row[t] = rel[ic];
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+                    const int i = lo[t] + i0 + lane;
+                    if (col[t] >= 0 && i < it.ib)
+                        B[(row[t] - blk.i0) + col[t] * rowsIn] += pinned(v[t]);
+                }
+                if (!any)
+                    break;
+            }
         }
     }
     __syncthreads();
+    // The offsets read once: the stores below could alias the plan's arrays as far as the compiler knows.
+    T* const panel = panels + p.panelOff[s];
+    T* const update = updates + p.updateOff[s];
     forEach2D(rowsIn, colsIn, [&](int il, int jl) {
         const int i = blk.i0 + il, j = blk.j0 + jl;
         if (j < c)
-            panels[p.panelOff[s] + i + int64_t(j) * n] = B[il + jl * rowsIn];
+            panel[i + int64_t(j) * n] = B[il + jl * rowsIn];
         else if (i >= j)
-            updates[p.updateOff[s] + (i - c) + int64_t(j - c) * r] = B[il + jl * rowsIn];
+            update[(i - c) + int64_t(j - c) * r] = B[il + jl * rowsIn];
     });
 }
 
@@ -268,82 +315,6 @@ __device__ inline double broadcastLane(double v, int lane)
     const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(bits), lane));
     const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(bits >> 32), lane));
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
-
-// The diagonal block [k0, k0 + kb) of a front's panel factorised in place (kb <= PANEL_BLOCK), one wave per front.
-// Elimination: lane i holds row i of the block in registers, from the current column on (the row shifts by one
-// register per step, so every register index is a constant); each step publishes L(·, K) in LDS (one store per lane)
-// and every lane reads the column back by broadcast reads. Inverse L⁻¹ of the unit lower part: lane i holds column i
-// of the inverse from row K on (shifting likewise), x(K + t) −= L(K + t, K)·x(K) (Gauss-Jordan on [L | I]), L(·, K)
-// read from LDS, each finished entry x(K) written out. Rows beyond kb hold the identity.
-template <typename T>
-__global__ __launch_bounds__(64) void diagonalBlockKernel(DevicePlan p, T* panels, T* linv, const int32_t* list,
-                                                          int k0, double* diagOut, int32_t* zeroAt)
-{
-    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1;
-    __shared__ T Lsh[B * LD]; // Lsh[j·LD + i]: L(i, j) below the diagonal, D on it, 0 above
-    const int s = list[blockIdx.x];
-    const int c = p.ncol[s], n = c + p.nrow[s];
-    const int kb = min(B, c - k0);
-    T* panel = panels + p.panelOff[s];
-    const int i = threadIdx.x;
-    T a[B];
-#pragma unroll
-    for (int j = 0; j < B; ++j)
-        a[j] = (i < kb && j <= i) ? panel[(k0 + i) + int64_t(k0 + j) * n] : T(i == j ? 1 : 0);
-    for (int K = 0; K < B; ++K)
-    {
-        const T pivot = broadcastLane(a[0], K);
-        if (i == 0 && K < kb && isBadPivot(pivot))
-            atomicMin(&zeroAt[s], k0 + K);
-        const T l = i > K ? a[0] / pivot : T(0);
-        T* column = Lsh + K * LD;
-        column[i] = i > K ? l : (i == K ? a[0] : T(0));
-        __syncthreads(); // one wave: orders the column's stores before its reads
-#pragma unroll
-        for (int j = 1; j < B; ++j)
-        {
-            T v = a[j];
-            // The index wraps inside the column (never read past it); the row is only used when K + j <= i.
-            const T lj = column[(K + j) & (B - 1)];
-            if (K + j <= i)
-                v -= l * (lj * pivot);
-            a[j - 1] = v;
-            if ((j & 15) == 0)
-                __builtin_amdgcn_sched_barrier(0); // bounds the LDS reads in flight (registers)
-        }
-        a[B - 1] = T(0);
-    }
-    __syncthreads();
-    for (int j = 0; j < kb; ++j)
-        if (i >= j && i < kb)
-            panel[(k0 + i) + int64_t(k0 + j) * n] = Lsh[j * LD + i];
-    if (i < kb)
-        diagOut[p.first[s] + k0 + i] = double(Lsh[i * LD + i]);
-    T* out = linv + p.linvOff[s] + int64_t(k0) * B;
-    T* x = a; // the row registers, free now: x[t] = L⁻¹(K + t, i)
-#pragma unroll
-    for (int q = 0; q < B; ++q)
-        x[q] = T(q == i ? 1 : 0);
-    for (int K = 0; K < B; ++K)
-    {
-        const T xk = x[0];
-        if (i < kb && K < kb)
-            out[K + i * kb] = xk;
-        const T* column = Lsh + K * LD;
-#pragma unroll
-        for (int t = 1; t < B; ++t)
-        {
-            T v = x[t];
-            const T lt = column[(K + t) & (B - 1)];
-            if (K + t < B)
-                v -= lt * xk;
-            x[t - 1] = v;
-            if ((t & 15) == 0)
-                __builtin_amdgcn_sched_barrier(0);
-        }
-        x[B - 1] = T(0);
-    }
 }
 
 // MFMA tiles, 64 × 64 per workgroup, four waves of 32 × 32 in 16 × 16 × 4 blocks:
@@ -376,6 +347,193 @@ template <> struct Mfma<float>
     }
     __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
+
+// The diagonal block [k0, k0 + kb) of a front's panel factorised in place (kb <= PANEL_BLOCK), with the inverse of
+// its unit lower factor; one workgroup of four waves per front, the block in LDS (rows and columns from kb on hold
+// the identity, which the elimination leaves unchanged):
+//   elimination by sub-panels of 16 columns: wave 0 eliminates the sub-panel's columns one after the other, a row per
+//     lane, the pivot d_k and W(j, k) = L(j, k)·d_k broadcast from their lanes by v_readlane (the reference's chain,
+//     DenseBlockKernels.cpp:376-390, inside the sub-panel); then the waves subtract L·W of the sub-panel from the
+//     trailing lower 16 × 16 tiles with MFMA;
+//   inverse X = L⁻¹: the four diagonal tiles by forward substitution (wave w: X_ww, lane j its column j), then the
+//     tiles below, by distance d = I − J = 1, 2, 3: X_IJ = −X_II·(Σ_{K=J}^{I−1} L_IK·X_KJ), two MFMA products, the
+//     first one's accumulator used directly as the second one's operand (its k-steps follow the accumulator rows).
+// LDS: S[j·LD + i] holds L(i, j) for i > j, D(j) for i = j, and X(j, i) for i < j (the inverse, transposed, in the
+// upper triangle, which the elimination leaves free).
+template <typename T>
+__global__ __launch_bounds__(256, 2) void diagonalBlockKernel(DevicePlan p, T* panels, T* linv, const int32_t* list,
+                                                           int k0, double* diagOut, int32_t* zeroAt)
+{
+    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1, SP = 16;
+    static_assert(B == 64, "four sub-panels of 16 columns");
+    typedef typename Mfma<T>::Acc Acc;
+    __shared__ T S[B * LD];
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], n = c + p.nrow[s];
+    const int kb = min(B, c - k0);
+    T* panel = panels + p.panelOff[s];
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int64_t base = k0 + int64_t(k0) * n;
+
+    // Every load is issued (from the block's first element where the position is outside the lower triangle) and
+    // the value selected afterwards: no branch waits on an earlier load.
+    // Thread (wave, lane) moves rows i = lane of the columns j = wave + 4q.
+    {
+        const T* src = panel + base + int64_t(wave) * n;
+        T v[B / 4];
+#pragma unroll
+        for (int q = 0; q < B / 4; ++q)
+        {
+            // Columns from kb on lie outside the panel (past the end of the panel array for the last supernode):
+            // their loads read the block's first element instead.
+            const int j = wave + 4 * q;
+            const T* column = j < kb ? src : panel + base;
+            v[q] = column[(lane < kb && j < kb && lane >= j) ? lane : 0];
+            src += 4 * int64_t(n);
+        }
+#pragma unroll
+        for (int q = 0; q < B / 4; ++q)
+        {
+            const int j = wave + 4 * q;
+            const bool inside = lane < kb && j < kb && lane >= j;
+            S[j * LD + lane] = inside ? pinned(v[q]) : T(lane == j ? 1 : 0);
+        }
+    }
+    __syncthreads();
+
+    for (int c0 = 0; c0 < kb; c0 += SP)
+    {
+        if (wave == 0)
+        {
+            T a[SP]; // row `lane` of the sub-panel
+#pragma unroll
+            for (int t = 0; t < SP; ++t)
+                a[t] = S[(c0 + t) * LD + lane];
+#pragma unroll
+            for (int k = 0; k < SP; ++k)
+            {
+                const int K = c0 + k;
+                const T pivot = broadcastLane(a[k], K);
+                if (lane == 0 && K < kb && isBadPivot(pivot))
+                    atomicMin(&zeroAt[s], k0 + K);
+                const T l = a[k] / pivot; // L(lane, K) on the lanes below K
+                const T w = l * pivot;
+#pragma unroll
+                for (int t = k + 1; t < SP; ++t)
+                    a[t] -= l * broadcastLane(w, c0 + t);
+                if (lane > K)
+                    a[k] = l;
+            }
+#pragma unroll
+            for (int t = 0; t < SP; ++t)
+                if (lane >= c0 + t)
+                    S[(c0 + t) * LD + lane] = a[t];
+        }
+        __syncthreads();
+        // Trailing tiles (I, J), c0/16 < J <= I < 4, in the order J, then I.
+        const int P = c0 / SP;
+        const int tiles = (3 - P) * (4 - P) / 2;
+        for (int q = wave; q < tiles; q += 4)
+        {
+            int J = P + 1, I = q;
+            while (I >= 4 - J)
+            {
+                I -= 4 - J;
+                ++J;
+            }
+            I += J;
+            const int col = SP * J + (lane & 15);
+            Acc acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                acc[v] = S[col * LD + SP * I + Mfma<T>::row(lane, v)];
+#pragma unroll
+            for (int kk = 0; kk < SP; kk += 4)
+            {
+                const int pc = c0 + kk + (lane >> 4);
+                const T av = -S[pc * LD + SP * I + (lane & 15)];
+                const T bv = S[pc * LD + col] * S[pc * LD + pc];
+                acc = Mfma<T>::step(av, bv, acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                S[col * LD + SP * I + Mfma<T>::row(lane, v)] = acc[v];
+        }
+        __syncthreads();
+    }
+
+    // Diagonal tiles of the inverse: lane j (lanes from 16 on repeat lane j mod 16 and store nothing).
+    {
+        const int J0 = SP * wave, j = lane & 15;
+        T x[SP];
+#pragma unroll
+        for (int i = 0; i < SP; ++i)
+            x[i] = T(i == j ? 1 : 0);
+#pragma unroll
+        for (int k = 0; k < SP; ++k)
+        {
+#pragma unroll
+            for (int i = k + 1; i < SP; ++i)
+                x[i] -= S[(J0 + k) * LD + J0 + i] * x[k];
+            asm volatile("" ::: "memory"); // one column's reads in flight at a time (registers)
+        }
+        if (lane < SP)
+        {
+#pragma unroll
+            for (int i = 1; i < SP; ++i)
+                if (i > j)
+                    S[(J0 + i) * LD + J0 + j] = x[i];
+        }
+    }
+    __syncthreads();
+    for (int d = 1; d < 4; ++d)
+    {
+        if (wave < 4 - d)
+        {
+            const int J = wave, I = wave + d;
+            const int m = lane & 15;
+            Acc t = {0, 0, 0, 0};
+            for (int K = J; K < I; ++K)
+            {
+#pragma unroll
+                for (int kk = 0; kk < SP; kk += 4)
+                {
+                    const int rk = SP * K + kk + (lane >> 4), cj = SP * J + m;
+                    const T av = S[rk * LD + SP * I + m];                                 // L(16I + m, rk)
+                    const T bv = rk > cj ? S[rk * LD + cj] : T(rk == cj ? 1 : 0);           // X(rk, cj)
+                    t = Mfma<T>::step(av, bv, t);
+                }
+            }
+            Acc u = {0, 0, 0, 0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int kr = Mfma<T>::row(lane, v);
+                const T xv = m > kr ? S[(SP * I + m) * LD + SP * I + kr] : T(m == kr ? 1 : 0); // X_II(m, kr)
+                u = Mfma<T>::step(xv, t[v], u);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                S[(SP * I + Mfma<T>::row(lane, v)) * LD + SP * J + m] = -u[v];
+        }
+        __syncthreads();
+    }
+
+    T* out = linv + p.linvOff[s] + int64_t(k0) * B;
+    T* dst = panel + base + int64_t(wave) * n;
+    for (int j = wave; j < kb; j += 4)
+    {
+        if (lane < kb)
+        {
+            if (lane >= j)
+                dst[lane] = S[j * LD + lane];
+            out[lane + j * kb] = lane == j ? T(1) : (lane > j ? S[lane * LD + j] : T(0));
+        }
+        dst += 4 * int64_t(n);
+    }
+    if (tid < kb)
+        diagOut[p.first[s] + k0 + tid] = double(S[tid * LD + tid]);
+}
 
 constexpr int TILE = 64;
 constexpr int TILE_K = 16;
@@ -965,6 +1123,122 @@ __global__ void axpyColumnsKernel(double* x, const double* d, int64_t ld, int64_
     }
 }
 
+// ---- Krylov vectors: n × k row-major (k <= 16), a row per group of 16 lanes ---------------------------------------
+//
+// Thread t of a workgroup of 256 handles column h = t mod 16 (idle when h >= k) of the rows 16·g + t / 16 + ...,
+// g the workgroup: no integer division, and the k values of a row are one contiguous run of loads.
+
+constexpr int VECTOR_ROWS = 16; // rows of a workgroup's pass
+
+// out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC).
+__global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets, const int32_t* cols,
+                                                         const double* values, const double* x, const double* b,
+                                                         double* out, int64_t n, int k)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    {
+        double sum = 0.0;
+        for (int64_t e = offsets[i]; e < offsets[i + 1]; ++e)
+            sum += values[e] * x[int64_t(cols[e]) * k + h];
+        out[i * k + h] = b ? b[i * k + h] - sum : sum;
+    }
+}
+
+// Per column h the partial sums Σ_i a(i, h)·b(i, h) of this workgroup's rows, in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void columnDotsKernel(const double* a, const double* b, int64_t n, int k,
+                                                       double* partial)
+{
+    __shared__ double sums[256];
+    const int h = threadIdx.x & 15;
+    double sum = 0.0;
+    if (h < k)
+        for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+             i += int64_t(gridDim.x) * VECTOR_ROWS)
+            sum += a[i * k + h] * b[i * k + h];
+    sums[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x < 16)
+    {
+        double total = 0.0;
+        for (int r = 0; r < 16; ++r)
+            total += sums[r * 16 + threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = total;
+    }
+}
+
+// out[h] = Σ over the workgroups' partial sums, in workgroup order.
+__global__ void sumPartialsKernel(const double* partial, int groups, double* out)
+{
+    const int h = threadIdx.x;
+    if (h >= 16)
+        return;
+    double total = 0.0;
+    for (int g = 0; g < groups; ++g)
+        total += partial[g * 16 + h];
+    out[h] = total;
+}
+
+// Per column the largest |v| (bits of a non-negative double, ordered as the values; NaN wins).
+__global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int64_t n, int k, unsigned long long* out)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    {
+        const double a = fabs(v[i * k + h]);
+        m = a > m || a != a ? a : m;
+    }
+    atomicMax(&out[h], static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// x += α_h·p, r −= α_h·q.
+__global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const double* p, const double* q,
+                                                   ColumnCoefficients alpha, int64_t n, int k)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    const double a = alpha.v[h];
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    {
+        x[i * k + h] += a * p[i * k + h];
+        r[i * k + h] -= a * q[i * k + h];
+    }
+}
+
+// p = z + β_h·p.
+__global__ __launch_bounds__(256) void cgDirectionKernel(double* p, const double* z, ColumnCoefficients beta,
+                                                        int64_t n, int k)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    const double b = beta.v[h];
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+        p[i * k + h] = z[i * k + h] + b * p[i * k + h];
+}
+
+// dst(i, h) = src(i, h)·scale[i] (scale null: a copy).
+__global__ __launch_bounds__(256) void scaleRowsKernel(const double* src, double* dst, const double* scale, int64_t n,
+                                                      int k)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+        dst[i * k + h] = scale ? src[i * k + h] * scale[i] : src[i * k + h];
+}
+
 // ---- launchers -----------------------------------------------------------------------------------------------------
 
 namespace
@@ -1000,7 +1274,7 @@ hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* pane
 
 template <typename T>
 hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates,
-                          const AssemblyBlock* blocks, int count, hipStream_t stream)
+                          const AssemblyBlock* blocks, const AssemblyItem* items, int count, hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
@@ -1009,7 +1283,8 @@ hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(assembleKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, blocks);
+    hipLaunchKernelGGL(assembleKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, blocks,
+                       items);
     return hipGetLastError();
 }
 
@@ -1020,7 +1295,7 @@ hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const i
     if (frontCount == 0)
         return hipSuccess;
     // One wave per block: its barriers cost next to nothing.
-    hipLaunchKernelGGL(diagonalBlockKernel<T>, dim3(frontCount), dim3(64), 0, stream, p, panels, linv, fronts, k0,
+    hipLaunchKernelGGL(diagonalBlockKernel<T>, dim3(frontCount), dim3(256), 0, stream, p, panels, linv, fronts, k0,
                        diagOut, zeroAt);
     return hipGetLastError();
 }
@@ -1133,6 +1408,54 @@ hipError_t launchAxpyColumns(double* x, const double* d, int64_t ld, int64_t n, 
     return hipGetLastError();
 }
 
+static int vectorGroups(int64_t n)
+{
+    const int64_t g = (n + VECTOR_ROWS - 1) / VECTOR_ROWS;
+    return int(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
+                              const double* b, double* out, int64_t n, int k, hipStream_t stream)
+{
+    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, offsets, cols, values, x, b,
+                       out, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, double* partial, double* out,
+                            hipStream_t stream)
+{
+    hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
+    hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(64), 0, stream, partial, DOT_GROUPS, out);
+    return hipGetLastError();
+}
+
+hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, v, n, k, out);
+    return hipGetLastError();
+}
+
+hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, const ColumnCoefficients& alpha,
+                        int64_t n, int k, hipStream_t stream)
+{
+    hipLaunchKernelGGL(cgStepKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, x, r, p, q, alpha, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
+                             hipStream_t stream)
+{
+    hipLaunchKernelGGL(cgDirectionKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, p, z, beta, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream)
+{
+    hipLaunchKernelGGL(scaleRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, src, dst, scale, n, k);
+    return hipGetLastError();
+}
+
 hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const float* af, const float* bf, float* df,
                            hipStream_t stream)
 {
@@ -1143,8 +1466,8 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
 #define OOSG_INSTANTIATE(T)                                                                                            \
     template hipError_t launchSmallFronts<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, T*, const int32_t*, int,  \
                                              int, double*, int32_t*, hipStream_t);                                     \
-    template hipError_t launchAssembly<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, const AssemblyBlock*, int,   \
-                                          hipStream_t);                                                                \
+    template hipError_t launchAssembly<T>(const DevicePlan&, const DeviceMatrix&, T*, T*, const AssemblyBlock*,        \
+                                          const AssemblyItem*, int, hipStream_t);                                      \
     template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, T*, const int32_t*, int, int, double*,          \
                                                 int32_t*, hipStream_t);                                                \
     template hipError_t launchUpdateTiles<T>(T*, T*, const T*, const UpdateTile*, int, hipStream_t);                   \

@@ -252,17 +252,21 @@ struct oosg_plan
     std::vector<Launch> schedule;
     std::vector<int32_t> smallList, stepFronts;
     std::vector<oosg::AssemblyBlock> blocks;
+    std::vector<oosg::AssemblyItem> items;
     std::vector<oosg::UpdateTile> tiles;
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dSmallList, dStepFronts, dBlocks, dTiles, dLevelList, dForwardTiles, dBackwardBlocks, dWideItems;
+    DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dLevelList, dForwardTiles, dBackwardBlocks,
+        dWideItems;
     oosg::DevicePlan dp{};
 
     DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
     DeviceBuffer panelCache[2]; // panels given back by destroyed factors, per precision
     DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
+    // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
+    DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
     uint64_t budget = 0;
     std::vector<double> levelPanelValues; // per level: Σ c·(c + r), the panel values a sweep of the level reads
     Profiler profiler;
@@ -288,7 +292,7 @@ struct oosg_matrix
     std::vector<double> diagonal; // a_jj, 0 when not stored
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
-    DeviceBuffer dOffsets, dRows, dValues, dPosition, dScale, dCsrOffsets, dCsrCols, dCsrValues;
+    DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
     std::vector<double> scale;
 };
 
@@ -550,9 +554,26 @@ void buildSchedule(oosg_plan& p)
         }
         if (large.empty())
             continue;
-        // Assembly blocks.
+        // Assembly blocks, each with the parts of the children's update matrices that land in it.
         const int64_t blockStart = static_cast<int64_t>(p.blocks.size());
         int maxC = 0;
+        const auto addBlock = [&p](int32_t s, int j0, int j1, int i0, int i1)
+        {
+            oosg::AssemblyBlock blk{s, j0, j1, i0, i1, static_cast<int32_t>(p.items.size()), 0, 0};
+            for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+            {
+                const int32_t ch = p.children[q];
+                const int32_t* rel = p.rel.data() + p.rowStart[ch];
+                const int32_t rc = p.nrow[ch];
+                const auto at = [&](int value)
+                { return static_cast<int32_t>(std::lower_bound(rel, rel + rc, value) - rel); };
+                const oosg::AssemblyItem it{ch, at(j0), at(j1), at(i0), at(i1), 0};
+                if (it.q0 < it.q1 && std::max(it.q0, it.ia) < it.ib)
+                    p.items.push_back(it);
+            }
+            blk.itemEnd = static_cast<int32_t>(p.items.size());
+            p.blocks.push_back(blk);
+        };
         for (const int32_t s : large)
         {
             const int c = p.ncol[s], n = c + p.nrow[s];
@@ -567,13 +588,13 @@ void buildSchedule(oosg_plan& p)
                     int j1 = std::min(n, j0 + std::max(1, oosg::ASSEMBLY_BLOCK_VALUES / height));
                     if (j0 < c && j1 > c)
                         j1 = c;
-                    p.blocks.push_back({s, j0, j1, i0, n, 0});
+                    addBlock(s, j0, j1, i0, n);
                     j0 = j1;
                 }
                 else
                 {
                     for (int i = i0; i < n; i += oosg::ASSEMBLY_BLOCK_VALUES)
-                        p.blocks.push_back({s, j0, j0 + 1, i, std::min(n, i + oosg::ASSEMBLY_BLOCK_VALUES), 0});
+                        addBlock(s, j0, j0 + 1, i, std::min(n, i + oosg::ASSEMBLY_BLOCK_VALUES));
                     j0 += 1;
                 }
             }
@@ -711,6 +732,7 @@ void uploadPlan(oosg_plan& p)
     p.dSmallList = upload(p.smallList, st, "plan: small fronts");
     p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
     p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
+    p.dItems = upload(p.items, st, "plan: assembly items");
     p.dTiles = upload(p.tiles, st, "plan: update tiles");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
     p.dp.first = p.dFirst.as<int32_t>();
@@ -735,11 +757,13 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
 {
     const hipStream_t st = p.stream;
     oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
-                         m.dPosition.as<int32_t>(), scaled ? m.dScale.as<double>() : nullptr};
+                         m.dPosition.as<int32_t>(), m.dColumn.as<int32_t>(),
+                         scaled ? m.dScale.as<double>() : nullptr};
     T* updates = p.updates.as<T>();
     const int32_t* smallList = p.dSmallList.as<int32_t>();
     const int32_t* stepFronts = p.dStepFronts.as<int32_t>();
     const oosg::AssemblyBlock* blocks = p.dBlocks.as<oosg::AssemblyBlock>();
+    const oosg::AssemblyItem* items = p.dItems.as<oosg::AssemblyItem>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
     for (const Launch& l : p.schedule)
     {
@@ -752,7 +776,7 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
                   "factor: small fronts");
             break;
         case LaunchKind::Assemble:
-            check(oosg::launchAssembly<T>(p.dp, a, panels, updates, blocks + l.offset, l.count, st),
+            check(oosg::launchAssembly<T>(p.dp, a, panels, updates, blocks + l.offset, items, l.count, st),
                   "factor: assembly");
             break;
         case LaunchKind::Diagonal:
@@ -818,7 +842,9 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
     {
         const double pivot = f.diag[j];
         ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
-        const double ratio = m.diagonal[j] / pivot;
+        const double* reference = options.reference_diagonal;
+        const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
+        const double ratio = diagonal / pivot;
         if (largestAt < 0 || ratio > largest)
         {
             largest = ratio;
@@ -940,6 +966,172 @@ std::vector<double> columnMax(oosg_plan& p, const double* v, int64_t ld, int64_t
     return out;
 }
 
+// Row-major n × k vectors (k <= 16) of the conjugate-gradient solve.
+std::vector<double> columnDots(oosg_plan& p, const double* a, const double* b, int k)
+{
+    ensure(p.cgPartial, static_cast<size_t>(oosg::DOT_GROUPS) * 16 * sizeof(double), "solve: partial sums");
+    ensure(p.cgDots, 16 * sizeof(double), "solve: dot products");
+    check(oosg::launchColumnDots(a, b, p.n, k, p.cgPartial.as<double>(), p.cgDots.as<double>(), p.stream),
+          "solve: dot products");
+    std::vector<double> out(16);
+    check(hipMemcpyAsync(out.data(), p.cgDots.ptr, 16 * sizeof(double), hipMemcpyDeviceToHost, p.stream),
+          "solve: dot products");
+    check(hipStreamSynchronize(p.stream), "solve: dot products");
+    out.resize(static_cast<size_t>(k));
+    return out;
+}
+
+std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
+{
+    ensure(p.solveNorms, 16 * sizeof(unsigned long long), "solve: norms");
+    std::vector<unsigned long long> bits(16, 0);
+    check(hipMemcpyAsync(p.solveNorms.ptr, bits.data(), 16 * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                         p.stream),
+          "solve: norms");
+    check(oosg::launchColumnMaxRows(v, p.n, k, p.solveNorms.as<unsigned long long>(), p.stream), "solve: norms");
+    check(hipMemcpyAsync(bits.data(), p.solveNorms.ptr, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         p.stream),
+          "solve: norms");
+    check(hipStreamSynchronize(p.stream), "solve: norms");
+    std::vector<double> out(static_cast<size_t>(k));
+    for (int h = 0; h < k; ++h)
+        std::memcpy(&out[h], &bits[h], sizeof(double));
+    return out;
+}
+
+// out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides.
+void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k)
+{
+    oosg_plan& p = *f.plan;
+    const int64_t n = p.n;
+    ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
+    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
+           "solve: update vectors");
+    const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
+    check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
+    if (f.precision == OOSG_FP32)
+        runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k);
+    else
+        runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k);
+    check(oosg::launchScaleRows(p.solveX.as<double>(), out, scale, n, k, p.stream), "solve: scaling");
+}
+
+// The residual product of a step: its bytes for the profile.
+double residualBytes(const oosg_matrix& m, int64_t n, int k)
+{
+    return static_cast<double>(m.csrFromCsc.size()) * 12.0 + static_cast<double>(n + 1) * 8.0 +
+           2.0 * static_cast<double>(n) * k * 8.0;
+}
+
+// Conjugate gradients on A (symmetric positive definite) preconditioned by the factor (FP32 fronts of the scaled
+// matrix): per column h, x_0 = 0, r_0 = b, z = M⁻¹r, p = z; α = (r·z)/(p·Ap), x += αp, r −= αAp; β = (r'·z')/(r·z),
+// p = z' + βp. When the recurrence's residual meets the target, the true residual b − A·x decides (and replaces the
+// recurrence's when it does not). A converged column stops moving (α = β = 0). Returns the solves with the factor.
+void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
+              double target, int32_t maxSteps, oosg_solve_report& report)
+{
+    oosg_plan& p = *f.plan;
+    const hipStream_t st = p.stream;
+    const int64_t n = p.n;
+    const size_t bytes = static_cast<size_t>(n) * 16 * sizeof(double);
+    ensure(p.cgB, bytes, "solve: conjugate gradients");
+    ensure(p.cgX, bytes, "solve: conjugate gradients");
+    ensure(p.cgR, bytes, "solve: conjugate gradients");
+    ensure(p.cgZ, bytes, "solve: conjugate gradients");
+    ensure(p.cgP, bytes, "solve: conjugate gradients");
+    ensure(p.cgQ, bytes, "solve: conjugate gradients");
+    double* B = p.cgB.as<double>();
+    double* X = p.cgX.as<double>();
+    double* R = p.cgR.as<double>();
+    double* Z = p.cgZ.as<double>();
+    double* P = p.cgP.as<double>();
+    double* Q = p.cgQ.as<double>();
+    const int64_t* offsets = m.dCsrOffsets.as<int64_t>();
+    const int32_t* cols = m.dCsrCols.as<int32_t>();
+    const double* values = m.dCsrValues.as<double>();
+    const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
+
+    check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+    const std::vector<double> bNorm = columnMaxRows(p, B, k);
+    check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
+          "solve: zero");
+    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+    applyFactorRows(f, R, Z, k);
+    int32_t solves = 1;
+    check(hipMemcpyAsync(P, Z, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: direction");
+    std::vector<double> rho = columnDots(p, R, Z, k);
+    std::vector<char> active(static_cast<size_t>(k), 1);
+    std::vector<double> eta(static_cast<size_t>(k), INFINITY);
+    const auto backwardErrors = [&](const double* residual)
+    {
+        const std::vector<double> rN = columnMaxRows(p, residual, k);
+        const std::vector<double> xN = columnMaxRows(p, X, k);
+        std::vector<double> e(static_cast<size_t>(k));
+        for (int h = 0; h < k; ++h)
+        {
+            const double denominator = m.normInf * xN[h] + bNorm[h];
+            const double v = denominator > 0 ? rN[h] / denominator : rN[h];
+            e[h] = v != v ? INFINITY : v;
+        }
+        return e;
+    };
+    bool converged = false;
+    for (int32_t iteration = 0;; ++iteration)
+    {
+        const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
+        check(oosg::launchMultiplyRows(offsets, cols, values, P, nullptr, Q, n, k, st), "solve: product");
+        p.profiler.end(slot, st);
+        const std::vector<double> sigma = columnDots(p, P, Q, k);
+        oosg::ColumnCoefficients alpha{};
+        for (int h = 0; h < k; ++h)
+            alpha.v[h] = active[h] && sigma[h] != 0.0 ? rho[h] / sigma[h] : 0.0;
+        check(oosg::launchCgStep(X, R, P, Q, alpha, n, k, st), "solve: step");
+        const std::vector<double> estimate = backwardErrors(R);
+        bool check_ = true;
+        for (int h = 0; h < k; ++h)
+            if (active[h] && !(estimate[h] <= target))
+                check_ = false;
+        if (check_ || solves >= maxSteps)
+        {
+            // The true residual, into Q.
+            const int slot2 = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
+            check(oosg::launchMultiplyRows(offsets, cols, values, X, B, Q, n, k, st), "solve: residual");
+            p.profiler.end(slot2, st);
+            const std::vector<double> trueEta = backwardErrors(Q);
+            converged = true;
+            for (int h = 0; h < k; ++h)
+            {
+                if (active[h])
+                    eta[h] = trueEta[h];
+                if (active[h] && trueEta[h] <= target)
+                    active[h] = 0;
+                converged = converged && !active[h];
+            }
+            if (converged || solves >= maxSteps)
+                break;
+            // Residual replacement: the recurrence drifted from b − A·x.
+            check(hipMemcpyAsync(R, Q, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+        }
+        applyFactorRows(f, R, Z, k);
+        ++solves;
+        const std::vector<double> rhoNew = columnDots(p, R, Z, k);
+        oosg::ColumnCoefficients beta{};
+        for (int h = 0; h < k; ++h)
+        {
+            beta.v[h] = active[h] && rho[h] != 0.0 ? rhoNew[h] / rho[h] : 0.0;
+            if (active[h])
+                rho[h] = rhoNew[h];
+        }
+        check(oosg::launchCgDirection(P, Z, beta, n, k, st), "solve: direction");
+    }
+    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
+    report.steps = solves;
+    report.converged = converged ? 1 : 0;
+    report.backward_error = 0.0;
+    for (int h = 0; h < k; ++h)
+        report.backward_error = std::max(report.backward_error, eta[h]);
+}
+
 } // namespace
 
 extern "C" {
@@ -977,6 +1169,7 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                                        p->panelOff.size() * 24 +
                                        (p->forwardTiles.size() + p->backwardBlocks.size() + p->wideItems.size()) * 8 +
                                        p->blocks.size() * sizeof(oosg::AssemblyBlock) +
+                                       p->items.size() * sizeof(oosg::AssemblyItem) +
                                        p->tiles.size() * sizeof(oosg::UpdateTile);
             const uint64_t values =
                 static_cast<uint64_t>(p->updateOff[p->S] + p->panelOff[p->S] + p->linvOff[p->S]) * sizeof(double);
@@ -1088,7 +1281,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->rows.assign(csc->rows, csc->rows + m->nnz);
             std::vector<double> values(csc->values, csc->values + m->nnz);
             // Front positions; the structure test of FrontAssembly.cpp:22-28.
-            std::vector<int32_t> position(static_cast<size_t>(m->nnz));
+            std::vector<int32_t> position(static_cast<size_t>(m->nnz)), column(static_cast<size_t>(m->nnz));
             m->diagonal.assign(n, 0.0);
             for (int32_t s = 0; s < plan->S; ++s)
             {
@@ -1120,6 +1313,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
                                  "supernodal factorisation: the matrix entry (" + std::to_string(row) + ", " +
                                      std::to_string(j) + ") lies outside the structure of the symbolic factor");
                         position[k] = pos;
+                        column[k] = j - f;
                         if (row == j)
                             m->diagonal[j] = values[k];
                     }
@@ -1186,6 +1380,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dRows = upload(m->rows, st, "matrix: rows");
             m->dValues = upload(values, st, "matrix: values");
             m->dPosition = upload(position, st, "matrix: positions");
+            m->dColumn = upload(column, st, "matrix: columns");
             m->dScale = upload(m->scale, st, "matrix: scales");
             m->dCsrOffsets = upload(rowCount, st, "matrix: row offsets");
             m->dCsrCols = upload(csrCols, st, "matrix: row columns");
@@ -1482,6 +1677,20 @@ oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m
             ensure(p.solveR, bytes, "refined solve: residuals");
             double* d = p.solveD.as<double>();
             double* r = p.solveR.as<double>();
+            if (f->precision == OOSG_FP32)
+            {
+                const auto t0 = std::chrono::steady_clock::now();
+                solvePcg(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
+                check(hipStreamSynchronize(st), "refined solve");
+                p.profiler.collect();
+                local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (!local.converged)
+                    fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " +
+                                                 std::to_string(local.backward_error) + " did not reach " +
+                                                 std::to_string(target) + " in " + std::to_string(local.steps) +
+                                                 " solves");
+                return;
+            }
             const std::vector<double> bNorm = columnMax(p, b, ld, n, k);
             check(hipEventRecord(p.start, st), "hipEventRecord");
             const auto t0 = std::chrono::steady_clock::now();
