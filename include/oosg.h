@@ -177,6 +177,11 @@ oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, c
 oosg_status oosg_plan_profile(oosg_plan *plan, int32_t enable);
 oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64_t *launches, double *work);
 
+/* Replays of the launch sequences as HIP graphs (default on): a factorisation's launches, and a solve's sweeps, are
+ * captured once per set of buffers and replayed by one graph launch (the role of the reference's reusable solve plan,
+ * SolvePlan.hpp:36-144, on the device side). Off: every kernel is launched eagerly. The profiler launches eagerly. */
+oosg_status oosg_plan_use_graphs(oosg_plan *plan, int32_t enable);
+
 /* Blocks until the plan's stream is idle. */
 oosg_status oosg_plan_synchronize(oosg_plan *plan);
 

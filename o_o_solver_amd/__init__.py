@@ -269,6 +269,10 @@ class Plan:
     def synchronize(self):
         _check(self._lib.oosg_plan_synchronize(self.handle))
 
+    def use_graphs(self, enable=True):
+        """Replays of the factorisation's and the sweeps' launch sequences as HIP graphs (default on)."""
+        _check(self._lib.oosg_plan_use_graphs(self.handle, 1 if enable else 0))
+
     def __del__(self):
         if getattr(self, "handle", None):
             self._lib.oosg_plan_destroy(self.handle)

@@ -69,6 +69,9 @@ struct UpdateTile
     int32_t m, n, k, target, lower, mode;
 };
 
+// Sets the LDS limits of the kernels that need more than the default; once per process and device, before any launch.
+hipError_t configureKernels();
+
 template <typename T>
 hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* panels, T* updates, T* linv,
                              const int32_t* list, int count, int maxOrder, double* diagOut, int32_t* zeroAt,

@@ -536,68 +536,99 @@ __global__ __launch_bounds__(256, 2) void diagonalBlockKernel(DevicePlan p, T* p
 }
 
 constexpr int TILE = 64;
-constexpr int TILE_K = 16;
+constexpr int TILE_K = 32; // terms of a chunk staged in LDS
 
+// The tile a workgroup takes: consecutive tiles of the list (the tiles of one front and panel step, which share their
+// A and B blocks) go to the workgroups of one XCD (the hardware deals workgroups to the 8 XCDs in turn), so that the
+// shared blocks are read into one L2.
+__device__ inline int tileOfWorkgroup(int count)
+{
+    const int b = blockIdx.x, full = count - count % 8;
+    return b < full ? (b & 7) * (full >> 3) + (b >> 3) : b;
+}
+
+// Thread t stages row t mod 64 of A and of B for the terms t / 64 + 4u (u < TILE_K / 4) of a chunk; each chunk's loads
+// are issued while the previous chunk is multiplied (registers as the second buffer), from positions clamped into the
+// operands; mode 0 accumulates onto C itself (the accumulators start from the C tile, A enters negated).
 template <typename T>
 __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, const T* linv,
-                                                         const UpdateTile* tiles)
+                                                         const UpdateTile* tiles, int count)
 {
+    constexpr int PER = TILE * TILE_K / 256;
     __shared__ T As[TILE_K][TILE + 4];
     __shared__ T Bs[TILE_K][TILE + 4];
-    const UpdateTile t = tiles[blockIdx.x];
+    const UpdateTile t = tiles[tileOfWorkgroup(count)];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const bool solve = t.mode == 1;
+    const T* A = panels + t.aOff;
+    const T* Bm = (solve ? linv : panels) + t.bOff;
+    const T* dv = panels + t.dOff;
+    const int64_t ld = t.ld, ldb = solve ? t.ldb : t.ld;
+    T* C = (t.target == 0 ? panels : updates) + t.cOff;
+
     typename Mfma<T>::Acc acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
-            acc[a][b] = typename Mfma<T>::Acc{0, 0, 0, 0};
-    const T* A = panels + t.aOff;
-    const T* Bm = (t.mode == 1 ? linv : panels) + t.bOff;
-    const T* dv = panels + t.dOff;
-    const int64_t ld = t.ld, ldb = t.mode == 1 ? t.ldb : t.ld;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int i = min(wm + a * 16 + Mfma<T>::row(lane, v), t.m - 1);
+                const int j = min(wn + b * 16 + (lane & 15), t.n - 1);
+                acc[a][b][v] = solve ? T(0) : C[i + int64_t(j) * t.ldc];
+            }
+
+    const int row = tid & (TILE - 1), kq = tid >> 6;
+    const T* Ar = A + min(row, t.m - 1);
+    const T* Br = Bm + min(row, t.n - 1);
+    T ra[PER], rb[PER], rd[PER];
+    const auto load = [&](int p0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int64_t pp = min(p0 + kq + 4 * u, t.k - 1);
+            ra[u] = Ar[pp * ld];
+            rb[u] = Br[pp * ldb];
+            rd[u] = solve ? T(1) : dv[pp * (ld + 1)];
+        }
+    };
+    load(0);
     for (int p0 = 0; p0 < t.k; p0 += TILE_K)
     {
-        // Stage A and B (times D in mode 0): 64 rows × 16 terms each, 4 elements per thread, coalesced along the rows.
+        __syncthreads(); // the previous chunk's products have read the LDS
 #pragma unroll
-        for (int q = 0; q < (TILE * TILE_K) / 256; ++q)
+        for (int u = 0; u < PER; ++u)
         {
-            const int idx = tid + q * 256;
-            const int row = idx & (TILE - 1), kk = idx >> 6;
-            const int pp = p0 + kk;
-            T av = T(0), bv = T(0);
-            if (pp < t.k)
-            {
-                if (row < t.m)
-                    av = A[row + pp * ld];
-                if (row < t.n)
-                    bv = t.mode == 1 ? Bm[row + pp * ldb] : Bm[row + pp * ld] * dv[pp * (ld + 1)];
-            }
-            As[kk][row] = av;
-            Bs[kk][row] = bv;
+            const int kk = kq + 4 * u;
+            const bool term = p0 + kk < t.k;
+            const T av = pinned(ra[u]), bv = pinned(rb[u]), dd = pinned(rd[u]);
+            As[kk][row] = term && row < t.m ? (solve ? av : -av) : T(0);
+            Bs[kk][row] = term && row < t.n ? (solve ? bv : bv * dd) : T(0);
         }
         __syncthreads();
+        if (p0 + TILE_K < t.k)
+            load(p0 + TILE_K);
 #pragma unroll
         for (int kk = 0; kk < TILE_K; kk += 4)
         {
             const int ka = kk + (lane >> 4);
+            T av[2], bv[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
-            {
-                const T av = As[ka][wm + a * 16 + (lane & 15)];
+                av[a] = As[ka][wm + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bv[b] = Bs[ka][wn + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                {
-                    const T bv = Bs[ka][wn + b * 16 + (lane & 15)];
-                    acc[a][b] = Mfma<T>::step(av, bv, acc[a][b]);
-                }
-            }
+                    acc[a][b] = Mfma<T>::step(av[a], bv[b], acc[a][b]);
         }
-        __syncthreads();
     }
-    T* C = (t.target == 0 ? panels : updates) + t.cOff;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -609,10 +640,7 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
                 const int j = wn + b * 16 + (lane & 15);
                 if (i >= t.m || j >= t.n || (t.lower && i < j))
                     continue;
-                if (t.mode == 1)
-                    C[i + int64_t(j) * t.ldc] = acc[a][b][v] / dv[j * (ld + 1)];
-                else
-                    C[i + int64_t(j) * t.ldc] -= acc[a][b][v];
+                C[i + int64_t(j) * t.ldc] = solve ? acc[a][b][v] / dv[j * (ld + 1)] : acc[a][b][v];
             }
 }
 
@@ -649,14 +677,23 @@ constexpr int SB = 64;       // rows or columns of a staged panel block
 constexpr int SBP = SB + 1;  // its padded leading dimension in LDS
 constexpr int KMAX = 16;     // right-hand sides per sweep
 
-// blk[q·SBP + i] = L(row0 + i, col0 + q), zero outside [0, rows) × [0, cols).
+// blk[q·SBP + i] = L(row0 + i, col0 + q), zero outside [0, rows) × [0, cols) (1 <= rows, cols <= SB); a workgroup of
+// 256 threads, thread t moving row t mod 64 of the columns t / 64 + 4u. Every load is issued, from a position clamped
+// into the block, before any is used.
 template <typename TP>
 __device__ inline void stageBlock(double* blk, const TP* L, int64_t ld, int row0, int rows, int col0, int cols)
 {
-    for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x)
+    const int i = threadIdx.x & (SB - 1), q0 = threadIdx.x >> 6;
+    const TP* src = L + (row0 + min(i, rows - 1)) + int64_t(col0) * ld;
+    TP v[SB / 4];
+#pragma unroll
+    for (int u = 0; u < SB / 4; ++u)
+        v[u] = src[int64_t(min(q0 + 4 * u, cols - 1)) * ld];
+#pragma unroll
+    for (int u = 0; u < SB / 4; ++u)
     {
-        const int i = idx & (SB - 1), q = idx >> 6;
-        blk[q * SBP + i] = (i < rows && q < cols) ? double(L[(row0 + i) + int64_t(col0 + q) * ld]) : 0.0;
+        const int q = q0 + 4 * u;
+        blk[q * SBP + i] = (i < rows && q < cols) ? double(pinned(v[u])) : 0.0;
     }
 }
 
@@ -726,13 +763,25 @@ __device__ inline void applyBlockInverse(const TP* inv, int qb, double* blk, dou
     __syncthreads();
 }
 
-// ys ← the rows row0 .. row0 + qb of X (k columns), zero padded to SB × KMAX.
+// ys ← the rows row0 .. row0 + qb of X (k columns), zero padded to SB × KMAX (1 <= qb <= SB); loads first, from
+// positions clamped into the rows.
 __device__ inline void loadBlockRows(double* ys, const double* X, int64_t row0, int qb, int k)
 {
-    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
+    constexpr int PER = SB * KMAX / 256;
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
     {
+        const int idx = threadIdx.x + 256 * u;
         const int i = idx >> 4, h = idx & (KMAX - 1);
-        ys[idx] = (i < qb && h < k) ? X[(row0 + i) * k + h] : 0.0;
+        v[u] = X[(row0 + min(i, qb - 1)) * k + min(h, k - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+    {
+        const int idx = threadIdx.x + 256 * u;
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        ys[idx] = (i < qb && h < k) ? pinned(v[u]) : 0.0;
     }
 }
 
@@ -859,20 +908,33 @@ __global__ void diagonalSolveKernel(double* X, const double* d, int64_t n, int k
     }
 }
 
-// ys ← the rows rows[i0 − c + i] (or first + i0 + i below c) of X, for i < ib.
+// xs ← the rows rows[i0 − c + i] (or first + i0 + i below c) of X, for i < ib (1 <= ib <= SB); the row indices are
+// loaded first, then the values, from positions clamped into the rows.
 __device__ inline void gatherRows(double* xs, const double* X, const int32_t* rows, int first, int c, int i0, int ib,
                                   int k)
 {
-    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
+    constexpr int PER = SB * KMAX / 256;
+    int64_t row[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
     {
+        const int idx = threadIdx.x + 256 * u;
+        const int at = i0 + min(idx >> 4, ib - 1);
+        row[u] = at < c ? int64_t(first + at) : int64_t(rows[max(at - c, 0)]);
+    }
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+    {
+        const int idx = threadIdx.x + 256 * u;
+        v[u] = X[row[u] * k + min(idx & (KMAX - 1), k - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+    {
+        const int idx = threadIdx.x + 256 * u;
         const int i = idx >> 4, h = idx & (KMAX - 1);
-        double v = 0.0;
-        if (i < ib && h < k)
-        {
-            const int row = i0 + i;
-            v = row < c ? X[int64_t(first + row) * k + h] : X[int64_t(rows[row - c]) * k + h];
-        }
-        xs[idx] = v;
+        xs[idx] = (i < ib && h < k) ? pinned(v[u]) : 0.0;
     }
 }
 
@@ -1263,10 +1325,6 @@ hipError_t launchSmallFronts(const DevicePlan& p, const DeviceMatrix& a, T* pane
     if (count == 0)
         return hipSuccess;
     const size_t bytes = smallFrontBytes<T>(maxOrder);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&smallFrontKernel<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
-    if (e != hipSuccess)
-        return e;
     hipLaunchKernelGGL(smallFrontKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, linv, list,
                        diagOut, zeroAt);
     return hipGetLastError();
@@ -1279,10 +1337,6 @@ hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels,
     if (count == 0)
         return hipSuccess;
     const size_t bytes = size_t(ASSEMBLY_BLOCK_VALUES) * sizeof(T);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&assembleKernel<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
-    if (e != hipSuccess)
-        return e;
     hipLaunchKernelGGL(assembleKernel<T>, dim3(count), dim3(256), bytes, stream, p, a, panels, updates, blocks,
                        items);
     return hipGetLastError();
@@ -1306,7 +1360,7 @@ hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateT
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, linv, tiles);
+    hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, linv, tiles, count);
     return hipGetLastError();
 }
 
@@ -1406,6 +1460,25 @@ hipError_t launchAxpyColumns(double* x, const double* d, int64_t ld, int64_t n, 
 {
     hipLaunchKernelGGL(axpyColumnsKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, x, d, ld, n, k);
     return hipGetLastError();
+}
+
+// The dynamic LDS limits of the kernels that take more than the default, set once per process before any launch
+// (outside any graph capture).
+hipError_t configureKernels()
+{
+    const void* kernels[4] = {reinterpret_cast<const void*>(&smallFrontKernel<double>),
+                              reinterpret_cast<const void*>(&smallFrontKernel<float>),
+                              reinterpret_cast<const void*>(&assembleKernel<double>),
+                              reinterpret_cast<const void*>(&assembleKernel<float>)};
+    const size_t bytes[4] = {smallFrontBytes<double>(SMALL_FRONT_MAX), smallFrontBytes<float>(SMALL_FRONT_MAX),
+                             size_t(ASSEMBLY_BLOCK_VALUES) * sizeof(double), size_t(ASSEMBLY_BLOCK_VALUES) * sizeof(float)};
+    for (int q = 0; q < 4; ++q)
+    {
+        const hipError_t e = hipFuncSetAttribute(kernels[q], hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes[q]));
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 static int vectorGroups(int64_t n)

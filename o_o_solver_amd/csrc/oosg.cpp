@@ -215,6 +215,50 @@ struct Profiler
     }
 };
 
+// A launch sequence replayed as a HIP graph: captured from the stream the first time, and again whenever the buffers
+// it names (its key) change; otherwise one hipGraphLaunch replaces its launches.
+struct GraphCache
+{
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> key;
+
+    ~GraphCache() { clear(); }
+    void clear()
+    {
+        if (exec != nullptr)
+            (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    template <typename F> void run(hipStream_t stream, std::vector<const void*> newKey, F&& body)
+    {
+        if (exec == nullptr || newKey != key)
+        {
+            clear();
+            check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph: capture");
+            try
+            {
+                body();
+            }
+            catch (...)
+            {
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(stream, &partial);
+                if (partial != nullptr)
+                    (void)hipGraphDestroy(partial);
+                throw;
+            }
+            hipGraph_t graph = nullptr;
+            check(hipStreamEndCapture(stream, &graph), "graph: capture");
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            check(e, "graph: instantiate");
+            key = std::move(newKey);
+        }
+        check(hipGraphLaunch(exec, stream), "graph: launch");
+    }
+};
+
 double frontFlops(int64_t c, int64_t r)
 {
     double f = 0.0;
@@ -264,15 +308,24 @@ struct oosg_plan
     DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
     DeviceBuffer panelCache[2]; // panels given back by destroyed factors, per precision
     DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
+    DeviceBuffer diagCache;     // and their pivots
+    DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
     DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
     uint64_t budget = 0;
+    // Launch sequences replayed as graphs (off while the profiler times single launches): the factorisation per
+    // precision, and the sweeps.
+    bool graphs = true;
+    GraphCache factorGraph[2], sweepGraph;
     std::vector<double> levelPanelValues; // per level: Σ c·(c + r), the panel values a sweep of the level reads
     Profiler profiler;
 
     ~oosg_plan()
     {
+        factorGraph[0].clear();
+        factorGraph[1].clear();
+        sweepGraph.clear();
         if (start)
             (void)hipEventDestroy(start);
         if (stop)
@@ -312,6 +365,8 @@ struct oosg_factor
             plan->panelCache[precision] = std::move(panels);
         if (plan != nullptr && linv.ptr != nullptr && plan->linvCache[precision].ptr == nullptr)
             plan->linvCache[precision] = std::move(linv);
+        if (plan != nullptr && dDiag.ptr != nullptr && plan->diagCache.ptr == nullptr)
+            plan->diagCache = std::move(dDiag);
     }
 };
 
@@ -925,6 +980,28 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     }
 }
 
+// p.solveX ← factor⁻¹·p.solveX (row-major n × k), as a graph replay unless the profiler is on.
+void sweeps(const oosg_factor& f, int k)
+{
+    oosg_plan& p = *f.plan;
+    const auto body = [&]
+    {
+        if (f.precision == OOSG_FP32)
+            runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k);
+        else
+            runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k);
+    };
+    if (!p.graphs || p.profiler.enabled)
+    {
+        body();
+        return;
+    }
+    p.sweepGraph.run(p.stream,
+                     {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr,
+                      reinterpret_cast<const void*>(static_cast<intptr_t>(k * 2 + f.precision))},
+                     body);
+}
+
 // b (device, column-major, ld) ← factor⁻¹·b, by blocks of 16 columns.
 void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
 {
@@ -940,10 +1017,7 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
         const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
         double* block = b + c0 * ld;
         check(oosg::launchToRowMajor(block, ld, p.solveX.as<double>(), n, k, scale, p.stream), "solve: layout");
-        if (f.precision == OOSG_FP32)
-            runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k);
-        else
-            runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k);
+        sweeps(f, k);
         check(oosg::launchFromRowMajor(p.solveX.as<double>(), block, ld, n, k, scale, 0, p.stream), "solve: layout");
     }
 }
@@ -1009,10 +1083,7 @@ void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k)
            "solve: update vectors");
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
     check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
-    if (f.precision == OOSG_FP32)
-        runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k);
-    else
-        runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k);
+    sweeps(f, k);
     check(oosg::launchScaleRows(p.solveX.as<double>(), out, scale, n, k, p.stream), "solve: scaling");
 }
 
@@ -1181,6 +1252,7 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                          std::to_string((static_cast<uint64_t>(p->updateOff[p->S]) * 8) >> 20) + " MiB), " +
                          std::to_string(p->budget >> 20) + " MiB are available");
             check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+            check(oosg::configureKernels(), "plan: kernel limits");
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
             uploadPlan(*p);
@@ -1243,6 +1315,24 @@ oosg_status oosg_plan_profile_read(const oosg_plan* plan, double* seconds, int64
                     launches[k] = plan->profiler.launches[k];
                 if (work)
                     work[k] = plan->profiler.work[k];
+            }
+        });
+}
+
+oosg_status oosg_plan_use_graphs(oosg_plan* plan, int32_t enable)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            setDevice(plan);
+            plan->graphs = enable != 0;
+            if (!plan->graphs)
+            {
+                plan->factorGraph[0].clear();
+                plan->factorGraph[1].clear();
+                plan->sweepGraph.clear();
             }
         });
 }
@@ -1482,18 +1572,35 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             else
                 f->linv.allocate(linvBytes, "factor: inverses of the diagonal blocks");
             DeviceBuffer& diag = f->dDiag;
-            DeviceBuffer zero;
-            diag.allocate(static_cast<size_t>(plan->n) * sizeof(double), "factor: pivots");
-            zero.allocate(static_cast<size_t>(std::max(plan->S, 1)) * sizeof(int32_t), "factor: pivot status");
-            const hipStream_t st = plan->stream;
-            check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
-            check(hipEventRecord(plan->start, st), "hipEventRecord");
-            if (opts.precision == OOSG_FP32)
-                runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
-                                 zero.as<int32_t>(), true);
+            const size_t diagBytes = static_cast<size_t>(plan->n) * sizeof(double);
+            if (plan->diagCache.ptr != nullptr && plan->diagCache.bytes >= diagBytes)
+                diag = std::move(plan->diagCache);
             else
-                runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(), diag.as<double>(),
-                                  zero.as<int32_t>(), false);
+                diag.allocate(diagBytes, "factor: pivots");
+            ensure(plan->pivotStatus, static_cast<size_t>(std::max(plan->S, 1)) * sizeof(int32_t),
+                   "factor: pivot status");
+            DeviceBuffer& zero = plan->pivotStatus;
+            const hipStream_t st = plan->stream;
+            check(hipEventRecord(plan->start, st), "hipEventRecord");
+            const auto body = [&]
+            {
+                check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
+                if (opts.precision == OOSG_FP32)
+                    runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
+                                     zero.as<int32_t>(), true);
+                else
+                    runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(),
+                                      diag.as<double>(), zero.as<int32_t>(), false);
+            };
+            if (plan->graphs && !plan->profiler.enabled)
+                plan->factorGraph[opts.precision].run(
+                    st,
+                    {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,
+                     matrix->dColumn.ptr, matrix->dOffsets.ptr, matrix->dRows.ptr, matrix->dScale.ptr,
+                     plan->updates.ptr},
+                    body);
+            else
+                body();
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
             std::vector<int32_t> zeroAt(static_cast<size_t>(plan->S));
             f->diag.resize(static_cast<size_t>(plan->n));

@@ -108,3 +108,82 @@ def shell_plate_c2(scale=500, seed=20261203):
     """The shell plate of the headline workload: scale × scale quad4 over 1 m, interior nodes jittered ±20 % of the
     element size, per-element thickness log-uniform in [1, 4] mm, E = 70 GPa, ν = 0.3, clamped edges."""
     return plate(scale, jitter=0.2, seed=seed, thickness=(1e-3, 4e-3), thickness_mode="log-uniform")
+
+
+class Batch:
+    """Independent models factorised as one: the block-diagonal union of their matrices, whose elimination forest
+    has one tree per model, so that every level's fronts of all the models go into the same launches. Each model is
+    ordered by its own dissection; its equations are one contiguous range of the union's."""
+
+    def __init__(self, members, name=None):
+        self.members = list(members)
+        self.name = name or f"batch of {len(self.members)}"
+        self.symbolic = None
+        self.matrix = None
+        self.ranges = None  # per member (first equation, equation count), in factor order
+
+    @property
+    def n(self):
+        return sum(w.n for w in self.members)
+
+    def analyse(self):
+        node_base = np.cumsum([0] + [w.node_count for w in self.members])
+        offsets, adjacency, weights, order = [np.zeros(1, np.int64)], [], [], []
+        edge_base = 0
+        for w, base in zip(self.members, node_base[:-1]):
+            offsets.append(w.graph_offsets[1:] + edge_base)
+            edge_base += int(w.graph_offsets[-1])
+            adjacency.append(w.adjacency + base)
+            weights.append(w.weights)
+            order.append(nested_dissection(w.graph_offsets, w.adjacency, w.weights, w.coordinates) + base)
+        self.graph_offsets = np.concatenate(offsets)
+        self.adjacency = np.concatenate(adjacency).astype(np.int32)
+        self.weights = np.concatenate(weights).astype(np.int32)
+        self.symbolic = SymbolicFactor.analyse(self.graph_offsets, self.adjacency, self.weights,
+                                               np.concatenate(order).astype(np.int32))
+        # Each model is one tree of the forest: its nodes are one contiguous run of the final order.
+        node_order = self.symbolic.node_order
+        owner = np.searchsorted(node_base, node_order, side="right") - 1
+        starts = np.flatnonzero(np.r_[True, owner[1:] != owner[:-1]])
+        if len(starts) != len(self.members):
+            raise ValueError("batch: a model's nodes are not contiguous in the factor order")
+        dof_of_position = np.concatenate([[0], np.cumsum(self.weights[node_order])])
+        self.ranges = [None] * len(self.members)
+        columns, rows, values, col_offsets = [], [], [], [np.zeros(1, np.int64)]
+        nnz = 0
+        for q, start in enumerate(starts):
+            m = int(owner[start])
+            w = self.members[m]
+            local_order = (node_order[start:start + w.node_count] - node_base[m]).astype(np.int32)
+            w.symbolic = None
+            csc = w.csc(local_order)
+            w.local_order = local_order
+            eq0 = int(dof_of_position[start])
+            self.ranges[m] = (eq0, w.n)
+            col_offsets.append(csc.offsets[1:] + nnz)
+            nnz += int(csc.offsets[-1])
+            rows.append(csc.rows + eq0)
+            values.append(csc.values)
+        self.matrix = SymmetricCsc(self.n, np.concatenate(col_offsets), np.concatenate(rows).astype(np.int32),
+                                   np.concatenate(values))
+        return self.symbolic
+
+    def loads(self, kind="random", nrhs=1, seed=1):
+        """The members' loads (member m with seed + m) stacked in the union's equation order."""
+        b = np.zeros((self.n, nrhs), order="F")
+        for m, w in enumerate(self.members):
+            eq0, count = self.ranges[m]
+            block = np.zeros((nrhs, w.n))
+            _native.models().oosm_loads(w.handle, _ptr(w.local_order), {"pressure": 0, "random": 1, "top": 2}[kind],
+                                        int(nrhs), int(seed + m), _ptr(block))
+            b[eq0:eq0 + count, :] = block.T
+        return b
+
+
+def plate_batch_c4(count=512, seed=20261204, sides=(20, 80)):
+    """The batch of clamped square plates: side n_e ~ U{20..80} elements over 1 m, thickness ~ U[1, 5] mm per
+    element, E = 70 GPa, ν = 0.3."""
+    rng = np.random.default_rng(seed)
+    members = [plate(int(rng.integers(sides[0], sides[1] + 1)), seed=seed + q, thickness=(1e-3, 5e-3),
+                     thickness_mode="uniform") for q in range(count)]
+    return Batch(members, name=f"batch of {count} plates")

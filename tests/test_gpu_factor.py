@@ -19,9 +19,12 @@ pytestmark = pytest.mark.gpu
 
 
 def backward_error(matrix, x, b):
-    dense = matrix.to_dense()
-    r = b - dense @ x
-    return np.max(np.abs(r), axis=0).max() / (np.abs(dense).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+    import scipy.sparse as sp
+    lower = sp.csc_matrix((matrix.values, matrix.rows, matrix.offsets), shape=(matrix.n, matrix.n))
+    full = (lower + sp.tril(lower, -1).T).tocsr()
+    r = b - full @ x
+    norm = np.abs(full).sum(axis=1).max()
+    return np.max(np.abs(r), axis=0).max() / (norm * np.abs(x).max() + np.abs(b).max())
 
 
 def test_mfma_lane_layout(gpu_device):
@@ -230,3 +233,79 @@ def test_fp32_fronts_refined_to_fp64_accuracy(gpu_device, case):
     x32 = x32.cpu().numpy()
     x64 = x64.cpu().numpy()
     assert np.max(np.abs(x32 - x64)) <= 1e-6 * np.max(np.abs(x64))
+
+
+def test_batch_factor_equals_the_members_factors(gpu_device):
+    # A batch of independent plates in one plan (one tree per plate): every front's arithmetic depends on its own
+    # data only, so the batched pivots are the members' own pivots, bit for bit.
+    print("[gpu] batch of plates", flush=True)
+    bt = models.plate_batch_c4(count=6, seed=21, sides=(6, 30))
+    sym = bt.analyse()
+    batched = SupernodalFactor.factorise(sym, bt.matrix)
+    assert batched.inertia.positive == bt.n
+    d = batched.diagonal()
+    for m, w in enumerate(bt.members):
+        own = SymbolicFactor.analyse(w.graph_offsets, w.adjacency, w.weights, w.local_order)
+        f = SupernodalFactor.factorise(own, w.csc(w.local_order))
+        eq0, count = bt.ranges[m]
+        assert np.array_equal(d[eq0:eq0 + count], f.diagonal())
+    b = bt.loads("random", nrhs=1, seed=5)
+    x = FactorSolve(batched).solve(b)
+    assert backward_error(bt.matrix, x, b) <= 1e-12
+
+
+def test_reference_diagonal_of_the_ratio_test(gpu_device):
+    # FactorOptions::referenceDiagonal: the ratio test divides the reference (where it is not negative) by d_j.
+    w = CASES["plate 12"]()
+    sym = w.analyse()
+    plain = SupernodalFactor.factorise(sym, w.matrix)
+    d = plain.diagonal()
+    same = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(reference_diagonal=np.full(w.n, -1.0)),
+                                      device_matrix=plain.device_matrix)
+    assert same.pivot_diagnostics.issue_count == plain.pivot_diagnostics.issue_count == 0
+    j = w.n // 3
+    reference = np.full(w.n, -1.0)
+    reference[j] = 2.0 * 1e7 * d[j]
+    options = FactorOptions(pivot_policy=oos.PivotPolicy(bailout=-1), reference_diagonal=reference)
+    flagged = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=plain.device_matrix)
+    diagnostics = flagged.pivot_diagnostics
+    assert diagnostics.issue_count == 1 and diagnostics.first_issue_equation == j
+    assert not diagnostics.first_issue_negative
+    with pytest.raises(oos.SingularityError, match=f"equation {j}"):
+        SupernodalFactor.factorise(sym, w.matrix, FactorOptions(reference_diagonal=reference),
+                                   device_matrix=plain.device_matrix)
+    with pytest.raises(ValueError, match="reference diagonal"):
+        SupernodalFactor.factorise(sym, w.matrix, FactorOptions(reference_diagonal=np.ones(3)),
+                                   device_matrix=plain.device_matrix)
+
+
+def test_graph_replays_equal_eager_launches(gpu_device):
+    # The factorisation and the sweeps replayed as HIP graphs give the eager launches' results bit for bit, replay
+    # after replay, with other device work and synchronisations between the replays.
+    import torch
+    print("[gpu] graph replays", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    plan = sym.plan()
+    plan.use_graphs(False)
+    eager = SupernodalFactor.factorise(sym, w.matrix)
+    dm = eager.device_matrix
+    d0 = eager.diagonal()
+    b = w.loads("random", nrhs=3, seed=9)
+    x0 = FactorSolve(eager).solve(b)
+    bd = torch.tensor(b, device=gpu_device)
+    e32 = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=Precision.FP32), device_matrix=dm)
+    y0, r0 = FactorSolve(e32).solve_refined(bd, target=1e-10, max_steps=30)
+    del eager, e32
+    plan.use_graphs(True)
+    for replay in range(3):
+        f = SupernodalFactor.factorise(sym, w.matrix, device_matrix=dm)
+        assert np.array_equal(f.diagonal(), d0), replay
+        assert np.array_equal(FactorSolve(f).solve(b), x0), replay
+        assert float(torch.ones(4096, device=gpu_device).sum().item()) == 4096.0
+        del f
+        f32 = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=Precision.FP32), device_matrix=dm)
+        y, r = FactorSolve(f32).solve_refined(bd, target=1e-10, max_steps=30)
+        assert r.steps == r0.steps and torch.equal(y, y0), replay
+        torch.cuda.synchronize()
+        del f32
