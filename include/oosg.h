@@ -63,7 +63,11 @@ typedef struct oosg_csc_desc {
 
 typedef enum oosg_pivoting {
     OOSG_POSITIVE_DEFINITE = 0, /* PivotingMode::PositiveDefinite (SupernodalFactor.hpp:34-42) */
-    OOSG_INDEFINITE = 1         /* PivotingMode::Indefinite: Bunch-Kaufman fronts, delayed pivots */
+    /* PivotingMode::Indefinite, FP64 only. The fronts are eliminated with 1 x 1 pivots in the static order (no
+     * Bunch-Kaufman interchanges, no delayed pivots yet): the inertia is the matrix's own (Sylvester) unless a pivot
+     * breaks down, which is reported as OOSG_SINGULAR; largest_ratio holds max |a_jj / d_j|, the growth near a
+     * breakdown. No ratio test, as in the reference's indefinite mode. */
+    OOSG_INDEFINITE = 1
 } oosg_pivoting;
 
 typedef enum oosg_precision {
@@ -184,6 +188,33 @@ oosg_status oosg_plan_use_graphs(oosg_plan *plan, int32_t enable);
 
 /* Blocks until the plan's stream is idle. */
 oosg_status oosg_plan_synchronize(oosg_plan *plan);
+
+/* ---- the learned precision selector (no counterpart in the reference) -------------------------------------- */
+
+/* A small MLP (8 features → 32 → 32 → 1, tanh hidden layers, logistic output) that gives p(FP32 fronts with FP64
+ * refinement reach the target backward error) for a matrix of a plan; FP32 on the device, forward, backward and Adam
+ * steps as HIP kernels with deterministic reductions. Arrays are host memory. */
+#define OOSG_SELECTOR_FEATURES 8
+#define OOSG_SELECTOR_PARAMETERS 1377
+typedef struct oosg_selector oosg_selector;
+/* Parameters drawn Glorot-uniform from `seed` (biases zero). */
+oosg_status oosg_selector_create(int32_t device, uint64_t seed, oosg_selector **out);
+void oosg_selector_destroy(oosg_selector *selector);
+/* The flat parameter array: W1 (32 × 8 row-major), b1 (32), W2 (32 × 32), b2 (32), W3 (32), b3 (1). */
+oosg_status oosg_selector_parameters(const oosg_selector *selector, float *params);
+oosg_status oosg_selector_set_parameters(oosg_selector *selector, const float *params);
+/* The 8 features of a matrix of a plan (normalised to about unit range): log10 n, log10 flops, log10 of the largest
+ * front order, log10 of the spread of the positive diagonal, the mean and the minimum of log10(a_jj / Σ_i≠j |a_ij|),
+ * the fraction of equations whose diagonal does not dominate, log10(‖A‖∞ / max a_jj). */
+oosg_status oosg_selector_features(const oosg_plan *plan, const oosg_matrix *matrix, float *features);
+/* p for count × 8 features. */
+oosg_status oosg_selector_predict(oosg_selector *selector, const float *features, int64_t count, float *probability);
+/* The mean binary cross-entropy over count samples (labels 0 or 1) and its gradient by the parameters. */
+oosg_status oosg_selector_gradient(oosg_selector *selector, const float *features, const float *labels, int64_t count,
+                                   float *loss, float *gradient);
+/* `steps` full-batch Adam steps (learning rate lr); *loss receives the loss before the last step. */
+oosg_status oosg_selector_train(oosg_selector *selector, const float *features, const float *labels, int64_t count,
+                                int32_t steps, float learning_rate, float *loss);
 
 /* ---- host analysis (the reference keeps these on the host too) --------------------------------------------- */
 

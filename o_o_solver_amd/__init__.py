@@ -68,6 +68,8 @@ class PivotingMode(enum.IntEnum):
 class Precision(enum.IntEnum):
     FP64 = 0
     FP32 = 1
+    # The learned selector chooses FP32 or FP64 per matrix (FactorOptions.selector, or the committed weights).
+    LEARNED = 2
 
 
 @dataclass
@@ -87,6 +89,10 @@ class FactorOptions:
     # The diagonal the ratio test divides by the pivots: one value per equation in factor order (a negative value
     # keeps the matrix diagonal there), or None (FactorOptions::referenceDiagonal, SupernodalFactor.hpp:75).
     reference_diagonal: object = None
+    # Precision.LEARNED: the selector (a selector.PrecisionSelector; None: the committed weights) and the probability
+    # from which it picks FP32.
+    selector: object = None
+    fp32_threshold: float = 0.5
 
 
 @dataclass(frozen=True)
@@ -135,6 +141,19 @@ class SymmetricCsc:
 
     def desc(self):
         return _native.CscDesc(self.n, _ptr(self.offsets), _ptr(self.rows), _ptr(self.values))
+
+    def multiply(self, x):
+        """y = A·x from the lower triangle (SymmetricCsc::multiply, SymmetricCsc.hpp:86-96), host arrays n or n × k."""
+        x = np.asarray(x, dtype=np.float64)
+        columns = np.repeat(np.arange(self.n), np.diff(self.offsets))
+        off = self.rows != columns
+        one = x.ndim == 1
+        xs = x.reshape(self.n, -1)
+        y = np.zeros_like(xs)
+        for h in range(xs.shape[1]):
+            y[:, h] = np.bincount(self.rows, weights=self.values * xs[columns, h], minlength=self.n)
+            y[:, h] += np.bincount(columns[off], weights=self.values[off] * xs[self.rows[off], h], minlength=self.n)
+        return y.reshape(-1) if one else y
 
     def to_dense(self):
         """The full matrix (tests, small sizes)."""
@@ -349,13 +368,24 @@ class SupernodalFactor:
                 raise ValueError(f"factorisation: the reference diagonal of the ratio test has {reference.size} "
                                  f"values and the matrix {symbolic.n} equations; give one value per equation "
                                  "(a negative value where the matrix diagonal is kept) or none")
-        opts = _native.FactorOptionsC(int(options.pivoting), int(options.precision),
+        precision = options.precision
+        selected = None
+        if precision == Precision.LEARNED:
+            from .selector import PrecisionSelector
+            selector = options.selector or PrecisionSelector.trained(device)
+            if selector is None:
+                raise ValueError("factorise: Precision.LEARNED needs a selector (no committed weights found)")
+            selected = float(selector.predict(selector.features(plan, device_matrix))[0])
+            precision = Precision.FP32 if selected >= options.fp32_threshold else Precision.FP64
+        opts = _native.FactorOptionsC(int(options.pivoting), int(precision),
                                       float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
                                       _ptr(reference))
         handle = ctypes.c_void_p()
         lib = _native.lib()
         _check(lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle)))
-        return cls(handle, plan, device_matrix, options)
+        factor = cls(handle, plan, device_matrix, options)
+        factor.selected_probability = selected
+        return factor
 
     @property
     def dimension(self):

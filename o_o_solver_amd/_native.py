@@ -65,6 +65,14 @@ _SIGNATURES = {
                                           c_int32, ctypes.POINTER(SolveReportC)]),
     "oosg_matrix_multiply_device": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64]),
     "oosg_debug_mfma_probe": (c_int, [c_void_p] * 6),
+    "oosg_selector_create": (c_int, [c_int32, ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
+    "oosg_selector_destroy": (None, [c_void_p]),
+    "oosg_selector_parameters": (c_int, [c_void_p, c_void_p]),
+    "oosg_selector_set_parameters": (c_int, [c_void_p, c_void_p]),
+    "oosg_selector_features": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "oosg_selector_predict": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "oosg_selector_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "oosg_selector_train": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, ctypes.c_float, c_void_p]),
     "oosa_order_dissection": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "oosa_analyse": (c_void_p, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int]),
     "oosa_free": (None, [c_void_p]),

@@ -8,6 +8,7 @@
 // launches of a factorisation are known from the symbolic structure alone and are listed once, in the plan.
 
 #include "kernels.h"
+#include "selector.h"
 
 #include "../../include/oosg.h"
 
@@ -343,6 +344,7 @@ struct oosg_matrix
     std::vector<int64_t> offsets;
     std::vector<int32_t> rows;
     std::vector<double> diagonal; // a_jj, 0 when not stored
+    std::vector<double> hostValues; // the values as last given
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
     DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
@@ -881,6 +883,13 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
         if ((failedHere || failedBelow[s]) && p.parent[s] >= 0)
             failedBelow[p.parent[s]] = 1;
     }
+    const bool indefinite = options.pivoting == OOSG_INDEFINITE;
+    if (zeroCount > 0 && indefinite)
+        fail(OOSG_SINGULAR, "indefinite factorisation: " + std::to_string(zeroCount) +
+                                " zero (or not finite) pivots met in the static order (1 x 1 pivots, no "
+                                "interchanges), at:\n" +
+                                equationList(zeroPivots, zeroCount) +
+                                "the matrix is singular or needs pivoting; shift the pencil away from an eigenvalue");
     if (zeroCount > 0)
         fail(OOSG_SINGULAR, "the stiffness matrix is singular: " + std::to_string(zeroCount) +
                                 " zero (or not finite) pivots met in the factorisation, a mechanism or an "
@@ -888,6 +897,26 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
                                 equationList(zeroPivots, zeroCount) +
                                 "constrain these DOFs (single-point constraints) or connect them to the structure");
     oosg_factor_info& info = f.info;
+    if (indefinite)
+    {
+        // The inertia by the signs of the pivots (Sylvester: that of the matrix when no pivot broke down); the
+        // ratio test belongs to the positive definite mode (MultifrontalDriver.cpp:793-815). largest_ratio reports
+        // the largest |a_jj / d_j|, the growth a pivot close to breaking down shows.
+        info.first_issue_equation = -1;
+        info.largest_ratio_equation = -1;
+        for (int32_t j = 0; j < p.n; ++j)
+        {
+            const double pivot = f.diag[j];
+            ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
+            const double growth = std::fabs(m.diagonal[j] / pivot);
+            if (info.largest_ratio_equation < 0 || growth > info.largest_ratio)
+            {
+                info.largest_ratio = growth;
+                info.largest_ratio_equation = j;
+            }
+        }
+        return;
+    }
     const double maxRatio = options.max_ratio > 0 ? options.max_ratio : 1.0e7;
     double largest = 0.0;
     int32_t largestAt = -1;
@@ -1319,6 +1348,281 @@ oosg_status oosg_plan_profile_read(const oosg_plan* plan, double* seconds, int64
         });
 }
 
+// ---- the learned precision selector ------------------------------------------------------------------------------
+
+} // extern "C"
+
+struct oosg_selector
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer params, m, v, gradient, loss;
+    int step = 0; // Adam steps taken
+    ~oosg_selector()
+    {
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace
+{
+
+// Device work arrays of a batch: features, labels, activations, deltas.
+struct SelectorBatch
+{
+    DeviceBuffer x, y, a1, a2, p, z, d3, d2, d1;
+    SelectorBatch(int64_t count, hipStream_t stream, const float* features, const float* labels)
+    {
+        const size_t n = static_cast<size_t>(count);
+        const size_t h = static_cast<size_t>(oosg::SELECTOR_HIDDEN);
+        x.allocate(n * oosg::SELECTOR_FEATURES * sizeof(float), "selector: features");
+        check(hipMemcpyAsync(x.ptr, features, n * oosg::SELECTOR_FEATURES * sizeof(float), hipMemcpyHostToDevice,
+                             stream),
+              "selector: features");
+        if (labels != nullptr)
+        {
+            y.allocate(n * sizeof(float), "selector: labels");
+            check(hipMemcpyAsync(y.ptr, labels, n * sizeof(float), hipMemcpyHostToDevice, stream), "selector: labels");
+            a1.allocate(n * h * sizeof(float), "selector: activations");
+            a2.allocate(n * h * sizeof(float), "selector: activations");
+            d3.allocate(n * sizeof(float), "selector: deltas");
+            d2.allocate(n * h * sizeof(float), "selector: deltas");
+            d1.allocate(n * h * sizeof(float), "selector: deltas");
+        }
+        p.allocate(n * sizeof(float), "selector: probabilities");
+        z.allocate(n * sizeof(float), "selector: logits");
+    }
+};
+
+void checkBatch(const oosg_selector* s, const float* features, const float* labels, int64_t count, bool needLabels)
+{
+    if (s == nullptr || features == nullptr || count <= 0 || (needLabels && labels == nullptr))
+        fail(OOSG_INVALID_ARGUMENT, "selector: null argument or empty batch");
+    if (needLabels)
+        for (int64_t b = 0; b < count; ++b)
+            if (!(labels[b] == 0.0f || labels[b] == 1.0f))
+                fail(OOSG_INVALID_ARGUMENT, "selector: labels are 0 or 1");
+}
+
+} // namespace
+
+extern "C" {
+
+oosg_status oosg_selector_create(int32_t device, uint64_t seed, oosg_selector** out)
+{
+    return guarded(
+        [&]
+        {
+            if (out == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
+            *out = nullptr;
+            auto s = std::make_unique<oosg_selector>();
+            s->device = device;
+            check(hipSetDevice(device), "hipSetDevice");
+            check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+            const size_t bytes = oosg::SELECTOR_PARAMETERS * sizeof(float);
+            s->params.allocate(bytes, "selector: parameters");
+            s->m.allocate(bytes, "selector: moments");
+            s->v.allocate(bytes, "selector: moments");
+            s->gradient.allocate(bytes, "selector: gradient");
+            s->loss.allocate(sizeof(float), "selector: loss");
+            // Glorot-uniform weights from a splitmix64 sequence, zero biases.
+            std::vector<float> w(oosg::SELECTOR_PARAMETERS, 0.0f);
+            uint64_t state = seed;
+            const auto uniform = [&state]
+            {
+                uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                return static_cast<double>(z >> 11) * 0x1.0p-53;
+            };
+            const int F = oosg::SELECTOR_FEATURES, H = oosg::SELECTOR_HIDDEN;
+            const auto fill = [&](int offset, int count, int fanIn, int fanOut)
+            {
+                const double a = std::sqrt(6.0 / (fanIn + fanOut));
+                for (int q = 0; q < count; ++q)
+                    w[offset + q] = static_cast<float>((2.0 * uniform() - 1.0) * a);
+            };
+            fill(0, H * F, F, H);
+            fill(H * F + H, H * H, H, H);
+            fill(H * F + H + H * H + H, H, H, 1);
+            check(hipMemcpyAsync(s->params.ptr, w.data(), bytes, hipMemcpyHostToDevice, s->stream), "selector");
+            std::vector<float> zero(oosg::SELECTOR_PARAMETERS, 0.0f);
+            check(hipMemcpyAsync(s->m.ptr, zero.data(), bytes, hipMemcpyHostToDevice, s->stream), "selector");
+            check(hipMemcpyAsync(s->v.ptr, zero.data(), bytes, hipMemcpyHostToDevice, s->stream), "selector");
+            check(hipStreamSynchronize(s->stream), "selector");
+            *out = s.release();
+        });
+}
+
+void oosg_selector_destroy(oosg_selector* selector)
+{
+    if (selector == nullptr)
+        return;
+    (void)hipSetDevice(selector->device);
+    delete selector;
+}
+
+oosg_status oosg_selector_parameters(const oosg_selector* s, float* params)
+{
+    return guarded(
+        [&]
+        {
+            if (s == nullptr || params == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
+            check(hipSetDevice(s->device), "hipSetDevice");
+            check(hipMemcpyAsync(params, s->params.ptr, oosg::SELECTOR_PARAMETERS * sizeof(float),
+                                 hipMemcpyDeviceToHost, s->stream),
+                  "selector: parameters");
+            check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
+oosg_status oosg_selector_set_parameters(oosg_selector* s, const float* params)
+{
+    return guarded(
+        [&]
+        {
+            if (s == nullptr || params == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
+            check(hipSetDevice(s->device), "hipSetDevice");
+            check(hipMemcpyAsync(s->params.ptr, params, oosg::SELECTOR_PARAMETERS * sizeof(float),
+                                 hipMemcpyHostToDevice, s->stream),
+                  "selector: parameters");
+            check(hipStreamSynchronize(s->stream), "selector");
+            s->step = 0;
+        });
+}
+
+oosg_status oosg_selector_features(const oosg_plan* plan, const oosg_matrix* m, float* features)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || m == nullptr || features == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
+            if (m->plan != plan)
+                fail(OOSG_INVALID_ARGUMENT, "selector: the matrix was made for another plan");
+            const int32_t n = plan->n;
+            int64_t largestFront = 1;
+            for (int32_t s = 0; s < plan->S; ++s)
+                largestFront = std::max<int64_t>(largestFront, plan->ncol[s] + plan->nrow[s]);
+            // Off-diagonal absolute row sums of the symmetric matrix.
+            std::vector<double> off(static_cast<size_t>(n), 0.0);
+            const std::vector<double>& values = m->hostValues;
+            for (int32_t j = 0; j < n; ++j)
+                for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
+                    if (m->rows[k] != j)
+                    {
+                        off[m->rows[k]] += std::fabs(values[k]);
+                        off[j] += std::fabs(values[k]);
+                    }
+            double dmin = INFINITY, dmax = 0.0, meanDominance = 0.0, minDominance = 3.0;
+            int64_t weak = 0;
+            for (int32_t j = 0; j < n; ++j)
+            {
+                const double a = m->diagonal[j];
+                if (a > 0.0 && std::isfinite(a))
+                {
+                    dmin = std::min(dmin, a);
+                    dmax = std::max(dmax, a);
+                }
+                double dominance = off[j] > 0.0 ? std::log10(std::fabs(a) / off[j]) : 3.0;
+                dominance = std::min(3.0, std::max(-3.0, dominance != dominance ? -3.0 : dominance));
+                meanDominance += dominance;
+                minDominance = std::min(minDominance, dominance);
+                weak += std::fabs(a) < off[j] ? 1 : 0;
+            }
+            meanDominance /= std::max(n, 1);
+            const double spread = dmax > 0.0 ? std::log10(dmax / dmin) : 0.0;
+            const double norm = dmax > 0.0 ? std::log10(m->normInf / dmax) : 0.0;
+            const double raw[8] = {std::log10(std::max(n, 1)), std::log10(std::max(plan->flops, 1.0)),
+                                   std::log10(static_cast<double>(largestFront)), spread, meanDominance,
+                                   minDominance, static_cast<double>(weak) / std::max(n, 1), norm};
+            // Fixed affine maps to about unit range.
+            const double center[8] = {5.0, 10.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0};
+            const double width[8] = {2.0, 3.0, 1.0, 4.0, 1.0, 1.0, 1.0, 1.0};
+            for (int q = 0; q < 8; ++q)
+                features[q] = static_cast<float>((raw[q] - center[q]) / width[q]);
+        });
+}
+
+oosg_status oosg_selector_predict(oosg_selector* s, const float* features, int64_t count, float* probability)
+{
+    return guarded(
+        [&]
+        {
+            checkBatch(s, features, nullptr, count, false);
+            if (probability == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
+            check(hipSetDevice(s->device), "hipSetDevice");
+            SelectorBatch batch(count, s->stream, features, nullptr);
+            check(oosg::selectorForward(s->params.as<float>(), batch.x.as<float>(), count, nullptr, nullptr,
+                                        batch.p.as<float>(), nullptr, s->stream),
+                  "selector: forward");
+            check(hipMemcpyAsync(probability, batch.p.ptr, static_cast<size_t>(count) * sizeof(float),
+                                 hipMemcpyDeviceToHost, s->stream),
+                  "selector: forward");
+            check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
+oosg_status oosg_selector_gradient(oosg_selector* s, const float* features, const float* labels, int64_t count,
+                                   float* loss, float* gradient)
+{
+    return guarded(
+        [&]
+        {
+            checkBatch(s, features, labels, count, true);
+            check(hipSetDevice(s->device), "hipSetDevice");
+            SelectorBatch b(count, s->stream, features, labels);
+            check(oosg::selectorGradient(s->params.as<float>(), b.x.as<float>(), b.y.as<float>(), count,
+                                         b.a1.as<float>(), b.a2.as<float>(), b.p.as<float>(), b.z.as<float>(),
+                                         b.d3.as<float>(), b.d2.as<float>(), b.d1.as<float>(),
+                                         s->gradient.as<float>(), s->loss.as<float>(), s->stream),
+                  "selector: gradient");
+            if (gradient != nullptr)
+                check(hipMemcpyAsync(gradient, s->gradient.ptr, oosg::SELECTOR_PARAMETERS * sizeof(float),
+                                     hipMemcpyDeviceToHost, s->stream),
+                      "selector: gradient");
+            if (loss != nullptr)
+                check(hipMemcpyAsync(loss, s->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s->stream),
+                      "selector: loss");
+            check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
+oosg_status oosg_selector_train(oosg_selector* s, const float* features, const float* labels, int64_t count,
+                                int32_t steps, float learning_rate, float* loss)
+{
+    return guarded(
+        [&]
+        {
+            checkBatch(s, features, labels, count, true);
+            if (steps < 1 || !(learning_rate > 0.0f))
+                fail(OOSG_INVALID_ARGUMENT, "selector: steps >= 1 and a positive learning rate");
+            check(hipSetDevice(s->device), "hipSetDevice");
+            SelectorBatch b(count, s->stream, features, labels);
+            for (int32_t q = 0; q < steps; ++q)
+            {
+                check(oosg::selectorGradient(s->params.as<float>(), b.x.as<float>(), b.y.as<float>(), count,
+                                             b.a1.as<float>(), b.a2.as<float>(), b.p.as<float>(), b.z.as<float>(),
+                                             b.d3.as<float>(), b.d2.as<float>(), b.d1.as<float>(),
+                                             s->gradient.as<float>(), s->loss.as<float>(), s->stream),
+                      "selector: gradient");
+                check(oosg::selectorAdam(s->params.as<float>(), s->gradient.as<float>(), s->m.as<float>(),
+                                         s->v.as<float>(), learning_rate, ++s->step, s->stream),
+                      "selector: step");
+            }
+            if (loss != nullptr)
+                check(hipMemcpyAsync(loss, s->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s->stream),
+                      "selector: loss");
+            check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
 oosg_status oosg_plan_use_graphs(oosg_plan* plan, int32_t enable)
 {
     return guarded(
@@ -1469,6 +1773,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dOffsets = upload(m->offsets, st, "matrix: offsets");
             m->dRows = upload(m->rows, st, "matrix: rows");
             m->dValues = upload(values, st, "matrix: values");
+            m->hostValues = values;
             m->dPosition = upload(position, st, "matrix: positions");
             m->dColumn = upload(column, st, "matrix: columns");
             m->dScale = upload(m->scale, st, "matrix: scales");
@@ -1527,6 +1832,7 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
                                  hipMemcpyHostToDevice, st),
                   "matrix: row values");
             check(hipStreamSynchronize(st), "matrix: values");
+            m->hostValues = std::move(v);
         });
 }
 
@@ -1553,8 +1859,11 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             opts.max_ratio = 1.0e7;
             if (options != nullptr)
                 opts = *options;
-            if (opts.pivoting != OOSG_POSITIVE_DEFINITE)
-                fail(OOSG_INVALID_ARGUMENT, "factorise: the indefinite mode (Bunch-Kaufman fronts) is not built yet");
+            if (opts.pivoting != OOSG_POSITIVE_DEFINITE && opts.pivoting != OOSG_INDEFINITE)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: unknown pivoting mode");
+            if (opts.pivoting == OOSG_INDEFINITE && opts.precision != OOSG_FP64)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: indefinite factors are FP64 (their inertia decides Sturm "
+                                            "counts)");
             if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: unknown precision");
             setDevice(plan);

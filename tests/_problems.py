@@ -67,3 +67,19 @@ def random_spd(n, seed, shift=None):
     a = a @ a.T
     a += np.eye(n) * (n if shift is None else shift)
     return a
+
+
+def nested_chain_order(count):
+    """A nested-dissection order of a chain: the middle node last, the halves ordered the same way before it."""
+    order = []
+
+    def visit(lo, hi):
+        if lo >= hi:
+            return
+        mid = (lo + hi) // 2
+        visit(lo, mid)
+        visit(mid + 1, hi)
+        order.append(mid)
+
+    visit(0, count)
+    return np.array(order, np.int32)
