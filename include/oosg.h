@@ -174,7 +174,7 @@ oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, c
 
 /* Per-kernel device times, by HIP events recorded on the plan's stream around every launch while enabled
  * (enabling resets the sums). Kinds, in order: 0 small fronts (fused assembly and elimination), 1 assembly of large
- * fronts, 2 diagonal blocks, 3 panel solves (rows below a diagonal block), 4 panel updates (MFMA tiles), 5 update
+ * fronts, 2 diagonal blocks (and the whole inverses of the wide supernodes' triangles), 3 panel solves (rows below a diagonal block), 4 panel updates (MFMA tiles), 5 update
  * matrices (MFMA tiles), 6 forward sweeps, 7 backward sweeps, 8 residual products. work[k] is the algorithmic work of
  * the launches timed: flops for kinds 0-5, bytes for kinds 6-8 (the panels a sweep reads; the matrix and vectors of a
  * residual). Arrays of 9 values; any may be null. */

@@ -15,8 +15,6 @@ constexpr int PANEL_BLOCK = 64;
 constexpr int SMALL_FRONT_MAX = 96;
 // Elements of a front a workgroup of the assembly kernel holds in LDS.
 constexpr int ASSEMBLY_BLOCK_VALUES = 4096;
-// Supernodes with at least this many rows below their columns spread the products of their sweeps over workgroups.
-constexpr int BIG_ROWS = 256;
 // Supernodes with at least this many columns run the column blocks of their sweeps as steps over many workgroups.
 constexpr int WIDE_COLUMNS = 192;
 
@@ -69,6 +67,21 @@ struct UpdateTile
     int32_t m, n, k, target, lower, mode;
 };
 
+// The values of the inverses of the diagonal blocks of a supernode of c columns (blocks of PANEL_BLOCK, each kb²).
+__host__ __device__ inline int64_t blockInverseValues(int64_t c)
+{
+    const int64_t full = c / PANEL_BLOCK, rest = c % PANEL_BLOCK;
+    return full * PANEL_BLOCK * PANEL_BLOCK + rest * rest;
+}
+
+// Supernodes with at least WIDE_COLUMNS columns also keep the whole inverse X = L_FF⁻¹ of their unit lower triangle
+// (c × c, column-major, leading dimension c) in the inverses' array, after their diagonal blocks' inverses: at
+// linvOff[s] + blockInverseValues(c).
+__host__ __device__ inline bool keepsTriangleInverse(int c)
+{
+    return c >= WIDE_COLUMNS;
+}
+
 // Sets the LDS limits of the kernels that need more than the default; once per process and device, before any launch.
 hipError_t configureKernels();
 
@@ -83,6 +96,19 @@ hipError_t launchAssembly(const DevicePlan& p, const DeviceMatrix& a, T* panels,
 template <typename T>
 hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const int32_t* fronts, int frontCount, int k0,
                                 double* diagOut, int32_t* zeroAt, hipStream_t stream);
+// A step of the whole inverses X = L_FF⁻¹ of the wide supernodes' triangles (items (s, I << 16 | J)): step K >= 0
+// updates T_IJ = −Σ_{K'<=K} L_IK'·X_K'J for I > K >= J; step −1 sets X_IJ = L_II⁻¹·T_IJ (I > J) and X_II = L_II⁻¹.
+template <typename T>
+hipError_t launchTriangleInverse(const DevicePlan& p, const T* panels, T* linv, const int2* items, int count, int step,
+                                 hipStream_t stream);
+// The triangles of the wide supernodes in a sweep: items (s, I) write W(rows of block I) = Σ_{J<=I} X_IJ·X(block J)
+// (forward) or Σ_{J>=I} X_JIᵀ·X(block J) (transpose), then copyRows sets X ← W on those rows.
+template <typename TP>
+hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, double* W, int k,
+                                  const int2* items, int count, int transpose, hipStream_t stream);
+hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
+                                  hipStream_t stream);
+
 template <typename T>
 hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
                              hipStream_t stream);
@@ -94,13 +120,6 @@ hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, 
 template <typename TP>
 hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
                           int count, const int2* columnBlocks, int blockCount, hipStream_t stream);
-// One step of the sweeps of the wide supernodes of a level (items (supernode, tile or block), −1: finalise a block).
-template <typename TP>
-hipError_t launchForwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                             const int2* items, int count, int step, hipStream_t stream);
-template <typename TP>
-hipError_t launchBackwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
-                              const int2* items, int count, int step, hipStream_t stream);
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream);
 
 hipError_t launchFillInt32(int32_t* data, int64_t count, int32_t value, hipStream_t stream);

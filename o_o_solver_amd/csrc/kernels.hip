@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <utility>
 
@@ -644,6 +645,157 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
             }
 }
 
+// ---- whole inverses of the wide supernodes' triangles -----------------------------------------------------------
+//
+// X = L_FF⁻¹ (c × c, leading dimension c) by blocks of 64, right-looking: with T_IJ = −Σ_{K<I} L_IK·X_KJ (I > J), X_IJ =
+// L_II⁻¹·T_IJ. Step K (one launch) takes the items (s, I, J), I > K >= J: T_IJ = −L_IK·X_KJ (J = K, X_KK = L_KK⁻¹) or
+// T_IJ −= L_IK·X_KJ (J < K), where X_KJ = L_KK⁻¹·T_KJ is formed in the workgroup from T_KJ (final since step K − 1; no
+// item of step K writes row block K). The last launch (items (s, I, J), I >= J) sets X_IJ = L_II⁻¹·T_IJ in place, or
+// X_II = L_II⁻¹ from the diagonal block's inverse. Blocks other than the last are full (64), so only row block I can
+// be narrower (ib). Four waves of 32 × 32 over a 64 × 64 tile, 16 × 16 × 4 MFMA blocks; products of 64 terms staged
+// through LDS in chunks of 32.
+
+// acc += A·B over 64 terms for the tile: A(i, t) = a(i, t) for i < rowsA (zero beyond), B(t, j) = b(t, j) (terms beyond
+// `terms` read as zero); a, b give values at positions clamped into range.
+template <typename T, typename FA, typename FB>
+__device__ inline void tileProduct(typename Mfma<T>::Acc (&acc)[2][2], T (*As)[TILE + 4], T (*Bs)[TILE + 4], int rowsA,
+                                   int terms, FA a, FB b)
+{
+    constexpr int KC = 32;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int row = tid & (TILE - 1), kq = tid >> 6; // A: row, terms kq + 4u
+    const int bk = tid & (KC - 1), bj = tid >> 5;    // B: term bk, columns bj + 8u
+    for (int p0 = 0; p0 < TILE; p0 += KC)
+    {
+        T av[KC / 4], bv[TILE / 8];
+#pragma unroll
+        for (int u = 0; u < KC / 4; ++u)
+            av[u] = a(min(row, rowsA - 1), min(p0 + kq + 4 * u, terms - 1));
+#pragma unroll
+        for (int u = 0; u < TILE / 8; ++u)
+            bv[u] = b(min(p0 + bk, terms - 1), bj + 8 * u);
+        __syncthreads(); // the previous chunk's products have read the LDS
+#pragma unroll
+        for (int u = 0; u < KC / 4; ++u)
+            As[kq + 4 * u][row] = row < rowsA && p0 + kq + 4 * u < terms ? pinned(av[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < TILE / 8; ++u)
+            Bs[bk][bj + 8 * u] = p0 + bk < terms ? pinned(bv[u]) : T(0);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4)
+        {
+            const int ka = kk + (lane >> 4);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = Mfma<T>::step(As[ka][wm + x * 16 + (lane & 15)], Bs[ka][wn + y * 16 + (lane & 15)],
+                                              acc[x][y]);
+        }
+    }
+}
+
+template <typename T> __device__ inline void zeroTile(typename Mfma<T>::Acc (&acc)[2][2])
+{
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+            acc[x][y] = typename Mfma<T>::Acc{0, 0, 0, 0};
+}
+
+// The element (i, j) of the tile that acc[x][y][v] holds on this lane.
+__device__ inline int tileRow(int x, int v, int lane, int wave, bool fp64)
+{
+    return (wave >> 1) * 32 + x * 16 + (fp64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const T* panels, T* linv, const int2* items,
+                                                             int step)
+{
+    __shared__ T As[32][TILE + 4];
+    __shared__ T Bs[32][TILE + 4];
+    __shared__ T Ps[TILE][TILE + 4]; // the first product, as the second one's B operand: Ps[t][j]
+    const int2 item = items[blockIdx.x];
+    const int s = item.x, I = item.y >> 16, J = item.y & 0xFFFF;
+    const int c = p.ncol[s], n = c + p.nrow[s];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wn = (wave & 1) * 32;
+    T* X = linv + p.linvOff[s] + blockInverseValues(c);
+    const int i0 = TILE * I, j0 = TILE * J, ib = min(TILE, c - i0);
+    const bool fp64 = sizeof(T) == 8;
+    typename Mfma<T>::Acc acc[2][2];
+    if (step < 0)
+    {
+        // The last launch: X_II from the diagonal block's inverse, or X_IJ = L_II⁻¹·T_IJ.
+        const T* inv = linv + p.linvOff[s] + int64_t(i0) * TILE; // ib × ib, leading dimension ib
+        if (I == J)
+        {
+            for (int j = wave; j < ib; j += 4)
+                if (lane < ib)
+                    X[(i0 + lane) + int64_t(i0 + j) * c] = inv[lane + j * ib];
+            return;
+        }
+        zeroTile<T>(acc);
+        tileProduct<T>(acc, As, Bs, ib, ib, [&](int i, int t) { return inv[i + t * ib]; },
+                       [&](int t, int j) { return X[(i0 + t) + int64_t(j0 + j) * c]; });
+        __syncthreads(); // every wave has read T_IJ before any writes X_IJ over it
+    }
+    else
+    {
+        const int K = step, k0 = TILE * K;
+        const T* invK = linv + p.linvOff[s] + int64_t(k0) * TILE; // 64 × 64 (block K is full)
+        const T* L = panels + p.panelOff[s];
+        if (J < K)
+        {
+            // P = L_KK⁻¹·T_KJ into LDS.
+            zeroTile<T>(acc);
+            tileProduct<T>(acc, As, Bs, TILE, TILE, [&](int i, int t) { return invK[i + t * TILE]; },
+                           [&](int t, int j) { return X[(k0 + t) + int64_t(j0 + j) * c]; });
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        Ps[tileRow(x, v, lane, wave, fp64)][wn + y * 16 + (lane & 15)] = acc[x][y][v];
+            __syncthreads(); // P complete before any wave reads it
+            // T_IJ −= L_IK·P: the accumulators start from T_IJ, L enters negated.
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[x][y][v] = X[(i0 + min(tileRow(x, v, lane, wave, fp64), ib - 1)) +
+                                         int64_t(j0 + wn + y * 16 + (lane & 15)) * c];
+            tileProduct<T>(acc, As, Bs, ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(k0 + t) * n]; },
+                           [&](int t, int j) { return Ps[t][j]; });
+        }
+        else
+        {
+            // T_IK = −L_IK·L_KK⁻¹.
+            zeroTile<T>(acc);
+            tileProduct<T>(acc, As, Bs, ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(k0 + t) * n]; },
+                           [&](int t, int j) { return invK[t + j * TILE]; });
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int i = tileRow(x, v, lane, wave, fp64), j = wn + y * 16 + (lane & 15);
+                if (i < ib)
+                    X[(i0 + i) + int64_t(j0 + j) * c] = acc[x][y][v];
+            }
+}
+
 __global__ void mfmaProbeKernel(const double* a, const double* b, double* d, const float* af, const float* bf,
                                 float* df)
 {
@@ -670,18 +822,21 @@ __global__ void mfmaProbeKernel(const double* a, const double* b, double* d, con
 //             to all of X after the last level;
 //   backward  X(cols) −= L_RFᵀ·X(rows), then by blocks J from the last: y_J −= L(rows after J, J)ᵀ·X(those rows),
 //             X_J ← L_JJ⁻ᵀ·y_J.
-// The products of a supernode with many rows (BIG_ROWS or more) are spread over workgroups: row tiles of L_RF in the
-// forward sweep, column blocks in the backward sweep (each owns its outputs: no reduction across workgroups).
+// The products with L_RF (the rows below a supernode's columns) are spread over workgroups for every supernode: row
+// tiles of 64 in the forward sweep (after the level's triangles), column blocks of 64 in the backward sweep (before
+// them); each workgroup owns its outputs, so there is no reduction across workgroups. A workgroup per supernode (or,
+// for the wide ones, per step and row tile) does the triangle L_FF.
 
 constexpr int SB = 64;       // rows or columns of a staged panel block
 constexpr int SBP = SB + 1;  // its padded leading dimension in LDS
 constexpr int KMAX = 16;     // right-hand sides per sweep
 
-// blk[q·SBP + i] = L(row0 + i, col0 + q), zero outside [0, rows) × [0, cols) (1 <= rows, cols <= SB); a workgroup of
+// blk[q·SBP + i] = L(row0 + i, col0 + q) in the panels' precision, zero outside [0, rows) × [0, cols)
+// (1 <= rows, cols <= SB); a workgroup of
 // 256 threads, thread t moving row t mod 64 of the columns t / 64 + 4u. Every load is issued, from a position clamped
 // into the block, before any is used.
 template <typename TP>
-__device__ inline void stageBlock(double* blk, const TP* L, int64_t ld, int row0, int rows, int col0, int cols)
+__device__ inline void stageBlock(TP* blk, const TP* L, int64_t ld, int row0, int rows, int col0, int cols)
 {
     const int i = threadIdx.x & (SB - 1), q0 = threadIdx.x >> 6;
     const TP* src = L + (row0 + min(i, rows - 1)) + int64_t(col0) * ld;
@@ -693,32 +848,32 @@ __device__ inline void stageBlock(double* blk, const TP* L, int64_t ld, int row0
     for (int u = 0; u < SB / 4; ++u)
     {
         const int q = q0 + 4 * u;
-        blk[q * SBP + i] = (i < rows && q < cols) ? double(pinned(v[u])) : 0.0;
+        blk[q * SBP + i] = (i < rows && q < cols) ? pinned(v[u]) : TP(0);
     }
 }
 
-// Wave w: acc(i, h) += Σ_q blk(i, q)·v(q, h) for its 16 rows i = 16w + ..., v[q·KMAX + h].
-__device__ inline double4_t rowsTimesVectors(const double* blk, const double* v, double4_t acc)
+// Wave w: acc(i, h) += Σ_q blk(i, q)·v(q, h) for its 16 rows i = 16w + ..., v[q·KMAX + h] (in FP64).
+template <typename TP> __device__ inline double4_t rowsTimesVectors(const TP* blk, const double* v, double4_t acc)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll 4
     for (int kk = 0; kk < SB; kk += 4)
     {
-        const double a = blk[(kk + (lane >> 4)) * SBP + 16 * w + (lane & 15)];
+        const double a = double(blk[(kk + (lane >> 4)) * SBP + 16 * w + (lane & 15)]);
         const double b = v[(kk + (lane >> 4)) * KMAX + (lane & 15)];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     return acc;
 }
 
-// Wave w: acc(q, h) += Σ_i blk(i, q)·v(i, h) for its 16 columns q = 16w + ..., v[i·KMAX + h].
-__device__ inline double4_t columnsTimesVectors(const double* blk, const double* v, double4_t acc)
+// Wave w: acc(q, h) += Σ_i blk(i, q)·v(i, h) for its 16 columns q = 16w + ..., v[i·KMAX + h] (in FP64).
+template <typename TP> __device__ inline double4_t columnsTimesVectors(const TP* blk, const double* v, double4_t acc)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll 4
     for (int kk = 0; kk < SB; kk += 4)
     {
-        const double a = blk[(16 * w + (lane & 15)) * SBP + kk + (lane >> 4)];
+        const double a = double(blk[(16 * w + (lane & 15)) * SBP + kk + (lane >> 4)]);
         const double b = v[(kk + (lane >> 4)) * KMAX + (lane & 15)];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
@@ -735,21 +890,16 @@ __device__ inline int accColumn()
     return threadIdx.x & 15;
 }
 
-__device__ inline bool isBigSupernode(int r)
-{
-    return r >= BIG_ROWS;
-}
-
-// Wide supernodes run their column blocks as steps of their own (forwardWideKernel, backwardWideKernel).
+// Wide supernodes solve their triangles by the whole inverse (triangleProductsKernel).
 __device__ inline bool isWideSupernode(int c)
 {
-    return c >= WIDE_COLUMNS;
+    return keepsTriangleInverse(c);
 }
 
 // ys ← inv·ys (transpose: invᵀ·ys) for the unit lower inverse of a diagonal block (column-major qb × qb), staged in
 // blk and applied by MFMA; ys is SB × KMAX, zero beyond its rows and columns.
 template <typename TP>
-__device__ inline void applyBlockInverse(const TP* inv, int qb, double* blk, double* ys, bool transpose)
+__device__ inline void applyBlockInverse(const TP* inv, int qb, TP* blk, double* ys, bool transpose)
 {
     stageBlock(blk, inv, qb, 0, qb, 0, qb);
     __syncthreads();
@@ -800,7 +950,7 @@ template <typename TP>
 __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
                                                      double* U, int k, const int32_t* list)
 {
-    __shared__ double blk[SB * SBP];
+    __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
@@ -816,23 +966,47 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
         const int rc = p.nrow[ch];
         const int32_t* rel = p.rel + p.rowStart[ch];
         const double* uc = U + p.rowStart[ch] * k;
-        for (int idx = tid; idx < rc * KMAX; idx += nt)
+        // Eight elements per thread per pass (distinct targets: a child's rows map to distinct rows of s): the child's
+        // values and positions are loaded first (from positions clamped into its rows), then the eight targets, then
+        // the eight sums are stored.
+        constexpr int UNROLL = 8;
+        for (int base = tid; base < rc * KMAX; base += UNROLL * nt)
         {
-            const int i = idx >> 4, h = idx & (KMAX - 1);
-            if (h >= k)
-                continue;
-            const int pos = rel[i];
-            const double v = uc[int64_t(i) * k + h];
-            if (pos < c)
-                X[int64_t(first + pos) * k + h] += v;
-            else
-                u[int64_t(pos - c) * k + h] += v;
+            int pos[UNROLL];
+            double v[UNROLL];
+#pragma unroll
+            for (int t = 0; t < UNROLL; ++t)
+            {
+                const int idx = base + t * nt;
+                const int i = min(idx >> 4, rc - 1), h = min(idx & (KMAX - 1), k - 1);
+                pos[t] = rel[i];
+                v[t] = uc[int64_t(i) * k + h];
+            }
+            double* y[UNROLL];
+            double old[UNROLL];
+#pragma unroll
+            for (int t = 0; t < UNROLL; ++t)
+            {
+                const int idx = base + t * nt;
+                const int h = min(idx & (KMAX - 1), k - 1);
+                const int at = pinned(pos[t]);
+                y[t] = at < c ? X + int64_t(first + at) * k + h : u + int64_t(at - c) * k + h;
+                old[t] = *y[t];
+            }
+#pragma unroll
+            for (int t = 0; t < UNROLL; ++t)
+            {
+                const int idx = base + t * nt;
+                if (idx < rc * KMAX && (idx & (KMAX - 1)) < k)
+                    *y[t] = pinned(old[t]) + pinned(v[t]);
+            }
         }
     }
     __syncthreads();
     if (isWideSupernode(c))
-        return; // its column blocks are steps of forwardWideKernel
-    const int rowsEnd = isBigSupernode(r) ? c : n;
+        return; // its triangle is a product with the whole inverse (triangleProductsKernel)
+    // The triangle L_FF only: the rows below (L_RF) are the row tiles of forwardRowsKernel.
+    const int rowsEnd = c;
     for (int q0 = 0; q0 < c; q0 += SB)
     {
         const int qb = min(SB, c - q0);
@@ -868,7 +1042,7 @@ template <typename TP>
 __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP* panels, const double* X, double* U,
                                                          int k, const int2* tiles)
 {
-    __shared__ double blk[SB * SBP];
+    __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
     const int2 tile = tiles[blockIdx.x];
     const int s = tile.x;
@@ -897,15 +1071,14 @@ __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP*
     }
 }
 
-// X ← D⁻¹·X, X row-major n × k.
-__global__ void diagonalSolveKernel(double* X, const double* d, int64_t n, int k)
+// X ← D⁻¹·X, X row-major n × k: a row per 16 lanes, lane h its column h.
+__global__ __launch_bounds__(256) void diagonalSolveKernel(double* X, const double* d, int64_t n, int k)
 {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    {
-        const double di = d[i];
-        for (int h = 0; h < k; ++h)
-            X[i * k + h] /= di;
-    }
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    for (int64_t i = blockIdx.x * int64_t(16) + (threadIdx.x >> 4); i < n; i += int64_t(gridDim.x) * 16)
+        X[i * k + h] /= d[i];
 }
 
 // xs ← the rows rows[i0 − c + i] (or first + i0 + i below c) of X, for i < ib (1 <= ib <= SB); the row indices are
@@ -943,7 +1116,7 @@ template <typename TP>
 __global__ __launch_bounds__(256) void backwardColumnsKernel(DevicePlan p, const TP* panels, double* X, int k,
                                                              const int2* blocks)
 {
-    __shared__ double blk[SB * SBP];
+    __shared__ TP blk[SB * SBP];
     __shared__ double xs[SB * KMAX];
     const int2 block = blocks[blockIdx.x];
     const int s = block.x;
@@ -975,13 +1148,13 @@ template <typename TP>
 __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X, int k,
                                                       const int32_t* list)
 {
-    __shared__ double blk[SB * SBP];
+    __shared__ TP blk[SB * SBP];
     __shared__ double xs[SB * KMAX];
     __shared__ double ys[SB * KMAX];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     if (isWideSupernode(c))
-        return; // backwardColumnsKernel and backwardWideKernel did it
+        return; // backwardColumnsKernel and triangleProductsKernel did it
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
     const int h = accColumn();
@@ -992,7 +1165,7 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
         // Dot products of the block's columns with the final rows after it: the supernode's own columns after the
         // block, then the rows of L_RF (unless a big supernode, whose column blocks did them).
         double4_t acc = {0, 0, 0, 0};
-        const int rowsEnd = isBigSupernode(r) ? c : n;
+        const int rowsEnd = c; // the rows of L_RF are the column blocks of backwardColumnsKernel
         for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
             const int ib = min(SB, rowsEnd - i0);
@@ -1020,80 +1193,58 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
     }
 }
 
-// One step J of the forward sweep of the wide supernodes of a level. Item (s, t >= 0): rows [q0 + qb + 64t, +64) of s
-// −= L(rows, J)·z_J with z_J = L_JJ⁻¹·X_J (every item recomputes z_J: X_J is read-only during the step). Item (s, −1):
-// X_{J−1} ← L⁻¹·X_{J−1}, the block the previous step used (no item of this step reads it).
+// The triangle of a wide supernode in a sweep, by the whole inverse X = L_FF⁻¹: item (s, I) writes rows of block I
+// of W: Σ_{J<=I} X_IJ·x_J (forward) or Σ_{J>=I} X_JIᵀ·x_J (transpose), x = X(cols of s); every item reads x only.
 template <typename TP>
-__global__ __launch_bounds__(256) void forwardWideKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
-                                                         double* U, int k, const int2* items, int J)
+__global__ __launch_bounds__(256) void triangleProductsKernel(DevicePlan p, const TP* linv, const double* X, double* W,
+                                                              int k, const int2* items, int transpose)
 {
-    __shared__ double blk[SB * SBP];
+    __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
     const int2 item = items[blockIdx.x];
-    const int s = item.x;
-    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
-    const int block = item.y < 0 ? J - 1 : J;
-    const int q0 = block * SB, qb = min(SB, c - q0);
-    loadBlockRows(ys, X, first + q0, qb, k);
-    applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, false);
-    if (item.y < 0)
+    const int s = item.x, I = item.y;
+    const int c = p.ncol[s], first = p.first[s], nb = (c + SB - 1) / SB;
+    const TP* Xinv = linv + p.linvOff[s] + blockInverseValues(c);
+    const int ib = min(SB, c - SB * I);
+    double4_t acc = {0, 0, 0, 0};
+    const int jFirst = transpose ? I : 0, jEnd = transpose ? nb : I + 1;
+    for (int J = jFirst; J < jEnd; ++J)
     {
-        storeBlockRows(ys, X, first + q0, qb, k);
-        return;
+        const int jb = min(SB, c - SB * J);
+        loadBlockRows(ys, X, first + SB * J, jb, k);
+        if (transpose)
+            stageBlock(blk, Xinv, c, SB * J, jb, SB * I, ib);
+        else
+            stageBlock(blk, Xinv, c, SB * I, ib, SB * J, jb);
+        __syncthreads();
+        acc = transpose ? columnsTimesVectors(blk, ys, acc) : rowsTimesVectors(blk, ys, acc);
+        __syncthreads();
     }
-    const int i0 = q0 + qb + item.y * SB;
-    const int rows = min(SB, n - i0);
-    const TP* L = panels + p.panelOff[s];
-    stageBlock(blk, L, n, i0, rows, q0, qb);
-    __syncthreads();
-    const double4_t acc = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
-    double* u = U + p.rowStart[s] * k;
     const int h = accColumn();
 #pragma unroll
     for (int v = 0; v < 4; ++v)
     {
-        const int i = i0 + accRow(v);
-        if (i < n && h < k)
-        {
-            double* y = i < c ? X + int64_t(first + i) * k : u + int64_t(i - c) * k;
-            y[h] -= acc[v];
-        }
+        const int i = accRow(v);
+        if (i < ib && h < k)
+            W[int64_t(first + SB * I + i) * k + h] = acc[v];
     }
 }
 
-// One step J of the backward sweep of the wide supernodes of a level (J decreasing). Item (s, I >= 0), I < J:
-// X_I −= L(rows of block J, cols of block I)ᵀ·x_J with x_J = L_JJ⁻ᵀ·X_J (recomputed by every item). Item (s, −1):
-// X_{J+1} ← L⁻ᵀ·X_{J+1}, the block the previous step used.
-template <typename TP>
-__global__ __launch_bounds__(256) void backwardWideKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
-                                                          int k, const int2* items, int J)
+// X ← W on the rows of the items' blocks.
+__global__ __launch_bounds__(256) void copyTriangleRowsKernel(DevicePlan p, const double* W, double* X, int k,
+                                                              const int2* items)
 {
-    __shared__ double blk[SB * SBP];
-    __shared__ double ys[SB * KMAX];
     const int2 item = items[blockIdx.x];
-    const int s = item.x;
-    const int c = p.ncol[s], n = c + p.nrow[s], first = p.first[s];
-    const int block = item.y < 0 ? J + 1 : J;
-    const int q0 = block * SB, qb = min(SB, c - q0);
-    loadBlockRows(ys, X, first + q0, qb, k);
-    applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, true);
-    if (item.y < 0)
+    const int c = p.ncol[item.x], first = p.first[item.x];
+    const int ib = min(SB, c - SB * item.y);
+    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
     {
-        storeBlockRows(ys, X, first + q0, qb, k);
-        return;
-    }
-    const int c0 = item.y * SB, cb = min(SB, c - c0);
-    const TP* L = panels + p.panelOff[s];
-    stageBlock(blk, L, n, q0, qb, c0, cb);
-    __syncthreads();
-    const double4_t acc = columnsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
-    const int h = accColumn();
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-    {
-        const int q = accRow(v);
-        if (q < cb && h < k)
-            X[int64_t(first + c0 + q) * k + h] -= acc[v];
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        if (i < ib && h < k)
+        {
+            const int64_t at = int64_t(first + SB * item.y + i) * k + h;
+            X[at] = W[at];
+        }
     }
 }
 
@@ -1204,6 +1355,7 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
          i += int64_t(gridDim.x) * VECTOR_ROWS)
     {
         double sum = 0.0;
+#pragma unroll 4
         for (int64_t e = offsets[i]; e < offsets[i + 1]; ++e)
             sum += values[e] * x[int64_t(cols[e]) * k + h];
         out[i * k + h] = b ? b[i * k + h] - sum : sum;
@@ -1232,32 +1384,52 @@ __global__ __launch_bounds__(256) void columnDotsKernel(const double* a, const d
     }
 }
 
-// out[h] = Σ over the workgroups' partial sums, in workgroup order.
-__global__ void sumPartialsKernel(const double* partial, int groups, double* out)
+// out[h] = Σ over the workgroups' partial sums: 16 threads per column sum runs of `groups / 16` workgroups, then the
+// 16 run sums are added in order (a fixed order: deterministic). groups is a multiple of 16.
+__global__ __launch_bounds__(256) void sumPartialsKernel(const double* partial, int groups, double* out)
 {
-    const int h = threadIdx.x;
-    if (h >= 16)
-        return;
+    __shared__ double runs[256];
+    const int h = threadIdx.x & 15, run = threadIdx.x >> 4, length = groups >> 4;
     double total = 0.0;
-    for (int g = 0; g < groups; ++g)
+#pragma unroll 8
+    for (int g = run * length; g < (run + 1) * length; ++g)
         total += partial[g * 16 + h];
-    out[h] = total;
+    runs[threadIdx.x] = total;
+    __syncthreads();
+    if (threadIdx.x < 16)
+    {
+        double sum = 0.0;
+        for (int r = 0; r < 16; ++r)
+            sum += runs[r * 16 + threadIdx.x];
+        out[threadIdx.x] = sum;
+    }
 }
 
-// Per column the largest |v| (bits of a non-negative double, ordered as the values; NaN wins).
+// Per column the largest |v| (bits of a non-negative double, ordered as the values; NaN wins): the workgroup's 16
+// row slots combined in LDS, then one atomic per workgroup and column.
 __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int64_t n, int k, unsigned long long* out)
 {
+    __shared__ double slots[256];
     const int h = threadIdx.x & 15;
-    if (h >= k)
-        return;
     double m = 0.0;
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    if (h < k)
+        for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+             i += int64_t(gridDim.x) * VECTOR_ROWS)
+        {
+            const double a = fabs(v[i * k + h]);
+            m = a > m || a != a ? a : m;
+        }
+    slots[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x < k)
     {
-        const double a = fabs(v[i * k + h]);
-        m = a > m || a != a ? a : m;
+        for (int r = 1; r < 16; ++r)
+        {
+            const double a = slots[r * 16 + threadIdx.x];
+            m = a > m || a != a ? a : m;
+        }
+        atomicMax(&out[threadIdx.x], static_cast<unsigned long long>(__double_as_longlong(m)));
     }
-    atomicMax(&out[h], static_cast<unsigned long long>(__double_as_longlong(m)));
 }
 
 // x += α_h·p, r −= α_h·q.
@@ -1388,29 +1560,37 @@ hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv,
     return hipGetLastError();
 }
 
-template <typename TP>
-hipError_t launchForwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                             const int2* items, int count, int step, hipStream_t stream)
+template <typename T>
+hipError_t launchTriangleInverse(const DevicePlan& p, const T* panels, T* linv, const int2* items, int count, int step,
+                                 hipStream_t stream)
 {
     if (count > 0)
-        hipLaunchKernelGGL(forwardWideKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, items,
-                           step);
+        hipLaunchKernelGGL(triangleInverseKernel<T>, dim3(count), dim3(256), 0, stream, p, panels, linv, items, step);
     return hipGetLastError();
 }
 
 template <typename TP>
-hipError_t launchBackwardWide(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
-                              const int2* items, int count, int step, hipStream_t stream)
+hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, double* W, int k,
+                                  const int2* items, int count, int transpose, hipStream_t stream)
 {
     if (count > 0)
-        hipLaunchKernelGGL(backwardWideKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, k, items,
-                           step);
+        hipLaunchKernelGGL(triangleProductsKernel<TP>, dim3(count), dim3(256), 0, stream, p, linv, X, W, k, items,
+                           transpose);
+    return hipGetLastError();
+}
+
+hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
+                                  hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(copyTriangleRowsKernel, dim3(count), dim3(256), 0, stream, p, W, X, k, items);
     return hipGetLastError();
 }
 
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(diagonalSolveKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, X, d, n, k);
+    hipLaunchKernelGGL(diagonalSolveKernel, dim3(std::max<int64_t>(1, std::min<int64_t>((n + 15) / 16, 8192))),
+                       dim3(256), 0, stream, X, d, n, k);
     return hipGetLastError();
 }
 
@@ -1499,13 +1679,15 @@ hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, 
                             hipStream_t stream)
 {
     hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
-    hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(64), 0, stream, partial, DOT_GROUPS, out);
+    static_assert(DOT_GROUPS % 16 == 0, "runs of the partial sums");
+    hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(256), 0, stream, partial, DOT_GROUPS, out);
     return hipGetLastError();
 }
 
 hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream)
 {
-    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, v, n, k, out);
+    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(std::min(vectorGroups(n), 1024)), dim3(256), 0, stream, v, n, k,
+                       out);
     return hipGetLastError();
 }
 
@@ -1548,10 +1730,9 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
                                          int, const int2*, int, hipStream_t);                                          \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int2*, int, hipStream_t);                                              \
-    template hipError_t launchForwardWide<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \
-                                             const int2*, int, int, hipStream_t);                                      \
-    template hipError_t launchBackwardWide<T>(const DevicePlan&, const T*, const T*, double*, int, const int2*, int,   \
-                                              int, hipStream_t);                                                       \
+    template hipError_t launchTriangleInverse<T>(const DevicePlan&, const T*, T*, const int2*, int, int, hipStream_t); \
+    template hipError_t launchTriangleProducts<T>(const DevicePlan&, const T*, const double*, double*, int,            \
+                                                  const int2*, int, int, hipStream_t);                                 \
     template size_t smallFrontBytes<T>(int);
 
 OOSG_INSTANTIATE(double)

@@ -124,7 +124,8 @@ enum class LaunchKind
     Small,
     Assemble,
     Diagonal,
-    Tiles
+    Tiles,
+    Inverse // a round of the wide supernodes' triangle inverses
 };
 
 // The kernels as the profile counts them.
@@ -281,15 +282,19 @@ struct oosg_plan
     // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward).
     std::vector<int2> forwardTiles, backwardBlocks;
     std::vector<int64_t> forwardTileStart, backwardBlockStart;
-    // The steps of the wide supernodes' sweeps: per level, (step, offset, count) into wideItems.
-    struct WideStep
+    // The wide supernodes' triangle blocks (s, I), by level (the sweeps' products with the whole inverse), and per
+    // level the rounds of the factorisation that build the inverses.
+    std::vector<int2> triangleItems;
+    std::vector<int64_t> triangleStart;
+    struct InverseRound
     {
-        int step;
+        int step; // −1: the last launch
         int64_t offset;
         int count;
+        double flops;
     };
-    std::vector<int2> wideItems;
-    std::vector<std::vector<WideStep>> wideForward, wideBackward;
+    std::vector<int2> inverseItems;
+    std::vector<std::vector<InverseRound>> inverseRounds;
     int32_t levels = 0;
     int64_t smallFronts = 0, largeFronts = 0;
     double flops = 0.0;
@@ -303,7 +308,7 @@ struct oosg_plan
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dLevelList, dForwardTiles, dBackwardBlocks,
-        dWideItems;
+        dTriangleItems, dInverseItems;
     oosg::DevicePlan dp{};
 
     DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
@@ -312,6 +317,7 @@ struct oosg_plan
     DeviceBuffer diagCache;     // and their pivots
     DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
+    DeviceBuffer solveW; // the triangle products' results (rows of the wide supernodes, n × k row-major)
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
     DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
     uint64_t budget = 0;
@@ -475,9 +481,8 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
         const int64_t c = p.ncol[k], r = p.nrow[k];
         p.panelOff[k + 1] = p.panelOff[k] + c * (c + r);
         p.updateOff[k + 1] = p.updateOff[k] + r * r;
-        int64_t inverses = 0;
-        for (int64_t q0 = 0; q0 < c; q0 += oosg::PANEL_BLOCK)
-            inverses += std::min<int64_t>(oosg::PANEL_BLOCK, c - q0) * std::min<int64_t>(oosg::PANEL_BLOCK, c - q0);
+        // The diagonal blocks' inverses, then (wide supernodes) the whole inverse of the triangle.
+        const int64_t inverses = oosg::blockInverseValues(c) + (oosg::keepsTriangleInverse(static_cast<int>(c)) ? c * c : 0);
         p.linvOff[k + 1] = p.linvOff[k] + inverses;
         for (int64_t m = r; m < r + c; ++m)
             p.flops += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
@@ -505,65 +510,60 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
     // The sweeps' row tiles and column blocks of the big supernodes, level by level.
     p.forwardTileStart.assign(p.levels + 1, 0);
     p.backwardBlockStart.assign(p.levels + 1, 0);
+    p.triangleStart.assign(p.levels + 1, 0);
     for (int32_t l = 0; l < p.levels; ++l)
     {
         int maxBlocks = 0;
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
             const int32_t s = p.levelList[q];
-            const bool wide = p.ncol[s] >= oosg::WIDE_COLUMNS, big = p.nrow[s] >= oosg::BIG_ROWS;
+            const bool wide = p.ncol[s] >= oosg::WIDE_COLUMNS;
             if (wide)
                 maxBlocks = std::max(maxBlocks, (p.ncol[s] + 63) / 64);
-            if (big && !wide)
-                for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
-                    p.forwardTiles.push_back(make_int2(s, t));
-            if ((big || wide) && p.nrow[s] > 0)
+            // The L_RF products of every supernode with rows below: row tiles (forward), column blocks (backward).
+            for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
+                p.forwardTiles.push_back(make_int2(s, t));
+            if (p.nrow[s] > 0)
                 for (int32_t b = 0; b * 64 < p.ncol[s]; ++b)
                     p.backwardBlocks.push_back(make_int2(s, b));
         }
         p.forwardTileStart[l + 1] = static_cast<int64_t>(p.forwardTiles.size());
         p.backwardBlockStart[l + 1] = static_cast<int64_t>(p.backwardBlocks.size());
-        // Wide supernodes: forward steps J = 0 .. maxBlocks (the last only finalises), backward J = maxBlocks − 1 .. −1.
-        std::vector<oosg_plan::WideStep> forward, backward;
-        for (int J = 0; maxBlocks > 0 && J <= maxBlocks; ++J)
+        // Wide supernodes: their triangle's blocks (s, I) for the sweeps' products with the whole inverse, and the
+        // rounds d of the factorisation that build that inverse (items (s, I), I >= d, in round d).
+        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
-            const int64_t offset = static_cast<int64_t>(p.wideItems.size());
-            for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
-            {
-                const int32_t s = p.levelList[q];
-                const int c = p.ncol[s], n = c + p.nrow[s], blocks = (c + 63) / 64;
-                if (c < oosg::WIDE_COLUMNS)
-                    continue;
-                if (J >= 1 && J <= blocks)
-                    p.wideItems.push_back(make_int2(s, -1));
-                if (J < blocks)
-                {
-                    const int below = std::min(c, (J + 1) * 64);
-                    for (int t = 0; below + t * 64 < n; ++t)
-                        p.wideItems.push_back(make_int2(s, t));
-                }
-            }
-            forward.push_back({J, offset, static_cast<int>(p.wideItems.size() - offset)});
+            const int32_t s = p.levelList[q];
+            if (!oosg::keepsTriangleInverse(p.ncol[s]))
+                continue;
+            for (int32_t I = 0; I * 64 < p.ncol[s]; ++I)
+                p.triangleItems.push_back(make_int2(s, I));
         }
-        for (int J = maxBlocks - 1; maxBlocks > 0 && J >= -1; --J)
+        p.triangleStart[l + 1] = static_cast<int64_t>(p.triangleItems.size());
+        std::vector<oosg_plan::InverseRound> rounds;
+        // Steps K = 0 .. maxBlocks − 2 (items (s, I, J), I > K >= J), then the last launch (items I >= J, step −1).
+        for (int K = 0; K <= maxBlocks - 1; ++K)
         {
-            const int64_t offset = static_cast<int64_t>(p.wideItems.size());
+            const bool last = K == maxBlocks - 1;
+            oosg_plan::InverseRound round{last ? -1 : K, static_cast<int64_t>(p.inverseItems.size()), 0, 0.0};
             for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
             {
                 const int32_t s = p.levelList[q];
                 const int blocks = (p.ncol[s] + 63) / 64;
-                if (p.ncol[s] < oosg::WIDE_COLUMNS)
+                if (!oosg::keepsTriangleInverse(p.ncol[s]))
                     continue;
-                if (J + 1 >= 0 && J + 1 < blocks)
-                    p.wideItems.push_back(make_int2(s, -1));
-                if (J >= 0 && J < blocks)
-                    for (int I = 0; I < J; ++I)
-                        p.wideItems.push_back(make_int2(s, I));
+                for (int I = 0; I < blocks; ++I)
+                    for (int J = 0; J <= I; ++J)
+                        if (last ? true : (I > K && J <= K))
+                        {
+                            p.inverseItems.push_back(make_int2(s, (I << 16) | J));
+                            round.flops += 2.0 * 64.0 * 64.0 * 64.0 * (last || J == K ? 1.0 : 2.0);
+                        }
             }
-            backward.push_back({J, offset, static_cast<int>(p.wideItems.size() - offset)});
+            round.count = static_cast<int>(p.inverseItems.size() - round.offset);
+            rounds.push_back(round);
         }
-        p.wideForward.push_back(std::move(forward));
-        p.wideBackward.push_back(std::move(backward));
+        p.inverseRounds.push_back(std::move(rounds));
     }
 }
 
@@ -766,6 +766,17 @@ void buildSchedule(oosg_plan& p)
                 tilesLaunch.flops += tileFlops(p.tiles[q]);
             p.schedule.push_back(tilesLaunch);
         }
+        // The whole inverses of the level's wide triangles (their panels are final), one launch per round; timed
+        // with the diagonal blocks.
+        for (const oosg_plan::InverseRound& round : p.inverseRounds[l])
+        {
+            Launch inverse{LaunchKind::Inverse, PROFILE_DIAGONAL};
+            inverse.offset = round.offset;
+            inverse.count = round.count;
+            inverse.k0 = round.step;
+            inverse.flops = round.flops;
+            p.schedule.push_back(inverse);
+        }
     }
 }
 
@@ -785,7 +796,8 @@ void uploadPlan(oosg_plan& p)
     p.dLinvOff = upload(p.linvOff, st, "plan: inverse offsets");
     p.dForwardTiles = upload(p.forwardTiles, st, "plan: forward row tiles");
     p.dBackwardBlocks = upload(p.backwardBlocks, st, "plan: backward column blocks");
-    p.dWideItems = upload(p.wideItems, st, "plan: wide supernode steps");
+    p.dTriangleItems = upload(p.triangleItems, st, "plan: triangle blocks");
+    p.dInverseItems = upload(p.inverseItems, st, "plan: triangle inverse blocks");
     p.dSmallList = upload(p.smallList, st, "plan: small fronts");
     p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
     p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
@@ -821,6 +833,7 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     const int32_t* stepFronts = p.dStepFronts.as<int32_t>();
     const oosg::AssemblyBlock* blocks = p.dBlocks.as<oosg::AssemblyBlock>();
     const oosg::AssemblyItem* items = p.dItems.as<oosg::AssemblyItem>();
+    const int2* inverseItems = p.dInverseItems.as<int2>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
     for (const Launch& l : p.schedule)
     {
@@ -844,6 +857,10 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
         case LaunchKind::Tiles:
             check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
                   "factor: update tiles");
+            break;
+        case LaunchKind::Inverse:
+            check(oosg::launchTriangleInverse<T>(p.dp, panels, linv, inverseItems + l.offset, l.count, l.k0, st),
+                  "factor: triangle inverses");
             break;
         }
         p.profiler.end(slot, st);
@@ -976,32 +993,42 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     const int32_t* levels = p.dLevelList.as<int32_t>();
     const int2* forwardTiles = p.dForwardTiles.as<int2>();
     const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
-    const int2* wide = p.dWideItems.as<int2>();
+    const int2* triangle = p.dTriangleItems.as<int2>();
+    double* W = p.solveW.as<double>();
     // The profile counts bytes as the work of the sweeps (they are bound by HBM): the level's panels, read once.
     for (int32_t l = 0; l < p.levels; ++l)
     {
         const int slot = p.profiler.begin(PROFILE_FORWARD, p.levelPanelValues[l] * sizeof(TP), st);
+        // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses, into W, then
+        // back into X), then every L_RF row tile of the level.
         check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
-                                      p.levelStart[l + 1] - p.levelStart[l], forwardTiles + p.forwardTileStart[l],
-                                      static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]), st),
+                                      p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, st),
               "solve: forward sweep");
-        for (const oosg_plan::WideStep& w : p.wideForward[l])
-            check(oosg::launchForwardWide<TP>(p.dp, panels, linv, X, U, k, wide + w.offset, w.count, w.step, st),
-                  "solve: forward sweep (wide)");
+        const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
+        check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles, 0, st),
+              "solve: forward sweep (triangles)");
+        check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+              "solve: forward sweep (triangles)");
+        check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, nullptr, 0, forwardTiles + p.forwardTileStart[l],
+                                      static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]), st),
+              "solve: forward sweep (rows)");
         p.profiler.end(slot, st);
     }
     check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
     for (int32_t l = p.levels - 1; l >= 0; --l)
     {
         const int slot = p.profiler.begin(PROFILE_BACKWARD, p.levelPanelValues[l] * sizeof(TP), st);
-        // The L_RFᵀ products of the big and wide supernodes first (column blocks), then the wide steps, then the rest.
+        // The L_RFᵀ products of every supernode first (column blocks), then the wide supernodes' triangles (with
+        // their whole inverses), then the other triangles.
         const int blocks = static_cast<int>(p.backwardBlockStart[l + 1] - p.backwardBlockStart[l]);
         check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0, backwardBlocks + p.backwardBlockStart[l],
                                        blocks, st),
               "solve: backward sweep");
-        for (const oosg_plan::WideStep& w : p.wideBackward[l])
-            check(oosg::launchBackwardWide<TP>(p.dp, panels, linv, X, k, wide + w.offset, w.count, w.step, st),
-                  "solve: backward sweep (wide)");
+        const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
+        check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles, 1, st),
+              "solve: backward sweep (triangles)");
+        check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+              "solve: backward sweep (triangles)");
         check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
                                        p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, st),
               "solve: backward sweep");
@@ -1026,7 +1053,7 @@ void sweeps(const oosg_factor& f, int k)
         return;
     }
     p.sweepGraph.run(p.stream,
-                     {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr,
+                     {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr, p.solveW.ptr,
                       reinterpret_cast<const void*>(static_cast<intptr_t>(k * 2 + f.precision))},
                      body);
 }
@@ -1037,8 +1064,11 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     oosg_plan& p = *f.plan;
     const int64_t n = p.n;
     const int64_t width = std::min<int64_t>(nrhs, 16);
-    ensure(p.solveX, static_cast<size_t>(n * width) * sizeof(double), "solve: right-hand sides");
-    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * width) * sizeof(double),
+    (void)width;
+    // Sized for 16 columns whatever the call: the buffers (and the graphs replaying the sweeps on them) stay put.
+    ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
+    ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
+    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
            "solve: update vectors");
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
     for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
@@ -1108,6 +1138,7 @@ void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k)
     oosg_plan& p = *f.plan;
     const int64_t n = p.n;
     ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
+    ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
     ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
            "solve: update vectors");
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
@@ -1267,7 +1298,8 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                                         p->smallList.size() + p->stepFronts.size() + p->levelList.size()) *
                                            4 +
                                        p->panelOff.size() * 24 +
-                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->wideItems.size()) * 8 +
+                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->triangleItems.size() +
+                                        p->inverseItems.size()) * 8 +
                                        p->blocks.size() * sizeof(oosg::AssemblyBlock) +
                                        p->items.size() * sizeof(oosg::AssemblyItem) +
                                        p->tiles.size() * sizeof(oosg::UpdateTile);
