@@ -26,6 +26,17 @@ METRIC = "factor+solve s at 1e-10 backward err: mixed FP32+IR vs FP64 GPU; % MFM
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
+# The HIP kernel behind each profile kind (its name in a rocprofv3 trace; the three tile kinds share one kernel).
+KIND_KERNELS = {"small_fronts": "smallFrontKernel", "assembly": "assembleKernel",
+                "diagonal_blocks": "diagonalBlockKernel", "panel_solve": "updateTilesKernel",
+                "panel_update_tiles": "updateTilesKernel", "update_matrix_tiles": "updateTilesKernel",
+                "forward_triangles": "forwardKernel", "backward_triangles": "backwardKernel",
+                "residual": "multiplyRowsKernel", "forward_rows": "forwardRowsKernel",
+                "backward_columns": "backwardColumnsKernel", "triangle_products": "triangleProductsKernel",
+                "triangle_inverses": "triangleInverseKernel"}
+# Profile kinds whose work is bytes (the sweeps and the residual products); the others count flops.
+BYTE_KINDS = ("forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
+              "triangle_products")
 
 
 def log(message):
@@ -165,7 +176,7 @@ def main():
     prof = mixed_prof
     dominant = max(prof, key=lambda k: prof[k][0])
     seconds, launches, work = prof[dominant]  # of the one profiled step
-    if dominant in ("forward_sweep", "backward_sweep", "residual"):
+    if dominant in BYTE_KINDS:
         achieved = work / seconds / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM, 4), "traffic": None}
@@ -174,14 +185,13 @@ def main():
         achieved = work / seconds / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                     "frac": round(achieved / peak, 4), "traffic": None}
-    roofline.update({"kernel": dominant, "launches_per_step": launches,
+    roofline.update({"kernel": dominant, "hip_kernel": KIND_KERNELS[dominant], "launches_per_step": launches,
                      "avg_launch_ms": round(seconds / launches * 1e3, 4),
                      "work_per_launch": work / launches,
                      "share_of_device_time": round(seconds / sum(v[0] for v in prof.values()), 3)})
     def breakdown(profile):
         return {k: {"ms_per_step": round(v[0] * 1e3, 3), "launches_per_step": v[1],
-                    "rate": round(v[2] / v[0] / (1e9 if k in ("forward_sweep", "backward_sweep", "residual")
-                                                 else 1e12), 2) if v[0] > 0 else None}
+                    "rate": round(v[2] / v[0] / (1e9 if k in BYTE_KINDS else 1e12), 2) if v[0] > 0 else None}
                 for k, v in profile.items()}
     kernels = breakdown(prof)
 

@@ -173,11 +173,15 @@ oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, c
                                   float *df);
 
 /* Per-kernel device times, by HIP events recorded on the plan's stream around every launch while enabled
- * (enabling resets the sums). Kinds, in order: 0 small fronts (fused assembly and elimination), 1 assembly of large
- * fronts, 2 diagonal blocks (and the whole inverses of the wide supernodes' triangles), 3 panel solves (rows below a diagonal block), 4 panel updates (MFMA tiles), 5 update
- * matrices (MFMA tiles), 6 forward sweeps, 7 backward sweeps, 8 residual products. work[k] is the algorithmic work of
- * the launches timed: flops for kinds 0-5, bytes for kinds 6-8 (the panels a sweep reads; the matrix and vectors of a
- * residual). Arrays of 9 values; any may be null. */
+ * (enabling resets the sums; launches are eager while enabled). OOSG_PROFILE_KINDS kinds, in order: 0 small fronts
+ * (fused assembly and elimination), 1 assembly of large fronts, 2 diagonal blocks, 3 panel solves (rows below a
+ * diagonal block), 4 panel updates (MFMA tiles), 5 update matrices (MFMA tiles), 6 forward sweep gathers and
+ * triangles, 7 backward sweep triangles, 8 residual products, 9 forward sweep row tiles (L_RF), 10 backward sweep
+ * column blocks (L_RFᵀ), 11 products with the wide triangles' whole inverses (both sweeps), 12 the whole inverses
+ * (factorisation). work[k] is the algorithmic work of the launches timed: flops for kinds 0-5 and 12; bytes for
+ * kinds 6-11 (the values of L or of an inverse read once, and the right-hand-side rows read or written once; the
+ * matrix and the vectors of a residual). Arrays of OOSG_PROFILE_KINDS values; any may be null. */
+#define OOSG_PROFILE_KINDS 13
 oosg_status oosg_plan_profile(oosg_plan *plan, int32_t enable);
 oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64_t *launches, double *work);
 

@@ -266,7 +266,8 @@ class Plan:
         return dict(zip(keys, s.tolist()))
 
     PROFILE_KINDS = ["small_fronts", "assembly", "diagonal_blocks", "panel_solve", "panel_update_tiles",
-                     "update_matrix_tiles", "forward_sweep", "backward_sweep", "residual"]
+                     "update_matrix_tiles", "forward_triangles", "backward_triangles", "residual", "forward_rows",
+                     "backward_columns", "triangle_products", "triangle_inverses"]
 
     def profile(self, enable=True):
         """Per-kernel device times by HIP events on the plan's stream (enabling resets the sums)."""
@@ -275,9 +276,10 @@ class Plan:
     def profile_read(self):
         """{kind: (seconds, launches, work)}: work in flops for the factor kernels, in bytes for the sweeps and the
         residuals."""
-        seconds = np.zeros(9)
-        launches = np.zeros(9, np.int64)
-        work = np.zeros(9)
+        kinds = len(self.PROFILE_KINDS)
+        seconds = np.zeros(kinds)
+        launches = np.zeros(kinds, np.int64)
+        work = np.zeros(kinds)
         _check(self._lib.oosg_plan_profile_read(self.handle, _ptr(seconds), _ptr(launches), _ptr(work)))
         return {k: (float(seconds[i]), int(launches[i]), float(work[i])) for i, k in enumerate(self.PROFILE_KINDS)}
 

@@ -831,25 +831,42 @@ constexpr int SB = 64;       // rows or columns of a staged panel block
 constexpr int SBP = SB + 1;  // its padded leading dimension in LDS
 constexpr int KMAX = 16;     // right-hand sides per sweep
 
-// blk[q·SBP + i] = L(row0 + i, col0 + q) in the panels' precision, zero outside [0, rows) × [0, cols)
-// (1 <= rows, cols <= SB); a workgroup of
-// 256 threads, thread t moving row t mod 64 of the columns t / 64 + 4u. Every load is issued, from a position clamped
-// into the block, before any is used.
+// A block blk[q·SBP + i] = L(row0 + i, col0 + q) in the panels' precision, zero outside [0, rows) × [0, cols)
+// (1 <= rows, cols <= SB), staged through registers: load() issues the global loads (thread t: row t mod 64 of the
+// columns t / 64 + 4u, positions clamped into the block), store() writes LDS; a workgroup of 256 threads. Between
+// the two, the loads are in flight while the workgroup computes.
+template <typename TP> struct StagedBlock
+{
+    TP v[SB / 4];
+    int rows, cols;
+    __device__ inline void load(const TP* L, int64_t ld, int row0, int rows_, int col0, int cols_)
+    {
+        rows = rows_;
+        cols = cols_;
+        const int i = threadIdx.x & (SB - 1), q0 = threadIdx.x >> 6;
+        const TP* src = L + (row0 + min(i, rows - 1)) + int64_t(col0) * ld;
+#pragma unroll
+        for (int u = 0; u < SB / 4; ++u)
+            v[u] = src[int64_t(min(q0 + 4 * u, cols - 1)) * ld];
+    }
+    __device__ inline void store(TP* blk) const
+    {
+        const int i = threadIdx.x & (SB - 1), q0 = threadIdx.x >> 6;
+#pragma unroll
+        for (int u = 0; u < SB / 4; ++u)
+        {
+            const int q = q0 + 4 * u;
+            blk[q * SBP + i] = (i < rows && q < cols) ? pinned(v[u]) : TP(0);
+        }
+    }
+};
+
 template <typename TP>
 __device__ inline void stageBlock(TP* blk, const TP* L, int64_t ld, int row0, int rows, int col0, int cols)
 {
-    const int i = threadIdx.x & (SB - 1), q0 = threadIdx.x >> 6;
-    const TP* src = L + (row0 + min(i, rows - 1)) + int64_t(col0) * ld;
-    TP v[SB / 4];
-#pragma unroll
-    for (int u = 0; u < SB / 4; ++u)
-        v[u] = src[int64_t(min(q0 + 4 * u, cols - 1)) * ld];
-#pragma unroll
-    for (int u = 0; u < SB / 4; ++u)
-    {
-        const int q = q0 + 4 * u;
-        blk[q * SBP + i] = (i < rows && q < cols) ? pinned(v[u]) : TP(0);
-    }
+    StagedBlock<TP> b;
+    b.load(L, ld, row0, rows, col0, cols);
+    b.store(blk);
 }
 
 // Wave w: acc(i, h) += Σ_q blk(i, q)·v(q, h) for its 16 rows i = 16w + ..., v[q·KMAX + h] (in FP64).
@@ -913,26 +930,61 @@ __device__ inline void applyBlockInverse(const TP* inv, int qb, TP* blk, double*
     __syncthreads();
 }
 
-// ys ← the rows row0 .. row0 + qb of X (k columns), zero padded to SB × KMAX (1 <= qb <= SB); loads first, from
-// positions clamped into the rows.
+// SB rows of k right-hand-side values (X row-major, n × k) for ys (SB × KMAX, zero padded), staged through registers:
+// the rows row0 .. row0 + qb (load), or rows gathered by the row list of a supernode (gather: rows[i0 − c + i], or
+// first + i0 + i below c); positions clamped into the rows (1 <= qb <= SB).
+struct StagedRows
+{
+    static constexpr int PER = SB * KMAX / 256;
+    double v[PER];
+    int count, k;
+    __device__ inline void load(const double* X, int64_t row0, int qb, int k_)
+    {
+        count = qb;
+        k = k_;
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int idx = threadIdx.x + 256 * u;
+            v[u] = X[(row0 + min(idx >> 4, qb - 1)) * k + min(idx & (KMAX - 1), k - 1)];
+        }
+    }
+    __device__ inline void gather(const double* X, const int32_t* rows, int first, int c, int i0, int ib, int k_)
+    {
+        count = ib;
+        k = k_;
+        int64_t row[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int idx = threadIdx.x + 256 * u;
+            const int at = i0 + min(idx >> 4, ib - 1);
+            row[u] = at < c ? int64_t(first + at) : int64_t(rows[max(at - c, 0)]);
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int idx = threadIdx.x + 256 * u;
+            v[u] = X[row[u] * k + min(idx & (KMAX - 1), k - 1)];
+        }
+    }
+    __device__ inline void store(double* ys) const
+    {
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int idx = threadIdx.x + 256 * u;
+            const int i = idx >> 4, h = idx & (KMAX - 1);
+            ys[idx] = (i < count && h < k) ? pinned(v[u]) : 0.0;
+        }
+    }
+};
+
 __device__ inline void loadBlockRows(double* ys, const double* X, int64_t row0, int qb, int k)
 {
-    constexpr int PER = SB * KMAX / 256;
-    double v[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-        const int idx = threadIdx.x + 256 * u;
-        const int i = idx >> 4, h = idx & (KMAX - 1);
-        v[u] = X[(row0 + min(i, qb - 1)) * k + min(h, k - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-        const int idx = threadIdx.x + 256 * u;
-        const int i = idx >> 4, h = idx & (KMAX - 1);
-        ys[idx] = (i < qb && h < k) ? pinned(v[u]) : 0.0;
-    }
+    StagedRows r;
+    r.load(X, row0, qb, k);
+    r.store(ys);
 }
 
 // The rows row0 .. row0 + qb of X ← ys.
@@ -1051,14 +1103,24 @@ __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP*
     const int rows = min(SB, n - i0);
     const TP* L = panels + p.panelOff[s];
     double4_t acc = {0, 0, 0, 0};
+    // Block q0 + SB is loaded while block q0 is multiplied.
+    StagedBlock<TP> nextL;
+    StagedRows nextY;
+    nextL.load(L, n, i0, rows, 0, min(SB, c));
+    nextY.load(X, first, min(SB, c), k);
     for (int q0 = 0; q0 < c; q0 += SB)
     {
-        const int qb = min(SB, c - q0);
-        loadBlockRows(ys, X, first + q0, qb, k);
-        stageBlock(blk, L, n, i0, rows, q0, qb);
+        __syncthreads(); // the previous block's products have read the LDS
+        nextL.store(blk);
+        nextY.store(ys);
         __syncthreads();
+        if (q0 + SB < c)
+        {
+            const int qb = min(SB, c - q0 - SB);
+            nextL.load(L, n, i0, rows, q0 + SB, qb);
+            nextY.load(X, first + q0 + SB, qb, k);
+        }
         acc = rowsTimesVectors(blk, ys, acc);
-        __syncthreads();
     }
     double* u = U + p.rowStart[s] * k;
     const int h = accColumn();
@@ -1081,34 +1143,12 @@ __global__ __launch_bounds__(256) void diagonalSolveKernel(double* X, const doub
         X[i * k + h] /= d[i];
 }
 
-// xs ← the rows rows[i0 − c + i] (or first + i0 + i below c) of X, for i < ib (1 <= ib <= SB); the row indices are
-// loaded first, then the values, from positions clamped into the rows.
 __device__ inline void gatherRows(double* xs, const double* X, const int32_t* rows, int first, int c, int i0, int ib,
                                   int k)
 {
-    constexpr int PER = SB * KMAX / 256;
-    int64_t row[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-        const int idx = threadIdx.x + 256 * u;
-        const int at = i0 + min(idx >> 4, ib - 1);
-        row[u] = at < c ? int64_t(first + at) : int64_t(rows[max(at - c, 0)]);
-    }
-    double v[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-        const int idx = threadIdx.x + 256 * u;
-        v[u] = X[row[u] * k + min(idx & (KMAX - 1), k - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-        const int idx = threadIdx.x + 256 * u;
-        const int i = idx >> 4, h = idx & (KMAX - 1);
-        xs[idx] = (i < ib && h < k) ? pinned(v[u]) : 0.0;
-    }
+    StagedRows r;
+    r.gather(X, rows, first, c, i0, ib, k);
+    r.store(xs);
 }
 
 // X(cols of block J) −= L_RF(:, J)ᵀ·X(rows), for the column blocks of the big and wide supernodes of a level.
@@ -1125,14 +1165,24 @@ __global__ __launch_bounds__(256) void backwardColumnsKernel(DevicePlan p, const
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
     double4_t acc = {0, 0, 0, 0};
+    // Row block i0 + SB is loaded while block i0 is multiplied.
+    StagedBlock<TP> nextL;
+    StagedRows nextX;
+    nextL.load(L, n, c, min(SB, n - c), q0, qb);
+    nextX.gather(X, rows, first, c, c, min(SB, n - c), k);
     for (int i0 = c; i0 < n; i0 += SB)
     {
-        const int ib = min(SB, n - i0);
-        gatherRows(xs, X, rows, first, c, i0, ib, k);
-        stageBlock(blk, L, n, i0, ib, q0, qb);
+        __syncthreads(); // the previous block's products have read the LDS
+        nextL.store(blk);
+        nextX.store(xs);
         __syncthreads();
+        if (i0 + SB < n)
+        {
+            const int ib = min(SB, n - i0 - SB);
+            nextL.load(L, n, i0 + SB, ib, q0, qb);
+            nextX.gather(X, rows, first, c, i0 + SB, ib, k);
+        }
         acc = columnsTimesVectors(blk, xs, acc);
-        __syncthreads();
     }
     const int h = accColumn();
 #pragma unroll
@@ -1208,17 +1258,28 @@ __global__ __launch_bounds__(256) void triangleProductsKernel(DevicePlan p, cons
     const int ib = min(SB, c - SB * I);
     double4_t acc = {0, 0, 0, 0};
     const int jFirst = transpose ? I : 0, jEnd = transpose ? nb : I + 1;
-    for (int J = jFirst; J < jEnd; ++J)
+    // Block J + 1 is loaded while block J is multiplied.
+    StagedBlock<TP> nextB;
+    StagedRows nextY;
+    const auto loadJ = [&](int J)
     {
         const int jb = min(SB, c - SB * J);
-        loadBlockRows(ys, X, first + SB * J, jb, k);
+        nextY.load(X, first + SB * J, jb, k);
         if (transpose)
-            stageBlock(blk, Xinv, c, SB * J, jb, SB * I, ib);
+            nextB.load(Xinv, c, SB * J, jb, SB * I, ib);
         else
-            stageBlock(blk, Xinv, c, SB * I, ib, SB * J, jb);
+            nextB.load(Xinv, c, SB * I, ib, SB * J, jb);
+    };
+    loadJ(jFirst);
+    for (int J = jFirst; J < jEnd; ++J)
+    {
+        __syncthreads(); // the previous block's products have read the LDS
+        nextB.store(blk);
+        nextY.store(ys);
         __syncthreads();
+        if (J + 1 < jEnd)
+            loadJ(J + 1);
         acc = transpose ? columnsTimesVectors(blk, ys, acc) : rowsTimesVectors(blk, ys, acc);
-        __syncthreads();
     }
     const int h = accColumn();
 #pragma unroll
@@ -1343,7 +1404,8 @@ __global__ void axpyColumnsKernel(double* x, const double* d, int64_t ld, int64_
 
 constexpr int VECTOR_ROWS = 16; // rows of a workgroup's pass
 
-// out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC).
+// out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC); a row per
+// group of 16 lanes (four rows in flight per wave), lane h its column h.
 __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets, const int32_t* cols,
                                                          const double* values, const double* x, const double* b,
                                                          double* out, int64_t n, int k)

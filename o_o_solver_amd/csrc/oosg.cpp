@@ -137,9 +137,13 @@ enum ProfileKind
     PROFILE_PANEL_SOLVE,
     PROFILE_PANEL_UPDATE,
     PROFILE_UPDATE_MATRIX,
-    PROFILE_FORWARD,
-    PROFILE_BACKWARD,
-    PROFILE_RESIDUAL,
+    PROFILE_FORWARD,           // forward sweep: gathers and the triangles of supernodes not wide
+    PROFILE_BACKWARD,          // backward sweep: the triangles of supernodes not wide
+    PROFILE_RESIDUAL,          // residual products
+    PROFILE_FORWARD_ROWS,      // forward sweep: the L_RF row tiles
+    PROFILE_BACKWARD_COLUMNS,  // backward sweep: the L_RFᵀ column blocks
+    PROFILE_TRIANGLE_PRODUCTS, // both sweeps: the wide triangles, products with their whole inverses
+    PROFILE_TRIANGLE_INVERSES, // factorisation: the whole inverses of the wide triangles
     PROFILE_KINDS
 };
 
@@ -326,6 +330,14 @@ struct oosg_plan
     bool graphs = true;
     GraphCache factorGraph[2], sweepGraph;
     std::vector<double> levelPanelValues; // per level: Σ c·(c + r), the panel values a sweep of the level reads
+    // The algorithmic work of each sweep launch of a level: the values of L (or of an inverse) it reads once, and the
+    // rows of k right-hand-side values it reads or writes once (× k·8 bytes at run time).
+    struct SweepWork
+    {
+        double triangleValues = 0, rowValues = 0, wideValues = 0;
+        double forwardRows = 0, backwardRows = 0, rowTileRows = 0, columnRows = 0, wideRows = 0;
+    };
+    std::vector<SweepWork> sweepWork;
     Profiler profiler;
 
     ~oosg_plan()
@@ -499,12 +511,38 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
         p.levelStart[l + 1] += p.levelStart[l];
     p.levelList.resize(S);
     p.levelPanelValues.assign(p.levels, 0.0);
+    p.sweepWork.assign(p.levels, oosg_plan::SweepWork{});
     {
         std::vector<int32_t> fill(p.levels, 0);
         for (int32_t k = 0; k < S; ++k)
         {
             p.levelList[p.levelStart[height[k]] + fill[height[k]]++] = k;
             p.levelPanelValues[height[k]] += static_cast<double>(p.panelOff[k + 1] - p.panelOff[k]);
+            oosg_plan::SweepWork& w = p.sweepWork[height[k]];
+            const double c = p.ncol[k], r = p.nrow[k];
+            double inBlocks = 0; // values of the diagonal blocks' lower triangles
+            for (int q0 = 0; q0 < p.ncol[k]; q0 += oosg::PANEL_BLOCK)
+            {
+                const double kb = std::min(oosg::PANEL_BLOCK, p.ncol[k] - q0);
+                inBlocks += kb * (kb + 1) / 2;
+            }
+            if (oosg::keepsTriangleInverse(p.ncol[k]))
+            {
+                w.wideValues += c * (c + 1) / 2; // the whole inverse's lower triangle
+                w.wideRows += 2 * c;             // x_F read, W written
+            }
+            else
+                w.triangleValues += oosg::blockInverseValues(p.ncol[k]) + (c * (c + 1) / 2 - inBlocks);
+            // Gathers: each child's update vector read once, its target rows read and written; X_F read and written,
+            // the update vector zeroed.
+            double gathered = 0;
+            for (int32_t q = p.childStart[k]; q < p.childStart[k + 1]; ++q)
+                gathered += 3.0 * p.nrow[p.children[q]];
+            w.forwardRows += gathered + 2 * c + r;
+            w.backwardRows += 2 * c;
+            w.rowValues += r * c;
+            w.rowTileRows += r > 0 ? 2 * r + c : 0;  // update vector read and written, x_F read
+            w.columnRows += r > 0 ? r + 2 * c : 0;   // X_R read, X_F read and written
         }
     }
     // The sweeps' row tiles and column blocks of the big supernodes, level by level.
@@ -766,11 +804,10 @@ void buildSchedule(oosg_plan& p)
                 tilesLaunch.flops += tileFlops(p.tiles[q]);
             p.schedule.push_back(tilesLaunch);
         }
-        // The whole inverses of the level's wide triangles (their panels are final), one launch per round; timed
-        // with the diagonal blocks.
+        // The whole inverses of the level's wide triangles (their panels are final), one launch per step.
         for (const oosg_plan::InverseRound& round : p.inverseRounds[l])
         {
-            Launch inverse{LaunchKind::Inverse, PROFILE_DIAGONAL};
+            Launch inverse{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
             inverse.offset = round.offset;
             inverse.count = round.count;
             inverse.k0 = round.step;
@@ -995,44 +1032,73 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
     const int2* triangle = p.dTriangleItems.as<int2>();
     double* W = p.solveW.as<double>();
-    // The profile counts bytes as the work of the sweeps (they are bound by HBM): the level's panels, read once.
+    // The profile counts bytes as the work of the sweeps (they are bound by HBM): the values of L (or of an inverse)
+    // a launch reads once, and the right-hand-side rows it reads or writes once.
+    const double T = sizeof(TP), row = 8.0 * k;
+    const auto timed = [&](int kind, double work, auto&& launch)
+    {
+        const int slot = p.profiler.begin(kind, work, st);
+        launch();
+        p.profiler.end(slot, st);
+    };
     for (int32_t l = 0; l < p.levels; ++l)
     {
-        const int slot = p.profiler.begin(PROFILE_FORWARD, p.levelPanelValues[l] * sizeof(TP), st);
+        const oosg_plan::SweepWork& w = p.sweepWork[l];
         // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses, into W, then
         // back into X), then every L_RF row tile of the level.
-        check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
-                                      p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, st),
-              "solve: forward sweep");
+        timed(PROFILE_FORWARD, w.triangleValues * T + w.forwardRows * row, [&] {
+            check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
+                                          p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, st),
+                  "solve: forward sweep");
+        });
         const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
-        check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles, 0, st),
-              "solve: forward sweep (triangles)");
-        check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
-              "solve: forward sweep (triangles)");
-        check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, nullptr, 0, forwardTiles + p.forwardTileStart[l],
-                                      static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]), st),
-              "solve: forward sweep (rows)");
-        p.profiler.end(slot, st);
+        if (triangles > 0)
+        {
+            timed(PROFILE_TRIANGLE_PRODUCTS, w.wideValues * T + w.wideRows * row, [&] {
+                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles,
+                                                       0, st),
+                      "solve: forward sweep (triangles)");
+            });
+            check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+                  "solve: forward sweep (triangles)");
+        }
+        const int tiles = static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]);
+        if (tiles > 0)
+            timed(PROFILE_FORWARD_ROWS, w.rowValues * T + w.rowTileRows * row, [&] {
+                check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, nullptr, 0,
+                                              forwardTiles + p.forwardTileStart[l], tiles, st),
+                      "solve: forward sweep (rows)");
+            });
     }
     check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
     for (int32_t l = p.levels - 1; l >= 0; --l)
     {
-        const int slot = p.profiler.begin(PROFILE_BACKWARD, p.levelPanelValues[l] * sizeof(TP), st);
+        const oosg_plan::SweepWork& w = p.sweepWork[l];
         // The L_RFᵀ products of every supernode first (column blocks), then the wide supernodes' triangles (with
         // their whole inverses), then the other triangles.
         const int blocks = static_cast<int>(p.backwardBlockStart[l + 1] - p.backwardBlockStart[l]);
-        check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0, backwardBlocks + p.backwardBlockStart[l],
-                                       blocks, st),
-              "solve: backward sweep");
+        if (blocks > 0)
+            timed(PROFILE_BACKWARD_COLUMNS, w.rowValues * T + w.columnRows * row, [&] {
+                check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0,
+                                               backwardBlocks + p.backwardBlockStart[l], blocks, st),
+                      "solve: backward sweep (columns)");
+            });
         const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
-        check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles, 1, st),
-              "solve: backward sweep (triangles)");
-        check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
-              "solve: backward sweep (triangles)");
-        check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
-                                       p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, st),
-              "solve: backward sweep");
-        p.profiler.end(slot, st);
+        if (triangles > 0)
+        {
+            timed(PROFILE_TRIANGLE_PRODUCTS, w.wideValues * T + w.wideRows * row, [&] {
+                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles,
+                                                       1, st),
+                      "solve: backward sweep (triangles)");
+            });
+            check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+                  "solve: backward sweep (triangles)");
+        }
+        timed(PROFILE_BACKWARD, w.triangleValues * T + w.backwardRows * row, [&] {
+            check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
+                                           p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, st),
+                  "solve: backward sweep");
+        });
     }
 }
 
