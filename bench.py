@@ -1,6 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark of the numeric multifrontal LDLᵀ on one MI355X: factor + solve to a backward error of 1e-10, mixed
-(FP32 fronts on MFMA + FP64 iterative refinement) against the plain FP64 GPU factor.
+precision against the plain FP64 GPU factor.
+
+The mixed path is the learned precision selector's choice per matrix (FP32 fronts on MFMA refined in FP64 by
+conjugate gradients, or FP64 fronts when it predicts the FP32 way would be slower; FP32 when no trained weights are
+present). The FP32-fronts path is also timed on its own ("fp32_fronts_cg") and its kernels give the roofline.
 
 Workload: the 500 × 500 quad4 shell plate (1,494,006 DOF), interior nodes jittered ±20 % of the element size, element
 thickness log-uniform in [1, 4] mm, E = 70 GPa, ν = 0.3, clamped edges, 16 load cases of independent N(0, 1) forces on
@@ -125,7 +129,11 @@ def main():
     stats = plan.stats()
     log(f"plan + upload ({time.perf_counter() - t:.1f} s): {stats}")
 
-    def mixed_step():
+    from o_o_solver_amd.selector import PrecisionSelector
+    selector = PrecisionSelector.trained()
+    choice = {"probability": None, "precision": "fp32"}
+
+    def fp32_step():
         f = SupernodalFactor.factorise(sym, workload.matrix, FactorOptions(precision=Precision.FP32),
                                        device_matrix=dm)
         try:
@@ -138,6 +146,14 @@ def main():
             x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=4)
             fallback = True
         return f.factor_seconds, rep, fallback
+
+    def mixed_step():
+        # The learned selector decides between FP32 fronts (refined by conjugate gradients) and FP64 fronts.
+        if selector is not None:
+            p = float(selector.predict(PrecisionSelector.features(plan, dm))[0])
+            choice["probability"] = p
+            choice["precision"] = "fp32" if p >= 0.5 else "fp64"
+        return fp32_step() if choice["precision"] == "fp32" else fp64_step()
 
     def fp64_step():
         f = SupernodalFactor.factorise(sym, workload.matrix, device_matrix=dm)
@@ -170,10 +186,13 @@ def main():
         return elapsed / args.steps, float(np.mean(fs)), reps, any(r[2] for r in results), prof
 
     fp64_s, fp64_factor, fp64_reps, _, fp64_prof = run(fp64_step, "fp64", True)
-    mixed_s, mixed_factor, mixed_reps, fallback, mixed_prof = run(mixed_step, "mixed", True)
+    fp32_s, fp32_factor, fp32_reps, fp32_fallback, fp32_prof = run(fp32_step, "fp32 fronts + cg", True)
+    mixed_s, mixed_factor, mixed_reps, fallback, _ = run(mixed_step, "mixed", False)
+    log(f"selector: p(fp32 pays) = {choice['probability']}, chose {choice['precision']}")
 
-    # The roofline of the kernel that takes the most device time in the headline (mixed) path.
-    prof = mixed_prof
+    # The roofline of the kernel that takes the most device time in the FP32 fronts' path (the mixed precision's
+    # own kernels; the FP64 path's are reported beside it).
+    prof = fp32_prof
     dominant = max(prof, key=lambda k: prof[k][0])
     seconds, launches, work = prof[dominant]  # of the one profiled step
     if dominant in BYTE_KINDS:
@@ -181,7 +200,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM, 4), "traffic": None}
     else:
-        peak = PEAK_FP64 if fallback else PEAK_FP32
+        peak = PEAK_FP64 if fp32_fallback else PEAK_FP32
         achieved = work / seconds / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                     "frac": round(achieved / peak, 4), "traffic": None}
@@ -231,12 +250,18 @@ def main():
                   "solve_s": round(float(np.mean([r.seconds for r in mixed_reps])), 5),
                   "solves": max(r.steps for r in mixed_reps),
                   "backward_error": max(r.backward_error for r in mixed_reps),
-                  "fp64_fallback": fallback,
-                  "factor_tflops": round(flops / mixed_factor / 1e12, 2),
-                  "mfma_frac": round(flops / mixed_factor / 1e12 / PEAK_FP32, 4)},
+                  "precision_chosen": choice["precision"], "selector_probability": choice["probability"],
+                  "fp64_fallback": fallback},
+        "fp32_fronts_cg": {"seconds": round(fp32_s, 5), "factor_s": round(fp32_factor, 5),
+                           "solve_s": round(float(np.mean([r.seconds for r in fp32_reps])), 5),
+                           "solves": max(r.steps for r in fp32_reps),
+                           "backward_error": max(r.backward_error for r in fp32_reps),
+                           "fp64_fallback": fp32_fallback,
+                           "factor_tflops": round(flops / fp32_factor / 1e12, 2),
+                           "mfma_frac": round(flops / fp32_factor / 1e12 / PEAK_FP32, 4)},
         "speedup_mixed_vs_fp64": round(fp64_s / mixed_s, 3),
         "roofline": roofline,
-        "kernels": kernels,
+        "kernels_fp32_fronts_cg": kernels,
         "kernels_fp64": breakdown(fp64_prof),
         "cpu_baseline": baseline,
     }

@@ -207,8 +207,8 @@ void oosg_selector_destroy(oosg_selector *selector);
 /* The flat parameter array: W1 (32 × 8 row-major), b1 (32), W2 (32 × 32), b2 (32), W3 (32), b3 (1). */
 oosg_status oosg_selector_parameters(const oosg_selector *selector, float *params);
 oosg_status oosg_selector_set_parameters(oosg_selector *selector, const float *params);
-/* The 8 features of a matrix of a plan (normalised to about unit range): log10 n, log10 flops, log10 of the largest
- * front order, log10 of the spread of the positive diagonal, the mean and the minimum of log10(a_jj / Σ_i≠j |a_ij|),
+/* The 8 features of a matrix of a plan (normalised to about unit range): log10 of the factorisation's flops per
+ * stored factor value, log10 flops, log10 of the largest front order, log10 of the spread of the positive diagonal, the mean and the minimum of log10(a_jj / Σ_i≠j |a_ij|),
  * the fraction of equations whose diagonal does not dominate, log10(‖A‖∞ / max a_jj). */
 oosg_status oosg_selector_features(const oosg_plan *plan, const oosg_matrix *matrix, float *features);
 /* p for count × 8 features. */

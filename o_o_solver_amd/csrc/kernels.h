@@ -129,10 +129,9 @@ hipError_t launchFromRowMajor(const double* src, double* dst, int64_t ld, int64_
                               int accumulate, hipStream_t stream);
 hipError_t launchResidual(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
                           const double* b, double* out, int64_t ld, int64_t n, int k, hipStream_t stream);
-hipError_t launchColumnMax(const double* v, int64_t ld, int64_t n, int k, unsigned long long* out,
-                           hipStream_t stream);
+
 hipError_t launchFillDouble(double* data, int64_t ld, int64_t n, int k, double value, hipStream_t stream);
-hipError_t launchAxpyColumns(double* x, const double* d, int64_t ld, int64_t n, int k, hipStream_t stream);
+
 // Per-column coefficients of the Krylov updates (k <= 16), passed by value.
 struct ColumnCoefficients
 {
@@ -154,6 +153,7 @@ hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, 
 hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
                              hipStream_t stream);
 hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream);
+hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream);
 hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const float* af, const float* bf, float* df,
                            hipStream_t stream);
 

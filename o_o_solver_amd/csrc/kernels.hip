@@ -1360,41 +1360,11 @@ __global__ void residualKernel(const int64_t* offsets, const int32_t* cols, cons
     }
 }
 
-// Per column the largest |v| (as the bits of a non-negative double, whose order is the order of the values).
-__global__ void columnMaxKernel(const double* v, int64_t ld, int64_t n, int k, unsigned long long* out)
-{
-    const int h = blockIdx.y;
-    double m = 0.0;
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    {
-        const double a = fabs(v[i + h * ld]);
-        m = a > m || a != a ? a : m;
-    }
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        const double o = __shfl_xor(m, off);
-        m = o > m || o != o ? o : m;
-    }
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(&out[h], static_cast<unsigned long long>(__double_as_longlong(m)));
-}
-
 __global__ void fillDoubleKernel(double* data, int64_t ld, int64_t n, int k, double value)
 {
     for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < n * k;
          idx += int64_t(gridDim.x) * blockDim.x)
         data[(idx % n) + (idx / n) * ld] = value;
-}
-
-__global__ void axpyColumnsKernel(double* x, const double* d, int64_t ld, int64_t n, int k)
-{
-    for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < n * k;
-         idx += int64_t(gridDim.x) * blockDim.x)
-    {
-        const int64_t i = idx % n;
-        const int h = int(idx / n);
-        x[i + h * ld] += d[i + h * ld];
-    }
 }
 
 // ---- Krylov vectors: n × k row-major (k <= 16), a row per group of 16 lanes ---------------------------------------
@@ -1508,6 +1478,17 @@ __global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const 
         x[i * k + h] += a * p[i * k + h];
         r[i * k + h] -= a * q[i * k + h];
     }
+}
+
+// x += z (row-major n × k).
+__global__ __launch_bounds__(256) void addRowsKernel(double* x, const double* z, int64_t n, int k)
+{
+    const int h = threadIdx.x & 15;
+    if (h >= k)
+        return;
+    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
+         i += int64_t(gridDim.x) * VECTOR_ROWS)
+        x[i * k + h] += z[i * k + h];
 }
 
 // p = z + β_h·p.
@@ -1685,12 +1666,6 @@ hipError_t launchResidual(const int64_t* offsets, const int32_t* cols, const dou
     return hipGetLastError();
 }
 
-hipError_t launchColumnMax(const double* v, int64_t ld, int64_t n, int k, unsigned long long* out, hipStream_t stream)
-{
-    const int blocks = blocksFor(n, 256) > 1024 ? 1024 : blocksFor(n, 256);
-    hipLaunchKernelGGL(columnMaxKernel, dim3(blocks, k), dim3(256), 0, stream, v, ld, n, k, out);
-    return hipGetLastError();
-}
 
 hipError_t launchFillDouble(double* data, int64_t ld, int64_t n, int k, double value, hipStream_t stream)
 {
@@ -1698,11 +1673,6 @@ hipError_t launchFillDouble(double* data, int64_t ld, int64_t n, int k, double v
     return hipGetLastError();
 }
 
-hipError_t launchAxpyColumns(double* x, const double* d, int64_t ld, int64_t n, int k, hipStream_t stream)
-{
-    hipLaunchKernelGGL(axpyColumnsKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, x, d, ld, n, k);
-    return hipGetLastError();
-}
 
 // The dynamic LDS limits of the kernels that take more than the default, set once per process before any launch
 // (outside any graph capture).
@@ -1757,6 +1727,12 @@ hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, 
                         int64_t n, int k, hipStream_t stream)
 {
     hipLaunchKernelGGL(cgStepKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, x, r, p, q, alpha, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream)
+{
+    hipLaunchKernelGGL(addRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, x, z, n, k);
     return hipGetLastError();
 }
 

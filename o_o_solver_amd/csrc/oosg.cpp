@@ -363,6 +363,9 @@ struct oosg_matrix
     std::vector<int32_t> rows;
     std::vector<double> diagonal; // a_jj, 0 when not stored
     std::vector<double> hostValues; // the values as last given
+    // Statistics of the values for the precision selector: mean and minimum of log10(|a_jj| / Σ_i≠j |a_ij|) (clamped
+    // to [−3, 3]), the fraction of equations whose diagonal does not dominate, the smallest and largest positive a_jj.
+    double meanDominance = 0, minDominance = 3, weakFraction = 0, diagonalMin = INFINITY, diagonalMax = 0;
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
     DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
@@ -1147,22 +1150,39 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     }
 }
 
-std::vector<double> columnMax(oosg_plan& p, const double* v, int64_t ld, int64_t n, int k)
+// The selector's statistics of a matrix's values (oosg_matrix::meanDominance and the following fields).
+void valueStatistics(oosg_matrix& m)
 {
-    ensure(p.solveNorms, 16 * sizeof(unsigned long long), "solve: norms");
-    std::vector<unsigned long long> bits(static_cast<size_t>(k), 0);
-    check(hipMemcpyAsync(p.solveNorms.ptr, bits.data(), k * sizeof(unsigned long long), hipMemcpyHostToDevice,
-                         p.stream),
-          "solve: norms");
-    check(oosg::launchColumnMax(v, ld, n, k, p.solveNorms.as<unsigned long long>(), p.stream), "solve: norms");
-    check(hipMemcpyAsync(bits.data(), p.solveNorms.ptr, k * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                         p.stream),
-          "solve: norms");
-    check(hipStreamSynchronize(p.stream), "solve: norms");
-    std::vector<double> out(static_cast<size_t>(k));
-    for (int h = 0; h < k; ++h)
-        std::memcpy(&out[h], &bits[h], sizeof(double));
-    return out;
+    const int32_t n = m.n;
+    std::vector<double> off(static_cast<size_t>(n), 0.0);
+    for (int32_t j = 0; j < n; ++j)
+        for (int64_t k = m.offsets[j]; k < m.offsets[j + 1]; ++k)
+            if (m.rows[k] != j)
+            {
+                off[m.rows[k]] += std::fabs(m.hostValues[k]);
+                off[j] += std::fabs(m.hostValues[k]);
+            }
+    m.meanDominance = 0;
+    m.minDominance = 3;
+    m.diagonalMin = INFINITY;
+    m.diagonalMax = 0;
+    int64_t weak = 0;
+    for (int32_t j = 0; j < n; ++j)
+    {
+        const double a = m.diagonal[j];
+        if (a > 0.0 && std::isfinite(a))
+        {
+            m.diagonalMin = std::min(m.diagonalMin, a);
+            m.diagonalMax = std::max(m.diagonalMax, a);
+        }
+        double dominance = off[j] > 0.0 ? std::log10(std::fabs(a) / off[j]) : 3.0;
+        dominance = std::min(3.0, std::max(-3.0, dominance != dominance ? -3.0 : dominance));
+        m.meanDominance += dominance;
+        m.minDominance = std::min(m.minDominance, dominance);
+        weak += std::fabs(a) < off[j] ? 1 : 0;
+    }
+    m.meanDominance /= std::max(n, 1);
+    m.weakFraction = static_cast<double>(weak) / std::max(n, 1);
 }
 
 // Row-major n × k vectors (k <= 16) of the conjugate-gradient solve.
@@ -1327,6 +1347,64 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
     report.backward_error = 0.0;
     for (int h = 0; h < k; ++h)
         report.backward_error = std::max(report.backward_error, eta[h]);
+}
+
+// Iterative refinement in double precision on row-major vectors: x_0 = 0, r_0 = b; d = factor⁻¹·r, x += d,
+// r = b − A·x; stops at the backward-error target, after max_steps solves, or when a step does not halve the
+// backward error (the factor is then not accurate enough for the matrix).
+void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
+                     double target, int32_t maxSteps, oosg_solve_report& report)
+{
+    oosg_plan& p = *f.plan;
+    const hipStream_t st = p.stream;
+    const int64_t n = p.n;
+    const size_t bytes = static_cast<size_t>(n) * 16 * sizeof(double);
+    ensure(p.cgB, bytes, "solve: refinement");
+    ensure(p.cgX, bytes, "solve: refinement");
+    ensure(p.cgR, bytes, "solve: refinement");
+    ensure(p.cgZ, bytes, "solve: refinement");
+    double* B = p.cgB.as<double>();
+    double* X = p.cgX.as<double>();
+    double* R = p.cgR.as<double>();
+    double* Z = p.cgZ.as<double>();
+    const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
+    check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+    const std::vector<double> bNorm = columnMaxRows(p, B, k);
+    check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
+          "solve: zero");
+    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+    double previous = INFINITY;
+    report.converged = 0;
+    for (int32_t step = 1; step <= std::max(1, maxSteps); ++step)
+    {
+        applyFactorRows(f, R, Z, k);
+        check(oosg::launchAddRows(X, Z, n, k, st), "solve: update");
+        const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
+        check(oosg::launchMultiplyRows(m.dCsrOffsets.as<int64_t>(), m.dCsrCols.as<int32_t>(), m.dCsrValues.as<double>(),
+                                       X, B, R, n, k, st),
+              "solve: residual");
+        p.profiler.end(slot, st);
+        const std::vector<double> rNorm = columnMaxRows(p, R, k);
+        const std::vector<double> xNorm = columnMaxRows(p, X, k);
+        double eta = 0.0;
+        for (int h = 0; h < k; ++h)
+        {
+            const double denominator = m.normInf * xNorm[h] + bNorm[h];
+            const double e = denominator > 0 ? rNorm[h] / denominator : rNorm[h];
+            eta = std::max(eta, e != e ? INFINITY : e);
+        }
+        report.steps = step;
+        report.backward_error = eta;
+        if (eta <= target)
+        {
+            report.converged = 1;
+            break;
+        }
+        if (step > 1 && !(eta < 0.5 * previous))
+            break;
+        previous = eta;
+    }
+    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
 }
 
 } // namespace
@@ -1603,44 +1681,25 @@ oosg_status oosg_selector_features(const oosg_plan* plan, const oosg_matrix* m, 
                 fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
             if (m->plan != plan)
                 fail(OOSG_INVALID_ARGUMENT, "selector: the matrix was made for another plan");
-            const int32_t n = plan->n;
             int64_t largestFront = 1;
+            double entries = 0.0; // Σ c(c + 1)/2 + c·r: the factor's stored values (its solves read them)
             for (int32_t s = 0; s < plan->S; ++s)
-                largestFront = std::max<int64_t>(largestFront, plan->ncol[s] + plan->nrow[s]);
-            // Off-diagonal absolute row sums of the symmetric matrix.
-            std::vector<double> off(static_cast<size_t>(n), 0.0);
-            const std::vector<double>& values = m->hostValues;
-            for (int32_t j = 0; j < n; ++j)
-                for (int64_t k = m->offsets[j]; k < m->offsets[j + 1]; ++k)
-                    if (m->rows[k] != j)
-                    {
-                        off[m->rows[k]] += std::fabs(values[k]);
-                        off[j] += std::fabs(values[k]);
-                    }
-            double dmin = INFINITY, dmax = 0.0, meanDominance = 0.0, minDominance = 3.0;
-            int64_t weak = 0;
-            for (int32_t j = 0; j < n; ++j)
             {
-                const double a = m->diagonal[j];
-                if (a > 0.0 && std::isfinite(a))
-                {
-                    dmin = std::min(dmin, a);
-                    dmax = std::max(dmax, a);
-                }
-                double dominance = off[j] > 0.0 ? std::log10(std::fabs(a) / off[j]) : 3.0;
-                dominance = std::min(3.0, std::max(-3.0, dominance != dominance ? -3.0 : dominance));
-                meanDominance += dominance;
-                minDominance = std::min(minDominance, dominance);
-                weak += std::fabs(a) < off[j] ? 1 : 0;
+                const double c = plan->ncol[s], r = plan->nrow[s];
+                largestFront = std::max<int64_t>(largestFront, plan->ncol[s] + plan->nrow[s]);
+                entries += c * (c + 1) / 2 + c * r;
             }
-            meanDominance /= std::max(n, 1);
+            const double dmin = m->diagonalMin, dmax = m->diagonalMax;
+            const double meanDominance = m->meanDominance, minDominance = m->minDominance;
             const double spread = dmax > 0.0 ? std::log10(dmax / dmin) : 0.0;
             const double norm = dmax > 0.0 ? std::log10(m->normInf / dmax) : 0.0;
-            const double raw[8] = {std::log10(std::max(n, 1)), std::log10(std::max(plan->flops, 1.0)),
+            // The factorisation's flops per stored value (its work against a solve's), then its size.
+            const double raw[8] = {std::log10(std::max(plan->flops, 1.0) / std::max(entries, 1.0)),
+                                   std::log10(std::max(plan->flops, 1.0)),
                                    std::log10(static_cast<double>(largestFront)), spread, meanDominance,
-                                   minDominance, static_cast<double>(weak) / std::max(n, 1), norm};
+                                   minDominance, m->weakFraction, norm};
             // Fixed affine maps to about unit range.
-            const double center[8] = {5.0, 10.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0};
+            const double center[8] = {3.0, 10.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0};
             const double width[8] = {2.0, 3.0, 1.0, 4.0, 1.0, 1.0, 1.0, 1.0};
             for (int q = 0; q < 8; ++q)
                 features[q] = static_cast<float>((raw[q] - center[q]) / width[q]);
@@ -1872,6 +1931,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dRows = upload(m->rows, st, "matrix: rows");
             m->dValues = upload(values, st, "matrix: values");
             m->hostValues = values;
+            valueStatistics(*m);
             m->dPosition = upload(position, st, "matrix: positions");
             m->dColumn = upload(column, st, "matrix: columns");
             m->dScale = upload(m->scale, st, "matrix: scales");
@@ -1931,6 +1991,7 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
                   "matrix: row values");
             check(hipStreamSynchronize(st), "matrix: values");
             m->hostValues = std::move(v);
+            valueStatistics(*m);
         });
 }
 
@@ -2185,72 +2246,18 @@ oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m
             setDevice(&p);
             const hipStream_t st = p.stream;
             const int k = static_cast<int>(nrhs);
-            const int64_t n = p.n;
-            const size_t bytes = static_cast<size_t>(ld) * nrhs * sizeof(double);
-            ensure(p.solveD, bytes, "refined solve: corrections");
-            ensure(p.solveR, bytes, "refined solve: residuals");
-            double* d = p.solveD.as<double>();
-            double* r = p.solveR.as<double>();
-            if (f->precision == OOSG_FP32)
-            {
-                const auto t0 = std::chrono::steady_clock::now();
-                solvePcg(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
-                check(hipStreamSynchronize(st), "refined solve");
-                p.profiler.collect();
-                local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                if (!local.converged)
-                    fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " +
-                                                 std::to_string(local.backward_error) + " did not reach " +
-                                                 std::to_string(target) + " in " + std::to_string(local.steps) +
-                                                 " solves");
-                return;
-            }
-            const std::vector<double> bNorm = columnMax(p, b, ld, n, k);
-            check(hipEventRecord(p.start, st), "hipEventRecord");
             const auto t0 = std::chrono::steady_clock::now();
-            check(oosg::launchFillDouble(x, ld, n, k, 0.0, st), "refined solve: zero");
-            check(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, st), "refined solve: residual");
-            double previous = INFINITY;
-            for (int32_t step = 1; step <= std::max(1, max_steps); ++step)
-            {
-                check(hipMemcpyAsync(d, r, bytes, hipMemcpyDeviceToDevice, st), "refined solve: correction");
-                solveDevice(*f, d, ld, nrhs);
-                check(oosg::launchAxpyColumns(x, d, ld, n, k, st), "refined solve: update");
-                const double residualBytes = static_cast<double>(m->csrFromCsc.size()) * 12.0 +
-                                             static_cast<double>(n + 1) * 8.0 + 2.0 * static_cast<double>(n) * k * 8.0;
-                const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes, st);
-                check(oosg::launchResidual(m->dCsrOffsets.as<int64_t>(), m->dCsrCols.as<int32_t>(),
-                                           m->dCsrValues.as<double>(), x, b, r, ld, n, k, st),
-                      "refined solve: residual");
-                p.profiler.end(slot, st);
-                const std::vector<double> rNorm = columnMax(p, r, ld, n, k);
-                const std::vector<double> xNorm = columnMax(p, x, ld, n, k);
-                double eta = 0.0;
-                for (int h = 0; h < k; ++h)
-                {
-                    const double denominator = m->normInf * xNorm[h] + bNorm[h];
-                    const double e = denominator > 0 ? rNorm[h] / denominator : rNorm[h];
-                    eta = std::max(eta, e != e ? INFINITY : e);
-                }
-                local.steps = step;
-                local.backward_error = eta;
-                if (eta <= target)
-                {
-                    local.converged = 1;
-                    break;
-                }
-                if (step > 1 && !(eta < 0.5 * previous))
-                    break; // stalled: the factor is not accurate enough for this matrix
-                previous = eta;
-            }
-            check(hipEventRecord(p.stop, st), "hipEventRecord");
+            if (f->precision == OOSG_FP32)
+                solvePcg(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
+            else
+                solveRefinement(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
             check(hipStreamSynchronize(st), "refined solve");
             p.profiler.collect();
             local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (!local.converged)
                 fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +
                                              " did not reach " + std::to_string(target) + " in " +
-                                             std::to_string(local.steps) + " steps");
+                                             std::to_string(local.steps) + " solves");
         });
     if (report != nullptr)
         *report = local;
