@@ -39,6 +39,7 @@ KIND_KERNELS = {"small_fronts": "smallFrontKernel", "assembly": "assembleKernel"
                 "backward_columns": "backwardColumnsKernel", "triangle_products": "triangleProductsKernel",
                 "triangle_inverses": "triangleInverseKernel"}
 # Profile kinds whose work is bytes (the sweeps and the residual products); the others count flops.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_shell_plate_c2.txt")
 BYTE_KINDS = ("forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
               "triangle_products")
 
@@ -106,6 +107,7 @@ def main():
     import torch
 
     import o_o_solver_amd as oos
+    from o_o_solver_amd import profiling
     from o_o_solver_amd import FactorOptions, FactorSolve, NotConvergedError, Precision, SupernodalFactor, models
 
     if not torch.cuda.is_available():
@@ -190,29 +192,43 @@ def main():
     mixed_s, mixed_factor, mixed_reps, fallback, _ = run(mixed_step, "mixed", False)
     log(f"selector: p(fp32 pays) = {choice['probability']}, chose {choice['precision']}")
 
-    # The roofline of the kernel that takes the most device time in the FP32 fronts' path (the mixed precision's
-    # own kernels; the FP64 path's are reported beside it).
-    prof = fp32_prof
-    dominant = max(prof, key=lambda k: prof[k][0])
-    seconds, launches, work = prof[dominant]  # of the one profiled step
-    if dominant in BYTE_KINDS:
+    # The roofline of the HIP kernel that takes the most device time in the path the mixed step ran (the profile
+    # kinds summed per kernel: the three tile kinds are one kernel), with its HBM traffic per launch from the
+    # committed rocprofv3 counter summary of this workload when it lists that kernel.
+    chose_fp32 = choice["precision"] == "fp32"
+    prof = fp32_prof if chose_fp32 else fp64_prof
+    kernel_type = "<float>" if chose_fp32 and not fp32_fallback else "<double>"
+    per_kernel = {}
+    for kind, (sec, count, work) in prof.items():
+        name = KIND_KERNELS[kind] + ("" if kind == "residual" else kernel_type)
+        acc = per_kernel.setdefault(name, {"seconds": 0.0, "launches": 0, "work": 0.0, "kinds": []})
+        acc["seconds"] += sec
+        acc["launches"] += count
+        acc["work"] += work
+        acc["kinds"].append(kind)
+    dominant = max(per_kernel, key=lambda k: per_kernel[k]["seconds"])
+    top = per_kernel[dominant]
+    seconds, launches, work = top["seconds"], top["launches"], top["work"]  # of the one profiled step
+    traffic = profiling.read_traffic(TRAFFIC_SUMMARY, dominant)
+    if top["kinds"][0] in BYTE_KINDS:
         achieved = work / seconds / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
-                    "frac": round(achieved / PEAK_HBM, 4), "traffic": None}
+                    "frac": round(achieved / PEAK_HBM, 4)}
     else:
-        peak = PEAK_FP64 if fp32_fallback else PEAK_FP32
+        peak = PEAK_FP32 if kernel_type == "<float>" else PEAK_FP64
         achieved = work / seconds / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                    "frac": round(achieved / peak, 4), "traffic": None}
-    roofline.update({"kernel": dominant, "hip_kernel": KIND_KERNELS[dominant], "launches_per_step": launches,
-                     "avg_launch_ms": round(seconds / launches * 1e3, 4),
+                    "frac": round(achieved / peak, 4)}
+    roofline.update({"traffic": round(traffic) if traffic else None, "hip_kernel": dominant, "kinds": top["kinds"],
+                     "launches_per_step": launches, "avg_launch_ms": round(seconds / launches * 1e3, 4),
                      "work_per_launch": work / launches,
-                     "share_of_device_time": round(seconds / sum(v[0] for v in prof.values()), 3)})
+                     "share_of_device_time": round(seconds / sum(v[0] for v in prof.values()), 3),
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if traffic else None})
     def breakdown(profile):
         return {k: {"ms_per_step": round(v[0] * 1e3, 3), "launches_per_step": v[1],
                     "rate": round(v[2] / v[0] / (1e9 if k in BYTE_KINDS else 1e12), 2) if v[0] > 0 else None}
                 for k, v in profile.items()}
-    kernels = breakdown(prof)
+    kernels = breakdown(fp32_prof)
 
     baseline = None
     if args.cpu_baseline == "auto":

@@ -24,7 +24,7 @@ static_assert(OFF_B3 + 1 == SELECTOR_PARAMETERS, "parameter layout");
 
 // One thread per sample: z1 = W1·x + b1, a1 = tanh z1, z2 = W2·a1 + b2, a2 = tanh z2, out = W3·a2 + b3,
 // p = 1 / (1 + e^−out). a1, a2 (count × H) are kept for the backward pass when not null.
-__global__ __launch_bounds__(256) void forwardKernel(const float* params, const float* x, int64_t count, float* a1,
+__global__ __launch_bounds__(256) void selectorForwardKernel(const float* params, const float* x, int64_t count, float* a1,
                                                      float* a2, float* probability, float* logit)
 {
     __shared__ float w[SELECTOR_PARAMETERS];
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void forwardKernel(const float* params, const 
 
 // Per sample the output error d3 = (p − y)/count (the mean binary cross-entropy's derivative by the logit) and the
 // back-propagated d2 = (W3 ∘ d3)·(1 − a2²), d1 = (W2ᵀ·d2)·(1 − a1²).
-__global__ __launch_bounds__(256) void deltaKernel(const float* params, const float* probability, const float* labels,
+__global__ __launch_bounds__(256) void selectorDeltaKernel(const float* params, const float* probability, const float* labels,
                                                    const float* a1, const float* a2, int64_t count, float* d3,
                                                    float* d2, float* d1)
 {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void deltaKernel(const float* params, const fl
 }
 
 // One thread per parameter: its gradient Σ_b (delta of its output)·(input of its weight), over the samples in order.
-__global__ __launch_bounds__(256) void gradientKernel(const float* x, const float* a1, const float* a2,
+__global__ __launch_bounds__(256) void selectorGradientKernel(const float* x, const float* a1, const float* a2,
                                                       const float* d3, const float* d2, const float* d1,
                                                       int64_t count, float* gradient)
 {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void gradientKernel(const float* x, const floa
 
 // The mean binary cross-entropy, by one workgroup in a fixed order: −mean(y·log p + (1 − y)·log(1 − p)), from the
 // logits (log p = −softplus(−z), log(1 − p) = −softplus(z)).
-__global__ __launch_bounds__(256) void lossKernel(const float* logit, const float* labels, int64_t count, float* loss)
+__global__ __launch_bounds__(256) void selectorLossKernel(const float* logit, const float* labels, int64_t count, float* loss)
 {
     __shared__ float part[256];
     float s = 0.0f;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void lossKernel(const float* logit, const floa
 }
 
 // Adam (Kingma & Ba 2015): m = β1 m + (1 − β1) g, v = β2 v + (1 − β2) g², θ −= lr·m̂/(sqrt(v̂) + ε).
-__global__ void adamKernel(float* params, const float* gradient, float* m, float* v, float lr, float correction1,
+__global__ void selectorAdamKernel(float* params, const float* gradient, float* m, float* v, float lr, float correction1,
                            float correction2)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,7 +185,7 @@ hipError_t selectorForward(const float* params, const float* x, int64_t count, f
 {
     if (count <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(forwardKernel, dim3(groupsFor(count)), dim3(256), 0, stream, params, x, count, a1, a2,
+    hipLaunchKernelGGL(selectorForwardKernel, dim3(groupsFor(count)), dim3(256), 0, stream, params, x, count, a1, a2,
                        probability, logit);
     return hipGetLastError();
 }
@@ -199,11 +199,11 @@ hipError_t selectorGradient(const float* params, const float* x, const float* la
     hipError_t e = selectorForward(params, x, count, a1, a2, probability, logit, stream);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(deltaKernel, dim3(groupsFor(count)), dim3(256), 0, stream, params, probability, labels, a1, a2,
+    hipLaunchKernelGGL(selectorDeltaKernel, dim3(groupsFor(count)), dim3(256), 0, stream, params, probability, labels, a1, a2,
                        count, d3, d2, d1);
-    hipLaunchKernelGGL(gradientKernel, dim3(groupsFor(SELECTOR_PARAMETERS)), dim3(256), 0, stream, x, a1, a2, d3, d2,
+    hipLaunchKernelGGL(selectorGradientKernel, dim3(groupsFor(SELECTOR_PARAMETERS)), dim3(256), 0, stream, x, a1, a2, d3, d2,
                        d1, count, gradient);
-    hipLaunchKernelGGL(lossKernel, dim3(1), dim3(256), 0, stream, logit, labels, count, loss);
+    hipLaunchKernelGGL(selectorLossKernel, dim3(1), dim3(256), 0, stream, logit, labels, count, loss);
     return hipGetLastError();
 }
 
@@ -211,7 +211,7 @@ hipError_t selectorAdam(float* params, const float* gradient, float* m, float* v
                         hipStream_t stream)
 {
     const float c1 = 1.0f - powf(0.9f, float(step)), c2 = 1.0f - powf(0.999f, float(step));
-    hipLaunchKernelGGL(adamKernel, dim3(groupsFor(SELECTOR_PARAMETERS)), dim3(256), 0, stream, params, gradient, m, v,
+    hipLaunchKernelGGL(selectorAdamKernel, dim3(groupsFor(SELECTOR_PARAMETERS)), dim3(256), 0, stream, params, gradient, m, v,
                        lr, c1, c2);
     return hipGetLastError();
 }

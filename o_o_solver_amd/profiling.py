@@ -1,0 +1,144 @@
+"""Summaries of rocprofv3 output for the benchmark's kernels, and the copy that calibrates the HBM counters.
+
+    python -m o_o_solver_amd.profiling copy
+        A device-to-device copy of 2 GiB (float64), nothing else: run under `rocprofv3 --pmc FETCH_SIZE` and
+        `--pmc WRITE_SIZE` it gives what each counter reads for a known number of bytes.
+    python -m o_o_solver_amd.profiling stats KERNEL_STATS_CSV OUT [--command CMD]
+        The per-kernel table of a `rocprofv3 --kernel-trace --stats -f csv` run (calls, total and average duration).
+    python -m o_o_solver_amd.profiling traffic --fetch CSV --write CSV --copy-fetch CSV --copy-write CSV OUT
+        Per kernel, the average FETCH_SIZE and WRITE_SIZE per dispatch (KiB, as the counters read), each scaled by
+        copy bytes / counter reading of the calibration copy, and their sum: the HBM bytes per launch.
+
+read_traffic(path, kernel) returns the calibrated bytes per launch of one kernel from a traffic summary.
+"""
+
+import argparse
+import csv
+import collections
+import os
+import re
+import sys
+
+COPY_BYTES = 1 << 31  # the calibration copy: 2^28 float64 values read and written once
+
+
+def kernel_name(full):
+    """'void oosg::updateTilesKernel<double>(double*, ...)' → 'updateTilesKernel<double>'."""
+    name = full.strip().replace("(anonymous namespace)::", "")
+    if name.startswith("void "):
+        name = name[5:]
+    depth, end = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            end = i
+            break
+    name = name[:end]
+    return re.sub(r"^(oosg::)+", "", name)
+
+
+def counter_per_kernel(path, counter):
+    """{kernel: [value per dispatch]} of one counter of a counter_collection.csv."""
+    values = collections.defaultdict(list)
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                values[kernel_name(row["Kernel_Name"])].append(float(row["Counter_Value"]))
+    return values
+
+
+def copy_factor(path, counter):
+    """COPY_BYTES / (the counter's reading, in bytes, for the calibration copy's kernel)."""
+    readings = counter_per_kernel(path, counter)
+    copies = [v for k, vs in readings.items() if "copyBuffer" in k for v in vs]
+    if len(copies) != 1:
+        raise ValueError(f"{path}: expected one copy kernel dispatch, found {len(copies)}")
+    return COPY_BYTES / (copies[0] * 1024.0)
+
+
+def stats(csv_path, out, command):
+    with open(csv_path) as f:
+        rows = list(csv.DictReader(f))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = [f"# rocprofv3 kernel statistics", f"# command: {command}", "#",
+             f"# {'kernel':<44} {'calls':>7} {'total ms':>10} {'avg us':>10} {'min us':>9} {'max us':>9} {'share':>6}"]
+    for r in rows:
+        lines.append(f"  {kernel_name(r['Name']):<44} {int(r['Calls']):>7} {float(r['TotalDurationNs']) / 1e6:>10.3f} "
+                     f"{float(r['AverageNs']) / 1e3:>10.2f} {float(r['MinNs']) / 1e3:>9.2f} "
+                     f"{float(r['MaxNs']) / 1e3:>9.2f} {float(r['TotalDurationNs']) / total:>6.3f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def traffic(fetch, write, copy_fetch, copy_write, out, commands):
+    kf = copy_factor(copy_fetch, "FETCH_SIZE")
+    kw = copy_factor(copy_write, "WRITE_SIZE")
+    f = counter_per_kernel(fetch, "FETCH_SIZE")
+    w = counter_per_kernel(write, "WRITE_SIZE")
+    lines = ["# HBM traffic per dispatch: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), each",
+             "# scaled by copy bytes / reading of a plain 2 GiB device copy under the same counter.",
+             f"# calibration: FETCH_SIZE x {kf:.4f}, WRITE_SIZE x {kw:.4f}"]
+    lines += [f"# command: {c}" for c in commands]
+    lines += ["#", f"# {'kernel':<40} {'dispatches':>10} {'fetch KiB':>12} {'write KiB':>12} {'traffic bytes':>16}"]
+    for k in sorted(set(f) & set(w)):
+        fa = sum(f[k]) / len(f[k])
+        wa = sum(w[k]) / len(w[k])
+        bytes_ = (fa * kf + wa * kw) * 1024.0
+        lines.append(f"  {k:<40} {len(f[k]):>10} {fa:>12.1f} {wa:>12.1f} {bytes_:>16.0f}")
+    with open(out, "w") as fo:
+        fo.write("\n".join(lines) + "\n")
+
+
+def read_traffic(path, kernel):
+    """The calibrated HBM bytes per launch of `kernel` in a traffic summary, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            parts = line.split()
+            if len(parts) == 5 and parts[0] == kernel:
+                return float(parts[4])
+    return None
+
+
+def copy():
+    import torch
+    n = COPY_BYTES // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {COPY_BYTES} bytes", flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="what", required=True)
+    sub.add_parser("copy")
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("out")
+    s.add_argument("--command", default="")
+    t = sub.add_parser("traffic")
+    for name in ("fetch", "write", "copy-fetch", "copy-write"):
+        t.add_argument("--" + name, required=True)
+    t.add_argument("--command", action="append", default=[])
+    t.add_argument("out")
+    args = ap.parse_args(argv)
+    if args.what == "copy":
+        copy()
+    elif args.what == "stats":
+        stats(args.csv, args.out, args.command)
+    else:
+        traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.out, args.command)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
