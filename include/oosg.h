@@ -63,10 +63,10 @@ typedef struct oosg_csc_desc {
 
 typedef enum oosg_pivoting {
     OOSG_POSITIVE_DEFINITE = 0, /* PivotingMode::PositiveDefinite (SupernodalFactor.hpp:34-42) */
-    /* PivotingMode::Indefinite, FP64 only. The fronts are eliminated with 1 x 1 pivots in the static order (no
-     * Bunch-Kaufman interchanges, no delayed pivots yet): the inertia is the matrix's own (Sylvester) unless a pivot
-     * breaks down, which is reported as OOSG_SINGULAR; largest_ratio holds max |a_jj / d_j|, the growth near a
-     * breakdown. No ratio test, as in the reference's indefinite mode. */
+    /* PivotingMode::Indefinite, FP64 only: Bunch-Kaufman pivoting among each front's candidates (its columns and
+     * those its children delayed), 1 x 1 and 2 x 2 pivots, candidates that cannot be eliminated delayed to the parent
+     * (MultifrontalDriver.cpp:346-436, DenseBlockKernels.cpp:448-580). The inertia counts 2 x 2 blocks by the sign of
+     * their determinant. No ratio test, as in the reference. A root front that would delay is OOSG_INTERNAL. */
     OOSG_INDEFINITE = 1
 } oosg_pivoting;
 
@@ -148,7 +148,14 @@ oosg_status oosg_factor_info_get(const oosg_factor *factor, oosg_factor_info *in
  * may be null; *eliminated receives e. For tests and diagnostics. */
 oosg_status oosg_factor_panel(const oosg_factor *factor, int32_t supernode, double *values, int32_t *rows,
                               int32_t *eliminated);
-/* D of each equation (n values). */
+/* The shape of that panel: its rows (c + r, or c + d + r with d columns delayed into the front by its children in
+ * indefinite mode) and its eliminated columns e (FactorPanel::rows.size(), eliminatedCount()). */
+oosg_status oosg_factor_panel_shape(const oosg_factor *factor, int32_t supernode, int32_t *row_count,
+                                    int32_t *eliminated);
+/* The subdiagonal of D over that panel's e pivots: e_k of a 2 × 2 block starting at k, 0 elsewhere
+ * (FactorPanel::subdiagonal, SupernodalFactor.hpp:110-125). All zero in positive definite mode. */
+oosg_status oosg_factor_subdiagonal(const oosg_factor *factor, int32_t supernode, double *subdiagonal);
+/* D of each equation (n values): for a 2 × 2 block, its diagonal entries. */
 oosg_status oosg_factor_diagonal(const oosg_factor *factor, double *d);
 
 /* X ← A⁻¹·B in place (FactorSolve::solve, FactorSolve.hpp:77-81): b is n × nrhs column-major, leading dimension

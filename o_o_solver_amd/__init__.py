@@ -428,15 +428,25 @@ class SupernodalFactor:
         return Precision(int(self._info.precision))
 
     def panel(self, s):
-        """(values (c + r) × e, rows) of supernode s, in the reference's FactorPanel layout."""
-        sym = self.plan.symbolic
-        c = int(sym.column_count[s])
-        m = c + sym.row_count(s)
-        values = np.zeros(m * c)
+        """(values m × e, rows) of supernode s, in the reference's FactorPanel layout: m = c + r rows, or c + d + r when
+        its children delayed d columns into its front (indefinite mode); e eliminated columns."""
+        m, e = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.oosg_factor_panel_shape(self.handle, int(s), ctypes.byref(m), ctypes.byref(e)))
+        m, e = m.value, e.value
+        values = np.zeros(m * e)
         rows = np.zeros(m, np.int32)
         eliminated = ctypes.c_int32()
         _check(self._lib.oosg_factor_panel(self.handle, int(s), _ptr(values), _ptr(rows), ctypes.byref(eliminated)))
-        return values.reshape(c, m).T, rows
+        return values.reshape(e, m).T, rows
+
+    def subdiagonal(self, s):
+        """The subdiagonal of D over the e pivots of supernode s (FactorPanel::subdiagonal): e_k of a 2 × 2 block
+        starting at k, 0 elsewhere."""
+        m, e = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.oosg_factor_panel_shape(self.handle, int(s), ctypes.byref(m), ctypes.byref(e)))
+        out = np.zeros(e.value)
+        _check(self._lib.oosg_factor_subdiagonal(self.handle, int(s), _ptr(out)))
+        return out
 
     def diagonal(self):
         d = np.zeros(self.dimension)

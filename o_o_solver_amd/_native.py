@@ -59,6 +59,8 @@ _SIGNATURES = {
     "oosg_factor_info_get": (c_int, [c_void_p, ctypes.POINTER(FactorInfoC)]),
     "oosg_factor_panel": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "oosg_factor_diagonal": (c_int, [c_void_p, c_void_p]),
+    "oosg_factor_panel_shape": (c_int, [c_void_p, c_int32, c_void_p, c_void_p]),
+    "oosg_factor_subdiagonal": (c_int, [c_void_p, c_int32, c_void_p]),
     "oosg_solve": (c_int, [c_void_p, c_void_p, c_int64, c_int64]),
     "oosg_solve_device": (c_int, [c_void_p, c_void_p, c_int64, c_int64]),
     "oosg_solve_refined_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_double,

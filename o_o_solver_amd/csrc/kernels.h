@@ -67,6 +67,43 @@ struct UpdateTile
     int32_t m, n, k, target, lower, mode;
 };
 
+// A pivoted front of the indefinite factorisation (device pointers): F, N × N column-major, positions [own columns c,
+// delayed columns d, rows]; W, N × C, the updated columns (L·D) by position; perm, diag, sub: per candidate position,
+// its index among the candidates before the interchanges, D, and the subdiagonal of a 2 × 2 block (0 otherwise).
+struct PivotFront
+{
+    double* F;
+    double* W;
+    int32_t* perm;
+    double* diag;
+    double* sub;
+    int32_t N, C, c, d, first, pad;
+};
+
+// Where a front's Bunch–Kaufman elimination stands: the next position k, the end of the candidates still to try in
+// this pass, the start of the pass, done (1: e = k), and the 2 × 2 blocks taken.
+struct PivotState
+{
+    int32_t k, end, passStart, done, twoByTwo, pad0, pad1, pad2;
+};
+
+// Columns [j0, j1) of a child's update matrix (M × M at src, leading dimension lds) added into its parent's front
+// (dst, leading dimension ldd) at the positions map[0 .. M).
+struct ExtendItem
+{
+    const double* src;
+    double* dst;
+    const int32_t* map;
+    int32_t lds, ldd, M, j0, j1, pad;
+};
+
+struct CopyRange
+{
+    const double* src;
+    double* dst;
+    int64_t count;
+};
+
 // The values of the inverses of the diagonal blocks of a supernode of c columns (blocks of PANEL_BLOCK, each kb²).
 __host__ __device__ inline int64_t blockInverseValues(int64_t c)
 {
@@ -154,6 +191,19 @@ hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficient
                              hipStream_t stream);
 hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream);
 hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream);
+hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateTile* tiles, int count,
+                                  hipStream_t stream);
+hipError_t launchBunchKaufman(const PivotFront* fronts, PivotState* states, const int32_t* list, int count, int nb,
+                              hipStream_t stream);
+hipError_t launchPivotMatrix(const DeviceMatrix& a, const PivotFront* fronts, int count, hipStream_t stream);
+hipError_t launchPivotExtendAdd(const ExtendItem* items, int count, hipStream_t stream);
+hipError_t launchCopyRanges(const CopyRange* ranges, int count, hipStream_t stream);
+hipError_t launchUnitBlockInverses(const DevicePlan& p, const double* panels, double* linv, const int2* items,
+                                   int count, hipStream_t stream);
+hipError_t launchBlockDiagonalSolve(double* X, const double* d, const double* sub, int64_t n, int k,
+                                    hipStream_t stream);
+hipError_t launchPermuteRows(const double* src, double* dst, const int32_t* perm, int64_t n, int k, bool scatter,
+                             hipStream_t stream);
 hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const float* af, const float* bf, float* df,
                            hipStream_t stream);
 

@@ -551,7 +551,9 @@ __device__ inline int tileOfWorkgroup(int count)
 // Thread t stages row t mod 64 of A and of B for the terms t / 64 + 4u (u < TILE_K / 4) of a chunk; each chunk's loads
 // are issued while the previous chunk is multiplied (registers as the second buffer), from positions clamped into the
 // operands; mode 0 accumulates onto C itself (the accumulators start from the C tile, A enters negated).
-template <typename T>
+// PLAIN: C −= A·Bᵀ with B in the third array (leading dimension ldb) and no diagonal: the trailing updates of the
+// pivoted fronts, whose B holds the rows of L·D already (D has 2 × 2 blocks there).
+template <typename T, bool PLAIN = false>
 __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, const T* linv,
                                                          const UpdateTile* tiles, int count)
 {
@@ -562,12 +564,12 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const bool solve = t.mode == 1;
+    const bool solve = !PLAIN && t.mode == 1;
     const T* A = panels + t.aOff;
-    const T* Bm = (solve ? linv : panels) + t.bOff;
+    const T* Bm = (solve || PLAIN ? linv : panels) + t.bOff;
     const T* dv = panels + t.dOff;
-    const int64_t ld = t.ld, ldb = solve ? t.ldb : t.ld;
-    T* C = (t.target == 0 ? panels : updates) + t.cOff;
+    const int64_t ld = t.ld, ldb = solve || PLAIN ? t.ldb : t.ld;
+    T* C = (PLAIN || t.target == 0 ? panels : updates) + t.cOff;
 
     typename Mfma<T>::Acc acc[2][2];
 #pragma unroll
@@ -593,7 +595,7 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
             const int64_t pp = min(p0 + kq + 4 * u, t.k - 1);
             ra[u] = Ar[pp * ld];
             rb[u] = Br[pp * ldb];
-            rd[u] = solve ? T(1) : dv[pp * (ld + 1)];
+            rd[u] = solve || PLAIN ? T(1) : dv[pp * (ld + 1)];
         }
     };
     load(0);
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
             const bool term = p0 + kk < t.k;
             const T av = pinned(ra[u]), bv = pinned(rb[u]), dd = pinned(rd[u]);
             As[kk][row] = term && row < t.m ? (solve ? av : -av) : T(0);
-            Bs[kk][row] = term && row < t.n ? (solve ? bv : bv * dd) : T(0);
+            Bs[kk][row] = term && row < t.n ? (solve || PLAIN ? bv : bv * dd) : T(0);
         }
         __syncthreads();
         if (p0 + TILE_K < t.k)
@@ -1203,8 +1205,8 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
     __shared__ double ys[SB * KMAX];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
-    if (isWideSupernode(c))
-        return; // backwardColumnsKernel and triangleProductsKernel did it
+    if (isWideSupernode(c) || c == 0)
+        return; // backwardColumnsKernel and triangleProductsKernel did it; or no columns (all delayed)
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
     const int h = accColumn();
@@ -1576,6 +1578,16 @@ hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateT
     if (count == 0)
         return hipSuccess;
     hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, linv, tiles, count);
+    return hipGetLastError();
+}
+
+hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateTile* tiles, int count,
+                                  hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL((updateTilesKernel<double, true>), dim3(count), dim3(256), 0, stream, fronts, nullptr, w,
+                       tiles, count);
     return hipGetLastError();
 }
 

@@ -279,6 +279,7 @@ struct oosg_plan
 {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool ownsStream = true; // false: the stream of another plan (the structure of a pivoted factor)
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
@@ -320,6 +321,12 @@ struct oosg_plan
     DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
     DeviceBuffer diagCache;     // and their pivots
     DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
+    // The indefinite factorisation's working storage, kept for the next one: the fronts of each level, W (the
+    // updated columns of a level's fronts), the candidates' permutations, D and subdiagonals, the front descriptors
+    // and states, and the launch lists.
+    std::vector<DeviceBuffer> pivotFronts;
+    DeviceBuffer pivotW, pivotPerm, pivotDiag, pivotSub, pivotDesc, pivotStates, pivotList, pivotTiles, pivotItems,
+        pivotMaps, pivotRanges, pivotInverseItems;
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
     DeviceBuffer solveW; // the triangle products' results (rows of the wide supernodes, n × k row-major)
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
@@ -349,7 +356,7 @@ struct oosg_plan
             (void)hipEventDestroy(start);
         if (stop)
             (void)hipEventDestroy(stop);
-        if (stream)
+        if (stream && ownsStream)
             (void)hipStreamDestroy(stream);
     }
 };
@@ -372,10 +379,26 @@ struct oosg_matrix
     std::vector<double> scale;
 };
 
+// A factor with Bunch–Kaufman pivoting: its structure after the interchanges and delays, as a plan of its own (for
+// the sweeps and the inverses) whose equations are numbered in pivot order, supernode after supernode.
+struct PivotedFactor
+{
+    std::unique_ptr<oosg_plan> structure;
+    DeviceBuffer perm; // pivot position → equation
+    DeviceBuffer sub;  // the subdiagonal of D by pivot position (0 outside 2 × 2 blocks)
+    std::vector<int32_t> order, eliminated; // per supernode: its front's order N and its e
+    std::vector<int64_t> rowsStart;         // per supernode: the start of its panel rows
+    std::vector<int32_t> panelRows;         // the equations of each front's positions after the interchanges
+    std::vector<int32_t> pivotOf;           // equation → pivot position
+    std::vector<double> diag, sub_;         // D and its subdiagonal by pivot position
+    int64_t delayed = 0, twoByTwo = 0;
+};
+
 struct oosg_factor
 {
     oosg_plan* plan = nullptr;
     int32_t precision = OOSG_FP64;
+    std::unique_ptr<PivotedFactor> pivoted; // indefinite mode
     DeviceBuffer panels;
     DeviceBuffer linv;  // the inverses of the diagonal blocks of PANEL_BLOCK columns (L_JJ⁻¹)
     DeviceBuffer dDiag; // D as the solves divide by it (FP32: of the scaled matrix)
@@ -384,6 +407,8 @@ struct oosg_factor
     oosg_factor_info info{};
     ~oosg_factor()
     {
+        if (pivoted)
+            return; // sized for its own structure
         if (plan != nullptr && panels.ptr != nullptr && plan->panelCache[precision].ptr == nullptr)
             plan->panelCache[precision] = std::move(panels);
         if (plan != nullptr && linv.ptr != nullptr && plan->linvCache[precision].ptr == nullptr)
@@ -407,6 +432,8 @@ void setDevice(const oosg_plan* plan)
 }
 
 // ---- plan --------------------------------------------------------------------------------------------------------
+
+void deriveLevels(oosg_plan& p);
 
 void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
 {
@@ -486,7 +513,14 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
             }
         }
     }
-    // Offsets, levels by height, flops.
+    deriveLevels(p);
+}
+
+// From the structure (first, ncol, nrow, parent, children): the offsets of the panels, update matrices and inverses,
+// the levels by height, the flops, and the sweeps' work lists by level.
+void deriveLevels(oosg_plan& p)
+{
+    const int32_t S = p.S;
     p.panelOff.assign(S + 1, 0);
     p.updateOff.assign(S + 1, 0);
     p.linvOff.assign(S + 1, 0);
@@ -562,8 +596,10 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
             if (wide)
                 maxBlocks = std::max(maxBlocks, (p.ncol[s] + 63) / 64);
             // The L_RF products of every supernode with rows below: row tiles (forward), column blocks (backward).
-            for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
-                p.forwardTiles.push_back(make_int2(s, t));
+            // A supernode of a pivoted factor whose columns were all delayed has none (c = 0).
+            if (p.ncol[s] > 0)
+                for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
+                    p.forwardTiles.push_back(make_int2(s, t));
             if (p.nrow[s] > 0)
                 for (int32_t b = 0; b * 64 < p.ncol[s]; ++b)
                     p.backwardBlocks.push_back(make_int2(s, b));
@@ -820,7 +856,8 @@ void buildSchedule(oosg_plan& p)
     }
 }
 
-void uploadPlan(oosg_plan& p)
+// The arrays the sweeps and the whole inverses read, and the device view of the structure.
+void uploadSweepPlan(oosg_plan& p)
 {
     const hipStream_t st = p.stream;
     p.dFirst = upload(p.first, st, "plan: first columns");
@@ -838,11 +875,6 @@ void uploadPlan(oosg_plan& p)
     p.dBackwardBlocks = upload(p.backwardBlocks, st, "plan: backward column blocks");
     p.dTriangleItems = upload(p.triangleItems, st, "plan: triangle blocks");
     p.dInverseItems = upload(p.inverseItems, st, "plan: triangle inverse blocks");
-    p.dSmallList = upload(p.smallList, st, "plan: small fronts");
-    p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
-    p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
-    p.dItems = upload(p.items, st, "plan: assembly items");
-    p.dTiles = upload(p.tiles, st, "plan: update tiles");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
     p.dp.first = p.dFirst.as<int32_t>();
     p.dp.ncol = p.dNcol.as<int32_t>();
@@ -855,6 +887,17 @@ void uploadPlan(oosg_plan& p)
     p.dp.childStart = p.dChildStart.as<int32_t>();
     p.dp.children = p.dChildren.as<int32_t>();
     p.dp.linvOff = p.dLinvOff.as<int64_t>();
+}
+
+void uploadPlan(oosg_plan& p)
+{
+    uploadSweepPlan(p);
+    const hipStream_t st = p.stream;
+    p.dSmallList = upload(p.smallList, st, "plan: small fronts");
+    p.dStepFronts = upload(p.stepFronts, st, "plan: panel step fronts");
+    p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
+    p.dItems = upload(p.items, st, "plan: assembly items");
+    p.dTiles = upload(p.tiles, st, "plan: update tiles");
     p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
 }
@@ -940,40 +983,14 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
         if ((failedHere || failedBelow[s]) && p.parent[s] >= 0)
             failedBelow[p.parent[s]] = 1;
     }
-    const bool indefinite = options.pivoting == OOSG_INDEFINITE;
-    if (zeroCount > 0 && indefinite)
-        fail(OOSG_SINGULAR, "indefinite factorisation: " + std::to_string(zeroCount) +
-                                " zero (or not finite) pivots met in the static order (1 x 1 pivots, no "
-                                "interchanges), at:\n" +
-                                equationList(zeroPivots, zeroCount) +
-                                "the matrix is singular or needs pivoting; shift the pencil away from an eigenvalue");
     if (zeroCount > 0)
         fail(OOSG_SINGULAR, "the stiffness matrix is singular: " + std::to_string(zeroCount) +
                                 " zero (or not finite) pivots met in the factorisation, a mechanism or an "
                                 "unconstrained DOF at:\n" +
                                 equationList(zeroPivots, zeroCount) +
-                                "constrain these DOFs (single-point constraints) or connect them to the structure");
+                                "constrain these DOFs (single-point constraints) or connect them to the structure; "
+                                "with a matrix that is meant to be indefinite, factorise it in indefinite mode");
     oosg_factor_info& info = f.info;
-    if (indefinite)
-    {
-        // The inertia by the signs of the pivots (Sylvester: that of the matrix when no pivot broke down); the
-        // ratio test belongs to the positive definite mode (MultifrontalDriver.cpp:793-815). largest_ratio reports
-        // the largest |a_jj / d_j|, the growth a pivot close to breaking down shows.
-        info.first_issue_equation = -1;
-        info.largest_ratio_equation = -1;
-        for (int32_t j = 0; j < p.n; ++j)
-        {
-            const double pivot = f.diag[j];
-            ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
-            const double growth = std::fabs(m.diagonal[j] / pivot);
-            if (info.largest_ratio_equation < 0 || growth > info.largest_ratio)
-            {
-                info.largest_ratio = growth;
-                info.largest_ratio_equation = j;
-            }
-        }
-        return;
-    }
     const double maxRatio = options.max_ratio > 0 ? options.max_ratio : 1.0e7;
     double largest = 0.0;
     int32_t largestAt = -1;
@@ -1017,15 +1034,456 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
 
 // ---- solves --------------------------------------------------------------------------------------------------------
 
+// The inertia and pivot counts of a pivoted factor (MultifrontalDriver::finish, MultifrontalDriver.cpp:764-795): 1 × 1
+// pivots by their sign, 2 × 2 blocks by the sign of the determinant, then of the trace. No ratio test: it belongs to
+// the positive definite mode.
+void finishPivoted(oosg_factor& f)
+{
+    const PivotedFactor& pf = *f.pivoted;
+    oosg_factor_info& info = f.info;
+    info.first_issue_equation = -1;
+    info.first_issue_negative = 0;
+    info.largest_ratio = 0.0;
+    info.largest_ratio_equation = -1;
+    const size_t n = pf.diag.size();
+    for (size_t k = 0; k < n;)
+    {
+        const double pivot = pf.diag[k];
+        if (k + 1 < n && pf.sub_[k] != 0.0)
+        {
+            const double second = pf.diag[k + 1];
+            const double determinant = pivot * second - pf.sub_[k] * pf.sub_[k];
+            if (determinant < 0.0)
+            {
+                ++info.positive;
+                ++info.negative;
+            }
+            else if (determinant > 0.0)
+                (pivot > 0.0 ? info.positive : info.negative) += 2;
+            else
+            {
+                ++info.zero;
+                const double trace = pivot + second;
+                ++(trace > 0.0 ? info.positive : trace < 0.0 ? info.negative : info.zero);
+            }
+            k += 2;
+            continue;
+        }
+        ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
+        ++k;
+    }
+    info.delayed_pivots = pf.delayed;
+    info.two_by_two_pivots = pf.twoByTwo;
+}
+
 void ensure(DeviceBuffer& b, size_t bytes, const char* what)
 {
     if (b.bytes < bytes)
         b.allocate(bytes, what);
 }
 
-// X (row-major n × k, k <= 16, in p.solveX) ← factor⁻¹·X.
+// ---- indefinite mode: Bunch–Kaufman fronts -------------------------------------------------------------------------
+
+template <typename T> void uploadInto(DeviceBuffer& b, const std::vector<T>& host, hipStream_t st, const char* what)
+{
+    ensure(b, std::max<size_t>(host.size(), 1) * sizeof(T), what);
+    if (!host.empty())
+        check(hipMemcpyAsync(b.ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, st), what);
+}
+
+// Tiles of C −= A·Bᵀ over the lower part of rows [c0, N) × columns [c0, c1) of a pivoted front (element offsets fOff
+// in the level's fronts and wOff in W), terms t in [t0, t1): A(i, t) = F(i, t), B(j, t) = W(j, t).
+void addPivotTiles(std::vector<oosg::UpdateTile>& tiles, int64_t fOff, int64_t wOff, int N, int c0, int c1, int t0,
+                   int t1)
+{
+    for (int J0 = c0; J0 < c1; J0 += 64)
+        for (int I0 = J0; I0 < N; I0 += 64)
+        {
+            oosg::UpdateTile t{};
+            t.aOff = fOff + I0 + int64_t(t0) * N;
+            t.bOff = wOff + J0 + int64_t(t0) * N;
+            t.cOff = fOff + I0 + int64_t(J0) * N;
+            t.ld = N;
+            t.ldb = N;
+            t.ldc = N;
+            t.m = std::min(64, N - I0);
+            t.n = std::min(64, c1 - J0);
+            t.k = t1 - t0;
+            t.lower = I0 == J0;
+            tiles.push_back(t);
+        }
+}
+
+// The indefinite factorisation (MultifrontalDriver::eliminatePivoted, MultifrontalDriver.cpp:346-436), level by level:
+// each front [own columns, the columns its children delayed (in child order), rows] is assembled, eliminated with
+// Bunch–Kaufman pivoting among its candidates in blocks of PANEL_BLOCK columns, the candidates it cannot eliminate
+// delayed to its parent; a root front cannot delay. The host reads each front's state after every block step (the
+// trailing tiles depend on how far it got) and its permutation after the level. Then the panels (the first e columns
+// of each front) are packed, the structure in pivot order is built for the sweeps, and the inverses of the unit
+// lower triangles are formed.
+void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
+{
+    const hipStream_t st = p.stream;
+    const int32_t S = p.S;
+    auto pf = std::make_unique<PivotedFactor>();
+    pf->order.assign(S, 0);
+    pf->eliminated.assign(S, 0);
+    std::vector<int32_t> cand(S, 0), delayedIn(S, 0);
+    std::vector<std::vector<int32_t>> frontRows(S);
+    std::vector<double*> frontAt(S, nullptr);
+    std::vector<std::vector<double>> dOf(S), sOf(S);
+    const oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
+                               m.dPosition.as<int32_t>(), m.dColumn.as<int32_t>(), nullptr};
+    if (static_cast<int32_t>(p.pivotFronts.size()) < p.levels)
+        p.pivotFronts.resize(p.levels);
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        const int32_t* list = p.levelList.data() + p.levelStart[l];
+        const int count = p.levelStart[l + 1] - p.levelStart[l];
+        std::vector<int64_t> fOff(count), wOff(count), cOff(count);
+        int64_t fTotal = 0, wTotal = 0, cTotal = 0;
+        int maxChildren = 0;
+        for (int q = 0; q < count; ++q)
+        {
+            const int32_t s = list[q];
+            int32_t d = 0;
+            for (int32_t x = p.childStart[s]; x < p.childStart[s + 1]; ++x)
+                d += cand[p.children[x]] - pf->eliminated[p.children[x]];
+            maxChildren = std::max(maxChildren, p.childStart[s + 1] - p.childStart[s]);
+            delayedIn[s] = d;
+            cand[s] = p.ncol[s] + d;
+            const int32_t N = cand[s] + p.nrow[s];
+            pf->order[s] = N;
+            fOff[q] = fTotal;
+            fTotal += int64_t(N) * N;
+            wOff[q] = wTotal;
+            wTotal += int64_t(N) * cand[s];
+            cOff[q] = cTotal;
+            cTotal += cand[s];
+        }
+        ensure(p.pivotFronts[l], static_cast<size_t>(std::max<int64_t>(fTotal, 1)) * sizeof(double),
+               "factor: pivoted fronts");
+        ensure(p.pivotW, static_cast<size_t>(std::max<int64_t>(wTotal, 1)) * sizeof(double), "factor: W");
+        ensure(p.pivotPerm, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(int32_t), "factor: pivots");
+        ensure(p.pivotDiag, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
+        ensure(p.pivotSub, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
+        double* F0 = p.pivotFronts[l].as<double>();
+        double* W0 = p.pivotW.as<double>();
+        std::vector<oosg::PivotFront> fronts(count);
+        std::vector<oosg::PivotState> states(count);
+        std::vector<int32_t> perm0(static_cast<size_t>(cTotal));
+        for (int q = 0; q < count; ++q)
+        {
+            const int32_t s = list[q];
+            oosg::PivotFront& fr = fronts[q];
+            fr.F = F0 + fOff[q];
+            fr.W = W0 + wOff[q];
+            fr.perm = p.pivotPerm.as<int32_t>() + cOff[q];
+            fr.diag = p.pivotDiag.as<double>() + cOff[q];
+            fr.sub = p.pivotSub.as<double>() + cOff[q];
+            fr.N = pf->order[s];
+            fr.C = cand[s];
+            fr.c = p.ncol[s];
+            fr.d = delayedIn[s];
+            fr.first = p.first[s];
+            frontAt[s] = fr.F;
+            states[q] = oosg::PivotState{0, cand[s], 0, 0, 0, 0, 0, 0};
+            for (int32_t i = 0; i < cand[s]; ++i)
+                perm0[cOff[q] + i] = i;
+        }
+        uploadInto(p.pivotDesc, fronts, st, "factor: fronts");
+        uploadInto(p.pivotStates, states, st, "factor: front states");
+        check(hipMemcpyAsync(p.pivotPerm.ptr, perm0.data(), perm0.size() * sizeof(int32_t), hipMemcpyHostToDevice, st),
+              "factor: pivots");
+        const oosg::PivotFront* desc = p.pivotDesc.as<oosg::PivotFront>();
+        oosg::PivotState* dStates = p.pivotStates.as<oosg::PivotState>();
+        check(oosg::launchFillDouble(F0, fTotal, fTotal, 1, 0.0, st), "factor: fronts");
+        check(oosg::launchPivotMatrix(a, desc, count, st), "factor: matrix entries");
+        // The children's update matrices, the first child of every front, then the second, and so on.
+        std::vector<int32_t> maps;
+        std::vector<oosg::ExtendItem> items;
+        std::vector<int64_t> itemMap;
+        std::vector<int> rankStart(static_cast<size_t>(maxChildren) + 1, 0);
+        for (int rank = 0; rank < maxChildren; ++rank)
+        {
+            rankStart[rank] = static_cast<int>(items.size());
+            for (int q = 0; q < count; ++q)
+            {
+                const int32_t s = list[q];
+                if (p.childStart[s] + rank >= p.childStart[s + 1])
+                    continue;
+                int32_t delayedStart = p.ncol[s];
+                for (int32_t x = p.childStart[s]; x < p.childStart[s] + rank; ++x)
+                    delayedStart += cand[p.children[x]] - pf->eliminated[p.children[x]];
+                const int32_t ch = p.children[p.childStart[s] + rank];
+                const int32_t Nc = pf->order[ch], ec = pf->eliminated[ch], M = Nc - ec;
+                if (M == 0)
+                    continue;
+                const int64_t mapOff = static_cast<int64_t>(maps.size());
+                for (int32_t i = 0; i < cand[ch] - ec; ++i)
+                    maps.push_back(delayedStart + i);
+                for (int32_t i = 0; i < p.nrow[ch]; ++i)
+                {
+                    const int32_t rel = p.rel[p.rowStart[ch] + i];
+                    maps.push_back(rel < p.ncol[s] ? rel : rel + delayedIn[s]);
+                }
+                const double* src = frontAt[ch] + ec + int64_t(ec) * Nc;
+                for (int32_t j0 = 0; j0 < M;)
+                {
+                    const int32_t j1 = std::min(M, j0 + std::max(1, 16384 / (M - j0)));
+                    items.push_back(oosg::ExtendItem{src, frontAt[s], nullptr, Nc, pf->order[s], M, j0, j1, 0});
+                    itemMap.push_back(mapOff);
+                    j0 = j1;
+                }
+            }
+        }
+        rankStart[maxChildren] = static_cast<int>(items.size());
+        if (!items.empty())
+        {
+            uploadInto(p.pivotMaps, maps, st, "factor: extend-add positions");
+            for (size_t x = 0; x < items.size(); ++x)
+                items[x].map = p.pivotMaps.as<int32_t>() + itemMap[x];
+            uploadInto(p.pivotItems, items, st, "factor: extend-add items");
+            for (int rank = 0; rank < maxChildren; ++rank)
+                check(oosg::launchPivotExtendAdd(p.pivotItems.as<oosg::ExtendItem>() + rankStart[rank],
+                                                 rankStart[rank + 1] - rankStart[rank], st),
+                      "factor: extend-add");
+        }
+        // Block steps until every front of the level is done, each followed by its trailing tiles.
+        std::vector<int32_t> active(count), next;
+        for (int q = 0; q < count; ++q)
+            active[q] = q;
+        std::vector<int32_t> kNow(count, 0);
+        std::vector<oosg::UpdateTile> tiles;
+        // Every step of a front eliminates PANEL_BLOCK columns or ends a pass, and a pass that eliminates nothing
+        // ends the front: this bounds the steps.
+        int32_t maxCand = 0;
+        for (int q = 0; q < count; ++q)
+            maxCand = std::max(maxCand, cand[list[q]]);
+        int64_t stepsLeft = 2 * int64_t(maxCand) + 8;
+        while (!active.empty())
+        {
+            if (--stepsLeft < 0)
+                fail(OOSG_INTERNAL, "factor: the Bunch-Kaufman steps of a level did not end");
+            uploadInto(p.pivotList, active, st, "factor: active fronts");
+            check(oosg::launchBunchKaufman(desc, dStates, p.pivotList.as<int32_t>(), static_cast<int>(active.size()),
+                                           oosg::PANEL_BLOCK, st),
+                  "factor: Bunch-Kaufman");
+            check(hipMemcpyAsync(states.data(), dStates, states.size() * sizeof(oosg::PivotState),
+                                 hipMemcpyDeviceToHost, st),
+                  "factor: front states");
+            check(hipStreamSynchronize(st), "factor: Bunch-Kaufman");
+            tiles.clear();
+            next.clear();
+            for (const int32_t q : active)
+            {
+                const int32_t s = list[q];
+                const int kb = kNow[q], ka = states[q].k;
+                if (ka < kb || ka > cand[s])
+                    fail(OOSG_INTERNAL, "factor: a front's elimination went back");
+                if (ka > kb)
+                    addPivotTiles(tiles, fOff[q], wOff[q], pf->order[s], ka, cand[s], kb, ka);
+                kNow[q] = ka;
+                if (!states[q].done)
+                    next.push_back(q);
+            }
+            if (!tiles.empty())
+            {
+                uploadInto(p.pivotTiles, tiles, st, "factor: trailing tiles");
+                check(oosg::launchPlainUpdateTiles(F0, W0, p.pivotTiles.as<oosg::UpdateTile>(),
+                                                   static_cast<int>(tiles.size()), st),
+                      "factor: trailing tiles");
+            }
+            active.swap(next);
+        }
+        // The rows below the candidates: their update matrix part −= L·Wᵀ over the eliminated columns.
+        tiles.clear();
+        for (int q = 0; q < count; ++q)
+        {
+            const int32_t s = list[q];
+            const int e = states[q].k, N = pf->order[s];
+            if (e > 0 && N > cand[s])
+                addPivotTiles(tiles, fOff[q], wOff[q], N, cand[s], N, 0, e);
+        }
+        if (!tiles.empty())
+        {
+            uploadInto(p.pivotTiles, tiles, st, "factor: update tiles");
+            check(oosg::launchPlainUpdateTiles(F0, W0, p.pivotTiles.as<oosg::UpdateTile>(),
+                                               static_cast<int>(tiles.size()), st),
+                  "factor: update tiles");
+        }
+        std::vector<int32_t> permH(static_cast<size_t>(cTotal));
+        std::vector<double> dH(static_cast<size_t>(cTotal)), sH(static_cast<size_t>(cTotal));
+        check(hipMemcpyAsync(permH.data(), p.pivotPerm.ptr, permH.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+              "factor: pivots");
+        check(hipMemcpyAsync(dH.data(), p.pivotDiag.ptr, dH.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+              "factor: pivots");
+        check(hipMemcpyAsync(sH.data(), p.pivotSub.ptr, sH.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+              "factor: pivots");
+        check(hipStreamSynchronize(st), "factor: pivots");
+        for (int q = 0; q < count; ++q)
+        {
+            const int32_t s = list[q];
+            const int32_t e = states[q].k, C = cand[s], N = pf->order[s];
+            if (p.parent[s] < 0 && e < C)
+                fail(OOSG_INTERNAL, "supernodal factorisation: the root front " + std::to_string(s) + " delayed " +
+                                        std::to_string(C - e) + " pivots, which a root front cannot do");
+            // The equations of the front's positions before the interchanges, then after them.
+            std::vector<int32_t> before;
+            before.reserve(static_cast<size_t>(N));
+            for (int32_t i = 0; i < p.ncol[s]; ++i)
+                before.push_back(p.first[s] + i);
+            for (int32_t x = p.childStart[s]; x < p.childStart[s + 1]; ++x)
+            {
+                const int32_t ch = p.children[x];
+                before.insert(before.end(), frontRows[ch].begin() + pf->eliminated[ch],
+                              frontRows[ch].begin() + cand[ch]);
+            }
+            before.insert(before.end(), p.rows.begin() + p.rowStart[s], p.rows.begin() + p.rowStart[s + 1]);
+            std::vector<int32_t>& rows = frontRows[s];
+            rows.resize(static_cast<size_t>(N));
+            for (int32_t k = 0; k < C; ++k)
+            {
+                const int32_t from = permH[cOff[q] + k];
+                if (from < 0 || from >= C)
+                    fail(OOSG_INTERNAL, "factor: a candidate permutation is out of range");
+                rows[k] = before[from];
+            }
+            for (int32_t k = C; k < N; ++k)
+                rows[k] = before[k];
+            dOf[s].assign(dH.begin() + cOff[q], dH.begin() + cOff[q] + e);
+            sOf[s].assign(sH.begin() + cOff[q], sH.begin() + cOff[q] + e);
+            pf->eliminated[s] = e;
+            pf->twoByTwo += states[q].twoByTwo;
+            pf->delayed += C - e;
+        }
+    }
+
+    // The panels: the first e columns of each front, one contiguous range each.
+    const int32_t n = p.n;
+    std::vector<int64_t> panelStart(S + 1, 0);
+    for (int32_t s = 0; s < S; ++s)
+        panelStart[s + 1] = panelStart[s] + int64_t(pf->order[s]) * pf->eliminated[s];
+    f.panels.allocate(static_cast<size_t>(std::max<int64_t>(panelStart[S], 1)) * sizeof(double), "factor: panels");
+    std::vector<oosg::CopyRange> ranges;
+    for (int32_t s = 0; s < S; ++s)
+        for (int64_t q0 = 0; q0 < panelStart[s + 1] - panelStart[s]; q0 += 65536)
+            ranges.push_back(oosg::CopyRange{frontAt[s] + q0, f.panels.as<double>() + panelStart[s] + q0,
+                                             std::min<int64_t>(65536, panelStart[s + 1] - panelStart[s] - q0)});
+    uploadInto(p.pivotRanges, ranges, st, "factor: panel copies");
+    check(oosg::launchCopyRanges(p.pivotRanges.as<oosg::CopyRange>(), static_cast<int>(ranges.size()), st),
+          "factor: panels");
+
+    // Pivot order: the equations eliminated by supernode s get first'(s) .. first'(s) + e − 1.
+    auto d = std::make_unique<oosg_plan>();
+    d->device = p.device;
+    d->stream = st;
+    d->ownsStream = false;
+    d->graphs = false;
+    d->n = n;
+    d->S = S;
+    d->first.assign(S, 0);
+    d->ncol = pf->eliminated;
+    d->nrow.assign(S, 0);
+    d->parent = p.parent;
+    d->childStart = p.childStart;
+    d->children = p.children;
+    pf->pivotOf.assign(static_cast<size_t>(n), -1);
+    std::vector<int32_t> permOf(static_cast<size_t>(n), -1);
+    pf->diag.assign(static_cast<size_t>(n), 0.0);
+    pf->sub_.assign(static_cast<size_t>(n), 0.0);
+    pf->rowsStart.assign(S + 1, 0);
+    int32_t next = 0;
+    for (int32_t s = 0; s < S; ++s)
+    {
+        d->first[s] = next;
+        d->nrow[s] = pf->order[s] - pf->eliminated[s];
+        for (int32_t k = 0; k < pf->eliminated[s]; ++k)
+        {
+            const int32_t eq = frontRows[s][k];
+            if (eq < 0 || eq >= n || pf->pivotOf[eq] >= 0)
+                fail(OOSG_INTERNAL, "factor: an equation was eliminated twice");
+            pf->pivotOf[eq] = next;
+            permOf[next] = eq;
+            pf->diag[next] = dOf[s][k];
+            pf->sub_[next] = sOf[s][k];
+            ++next;
+        }
+        pf->rowsStart[s + 1] = pf->rowsStart[s] + pf->order[s];
+    }
+    if (next != n)
+        fail(OOSG_INTERNAL, "factor: " + std::to_string(n - next) + " equations were not eliminated");
+    pf->panelRows.reserve(static_cast<size_t>(pf->rowsStart[S]));
+    for (int32_t s = 0; s < S; ++s)
+        pf->panelRows.insert(pf->panelRows.end(), frontRows[s].begin(), frontRows[s].end());
+    d->rowStart.assign(S + 1, 0);
+    for (int32_t s = 0; s < S; ++s)
+        d->rowStart[s + 1] = d->rowStart[s] + d->nrow[s];
+    d->rows.resize(static_cast<size_t>(d->rowStart[S]));
+    d->rel.assign(d->rows.size(), 0);
+    for (int32_t s = 0; s < S; ++s)
+        for (int32_t i = 0; i < d->nrow[s]; ++i)
+            d->rows[d->rowStart[s] + i] = pf->pivotOf[frontRows[s][pf->eliminated[s] + i]];
+    {
+        // Each child's rows → positions in its parent's panel rows.
+        std::vector<int32_t> posOf(static_cast<size_t>(n), -1);
+        for (int32_t s = 0; s < S; ++s)
+        {
+            if (p.childStart[s] == p.childStart[s + 1])
+                continue;
+            for (int32_t k = 0; k < pf->order[s]; ++k)
+                posOf[frontRows[s][k]] = k;
+            for (int32_t x = p.childStart[s]; x < p.childStart[s + 1]; ++x)
+            {
+                const int32_t ch = p.children[x];
+                for (int32_t i = 0; i < d->nrow[ch]; ++i)
+                {
+                    const int32_t at = posOf[frontRows[ch][pf->eliminated[ch] + i]];
+                    if (at < 0)
+                        fail(OOSG_INTERNAL, "factor: a row of a child is not in its parent's front");
+                    d->rel[d->rowStart[ch] + i] = at;
+                }
+            }
+            for (int32_t k = 0; k < pf->order[s]; ++k)
+                posOf[frontRows[s][k]] = -1;
+        }
+    }
+    for (int32_t s = 0; s < S; ++s)
+        if (d->parent[s] < 0 && d->nrow[s] > 0)
+            fail(OOSG_INTERNAL, "factor: a root front kept rows");
+    deriveLevels(*d);
+    uploadSweepPlan(*d);
+    // The inverses of the unit lower triangles: the diagonal blocks, then the wide triangles' whole inverses.
+    f.linv.allocate(static_cast<size_t>(std::max<int64_t>(d->linvOff[S], 1)) * sizeof(double), "factor: inverses");
+    std::vector<int2> blockItems;
+    for (int32_t s = 0; s < S; ++s)
+        for (int32_t J = 0; J * oosg::PANEL_BLOCK < d->ncol[s]; ++J)
+            blockItems.push_back(make_int2(s, J));
+    uploadInto(p.pivotInverseItems, blockItems, st, "factor: inverse blocks");
+    check(oosg::launchUnitBlockInverses(d->dp, f.panels.as<double>(), f.linv.as<double>(),
+                                        p.pivotInverseItems.as<int2>(), static_cast<int>(blockItems.size()), st),
+          "factor: inverses");
+    for (int32_t l = 0; l < d->levels; ++l)
+        for (const oosg_plan::InverseRound& round : d->inverseRounds[l])
+            check(oosg::launchTriangleInverse<double>(d->dp, f.panels.as<double>(), f.linv.as<double>(),
+                                                      d->dInverseItems.as<int2>() + round.offset, round.count,
+                                                      round.step, st),
+                  "factor: triangle inverses");
+    f.dDiag = upload(pf->diag, st, "factor: pivots");
+    pf->sub = upload(pf->sub_, st, "factor: pivots");
+    pf->perm = upload(permOf, st, "factor: pivot order");
+    f.diag.assign(static_cast<size_t>(n), 0.0);
+    for (int32_t eq = 0; eq < n; ++eq)
+        f.diag[eq] = pf->diag[pf->pivotOf[eq]];
+    check(hipStreamSynchronize(st), "factor");
+    pf->structure = std::move(d);
+    f.pivoted = std::move(pf);
+}
+
+// X (row-major n × k, k <= 16, in p.solveX) ← factor⁻¹·X. sub: the subdiagonal of a block diagonal D (null: D is
+// diagonal).
 template <typename TP>
-void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k)
+void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k, const double* sub = nullptr)
 {
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
@@ -1073,7 +1531,10 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
                       "solve: forward sweep (rows)");
             });
     }
-    check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
+    if (sub != nullptr)
+        check(oosg::launchBlockDiagonalSolve(X, diag, sub, p.n, k, st), "solve: block diagonal");
+    else
+        check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
     for (int32_t l = p.levels - 1; l >= 0; --l)
     {
         const oosg_plan::SweepWork& w = p.sweepWork[l];
@@ -1109,6 +1570,24 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
 void sweeps(const oosg_factor& f, int k)
 {
     oosg_plan& p = *f.plan;
+    if (f.pivoted)
+    {
+        // In pivot order: X permuted into the structure's own X, swept, and permuted back.
+        oosg_plan& d = *f.pivoted->structure;
+        const int64_t n = p.n;
+        ensure(d.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
+        ensure(d.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
+        ensure(d.solveU, static_cast<size_t>(std::max<int64_t>(d.rowStart[d.S], 1) * 16) * sizeof(double),
+               "solve: update vectors");
+        const int32_t* perm = f.pivoted->perm.as<int32_t>();
+        check(oosg::launchPermuteRows(p.solveX.as<double>(), d.solveX.as<double>(), perm, n, k, false, p.stream),
+              "solve: pivot order");
+        runSweeps<double>(d, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k,
+                          f.pivoted->sub.as<double>());
+        check(oosg::launchPermuteRows(d.solveX.as<double>(), p.solveX.as<double>(), perm, n, k, true, p.stream),
+              "solve: pivot order");
+        return;
+    }
     const auto body = [&]
     {
         if (f.precision == OOSG_FP32)
@@ -2023,6 +2502,26 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             if (opts.pivoting == OOSG_INDEFINITE && opts.precision != OOSG_FP64)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: indefinite factors are FP64 (their inertia decides Sturm "
                                             "counts)");
+            if (opts.pivoting == OOSG_INDEFINITE)
+            {
+                setDevice(plan);
+                auto f = std::make_unique<oosg_factor>();
+                f->plan = plan;
+                f->precision = OOSG_FP64;
+                const hipStream_t st = plan->stream;
+                check(hipEventRecord(plan->start, st), "hipEventRecord");
+                factorisePivoted(*plan, *matrix, *f);
+                check(hipEventRecord(plan->stop, st), "hipEventRecord");
+                check(hipStreamSynchronize(st), "factor");
+                float ms = 0.0f;
+                check(hipEventElapsedTime(&ms, plan->start, plan->stop), "hipEventElapsedTime");
+                f->info.factor_seconds = ms * 1e-3;
+                f->info.flops = plan->flops;
+                f->info.precision = OOSG_FP64;
+                finishPivoted(*f);
+                *out = f.release();
+                return;
+            }
             if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: unknown precision");
             setDevice(plan);
@@ -2126,6 +2625,23 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
             const oosg_plan& p = *f->plan;
             if (s < 0 || s >= p.S)
                 fail(OOSG_INVALID_ARGUMENT, "panel: no supernode " + std::to_string(s));
+            if (f->pivoted)
+            {
+                const PivotedFactor& pf = *f->pivoted;
+                const int32_t e = pf.eliminated[s], N = pf.order[s];
+                if (eliminated != nullptr)
+                    *eliminated = e;
+                if (rows != nullptr)
+                    std::copy(pf.panelRows.begin() + pf.rowsStart[s], pf.panelRows.begin() + pf.rowsStart[s + 1], rows);
+                if (values != nullptr && e > 0)
+                {
+                    setDevice(&p);
+                    check(hipMemcpy(values, f->panels.as<double>() + pf.structure->panelOff[s],
+                                    static_cast<size_t>(N) * e * sizeof(double), hipMemcpyDeviceToHost),
+                          "panel: copy");
+                }
+                return;
+            }
             const int32_t c = p.ncol[s], r = p.nrow[s];
             const size_t count = static_cast<size_t>(c) * (c + r);
             if (eliminated != nullptr)
@@ -2151,6 +2667,44 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
                 check(hipMemcpy(values, f->panels.as<double>() + p.panelOff[s], count * sizeof(double),
                                 hipMemcpyDeviceToHost),
                       "panel: copy");
+        });
+}
+
+oosg_status oosg_factor_panel_shape(const oosg_factor* f, int32_t s, int32_t* row_count, int32_t* eliminated)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "panel: null factor");
+            const oosg_plan& p = *f->plan;
+            if (s < 0 || s >= p.S)
+                fail(OOSG_INVALID_ARGUMENT, "panel: no supernode " + std::to_string(s));
+            if (row_count != nullptr)
+                *row_count = f->pivoted ? f->pivoted->order[s] : p.ncol[s] + p.nrow[s];
+            if (eliminated != nullptr)
+                *eliminated = f->pivoted ? f->pivoted->eliminated[s] : p.ncol[s];
+        });
+}
+
+oosg_status oosg_factor_subdiagonal(const oosg_factor* f, int32_t s, double* subdiagonal)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || subdiagonal == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "panel: null argument");
+            const oosg_plan& p = *f->plan;
+            if (s < 0 || s >= p.S)
+                fail(OOSG_INVALID_ARGUMENT, "panel: no supernode " + std::to_string(s));
+            if (!f->pivoted)
+            {
+                std::fill(subdiagonal, subdiagonal + p.ncol[s], 0.0);
+                return;
+            }
+            const PivotedFactor& pf = *f->pivoted;
+            const int32_t first = pf.structure->first[s];
+            std::copy(pf.sub_.begin() + first, pf.sub_.begin() + first + pf.eliminated[s], subdiagonal);
         });
 }
 

@@ -1,6 +1,5 @@
-"""The indefinite factor on the GPU (1 × 1 pivots in the static order) and the Sturm counts of shifted pencils,
-pinned by the reference's Bunch-Kaufman inertia (golden fixture), by closed-form eigenvalues and by dense
-generalized eigenvalues."""
+"""The indefinite factor on the GPU (Bunch-Kaufman pivots) and the Sturm counts of shifted pencils, pinned by the
+reference's inertia (golden fixture), by closed-form eigenvalues and by dense generalized eigenvalues."""
 
 import numpy as np
 import pytest
@@ -15,8 +14,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_golden_grid_indefinite_inertia_matches_bunch_kaufman(gpu_device):
-    # The reference factorised this matrix (diagonal 3.3, −1 couplings) with Bunch-Kaufman pivots; its inertia is
-    # the matrix's own, which the static-order factor must find, and the solutions agree.
+    # The reference factorised this matrix (diagonal 3.3, −1 couplings) with Bunch-Kaufman pivots; the GPU factor
+    # finds its inertia, and the solutions agree.
     g = np.load(__file__.rsplit("/", 1)[0] + "/golden/grid20.npz")
     sym = SymbolicFactor(400, g["first_column"], g["column_count"], g["parent"], g["row_start"], g["rows"],
                          node_order=g["node_order"])
@@ -61,10 +60,11 @@ def test_chain_laplacian_sturm_counts_are_closed_form(gpu_device):
     # The eigenvalues between the lowest and the highest Sturm point are all accounted for.
     v = pencil.sturm.verify(eigenvalues)
     assert not v.has_mismatch and v.verified_count == 299 - 3
-    # σ = 2 (the midpoint of λ150 and λ151) makes every diagonal term zero: 1 × 1 pivots in the static order break
-    # down at once, which the factor reports rather than returning a count.
-    with pytest.raises(Exception, match="static order"):
-        pencil.factorise(2.0)
+    # σ = 2 (the midpoint of λ150 and λ151) makes every diagonal term zero: no 1 × 1 pivot in the static order would
+    # do, 2 × 2 blocks and interchanges find the count.
+    f = pencil.factorise(2.0)
+    assert f.count_below == 150
+    assert f.factor.two_by_two_pivot_count > 0
 
 
 def test_plate_pencil_counts_match_dense_eigenvalues(gpu_device):
