@@ -37,7 +37,7 @@ KIND_KERNELS = {"small_fronts": "smallFrontKernel", "assembly": "assembleKernel"
                 "forward_triangles": "forwardKernel", "backward_triangles": "backwardKernel",
                 "residual": "multiplyRowsKernel", "forward_rows": "forwardRowsKernel",
                 "backward_columns": "backwardColumnsKernel", "triangle_products": "triangleProductsKernel",
-                "triangle_inverses": "triangleInverseKernel"}
+                "triangle_inverses": "triangleInverseKernel", "update_matrix_big_tiles": "bigTilesKernel"}
 # Profile kinds whose work is bytes (the sweeps and the residual products); the others count flops.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_shell_plate_c2.txt")
 BYTE_KINDS = ("forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
@@ -130,6 +130,14 @@ def main():
     torch.cuda.synchronize()
     stats = plan.stats()
     log(f"plan + upload ({time.perf_counter() - t:.1f} s): {stats}")
+
+    # The autotuner picks the update-matrix tile size per level by timing both on this matrix (setup, untimed).
+    t = time.perf_counter()
+    tuning = {}
+    for precision in (Precision.FP64, Precision.FP32):
+        big, tuned = plan.autotune(dm, precision)
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned}
+    log(f"autotune ({time.perf_counter() - t:.1f} s): {tuning}")
 
     from o_o_solver_amd.selector import PrecisionSelector
     selector = PrecisionSelector.trained()
@@ -277,6 +285,7 @@ def main():
                            "mfma_frac": round(flops / fp32_factor / 1e12 / PEAK_FP32, 4)},
         "speedup_mixed_vs_fp64": round(fp64_s / mixed_s, 3),
         "roofline": roofline,
+        "autotune": tuning,
         "kernels_fp32_fronts_cg": kernels,
         "kernels_fp64": breakdown(fp64_prof),
         "cpu_baseline": baseline,

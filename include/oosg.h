@@ -185,10 +185,11 @@ oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, c
  * diagonal block), 4 panel updates (MFMA tiles), 5 update matrices (MFMA tiles), 6 forward sweep gathers and
  * triangles, 7 backward sweep triangles, 8 residual products, 9 forward sweep row tiles (L_RF), 10 backward sweep
  * column blocks (L_RFᵀ), 11 products with the wide triangles' whole inverses (both sweeps), 12 the whole inverses
- * (factorisation). work[k] is the algorithmic work of the launches timed: flops for kinds 0-5 and 12; bytes for
- * kinds 6-11 (the values of L or of an inverse read once, and the right-hand-side rows read or written once; the
- * matrix and the vectors of a residual). Arrays of OOSG_PROFILE_KINDS values; any may be null. */
-#define OOSG_PROFILE_KINDS 13
+ * (factorisation), 13 update matrices on 128 x 128 tiles. work[k] is the algorithmic work of the launches timed:
+ * flops for kinds 0-5, 12 and 13; bytes for kinds 6-11 (the values of L or of an inverse read once, and the
+ * right-hand-side rows read or written once; the matrix and the vectors of a residual). Arrays of OOSG_PROFILE_KINDS
+ * values; any may be null. */
+#define OOSG_PROFILE_KINDS 14
 oosg_status oosg_plan_profile(oosg_plan *plan, int32_t enable);
 oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64_t *launches, double *work);
 
@@ -196,6 +197,17 @@ oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64
  * captured once per set of buffers and replayed by one graph launch (the role of the reference's reusable solve plan,
  * SolvePlan.hpp:36-144, on the device side). Off: every kernel is launched eagerly. The profiler launches eagerly. */
 oosg_status oosg_plan_use_graphs(oosg_plan *plan, int32_t enable);
+
+/* The tile size of the update-matrix products (no counterpart in the reference): 0 64 x 64 tiles everywhere, 1
+ * 128 x 128 tiles everywhere, 2 (default) by level, as oosg_plan_autotune measured, or before any measurement big
+ * tiles where a level has at least 2048 of them. Both sizes give the same bits (the same k-steps of the same MFMA). */
+oosg_status oosg_plan_tile_policy(oosg_plan *plan, int32_t policy);
+/* The autotuner (no counterpart in the reference): factorises `matrix` (of the plan's structure) twice with each tile
+ * size in `precision`, times every update-matrix launch with HIP events, and keeps per level the faster size (policy
+ * 2). *big_launches receives the launches that chose 128 x 128 tiles, *tuned_launches those measured (either may be
+ * null). The factors are discarded; the captured graphs of that precision are rebuilt on the next factorisation. */
+oosg_status oosg_plan_autotune(oosg_plan *plan, const oosg_matrix *matrix, int32_t precision, int32_t *big_launches,
+                               int32_t *tuned_launches);
 
 /* Blocks until the plan's stream is idle. */
 oosg_status oosg_plan_synchronize(oosg_plan *plan);

@@ -267,7 +267,7 @@ class Plan:
 
     PROFILE_KINDS = ["small_fronts", "assembly", "diagonal_blocks", "panel_solve", "panel_update_tiles",
                      "update_matrix_tiles", "forward_triangles", "backward_triangles", "residual", "forward_rows",
-                     "backward_columns", "triangle_products", "triangle_inverses"]
+                     "backward_columns", "triangle_products", "triangle_inverses", "update_matrix_big_tiles"]
 
     def profile(self, enable=True):
         """Per-kernel device times by HIP events on the plan's stream (enabling resets the sums)."""
@@ -293,6 +293,19 @@ class Plan:
     def use_graphs(self, enable=True):
         """Replays of the factorisation's and the sweeps' launch sequences as HIP graphs (default on)."""
         _check(self._lib.oosg_plan_use_graphs(self.handle, 1 if enable else 0))
+
+    def tile_policy(self, policy):
+        """The update-matrix tile size: 0 64 × 64, 1 128 × 128, 2 by level (the autotuner's choice; the default)."""
+        _check(self._lib.oosg_plan_tile_policy(self.handle, int(policy)))
+
+    def autotune(self, device_matrix, precision=None):
+        """Times both tile sizes of every update-matrix launch on `device_matrix` (a matrix of this plan) and keeps
+        the faster per level. Returns (launches that chose 128 × 128 tiles, launches measured)."""
+        precision = Precision.FP64 if precision is None else Precision(precision)
+        big, tuned = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.oosg_plan_autotune(self.handle, device_matrix.handle, int(precision), ctypes.byref(big),
+                                            ctypes.byref(tuned)))
+        return big.value, tuned.value
 
     def __del__(self):
         if getattr(self, "handle", None):

@@ -146,6 +146,9 @@ hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const dou
 hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
                                   hipStream_t stream);
 
+// 128 × 128 tiles of mode 0 (the same bits as launchUpdateTiles on the same products).
+template <typename T>
+hipError_t launchBigTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream);
 template <typename T>
 hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
                              hipStream_t stream);

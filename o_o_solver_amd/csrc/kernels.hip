@@ -647,6 +647,105 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
             }
 }
 
+// The same product on 128 × 128 tiles (mode 0 only): four waves of 64 × 64 (4 × 4 MFMA blocks each), K staged by 16
+// through LDS with the next chunk loaded into registers meanwhile. Each operand value loaded feeds twice the flops of
+// a 64 × 64 tile, at a quarter of the tiles per front. The k-steps (four terms per MFMA, in increasing order) and the
+// operands (−A, B·d) are those of updateTilesKernel, so both give the same bits.
+constexpr int BIG = 128;
+constexpr int BIG_K = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void bigTilesKernel(T* panels, T* updates, const UpdateTile* tiles, int count)
+{
+    constexpr int PER = BIG * BIG_K / 256;
+    __shared__ T As[BIG_K][BIG + 4];
+    __shared__ T Bs[BIG_K][BIG + 4];
+    const UpdateTile t = tiles[tileOfWorkgroup(count)];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const T* A = panels + t.aOff;
+    const T* Bm = panels + t.bOff;
+    const T* dv = panels + t.dOff;
+    const int64_t ld = t.ld;
+    T* C = (t.target == 0 ? panels : updates) + t.cOff;
+
+    typename Mfma<T>::Acc acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int i = min(wm + a * 16 + Mfma<T>::row(lane, v), t.m - 1);
+                const int j = min(wn + b * 16 + (lane & 15), t.n - 1);
+                acc[a][b][v] = C[i + int64_t(j) * t.ldc];
+            }
+
+    const int row = tid & (BIG - 1), kq = tid >> 7;
+    const T* Ar = A + min(row, t.m - 1);
+    const T* Br = Bm + min(row, t.n - 1);
+    T ra[PER], rb[PER], rd[PER];
+    const auto load = [&](int p0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int64_t pp = min(p0 + kq + 2 * u, t.k - 1);
+            ra[u] = Ar[pp * ld];
+            rb[u] = Br[pp * ld];
+            rd[u] = dv[pp * (ld + 1)];
+        }
+    };
+    load(0);
+    for (int p0 = 0; p0 < t.k; p0 += BIG_K)
+    {
+        __syncthreads(); // the previous chunk's products have read the LDS
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+        {
+            const int kk = kq + 2 * u;
+            const bool term = p0 + kk < t.k;
+            const T av = pinned(ra[u]), bv = pinned(rb[u]), dd = pinned(rd[u]);
+            As[kk][row] = term && row < t.m ? -av : T(0);
+            Bs[kk][row] = term && row < t.n ? bv * dd : T(0);
+        }
+        __syncthreads();
+        if (p0 + BIG_K < t.k)
+            load(p0 + BIG_K);
+#pragma unroll
+        for (int kk = 0; kk < BIG_K; kk += 4)
+        {
+            const int ka = kk + (lane >> 4);
+            T av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                av[a] = As[ka][wm + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bv[b] = Bs[ka][wn + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = Mfma<T>::step(av[a], bv[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int i = wm + a * 16 + Mfma<T>::row(lane, v);
+                const int j = wn + b * 16 + (lane & 15);
+                if (i >= t.m || j >= t.n || (t.lower && i < j))
+                    continue;
+                C[i + int64_t(j) * t.ldc] = acc[a][b][v];
+            }
+}
+
 // ---- whole inverses of the wide supernodes' triangles -----------------------------------------------------------
 //
 // X = L_FF⁻¹ (c × c, leading dimension c) by blocks of 64, right-looking: with T_IJ = −Σ_{K<I} L_IK·X_KJ (I > J), X_IJ =
@@ -1581,6 +1680,15 @@ hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateT
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launchBigTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(bigTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, tiles, count);
+    return hipGetLastError();
+}
+
 hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateTile* tiles, int count,
                                   hipStream_t stream)
 {
@@ -1776,6 +1884,7 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, T*, const int32_t*, int, int, double*,          \
                                                 int32_t*, hipStream_t);                                                \
     template hipError_t launchUpdateTiles<T>(T*, T*, const T*, const UpdateTile*, int, hipStream_t);                   \
+    template hipError_t launchBigTiles<T>(T*, T*, const UpdateTile*, int, hipStream_t);                                  \
     template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \
                                          int, const int2*, int, hipStream_t);                                          \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \

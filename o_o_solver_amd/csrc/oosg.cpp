@@ -144,8 +144,10 @@ enum ProfileKind
     PROFILE_BACKWARD_COLUMNS,  // backward sweep: the L_RFᵀ column blocks
     PROFILE_TRIANGLE_PRODUCTS, // both sweeps: the wide triangles, products with their whole inverses
     PROFILE_TRIANGLE_INVERSES, // factorisation: the whole inverses of the wide triangles
+    PROFILE_UPDATE_MATRIX_BIG, // factorisation: update matrices on 128 × 128 tiles
     PROFILE_KINDS
 };
+static_assert(PROFILE_KINDS == OOSG_PROFILE_KINDS, "the profile kinds of the header");
 
 struct Launch
 {
@@ -156,6 +158,11 @@ struct Launch
     int k0 = 0;
     int maxOrder = 0;
     double flops = 0.0; // algorithmic flops of the launch
+    // Update-matrix launches: the same products as 128 × 128 tiles (the big tile list), and whether each precision
+    // runs them so (the tile policy and the autotuner set it).
+    int64_t bigOffset = 0;
+    int bigCount = 0;
+    bool big[2] = {false, false};
 };
 
 // Per-launch device times by HIP events on the plan's stream, summed by kernel kind.
@@ -308,11 +315,19 @@ struct oosg_plan
     std::vector<int32_t> smallList, stepFronts;
     std::vector<oosg::AssemblyBlock> blocks;
     std::vector<oosg::AssemblyItem> items;
-    std::vector<oosg::UpdateTile> tiles;
+    std::vector<oosg::UpdateTile> tiles, bigTiles;
+    // The choice of tile size for the update matrices: 0 64 × 64 everywhere, 1 128 × 128 everywhere, 2 by level
+    // (measured by oosg_plan_autotune, or a rule of thumb before).
+    int32_t tilePolicy = 2;
+    bool tuned[2] = {false, false};
+    // Per-launch device times of a tuning run (ms), when timeLaunches is set.
+    bool timeLaunches = false;
+    std::vector<float> launchMs;
+    std::vector<hipEvent_t> launchEvents;
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dLevelList, dForwardTiles, dBackwardBlocks,
+    DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dLevelList, dForwardTiles, dBackwardBlocks,
         dTriangleItems, dInverseItems;
     oosg::DevicePlan dp{};
 
@@ -349,6 +364,8 @@ struct oosg_plan
 
     ~oosg_plan()
     {
+        for (hipEvent_t e : launchEvents)
+            (void)hipEventDestroy(e);
         factorGraph[0].clear();
         factorGraph[1].clear();
         sweepGraph.clear();
@@ -841,6 +858,32 @@ void buildSchedule(oosg_plan& p)
             tilesLaunch.count = static_cast<int>(p.tiles.size() - tileStart);
             for (int64_t q = tileStart; q < static_cast<int64_t>(p.tiles.size()); ++q)
                 tilesLaunch.flops += tileFlops(p.tiles[q]);
+            // The same products on 128 × 128 tiles.
+            tilesLaunch.bigOffset = static_cast<int64_t>(p.bigTiles.size());
+            for (const int32_t s : large)
+            {
+                const int c = p.ncol[s], r = p.nrow[s], n = c + r;
+                for (int J0 = 0; J0 < r; J0 += 128)
+                    for (int I0 = J0; I0 < r; I0 += 128)
+                    {
+                        oosg::UpdateTile t{};
+                        t.cOff = p.updateOff[s] + I0 + int64_t(J0) * r;
+                        t.ldc = r;
+                        t.aOff = p.panelOff[s] + c + I0;
+                        t.bOff = p.panelOff[s] + c + J0;
+                        t.dOff = p.panelOff[s];
+                        t.ld = n;
+                        t.m = std::min(128, r - I0);
+                        t.n = std::min(128, r - J0);
+                        t.k = c;
+                        t.target = 1;
+                        t.lower = I0 == J0;
+                        p.bigTiles.push_back(t);
+                    }
+            }
+            tilesLaunch.bigCount = static_cast<int>(p.bigTiles.size() - tilesLaunch.bigOffset);
+            // Before measurement: big tiles where they still fill the chip several times over.
+            tilesLaunch.big[0] = tilesLaunch.big[1] = tilesLaunch.bigCount >= 2048;
             p.schedule.push_back(tilesLaunch);
         }
         // The whole inverses of the level's wide triangles (their panels are final), one launch per step.
@@ -898,6 +941,7 @@ void uploadPlan(oosg_plan& p)
     p.dBlocks = upload(p.blocks, st, "plan: assembly blocks");
     p.dItems = upload(p.items, st, "plan: assembly items");
     p.dTiles = upload(p.tiles, st, "plan: update tiles");
+    p.dBigTiles = upload(p.bigTiles, st, "plan: big update tiles");
     p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
 }
@@ -918,9 +962,23 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     const oosg::AssemblyItem* items = p.dItems.as<oosg::AssemblyItem>();
     const int2* inverseItems = p.dInverseItems.as<int2>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
-    for (const Launch& l : p.schedule)
+    const bool timing = p.timeLaunches;
+    if (timing)
+        while (p.launchEvents.size() < 2 * p.schedule.size())
+        {
+            hipEvent_t e = nullptr;
+            check(hipEventCreate(&e), "hipEventCreate");
+            p.launchEvents.push_back(e);
+        }
+    for (size_t li = 0; li < p.schedule.size(); ++li)
     {
-        const int slot = p.profiler.begin(l.profile, l.flops, st);
+        const Launch& l = p.schedule[li];
+        if (timing)
+            check(hipEventRecord(p.launchEvents[2 * li], st), "hipEventRecord");
+        const int precision = sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64;
+        const bool big = l.kind == LaunchKind::Tiles && l.bigCount > 0 &&
+                         (p.tilePolicy == 1 || (p.tilePolicy == 2 && l.big[precision]));
+        const int slot = p.profiler.begin(big ? PROFILE_UPDATE_MATRIX_BIG : l.profile, l.flops, st);
         switch (l.kind)
         {
         case LaunchKind::Small:
@@ -938,15 +996,32 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
                   "factor: diagonal blocks");
             break;
         case LaunchKind::Tiles:
-            check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
-                  "factor: update tiles");
+        {
+            if (big)
+                check(oosg::launchBigTiles<T>(panels, updates, p.dBigTiles.as<oosg::UpdateTile>() + l.bigOffset,
+                                              l.bigCount, st),
+                      "factor: update tiles");
+            else
+                check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
+                      "factor: update tiles");
             break;
+        }
         case LaunchKind::Inverse:
             check(oosg::launchTriangleInverse<T>(p.dp, panels, linv, inverseItems + l.offset, l.count, l.k0, st),
                   "factor: triangle inverses");
             break;
         }
         p.profiler.end(slot, st);
+        if (timing)
+            check(hipEventRecord(p.launchEvents[2 * li + 1], st), "hipEventRecord");
+    }
+    if (timing)
+    {
+        check(hipStreamSynchronize(st), "factor: timing");
+        p.launchMs.assign(p.schedule.size(), 0.0f);
+        for (size_t li = 0; li < p.schedule.size(); ++li)
+            check(hipEventElapsedTime(&p.launchMs[li], p.launchEvents[2 * li], p.launchEvents[2 * li + 1]),
+                  "hipEventElapsedTime");
     }
 }
 
@@ -2256,6 +2331,102 @@ oosg_status oosg_selector_train(oosg_selector* s, const float* features, const f
                 check(hipMemcpyAsync(loss, s->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s->stream),
                       "selector: loss");
             check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
+oosg_status oosg_plan_tile_policy(oosg_plan* plan, int32_t policy)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            if (policy < 0 || policy > 2)
+                fail(OOSG_INVALID_ARGUMENT, "plan: the tile policy is 0 (64 x 64), 1 (128 x 128) or 2 (by level)");
+            setDevice(plan);
+            if (plan->tilePolicy != policy)
+            {
+                plan->tilePolicy = policy;
+                plan->factorGraph[0].clear(); // the captured launches change
+                plan->factorGraph[1].clear();
+            }
+        });
+}
+
+oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32_t precision, int32_t* big_launches,
+                               int32_t* tuned_launches)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || matrix == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "autotune: null argument");
+            if (matrix->plan != plan)
+                fail(OOSG_INVALID_ARGUMENT, "autotune: the matrix was made for another plan");
+            if (precision != OOSG_FP64 && precision != OOSG_FP32)
+                fail(OOSG_INVALID_ARGUMENT, "autotune: unknown precision");
+            setDevice(plan);
+            oosg_plan& p = *plan;
+            const size_t bytes = valueBytes(precision);
+            DeviceBuffer panels, linv, diag, zero;
+            panels.allocate(static_cast<size_t>(std::max<int64_t>(p.panelOff[p.S], 1)) * bytes, "autotune: panels");
+            linv.allocate(static_cast<size_t>(std::max<int64_t>(p.linvOff[p.S], 1)) * bytes, "autotune: inverses");
+            diag.allocate(static_cast<size_t>(std::max(p.n, 1)) * sizeof(double), "autotune: pivots");
+            zero.allocate(static_cast<size_t>(std::max(p.S, 1)) * sizeof(int32_t), "autotune: pivot status");
+            const int32_t keep = p.tilePolicy;
+            const bool profiling = p.profiler.enabled;
+            p.profiler.enabled = false;
+            // Each policy's per-launch times: a warm run, then the measured one (eager launches, no graph).
+            const auto measure = [&](int32_t policy)
+            {
+                p.tilePolicy = policy;
+                p.timeLaunches = true;
+                std::vector<float> best;
+                for (int run = 0; run < 2; ++run)
+                {
+                    check(oosg::launchFillInt32(zero.as<int32_t>(), p.S, INT32_MAX, p.stream), "autotune: status");
+                    if (precision == OOSG_FP32)
+                        runFactor<float>(p, *matrix, panels.as<float>(), linv.as<float>(), diag.as<double>(),
+                                         zero.as<int32_t>(), true);
+                    else
+                        runFactor<double>(p, *matrix, panels.as<double>(), linv.as<double>(), diag.as<double>(),
+                                          zero.as<int32_t>(), false);
+                    best = p.launchMs;
+                }
+                p.timeLaunches = false;
+                return best;
+            };
+            std::vector<float> small, big;
+            try
+            {
+                small = measure(0);
+                big = measure(1);
+            }
+            catch (...)
+            {
+                p.tilePolicy = keep;
+                p.timeLaunches = false;
+                p.profiler.enabled = profiling;
+                throw;
+            }
+            p.profiler.enabled = profiling;
+            int32_t chosen = 0, tuned = 0;
+            for (size_t li = 0; li < p.schedule.size(); ++li)
+            {
+                Launch& l = p.schedule[li];
+                if (l.bigCount == 0)
+                    continue;
+                ++tuned;
+                l.big[precision] = big[li] < small[li];
+                chosen += l.big[precision] ? 1 : 0;
+            }
+            p.tilePolicy = 2;
+            p.tuned[precision] = true;
+            p.factorGraph[precision].clear();
+            if (big_launches != nullptr)
+                *big_launches = chosen;
+            if (tuned_launches != nullptr)
+                *tuned_launches = tuned;
         });
 }
 

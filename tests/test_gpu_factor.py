@@ -309,3 +309,33 @@ def test_graph_replays_equal_eager_launches(gpu_device):
         assert r.steps == r0.steps and torch.equal(y, y0), replay
         torch.cuda.synchronize()
         del f32
+
+
+def test_big_tiles_give_the_same_bits_and_the_autotuner_keeps_them(gpu_device):
+    # The update matrices on 128 × 128 tiles run the same MFMA k-steps as on 64 × 64 tiles: the factors are equal bit
+    # for bit, in both precisions, whichever size the autotuner then keeps per level.
+    print("[gpu] big tiles and autotuner", flush=True)
+    w = models.hex_block(14, seed=2, sigma=1.0)
+    sym = w.analyse()
+    plan = sym.plan()
+    assert max(sym.row_count(s) for s in range(sym.supernode_count)) >= 256
+    dm = plan.matrix(w.matrix)
+    b = w.loads("random", nrhs=2, seed=3)
+    for precision in (Precision.FP64, Precision.FP32):
+        options = FactorOptions(precision=precision)
+        plan.tile_policy(0)
+        f0 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+        panels0 = [f0.panel(s)[0] for s in range(sym.supernode_count)]
+        x0 = FactorSolve(f0).solve(b)
+        del f0
+        plan.tile_policy(1)
+        f1 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+        assert all(np.array_equal(f1.panel(s)[0], panels0[s]) for s in range(sym.supernode_count)), precision
+        del f1
+        big, tuned = plan.autotune(dm, precision)
+        print(f"[gpu] {precision.name}: {big} of {tuned} update-matrix launches chose 128 x 128 tiles", flush=True)
+        assert tuned > 0 and 0 <= big <= tuned
+        f2 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+        assert all(np.array_equal(f2.panel(s)[0], panels0[s]) for s in range(sym.supernode_count)), precision
+        assert np.array_equal(FactorSolve(f2).solve(b), x0)
+        del f2
