@@ -86,6 +86,65 @@ def cpu_baseline(workload, b, threads):
                       f"{threads} threads) and one solve of the {b.shape[1]} right-hand sides (FactorSolve::solve)"}
 
 
+def cylinder_main(args):
+    """The buckling shifts of the stiffened cylinder (360 × 200 quad4 skin, 24 stringers, 429,840 DOF): the first 20
+    buckling factors by bisection on GPU Sturm counts, then 8 shifts across them, each an indefinite FP64
+    factorisation (Bunch–Kaufman) and a solve on the GPU; the CPU baseline is the reference's indefinite factor of
+    the same 8 matrices on the host cores, whose Sturm counts must equal the GPU's."""
+    import numpy as np
+    import torch
+
+    from o_o_solver_amd import buckling
+    from o_o_solver_amd.pencil import combine, scale_symmetric
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU", file=sys.stderr)
+        sys.exit(1)
+    record = buckling.run(360, 200, 20, 8, 1e-7, log=log)
+    b = record.pop("buckling")
+    w = b.workload
+    shifts = record["shifts"]
+    baseline = None
+    if args.cpu_baseline == "auto":
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        if _oracle.reference() is not None:
+            threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+            h, _, _, _ = _oracle.ref_symbolic(w.graph_offsets, w.adjacency, w.weights,
+                                              order=np.array(w.symbolic.node_order, np.int32))
+            seconds = []
+            try:
+                for r in shifts:
+                    sigma = r["shift"]
+                    matrix = scale_symmetric(combine(1.0, b.stiffness, -sigma, b.load), b.pencil.scale)
+                    rc, msg, inertia, counts, times, _, _ = _oracle.ref_factor_solve(h, matrix, pivoting=1,
+                                                                                     threads=threads)
+                    if rc != 0:
+                        raise RuntimeError("reference factorisation failed: " + msg)
+                    r["reference_count"] = int(inertia[1])
+                    r["counts_equal"] = r["reference_count"] == r["count"]
+                    r["reference_delayed"], r["reference_two_by_two"] = int(counts[0]), int(counts[1])
+                    seconds.append(float(times[0]))
+                    log(f"reference at {sigma:.6g}: count {inertia[1]} ({'equal' if r['counts_equal'] else 'DIFFERS'}"
+                        f"), {times[0]:.2f} s")
+            finally:
+                _oracle.reference().ref_sym_free(h)
+            baseline = {"value": round(float(np.mean(seconds)), 4), "unit": "s", "cores": threads,
+                        "kind": "reference",
+                        "sample": "the reference's indefinite FP64 factorisation (Bunch-Kaufman, avx2 family) of "
+                                  "each of the 8 shifted matrices, mean seconds per shift"}
+    gpu = [r["factor_s"] for r in shifts]
+    out = {"metric": "indefinite FP64 factor s per buckling shift (Bunch-Kaufman), Sturm counts vs reference",
+           "value": round(float(np.mean(gpu)), 5), "unit": "s", "n_gpus": 1, "steps": len(shifts), "warmup": 0,
+           "ms_per_step": round(float(np.mean(gpu)) * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": record["workload"], "roots": 20, "shifts": 8},
+           "roots": record["roots"], "shifts": shifts, "min_relative_gap": record["min_relative_gap"],
+           "bisection_factorisations": record["bisection_factorisations"],
+           "counts_equal": all(r.get("counts_equal", True) for r in shifts), "cpu_baseline": baseline}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,11 +156,17 @@ def main():
     ap.add_argument("--max-steps", type=int, default=30, help="refinement steps of the mixed path at most")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: the host's CPU share, at most 16")
+    ap.add_argument("--workload", choices=["plate", "cylinder"], default="plate",
+                    help="plate: the headline (factor + solve of the 1.5M-DOF shell plate); cylinder: the buckling "
+                         "shifts of the stiffened cylinder (indefinite factors and their Sturm counts)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the factor of one matrix on one MI355X); "
               "run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cylinder":
+        cylinder_main(args)
+        return
 
     import numpy as np
     import torch

@@ -155,6 +155,14 @@ class SymmetricCsc:
             y[:, h] += np.bincount(columns[off], weights=self.values[off] * xs[self.rows[off], h], minlength=self.n)
         return y.reshape(-1) if one else y
 
+    def norm_inf(self):
+        """‖A‖∞: the largest row sum of |a_ij| over both triangles."""
+        columns = np.repeat(np.arange(self.n), np.diff(self.offsets))
+        off = self.rows != columns
+        sums = np.bincount(self.rows, weights=np.abs(self.values), minlength=self.n)
+        sums += np.bincount(columns[off], weights=np.abs(self.values[off]), minlength=self.n)
+        return float(np.max(sums)) if self.n else 0.0
+
     def to_dense(self):
         """The full matrix (tests, small sizes)."""
         dense = np.zeros((self.n, self.n))

@@ -96,6 +96,9 @@ _MODEL_SIGNATURES = {
     "oosm_graph": (None, [c_void_p, c_void_p, c_void_p]),
     "oosm_coords": (None, [c_void_p, c_void_p]),
     "oosm_csc": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oosm_csc_geometric": (None, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oosm_cylinder": (c_void_p, [c_int32, c_int32, c_double, c_double, c_double, c_double, c_double, c_int32,
+                                 c_double, c_double]),
     "oosm_loads": (None, [c_void_p, c_void_p, c_int, c_int32, ctypes.c_uint64, c_void_p]),
 }
 

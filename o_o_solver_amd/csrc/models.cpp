@@ -5,7 +5,10 @@
 //   - a flat plate of quadrilateral shells, 6 DOFs per node (u, v, w, θx, θy, θz): plane-stress membrane (2 × 2
 //     Gauss), Mindlin bending with MITC4 assumed transverse shear (Bathe & Dvorkin 1985), and a drilling penalty
 //     γ·∫(θz − ω)² dA tying θz to the in-plane rotation ω = ½(∂v/∂x − ∂u/∂y) (Hughes & Brezzi 1989);
-//   - a block of 8-node hexahedra, 3 DOFs per node (trilinear, 2 × 2 × 2 Gauss, isotropic).
+//   - a block of 8-node hexahedra, 3 DOFs per node (trilinear, 2 × 2 × 2 Gauss, isotropic);
+//   - a stiffened cylinder: the flat shells on the cylinder's facets (each quad's stiffness rotated from its own
+//     plane), bar stringers along meridians, and the geometric stiffness K_G of a uniform axial membrane force, the
+//     buckling pencil K + λ·K_G (K_G on K's pattern).
 // Fixed nodes (clamped edges, a fixed face) are removed with all their DOFs: every node kept has the same weight.
 // The matrix is stored by 6 × 6 (or 3 × 3) node blocks over the node graph, in both directions, then written as the
 // lower CSC of the DOFs in a given node order (the factor's), structural zeros kept as the reference keeps them.
@@ -59,6 +62,8 @@ struct Model
     std::vector<double> blocks;
     // Consistent load of a uniform pressure (w DOF), per kept node.
     std::vector<double> pressureLoad;
+    // The geometric stiffness, in the layout of `blocks` (empty: none).
+    std::vector<double> geometric;
 };
 
 std::size_t blockIndex(const Model& m, std::int32_t a, std::int32_t b)
@@ -107,7 +112,7 @@ void buildGraph(Model& m, const std::vector<std::int32_t>& conn, int nodesPerEle
     m.blocks.assign((static_cast<std::size_t>(m.nodeCount) + m.adj.size()) * d2, 0.0);
 }
 
-void scatterElement(Model& m, const std::int32_t* nodes, int count, const double* ke)
+void scatterInto(Model& m, std::vector<double>& target, const std::int32_t* nodes, int count, const double* ke)
 {
     const int d = m.dofPerNode;
     const int size = count * d;
@@ -121,12 +126,17 @@ void scatterElement(Model& m, const std::int32_t* nodes, int count, const double
             const std::int32_t b = nodes[j];
             if (b < 0)
                 continue;
-            double* block = m.blocks.data() + blockIndex(m, a, b) * static_cast<std::size_t>(d * d);
+            double* block = target.data() + blockIndex(m, a, b) * static_cast<std::size_t>(d * d);
             for (int p = 0; p < d; ++p)
                 for (int q = 0; q < d; ++q)
                     block[p * d + q] += ke[(i * d + p) * size + (j * d + q)];
         }
     }
+}
+
+void scatterElement(Model& m, const std::int32_t* nodes, int count, const double* ke)
+{
+    scatterInto(m, m.blocks, nodes, count, ke);
 }
 
 // ---- quadrilateral flat shell ------------------------------------------------------------------------------------
@@ -481,6 +491,134 @@ void* oosm_hex_block(std::int32_t nx, std::int32_t ny, std::int32_t nz, double l
     return m.release();
 }
 
+// A stiffened cylinder of radius R and length L: n_circ × n_axial quadrilateral shells of thickness t on the facets
+// of the cylinder (R cos θ, R sin θ, z), z ∈ [0, L], the nodes of both end rings fixed; bar stringers of area
+// stringer_area (modulus E) along every stringer_every-th meridian; and the geometric stiffness of a uniform axial
+// membrane force N (< 0: compression; force per unit length of the circumference), which the stringers carry as
+// P = N·A/t: K_G = N·∫ ∂N_a/∂z ∂N_b/∂z dA on each translation of the skin, P/h·[1 −1; −1 1] on the two translations
+// across a bar. The buckling factors λ solve (K + λ·K_G)φ = 0. Each facet is flat (its nodes lie on two parallel
+// generators): its stiffness is formed in its own frame (x along the chord, y along z) and rotated.
+void* oosm_cylinder(std::int32_t n_circ, std::int32_t n_axial, double radius, double length, double t, double E,
+                    double nu, std::int32_t stringer_every, double stringer_area, double axial_force)
+{
+    auto m = std::make_unique<Model>();
+    m->dofPerNode = 6;
+    const std::int32_t rings = n_axial + 1;
+    const auto node = [n_circ](std::int32_t i, std::int32_t j) { return static_cast<std::size_t>(j) * n_circ + i; };
+    std::vector<std::int32_t> kept(static_cast<std::size_t>(n_circ) * rings, -1);
+    std::int32_t count = 0;
+    for (std::int32_t j = 1; j < n_axial; ++j)
+        for (std::int32_t i = 0; i < n_circ; ++i)
+            kept[node(i, j)] = count++;
+    const double pi2 = 6.283185307179586;
+    const auto position = [&](std::int32_t i, std::int32_t j, double* p)
+    {
+        const double th = pi2 * (i % n_circ) / n_circ;
+        p[0] = radius * std::cos(th);
+        p[1] = radius * std::sin(th);
+        p[2] = length * j / n_axial;
+    };
+    m->nodeCount = count;
+    m->coords.resize(static_cast<std::size_t>(count) * 3);
+    for (std::int32_t j = 0; j < rings; ++j)
+        for (std::int32_t i = 0; i < n_circ; ++i)
+            if (kept[node(i, j)] >= 0)
+                position(i, j, &m->coords[3 * static_cast<std::size_t>(kept[node(i, j)])]);
+    std::vector<std::int32_t> conn(static_cast<std::size_t>(n_circ) * n_axial * 4);
+    for (std::int32_t j = 0; j < n_axial; ++j)
+        for (std::int32_t i = 0; i < n_circ; ++i)
+        {
+            const std::size_t e = static_cast<std::size_t>(j) * n_circ + i;
+            const std::int32_t i1 = (i + 1) % n_circ;
+            conn[4 * e] = kept[node(i, j)];
+            conn[4 * e + 1] = kept[node(i1, j)];
+            conn[4 * e + 2] = kept[node(i1, j + 1)];
+            conn[4 * e + 3] = kept[node(i, j + 1)];
+        }
+    buildGraph(*m, conn, 4);
+    m->geometric.assign(m->blocks.size(), 0.0);
+    m->pressureLoad.assign(static_cast<std::size_t>(count), 0.0);
+    std::vector<double> kl(24 * 24), kg(24 * 24), geo(24 * 24);
+    const double h = length / n_axial;
+    for (std::int32_t j = 0; j < n_axial; ++j)
+        for (std::int32_t i = 0; i < n_circ; ++i)
+        {
+            const std::size_t e = static_cast<std::size_t>(j) * n_circ + i;
+            double p0[3], p1[3];
+            position(i, j, p0);
+            position(i + 1, j, p1);
+            double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], 0.0};
+            const double chord = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1]);
+            e1[0] /= chord;
+            e1[1] /= chord;
+            const double e2[3] = {0.0, 0.0, 1.0};
+            const double e3[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                  e1[0] * e2[1] - e1[1] * e2[0]};
+            const double R[3][3] = {{e1[0], e1[1], e1[2]}, {e2[0], e2[1], e2[2]}, {e3[0], e3[1], e3[2]}};
+            const double x[4] = {0.0, chord, chord, 0.0}, y[4] = {0.0, 0.0, h, h};
+            shellQuadStiffness(x, y, E, nu, t, kl.data());
+            // kg = Tᵀ·kl·T, T the block diagonal of R over the 8 triads (translations and rotations of 4 nodes).
+            for (int a = 0; a < 24; ++a)
+                for (int b = 0; b < 24; ++b)
+                {
+                    // (kl·T)(a, b) = Σ_c kl(a, 3·(b/3) + c)·R(c, b mod 3)
+                    double v = 0.0;
+                    for (int c = 0; c < 3; ++c)
+                        v += kl[a * 24 + 3 * (b / 3) + c] * R[c][b % 3];
+                    geo[a * 24 + b] = v;
+                }
+            for (int a = 0; a < 24; ++a)
+                for (int b = 0; b < 24; ++b)
+                {
+                    double v = 0.0;
+                    for (int c = 0; c < 3; ++c)
+                        v += R[c][a % 3] * geo[(3 * (a / 3) + c) * 24 + b];
+                    kg[a * 24 + b] = v;
+                }
+            scatterElement(*m, &conn[4 * e], 4, kg.data());
+            // The membrane force's geometric stiffness, the same on each translation (and so in any frame).
+            std::fill(geo.begin(), geo.end(), 0.0);
+            for (int gy = 0; gy < 2; ++gy)
+                for (int gx = 0; gx < 2; ++gx)
+                {
+                    double n[4], dxi[4], deta[4];
+                    shapeQ4(GAUSS2[gx], GAUSS2[gy], n, dxi, deta);
+                    const double weight = 0.25 * chord * h; // dA = (chord/2)(h/2) dξ dη, unit Gauss weights
+                    for (int a = 0; a < 4; ++a)
+                        for (int b = 0; b < 4; ++b)
+                        {
+                            const double v = axial_force * (deta[a] * 2.0 / h) * (deta[b] * 2.0 / h) * weight;
+                            for (int c = 0; c < 3; ++c)
+                                geo[(6 * a + c) * 24 + 6 * b + c] += v;
+                        }
+                }
+            scatterInto(*m, m->geometric, &conn[4 * e], 4, geo.data());
+        }
+    if (stringer_every > 0 && stringer_area > 0.0)
+    {
+        const double k = E * stringer_area / h, pk = axial_force * stringer_area / t / h;
+        std::vector<double> kb(12 * 12), gb(12 * 12);
+        for (std::int32_t i = 0; i < n_circ; i += stringer_every)
+            for (std::int32_t j = 0; j < n_axial; ++j)
+            {
+                const std::int32_t bar[2] = {kept[node(i, j)], kept[node(i, j + 1)]};
+                std::fill(kb.begin(), kb.end(), 0.0);
+                std::fill(gb.begin(), gb.end(), 0.0);
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                    {
+                        const double sign = a == b ? 1.0 : -1.0;
+                        kb[(6 * a + 2) * 12 + 6 * b + 2] = sign * k;
+                        for (int c = 0; c < 2; ++c)
+                            gb[(6 * a + c) * 12 + 6 * b + c] = sign * pk;
+                    }
+                scatterElement(*m, bar, 2, kb.data());
+                scatterInto(*m, m->geometric, bar, 2, gb.data());
+            }
+    }
+    return m.release();
+}
+
 void oosm_free(void* handle)
 {
     delete static_cast<Model*>(handle);
@@ -513,10 +651,35 @@ void oosm_coords(void* handle, double* xyz)
 
 // The lower CSC of the DOFs in the order of `node_order` (node_order[p] = node at position p; the DOFs of a node
 // consecutive): offsets[n + 1], rows[nnz], values[nnz], each column's rows ascending from its diagonal.
+void writeCsc(const Model& m, const std::vector<double>& source, const std::int32_t* node_order,
+              std::int64_t* offsets, std::int32_t* rows, double* values);
+
 void oosm_csc(void* handle, const std::int32_t* node_order, std::int64_t* offsets, std::int32_t* rows,
               double* values)
 {
     const Model& m = *static_cast<Model*>(handle);
+    writeCsc(m, m.blocks, node_order, offsets, rows, values);
+}
+
+// The geometric stiffness on the same pattern (a model without one: all zero).
+void oosm_csc_geometric(void* handle, const std::int32_t* node_order, std::int64_t* offsets, std::int32_t* rows,
+                        double* values)
+{
+    const Model& m = *static_cast<Model*>(handle);
+    if (m.geometric.empty())
+    {
+        std::vector<double> zero(m.blocks.size(), 0.0);
+        writeCsc(m, zero, node_order, offsets, rows, values);
+    }
+    else
+        writeCsc(m, m.geometric, node_order, offsets, rows, values);
+}
+
+} // extern "C"
+
+void writeCsc(const Model& m, const std::vector<double>& source, const std::int32_t* node_order,
+              std::int64_t* offsets, std::int32_t* rows, double* values)
+{
     const int d = m.dofPerNode;
     const std::int32_t n = m.nodeCount;
     std::vector<std::int32_t> position(static_cast<std::size_t>(n));
@@ -536,7 +699,7 @@ void oosm_csc(void* handle, const std::int32_t* node_order, std::int64_t* offset
                 later.emplace_back(position[static_cast<std::size_t>(b)], b);
         }
         std::sort(later.begin(), later.end());
-        const double* diag = m.blocks.data() + blockIndex(m, a, a) * static_cast<std::size_t>(d * d);
+        const double* diag = source.data() + blockIndex(m, a, a) * static_cast<std::size_t>(d * d);
         for (int q = 0; q < d; ++q)
         {
             // Column q of node a: the diagonal block's rows q..d−1, then each later node's d rows.
@@ -549,7 +712,7 @@ void oosm_csc(void* handle, const std::int32_t* node_order, std::int64_t* offset
             for (const auto& [pb, b] : later)
             {
                 // Entry (DOF r of b, DOF q of a) = block(b, a)[r][q].
-                const double* block = m.blocks.data() + blockIndex(m, b, a) * static_cast<std::size_t>(d * d);
+                const double* block = source.data() + blockIndex(m, b, a) * static_cast<std::size_t>(d * d);
                 for (int r = 0; r < d; ++r)
                 {
                     rows[at] = pb * d + r;
@@ -561,6 +724,8 @@ void oosm_csc(void* handle, const std::int32_t* node_order, std::int64_t* offset
         }
     }
 }
+
+extern "C" {
 
 // Loads in the DOF order of `node_order`, column-major n × nrhs. kind 0: the uniform pressure (w DOF, one column,
 // shell models); kind 1: independent N(0, 1) forces on the translational DOFs (3 per node), rotations 0; kind 2:

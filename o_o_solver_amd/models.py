@@ -1,7 +1,8 @@
 """Synthetic finite-element workloads (liboosm.so): the matrices the factor and the solves are measured and tested
 on. The reference assembles its stiffness matrices from its own elements (src/elements, src/assembly), which are
 outside the numeric phase; these models are this project's own (flat MITC4 shells with a drilling penalty, trilinear
-hexahedra), generated from a seed so that a workload is the same on every machine.
+hexahedra, a stiffened cylinder with its geometric stiffness), generated from a seed so that a workload is the same
+on every machine.
 
 A Workload carries the node graph, coordinates and weights, the order (this project's nested dissection, or one
 given), the symbolic factor and the lower CSC in factor order, and builds loads in that order.
@@ -63,6 +64,18 @@ class Workload:
         m.oosm_csc(self.handle, _ptr(node_order), _ptr(offsets), _ptr(rows), _ptr(values))
         return SymmetricCsc(self.n, offsets, rows, values)
 
+    def geometric(self):
+        """The geometric stiffness K_G on the stiffness's pattern, in the factor's order (zero for models without
+        one); the buckling factors λ solve (K + λ·K_G)φ = 0."""
+        m = _native.models()
+        nnz = self.matrix.rows.shape[0]
+        offsets = np.zeros(self.n + 1, np.int64)
+        rows = np.zeros(nnz, np.int32)
+        values = np.zeros(nnz)
+        node_order = np.ascontiguousarray(self.symbolic.node_order, dtype=np.int32)
+        m.oosm_csc_geometric(self.handle, _ptr(node_order), _ptr(offsets), _ptr(rows), _ptr(values))
+        return SymmetricCsc(self.n, offsets, rows, values)
+
     def loads(self, kind="random", nrhs=1, seed=1):
         """n × nrhs (Fortran order) in the factor's equation order. kind: 'pressure' (uniform pressure, shells),
         'random' (N(0, 1) forces on the translational DOFs), 'top' (N(0, 1) on the top face's nodes)."""
@@ -102,6 +115,16 @@ def hex_block(nx, ny=None, nz=None, size=(1.0, 1.0, 1.0), seed=0, E_median=200e9
     nz = nx if nz is None else nz
     handle = _native.models().oosm_hex_block(nx, ny, nz, size[0], size[1], size[2], seed, E_median, sigma, nu)
     return _workload(f"hex block {nx}x{ny}x{nz}", handle)
+
+
+def cylinder(n_circ=360, n_axial=200, radius=1.0, length=2.0, thickness=2e-3, E=70e9, nu=0.3, stringer_every=15,
+             stringer_area=2e-4, axial_force=-1e3):
+    """A stiffened cylinder (the buckling workload): n_circ × n_axial quad4 shells on a cylinder of radius `radius`
+    and length `length`, both end rings fixed, bar stringers along every `stringer_every`-th meridian, and the
+    geometric stiffness of the uniform axial membrane force `axial_force` (N/m, compression < 0)."""
+    handle = _native.models().oosm_cylinder(n_circ, n_axial, radius, length, thickness, E, nu, stringer_every,
+                                            stringer_area, axial_force)
+    return _workload(f"cylinder {n_circ}x{n_axial}", handle)
 
 
 def shell_plate_c2(scale=500, seed=20261203):
