@@ -48,7 +48,7 @@ def log(message):
     print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, b, threads):
+def cpu_baseline(workload, b, threads, keep_solution=False):
     """The reference's own FP64 factor (AVX2 kernel family, the family it selects itself) and solve on the host
     cores, on the same symbolic structure and matrix: oracle/_ref, timed as one factorisation + one solve of the
     16 right-hand sides. Returns the cpu_baseline object, or None when the compiled reference is absent."""
@@ -78,7 +78,8 @@ def cpu_baseline(workload, b, threads):
     except OSError:
         pass
     total = float(times[0] + times[1])
-    return {"value": round(total, 4), "unit": "s", "cores": threads, "kind": "reference",
+    extra = {"x": x} if keep_solution else {}
+    return {**extra, "value": round(total, 4), "unit": "s", "cores": threads, "kind": "reference",
             "factor_s": round(float(times[0]), 4), "solve_s": round(float(times[1]), 4),
             "gflops_factor": round(flops / times[0] / 1e9, 1), "cpu": model,
             "hardware_concurrency": int(_oracle.reference().ref_hardware_threads()),
@@ -145,6 +146,85 @@ def cylinder_main(args):
     print(json.dumps(out), flush=True)
 
 
+def hex_main(args):
+    """The 64 × 64 × 64 hexahedral block (811,200 DOF; per-element E lognormal, median 200 GPa, σ = 1.5 in ln; base
+    fixed; 4 load cases of N(0, 1) forces on the top face): a step is one factorisation and one solve of the 4
+    right-hand sides to a backward error of 1e-10, with FP64 fronts and with FP32 fronts refined by conjugate
+    gradients. The CPU baseline is the reference's FP64 factor and solve; the GPU solutions' error against its
+    solution is reported."""
+    import numpy as np
+    import torch
+
+    from o_o_solver_amd import FactorOptions, FactorSolve, NotConvergedError, Precision, SupernodalFactor, models
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU", file=sys.stderr)
+        sys.exit(1)
+    t = time.perf_counter()
+    workload = models.hex_block(64, seed=20261204, sigma=1.5)
+    sym = workload.analyse()
+    log(f"{workload.name}: {workload.n} DOF, {sym.supernode_count} supernodes, {sym.flop_count:.3e} flops "
+        f"({time.perf_counter() - t:.1f} s)")
+    b = workload.loads("top", nrhs=4, seed=20261204)
+    plan = sym.plan()
+    dm = plan.matrix(workload.matrix)
+    bd = torch.tensor(b, device="cuda:0")
+    tuning = {}
+    for precision in (Precision.FP64, Precision.FP32):
+        big, tuned = plan.autotune(dm, precision)
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned}
+    log(f"autotune: {tuning}")
+    results = {}
+    for name, precision, steps in (("fp64", Precision.FP64, 4), ("fp32_fronts_cg", Precision.FP32, args.max_steps)):
+        times = []
+        for k in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f = SupernodalFactor.factorise(sym, workload.matrix, FactorOptions(precision=precision), device_matrix=dm)
+            try:
+                x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=steps)
+            except NotConvergedError as e:
+                x, rep = None, None
+                log(f"{name}: not converged ({e})")
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t
+            if k >= args.warmup:
+                times.append(elapsed)
+            factor_s = f.factor_seconds
+            del f
+        results[name] = {"seconds": round(float(np.mean(times)), 4), "factor_s": round(factor_s, 4),
+                         "factor_tflops": round(sym.flop_count / factor_s / 1e12, 2),
+                         "solves": rep.steps if rep else None,
+                         "backward_error": rep.backward_error if rep else None,
+                         "x": x.cpu().numpy() if x is not None else None}
+        log(f"{name}: {results[name]['seconds'] * 1e3:.1f} ms/step, factor {factor_s * 1e3:.1f} ms "
+            f"({results[name]['factor_tflops']} TFLOP/s), solves {results[name]['solves']}, eta "
+            f"{results[name]['backward_error']}")
+    baseline = None
+    if args.cpu_baseline == "auto":
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline: the reference's avx2 factor + solve on {threads} threads")
+        baseline = cpu_baseline(workload, b, threads, keep_solution=True)
+    if baseline is not None:
+        xr = baseline.pop("x")
+        for name in results:
+            x = results[name]["x"]
+            if x is not None:
+                results[name]["error_vs_reference"] = float(np.max(np.abs(x - xr)) / np.max(np.abs(xr)))
+    for name in results:
+        results[name].pop("x")
+    best = min(r["seconds"] for r in results.values())
+    out = {"metric": "factor+solve s at 1e-10 backward err (64^3 hex block): FP32 fronts + FP64 CG vs FP64 GPU",
+           "value": best, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(best * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"hex8 block 64^3, {workload.n} DOF, E lognormal (median 200 GPa, sigma 1.5), "
+                                  "base fixed; 4 N(0,1) top-face load cases", "flops": sym.flop_count,
+                      "supernodes": sym.supernode_count},
+           "autotune": tuning, **results, "cpu_baseline": baseline}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,9 +236,10 @@ def main():
     ap.add_argument("--max-steps", type=int, default=30, help="refinement steps of the mixed path at most")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: the host's CPU share, at most 16")
-    ap.add_argument("--workload", choices=["plate", "cylinder"], default="plate",
+    ap.add_argument("--workload", choices=["plate", "cylinder", "hex"], default="plate",
                     help="plate: the headline (factor + solve of the 1.5M-DOF shell plate); cylinder: the buckling "
-                         "shifts of the stiffened cylinder (indefinite factors and their Sturm counts)")
+                         "shifts of the stiffened cylinder (indefinite factors and their Sturm counts); hex: the "
+                         "64^3 hexahedral block (3D fronts), FP64 and FP32 fronts + refinement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the factor of one matrix on one MI355X); "
@@ -166,6 +247,9 @@ def main():
         sys.exit(2)
     if args.workload == "cylinder":
         cylinder_main(args)
+        return
+    if args.workload == "hex":
+        hex_main(args)
         return
 
     import numpy as np

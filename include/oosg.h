@@ -209,6 +209,11 @@ oosg_status oosg_plan_tile_policy(oosg_plan *plan, int32_t policy);
 oosg_status oosg_plan_autotune(oosg_plan *plan, const oosg_matrix *matrix, int32_t precision, int32_t *big_launches,
                                int32_t *tuned_launches);
 
+/* Diagnostic: the factorisation's launches in order, with their profile kind, level and algorithmic flops, and the
+ * device time of each in the last autotuner run (ms; -1 before one). Call with null arrays for the count. */
+oosg_status oosg_plan_launch_times(const oosg_plan *plan, int32_t *count, int32_t *kinds, int32_t *levels, float *ms,
+                                   double *flops);
+
 /* Blocks until the plan's stream is idle. */
 oosg_status oosg_plan_synchronize(oosg_plan *plan);
 

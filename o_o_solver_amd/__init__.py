@@ -306,6 +306,17 @@ class Plan:
         """The update-matrix tile size: 0 64 × 64, 1 128 × 128, 2 by level (the autotuner's choice; the default)."""
         _check(self._lib.oosg_plan_tile_policy(self.handle, int(policy)))
 
+    def launch_times(self):
+        """The factorisation's launches: (profile kind, level, ms in the last autotuner run, flops) arrays."""
+        count = ctypes.c_int32(0)
+        _check(self._lib.oosg_plan_launch_times(self.handle, ctypes.byref(count), None, None, None, None))
+        n = count.value
+        kinds, levels = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        ms, flops = np.zeros(n, np.float32), np.zeros(n)
+        _check(self._lib.oosg_plan_launch_times(self.handle, ctypes.byref(count), _ptr(kinds), _ptr(levels), _ptr(ms),
+                                                _ptr(flops)))
+        return kinds, levels, ms, flops
+
     def autotune(self, device_matrix, precision=None):
         """Times both tile sizes of every update-matrix launch on `device_matrix` (a matrix of this plan) and keeps
         the faster per level. Returns (launches that chose 128 × 128 tiles, launches measured)."""

@@ -61,10 +61,14 @@ struct AssemblyItem
 
 // One 64 × 64 MFMA tile (element offsets): mode 0 C −= A·diag(d)·Bᵀ with A, B, d in the panels and C in the panels
 // (target 0) or the update matrices (target 1); mode 1 C = A·Bᵀ·diag(d)⁻¹ with B an inverse of a diagonal block.
+// An update-matrix tile of extendTilesKernel (64 × 64 or 128 × 128) also adds its children's contributions: its
+// 64 × 64 quarters q take the assembly items [itemStarts[items + q], itemStarts[items + q + 1]); rowBase and colBase are
+// the front positions of its element (0, 0).
 struct UpdateTile
 {
     int64_t cOff, aOff, bOff, dOff, ld, ldb, ldc;
     int32_t m, n, k, target, lower, mode;
+    int32_t items, supernode, rowBase, colBase;
 };
 
 // A pivoted front of the indefinite factorisation (device pointers): F, N × N column-major, positions [own columns c,
@@ -146,9 +150,11 @@ hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const dou
 hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
                                   hipStream_t stream);
 
-// 128 × 128 tiles of mode 0 (the same bits as launchUpdateTiles on the same products).
+// The update matrices: C = −A·diag(d)·Bᵀ + Σ children's contributions, on TS × TS tiles (64 or 128; the same bits
+// for both sizes).
 template <typename T>
-hipError_t launchBigTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream);
+hipError_t launchExtendTiles(const DevicePlan& p, T* panels, T* updates, const UpdateTile* tiles, int count,
+                             const int32_t* itemStarts, const AssemblyItem* items, int tileSize, hipStream_t stream);
 template <typename T>
 hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
                              hipStream_t stream);

@@ -647,43 +647,45 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
             }
 }
 
-// The same product on 128 × 128 tiles (mode 0 only): four waves of 64 × 64 (4 × 4 MFMA blocks each), K staged by 16
-// through LDS with the next chunk loaded into registers meanwhile. Each operand value loaded feeds twice the flops of
-// a 64 × 64 tile, at a quarter of the tiles per front. The k-steps (four terms per MFMA, in increasing order) and the
-// operands (−A, B·d) are those of updateTilesKernel, so both give the same bits.
-constexpr int BIG = 128;
-constexpr int BIG_K = 16;
-
-template <typename T>
-__global__ __launch_bounds__(256) void bigTilesKernel(T* panels, T* updates, const UpdateTile* tiles, int count)
+// The update matrices U = −L_R·D·L_Rᵀ + Σ children's contributions (the extend-add of their update matrices into the
+// rows and columns below the supernode's own), on TS × TS tiles, TS = 64 or 128: four waves of TS/2 × TS/2 (NB × NB
+// MFMA blocks each), K staged through LDS by KC with the next chunk loaded into registers meanwhile. The products
+// start from zero; then, a 64 × 64 quarter at a time, the tile goes through LDS, its children's parts are added in the
+// children's order (the assembly items of that quarter, a barrier between children: each child's positions are
+// distinct), and the quarter is written once. With 128 × 128 tiles each operand value loaded feeds twice the flops.
+// The k-steps (four terms per MFMA, in increasing order) and the order of the additions do not depend on TS, so both
+// sizes give the same bits.
+template <typename T, int TS>
+__global__ __launch_bounds__(256) void extendTilesKernel(DevicePlan p, T* panels, T* updates, const UpdateTile* tiles,
+                                                         int count, const int32_t* itemStarts,
+                                                         const AssemblyItem* items)
 {
-    constexpr int PER = BIG * BIG_K / 256;
-    __shared__ T As[BIG_K][BIG + 4];
-    __shared__ T Bs[BIG_K][BIG + 4];
+    constexpr int NB = TS / 32;            // MFMA blocks per wave and dimension
+    constexpr int KC = TS == 64 ? 32 : 16; // terms of a chunk
+    constexpr int PER = TS * KC / 256;     // operand values a thread stages per chunk
+    constexpr int LD = TS + 4;
+    constexpr int QLD = 65; // a 64 × 64 quarter in LDS
+    constexpr int SHARED = 2 * KC * LD > 64 * QLD ? 2 * KC * LD : 64 * QLD;
+    __shared__ T lds[SHARED];
+    T (*As)[LD] = reinterpret_cast<T (*)[LD]>(lds);
+    T (*Bs)[LD] = reinterpret_cast<T (*)[LD]>(lds + KC * LD);
     const UpdateTile t = tiles[tileOfWorkgroup(count)];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
     const T* A = panels + t.aOff;
     const T* Bm = panels + t.bOff;
     const T* dv = panels + t.dOff;
     const int64_t ld = t.ld;
-    T* C = (t.target == 0 ? panels : updates) + t.cOff;
 
-    typename Mfma<T>::Acc acc[4][4];
+    typename Mfma<T>::Acc acc[NB][NB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-            {
-                const int i = min(wm + a * 16 + Mfma<T>::row(lane, v), t.m - 1);
-                const int j = min(wn + b * 16 + (lane & 15), t.n - 1);
-                acc[a][b][v] = C[i + int64_t(j) * t.ldc];
-            }
+        for (int b = 0; b < NB; ++b)
+            acc[a][b] = typename Mfma<T>::Acc{0, 0, 0, 0};
 
-    const int row = tid & (BIG - 1), kq = tid >> 7;
+    const int row = tid & (TS - 1), kq = tid / TS, kstep = 256 / TS;
     const T* Ar = A + min(row, t.m - 1);
     const T* Br = Bm + min(row, t.n - 1);
     T ra[PER], rb[PER], rd[PER];
@@ -691,59 +693,116 @@ __global__ __launch_bounds__(256) void bigTilesKernel(T* panels, T* updates, con
 #pragma unroll
         for (int u = 0; u < PER; ++u)
         {
-            const int64_t pp = min(p0 + kq + 2 * u, t.k - 1);
+            const int64_t pp = min(p0 + kq + kstep * u, t.k - 1);
             ra[u] = Ar[pp * ld];
             rb[u] = Br[pp * ld];
             rd[u] = dv[pp * (ld + 1)];
         }
     };
     load(0);
-    for (int p0 = 0; p0 < t.k; p0 += BIG_K)
+    for (int p0 = 0; p0 < t.k; p0 += KC)
     {
         __syncthreads(); // the previous chunk's products have read the LDS
 #pragma unroll
         for (int u = 0; u < PER; ++u)
         {
-            const int kk = kq + 2 * u;
+            const int kk = kq + kstep * u;
             const bool term = p0 + kk < t.k;
             const T av = pinned(ra[u]), bv = pinned(rb[u]), dd = pinned(rd[u]);
             As[kk][row] = term && row < t.m ? -av : T(0);
             Bs[kk][row] = term && row < t.n ? bv * dd : T(0);
         }
         __syncthreads();
-        if (p0 + BIG_K < t.k)
-            load(p0 + BIG_K);
+        if (p0 + KC < t.k)
+            load(p0 + KC);
 #pragma unroll
-        for (int kk = 0; kk < BIG_K; kk += 4)
+        for (int kk = 0; kk < KC; kk += 4)
         {
             const int ka = kk + (lane >> 4);
-            T av[4], bv[4];
+            T av[NB], bv[NB];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < NB; ++a)
                 av[a] = As[ka][wm + a * 16 + (lane & 15)];
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < NB; ++b)
                 bv[b] = Bs[ka][wn + b * 16 + (lane & 15)];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < NB; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < NB; ++b)
                     acc[a][b] = Mfma<T>::step(av[a], bv[b], acc[a][b]);
         }
     }
+
+    // The quarters: (qi, qj) over 64-row and 64-column halves (one quarter for TS = 64).
+    T(*Q)[QLD] = reinterpret_cast<T (*)[QLD]>(lds);
+    T* C = updates + t.cOff;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int q = 0; q < (TS / 64) * (TS / 64); ++q)
+    {
+        const int qi = (q / (TS / 64)) * 64, qj = (q % (TS / 64)) * 64;
+        const int qm = min(64, t.m - qi), qn = min(64, t.n - qj);
+        if (qm <= 0 || qn <= 0 || (t.lower && qi + qm - 1 < qj))
+            continue; // uniform: every thread skips the same quarters
+        __syncthreads(); // the LDS is free (products or the previous quarter done)
+        // The waves' accumulators that fall in this quarter, into Q (transposed: Q[column][row]).
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int a = 0; a < NB; ++a)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int i = wm + a * 16 + Mfma<T>::row(lane, v) - qi;
+                    const int j = wn + b * 16 + (lane & 15) - qj;
+                    if (i >= 0 && i < 64 && j >= 0 && j < 64)
+                        Q[j][i] = acc[a][b][v];
+                }
+        __syncthreads();
+        const int rowBase = t.rowBase + qi, colBase = t.colBase + qj;
+        const int itemEnd = itemStarts[t.items + q + 1];
+        for (int x = itemStarts[t.items + q]; x < itemEnd; ++x)
+        {
+            // The child's block, columns [q0, q1) × rows [ia, ib) (its lower part), spread over all the threads by
+            // consecutive rows; eight entries per thread per pass, every load issued before any is used.
+            const AssemblyItem it = items[x];
+            const int rc = p.nrow[it.child];
+            const int32_t* rel = p.rel + p.rowStart[it.child];
+            const T* u = updates + p.updateOff[it.child];
+            const int h = it.ib - it.ia, total = h * (it.q1 - it.q0);
+            constexpr int UNROLL = 8;
+            for (int base = tid; base < total; base += 256 * UNROLL)
             {
-                const int i = wm + a * 16 + Mfma<T>::row(lane, v);
-                const int j = wn + b * 16 + (lane & 15);
-                if (i >= t.m || j >= t.n || (t.lower && i < j))
-                    continue;
-                C[i + int64_t(j) * t.ldc] = acc[a][b][v];
+                int ci[UNROLL], ri[UNROLL];
+                T v[UNROLL];
+                bool in[UNROLL];
+#pragma unroll
+                for (int e = 0; e < UNROLL; ++e)
+                {
+                    const int idx = base + 256 * e;
+                    const int jj = it.q0 + idx / h, i = it.ia + idx % h;
+                    in[e] = idx < total && i >= jj;
+                    const int jc = min(jj, it.q1 - 1), ic = in[e] ? i : it.ia;
+                    ci[e] = rel[jc];
+                    ri[e] = rel[ic];
+                    v[e] = u[ic + int64_t(jc) * rc];
+                }
+#pragma unroll
+                for (int e = 0; e < UNROLL; ++e)
+                    if (in[e])
+                        Q[pinned(ci[e]) - colBase][pinned(ri[e]) - rowBase] += pinned(v[e]);
             }
+            __syncthreads(); // the next child may add to the same positions
+        }
+        // The quarter, written once (the lower part only on the diagonal).
+        for (int idx = tid; idx < 64 * 64; idx += 256)
+        {
+            const int i = idx & 63, j = idx >> 6;
+            const int ti = qi + i, tj = qj + j;
+            if (i < qm && j < qn && !(t.lower && ti < tj))
+                C[ti + int64_t(tj) * t.ldc] = Q[j][i];
+        }
+    }
 }
 
 // ---- whole inverses of the wide supernodes' triangles -----------------------------------------------------------
@@ -1681,11 +1740,19 @@ hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateT
 }
 
 template <typename T>
-hipError_t launchBigTiles(T* panels, T* updates, const UpdateTile* tiles, int count, hipStream_t stream)
+hipError_t launchExtendTiles(const DevicePlan& p, T* panels, T* updates, const UpdateTile* tiles, int count,
+                             const int32_t* itemStarts, const AssemblyItem* items, int tileSize, hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(bigTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, tiles, count);
+    if (tileSize == 128)
+        hipLaunchKernelGGL((extendTilesKernel<T, 128>), dim3(count), dim3(256), 0, stream, p, panels, updates, tiles,
+                           count, itemStarts, items);
+    else if (tileSize == 64)
+        hipLaunchKernelGGL((extendTilesKernel<T, 64>), dim3(count), dim3(256), 0, stream, p, panels, updates, tiles,
+                           count, itemStarts, items);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -1884,7 +1951,8 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, T*, const int32_t*, int, int, double*,          \
                                                 int32_t*, hipStream_t);                                                \
     template hipError_t launchUpdateTiles<T>(T*, T*, const T*, const UpdateTile*, int, hipStream_t);                   \
-    template hipError_t launchBigTiles<T>(T*, T*, const UpdateTile*, int, hipStream_t);                                  \
+    template hipError_t launchExtendTiles<T>(const DevicePlan&, T*, T*, const UpdateTile*, int, const int32_t*,        \
+                                             const AssemblyItem*, int, hipStream_t);                                   \
     template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \
                                          int, const int2*, int, hipStream_t);                                          \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \

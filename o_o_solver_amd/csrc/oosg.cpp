@@ -163,6 +163,7 @@ struct Launch
     int64_t bigOffset = 0;
     int bigCount = 0;
     bool big[2] = {false, false};
+    int level = 0;
 };
 
 // Per-launch device times by HIP events on the plan's stream, summed by kernel kind.
@@ -316,6 +317,9 @@ struct oosg_plan
     std::vector<oosg::AssemblyBlock> blocks;
     std::vector<oosg::AssemblyItem> items;
     std::vector<oosg::UpdateTile> tiles, bigTiles;
+    // The update-matrix tiles' extend-add items (children's parts), by quarter: itemStarts[tile.items + q].
+    std::vector<int32_t> itemStarts;
+    std::vector<oosg::AssemblyItem> extendItems;
     // The choice of tile size for the update matrices: 0 64 × 64 everywhere, 1 128 × 128 everywhere, 2 by level
     // (measured by oosg_plan_autotune, or a rule of thumb before).
     int32_t tilePolicy = 2;
@@ -327,7 +331,7 @@ struct oosg_plan
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dLevelList, dForwardTiles, dBackwardBlocks,
+    DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dTriangleItems, dInverseItems;
     oosg::DevicePlan dp{};
 
@@ -675,34 +679,54 @@ void buildSchedule(oosg_plan& p)
     using oosg::PANEL_BLOCK;
     for (int32_t l = 0; l < p.levels; ++l)
     {
+        const size_t firstLaunch = p.schedule.size();
+        struct LevelMark
+        {
+            oosg_plan& plan;
+            size_t from;
+            int32_t level;
+            ~LevelMark()
+            {
+                for (size_t q = from; q < plan.schedule.size(); ++q)
+                    plan.schedule[q].level = level;
+            }
+        } mark{p, firstLaunch, l};
         std::vector<int32_t> large;
-        int maxSmall = 0;
-        const int64_t smallStart = static_cast<int64_t>(p.smallList.size());
+        // Small fronts by order bucket, one launch each: a launch's LDS is sized for its largest front, so the
+        // buckets of smaller fronts fit more workgroups on a CU.
+        constexpr int BUCKETS[] = {48, 64, 80, oosg::SMALL_FRONT_MAX};
+        std::vector<int32_t> bucket[4];
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
             const int32_t s = p.levelList[q];
             const int order = p.ncol[s] + p.nrow[s];
             if (order <= oosg::SMALL_FRONT_MAX)
             {
-                p.smallList.push_back(s);
-                maxSmall = std::max(maxSmall, order);
+                int b = 0;
+                while (order > BUCKETS[b])
+                    ++b;
+                bucket[b].push_back(s);
             }
             else
                 large.push_back(s);
         }
-        const int smallCount = static_cast<int>(p.smallList.size() - smallStart);
-        p.smallFronts += smallCount;
-        p.largeFronts += static_cast<int64_t>(large.size());
-        if (smallCount > 0)
+        for (int b = 0; b < 4; ++b)
         {
+            if (bucket[b].empty())
+                continue;
             Launch launch{LaunchKind::Small, PROFILE_SMALL};
-            launch.offset = smallStart;
-            launch.count = smallCount;
-            launch.maxOrder = maxSmall;
-            for (int64_t q = smallStart; q < static_cast<int64_t>(p.smallList.size()); ++q)
-                launch.flops += frontFlops(p.ncol[p.smallList[q]], p.nrow[p.smallList[q]]);
+            launch.offset = static_cast<int64_t>(p.smallList.size());
+            launch.count = static_cast<int>(bucket[b].size());
+            for (const int32_t s : bucket[b])
+            {
+                p.smallList.push_back(s);
+                launch.maxOrder = std::max(launch.maxOrder, p.ncol[s] + p.nrow[s]);
+                launch.flops += frontFlops(p.ncol[s], p.nrow[s]);
+            }
+            p.smallFronts += launch.count;
             p.schedule.push_back(launch);
         }
+        p.largeFronts += static_cast<int64_t>(large.size());
         if (large.empty())
             continue;
         // Assembly blocks, each with the parts of the children's update matrices that land in it.
@@ -725,12 +749,14 @@ void buildSchedule(oosg_plan& p)
             blk.itemEnd = static_cast<int32_t>(p.items.size());
             p.blocks.push_back(blk);
         };
+        // The panel part only (columns < c): the update matrices take their children's parts in the update-matrix
+        // tiles (extendTilesKernel).
         for (const int32_t s : large)
         {
             const int c = p.ncol[s], n = c + p.nrow[s];
             maxC = std::max(maxC, c);
             int j0 = 0;
-            while (j0 < n)
+            while (j0 < c)
             {
                 const int i0 = j0 < c ? 0 : j0;
                 const int height = n - i0;
@@ -828,28 +854,59 @@ void buildSchedule(oosg_plan& p)
                 p.schedule.push_back(update);
             }
         }
-        // The update matrices: C(r × r) −= L_R·D·L_Rᵀ, K = c.
+        // The update matrices: U(r × r) = −L_R·D·L_Rᵀ + the children's parts, K = c. Each 64 × 64 quarter of a tile
+        // lists its children's parts (the assembly items of its rows and columns).
+        const auto addQuarterItems = [&p](int32_t s, int i0, int i1, int j0, int j1)
+        {
+            for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+            {
+                const int32_t ch = p.children[q];
+                const int32_t* rel = p.rel.data() + p.rowStart[ch];
+                const int32_t rc = p.nrow[ch];
+                const auto at = [&](int value)
+                { return static_cast<int32_t>(std::lower_bound(rel, rel + rc, value) - rel); };
+                const oosg::AssemblyItem it{ch, at(j0), at(j1), at(i0), at(i1), 0};
+                if (it.q0 < it.q1 && std::max(it.q0, it.ia) < it.ib)
+                    p.extendItems.push_back(it);
+            }
+        };
+        const auto addUpdateTile = [&](std::vector<oosg::UpdateTile>& list, int32_t s, int I0, int J0, int size)
+        {
+            const int c = p.ncol[s], r = p.nrow[s], n = c + r;
+            oosg::UpdateTile t{};
+            t.cOff = p.updateOff[s] + I0 + int64_t(J0) * r;
+            t.ldc = r;
+            t.aOff = p.panelOff[s] + c + I0;
+            t.bOff = p.panelOff[s] + c + J0;
+            t.dOff = p.panelOff[s];
+            t.ld = n;
+            t.m = std::min(size, r - I0);
+            t.n = std::min(size, r - J0);
+            t.k = c;
+            t.target = 1;
+            t.lower = I0 == J0;
+            t.supernode = s;
+            t.rowBase = c + I0;
+            t.colBase = c + J0;
+            t.items = static_cast<int32_t>(p.itemStarts.size());
+            for (int qi = 0; qi < size; qi += 64)
+                for (int qj = 0; qj < size; qj += 64)
+                {
+                    p.itemStarts.push_back(static_cast<int32_t>(p.extendItems.size()));
+                    const int qm = std::min(64, t.m - qi), qn = std::min(64, t.n - qj);
+                    if (qm > 0 && qn > 0 && !(t.lower && qi + qm - 1 < qj))
+                        addQuarterItems(s, c + I0 + qi, c + I0 + qi + qm, c + J0 + qj, c + J0 + qj + qn);
+                }
+            p.itemStarts.push_back(static_cast<int32_t>(p.extendItems.size()));
+            list.push_back(t);
+        };
         const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
         for (const int32_t s : large)
         {
-            const int c = p.ncol[s], r = p.nrow[s], n = c + r;
+            const int r = p.nrow[s];
             for (int J0 = 0; J0 < r; J0 += 64)
                 for (int I0 = J0; I0 < r; I0 += 64)
-                {
-                    oosg::UpdateTile t{};
-                    t.cOff = p.updateOff[s] + I0 + int64_t(J0) * r;
-                    t.ldc = r;
-                    t.aOff = p.panelOff[s] + c + I0;
-                    t.bOff = p.panelOff[s] + c + J0;
-                    t.dOff = p.panelOff[s];
-                    t.ld = n;
-                    t.m = std::min(64, r - I0);
-                    t.n = std::min(64, r - J0);
-                    t.k = c;
-                    t.target = 1;
-                    t.lower = I0 == J0;
-                    p.tiles.push_back(t);
-                }
+                    addUpdateTile(p.tiles, s, I0, J0, 64);
         }
         if (static_cast<int64_t>(p.tiles.size()) > tileStart)
         {
@@ -862,24 +919,10 @@ void buildSchedule(oosg_plan& p)
             tilesLaunch.bigOffset = static_cast<int64_t>(p.bigTiles.size());
             for (const int32_t s : large)
             {
-                const int c = p.ncol[s], r = p.nrow[s], n = c + r;
+                const int r = p.nrow[s];
                 for (int J0 = 0; J0 < r; J0 += 128)
                     for (int I0 = J0; I0 < r; I0 += 128)
-                    {
-                        oosg::UpdateTile t{};
-                        t.cOff = p.updateOff[s] + I0 + int64_t(J0) * r;
-                        t.ldc = r;
-                        t.aOff = p.panelOff[s] + c + I0;
-                        t.bOff = p.panelOff[s] + c + J0;
-                        t.dOff = p.panelOff[s];
-                        t.ld = n;
-                        t.m = std::min(128, r - I0);
-                        t.n = std::min(128, r - J0);
-                        t.k = c;
-                        t.target = 1;
-                        t.lower = I0 == J0;
-                        p.bigTiles.push_back(t);
-                    }
+                        addUpdateTile(p.bigTiles, s, I0, J0, 128);
             }
             tilesLaunch.bigCount = static_cast<int>(p.bigTiles.size() - tilesLaunch.bigOffset);
             // Before measurement: big tiles where they still fill the chip several times over.
@@ -942,6 +985,8 @@ void uploadPlan(oosg_plan& p)
     p.dItems = upload(p.items, st, "plan: assembly items");
     p.dTiles = upload(p.tiles, st, "plan: update tiles");
     p.dBigTiles = upload(p.bigTiles, st, "plan: big update tiles");
+    p.dItemStarts = upload(p.itemStarts, st, "plan: update tile items");
+    p.dExtendItems = upload(p.extendItems, st, "plan: update tile items");
     p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
 }
@@ -997,10 +1042,13 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
             break;
         case LaunchKind::Tiles:
         {
-            if (big)
-                check(oosg::launchBigTiles<T>(panels, updates, p.dBigTiles.as<oosg::UpdateTile>() + l.bigOffset,
-                                              l.bigCount, st),
-                      "factor: update tiles");
+            if (l.profile == PROFILE_UPDATE_MATRIX)
+                check(oosg::launchExtendTiles<T>(
+                          p.dp, panels, updates,
+                          big ? p.dBigTiles.as<oosg::UpdateTile>() + l.bigOffset : tiles + l.offset,
+                          big ? l.bigCount : l.count, p.dItemStarts.as<int32_t>(),
+                          p.dExtendItems.as<oosg::AssemblyItem>(), big ? 128 : 64, st),
+                      "factor: update matrices");
             else
                 check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
                       "factor: update tiles");
@@ -2331,6 +2379,37 @@ oosg_status oosg_selector_train(oosg_selector* s, const float* features, const f
                 check(hipMemcpyAsync(loss, s->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, s->stream),
                       "selector: loss");
             check(hipStreamSynchronize(s->stream), "selector");
+        });
+}
+
+oosg_status oosg_plan_launch_times(const oosg_plan* plan, int32_t* count, int32_t* kinds, int32_t* levels, float* ms,
+                                   double* flops)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || count == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            const int32_t n = static_cast<int32_t>(plan->schedule.size());
+            const bool measured = plan->launchMs.size() == plan->schedule.size();
+            if (kinds != nullptr || levels != nullptr || ms != nullptr || flops != nullptr)
+            {
+                if (*count < n)
+                    fail(OOSG_INVALID_ARGUMENT, "plan: launch arrays too short");
+                for (int32_t q = 0; q < n; ++q)
+                {
+                    const Launch& l = plan->schedule[q];
+                    if (kinds != nullptr)
+                        kinds[q] = l.profile;
+                    if (levels != nullptr)
+                        levels[q] = l.level;
+                    if (ms != nullptr)
+                        ms[q] = measured ? plan->launchMs[q] : -1.0f;
+                    if (flops != nullptr)
+                        flops[q] = l.flops;
+                }
+            }
+            *count = n;
         });
 }
 
