@@ -30,15 +30,16 @@ METRIC = "factor+solve s at 1e-10 backward err: mixed FP32+IR vs FP64 GPU; % MFM
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
-# The HIP kernel behind each profile kind (its name in a rocprofv3 trace; the three tile kinds share one kernel).
-KIND_KERNELS = {"small_fronts": "smallFrontKernel", "assembly": "assembleKernel",
-                "diagonal_blocks": "diagonalBlockKernel", "panel_solve": "updateTilesKernel",
-                "panel_update_tiles": "updateTilesKernel", "update_matrix_tiles": "updateTilesKernel",
-                "forward_triangles": "forwardKernel", "backward_triangles": "backwardKernel",
-                "residual": "multiplyRowsKernel", "forward_rows": "forwardRowsKernel",
-                "backward_columns": "backwardColumnsKernel", "triangle_products": "triangleProductsKernel",
-                "triangle_inverses": "triangleInverseKernel", "update_matrix_big_tiles": "bigTilesKernel"}
-# Profile kinds whose work is bytes (the sweeps and the residual products); the others count flops.
+# The HIP kernel behind each profile kind, as a rocprofv3 trace names it ({T}: double or float).
+KIND_KERNELS = {"small_fronts": "smallFrontKernel<{T}>", "assembly": "assembleKernel<{T}>",
+                "diagonal_blocks": "diagonalBlockKernel<{T}>", "panel_solve": "updateTilesKernel<{T},false>",
+                "panel_update_tiles": "updateTilesKernel<{T},false>",
+                "update_matrix_tiles": "extendTilesKernel<{T},64>",
+                "update_matrix_big_tiles": "extendTilesKernel<{T},128>",
+                "forward_triangles": "forwardKernel<{T}>", "backward_triangles": "backwardKernel<{T}>",
+                "residual": "multiplyRowsKernel", "forward_rows": "forwardRowsKernel<{T}>",
+                "backward_columns": "backwardColumnsKernel<{T}>", "triangle_products": "triangleProductsKernel<{T}>",
+                "triangle_inverses": "triangleInverseKernel<{T}>"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_shell_plate_c2.txt")
 BYTE_KINDS = ("forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
               "triangle_products")
@@ -171,8 +172,8 @@ def hex_main(args):
     bd = torch.tensor(b, device="cuda:0")
     tuning = {}
     for precision in (Precision.FP64, Precision.FP32):
-        big, tuned = plan.autotune(dm, precision)
-        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned}
+        big, tuned, streams = plan.autotune(dm, precision)
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams}
     log(f"autotune: {tuning}")
     results = {}
     for name, precision, steps in (("fp64", Precision.FP64, 4), ("fp32_fronts_cg", Precision.FP32, args.max_steps)):
@@ -284,8 +285,8 @@ def main():
     t = time.perf_counter()
     tuning = {}
     for precision in (Precision.FP64, Precision.FP32):
-        big, tuned = plan.autotune(dm, precision)
-        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned}
+        big, tuned, streams = plan.autotune(dm, precision)
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams}
     log(f"autotune ({time.perf_counter() - t:.1f} s): {tuning}")
 
     from o_o_solver_amd.selector import PrecisionSelector
@@ -354,10 +355,12 @@ def main():
     # committed rocprofv3 counter summary of this workload when it lists that kernel.
     chose_fp32 = choice["precision"] == "fp32"
     prof = fp32_prof if chose_fp32 else fp64_prof
-    kernel_type = "<float>" if chose_fp32 and not fp32_fallback else "<double>"
+    kernel_type = "float" if chose_fp32 and not fp32_fallback else "double"
     per_kernel = {}
     for kind, (sec, count, work) in prof.items():
-        name = KIND_KERNELS[kind] + ("" if kind == "residual" else kernel_type)
+        if count == 0:
+            continue
+        name = KIND_KERNELS[kind].format(T=kernel_type)
         acc = per_kernel.setdefault(name, {"seconds": 0.0, "launches": 0, "work": 0.0, "kinds": []})
         acc["seconds"] += sec
         acc["launches"] += count
@@ -372,7 +375,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM, 4)}
     else:
-        peak = PEAK_FP32 if kernel_type == "<float>" else PEAK_FP64
+        peak = PEAK_FP32 if kernel_type == "float" else PEAK_FP64
         achieved = work / seconds / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                     "frac": round(achieved / peak, 4)}

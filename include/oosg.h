@@ -202,12 +202,18 @@ oosg_status oosg_plan_use_graphs(oosg_plan *plan, int32_t enable);
  * 128 x 128 tiles everywhere, 2 (default) by level, as oosg_plan_autotune measured, or before any measurement big
  * tiles where a level has at least 2048 of them. Both sizes give the same bits (the same k-steps of the same MFMA). */
 oosg_status oosg_plan_tile_policy(oosg_plan *plan, int32_t policy);
+/* The streams of the factorisation in `precision` (no counterpart in the reference): 1, the plan's stream only
+ * (the default), or 3, the bulk of each step's trailing update and the whole inverses beside the critical path, ordered
+ * by events. The results are the same bits either way. */
+oosg_status oosg_plan_streams(oosg_plan *plan, int32_t precision, int32_t count);
 /* The autotuner (no counterpart in the reference): factorises `matrix` (of the plan's structure) twice with each tile
  * size in `precision`, times every update-matrix launch with HIP events, and keeps per level the faster size (policy
- * 2). *big_launches receives the launches that chose 128 x 128 tiles, *tuned_launches those measured (either may be
- * null). The factors are discarded; the captured graphs of that precision are rebuilt on the next factorisation. */
+ * 2). Then it times whole factorisations on the plan's stream alone and with the trailing updates' bulk and the whole
+ * inverses on two more streams beside it, and keeps the faster. *big_launches receives the launches that chose
+ * 128 x 128 tiles, *tuned_launches those measured, *streams the streams kept (1 or 3); any may be null. The factors
+ * are discarded; the captured graphs are rebuilt on the next factorisation. */
 oosg_status oosg_plan_autotune(oosg_plan *plan, const oosg_matrix *matrix, int32_t precision, int32_t *big_launches,
-                               int32_t *tuned_launches);
+                               int32_t *tuned_launches, int32_t *streams);
 
 /* Diagnostic: the factorisation's launches in order, with their profile kind, level and algorithmic flops, and the
  * device time of each in the last autotuner run (ms; -1 before one). Call with null arrays for the count. */

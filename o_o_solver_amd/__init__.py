@@ -317,14 +317,20 @@ class Plan:
                                                 _ptr(flops)))
         return kinds, levels, ms, flops
 
+    def streams(self, precision, count):
+        """The factorisation's streams in `precision`: 1 (default) or 3 (trailing updates and whole inverses beside
+        the critical path)."""
+        _check(self._lib.oosg_plan_streams(self.handle, int(Precision(precision)), int(count)))
+
     def autotune(self, device_matrix, precision=None):
         """Times both tile sizes of every update-matrix launch on `device_matrix` (a matrix of this plan) and keeps
-        the faster per level. Returns (launches that chose 128 × 128 tiles, launches measured)."""
+        the faster per level, then one stream against three for the whole factorisation. Returns (launches that chose
+        128 × 128 tiles, launches measured, streams kept)."""
         precision = Precision.FP64 if precision is None else Precision(precision)
-        big, tuned = ctypes.c_int32(), ctypes.c_int32()
+        big, tuned, streams = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(self._lib.oosg_plan_autotune(self.handle, device_matrix.handle, int(precision), ctypes.byref(big),
-                                            ctypes.byref(tuned)))
-        return big.value, tuned.value
+                                            ctypes.byref(tuned), ctypes.byref(streams)))
+        return big.value, tuned.value, streams.value
 
     def __del__(self):
         if getattr(self, "handle", None):

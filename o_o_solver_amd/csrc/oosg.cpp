@@ -121,6 +121,7 @@ template <typename T> DeviceBuffer upload(const std::vector<T>& host, hipStream_
 
 enum class LaunchKind
 {
+    Sync, // no kernel: only the launch's wait and record
     Small,
     Assemble,
     Diagonal,
@@ -164,6 +165,30 @@ struct Launch
     int bigCount = 0;
     bool big[2] = {false, false};
     int level = 0;
+    // The stream it runs on (LANE_CRITICAL, LANE_BULK, LANE_INVERSES), and the events it waits for before and
+    // records after it on that stream (-1: none).
+    int lane = 0, wait = -1, record = -1;
+};
+
+// The factorisation's streams: the critical path (diagonal blocks, panel solves, the next panel's update, assembly
+// and update matrices) on the plan's stream; the rest of each step's trailing update beside it; the whole inverses of
+// a level's wide triangles beside both, overlapping the next levels.
+enum
+{
+    LANE_CRITICAL = 0,
+    LANE_BULK = 1,
+    LANE_INVERSES = 2,
+    LANES = 3
+};
+enum
+{
+    EVENT_SOLVED = 0, // a step's panel solve is done: the bulk update may start
+    EVENT_RESTED,     // the bulk update of a step is done
+    EVENT_PANELS,     // a level's panels are final: its whole inverses may start
+    EVENT_FORK,
+    EVENT_JOIN_BULK,
+    EVENT_JOIN_INVERSES,
+    EVENTS
 };
 
 // Per-launch device times by HIP events on the plan's stream, summed by kernel kind.
@@ -288,6 +313,11 @@ struct oosg_plan
     int device = 0;
     hipStream_t stream = nullptr;
     bool ownsStream = true; // false: the stream of another plan (the structure of a pivoted factor)
+    hipStream_t lanes[LANES] = {nullptr, nullptr, nullptr}; // lanes[0] = stream
+    hipEvent_t laneEvents[EVENTS] = {};
+    // Whether the factorisation uses the bulk and inverse streams beside the critical one (set by the autotuner when
+    // it measures faster; never while the profiler times single launches).
+    bool concurrent[2] = {false, false}; // per precision
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
@@ -377,6 +407,12 @@ struct oosg_plan
             (void)hipEventDestroy(start);
         if (stop)
             (void)hipEventDestroy(stop);
+        for (hipEvent_t e : laneEvents)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (int lane = 1; lane < LANES; ++lane)
+            if (lanes[lane] && ownsStream)
+                (void)hipStreamDestroy(lanes[lane]);
         if (stream && ownsStream)
             (void)hipStreamDestroy(stream);
     }
@@ -674,9 +710,25 @@ double tileFlops(const oosg::UpdateTile& t)
     return 2.0 * t.k * elements;
 }
 
+Launch syncLaunch(int lane, int wait, int record)
+{
+    Launch l{LaunchKind::Sync, PROFILE_SMALL};
+    l.lane = lane;
+    l.wait = wait;
+    l.record = record;
+    return l;
+}
+
 void buildSchedule(oosg_plan& p)
 {
     using oosg::PANEL_BLOCK;
+    // Columns of a super-block: the trailing update beyond it is one product with K = SUPER_BLOCK.
+    constexpr int SUPER_BLOCK = 4 * PANEL_BLOCK;
+    // Every event is recorded once on the critical stream first, so that each wait refers to a record of the same
+    // sequence (a captured graph has no outside dependencies).
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_SOLVED));
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_RESTED));
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_PANELS));
     for (int32_t l = 0; l < p.levels; ++l)
     {
         const size_t firstLaunch = p.schedule.size();
@@ -816,44 +868,74 @@ void buildSchedule(oosg_plan& p)
                 }
             }
             solve.count = static_cast<int>(p.tiles.size() - solve.offset);
-            Launch update{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
-            const int64_t tileStart = static_cast<int64_t>(p.tiles.size());
-            for (const int32_t s : large)
+            // The trailing update, blocked twice: a step updates only the columns of its own super-block of
+            // SUPER_BLOCK columns (K = 64); after the super-block's last step, one product with K = SUPER_BLOCK
+            // updates the columns after it: the next super-block on the critical stream (its steps need them), the
+            // columns after that on the bulk stream, beside the next super-block's steps.
+            const int k1Step = k0 + PANEL_BLOCK;
+            const int super0 = k0 - k0 % SUPER_BLOCK, super1 = super0 + SUPER_BLOCK;
+            const auto addUpdates = [&](Launch& update, int colFrom, int colTo, int t0, int t1)
             {
-                const int c = p.ncol[s], n = c + p.nrow[s];
-                if (c <= k0)
-                    continue;
-                const int kb = std::min(PANEL_BLOCK, c - k0), k1 = k0 + kb;
-                for (int J0 = k1; J0 < c; J0 += 64)
-                    for (int I0 = J0; I0 < n; I0 += 64)
-                    {
-                        oosg::UpdateTile t{};
-                        t.cOff = p.panelOff[s] + I0 + int64_t(J0) * n;
-                        t.ldc = n;
-                        t.aOff = p.panelOff[s] + I0 + int64_t(k0) * n;
-                        t.bOff = p.panelOff[s] + J0 + int64_t(k0) * n;
-                        t.dOff = p.panelOff[s] + int64_t(k0) * (n + 1);
-                        t.ld = n;
-                        t.m = std::min(64, n - I0);
-                        t.n = std::min(64, c - J0);
-                        t.k = kb;
-                        t.target = 0;
-                        t.lower = I0 == J0;
-                        p.tiles.push_back(t);
-                        update.flops += tileFlops(t);
-                    }
+                update.offset = static_cast<int64_t>(p.tiles.size());
+                for (const int32_t s : large)
+                {
+                    const int c = p.ncol[s], n = c + p.nrow[s];
+                    const int from = colFrom, to = std::min(c, colTo);
+                    const int termsEnd = std::min(c, t1);
+                    if (termsEnd <= t0 || from >= to)
+                        continue;
+                    for (int J0 = from; J0 < to; J0 += 64)
+                        for (int I0 = J0; I0 < n; I0 += 64)
+                        {
+                            oosg::UpdateTile t{};
+                            t.cOff = p.panelOff[s] + I0 + int64_t(J0) * n;
+                            t.ldc = n;
+                            t.aOff = p.panelOff[s] + I0 + int64_t(t0) * n;
+                            t.bOff = p.panelOff[s] + J0 + int64_t(t0) * n;
+                            t.dOff = p.panelOff[s] + int64_t(t0) * (n + 1);
+                            t.ld = n;
+                            t.m = std::min(64, n - I0);
+                            t.n = std::min(64, to - J0);
+                            t.k = termsEnd - t0;
+                            t.target = 0;
+                            t.lower = I0 == J0;
+                            p.tiles.push_back(t);
+                            update.flops += tileFlops(t);
+                        }
+                }
+                update.count = static_cast<int>(p.tiles.size() - update.offset);
+            };
+            Launch inner{LaunchKind::Tiles, PROFILE_PANEL_UPDATE}, next{LaunchKind::Tiles, PROFILE_PANEL_UPDATE},
+                rest{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
+            addUpdates(inner, k1Step, super1, k0, k1Step);
+            if (k1Step == super1)
+            {
+                addUpdates(next, super1, super1 + SUPER_BLOCK, super0, super1);
+                addUpdates(rest, super1 + SUPER_BLOCK, INT32_MAX, super0, super1);
             }
             diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
+            diagonal.lane = solve.lane = inner.lane = next.lane = LANE_CRITICAL;
+            (solve.count > 0 ? solve : diagonal).record = EVENT_SOLVED;
             p.schedule.push_back(diagonal);
             if (solve.count > 0)
                 p.schedule.push_back(solve);
-            if (static_cast<int64_t>(p.tiles.size()) > tileStart)
+            if (inner.count > 0)
+                p.schedule.push_back(inner);
+            if (next.count > 0)
             {
-                update.offset = tileStart;
-                update.count = static_cast<int>(p.tiles.size() - tileStart);
-                p.schedule.push_back(update);
+                next.wait = EVENT_RESTED; // the previous super-block's bulk update wrote these columns too
+                p.schedule.push_back(next);
+            }
+            if (rest.count > 0)
+            {
+                rest.lane = LANE_BULK;
+                rest.wait = EVENT_SOLVED;
+                rest.record = EVENT_RESTED;
+                p.schedule.push_back(rest);
             }
         }
+        // The level's panels are final once the bulk stream is done with them.
+        p.schedule.push_back(syncLaunch(LANE_CRITICAL, EVENT_RESTED, EVENT_PANELS));
         // The update matrices: U(r × r) = −L_R·D·L_Rᵀ + the children's parts, K = c. Each 64 × 64 quarter of a tile
         // lists its children's parts (the assembly items of its rows and columns).
         const auto addQuarterItems = [&p](int32_t s, int i0, int i1, int j0, int j1)
@@ -930,6 +1012,7 @@ void buildSchedule(oosg_plan& p)
             p.schedule.push_back(tilesLaunch);
         }
         // The whole inverses of the level's wide triangles (their panels are final), one launch per step.
+        bool firstRound = true;
         for (const oosg_plan::InverseRound& round : p.inverseRounds[l])
         {
             Launch inverse{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
@@ -937,9 +1020,20 @@ void buildSchedule(oosg_plan& p)
             inverse.count = round.count;
             inverse.k0 = round.step;
             inverse.flops = round.flops;
+            inverse.lane = LANE_INVERSES;
+            if (firstRound)
+                inverse.wait = EVENT_PANELS;
+            firstRound = false;
             p.schedule.push_back(inverse);
         }
     }
+    // The other streams join the critical one (each first waits for a record of this sequence, so that both belong to
+    // it even when they had nothing to do).
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_FORK));
+    p.schedule.push_back(syncLaunch(LANE_BULK, EVENT_FORK, EVENT_JOIN_BULK));
+    p.schedule.push_back(syncLaunch(LANE_INVERSES, EVENT_FORK, EVENT_JOIN_INVERSES));
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, EVENT_JOIN_BULK, -1));
+    p.schedule.push_back(syncLaunch(LANE_CRITICAL, EVENT_JOIN_INVERSES, -1));
 }
 
 // The arrays the sweeps and the whole inverses read, and the device view of the structure.
@@ -1018,26 +1112,44 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     for (size_t li = 0; li < p.schedule.size(); ++li)
     {
         const Launch& l = p.schedule[li];
+        const bool lanes = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] && !p.profiler.enabled &&
+                           p.lanes[l.lane] != nullptr;
+        const hipStream_t ls = lanes ? p.lanes[l.lane] : st;
+        if (lanes && l.wait >= 0)
+            check(hipStreamWaitEvent(ls, p.laneEvents[l.wait], 0), "factor: stream order");
+        if (l.kind == LaunchKind::Sync)
+        {
+            if (lanes && l.record >= 0)
+                check(hipEventRecord(p.laneEvents[l.record], ls), "factor: stream order");
+            if (timing)
+            {
+                check(hipEventRecord(p.launchEvents[2 * li], ls), "hipEventRecord");
+                check(hipEventRecord(p.launchEvents[2 * li + 1], ls), "hipEventRecord");
+            }
+            continue;
+        }
         if (timing)
-            check(hipEventRecord(p.launchEvents[2 * li], st), "hipEventRecord");
+            check(hipEventRecord(p.launchEvents[2 * li], ls), "hipEventRecord");
         const int precision = sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64;
         const bool big = l.kind == LaunchKind::Tiles && l.bigCount > 0 &&
                          (p.tilePolicy == 1 || (p.tilePolicy == 2 && l.big[precision]));
-        const int slot = p.profiler.begin(big ? PROFILE_UPDATE_MATRIX_BIG : l.profile, l.flops, st);
+        const int slot = p.profiler.begin(big ? PROFILE_UPDATE_MATRIX_BIG : l.profile, l.flops, ls);
         switch (l.kind)
         {
+        case LaunchKind::Sync:
+            break;
         case LaunchKind::Small:
             check(oosg::launchSmallFronts<T>(p.dp, a, panels, updates, linv, smallList + l.offset, l.count,
-                                             l.maxOrder, diagOut, zeroAt, st),
+                                             l.maxOrder, diagOut, zeroAt, ls),
                   "factor: small fronts");
             break;
         case LaunchKind::Assemble:
-            check(oosg::launchAssembly<T>(p.dp, a, panels, updates, blocks + l.offset, items, l.count, st),
+            check(oosg::launchAssembly<T>(p.dp, a, panels, updates, blocks + l.offset, items, l.count, ls),
                   "factor: assembly");
             break;
         case LaunchKind::Diagonal:
             check(oosg::launchDiagonalBlocks<T>(p.dp, panels, linv, stepFronts + l.offset, l.count, l.k0, diagOut,
-                                                zeroAt, st),
+                                                zeroAt, ls),
                   "factor: diagonal blocks");
             break;
         case LaunchKind::Tiles:
@@ -1047,21 +1159,23 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
                           p.dp, panels, updates,
                           big ? p.dBigTiles.as<oosg::UpdateTile>() + l.bigOffset : tiles + l.offset,
                           big ? l.bigCount : l.count, p.dItemStarts.as<int32_t>(),
-                          p.dExtendItems.as<oosg::AssemblyItem>(), big ? 128 : 64, st),
+                          p.dExtendItems.as<oosg::AssemblyItem>(), big ? 128 : 64, ls),
                       "factor: update matrices");
             else
-                check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, st),
+                check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, ls),
                       "factor: update tiles");
             break;
         }
         case LaunchKind::Inverse:
-            check(oosg::launchTriangleInverse<T>(p.dp, panels, linv, inverseItems + l.offset, l.count, l.k0, st),
+            check(oosg::launchTriangleInverse<T>(p.dp, panels, linv, inverseItems + l.offset, l.count, l.k0, ls),
                   "factor: triangle inverses");
             break;
         }
-        p.profiler.end(slot, st);
+        p.profiler.end(slot, ls);
+        if (lanes && l.record >= 0)
+            check(hipEventRecord(p.laneEvents[l.record], ls), "factor: stream order");
         if (timing)
-            check(hipEventRecord(p.launchEvents[2 * li + 1], st), "hipEventRecord");
+            check(hipEventRecord(p.launchEvents[2 * li + 1], ls), "hipEventRecord");
     }
     if (timing)
     {
@@ -2059,6 +2173,11 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                          std::to_string((static_cast<uint64_t>(p->updateOff[p->S]) * 8) >> 20) + " MiB), " +
                          std::to_string(p->budget >> 20) + " MiB are available");
             check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+            p->lanes[LANE_CRITICAL] = p->stream;
+            for (int lane = 1; lane < LANES; ++lane)
+                check(hipStreamCreateWithFlags(&p->lanes[lane], hipStreamNonBlocking), "hipStreamCreate");
+            for (hipEvent_t& e : p->laneEvents)
+                check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             check(oosg::configureKernels(), "plan: kernel limits");
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
@@ -2090,7 +2209,8 @@ oosg_status oosg_plan_stats(const oosg_plan* plan, int64_t* stats)
             stats[3] = plan->updateOff[plan->S];
             stats[4] = plan->smallFronts;
             stats[5] = plan->largeFronts;
-            stats[6] = static_cast<int64_t>(plan->schedule.size());
+            stats[6] = std::count_if(plan->schedule.begin(), plan->schedule.end(),
+                                     [](const Launch& l) { return l.kind != LaunchKind::Sync; });
             stats[7] = static_cast<int64_t>(plan->tiles.size());
         });
 }
@@ -2413,6 +2533,23 @@ oosg_status oosg_plan_launch_times(const oosg_plan* plan, int32_t* count, int32_
         });
 }
 
+oosg_status oosg_plan_streams(oosg_plan* plan, int32_t precision, int32_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            if (precision != OOSG_FP64 && precision != OOSG_FP32)
+                fail(OOSG_INVALID_ARGUMENT, "plan: unknown precision");
+            if (count != 1 && count != LANES)
+                fail(OOSG_INVALID_ARGUMENT, "plan: the factorisation runs on 1 or 3 streams");
+            setDevice(plan);
+            plan->concurrent[precision] = count == LANES;
+            plan->factorGraph[precision].clear();
+        });
+}
+
 oosg_status oosg_plan_tile_policy(oosg_plan* plan, int32_t policy)
 {
     return guarded(
@@ -2433,7 +2570,7 @@ oosg_status oosg_plan_tile_policy(oosg_plan* plan, int32_t policy)
 }
 
 oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32_t precision, int32_t* big_launches,
-                               int32_t* tuned_launches)
+                               int32_t* tuned_launches, int32_t* streams)
 {
     return guarded(
         [&]
@@ -2501,11 +2638,43 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
             }
             p.tilePolicy = 2;
             p.tuned[precision] = true;
-            p.factorGraph[precision].clear();
+            // Then the whole factorisation, on one stream and on three (the best of three runs each, after a warm
+            // one): the streams stay only where they measure faster.
+            const auto wall = [&](bool lanes)
+            {
+                p.concurrent[precision] = lanes;
+                float best = INFINITY;
+                for (int run = 0; run < 4; ++run)
+                {
+                    check(hipEventRecord(p.start, p.stream), "hipEventRecord");
+                    check(oosg::launchFillInt32(zero.as<int32_t>(), p.S, INT32_MAX, p.stream), "autotune: status");
+                    if (precision == OOSG_FP32)
+                        runFactor<float>(p, *matrix, panels.as<float>(), linv.as<float>(), diag.as<double>(),
+                                         zero.as<int32_t>(), true);
+                    else
+                        runFactor<double>(p, *matrix, panels.as<double>(), linv.as<double>(), diag.as<double>(),
+                                          zero.as<int32_t>(), false);
+                    check(hipEventRecord(p.stop, p.stream), "hipEventRecord");
+                    check(hipEventSynchronize(p.stop), "autotune");
+                    float ms = 0.0f;
+                    check(hipEventElapsedTime(&ms, p.start, p.stop), "hipEventElapsedTime");
+                    if (run > 0)
+                        best = std::min(best, ms);
+                }
+                return best;
+            };
+            p.profiler.enabled = false;
+            const float serialMs = wall(false), lanesMs = wall(true);
+            p.profiler.enabled = profiling;
+            p.concurrent[precision] = lanesMs < serialMs;
+            p.factorGraph[0].clear();
+            p.factorGraph[1].clear();
             if (big_launches != nullptr)
                 *big_launches = chosen;
             if (tuned_launches != nullptr)
                 *tuned_launches = tuned;
+            if (streams != nullptr)
+                *streams = p.concurrent[precision] ? LANES : 1;
         });
 }
 
@@ -2814,7 +2983,7 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
                     st,
                     {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,
                      matrix->dColumn.ptr, matrix->dOffsets.ptr, matrix->dRows.ptr, matrix->dScale.ptr,
-                     plan->updates.ptr},
+                     plan->updates.ptr, reinterpret_cast<const void*>(static_cast<intptr_t>(plan->concurrent[opts.precision]))},
                     body);
             else
                 body();

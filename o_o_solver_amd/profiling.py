@@ -23,7 +23,7 @@ COPY_BYTES = 1 << 31  # the calibration copy: 2^28 float64 values read and writt
 
 
 def kernel_name(full):
-    """'void oosg::updateTilesKernel<double>(double*, ...)' → 'updateTilesKernel<double>'."""
+    """'void oosg::extendTilesKernel<double, 64>(...)' → 'extendTilesKernel<double,64>'."""
     name = full.strip().replace("(anonymous namespace)::", "")
     if name.startswith("void "):
         name = name[5:]
@@ -36,7 +36,7 @@ def kernel_name(full):
         elif ch == "(" and depth == 0:
             end = i
             break
-    name = name[:end]
+    name = name[:end].replace(", ", ",")  # one token: the summaries split their lines on spaces
     return re.sub(r"^(oosg::)+", "", name)
 
 

@@ -332,10 +332,38 @@ def test_big_tiles_give_the_same_bits_and_the_autotuner_keeps_them(gpu_device):
         f1 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
         assert all(np.array_equal(f1.panel(s)[0], panels0[s]) for s in range(sym.supernode_count)), precision
         del f1
-        big, tuned = plan.autotune(dm, precision)
-        print(f"[gpu] {precision.name}: {big} of {tuned} update-matrix launches chose 128 x 128 tiles", flush=True)
-        assert tuned > 0 and 0 <= big <= tuned
+        big, tuned, streams = plan.autotune(dm, precision)
+        print(f"[gpu] {precision.name}: {big} of {tuned} update-matrix launches chose 128 x 128 tiles, {streams} "
+              "streams", flush=True)
+        assert tuned > 0 and 0 <= big <= tuned and streams in (1, 3)
         f2 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
         assert all(np.array_equal(f2.panel(s)[0], panels0[s]) for s in range(sym.supernode_count)), precision
         assert np.array_equal(FactorSolve(f2).solve(b), x0)
         del f2
+
+
+def test_three_streams_give_the_same_bits(gpu_device):
+    # The trailing updates' bulk and the whole inverses on streams of their own, ordered by events: eager and captured
+    # runs give the one-stream factor bit for bit.
+    print("[gpu] three streams", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    plan = sym.plan()
+    dm = plan.matrix(w.matrix)
+    b = w.loads("random", nrhs=2, seed=4)
+    for precision in (Precision.FP64, Precision.FP32):
+        options = FactorOptions(precision=precision)
+        plan.streams(precision, 1)
+        f = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+        panels = [f.panel(s)[0] for s in range(sym.supernode_count)]
+        x = FactorSolve(f).solve(b)
+        del f
+        plan.streams(precision, 3)
+        for graphs in (False, True, True):
+            plan.use_graphs(graphs)
+            g = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+            assert all(np.array_equal(g.panel(s)[0], panels[s]) for s in range(sym.supernode_count)), (precision,
+                                                                                                        graphs)
+            assert np.array_equal(FactorSolve(g).solve(b), x)
+            del g
+        plan.streams(precision, 1)
