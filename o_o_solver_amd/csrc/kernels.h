@@ -155,9 +155,11 @@ hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* 
 template <typename T>
 hipError_t launchExtendTiles(const DevicePlan& p, T* panels, T* updates, const UpdateTile* tiles, int count,
                              const int32_t* itemStarts, const AssemblyItem* items, int tileSize, hipStream_t stream);
+// Panel solves (mode 1) and panel updates (mode 0); a mode-2 tile also factorises the diagonal block it finishes
+// (writing L, D, L11⁻¹, diagOut and zeroAt as the diagonal-block kernel does).
 template <typename T>
-hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
-                             hipStream_t stream);
+hipError_t launchUpdateTiles(const DevicePlan& p, T* panels, T* updates, T* linv, const UpdateTile* tiles, int count,
+                             double* diagOut, int32_t* zeroAt, bool fused, hipStream_t stream);
 // One level of the forward sweep: the supernodes of `list`, then the row tiles (supernode, tile) of its big ones.
 template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,

@@ -361,13 +361,18 @@ template <> struct Mfma<float>
 //     first one's accumulator used directly as the second one's operand (its k-steps follow the accumulator rows).
 // LDS: S[j·LD + i] holds L(i, j) for i > j, D(j) for i = j, and X(j, i) for i < j (the inverse, transposed, in the
 // upper triangle, which the elimination leaves free).
+// Factors the block held in S (LD = PANEL_BLOCK + 1: S[j·LD + i] = F(k0 + i, k0 + j) for i >= j, the identity
+// outside the kb × kb block) and writes it back: L and D into the panel, L11⁻¹ into the inverses, D into diagOut; a
+// zero or non-finite pivot is recorded in zeroAt. One workgroup of four waves.
+template <typename T>
+__device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int kb, T* panels, T* linv,
+                                 double* diagOut, int32_t* zeroAt);
+
 template <typename T>
 __global__ __launch_bounds__(256, 2) void diagonalBlockKernel(DevicePlan p, T* panels, T* linv, const int32_t* list,
                                                            int k0, double* diagOut, int32_t* zeroAt)
 {
-    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1, SP = 16;
-    static_assert(B == 64, "four sub-panels of 16 columns");
-    typedef typename Mfma<T>::Acc Acc;
+    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1;
     __shared__ T S[B * LD];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], n = c + p.nrow[s];
@@ -401,6 +406,20 @@ __global__ __launch_bounds__(256, 2) void diagonalBlockKernel(DevicePlan p, T* p
         }
     }
     __syncthreads();
+    factorBlockInLds(S, p, s, k0, kb, panels, linv, diagOut, zeroAt);
+}
+
+template <typename T>
+__device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int kb, T* panels, T* linv,
+                                 double* diagOut, int32_t* zeroAt)
+{
+    constexpr int B = PANEL_BLOCK, LD = PANEL_BLOCK + 1, SP = 16;
+    static_assert(B == 64, "four sub-panels of 16 columns");
+    typedef typename Mfma<T>::Acc Acc;
+    const int c = p.ncol[s], n = c + p.nrow[s];
+    T* panel = panels + p.panelOff[s];
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int64_t base = k0 + int64_t(k0) * n;
 
     for (int c0 = 0; c0 < kb; c0 += SP)
     {
@@ -553,13 +572,19 @@ __device__ inline int tileOfWorkgroup(int count)
 // operands; mode 0 accumulates onto C itself (the accumulators start from the C tile, A enters negated).
 // PLAIN: C −= A·Bᵀ with B in the third array (leading dimension ldb) and no diagonal: the trailing updates of the
 // pivoted fronts, whose B holds the rows of L·D already (D has 2 × 2 blocks there).
-template <typename T, bool PLAIN = false>
-__global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, const T* linv,
-                                                         const UpdateTile* tiles, int count)
+// Mode 2 (a diagonal tile of the next panel step's block): after its update the tile goes to LDS and its block is
+// factorised there (factorBlockInLds, the diagonal-block step fused into the update that finishes the block);
+// rowBase holds the block's first column and colBase its width.
+template <typename T, bool PLAIN = false, bool FUSE = false>
+__global__ __launch_bounds__(256, FUSE ? 2 : 1) void updateTilesKernel(DevicePlan p, T* panels, T* updates, T* linv,
+                                                                       const UpdateTile* tiles, int count,
+                                                                       double* diagOut, int32_t* zeroAt)
 {
     constexpr int PER = TILE * TILE_K / 256;
-    __shared__ T As[TILE_K][TILE + 4];
-    __shared__ T Bs[TILE_K][TILE + 4];
+    constexpr int SHARED = 2 * TILE_K * (TILE + 4) > TILE * (TILE + 1) ? 2 * TILE_K * (TILE + 4) : TILE * (TILE + 1);
+    __shared__ T lds[SHARED];
+    T (*As)[TILE + 4] = reinterpret_cast<T (*)[TILE + 4]>(lds);
+    T (*Bs)[TILE + 4] = reinterpret_cast<T (*)[TILE + 4]>(lds + TILE_K * (TILE + 4));
     const UpdateTile t = tiles[tileOfWorkgroup(count)];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -631,6 +656,33 @@ __global__ __launch_bounds__(256) void updateTilesKernel(T* panels, T* updates, 
                 for (int b = 0; b < 2; ++b)
                     acc[a][b] = Mfma<T>::step(av[a], bv[b], acc[a][b]);
         }
+    }
+    if (FUSE && t.mode == 2)
+    {
+        // The block [k0, k0 + kb)², now final, into LDS (the identity outside it), and factorised there.
+        constexpr int LD = TILE + 1;
+        const int kb = t.colBase;
+        __syncthreads(); // the products have read the LDS
+        for (int idx = tid; idx < TILE * LD; idx += 256)
+            lds[idx] = T(idx / LD == idx % LD ? 1 : 0);
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int i = wm + a * 16 + Mfma<T>::row(lane, v);
+                    const int j = wn + b * 16 + (lane & 15);
+                    if (i < kb && j < kb && i >= j)
+                        lds[j * LD + i] = acc[a][b][v];
+                    else if (i >= kb && i < t.m && j < t.n)
+                        C[i + int64_t(j) * t.ldc] = acc[a][b][v]; // rows below a partial block: updated as usual
+                }
+        __syncthreads();
+        factorBlockInLds(lds, p, t.supernode, t.rowBase, kb, panels, linv, diagOut, zeroAt);
+        return;
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -1730,12 +1782,17 @@ hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const i
 }
 
 template <typename T>
-hipError_t launchUpdateTiles(T* panels, T* updates, const T* linv, const UpdateTile* tiles, int count,
-                             hipStream_t stream)
+hipError_t launchUpdateTiles(const DevicePlan& p, T* panels, T* updates, T* linv, const UpdateTile* tiles, int count,
+                             double* diagOut, int32_t* zeroAt, bool fused, hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(updateTilesKernel<T>, dim3(count), dim3(256), 0, stream, panels, updates, linv, tiles, count);
+    if (fused)
+        hipLaunchKernelGGL((updateTilesKernel<T, false, true>), dim3(count), dim3(256), 0, stream, p, panels, updates,
+                           linv, tiles, count, diagOut, zeroAt);
+    else
+        hipLaunchKernelGGL((updateTilesKernel<T, false, false>), dim3(count), dim3(256), 0, stream, p, panels,
+                           updates, linv, tiles, count, diagOut, zeroAt);
     return hipGetLastError();
 }
 
@@ -1761,8 +1818,8 @@ hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateT
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL((updateTilesKernel<double, true>), dim3(count), dim3(256), 0, stream, fronts, nullptr, w,
-                       tiles, count);
+    hipLaunchKernelGGL((updateTilesKernel<double, true>), dim3(count), dim3(256), 0, stream, DevicePlan{}, fronts,
+                       nullptr, const_cast<double*>(w), tiles, count, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -1950,7 +2007,8 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
                                           const AssemblyItem*, int, hipStream_t);                                      \
     template hipError_t launchDiagonalBlocks<T>(const DevicePlan&, T*, T*, const int32_t*, int, int, double*,          \
                                                 int32_t*, hipStream_t);                                                \
-    template hipError_t launchUpdateTiles<T>(T*, T*, const T*, const UpdateTile*, int, hipStream_t);                   \
+    template hipError_t launchUpdateTiles<T>(const DevicePlan&, T*, T*, T*, const UpdateTile*, int, double*, int32_t*, \
+                                             bool, hipStream_t);                                                       \
     template hipError_t launchExtendTiles<T>(const DevicePlan&, T*, T*, const UpdateTile*, int, const int32_t*,        \
                                              const AssemblyItem*, int, hipStream_t);                                   \
     template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \

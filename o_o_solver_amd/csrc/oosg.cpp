@@ -168,6 +168,7 @@ struct Launch
     // The stream it runs on (LANE_CRITICAL, LANE_BULK, LANE_INVERSES), and the events it waits for before and
     // records after it on that stream (-1: none).
     int lane = 0, wait = -1, record = -1;
+    bool fuses = false; // tile launches: some tiles factorise the diagonal block they finish (mode 2)
 };
 
 // The factorisation's streams: the critical path (diagonal blocks, panel solves, the next panel's update, assembly
@@ -874,7 +875,9 @@ void buildSchedule(oosg_plan& p)
             // columns after that on the bulk stream, beside the next super-block's steps.
             const int k1Step = k0 + PANEL_BLOCK;
             const int super0 = k0 - k0 % SUPER_BLOCK, super1 = super0 + SUPER_BLOCK;
-            const auto addUpdates = [&](Launch& update, int colFrom, int colTo, int t0, int t1)
+            // `finishes`: the first column of the block whose diagonal tile, in this update, gets its last
+            // contribution (-1: none).
+            const auto addUpdates = [&](Launch& update, int colFrom, int colTo, int t0, int t1, int finishes)
             {
                 update.offset = static_cast<int64_t>(p.tiles.size());
                 for (const int32_t s : large)
@@ -899,6 +902,16 @@ void buildSchedule(oosg_plan& p)
                             t.k = termsEnd - t0;
                             t.target = 0;
                             t.lower = I0 == J0;
+                            if (I0 == J0 && J0 == finishes)
+                            {
+                                // This update finishes the next step's diagonal block: factorise it here.
+                                t.mode = 2;
+                                t.supernode = s;
+                                t.rowBase = J0;
+                                t.colBase = std::min(PANEL_BLOCK, c - J0);
+                                update.flops += frontFlops(t.colBase, 0);
+                                update.fuses = true;
+                            }
                             p.tiles.push_back(t);
                             update.flops += tileFlops(t);
                         }
@@ -907,16 +920,18 @@ void buildSchedule(oosg_plan& p)
             };
             Launch inner{LaunchKind::Tiles, PROFILE_PANEL_UPDATE}, next{LaunchKind::Tiles, PROFILE_PANEL_UPDATE},
                 rest{LaunchKind::Tiles, PROFILE_PANEL_UPDATE};
-            addUpdates(inner, k1Step, super1, k0, k1Step);
+            addUpdates(inner, k1Step, super1, k0, k1Step, k1Step < super1 ? k1Step : -1);
             if (k1Step == super1)
             {
-                addUpdates(next, super1, super1 + SUPER_BLOCK, super0, super1);
-                addUpdates(rest, super1 + SUPER_BLOCK, INT32_MAX, super0, super1);
+                addUpdates(next, super1, super1 + SUPER_BLOCK, super0, super1, super1);
+                addUpdates(rest, super1 + SUPER_BLOCK, INT32_MAX, super0, super1, -1);
             }
             diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
             diagonal.lane = solve.lane = inner.lane = next.lane = LANE_CRITICAL;
             (solve.count > 0 ? solve : diagonal).record = EVENT_SOLVED;
-            p.schedule.push_back(diagonal);
+            // From the second step on, the previous step's update factorised this step's diagonal blocks.
+            if (k0 == 0)
+                p.schedule.push_back(diagonal);
             if (solve.count > 0)
                 p.schedule.push_back(solve);
             if (inner.count > 0)
@@ -1162,7 +1177,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
                           p.dExtendItems.as<oosg::AssemblyItem>(), big ? 128 : 64, ls),
                       "factor: update matrices");
             else
-                check(oosg::launchUpdateTiles<T>(panels, updates, linv, tiles + l.offset, l.count, ls),
+                check(oosg::launchUpdateTiles<T>(p.dp, panels, updates, linv, tiles + l.offset, l.count, diagOut,
+                                                 zeroAt, l.fuses, ls),
                       "factor: update tiles");
             break;
         }
