@@ -275,7 +275,8 @@ class Plan:
 
     PROFILE_KINDS = ["small_fronts", "assembly", "diagonal_blocks", "panel_solve", "panel_update_tiles",
                      "update_matrix_tiles", "forward_triangles", "backward_triangles", "residual", "forward_rows",
-                     "backward_columns", "triangle_products", "triangle_inverses", "update_matrix_big_tiles"]
+                     "backward_columns", "triangle_products", "triangle_inverses", "update_matrix_big_tiles",
+                     "panel_update_fused_tiles"]
 
     def profile(self, enable=True):
         """Per-kernel device times by HIP events on the plan's stream (enabling resets the sums)."""

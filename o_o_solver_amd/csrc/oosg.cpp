@@ -146,6 +146,7 @@ enum ProfileKind
     PROFILE_TRIANGLE_PRODUCTS, // both sweeps: the wide triangles, products with their whole inverses
     PROFILE_TRIANGLE_INVERSES, // factorisation: the whole inverses of the wide triangles
     PROFILE_UPDATE_MATRIX_BIG, // factorisation: update matrices on 128 × 128 tiles
+    PROFILE_PANEL_UPDATE_FUSED, // factorisation: panel updates that also factorise the next diagonal blocks
     PROFILE_KINDS
 };
 static_assert(PROFILE_KINDS == OOSG_PROFILE_KINDS, "the profile kinds of the header");
@@ -911,6 +912,7 @@ void buildSchedule(oosg_plan& p)
                                 t.colBase = std::min(PANEL_BLOCK, c - J0);
                                 update.flops += frontFlops(t.colBase, 0);
                                 update.fuses = true;
+                                update.profile = PROFILE_PANEL_UPDATE_FUSED;
                             }
                             p.tiles.push_back(t);
                             update.flops += tileFlops(t);

@@ -367,3 +367,27 @@ def test_three_streams_give_the_same_bits(gpu_device):
             assert np.array_equal(FactorSolve(g).solve(b), x)
             del g
         plan.streams(precision, 1)
+
+
+def test_profile_counts_every_factor_launch(gpu_device):
+    # With the profiler on, every launch of the factorisation is timed under one kind: the factor kinds' launch counts
+    # add up to the schedule's, and the panel updates that factorise the next diagonal block have a kind of their own
+    # (a kernel instance of their own in a trace).
+    print("[gpu] profile kinds", flush=True)
+    w = CASES["hex 14"]()
+    sym = w.analyse()
+    plan = sym.plan()
+    dm = plan.matrix(w.matrix)
+    plan.profile(True)
+    f = SupernodalFactor.factorise(sym, w.matrix, device_matrix=dm)
+    prof = plan.profile_read()
+    plan.profile(False)
+    factor_kinds = ["small_fronts", "assembly", "diagonal_blocks", "panel_solve", "panel_update_tiles",
+                    "update_matrix_tiles", "triangle_inverses", "update_matrix_big_tiles", "panel_update_fused_tiles"]
+    assert sum(prof[k][1] for k in factor_kinds) == plan.stats()["launches"]
+    assert int(np.max(sym.column_count)) > 64
+    assert prof["panel_update_fused_tiles"][1] > 0 and prof["panel_update_fused_tiles"][0] > 0
+    # Assembly moves values and does no arithmetic: its work is 0; every other factor kind that ran did flops.
+    assert prof["assembly"][1] > 0 and prof["assembly"][2] == 0
+    assert all(prof[k][2] > 0 for k in factor_kinds if prof[k][1] > 0 and k != "assembly")
+    del f
