@@ -188,7 +188,8 @@ def statics_main(args, kind):
     tuning = {}
     for precision in (Precision.FP64, Precision.FP32):
         big, tuned, streams = plan.autotune(dm, precision)
-        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams}
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams,
+                                          "launch_mode": plan.launch_mode(precision)}
     log(f"autotune: {tuning}")
     results = {}
     for name, precision, steps in (("fp64", Precision.FP64, 4), ("fp32_fronts_cg", Precision.FP32, args.max_steps)):
@@ -299,7 +300,8 @@ def main():
     tuning = {}
     for precision in (Precision.FP64, Precision.FP32):
         big, tuned, streams = plan.autotune(dm, precision)
-        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams}
+        tuning[precision.name.lower()] = {"big_tile_launches": big, "launches_tuned": tuned, "streams": streams,
+                                          "launch_mode": plan.launch_mode(precision)}
     log(f"autotune ({time.perf_counter() - t:.1f} s): {tuning}")
 
     from o_o_solver_amd.selector import PrecisionSelector

@@ -207,12 +207,23 @@ oosg_status oosg_plan_tile_policy(oosg_plan *plan, int32_t policy);
  * (the default), or 3, the bulk of each step's trailing update and the whole inverses beside the critical path, ordered
  * by events. The results are the same bits either way. */
 oosg_status oosg_plan_streams(oosg_plan *plan, int32_t precision, int32_t count);
+/* How the factorisation in `precision` is launched (no counterpart in the reference): OOSG_LAUNCH_GRAPH, one HIP graph
+ * on the plan's stream (the default); OOSG_LAUNCH_GRAPH_STREAMS, one graph captured from the three streams
+ * (oosg_plan_streams(…, 3)); OOSG_LAUNCH_EAGER_STREAMS, the three streams launched one launch at a time, for runtimes
+ * that do not overlap a graph's branches as they overlap the streams. With graphs off (oosg_plan_use_graphs) every
+ * mode launches one launch at a time. The same bits in every mode. */
+#define OOSG_LAUNCH_GRAPH 0
+#define OOSG_LAUNCH_GRAPH_STREAMS 1
+#define OOSG_LAUNCH_EAGER_STREAMS 2
+oosg_status oosg_plan_launch_mode(oosg_plan *plan, int32_t precision, int32_t mode);
+oosg_status oosg_plan_get_launch_mode(const oosg_plan *plan, int32_t precision, int32_t *mode);
 /* The autotuner (no counterpart in the reference): factorises `matrix` (of the plan's structure) twice with each tile
  * size in `precision`, times every update-matrix launch with HIP events, and keeps per level the faster size (policy
- * 2). Then it times whole factorisations on the plan's stream alone and with the trailing updates' bulk and the whole
- * inverses on two more streams beside it, and keeps the faster. *big_launches receives the launches that chose
- * 128 x 128 tiles, *tuned_launches those measured, *streams the streams kept (1 or 3); any may be null. The factors
- * are discarded; the captured graphs are rebuilt on the next factorisation. */
+ * 2). Then it times whole factorisations in each launch mode (oosg_plan_launch_mode; the three-stream modes put the
+ * trailing updates' bulk and the whole inverses on two more streams beside the critical path), and keeps the fastest.
+ * *big_launches receives the launches that chose 128 x 128 tiles, *tuned_launches those measured, *streams the streams
+ * kept (1 or 3); any may be null. The factors are discarded; the captured graphs are rebuilt on the next
+ * factorisation. */
 oosg_status oosg_plan_autotune(oosg_plan *plan, const oosg_matrix *matrix, int32_t precision, int32_t *big_launches,
                                int32_t *tuned_launches, int32_t *streams);
 

@@ -323,10 +323,24 @@ class Plan:
         the critical path)."""
         _check(self._lib.oosg_plan_streams(self.handle, int(Precision(precision)), int(count)))
 
+    LAUNCH_MODES = ["graph", "graph_streams", "eager_streams"]
+
+    def launch_mode(self, precision, mode=None):
+        """How the factorisation in `precision` is launched: "graph" (one HIP graph on the plan's stream, the
+        default), "graph_streams" (one graph of three streams) or "eager_streams" (three streams, one launch at a
+        time). Sets the mode when given; returns the mode in force."""
+        precision = int(Precision(precision))
+        if mode is not None:
+            _check(self._lib.oosg_plan_launch_mode(self.handle, precision, self.LAUNCH_MODES.index(mode)))
+        out = ctypes.c_int32()
+        _check(self._lib.oosg_plan_get_launch_mode(self.handle, precision, ctypes.byref(out)))
+        return self.LAUNCH_MODES[out.value]
+
     def autotune(self, device_matrix, precision=None):
         """Times both tile sizes of every update-matrix launch on `device_matrix` (a matrix of this plan) and keeps
-        the faster per level, then one stream against three for the whole factorisation. Returns (launches that chose
-        128 × 128 tiles, launches measured, streams kept)."""
+        the faster per level, then the whole factorisation in each launch mode, keeping the fastest. Returns
+        (launches that chose 128 × 128 tiles, launches measured, streams kept); launch_mode(precision) gives the mode
+        kept."""
         precision = Precision.FP64 if precision is None else Precision(precision)
         big, tuned, streams = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(self._lib.oosg_plan_autotune(self.handle, device_matrix.handle, int(precision), ctypes.byref(big),

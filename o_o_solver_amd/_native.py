@@ -51,6 +51,8 @@ _SIGNATURES = {
     "oosg_plan_use_graphs": (c_int, [c_void_p, c_int32]),
     "oosg_plan_tile_policy": (c_int, [c_void_p, c_int32]),
     "oosg_plan_streams": (c_int, [c_void_p, c_int32, c_int32]),
+    "oosg_plan_launch_mode": (c_int, [c_void_p, c_int32, c_int32]),
+    "oosg_plan_get_launch_mode": (c_int, [c_void_p, c_int32, c_void_p]),
     "oosg_plan_launch_times": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oosg_plan_autotune": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "oosg_plan_profile": (c_int, [c_void_p, c_int32]),

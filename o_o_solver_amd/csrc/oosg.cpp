@@ -174,7 +174,7 @@ struct Launch
 
 // The factorisation's streams: the critical path (diagonal blocks, panel solves, the next panel's update, assembly
 // and update matrices) on the plan's stream; the rest of each step's trailing update beside it; the whole inverses of
-// a level's wide triangles beside both, overlapping the next levels.
+// a level's wide triangles beside both, each of their steps once the panel step it reads is done.
 enum
 {
     LANE_CRITICAL = 0,
@@ -186,7 +186,7 @@ enum
 {
     EVENT_SOLVED = 0, // a step's panel solve is done: the bulk update may start
     EVENT_RESTED,     // the bulk update of a step is done
-    EVENT_PANELS,     // a level's panels are final: its whole inverses may start
+    EVENT_PANELS,     // a level's panels are final: the last launch of its whole inverses may start
     EVENT_FORK,
     EVENT_JOIN_BULK,
     EVENT_JOIN_INVERSES,
@@ -320,6 +320,9 @@ struct oosg_plan
     // Whether the factorisation uses the bulk and inverse streams beside the critical one (set by the autotuner when
     // it measures faster; never while the profiler times single launches).
     bool concurrent[2] = {false, false}; // per precision
+    // Whether the factorisation is launched one launch at a time even where graphs are on (per precision; set by the
+    // autotuner where three streams launched so measure faster than any captured graph).
+    bool eager[2] = {false, false};
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
@@ -721,6 +724,19 @@ Launch syncLaunch(int lane, int wait, int record)
     return l;
 }
 
+// A launch of one step of a level's whole inverses, on the inverses' stream after the event `wait`.
+Launch inverseLaunch(const oosg_plan::InverseRound& round, int wait)
+{
+    Launch inverse{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
+    inverse.offset = round.offset;
+    inverse.count = round.count;
+    inverse.k0 = round.step;
+    inverse.flops = round.flops;
+    inverse.lane = LANE_INVERSES;
+    inverse.wait = wait;
+    return inverse;
+}
+
 void buildSchedule(oosg_plan& p)
 {
     using oosg::PANEL_BLOCK;
@@ -950,6 +966,16 @@ void buildSchedule(oosg_plan& p)
                 rest.record = EVENT_RESTED;
                 p.schedule.push_back(rest);
             }
+            // Step K = k0 / 64 of the level's whole inverses reads the panels' column block K (rows below block K),
+            // final once this step's solve is done: it runs beside the next steps instead of after the level.
+            const size_t K = static_cast<size_t>(k0 / PANEL_BLOCK);
+            const std::vector<oosg_plan::InverseRound>& rounds = p.inverseRounds[l];
+            if (K + 1 < rounds.size() && rounds[K].count > 0)
+            {
+                if (solve.count == 0)
+                    throw std::logic_error("plan: an inverse step without a panel solve");
+                p.schedule.push_back(inverseLaunch(rounds[K], EVENT_SOLVED));
+            }
         }
         // The level's panels are final once the bulk stream is done with them.
         p.schedule.push_back(syncLaunch(LANE_CRITICAL, EVENT_RESTED, EVENT_PANELS));
@@ -1028,21 +1054,9 @@ void buildSchedule(oosg_plan& p)
             tilesLaunch.big[0] = tilesLaunch.big[1] = tilesLaunch.bigCount >= 2048;
             p.schedule.push_back(tilesLaunch);
         }
-        // The whole inverses of the level's wide triangles (their panels are final), one launch per step.
-        bool firstRound = true;
-        for (const oosg_plan::InverseRound& round : p.inverseRounds[l])
-        {
-            Launch inverse{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
-            inverse.offset = round.offset;
-            inverse.count = round.count;
-            inverse.k0 = round.step;
-            inverse.flops = round.flops;
-            inverse.lane = LANE_INVERSES;
-            if (firstRound)
-                inverse.wait = EVENT_PANELS;
-            firstRound = false;
-            p.schedule.push_back(inverse);
-        }
+        // The last launch of the level's whole inverses (X_IJ = L_II⁻¹·T_IJ), once its panels are final.
+        if (!p.inverseRounds[l].empty() && p.inverseRounds[l].back().count > 0)
+            p.schedule.push_back(inverseLaunch(p.inverseRounds[l].back(), EVENT_PANELS));
     }
     // The other streams join the critical one (each first waits for a record of this sequence, so that both belong to
     // it even when they had nothing to do).
@@ -2564,7 +2578,41 @@ oosg_status oosg_plan_streams(oosg_plan* plan, int32_t precision, int32_t count)
                 fail(OOSG_INVALID_ARGUMENT, "plan: the factorisation runs on 1 or 3 streams");
             setDevice(plan);
             plan->concurrent[precision] = count == LANES;
+            plan->eager[precision] = false;
             plan->factorGraph[precision].clear();
+        });
+}
+
+oosg_status oosg_plan_launch_mode(oosg_plan* plan, int32_t precision, int32_t mode)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            if (precision != OOSG_FP64 && precision != OOSG_FP32)
+                fail(OOSG_INVALID_ARGUMENT, "plan: unknown precision");
+            if (mode < OOSG_LAUNCH_GRAPH || mode > OOSG_LAUNCH_EAGER_STREAMS)
+                fail(OOSG_INVALID_ARGUMENT, "plan: unknown launch mode");
+            setDevice(plan);
+            plan->concurrent[precision] = mode != OOSG_LAUNCH_GRAPH;
+            plan->eager[precision] = mode == OOSG_LAUNCH_EAGER_STREAMS;
+            plan->factorGraph[precision].clear();
+        });
+}
+
+oosg_status oosg_plan_get_launch_mode(const oosg_plan* plan, int32_t precision, int32_t* mode)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr || mode == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            if (precision != OOSG_FP64 && precision != OOSG_FP32)
+                fail(OOSG_INVALID_ARGUMENT, "plan: unknown precision");
+            *mode = !plan->concurrent[precision] ? OOSG_LAUNCH_GRAPH
+                    : plan->eager[precision]     ? OOSG_LAUNCH_EAGER_STREAMS
+                                                 : OOSG_LAUNCH_GRAPH_STREAMS;
         });
 }
 
@@ -2656,15 +2704,16 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
             }
             p.tilePolicy = 2;
             p.tuned[precision] = true;
-            // Then the whole factorisation, on one stream and on three (the best of three runs each, after a warm
-            // one): the streams stay only where they measure faster.
-            const auto wall = [&](bool lanes)
+            // Then the whole factorisation in each launch mode, launched as a factorisation launches it (the best of
+            // three runs each, after a warm one that captures the graph): one graph on the plan's stream, one graph
+            // of three streams, three streams launched one launch at a time (where the runtime does not overlap a
+            // graph's branches as it overlaps the streams). The fastest stays.
+            const auto wall = [&](int32_t mode)
             {
-                p.concurrent[precision] = lanes;
-                float best = INFINITY;
-                for (int run = 0; run < 4; ++run)
+                p.concurrent[precision] = mode != OOSG_LAUNCH_GRAPH;
+                GraphCache trial;
+                const auto body = [&]
                 {
-                    check(hipEventRecord(p.start, p.stream), "hipEventRecord");
                     check(oosg::launchFillInt32(zero.as<int32_t>(), p.S, INT32_MAX, p.stream), "autotune: status");
                     if (precision == OOSG_FP32)
                         runFactor<float>(p, *matrix, panels.as<float>(), linv.as<float>(), diag.as<double>(),
@@ -2672,6 +2721,15 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
                     else
                         runFactor<double>(p, *matrix, panels.as<double>(), linv.as<double>(), diag.as<double>(),
                                           zero.as<int32_t>(), false);
+                };
+                float best = INFINITY;
+                for (int run = 0; run < 4; ++run)
+                {
+                    check(hipEventRecord(p.start, p.stream), "hipEventRecord");
+                    if (p.graphs && mode != OOSG_LAUNCH_EAGER_STREAMS)
+                        trial.run(p.stream, {}, body);
+                    else
+                        body();
                     check(hipEventRecord(p.stop, p.stream), "hipEventRecord");
                     check(hipEventSynchronize(p.stop), "autotune");
                     float ms = 0.0f;
@@ -2682,9 +2740,20 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
                 return best;
             };
             p.profiler.enabled = false;
-            const float serialMs = wall(false), lanesMs = wall(true);
+            int32_t fastest = OOSG_LAUNCH_GRAPH;
+            float fastestMs = wall(OOSG_LAUNCH_GRAPH);
+            for (int32_t mode = OOSG_LAUNCH_GRAPH_STREAMS; mode <= (p.graphs ? OOSG_LAUNCH_EAGER_STREAMS : mode); ++mode)
+            {
+                const float ms = wall(mode);
+                if (ms < fastestMs)
+                {
+                    fastest = mode;
+                    fastestMs = ms;
+                }
+            }
             p.profiler.enabled = profiling;
-            p.concurrent[precision] = lanesMs < serialMs;
+            p.concurrent[precision] = fastest != OOSG_LAUNCH_GRAPH;
+            p.eager[precision] = fastest == OOSG_LAUNCH_EAGER_STREAMS;
             p.factorGraph[0].clear();
             p.factorGraph[1].clear();
             if (big_launches != nullptr)
@@ -2996,7 +3065,7 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
                     runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(),
                                       diag.as<double>(), zero.as<int32_t>(), false);
             };
-            if (plan->graphs && !plan->profiler.enabled)
+            if (plan->graphs && !plan->profiler.enabled && !plan->eager[opts.precision])
                 plan->factorGraph[opts.precision].run(
                     st,
                     {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,

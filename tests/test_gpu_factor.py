@@ -336,6 +336,7 @@ def test_big_tiles_give_the_same_bits_and_the_autotuner_keeps_them(gpu_device):
         print(f"[gpu] {precision.name}: {big} of {tuned} update-matrix launches chose 128 x 128 tiles, {streams} "
               "streams", flush=True)
         assert tuned > 0 and 0 <= big <= tuned and streams in (1, 3)
+        assert (plan.launch_mode(precision) == "graph") == (streams == 1)
         f2 = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
         assert all(np.array_equal(f2.panel(s)[0], panels0[s]) for s in range(sym.supernode_count)), precision
         assert np.array_equal(FactorSolve(f2).solve(b), x0)
@@ -359,14 +360,18 @@ def test_three_streams_give_the_same_bits(gpu_device):
         x = FactorSolve(f).solve(b)
         del f
         plan.streams(precision, 3)
-        for graphs in (False, True, True):
+        assert plan.launch_mode(precision) == "graph_streams"
+        for mode, graphs in (("graph_streams", False), ("graph_streams", True), ("graph_streams", True),
+                             ("eager_streams", True), ("eager_streams", True)):
+            plan.launch_mode(precision, mode)
             plan.use_graphs(graphs)
             g = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
             assert all(np.array_equal(g.panel(s)[0], panels[s]) for s in range(sym.supernode_count)), (precision,
-                                                                                                        graphs)
+                                                                                                        mode, graphs)
             assert np.array_equal(FactorSolve(g).solve(b), x)
             del g
         plan.streams(precision, 1)
+        assert plan.launch_mode(precision) == "graph"
 
 
 def test_profile_counts_every_factor_launch(gpu_device):
