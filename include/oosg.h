@@ -228,7 +228,8 @@ oosg_status oosg_plan_autotune(oosg_plan *plan, const oosg_matrix *matrix, int32
                                int32_t *tuned_launches, int32_t *streams);
 
 /* Diagnostic: the factorisation's launches in order, with their profile kind, level and algorithmic flops, and the
- * device time of each in the last autotuner run (ms; -1 before one). Call with null arrays for the count. */
+ * device time of each in the last autotuner run, one launch at a time on one stream, the update matrices with the tile
+ * size kept (ms; -1 before one). Call with null arrays for the count. */
 oosg_status oosg_plan_launch_times(const oosg_plan *plan, int32_t *count, int32_t *kinds, int32_t *levels, float *ms,
                                    double *flops);
 

@@ -209,8 +209,8 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
 // Assembly of a block of a front: columns [j0, j1), rows [i0, i1), accumulated in LDS in the reference's order (zero,
 // the matrix entries, the children in increasing order), then written to the panel (columns < c, the rows above the
 // diagonal as zeros) or to the update matrix (lower part). The matrix entries of the block's columns are one range
-// of the CSC, spread over the threads; a child's columns go to the waves in turn, four at a time, its rows across
-// the lanes, so that each lane has several independent loads in flight.
+// of the CSC, spread over the threads; a child's part, its columns by consecutive rows, too, so that every thread has
+// several independent loads in flight whatever the block's shape.
 template <typename T>
 __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix a, T* panels, T* updates,
                                                       const AssemblyBlock* blocks, const AssemblyItem* items)
@@ -222,7 +222,6 @@ __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     const int rowsIn = blk.i1 - blk.i0, colsIn = blk.j1 - blk.j0;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     for (int idx = tid; idx < rowsIn * colsIn; idx += nt)
         B[idx] = T(0);
     __syncthreads();
@@ -248,47 +247,30 @@ __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix
         const int rc = p.nrow[it.child];
         const int32_t* rel = p.rel + p.rowStart[it.child];
         const T* u = updates + p.updateOff[it.child];
-        for (int jj0 = it.q0 + 4 * wave; jj0 < it.q1; jj0 += 16)
+        // The child's columns [q0, q1) × rows [ia, ib) (its lower part), spread over all the threads by consecutive
+        // rows; eight entries per thread per pass, every load issued before any is used.
+        const int h = it.ib - it.ia, total = h * (it.q1 - it.q0);
+        constexpr int UNROLL = 8;
+        for (int base = tid; base < total; base += 256 * UNROLL)
         {
-            // Columns jj0 .. jj0 + 3 of the child (those below q1); every load of the four is issued first.
-            int col[4], lo[4];
-            const T* uc[4];
+            int ci[UNROLL], ri[UNROLL];
+            T v[UNROLL];
+            bool in[UNROLL];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int e = 0; e < UNROLL; ++e)
             {
-                const int jj = min(jj0 + t, it.q1 - 1);
-                col[t] = rel[jj];
-                lo[t] = max(jj, it.ia);
-                uc[t] = u + int64_t(jj) * rc;
+                const int idx = base + 256 * e;
+                const int jj = it.q0 + idx / h, i = it.ia + idx % h;
+                in[e] = idx < total && i >= jj;
+                const int jc = min(jj, it.q1 - 1), ic = in[e] ? i : it.ia;
+                ci[e] = rel[jc];
+                ri[e] = rel[ic];
+                v[e] = u[ic + int64_t(jc) * rc];
             }
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-                col[t] = jj0 + t < it.q1 ? pinned(col[t]) - blk.j0 : -1;
-            for (int i0 = 0;; i0 += 64)
-            {
-                bool any = false;
-                T v[4];
-                int row[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                {
-                    const int i = lo[t] + i0 + lane;
-                    const bool in = col[t] >= 0 && i < it.ib;
-                    any |= col[t] >= 0 && lo[t] + i0 < it.ib;
-                    const int ic = in ? i : lo[t];
-                    v[t] = uc[t][ic];
-                    row[t] = rel[ic];
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                {
-                    const int i = lo[t] + i0 + lane;
-                    if (col[t] >= 0 && i < it.ib)
-                        B[(row[t] - blk.i0) + col[t] * rowsIn] += pinned(v[t]);
-                }
-                if (!any)
-                    break;
-            }
+            for (int e = 0; e < UNROLL; ++e)
+                if (in[e])
+                    B[(pinned(ri[e]) - blk.i0) + (pinned(ci[e]) - blk.j0) * rowsIn] += pinned(v[e]);
         }
     }
     __syncthreads();

@@ -2702,6 +2702,11 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
                 l.big[precision] = big[li] < small[li];
                 chosen += l.big[precision] ? 1 : 0;
             }
+            // The launch times as the kept policy runs them (the update matrices with the size chosen).
+            p.launchMs = small;
+            for (size_t li = 0; li < p.schedule.size() && li < big.size(); ++li)
+                if (p.schedule[li].bigCount > 0 && p.schedule[li].big[precision])
+                    p.launchMs[li] = big[li];
             p.tilePolicy = 2;
             p.tuned[precision] = true;
             // Then the whole factorisation in each launch mode, launched as a factorisation launches it (the best of
