@@ -207,8 +207,8 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
 // ---- large fronts --------------------------------------------------------------------------------------------------
 
 // Assembly of a block of a front: columns [j0, j1), rows [i0, i1), accumulated in LDS in the reference's order (zero,
-// the matrix entries, the children in increasing order), then written to the panel (columns < c, the rows above the
-// diagonal as zeros) or to the update matrix (lower part). The matrix entries of the block's columns are one range
+// the matrix entries, the children in increasing order), then its lower part written to the panel (columns < c; the
+// panel's values above the diagonal are never read, nor written) or to the update matrix. The matrix entries of the block's columns are one range
 // of the CSC, spread over the threads; a child's part, its columns by consecutive rows, too, so that every thread has
 // several independent loads in flight whatever the block's shape.
 template <typename T>
@@ -279,9 +279,11 @@ __global__ __launch_bounds__(256) void assembleKernel(DevicePlan p, DeviceMatrix
     T* const update = updates + p.updateOff[s];
     forEach2D(rowsIn, colsIn, [&](int il, int jl) {
         const int i = blk.i0 + il, j = blk.j0 + jl;
+        if (i < j)
+            return;
         if (j < c)
             panel[i + int64_t(j) * n] = B[il + jl * rowsIn];
-        else if (i >= j)
+        else
             update[(i - c) + int64_t(j - c) * r] = B[il + jl * rowsIn];
     });
 }

@@ -828,7 +828,8 @@ void buildSchedule(oosg_plan& p)
             int j0 = 0;
             while (j0 < c)
             {
-                const int i0 = j0 < c ? 0 : j0;
+                // Rows from the block's first column down: the panel's values above the diagonal are never read.
+                const int i0 = j0;
                 const int height = n - i0;
                 if (height <= oosg::ASSEMBLY_BLOCK_VALUES)
                 {
@@ -3178,6 +3179,10 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
                 check(hipMemcpy(values, f->panels.as<double>() + p.panelOff[s], count * sizeof(double),
                                 hipMemcpyDeviceToHost),
                       "panel: copy");
+            // The device never reads or writes the values above the diagonal: they are zeros here.
+            const int64_t n = c + r;
+            for (int64_t j = 1; j < c; ++j)
+                std::fill(values + j * n, values + j * n + j, 0.0);
         });
 }
 
