@@ -218,4 +218,19 @@ hipError_t launchPermuteRows(const double* src, double* dst, const int32_t* perm
 hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const float* af, const float* bf, float* df,
                            hipStream_t stream);
 
+// The pivots' statistics of a positive definite factor (the ratio test of MultifrontalDriver::finish): the pivot of
+// equation j is d_j, or d_j / s_j² with the scales s of an FP32 factor; its ratio is the diagonal a_jj (or the
+// reference diagonal where that is >= 0) over it. Counts by sign; the largest ratio (as an ordered key, NaN ratios
+// left out) and the first equation that has it; the pivots that fail the test (ratio > maxRatio, or negative ratio or
+// pivot), the first of them and whether it fails by a negative value.
+struct PivotStatistics
+{
+    unsigned long long positive, negative, zero, issues;
+    unsigned long long largestKey;
+    int32_t largestAt, firstIssue, firstIssueNegative, pad;
+};
+hipError_t launchPivotStatistics(const double* diag, const double* scale, const double* diagonal,
+                                 const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
+                                 hipStream_t stream);
+
 } // namespace oosg

@@ -167,10 +167,31 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
         const int rc = p.nrow[ch];
         const int32_t* rel = p.rel + p.rowStart[ch];
         const T* u = updates + p.updateOff[ch];
-        forEach2D(rc, rc, [&](int i, int jj) {
-            if (i >= jj)
-                F[rel[i] + rel[jj] * n] += u[i + jj * rc];
-        });
+        // The child's lower part by consecutive rows, eight entries per thread per pass, every load issued before any
+        // is used (a child's entries land on distinct positions).
+        constexpr int UNROLL = 8;
+        const int total = rc * rc;
+        for (int base = tid; base < total; base += 256 * UNROLL)
+        {
+            int ri[UNROLL], ci[UNROLL];
+            T v[UNROLL];
+            bool in[UNROLL];
+#pragma unroll
+            for (int e = 0; e < UNROLL; ++e)
+            {
+                const int idx = base + 256 * e;
+                const int jj = min(idx / rc, rc - 1), i = idx % rc;
+                in[e] = idx < total && i >= jj;
+                const int ic = in[e] ? i : jj;
+                ri[e] = rel[ic];
+                ci[e] = rel[jj];
+                v[e] = u[ic + jj * rc];
+            }
+#pragma unroll
+            for (int e = 0; e < UNROLL; ++e)
+                if (in[e])
+                    F[pinned(ri[e]) + pinned(ci[e]) * n] += pinned(v[e]);
+        }
     }
     __syncthreads();
 
@@ -1974,6 +1995,134 @@ hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficient
 hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream)
 {
     hipLaunchKernelGGL(scaleRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, src, dst, scale, n, k);
+    return hipGetLastError();
+}
+
+// ---- pivot statistics -------------------------------------------------------------------------------------------
+
+// A double as an unsigned key of the same order (negative values below positive ones).
+__device__ inline unsigned long long orderedKey(double x)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    return (b >> 63) ? ~b : b | 0x8000000000000000ull;
+}
+
+struct PivotTerms
+{
+    double pivot, ratio;
+};
+
+__device__ inline PivotTerms pivotTerms(const double* diag, const double* scale, const double* diagonal,
+                                        const double* reference, int64_t j)
+{
+    const double pivot = scale != nullptr ? diag[j] / (scale[j] * scale[j]) : diag[j];
+    const double a = reference != nullptr && reference[j] >= 0.0 ? reference[j] : diagonal[j];
+    return {pivot, a / pivot};
+}
+
+__global__ void pivotStatisticsInitKernel(PivotStatistics* out)
+{
+    if (threadIdx.x == 0)
+        *out = PivotStatistics{0, 0, 0, 0, 0, INT32_MAX, INT32_MAX, 0, 0};
+}
+
+// Counts, the largest key and the first failing equation: per thread, then per workgroup through LDS, then one
+// atomic per workgroup and quantity.
+__global__ __launch_bounds__(256) void pivotStatisticsKernel(const double* diag, const double* scale,
+                                                             const double* diagonal, const double* reference, int64_t n,
+                                                             double maxRatio, PivotStatistics* out)
+{
+    __shared__ unsigned long long sums[4][256];
+    __shared__ unsigned long long keys[256];
+    __shared__ int32_t firsts[256];
+    unsigned long long positive = 0, negative = 0, zero = 0, issues = 0, key = 0;
+    int32_t first = INT32_MAX;
+    for (int64_t j = blockIdx.x * int64_t(256) + threadIdx.x; j < n; j += int64_t(gridDim.x) * 256)
+    {
+        const PivotTerms t = pivotTerms(diag, scale, diagonal, reference, j);
+        positive += t.pivot > 0.0;
+        negative += t.pivot < 0.0;
+        zero += !(t.pivot > 0.0) && !(t.pivot < 0.0);
+        if (!isnan(t.ratio))
+            key = max(key, orderedKey(t.ratio));
+        if (t.ratio > maxRatio || t.ratio < 0.0 || t.pivot < 0.0)
+        {
+            ++issues;
+            first = min(first, static_cast<int32_t>(j));
+        }
+    }
+    const int tid = threadIdx.x;
+    sums[0][tid] = positive;
+    sums[1][tid] = negative;
+    sums[2][tid] = zero;
+    sums[3][tid] = issues;
+    keys[tid] = key;
+    firsts[tid] = first;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1)
+    {
+        if (tid < half)
+        {
+            for (int q = 0; q < 4; ++q)
+                sums[q][tid] += sums[q][tid + half];
+            keys[tid] = max(keys[tid], keys[tid + half]);
+            firsts[tid] = min(firsts[tid], firsts[tid + half]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        atomicAdd(&out->positive, sums[0][0]);
+        atomicAdd(&out->negative, sums[1][0]);
+        atomicAdd(&out->zero, sums[2][0]);
+        atomicAdd(&out->issues, sums[3][0]);
+        atomicMax(&out->largestKey, keys[0]);
+        atomicMin(&out->firstIssue, firsts[0]);
+    }
+}
+
+// The first equation whose ratio has the largest key, and whether the first failing pivot fails by a negative value.
+__global__ __launch_bounds__(256) void pivotLargestKernel(const double* diag, const double* scale,
+                                                          const double* diagonal, const double* reference, int64_t n,
+                                                          PivotStatistics* out)
+{
+    const unsigned long long largest = out->largestKey;
+    const int32_t firstIssue = out->firstIssue;
+    int32_t at = INT32_MAX;
+    for (int64_t j = blockIdx.x * int64_t(256) + threadIdx.x; j < n; j += int64_t(gridDim.x) * 256)
+    {
+        const PivotTerms t = pivotTerms(diag, scale, diagonal, reference, j);
+        if (!isnan(t.ratio) && orderedKey(t.ratio) == largest)
+            at = min(at, static_cast<int32_t>(j));
+        if (j == firstIssue)
+            out->firstIssueNegative = t.ratio < 0.0 || t.pivot < 0.0 ? 1 : 0;
+    }
+    __shared__ int32_t ats[256];
+    ats[threadIdx.x] = at;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1)
+    {
+        if (threadIdx.x < half)
+            ats[threadIdx.x] = min(ats[threadIdx.x], ats[threadIdx.x + half]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMin(&out->largestAt, ats[0]);
+}
+
+hipError_t launchPivotStatistics(const double* diag, const double* scale, const double* diagonal,
+                                 const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
+                                 hipStream_t stream)
+{
+    hipLaunchKernelGGL(pivotStatisticsInitKernel, dim3(1), dim3(64), 0, stream, out);
+    if (n > 0)
+    {
+        const int groups = static_cast<int>(std::min<int64_t>(1024, (n + 255) / 256));
+        hipLaunchKernelGGL(pivotStatisticsKernel, dim3(groups), dim3(256), 0, stream, diag, scale, diagonal,
+                           reference, n, maxRatio, out);
+        hipLaunchKernelGGL(pivotLargestKernel, dim3(groups), dim3(256), 0, stream, diag, scale, diagonal, reference,
+                           n, out);
+    }
     return hipGetLastError();
 }
 

@@ -375,6 +375,7 @@ struct oosg_plan
     DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
     DeviceBuffer diagCache;     // and their pivots
     DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
+    DeviceBuffer pivotStats;    // the pivots' statistics of the last factorisation (oosg::PivotStatistics)
     // The indefinite factorisation's working storage, kept for the next one: the fronts of each level, W (the
     // updated columns of a level's fronts), the candidates' permutations, D and subdiagonals, the front descriptors
     // and states, and the launch lists.
@@ -438,6 +439,7 @@ struct oosg_matrix
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
     DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
+    DeviceBuffer dDiagonal; // a_jj on the device (the ratio test)
     std::vector<double> scale;
 };
 
@@ -465,7 +467,8 @@ struct oosg_factor
     DeviceBuffer linv;  // the inverses of the diagonal blocks of PANEL_BLOCK columns (L_JJ⁻¹)
     DeviceBuffer dDiag; // D as the solves divide by it (FP32: of the scaled matrix)
     DeviceBuffer scale; // FP32: the equations' scales of the factorised matrix
-    std::vector<double> diag;
+    // The pivots on the host (FP32: of the matrix, d_j / s_j²): filled on first use (pivoted factors: at once).
+    mutable std::vector<double> diag;
     oosg_factor_info info{};
     ~oosg_factor()
     {
@@ -1233,8 +1236,37 @@ std::string equationList(const std::vector<int32_t>& equations, size_t total)
 
 // The zero-pivot test, the inertia and the ratio test of MultifrontalDriver::finish
 // (src/linalg/supernodal/MultifrontalDriver.cpp:704-820), on the host from the pivots the kernels wrote.
+// The pivots of a positive definite factor on the host, copied on first use (FP32: d_j / s_j², those of the matrix).
+const std::vector<double>& hostDiagonal(const oosg_factor& f)
+{
+    if (!f.diag.empty() || f.plan->n == 0)
+        return f.diag;
+    const oosg_plan& p = *f.plan;
+    const size_t n = static_cast<size_t>(p.n);
+    std::vector<double> d(n);
+    check(hipMemcpy(d.data(), f.dDiag.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "factor: pivots");
+    if (f.precision == OOSG_FP32)
+    {
+        std::vector<double> scale(n);
+        check(hipMemcpy(scale.data(), f.scale.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "factor: scales");
+        for (size_t j = 0; j < n; ++j)
+            d[j] = d[j] / (scale[j] * scale[j]);
+    }
+    f.diag = std::move(d);
+    return f.diag;
+}
+
+// The inverse of oosg::orderedKey.
+double keyValue(unsigned long long key)
+{
+    const unsigned long long bits = (key >> 63) ? key & ~0x8000000000000000ull : ~key;
+    return __builtin_bit_cast(double, bits);
+}
+
+// Zero pivots, then the ratio test (MultifrontalDriver::finish): the counts by sign, the largest ratio, the pivots
+// that fail, from the statistics the device computed over the pivots.
 void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std::vector<int32_t>& zeroAt,
-            const oosg_factor_options& options)
+            const oosg::PivotStatistics& stats, const oosg_factor_options& options)
 {
     // A supernode whose descendant failed is skipped in the reference: only the failures with no failed descendant
     // are reported (their ancestors' values are meaningless).
@@ -1262,44 +1294,43 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
                                 "with a matrix that is meant to be indefinite, factorise it in indefinite mode");
     oosg_factor_info& info = f.info;
     const double maxRatio = options.max_ratio > 0 ? options.max_ratio : 1.0e7;
-    double largest = 0.0;
-    int32_t largestAt = -1;
-    std::vector<int32_t> issues;
-    info.first_issue_equation = -1;
-    for (int32_t j = 0; j < p.n; ++j)
+    info.positive = static_cast<int64_t>(stats.positive);
+    info.negative = static_cast<int64_t>(stats.negative);
+    info.zero = static_cast<int64_t>(stats.zero);
+    info.issue_count = static_cast<int64_t>(stats.issues);
+    info.first_issue_equation = stats.issues > 0 ? stats.firstIssue : -1;
+    info.first_issue_negative = stats.issues > 0 ? stats.firstIssueNegative : 0;
+    if (stats.largestAt != INT32_MAX)
     {
-        const double pivot = f.diag[j];
-        ++(pivot > 0.0 ? info.positive : pivot < 0.0 ? info.negative : info.zero);
+        info.largest_ratio = keyValue(stats.largestKey);
+        info.largest_ratio_equation = stats.largestAt;
+    }
+    else
+    {
+        // No ratio is a number (or no equation).
+        info.largest_ratio = p.n > 0 ? NAN : 0.0;
+        info.largest_ratio_equation = p.n > 0 ? 0 : -1;
+    }
+    if (info.issue_count > 0 && options.bailout >= 0)
+    {
+        // The failing equations, for the message, from the pivots on the host.
+        const std::vector<double>& d = hostDiagonal(f);
         const double* reference = options.reference_diagonal;
-        const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
-        const double ratio = diagonal / pivot;
-        if (largestAt < 0 || ratio > largest)
+        std::vector<int32_t> issues;
+        for (int32_t j = 0; j < p.n && issues.size() < 1000; ++j)
         {
-            largest = ratio;
-            largestAt = j;
-        }
-        const bool negative = ratio < 0.0 || pivot < 0.0;
-        if (ratio > maxRatio || negative)
-        {
-            if (info.issue_count == 0)
-            {
-                info.first_issue_equation = j;
-                info.first_issue_negative = negative ? 1 : 0;
-            }
-            ++info.issue_count;
-            if (issues.size() < 1000)
+            const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
+            const double ratio = diagonal / d[j];
+            if (ratio > maxRatio || ratio < 0.0 || d[j] < 0.0)
                 issues.push_back(j);
         }
-    }
-    info.largest_ratio = largest;
-    info.largest_ratio_equation = largestAt;
-    if (info.issue_count > 0 && options.bailout >= 0)
         fail(OOSG_SINGULAR, "the stiffness matrix is nearly singular or not positive definite: " +
                                 std::to_string(info.issue_count) +
                                 " pivots fail the factor diagonal ratio test (ratio of the matrix diagonal to the "
                                 "factor diagonal above " +
                                 std::to_string(maxRatio) + ", or negative):\n" +
                                 equationList(issues, static_cast<size_t>(info.issue_count)));
+    }
 }
 
 // ---- solves --------------------------------------------------------------------------------------------------------
@@ -2926,6 +2957,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dPosition = upload(position, st, "matrix: positions");
             m->dColumn = upload(column, st, "matrix: columns");
             m->dScale = upload(m->scale, st, "matrix: scales");
+            m->dDiagonal = upload(m->diagonal, st, "matrix: diagonal");
             m->dCsrOffsets = upload(rowCount, st, "matrix: row offsets");
             m->dCsrCols = upload(csrCols, st, "matrix: row columns");
             m->dCsrValues = upload(csrValues, st, "matrix: row values");
@@ -2977,6 +3009,9 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
             check(hipMemcpyAsync(m->dScale.ptr, m->scale.data(), m->scale.size() * sizeof(double),
                                  hipMemcpyHostToDevice, st),
                   "matrix: scales");
+            check(hipMemcpyAsync(m->dDiagonal.ptr, m->diagonal.data(), m->diagonal.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, st),
+                  "matrix: diagonal");
             check(hipMemcpyAsync(m->dCsrValues.ptr, csrValues.data(), csrValues.size() * sizeof(double),
                                  hipMemcpyHostToDevice, st),
                   "matrix: row values");
@@ -3081,13 +3116,36 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             else
                 body();
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
+            if (opts.precision == OOSG_FP32)
+            {
+                // The equations' scales the pivots are divided by, kept with the factor.
+                f->scale.allocate(static_cast<size_t>(std::max(plan->n, 1)) * sizeof(double), "factor: scales");
+                check(hipMemcpyAsync(f->scale.ptr, matrix->dScale.ptr, static_cast<size_t>(plan->n) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, st),
+                      "factor: scales");
+            }
+            // The ratio test's statistics over the pivots, on the device.
+            std::vector<double> referenceHost;
+            DeviceBuffer reference;
+            if (opts.reference_diagonal != nullptr)
+            {
+                referenceHost.assign(opts.reference_diagonal, opts.reference_diagonal + plan->n);
+                reference = upload(referenceHost, st, "factor: reference diagonal");
+            }
+            ensure(plan->pivotStats, sizeof(oosg::PivotStatistics), "factor: pivot statistics");
+            check(oosg::launchPivotStatistics(diag.as<double>(),
+                                              opts.precision == OOSG_FP32 ? f->scale.as<double>() : nullptr,
+                                              matrix->dDiagonal.as<double>(),
+                                              reference.ptr != nullptr ? reference.as<double>() : nullptr, plan->n,
+                                              opts.max_ratio > 0 ? opts.max_ratio : 1.0e7,
+                                              plan->pivotStats.as<oosg::PivotStatistics>(), st),
+                  "factor: pivot statistics");
             std::vector<int32_t> zeroAt(static_cast<size_t>(plan->S));
-            f->diag.resize(static_cast<size_t>(plan->n));
+            oosg::PivotStatistics stats{};
             check(hipMemcpyAsync(zeroAt.data(), zero.ptr, zeroAt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st),
                   "factor: status");
-            check(hipMemcpyAsync(f->diag.data(), diag.ptr, f->diag.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                 st),
-                  "factor: pivots");
+            check(hipMemcpyAsync(&stats, plan->pivotStats.ptr, sizeof(stats), hipMemcpyDeviceToHost, st),
+                  "factor: pivot statistics");
             check(hipStreamSynchronize(st), "factor");
             plan->profiler.collect();
             float ms = 0.0f;
@@ -3095,15 +3153,8 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             f->info.factor_seconds = ms * 1e-3;
             f->info.flops = plan->flops;
             f->info.precision = opts.precision;
-            if (opts.precision == OOSG_FP32)
-            {
-                // The pivots of S·A·S: d_j / s_j² are those of A (same signs: the inertia is kept).
-                f->scale = upload(matrix->scale, st, "factor: scales");
-                check(hipStreamSynchronize(st), "factor: scales");
-                for (int32_t j = 0; j < plan->n; ++j)
-                    f->diag[j] = f->diag[j] / (matrix->scale[j] * matrix->scale[j]);
-            }
-            finish(*plan, *matrix, *f, zeroAt, opts);
+            // FP32: the pivots of S·A·S divided by s_j² are those of A (same signs: the inertia is kept).
+            finish(*plan, *matrix, *f, zeroAt, stats, opts);
             *out = f.release();
         });
 }
@@ -3231,7 +3282,9 @@ oosg_status oosg_factor_diagonal(const oosg_factor* f, double* d)
         {
             if (f == nullptr || d == nullptr)
                 fail(OOSG_INVALID_ARGUMENT, "diagonal: null argument");
-            std::copy(f->diag.begin(), f->diag.end(), d);
+            setDevice(f->plan);
+            const std::vector<double>& diag = f->pivoted ? f->diag : hostDiagonal(*f);
+            std::copy(diag.begin(), diag.end(), d);
         });
 }
 

@@ -396,3 +396,36 @@ def test_profile_counts_every_factor_launch(gpu_device):
     assert prof["assembly"][1] > 0 and prof["assembly"][2] == 0
     assert all(prof[k][2] > 0 for k in factor_kinds if prof[k][1] > 0 and k != "assembly")
     del f
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32])
+def test_ratio_test_statistics_match_the_pivots(gpu_device, precision):
+    # The device's statistics of the ratio test (MultifrontalDriver::finish) against the same test done here over the
+    # pivots the factor returns: counts by sign, the largest ratio and its first equation, the failing pivots (a
+    # threshold at the median ratio makes about half of them fail) and the first of them.
+    print(f"[gpu] ratio test statistics {precision.name}", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    plain = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=precision))
+    d = plain.diagonal()
+    cols = np.repeat(np.arange(w.n), np.diff(w.matrix.offsets))
+    on = w.matrix.rows == cols
+    a = np.zeros(w.n)
+    a[cols[on]] = w.matrix.values[on]
+    ratio = a / d
+    threshold = float(np.median(ratio))
+    options = FactorOptions(precision=precision, pivot_policy=oos.PivotPolicy(max_ratio=threshold, bailout=-1))
+    f = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=plain.device_matrix)
+    assert np.array_equal(f.diagonal(), d)
+    assert (f.inertia.positive, f.inertia.negative, f.inertia.zero) == (int(np.sum(d > 0)), int(np.sum(d < 0)),
+                                                                         int(np.sum(d == 0)))
+    diagnostics = f.pivot_diagnostics
+    fails = np.flatnonzero((ratio > threshold) | (ratio < 0) | (d < 0))
+    assert diagnostics.issue_count == len(fails) > 0
+    assert diagnostics.first_issue_equation == fails[0] and not diagnostics.first_issue_negative
+    assert diagnostics.largest_ratio == np.max(ratio)
+    assert diagnostics.largest_ratio_equation == int(np.argmax(ratio))
+    with pytest.raises(oos.SingularityError, match=f"equation {fails[0]}"):
+        SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=precision,
+                                                                pivot_policy=oos.PivotPolicy(max_ratio=threshold)),
+                                   device_matrix=plain.device_matrix)
