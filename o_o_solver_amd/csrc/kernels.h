@@ -123,6 +123,15 @@ __host__ __device__ inline bool keepsTriangleInverse(int c)
     return c >= WIDE_COLUMNS;
 }
 
+// Small supernodes (at most SWEEP_INLINE_COLUMNS columns and SWEEP_INLINE_ROWS rows below them) do their L_RF
+// products in the sweeps inside the workgroup of their triangle, with no row tiles or column blocks of their own: the
+// low levels' thousands of small supernodes then take one launch and one workgroup each per sweep.
+constexpr int SWEEP_INLINE_COLUMNS = 64, SWEEP_INLINE_ROWS = 128;
+__host__ __device__ inline bool sweepRowsInline(int c, int r)
+{
+    return c <= SWEEP_INLINE_COLUMNS && r <= SWEEP_INLINE_ROWS;
+}
+
 // Sets the LDS limits of the kernels that need more than the default; once per process and device, before any launch.
 hipError_t configureKernels();
 

@@ -1274,8 +1274,9 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
     __syncthreads();
     if (isWideSupernode(c))
         return; // its triangle is a product with the whole inverse (triangleProductsKernel)
-    // The triangle L_FF only: the rows below (L_RF) are the row tiles of forwardRowsKernel.
-    const int rowsEnd = c;
+    // The triangle L_FF, and the rows below (L_RF) of a small supernode; a larger one's are the row tiles of
+    // forwardRowsKernel.
+    const int rowsEnd = sweepRowsInline(c, r) ? n : c;
     for (int q0 = 0; q0 < c; q0 += SB)
     {
         const int qb = min(SB, c - q0);
@@ -1432,7 +1433,8 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
         // Dot products of the block's columns with the final rows after it: the supernode's own columns after the
         // block, then the rows of L_RF (unless a big supernode, whose column blocks did them).
         double4_t acc = {0, 0, 0, 0};
-        const int rowsEnd = c; // the rows of L_RF are the column blocks of backwardColumnsKernel
+        // A larger supernode's rows of L_RF are the column blocks of backwardColumnsKernel.
+        const int rowsEnd = sweepRowsInline(c, r) ? n : c;
         for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
             const int ib = min(SB, rowsEnd - i0);

@@ -640,6 +640,15 @@ void deriveLevels(oosg_plan& p)
             double gathered = 0;
             for (int32_t q = p.childStart[k]; q < p.childStart[k + 1]; ++q)
                 gathered += 3.0 * p.nrow[p.children[q]];
+            if (oosg::sweepRowsInline(p.ncol[k], p.nrow[k]))
+            {
+                // L_RF in the triangle's workgroup: its values and the update vector's rows (written once more,
+                // after the zeroing) in the forward launch; X_R gathered in the backward one.
+                w.triangleValues += r * c;
+                w.forwardRows += gathered + 2 * c + 2 * r;
+                w.backwardRows += 2 * c + r;
+                continue;
+            }
             w.forwardRows += gathered + 2 * c + r;
             w.backwardRows += 2 * c;
             w.rowValues += r * c;
@@ -660,8 +669,11 @@ void deriveLevels(oosg_plan& p)
             const bool wide = p.ncol[s] >= oosg::WIDE_COLUMNS;
             if (wide)
                 maxBlocks = std::max(maxBlocks, (p.ncol[s] + 63) / 64);
-            // The L_RF products of every supernode with rows below: row tiles (forward), column blocks (backward).
-            // A supernode of a pivoted factor whose columns were all delayed has none (c = 0).
+            // The L_RF products of every supernode with rows below but the small ones (those do them in the
+            // workgroup of their triangle): row tiles (forward), column blocks (backward). A supernode of a pivoted
+            // factor whose columns were all delayed has none (c = 0).
+            if (oosg::sweepRowsInline(p.ncol[s], p.nrow[s]))
+                continue;
             if (p.ncol[s] > 0)
                 for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
                     p.forwardTiles.push_back(make_int2(s, t));
