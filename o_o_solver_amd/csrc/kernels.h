@@ -197,11 +197,13 @@ struct ColumnCoefficients
 // Workgroups of the column dot products (the partial sums array holds DOT_GROUPS × 16 values).
 constexpr int DOT_GROUPS = 1024;
 
-// Row-major n × k vectors (k <= 16): out = b − A·x (b null: A·x) with the CSR copy of A; column dot products (into
+// Row-major n × k vectors (k <= 16): out = b − A·x (b null: A·x) with the CSR copy of A, by its row groups (rows
+// [groups[g], groups[g + 1]), at most 8, with the same columns); column dot products (into
 // out[0, k), through `partial`); column max |v| (bits into out, which the caller zeroes); x += α p, r −= α q;
 // p = z + β p; dst = src·diag(scale) by rows.
-hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
-                              const double* b, double* out, int64_t n, int k, hipStream_t stream);
+hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values,
+                              const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
+                              int k, hipStream_t stream);
 hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, double* partial, double* out,
                             hipStream_t stream);
 hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream);

@@ -1593,23 +1593,72 @@ __global__ void fillDoubleKernel(double* data, int64_t ld, int64_t n, int k, dou
 
 constexpr int VECTOR_ROWS = 16; // rows of a workgroup's pass
 
-// out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC); a row per
-// group of 16 lanes (four rows in flight per wave), lane h its column h.
+// out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC), by row
+// groups: runs of up to ROW_GROUP consecutive rows with the same columns (the degrees of freedom of one node). A
+// group per 16 lanes (four groups in flight per wave), lane h its column h: each x row is read once for the whole
+// group, and each row's products are summed in its columns' order.
+constexpr int ROW_GROUP = 8;
+
 __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets, const int32_t* cols,
-                                                         const double* values, const double* x, const double* b,
-                                                         double* out, int64_t n, int k)
+                                                         const double* values, const int32_t* groups,
+                                                         int64_t groupCount, const double* x, const double* b,
+                                                         double* out, int k)
 {
     const int h = threadIdx.x & 15;
     if (h >= k)
         return;
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    for (int64_t g = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); g < groupCount;
+         g += int64_t(gridDim.x) * VECTOR_ROWS)
     {
-        double sum = 0.0;
-#pragma unroll 4
-        for (int64_t e = offsets[i]; e < offsets[i + 1]; ++e)
-            sum += values[e] * x[int64_t(cols[e]) * k + h];
-        out[i * k + h] = b ? b[i * k + h] - sum : sum;
+        const int32_t r0 = groups[g], rows = groups[g + 1] - r0;
+        const int64_t e0 = offsets[r0], degree = offsets[r0 + 1] - e0;
+        int64_t start[ROW_GROUP];
+        double sum[ROW_GROUP];
+#pragma unroll
+        for (int t = 0; t < ROW_GROUP; ++t)
+        {
+            start[t] = offsets[r0 + min(t, rows - 1)];
+            sum[t] = 0.0;
+        }
+        // Four entries at a time: their columns, x rows and values all loaded before the first product.
+        constexpr int BATCH = 4;
+        int64_t e = 0;
+        for (; e + BATCH <= degree; e += BATCH)
+        {
+            int c[BATCH];
+            double xv[BATCH], v[BATCH][ROW_GROUP];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+                c[u] = cols[e0 + e + u];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+            {
+                xv[u] = x[int64_t(c[u]) * k + h];
+#pragma unroll
+                for (int t = 0; t < ROW_GROUP; ++t)
+                    v[u][t] = t < rows ? values[start[t] + e + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+#pragma unroll
+                for (int t = 0; t < ROW_GROUP; ++t)
+                    sum[t] += v[u][t] * xv[u];
+        }
+        for (; e < degree; ++e)
+        {
+            const double xv = x[int64_t(cols[e0 + e]) * k + h];
+#pragma unroll
+            for (int t = 0; t < ROW_GROUP; ++t)
+                if (t < rows)
+                    sum[t] += values[start[t] + e] * xv;
+        }
+#pragma unroll
+        for (int t = 0; t < ROW_GROUP; ++t)
+            if (t < rows)
+            {
+                const int64_t at = int64_t(r0 + t) * k + h;
+                out[at] = b ? b[at] - sum[t] : sum[t];
+            }
     }
 }
 
@@ -1950,11 +1999,12 @@ static int vectorGroups(int64_t n)
     return int(g < 1 ? 1 : (g > 4096 ? 4096 : g));
 }
 
-hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
-                              const double* b, double* out, int64_t n, int k, hipStream_t stream)
+hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values,
+                              const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
+                              int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, offsets, cols, values, x, b,
-                       out, n, k);
+    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(groupCount)), dim3(256), 0, stream, offsets, cols, values,
+                       groups, groupCount, x, b, out, k);
     return hipGetLastError();
 }
 

@@ -439,6 +439,10 @@ struct oosg_matrix
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
     DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
+    // The CSR's row groups: runs of up to 8 consecutive rows with the same columns (a node's degrees of freedom);
+    // group g is rows [csrGroups[g], csrGroups[g + 1]).
+    std::vector<int32_t> csrGroups;
+    DeviceBuffer dCsrGroups;
     DeviceBuffer dDiagonal; // a_jj on the device (the ratio test)
     std::vector<double> scale;
 };
@@ -1992,22 +1996,34 @@ std::vector<double> columnDots(oosg_plan& p, const double* a, const double* b, i
     return out;
 }
 
-std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
+// Column maxima |v| of row-major n × k vectors into slot `slot` (of 3) of the plan's norms, on the stream.
+void startColumnMax(oosg_plan& p, const double* v, int k, int slot)
 {
-    ensure(p.solveNorms, 16 * sizeof(unsigned long long), "solve: norms");
-    std::vector<unsigned long long> bits(16, 0);
-    check(hipMemcpyAsync(p.solveNorms.ptr, bits.data(), 16 * sizeof(unsigned long long), hipMemcpyHostToDevice,
-                         p.stream),
-          "solve: norms");
-    check(oosg::launchColumnMaxRows(v, p.n, k, p.solveNorms.as<unsigned long long>(), p.stream), "solve: norms");
-    check(hipMemcpyAsync(bits.data(), p.solveNorms.ptr, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                         p.stream),
+    ensure(p.solveNorms, 3 * 16 * sizeof(unsigned long long), "solve: norms");
+    unsigned long long* out = p.solveNorms.as<unsigned long long>() + 16 * slot;
+    check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(out), 32, 0, p.stream), "solve: norms");
+    check(oosg::launchColumnMaxRows(v, p.n, k, out, p.stream), "solve: norms");
+}
+
+// The first `slots` slots of column maxima, read back with one copy and one wait.
+std::vector<std::vector<double>> finishColumnMax(oosg_plan& p, int slots, int k)
+{
+    std::vector<unsigned long long> bits(static_cast<size_t>(16 * slots));
+    check(hipMemcpyAsync(bits.data(), p.solveNorms.ptr, bits.size() * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, p.stream),
           "solve: norms");
     check(hipStreamSynchronize(p.stream), "solve: norms");
-    std::vector<double> out(static_cast<size_t>(k));
-    for (int h = 0; h < k; ++h)
-        std::memcpy(&out[h], &bits[h], sizeof(double));
+    std::vector<std::vector<double>> out(static_cast<size_t>(slots), std::vector<double>(static_cast<size_t>(k)));
+    for (int q = 0; q < slots; ++q)
+        for (int h = 0; h < k; ++h)
+            std::memcpy(&out[q][h], &bits[16 * q + h], sizeof(double));
     return out;
+}
+
+std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
+{
+    startColumnMax(p, v, k, 0);
+    return finishColumnMax(p, 1, k)[0];
 }
 
 // out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides.
@@ -2058,6 +2074,8 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
     const int64_t* offsets = m.dCsrOffsets.as<int64_t>();
     const int32_t* cols = m.dCsrCols.as<int32_t>();
     const double* values = m.dCsrValues.as<double>();
+    const int32_t* groups = m.dCsrGroups.as<int32_t>();
+    const int64_t groupCount = static_cast<int64_t>(m.csrGroups.size()) - 1;
     const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
 
     check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
@@ -2088,7 +2106,8 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
     for (int32_t iteration = 0;; ++iteration)
     {
         const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
-        check(oosg::launchMultiplyRows(offsets, cols, values, P, nullptr, Q, n, k, st), "solve: product");
+        check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, P, nullptr, Q, k, st),
+              "solve: product");
         p.profiler.end(slot, st);
         const std::vector<double> sigma = columnDots(p, P, Q, k);
         oosg::ColumnCoefficients alpha{};
@@ -2104,7 +2123,8 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
         {
             // The true residual, into Q.
             const int slot2 = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
-            check(oosg::launchMultiplyRows(offsets, cols, values, X, B, Q, n, k, st), "solve: residual");
+            check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, X, B, Q, k, st),
+                  "solve: residual");
             p.profiler.end(slot2, st);
             const std::vector<double> trueEta = backwardErrors(Q);
             converged = true;
@@ -2159,25 +2179,34 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
     double* Z = p.cgZ.as<double>();
-    const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
     check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
-    const std::vector<double> bNorm = columnMaxRows(p, B, k);
-    check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
-          "solve: zero");
-    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+    startColumnMax(p, B, k, 0); // read with the first step's norms
+    std::vector<double> bNorm;
     double previous = INFINITY;
     report.converged = 0;
     for (int32_t step = 1; step <= std::max(1, maxSteps); ++step)
     {
-        applyFactorRows(f, R, Z, k);
-        check(oosg::launchAddRows(X, Z, n, k, st), "solve: update");
+        // x_0 = 0, r_0 = b: the first correction is the first solution.
+        if (step == 1)
+            applyFactorRows(f, B, X, k);
+        else
+        {
+            applyFactorRows(f, R, Z, k);
+            check(oosg::launchAddRows(X, Z, n, k, st), "solve: update");
+        }
         const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
         check(oosg::launchMultiplyRows(m.dCsrOffsets.as<int64_t>(), m.dCsrCols.as<int32_t>(), m.dCsrValues.as<double>(),
-                                       X, B, R, n, k, st),
+                                       m.dCsrGroups.as<int32_t>(), static_cast<int64_t>(m.csrGroups.size()) - 1, X,
+                                       B, R, k, st),
               "solve: residual");
         p.profiler.end(slot, st);
-        const std::vector<double> rNorm = columnMaxRows(p, R, k);
-        const std::vector<double> xNorm = columnMaxRows(p, X, k);
+        startColumnMax(p, R, k, 1);
+        startColumnMax(p, X, k, 2);
+        const std::vector<std::vector<double>> norms = finishColumnMax(p, 3, k);
+        if (step == 1)
+            bNorm = norms[0];
+        const std::vector<double>& rNorm = norms[1];
+        const std::vector<double>& xNorm = norms[2];
         double eta = 0.0;
         for (int h = 0; h < k; ++h)
         {
@@ -2973,6 +3002,21 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dCsrOffsets = upload(rowCount, st, "matrix: row offsets");
             m->dCsrCols = upload(csrCols, st, "matrix: row columns");
             m->dCsrValues = upload(csrValues, st, "matrix: row values");
+            // Row groups for the residual products.
+            m->csrGroups.assign(1, 0);
+            for (int32_t i = 1, first = 0; i <= n; ++i)
+            {
+                const bool same = i < n && i - first < 8 &&
+                                  rowCount[i + 1] - rowCount[i] == rowCount[first + 1] - rowCount[first] &&
+                                  std::equal(csrCols.begin() + rowCount[i], csrCols.begin() + rowCount[i + 1],
+                                             csrCols.begin() + rowCount[first]);
+                if (!same)
+                {
+                    m->csrGroups.push_back(i);
+                    first = i;
+                }
+            }
+            m->dCsrGroups = upload(m->csrGroups, st, "matrix: row groups");
             check(hipStreamSynchronize(st), "matrix: upload");
             *out = m.release();
         });
