@@ -308,17 +308,20 @@ def main():
     selector = PrecisionSelector.trained()
     choice = {"probability": None, "precision": "fp32"}
 
+    # A step is one call of the combined factorisation and refined solve (the first solve's forward sweep of the
+    # lower levels runs beside the factorisation of the top ones; the same results as the two calls).
     def fp32_step():
-        f = SupernodalFactor.factorise(sym, workload.matrix, FactorOptions(precision=Precision.FP32),
-                                       device_matrix=dm)
         try:
-            x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=args.max_steps)
+            f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd,
+                                                             FactorOptions(precision=Precision.FP32),
+                                                             device_matrix=dm, target=args.target,
+                                                             max_steps=args.max_steps)
             fallback = False
-        except NotConvergedError:
+        except NotConvergedError as e:
             # The FP32 factor is not accurate enough: the FP64 factor decides.
-            del f
-            f = SupernodalFactor.factorise(sym, workload.matrix, device_matrix=dm)
-            x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=4)
+            e.factor = None
+            f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd, device_matrix=dm,
+                                                             target=args.target, max_steps=4)
             fallback = True
         return f.factor_seconds, rep, fallback
 
@@ -331,8 +334,8 @@ def main():
         return fp32_step() if choice["precision"] == "fp32" else fp64_step()
 
     def fp64_step():
-        f = SupernodalFactor.factorise(sym, workload.matrix, device_matrix=dm)
-        x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=4)
+        f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd, device_matrix=dm,
+                                                         target=args.target, max_steps=4)
         return f.factor_seconds, rep, False
 
     def run(step, label, profile):

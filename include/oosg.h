@@ -170,6 +170,16 @@ oosg_status oosg_solve_device(const oosg_factor *factor, double *b_device, int64
 oosg_status oosg_solve_refined_device(const oosg_factor *factor, const oosg_matrix *matrix, const double *b_device,
                                       double *x_device, int64_t ld, int64_t nrhs, double target, int32_t max_steps,
                                       oosg_solve_report *report);
+/* oosg_factorise followed by oosg_solve_refined_device, as the reference's caller runs them back to back
+ * (FsetSolver.cpp:136 and :171); no counterpart of its own in the reference. Where the factorisation is launched one
+ * launch at a time (OOSG_LAUNCH_EAGER_STREAMS, or graphs off) and the tree ends in levels of one supernode (the top
+ * separators), the first solve's forward sweep of the levels below them runs on a stream of its own beside their
+ * factorisation, whose panel steps leave most of the GPU idle. *factor receives the factor whenever the
+ * factorisation succeeds (also with OOSG_NOT_CONVERGED); the caller destroys it. Same results as the two calls. */
+oosg_status oosg_factorise_solve_device(oosg_plan *plan, const oosg_matrix *matrix, const oosg_factor_options *options,
+                                        const double *b_device, double *x_device, int64_t ld, int64_t nrhs,
+                                        double target, int32_t max_steps, oosg_factor **factor,
+                                        oosg_solve_report *report);
 /* y = A·x for n × nrhs column-major device arrays (SymmetricCsc::multiply, SymmetricCsc.hpp:86-96). */
 oosg_status oosg_matrix_multiply_device(const oosg_matrix *matrix, const double *x_device, double *y_device,
                                         int64_t ld, int64_t nrhs);

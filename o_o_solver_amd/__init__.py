@@ -408,9 +408,9 @@ class SupernodalFactor:
         self._info = info
 
     @classmethod
-    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None):
-        """SupernodalFactor::factorise (SupernodalFactor.hpp:176-183). `matrix` is a SymmetricCsc in the factor's
-        order; `device_matrix` (optional) is its device copy, reused across factorisations."""
+    def _call(cls, symbolic, matrix, options, device, hbm_budget_bytes, device_matrix):
+        """The plan, the device matrix, the C options (with the precision the selector picks, for LEARNED), the
+        reference diagonal they point to, and the selector's probability."""
         options = options or FactorOptions()
         plan = symbolic.plan(device, hbm_budget_bytes)
         if device_matrix is None:
@@ -435,12 +435,52 @@ class SupernodalFactor:
         opts = _native.FactorOptionsC(int(options.pivoting), int(precision),
                                       float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
                                       _ptr(reference))
+        return options, plan, device_matrix, opts, reference, selected
+
+    @classmethod
+    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None):
+        """SupernodalFactor::factorise (SupernodalFactor.hpp:176-183). `matrix` is a SymmetricCsc in the factor's
+        order; `device_matrix` (optional) is its device copy, reused across factorisations."""
+        options, plan, device_matrix, opts, reference, selected = cls._call(symbolic, matrix, options, device,
+                                                                            hbm_budget_bytes, device_matrix)
         handle = ctypes.c_void_p()
         lib = _native.lib()
         _check(lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle)))
         factor = cls(handle, plan, device_matrix, options)
         factor.selected_probability = selected
         return factor
+
+    @classmethod
+    def factorise_and_solve(cls, symbolic, matrix, b, options=None, device=0, hbm_budget_bytes=0,
+                            device_matrix=None, target=1e-10, max_steps=10):
+        """factorise, then FactorSolve.solve_refined of the device tensor b (n × k, k <= 16), as one call
+        (oosg_factorise_solve_device): the first solve's forward sweep of the lower levels runs beside the
+        factorisation of the top levels. Returns (factor, x, SolveReport); NotConvergedError carries the factor as
+        its `factor` attribute."""
+        import torch
+        options, plan, device_matrix, opts, reference, selected = cls._call(symbolic, matrix, options, device,
+                                                                            hbm_budget_bytes, device_matrix)
+        bc = _as_device_columns(b)
+        xc = torch.empty_like(bc)
+        handle = ctypes.c_void_p()
+        report = _native.SolveReportC()
+        lib = _native.lib()
+        status = lib.oosg_factorise_solve_device(plan.handle, device_matrix.handle, ctypes.byref(opts),
+                                                 bc.data_ptr(), xc.data_ptr(), bc.shape[1], bc.shape[0],
+                                                 float(target), int(max_steps), ctypes.byref(handle),
+                                                 ctypes.byref(report))
+        factor = None
+        if handle.value:
+            factor = cls(handle, plan, device_matrix, options)
+            factor.selected_probability = selected
+        rep = SolveReport(int(report.steps), bool(report.converged), float(report.backward_error),
+                          float(report.seconds))
+        if status == 7:
+            error = NotConvergedError(f"{lib.oosg_last_error().decode()} ({rep})")
+            error.factor = factor
+            raise error
+        _check(status)
+        return factor, (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
 
     @property
     def dimension(self):

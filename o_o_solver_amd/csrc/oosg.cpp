@@ -376,6 +376,13 @@ struct oosg_plan
     DeviceBuffer diagCache;     // and their pivots
     DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
     DeviceBuffer pivotStats;    // the pivots' statistics of the last factorisation (oosg::PivotStatistics)
+    // A combined factorisation and solve: from the first launch of level aheadLevel on (-1: off), the forward sweep of
+    // the levels below it (aheadK right-hand sides, in solveX) runs on sweepStream beside the factorisation. The
+    // levels from overlapLevel up hold one supernode each: the latency-bound top of the tree, whose panel steps leave
+    // most of the GPU to the sweep (-1: no such levels).
+    int32_t overlapLevel = -1, aheadLevel = -1, aheadK = 0;
+    hipStream_t sweepStream = nullptr;
+    hipEvent_t aheadEvents[LANES + 1] = {};
     // The indefinite factorisation's working storage, kept for the next one: the fronts of each level, W (the
     // updated columns of a level's fronts), the candidates' permutations, D and subdiagonals, the front descriptors
     // and states, and the launch lists.
@@ -419,6 +426,11 @@ struct oosg_plan
         for (int lane = 1; lane < LANES; ++lane)
             if (lanes[lane] && ownsStream)
                 (void)hipStreamDestroy(lanes[lane]);
+        for (hipEvent_t e : aheadEvents)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (sweepStream)
+            (void)hipStreamDestroy(sweepStream);
         if (stream && ownsStream)
             (void)hipStreamDestroy(stream);
     }
@@ -766,6 +778,14 @@ void buildSchedule(oosg_plan& p)
     p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_SOLVED));
     p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_RESTED));
     p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_PANELS));
+    // The top levels of at most OVERLAP_FRONTS supernodes each (a combined factorisation and solve sweeps the levels
+    // below them beside their factorisation). Measured on the 500 × 500 plate (one combined call, FP64): 41.5 ms with
+    // the levels of one supernode, 42.0 ms with those of up to 4 (the sweep then slows the 4 fronts' update
+    // matrices more than it gains), 42.7 ms without the overlap.
+    constexpr int32_t OVERLAP_FRONTS = 1;
+    p.overlapLevel = -1;
+    for (int32_t l = p.levels - 1; l >= 1 && p.levelStart[l + 1] - p.levelStart[l] <= OVERLAP_FRONTS; --l)
+        p.overlapLevel = l;
     for (int32_t l = 0; l < p.levels; ++l)
     {
         const size_t firstLaunch = p.schedule.size();
@@ -1138,6 +1158,9 @@ void uploadPlan(oosg_plan& p)
 
 // ---- factorisation -------------------------------------------------------------------------------------------------
 
+template <typename TP>
+void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t l0, int32_t l1, hipStream_t st);
+
 template <typename T>
 void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled)
 {
@@ -1153,6 +1176,7 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     const int2* inverseItems = p.dInverseItems.as<int2>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
     const bool timing = p.timeLaunches;
+    bool aheadIssued = false;
     if (timing)
         while (p.launchEvents.size() < 2 * p.schedule.size())
         {
@@ -1163,8 +1187,22 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     for (size_t li = 0; li < p.schedule.size(); ++li)
     {
         const Launch& l = p.schedule[li];
-        const bool lanes = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] && !p.profiler.enabled &&
-                           p.lanes[l.lane] != nullptr;
+        const bool concurrent = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] && !p.profiler.enabled;
+        if (p.aheadLevel > 0 && !aheadIssued && l.level >= p.aheadLevel)
+        {
+            // Everything launched so far is the factorisation of the levels below aheadLevel, on whichever streams
+            // it ran: the sweep stream waits for all of it, then sweeps those levels forward.
+            aheadIssued = true;
+            const int streams = concurrent ? LANES : 1;
+            for (int q = 0; q < streams; ++q)
+            {
+                check(hipEventRecord(p.aheadEvents[q], q == 0 ? st : p.lanes[q]), "factor: sweep ahead");
+                check(hipStreamWaitEvent(p.sweepStream, p.aheadEvents[q], 0), "factor: sweep ahead");
+            }
+            forwardSweep<T>(p, panels, linv, p.aheadK, 0, p.aheadLevel, p.sweepStream);
+            check(hipEventRecord(p.aheadEvents[LANES], p.sweepStream), "factor: sweep ahead");
+        }
+        const bool lanes = concurrent && p.lanes[l.lane] != nullptr;
         const hipStream_t ls = lanes ? p.lanes[l.lane] : st;
         if (lanes && l.wait >= 0)
             check(hipStreamWaitEvent(ls, p.laneEvents[l.wait], 0), "factor: stream order");
@@ -1252,6 +1290,8 @@ std::string equationList(const std::vector<int32_t>& equations, size_t total)
 
 // The zero-pivot test, the inertia and the ratio test of MultifrontalDriver::finish
 // (src/linalg/supernodal/MultifrontalDriver.cpp:704-820), on the host from the pivots the kernels wrote.
+void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options, oosg_factor** out);
+
 // The pivots of a positive definite factor on the host, copied on first use (FP32: d_j / s_j², those of the matrix).
 const std::vector<double>& hostDiagonal(const oosg_factor& f)
 {
@@ -1799,15 +1839,14 @@ void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
 
 // X (row-major n × k, k <= 16, in p.solveX) ← factor⁻¹·X. sub: the subdiagonal of a block diagonal D (null: D is
 // diagonal).
+// The forward sweep of the levels [l0, l1) on p.solveX, on `st`.
 template <typename TP>
-void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k, const double* sub = nullptr)
+void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t l0, int32_t l1, hipStream_t st)
 {
-    const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
     double* U = p.solveU.as<double>();
     const int32_t* levels = p.dLevelList.as<int32_t>();
     const int2* forwardTiles = p.dForwardTiles.as<int2>();
-    const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
     const int2* triangle = p.dTriangleItems.as<int2>();
     double* W = p.solveW.as<double>();
     // The profile counts bytes as the work of the sweeps (they are bound by HBM): the values of L (or of an inverse)
@@ -1819,7 +1858,7 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
         launch();
         p.profiler.end(slot, st);
     };
-    for (int32_t l = 0; l < p.levels; ++l)
+    for (int32_t l = l0; l < l1; ++l)
     {
         const oosg_plan::SweepWork& w = p.sweepWork[l];
         // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses, into W, then
@@ -1848,6 +1887,28 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
                       "solve: forward sweep (rows)");
             });
     }
+}
+
+// p.solveX ← factor⁻¹·p.solveX: the forward sweep from level `fromLevel` (the levels below it already swept), D⁻¹,
+// the backward sweep.
+template <typename TP>
+void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k, const double* sub = nullptr,
+               int32_t fromLevel = 0)
+{
+    const hipStream_t st = p.stream;
+    double* X = p.solveX.as<double>();
+    const int32_t* levels = p.dLevelList.as<int32_t>();
+    const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
+    const int2* triangle = p.dTriangleItems.as<int2>();
+    double* W = p.solveW.as<double>();
+    const double T = sizeof(TP), row = 8.0 * k;
+    const auto timed = [&](int kind, double work, auto&& launch)
+    {
+        const int slot = p.profiler.begin(kind, work, st);
+        launch();
+        p.profiler.end(slot, st);
+    };
+    forwardSweep<TP>(p, panels, linv, k, fromLevel, p.levels, st);
     if (sub != nullptr)
         check(oosg::launchBlockDiagonalSolve(X, diag, sub, p.n, k, st), "solve: block diagonal");
     else
@@ -1883,10 +1944,21 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     }
 }
 
-// p.solveX ← factor⁻¹·p.solveX (row-major n × k), as a graph replay unless the profiler is on.
-void sweeps(const oosg_factor& f, int k)
+// p.solveX ← factor⁻¹·p.solveX (row-major n × k), as a graph replay unless the profiler is on; with fromLevel > 0
+// the forward sweep of the levels below it is done already (a combined factorisation and solve), and the rest runs
+// launch by launch.
+void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
 {
     oosg_plan& p = *f.plan;
+    if (fromLevel > 0)
+    {
+        if (f.precision == OOSG_FP32)
+            runSweeps<float>(p, f.panels.as<float>(), f.linv.as<float>(), f.dDiag.as<double>(), k, nullptr, fromLevel);
+        else
+            runSweeps<double>(p, f.panels.as<double>(), f.linv.as<double>(), f.dDiag.as<double>(), k, nullptr,
+                              fromLevel);
+        return;
+    }
     if (f.pivoted)
     {
         // In pivot order: X permuted into the structure's own X, swept, and permuted back.
@@ -2026,18 +2098,27 @@ std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
     return finishColumnMax(p, 1, k)[0];
 }
 
-// out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides.
-void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k)
+// The solve buffers of a plan, sized for 16 right-hand sides.
+void ensureSolveBuffers(oosg_plan& p)
 {
-    oosg_plan& p = *f.plan;
     const int64_t n = p.n;
     ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
     ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
     ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
            "solve: update vectors");
+}
+
+// out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides. With
+// fromLevel > 0, p.solveX holds S·in already, swept forward through the levels below fromLevel.
+void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k, int32_t fromLevel = 0)
+{
+    oosg_plan& p = *f.plan;
+    const int64_t n = p.n;
+    ensureSolveBuffers(p);
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
-    check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
-    sweeps(f, k);
+    if (fromLevel == 0)
+        check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
+    sweeps(f, k, fromLevel);
     check(oosg::launchScaleRows(p.solveX.as<double>(), out, scale, n, k, p.stream), "solve: scaling");
 }
 
@@ -2053,7 +2134,7 @@ double residualBytes(const oosg_matrix& m, int64_t n, int k)
 // p = z' + βp. When the recurrence's residual meets the target, the true residual b − A·x decides (and replaces the
 // recurrence's when it does not). A converged column stops moving (α = β = 0). Returns the solves with the factor.
 void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
-              double target, int32_t maxSteps, oosg_solve_report& report)
+              double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
 {
     oosg_plan& p = *f.plan;
     const hipStream_t st = p.stream;
@@ -2078,12 +2159,14 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
     const int64_t groupCount = static_cast<int64_t>(m.csrGroups.size()) - 1;
     const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
 
-    check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+    // With aheadLevel > 0, B holds b already and the first solve's forward sweep is done below that level.
+    if (aheadLevel == 0)
+        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
     const std::vector<double> bNorm = columnMaxRows(p, B, k);
     check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
           "solve: zero");
     check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
-    applyFactorRows(f, R, Z, k);
+    applyFactorRows(f, R, Z, k, aheadLevel);
     int32_t solves = 1;
     check(hipMemcpyAsync(P, Z, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: direction");
     std::vector<double> rho = columnDots(p, R, Z, k);
@@ -2165,7 +2248,7 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
 // r = b − A·x; stops at the backward-error target, after max_steps solves, or when a step does not halve the
 // backward error (the factor is then not accurate enough for the matrix).
 void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
-                     double target, int32_t maxSteps, oosg_solve_report& report)
+                     double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
 {
     oosg_plan& p = *f.plan;
     const hipStream_t st = p.stream;
@@ -2179,7 +2262,9 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
     double* Z = p.cgZ.as<double>();
-    check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+    // With aheadLevel > 0, B holds b already and the first solve's forward sweep is done below that level.
+    if (aheadLevel == 0)
+        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
     startColumnMax(p, B, k, 0); // read with the first step's norms
     std::vector<double> bNorm;
     double previous = INFINITY;
@@ -2188,7 +2273,7 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     {
         // x_0 = 0, r_0 = b: the first correction is the first solution.
         if (step == 1)
-            applyFactorRows(f, B, X, k);
+            applyFactorRows(f, B, X, k, aheadLevel);
         else
         {
             applyFactorRows(f, R, Z, k);
@@ -2282,6 +2367,9 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
             for (int lane = 1; lane < LANES; ++lane)
                 check(hipStreamCreateWithFlags(&p->lanes[lane], hipStreamNonBlocking), "hipStreamCreate");
             for (hipEvent_t& e : p->laneEvents)
+                check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            check(hipStreamCreateWithFlags(&p->sweepStream, hipStreamNonBlocking), "hipStreamCreate");
+            for (hipEvent_t& e : p->aheadEvents)
                 check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             check(oosg::configureKernels(), "plan: kernel limits");
             check(hipEventCreate(&p->start), "hipEventCreate");
@@ -3088,8 +3176,18 @@ void oosg_matrix_destroy(oosg_matrix* m)
 oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options,
                            oosg_factor** out)
 {
-    return guarded(
-        [&]
+    return guarded([&] { factoriseInto(plan, matrix, options, out); });
+}
+
+} // extern "C"
+
+namespace
+{
+
+// The numeric factorisation behind oosg_factorise (and the combined factorisation and solve).
+void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options, oosg_factor** out)
+{
+    {
         {
             if (plan == nullptr || matrix == nullptr || out == nullptr)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: null argument");
@@ -3172,6 +3270,8 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             else
                 body();
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
+            if (plan->aheadLevel > 0)
+                check(hipStreamWaitEvent(st, plan->aheadEvents[LANES], 0), "factor: sweep ahead"); // its join
             if (opts.precision == OOSG_FP32)
             {
                 // The equations' scales the pivots are divided by, kept with the factor.
@@ -3212,8 +3312,13 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
             // FP32: the pivots of S·A·S divided by s_j² are those of A (same signs: the inertia is kept).
             finish(*plan, *matrix, *f, zeroAt, stats, opts);
             *out = f.release();
-        });
+        }
+    }
 }
+
+} // namespace
+
+extern "C" {
 
 void oosg_factor_destroy(oosg_factor* f)
 {
@@ -3432,6 +3537,74 @@ oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m
                 solveRefinement(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
             check(hipStreamSynchronize(st), "refined solve");
             p.profiler.collect();
+            local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (!local.converged)
+                fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +
+                                             " did not reach " + std::to_string(target) + " in " +
+                                             std::to_string(local.steps) + " solves");
+        });
+    if (report != nullptr)
+        *report = local;
+    return status;
+}
+
+oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options,
+                                        const double* b, double* x, int64_t ld, int64_t nrhs, double target,
+                                        int32_t max_steps, oosg_factor** factor, oosg_solve_report* report)
+{
+    oosg_solve_report local{};
+    const oosg_status status = guarded(
+        [&]
+        {
+            if (plan == nullptr || matrix == nullptr || b == nullptr || x == nullptr || factor == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "factorise and solve: null argument");
+            *factor = nullptr;
+            if (matrix->plan != plan)
+                fail(OOSG_INVALID_ARGUMENT, "factorise and solve: the matrix was made for another plan");
+            if (ld < std::max<int64_t>(plan->n, 1) || nrhs < 1 || nrhs > 16)
+                fail(OOSG_INVALID_ARGUMENT,
+                     "factorise and solve: 1 to 16 right-hand sides, leading dimension >= n");
+            oosg_factor_options opts{};
+            opts.max_ratio = 1.0e7;
+            if (options != nullptr)
+                opts = *options;
+            const bool fp32 = opts.precision == OOSG_FP32;
+            // The forward sweep goes ahead where the factorisation is launched one launch at a time on this call.
+            const bool ahead = opts.pivoting == OOSG_POSITIVE_DEFINITE &&
+                               (opts.precision == OOSG_FP64 || fp32) && plan->overlapLevel > 0 &&
+                               !plan->profiler.enabled && (!plan->graphs || plan->eager[opts.precision]);
+            setDevice(plan);
+            const hipStream_t st = plan->stream;
+            const int k = static_cast<int>(nrhs);
+            const int64_t n = plan->n;
+            if (ahead)
+            {
+                // b into the refinement's row-major copy, and scaled (FP32) into the sweeps' vectors.
+                ensureSolveBuffers(*plan);
+                ensure(plan->cgB, static_cast<size_t>(n) * 16 * sizeof(double), "solve: refinement");
+                check(oosg::launchToRowMajor(b, ld, plan->cgB.as<double>(), n, k, nullptr, st), "solve: layout");
+                check(oosg::launchScaleRows(plan->cgB.as<double>(), plan->solveX.as<double>(),
+                                            fp32 ? matrix->dScale.as<double>() : nullptr, n, k, st),
+                      "solve: scaling");
+                plan->aheadLevel = plan->overlapLevel;
+                plan->aheadK = k;
+            }
+            struct AheadOff
+            {
+                oosg_plan* p;
+                ~AheadOff() { p->aheadLevel = -1; }
+            } off{plan};
+            factoriseInto(plan, matrix, &opts, factor);
+            plan->aheadLevel = -1;
+            const oosg_factor& f = **factor;
+            const int32_t from = ahead ? plan->overlapLevel : 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            if (fp32)
+                solvePcg(f, *matrix, b, x, ld, k, target, std::max(1, max_steps), local, from);
+            else
+                solveRefinement(f, *matrix, b, x, ld, k, target, std::max(1, max_steps), local, from);
+            check(hipStreamSynchronize(st), "factorise and solve");
+            plan->profiler.collect();
             local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (!local.converged)
                 fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +

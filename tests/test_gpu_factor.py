@@ -429,3 +429,31 @@ def test_ratio_test_statistics_match_the_pivots(gpu_device, precision):
         SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=precision,
                                                                 pivot_policy=oos.PivotPolicy(max_ratio=threshold)),
                                    device_matrix=plain.device_matrix)
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32])
+def test_factorise_and_solve_equals_the_two_calls(gpu_device, precision):
+    # The combined call sweeps the lower levels forward beside the factorisation of the top ones (in the eager
+    # three-stream mode); in a graph mode it runs the two calls. Every way gives the bits of factorise +
+    # solve_refined.
+    import torch
+    print(f"[gpu] factorise and solve {precision.name}", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    plan = sym.plan()
+    dm = plan.matrix(w.matrix)
+    b = torch.tensor(w.loads("random", nrhs=3, seed=5), device="cuda:0")
+    options = FactorOptions(precision=precision)
+    f = SupernodalFactor.factorise(sym, w.matrix, options, device_matrix=dm)
+    x0, rep0 = FactorSolve(f).solve_refined(b, target=1e-10, max_steps=20)
+    panels = [f.panel(s)[0] for s in range(sym.supernode_count)]
+    del f
+    for mode in ("eager_streams", "graph", "eager_streams"):
+        plan.launch_mode(precision, mode)
+        g, x, rep = SupernodalFactor.factorise_and_solve(sym, w.matrix, b, options, device_matrix=dm, target=1e-10,
+                                                         max_steps=20)
+        assert all(np.array_equal(g.panel(s)[0], panels[s]) for s in range(sym.supernode_count)), mode
+        assert torch.equal(x, x0), mode
+        assert rep.steps == rep0.steps and rep.backward_error == rep0.backward_error <= 1e-10, mode
+        del g
+    plan.launch_mode(precision, "graph")
