@@ -85,6 +85,19 @@ template <typename T> __device__ inline void rankOneStep(T* F, int ld, int k, in
     }
 }
 
+// The same restricted to the panel of a small front: columns (k, cols), rows [j, rows).
+template <typename T>
+__device__ inline void panelRankOneStep(T* F, int ld, int k, int cols, int rows, const T* lk, T pivot)
+{
+    const int lane = threadIdx.x & 63, groups = blockDim.x >> 6;
+    for (int j = k + 1 + (threadIdx.x >> 6); j < cols; j += groups)
+    {
+        const T w = lk[j] * pivot;
+        for (int i = j + lane; i < rows; i += 64)
+            F[i + j * ld] -= lk[i] * w;
+    }
+}
+
 // The value as an opaque register: a load whose result feeds a select stays unconditional (the compiler would
 // otherwise move it into a branch and wait for it there), so the loads of a loop issue back to back.
 template <typename T> __device__ inline T pinned(T v)
@@ -128,11 +141,39 @@ __device__ inline void writeInverse(const T* F, int ld, int q0, int kb, T* out)
     forEach2D(kb, kb, [&](int i, int j) { out[i + j * kb] = i == j ? T(1) : (i > j ? F[(q0 + j) + (q0 + i) * ld] : T(0)); });
 }
 
+// ---- MFMA blocks ---------------------------------------------------------------------------------------------
+
+template <typename T> struct Mfma;
+
+// Operands of a 16 × 16 × 4 block: lane l holds A(l mod 16, l / 16) and B(l / 16, l mod 16); its four results are
+// D(row(l, v), l mod 16), v = 0..3, with the row mapping of each instruction (checked on the device by
+// oosg_debug_mfma_probe, which writes through these same functions).
+template <> struct Mfma<double>
+{
+    typedef double4_t Acc;
+    __device__ static inline Acc step(double a, double b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <> struct Mfma<float>
+{
+    typedef float4_t Acc;
+    __device__ static inline Acc step(float a, float b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 // ---- small fronts: assembly and elimination fused, the whole front in LDS ------------------------------------------
 
 // One workgroup per front of at most SMALL_FRONT_MAX rows: the front is assembled in LDS (zero, matrix entries,
-// children in order), its c columns are eliminated right-looking (two barriers per column), and the panel, the update
-// matrix and the inverses of the diagonal blocks are written once.
+// children in order), its c columns are eliminated right-looking within the panel (two barriers per column), the
+// update matrix is one MFMA product from the panel, and the panel, the update matrix and the inverses of the diagonal
+// blocks are written once.
 template <typename T>
 __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatrix a, T* panels, T* updates, T* linv,
                                                         const int32_t* list, double* diagOut, int32_t* zeroAt)
@@ -195,7 +236,7 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
     }
     __syncthreads();
 
-    // Right-looking elimination of the c columns.
+    // Right-looking elimination of the c columns within the panel (the reference's chain, column by column).
     for (int k = 0; k < c; ++k)
     {
         const T pivot = F[k + k * n];
@@ -204,7 +245,7 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
         for (int i = k + 1 + tid; i < n; i += nt)
             lcol[i] = F[i + k * n] / pivot;
         __syncthreads();
-        rankOneStep(F, n, k, n, lcol, pivot);
+        panelRankOneStep(F, n, k, c, n, lcol, pivot);
         for (int i = k + 1 + tid; i < n; i += nt)
             F[i + k * n] = lcol[i];
         const int q0 = (k / PANEL_BLOCK) * PANEL_BLOCK;
@@ -214,11 +255,46 @@ __global__ __launch_bounds__(256) void smallFrontKernel(DevicePlan p, DeviceMatr
 
     T* panel = panels + p.panelOff[s];
     forEach2D(n, c, [&](int i, int j) { panel[i + j * n] = i < j ? T(0) : F[i + j * n]; });
+    // The update matrix U = F_RR − L_R·D·L_Rᵀ as one product on 16 × 16 MFMA tiles (its lower tiles over the four
+    // waves), K = c in steps of four, from the front in LDS.
     T* update = updates + p.updateOff[s];
-    forEach2D(r, r, [&](int i, int j) {
-        if (i >= j)
-            update[i + j * r] = F[(c + i) + (c + j) * n];
-    });
+    {
+        typedef typename Mfma<T>::Acc Acc;
+        const int wave = tid >> 6, lane = tid & 63;
+        const int tiles = (r + 15) / 16;
+        for (int q = wave; q < tiles * (tiles + 1) / 2; q += 4)
+        {
+            int J = 0, I = q;
+            while (I >= tiles - J)
+            {
+                I -= tiles - J;
+                ++J;
+            }
+            I += J;
+            const int col = 16 * J + (lane & 15), colIn = min(col, r - 1);
+            Acc acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int row = min(16 * I + Mfma<T>::row(lane, v), r - 1);
+                acc[v] = F[(c + row) + (c + colIn) * n];
+            }
+            const int rowA = min(16 * I + (lane & 15), r - 1);
+            for (int k0 = 0; k0 < c; k0 += 4)
+            {
+                const int k = k0 + (lane >> 4), kIn = min(k, c - 1);
+                const T a = F[(c + rowA) + kIn * n], b = F[(c + colIn) + kIn * n] * F[kIn + kIn * n];
+                acc = Mfma<T>::step(k < c ? -a : T(0), k < c ? b : T(0), acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int row = 16 * I + Mfma<T>::row(lane, v);
+                if (row < r && col < r && row >= col)
+                    update[row + col * r] = acc[v];
+            }
+        }
+    }
     for (int k = tid; k < c; k += nt)
         diagOut[first + k] = double(F[k + k * n]);
     for (int q0 = 0; q0 < c; q0 += PANEL_BLOCK)
@@ -329,30 +405,6 @@ __device__ inline double broadcastLane(double v, int lane)
 //           diagonal block, L(i, blk) = F(i, blk)·L11⁻ᵀ·D⁻¹), in place (C = A: the tile reads its rows before writing).
 // A(i, p) = panels[aOff + i + p·ld]; mode 0: B(j, p) = panels[bOff + j + p·ld]; mode 1: B(j, p) = linv[bOff + j + p·ldb];
 // d_p = panels[dOff + p·(ld + 1)].
-template <typename T> struct Mfma;
-
-// Operands of a 16 × 16 × 4 block: lane l holds A(l mod 16, l / 16) and B(l / 16, l mod 16); its four results are
-// D(row(l, v), l mod 16), v = 0..3, with the row mapping of each instruction (checked on the device by
-// oosg_debug_mfma_probe, which writes through these same functions).
-template <> struct Mfma<double>
-{
-    typedef double4_t Acc;
-    __device__ static inline Acc step(double a, double b, Acc c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-
-template <> struct Mfma<float>
-{
-    typedef float4_t Acc;
-    __device__ static inline Acc step(float a, float b, Acc c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
 
 // The diagonal block [k0, k0 + kb) of a front's panel factorised in place (kb <= PANEL_BLOCK), with the inverse of
 // its unit lower factor; one workgroup of four waves per front, the block in LDS (rows and columns from kb on hold
