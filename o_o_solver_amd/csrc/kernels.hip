@@ -669,6 +669,14 @@ __global__ __launch_bounds__(256, FUSE ? 2 : 1) void updateTilesKernel(DevicePla
     const int row = tid & (TILE - 1), kq = tid >> 6;
     const T* Ar = A + min(row, t.m - 1);
     const T* Br = Bm + min(row, t.n - 1);
+    // A solve divides its columns by their pivots: loaded now, used after the products.
+    T pivots[2] = {T(1), T(1)};
+    if (solve)
+    {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            pivots[b] = dv[min(wn + b * 16 + (lane & 15), t.n - 1) * (ld + 1)];
+    }
     T ra[PER], rb[PER], rd[PER];
     const auto load = [&](int p0) {
 #pragma unroll
@@ -752,7 +760,7 @@ __global__ __launch_bounds__(256, FUSE ? 2 : 1) void updateTilesKernel(DevicePla
                 const int j = wn + b * 16 + (lane & 15);
                 if (i >= t.m || j >= t.n || (t.lower && i < j))
                     continue;
-                C[i + int64_t(j) * t.ldc] = solve ? acc[a][b][v] / dv[j * (ld + 1)] : acc[a][b][v];
+                C[i + int64_t(j) * t.ldc] = solve ? acc[a][b][v] / pivots[b] : acc[a][b][v];
             }
 }
 
