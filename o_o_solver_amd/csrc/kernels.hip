@@ -1340,14 +1340,34 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
     for (int q0 = 0; q0 < c; q0 += SB)
     {
         const int qb = min(SB, c - q0);
-        loadBlockRows(ys, X, first + q0, qb, k);
+        // The block's rows of X, its inverse and the first rows below it, loaded together.
+        StagedRows own;
+        own.load(X, first + q0, qb, k);
+        StagedBlock<TP> inverse, below;
+        inverse.load(linv + p.linvOff[s] + int64_t(q0) * SB, qb, 0, qb, 0, qb);
+        if (q0 + qb < rowsEnd)
+            below.load(L, n, q0 + qb, min(SB, rowsEnd - q0 - qb), q0, qb);
+        own.store(ys);
+        inverse.store(blk);
+        __syncthreads();
         // y_J ← L_JJ⁻¹·y_J.
-        applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, false);
+        {
+            const double4_t out = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
+            __syncthreads();
+            const int h = accColumn();
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                ys[accRow(v) * KMAX + h] = out[v];
+            __syncthreads();
+        }
         storeBlockRows(ys, X, first + q0, qb, k);
         // The rows below the block, 64 at a time.
         for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
-            stageBlock(blk, L, n, i0, min(SB, rowsEnd - i0), q0, qb);
+            if (i0 == q0 + qb)
+                below.store(blk);
+            else
+                stageBlock(blk, L, n, i0, min(SB, rowsEnd - i0), q0, qb);
             __syncthreads();
             const double4_t acc = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
             const int h = accColumn();
@@ -1490,6 +1510,11 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
     for (int q0 = lastBlock; q0 >= 0; q0 -= SB)
     {
         const int qb = min(SB, c - q0);
+        // The block's inverse and its own rows of X, loaded with the first products' operands (used after them).
+        StagedBlock<TP> inverse;
+        inverse.load(linv + p.linvOff[s] + int64_t(q0) * SB, qb, 0, qb, 0, qb);
+        StagedRows own;
+        own.load(X, first + q0, qb, k);
         // Dot products of the block's columns with the final rows after it: the supernode's own columns after the
         // block, then the rows of L_RF (unless a big supernode, whose column blocks did them).
         double4_t acc = {0, 0, 0, 0};
@@ -1504,19 +1529,26 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
             acc = columnsTimesVectors(blk, xs, acc);
             __syncthreads();
         }
-        for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
-            ys[idx] = 0.0;
+        own.store(ys); // zero beyond the block's rows and the k columns
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 4; ++v)
         {
             const int q = accRow(v);
             if (q < qb && h < k)
-                ys[q * KMAX + h] = X[int64_t(first + q0 + q) * k + h] - acc[v];
+                ys[q * KMAX + h] -= acc[v];
         }
+        inverse.store(blk);
         __syncthreads();
         // X_J ← L_JJ⁻ᵀ·y_J.
-        applyBlockInverse(linv + p.linvOff[s] + int64_t(q0) * SB, qb, blk, ys, true);
+        {
+            const double4_t out = columnsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                ys[accRow(v) * KMAX + h] = out[v];
+            __syncthreads();
+        }
         storeBlockRows(ys, X, first + q0, qb, k);
         __syncthreads();
     }
