@@ -490,8 +490,10 @@ __device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int k
             for (int k = 0; k < SP; ++k)
             {
                 const int K = c0 + k;
+                if (K >= kb)
+                    break; // the identity beyond the block: nothing to eliminate
                 const T pivot = broadcastLane(a[k], K);
-                if (lane == 0 && K < kb && isBadPivot(pivot))
+                if (lane == 0 && isBadPivot(pivot))
                     atomicMin(&zeroAt[s], k0 + K);
                 const T l = a[k] / pivot; // L(lane, K) on the lanes below K
                 const T w = l * pivot;
@@ -507,7 +509,8 @@ __device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int k
                     S[(c0 + t) * LD + lane] = a[t];
         }
         __syncthreads();
-        // Trailing tiles (I, J), c0/16 < J <= I < 4, in the order J, then I.
+        // Trailing tiles (I, J), c0/16 < J <= I < 4, in the order J, then I; those whose rows lie beyond the block
+        // hold the identity, which the update (with L zero there) leaves as it is.
         const int P = c0 / SP;
         const int tiles = (3 - P) * (4 - P) / 2;
         for (int q = wave; q < tiles; q += 4)
@@ -519,6 +522,8 @@ __device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int k
                 ++J;
             }
             I += J;
+            if (SP * I >= kb)
+                continue;
             const int col = SP * J + (lane & 15);
             Acc acc;
 #pragma unroll
@@ -539,7 +544,9 @@ __device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int k
         __syncthreads();
     }
 
-    // Diagonal tiles of the inverse: lane j (lanes from 16 on repeat lane j mod 16 and store nothing).
+    // Diagonal tiles of the inverse: lane j (lanes from 16 on repeat lane j mod 16 and store nothing); a tile beyond
+    // the block is the identity's, whose inverse's zeros the upper triangle already holds.
+    if (SP * wave < kb)
     {
         const int J0 = SP * wave, j = lane & 15;
         T x[SP];
@@ -565,7 +572,8 @@ __device__ void factorBlockInLds(T* S, const DevicePlan& p, int s, int k0, int k
     __syncthreads();
     for (int d = 1; d < 4; ++d)
     {
-        if (wave < 4 - d)
+        // Tiles whose rows lie beyond the block: L is zero there, and so is X.
+        if (wave < 4 - d && SP * (wave + d) < kb)
         {
             const int J = wave, I = wave + d;
             const int m = lane & 15;
