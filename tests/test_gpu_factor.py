@@ -457,3 +457,31 @@ def test_factorise_and_solve_equals_the_two_calls(gpu_device, precision):
         assert rep.steps == rep0.steps and rep.backward_error == rep0.backward_error <= 1e-10, mode
         del g
     plan.launch_mode(precision, "graph")
+
+
+def test_factorise_and_solve_reports_what_the_two_calls_report(gpu_device):
+    # A refinement that runs out of solves raises NotConvergedError with the factor attached (which the caller then
+    # uses or drops); a singular matrix raises SingularityError and returns no factor; a wrong launch mode is refused.
+    import torch
+    print("[gpu] factorise and solve errors", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    plan = sym.plan()
+    dm = plan.matrix(w.matrix)
+    b = torch.tensor(w.loads("random", nrhs=2, seed=6), device="cuda:0")
+    plan.launch_mode(Precision.FP32, "eager_streams")
+    with pytest.raises(oos.NotConvergedError) as caught:
+        SupernodalFactor.factorise_and_solve(sym, w.matrix, b, FactorOptions(precision=Precision.FP32),
+                                             device_matrix=dm, target=1e-15, max_steps=1)
+    factor = caught.value.factor
+    assert factor is not None and factor.inertia.positive == w.n
+    x, rep = FactorSolve(factor).solve_refined(b, target=1e-10, max_steps=20)
+    assert rep.converged and rep.backward_error <= 1e-10
+    del factor, caught
+    singular = SymmetricCsc(w.matrix.n, w.matrix.offsets, w.matrix.rows, w.matrix.values.copy())
+    first = singular.offsets[:-1]
+    singular.values[first[w.n // 2]] = 0.0  # a zero diagonal: the ratio test or a zero pivot fails there
+    with pytest.raises(oos.SingularityError):
+        SupernodalFactor.factorise_and_solve(sym, singular, b, target=1e-10, max_steps=4)
+    assert plan._lib.oosg_plan_launch_mode(plan.handle, int(Precision.FP64), 3) == 1  # OOSG_INVALID_ARGUMENT
+    plan.launch_mode(Precision.FP32, "graph")
