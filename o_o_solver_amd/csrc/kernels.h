@@ -123,6 +123,14 @@ __host__ __device__ inline bool keepsTriangleInverse(int c)
     return c >= WIDE_COLUMNS;
 }
 
+// The sweeps' L_RF products of a big supernode run in chunks of SWEEP_CHUNK blocks of 64 (columns of a forward row
+// tile, rows of a backward column block), each into a slot of partial sums that one more launch subtracts, in chunk
+// order, where the products are longer than one chunk: the top levels' few big supernodes then spread over many
+// workgroups. Items int4 {supernode, tile or block, chunk (-1: the whole range, straight into the result), slot};
+// subtractions int4 {first target row (of X, or of the update vectors U with bit 30 of .w), rows, first slot,
+// chunks}.
+constexpr int SWEEP_CHUNK = 8;
+
 // Small supernodes (at most SWEEP_INLINE_COLUMNS columns and SWEEP_INLINE_ROWS rows below them) do their L_RF
 // products in the sweeps inside the workgroup of their triangle, with no row tiles or column blocks of their own: the
 // low levels' thousands of small supernodes then take one launch and one workgroup each per sweep.
@@ -172,11 +180,15 @@ hipError_t launchUpdateTiles(const DevicePlan& p, T* panels, T* updates, T* linv
 // One level of the forward sweep: the supernodes of `list`, then the row tiles (supernode, tile) of its big ones.
 template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                         const int32_t* list, int count, const int2* rowTiles, int tileCount, hipStream_t stream);
+                         const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
+                         hipStream_t stream);
 // One level of the backward sweep: the column blocks (supernode, block) of its big supernodes, then all of `list`.
 template <typename TP>
 hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
-                          int count, const int2* columnBlocks, int blockCount, hipStream_t stream);
+                          int count, const int4* columnBlocks, int blockCount, double* partials, hipStream_t stream);
+// X or U rows −= Σ partial sums (the chunks of the subtractions' items), summed in chunk order.
+hipError_t launchSubtractPartials(double* X, double* U, int k, const int4* items, int count, const double* partials,
+                                  hipStream_t stream);
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream);
 
 hipError_t launchFillInt32(int32_t* data, int64_t count, int32_t value, hipStream_t stream);

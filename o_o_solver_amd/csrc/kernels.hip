@@ -1398,44 +1398,75 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
 // The update vectors of the big supernodes of a level: U(s)(rows of the tile) −= L_RF(tile, :)·X(cols(s)).
 template <typename TP>
 __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP* panels, const double* X, double* U,
-                                                         int k, const int2* tiles)
+                                                         int k, const int4* tiles, double* partials)
 {
     __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
-    const int2 tile = tiles[blockIdx.x];
+    const int4 tile = tiles[blockIdx.x];
     const int s = tile.x;
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     const int i0 = c + tile.y * SB;
     const int rows = min(SB, n - i0);
+    // The columns of this chunk (all of them without chunks).
+    const int qBegin = tile.z < 0 ? 0 : tile.z * SWEEP_CHUNK * SB;
+    const int qEnd = tile.z < 0 ? c : min(c, qBegin + SWEEP_CHUNK * SB);
     const TP* L = panels + p.panelOff[s];
     double4_t acc = {0, 0, 0, 0};
     // Block q0 + SB is loaded while block q0 is multiplied.
     StagedBlock<TP> nextL;
     StagedRows nextY;
-    nextL.load(L, n, i0, rows, 0, min(SB, c));
-    nextY.load(X, first, min(SB, c), k);
-    for (int q0 = 0; q0 < c; q0 += SB)
+    nextL.load(L, n, i0, rows, qBegin, min(SB, qEnd - qBegin));
+    nextY.load(X, first + qBegin, min(SB, qEnd - qBegin), k);
+    for (int q0 = qBegin; q0 < qEnd; q0 += SB)
     {
         __syncthreads(); // the previous block's products have read the LDS
         nextL.store(blk);
         nextY.store(ys);
         __syncthreads();
-        if (q0 + SB < c)
+        if (q0 + SB < qEnd)
         {
-            const int qb = min(SB, c - q0 - SB);
+            const int qb = min(SB, qEnd - q0 - SB);
             nextL.load(L, n, i0, rows, q0 + SB, qb);
             nextY.load(X, first + q0 + SB, qb, k);
         }
         acc = rowsTimesVectors(blk, ys, acc);
     }
-    double* u = U + p.rowStart[s] * k;
     const int h = accColumn();
+    if (tile.w >= 0)
+    {
+        double* out = partials + int64_t(tile.w) * SB * KMAX;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            out[accRow(v) * KMAX + h] = acc[v];
+        return;
+    }
+    double* u = U + p.rowStart[s] * k;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
     {
         const int i = accRow(v);
         if (i < rows && h < k)
             u[int64_t(i0 - c + i) * k + h] -= acc[v];
+    }
+}
+
+// X or U rows −= the sum of an item's chunks of partial sums, in chunk order; one workgroup per item.
+__global__ __launch_bounds__(256) void subtractPartialsKernel(double* X, double* U, int k, const int4* items,
+                                                              const double* partials)
+{
+    const int4 item = items[blockIdx.x];
+    const bool toU = (item.w >> 30) & 1;
+    const int chunks = item.w & ((1 << 30) - 1);
+    double* dst = (toU ? U : X) + int64_t(item.x) * k;
+    for (int idx = threadIdx.x; idx < item.y * KMAX; idx += blockDim.x)
+    {
+        const int i = idx / KMAX, h = idx % KMAX;
+        if (h >= k)
+            continue;
+        double sum = 0.0;
+        for (int ch = 0; ch < chunks; ++ch)
+            sum += partials[int64_t(item.z + ch) * SB * KMAX + idx];
+        dst[int64_t(i) * k + h] -= sum;
     }
 }
 
@@ -1460,37 +1491,48 @@ __device__ inline void gatherRows(double* xs, const double* X, const int32_t* ro
 // X(cols of block J) −= L_RF(:, J)ᵀ·X(rows), for the column blocks of the big and wide supernodes of a level.
 template <typename TP>
 __global__ __launch_bounds__(256) void backwardColumnsKernel(DevicePlan p, const TP* panels, double* X, int k,
-                                                             const int2* blocks)
+                                                             const int4* blocks, double* partials)
 {
     __shared__ TP blk[SB * SBP];
     __shared__ double xs[SB * KMAX];
-    const int2 block = blocks[blockIdx.x];
+    const int4 block = blocks[blockIdx.x];
     const int s = block.x;
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
     const int q0 = block.y * SB, qb = min(SB, c - q0);
+    // The rows of this chunk (all of L_RF without chunks).
+    const int iBegin = block.z < 0 ? c : c + block.z * SWEEP_CHUNK * SB;
+    const int iEnd = block.z < 0 ? n : min(n, iBegin + SWEEP_CHUNK * SB);
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
     double4_t acc = {0, 0, 0, 0};
     // Row block i0 + SB is loaded while block i0 is multiplied.
     StagedBlock<TP> nextL;
     StagedRows nextX;
-    nextL.load(L, n, c, min(SB, n - c), q0, qb);
-    nextX.gather(X, rows, first, c, c, min(SB, n - c), k);
-    for (int i0 = c; i0 < n; i0 += SB)
+    nextL.load(L, n, iBegin, min(SB, iEnd - iBegin), q0, qb);
+    nextX.gather(X, rows, first, c, iBegin, min(SB, iEnd - iBegin), k);
+    for (int i0 = iBegin; i0 < iEnd; i0 += SB)
     {
         __syncthreads(); // the previous block's products have read the LDS
         nextL.store(blk);
         nextX.store(xs);
         __syncthreads();
-        if (i0 + SB < n)
+        if (i0 + SB < iEnd)
         {
-            const int ib = min(SB, n - i0 - SB);
+            const int ib = min(SB, iEnd - i0 - SB);
             nextL.load(L, n, i0 + SB, ib, q0, qb);
             nextX.gather(X, rows, first, c, i0 + SB, ib, k);
         }
         acc = columnsTimesVectors(blk, xs, acc);
     }
     const int h = accColumn();
+    if (block.w >= 0)
+    {
+        double* out = partials + int64_t(block.w) * SB * KMAX;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            out[accRow(v) * KMAX + h] = acc[v];
+        return;
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v)
     {
@@ -1981,23 +2023,32 @@ hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateT
 
 template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                         const int32_t* list, int count, const int2* rowTiles, int tileCount, hipStream_t stream)
+                         const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
+                         hipStream_t stream)
 {
     if (count > 0)
         hipLaunchKernelGGL(forwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, list);
     if (tileCount > 0)
         hipLaunchKernelGGL(forwardRowsKernel<TP>, dim3(tileCount), dim3(256), 0, stream, p, panels, X, U, k,
-                           rowTiles);
+                           rowTiles, partials);
+    return hipGetLastError();
+}
+
+hipError_t launchSubtractPartials(double* X, double* U, int k, const int4* items, int count, const double* partials,
+                                  hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(subtractPartialsKernel, dim3(count), dim3(256), 0, stream, X, U, k, items, partials);
     return hipGetLastError();
 }
 
 template <typename TP>
 hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
-                          int count, const int2* columnBlocks, int blockCount, hipStream_t stream)
+                          int count, const int4* columnBlocks, int blockCount, double* partials, hipStream_t stream)
 {
     if (blockCount > 0)
         hipLaunchKernelGGL(backwardColumnsKernel<TP>, dim3(blockCount), dim3(256), 0, stream, p, panels, X, k,
-                           columnBlocks);
+                           columnBlocks, partials);
     if (count > 0)
         hipLaunchKernelGGL(backwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, k, list);
     return hipGetLastError();
@@ -2297,9 +2348,9 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchExtendTiles<T>(const DevicePlan&, T*, T*, const UpdateTile*, int, const int32_t*,        \
                                              const AssemblyItem*, int, hipStream_t);                                   \
     template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \
-                                         int, const int2*, int, hipStream_t);                                          \
+                                         int, const int4*, int, double*, hipStream_t);                                 \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
-                                          const int2*, int, hipStream_t);                                              \
+                                          const int4*, int, double*, hipStream_t);                                     \
     template hipError_t launchTriangleInverse<T>(const DevicePlan&, const T*, T*, const int2*, int, int, hipStream_t); \
     template hipError_t launchTriangleProducts<T>(const DevicePlan&, const T*, const double*, double*, int,            \
                                                   const int2*, int, int, hipStream_t);                                 \

@@ -313,6 +313,9 @@ double frontFlops(int64_t c, int64_t r)
 struct oosg_plan
 {
     int device = 0;
+    // Its holders: the caller that made it, and every matrix and factor made from it (the last release deletes it, so
+    // that a matrix or factor released after the plan still finds it).
+    int32_t references = 1;
     hipStream_t stream = nullptr;
     bool ownsStream = true; // false: the stream of another plan (the structure of a pivoted factor)
     hipStream_t lanes[LANES] = {nullptr, nullptr, nullptr}; // lanes[0] = stream
@@ -327,9 +330,12 @@ struct oosg_plan
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
     std::vector<int64_t> rowStart, panelOff, updateOff, linvOff;
-    // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward).
-    std::vector<int2> forwardTiles, backwardBlocks;
-    std::vector<int64_t> forwardTileStart, backwardBlockStart;
+    // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward), in chunks where
+    // their products are long, with the subtractions of the chunks' partial sums; the most slots of partial sums a
+    // level uses.
+    std::vector<int4> forwardTiles, backwardBlocks, forwardSubtractions, backwardSubtractions;
+    std::vector<int64_t> forwardTileStart, backwardBlockStart, forwardSubtractionStart, backwardSubtractionStart;
+    int64_t partialSlots = 0;
     // The wide supernodes' triangle blocks (s, I), by level (the sweeps' products with the whole inverse), and per
     // level the rounds of the factorisation that build the inverses.
     std::vector<int2> triangleItems;
@@ -367,6 +373,7 @@ struct oosg_plan
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
+        dForwardSubtractions, dBackwardSubtractions,
         dTriangleItems, dInverseItems;
     oosg::DevicePlan dp{};
 
@@ -390,6 +397,7 @@ struct oosg_plan
     DeviceBuffer pivotW, pivotPerm, pivotDiag, pivotSub, pivotDesc, pivotStates, pivotList, pivotTiles, pivotItems,
         pivotMaps, pivotRanges, pivotInverseItems;
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
+    DeviceBuffer solvePartials; // the sweeps' slots of partial sums (partialSlots × 64 × 16)
     DeviceBuffer solveW; // the triangle products' results (rows of the wide supernodes, n × k row-major)
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
     DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
@@ -675,10 +683,16 @@ void deriveLevels(oosg_plan& p)
     // The sweeps' row tiles and column blocks of the big supernodes, level by level.
     p.forwardTileStart.assign(p.levels + 1, 0);
     p.backwardBlockStart.assign(p.levels + 1, 0);
+    p.forwardSubtractionStart.assign(p.levels + 1, 0);
+    p.backwardSubtractionStart.assign(p.levels + 1, 0);
     p.triangleStart.assign(p.levels + 1, 0);
+    p.partialSlots = 0;
+    if (p.rowStart[p.S] > INT32_MAX)
+        fail(OOSG_INVALID_ARGUMENT, "plan: more than 2^31 rows below the supernodes");
     for (int32_t l = 0; l < p.levels; ++l)
     {
         int maxBlocks = 0;
+        int32_t forwardSlots = 0, backwardSlots = 0; // each sweep's slots of this level
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
             const int32_t s = p.levelList[q];
@@ -690,15 +704,46 @@ void deriveLevels(oosg_plan& p)
             // factor whose columns were all delayed has none (c = 0).
             if (oosg::sweepRowsInline(p.ncol[s], p.nrow[s]))
                 continue;
-            if (p.ncol[s] > 0)
-                for (int32_t t = 0; t * 64 < p.nrow[s]; ++t)
-                    p.forwardTiles.push_back(make_int2(s, t));
-            if (p.nrow[s] > 0)
-                for (int32_t b = 0; b * 64 < p.ncol[s]; ++b)
-                    p.backwardBlocks.push_back(make_int2(s, b));
+            const int32_t c = p.ncol[s], r = p.nrow[s];
+            const int32_t columnBlocks = (c + 63) / 64, rowTiles = (r + 63) / 64;
+            const int32_t columnChunks = (columnBlocks + oosg::SWEEP_CHUNK - 1) / oosg::SWEEP_CHUNK;
+            const int32_t rowChunks = (rowTiles + oosg::SWEEP_CHUNK - 1) / oosg::SWEEP_CHUNK;
+            if (c > 0)
+                for (int32_t t = 0; t < rowTiles; ++t)
+                {
+                    if (columnChunks == 1)
+                    {
+                        p.forwardTiles.push_back(make_int4(s, t, -1, -1));
+                        continue;
+                    }
+                    // The row tile's columns in chunks, then the subtraction of their sums from its update rows.
+                    for (int32_t ch = 0; ch < columnChunks; ++ch)
+                        p.forwardTiles.push_back(make_int4(s, t, ch, forwardSlots + ch));
+                    p.forwardSubtractions.push_back(make_int4(static_cast<int32_t>(p.rowStart[s]) + 64 * t,
+                                                              std::min(64, r - 64 * t), forwardSlots,
+                                                              columnChunks | (1 << 30)));
+                    forwardSlots += columnChunks;
+                }
+            if (r > 0)
+                for (int32_t b = 0; b < columnBlocks; ++b)
+                {
+                    if (rowChunks == 1)
+                    {
+                        p.backwardBlocks.push_back(make_int4(s, b, -1, -1));
+                        continue;
+                    }
+                    for (int32_t ch = 0; ch < rowChunks; ++ch)
+                        p.backwardBlocks.push_back(make_int4(s, b, ch, backwardSlots + ch));
+                    p.backwardSubtractions.push_back(make_int4(p.first[s] + 64 * b, std::min(64, c - 64 * b),
+                                                               backwardSlots, rowChunks));
+                    backwardSlots += rowChunks;
+                }
         }
+        p.partialSlots = std::max<int64_t>(p.partialSlots, std::max(forwardSlots, backwardSlots));
         p.forwardTileStart[l + 1] = static_cast<int64_t>(p.forwardTiles.size());
         p.backwardBlockStart[l + 1] = static_cast<int64_t>(p.backwardBlocks.size());
+        p.forwardSubtractionStart[l + 1] = static_cast<int64_t>(p.forwardSubtractions.size());
+        p.backwardSubtractionStart[l + 1] = static_cast<int64_t>(p.backwardSubtractions.size());
         // Wide supernodes: their triangle's blocks (s, I) for the sweeps' products with the whole inverse, and the
         // rounds d of the factorisation that build that inverse (items (s, I), I >= d, in round d).
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
@@ -1124,6 +1169,8 @@ void uploadSweepPlan(oosg_plan& p)
     p.dLinvOff = upload(p.linvOff, st, "plan: inverse offsets");
     p.dForwardTiles = upload(p.forwardTiles, st, "plan: forward row tiles");
     p.dBackwardBlocks = upload(p.backwardBlocks, st, "plan: backward column blocks");
+    p.dForwardSubtractions = upload(p.forwardSubtractions, st, "plan: forward subtractions");
+    p.dBackwardSubtractions = upload(p.backwardSubtractions, st, "plan: backward subtractions");
     p.dTriangleItems = upload(p.triangleItems, st, "plan: triangle blocks");
     p.dInverseItems = upload(p.inverseItems, st, "plan: triangle inverse blocks");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
@@ -1160,6 +1207,7 @@ void uploadPlan(oosg_plan& p)
 
 template <typename TP>
 void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t l0, int32_t l1, hipStream_t st);
+void ensureSolveBuffers(oosg_plan& p);
 
 template <typename T>
 void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled)
@@ -1291,6 +1339,23 @@ std::string equationList(const std::vector<int32_t>& equations, size_t total)
 // The zero-pivot test, the inertia and the ratio test of MultifrontalDriver::finish
 // (src/linalg/supernodal/MultifrontalDriver.cpp:704-820), on the host from the pivots the kernels wrote.
 void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options, oosg_factor** out);
+
+void holdPlan(oosg_plan* p)
+{
+    ++p->references;
+}
+
+void releasePlan(oosg_plan* p)
+{
+    if (--p->references > 0)
+        return;
+    (void)hipSetDevice(p->device);
+    if (p->stream)
+        (void)hipStreamSynchronize(p->stream);
+    delete p;
+    // A release reports nothing: an error of its calls must not surface as the next launch's error.
+    (void)hipGetLastError();
+}
 
 // The pivots of a positive definite factor on the host, copied on first use (FP32: d_j / s_j², those of the matrix).
 const std::vector<double>& hostDiagonal(const oosg_factor& f)
@@ -1846,9 +1911,13 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
     double* X = p.solveX.as<double>();
     double* U = p.solveU.as<double>();
     const int32_t* levels = p.dLevelList.as<int32_t>();
-    const int2* forwardTiles = p.dForwardTiles.as<int2>();
+    const int4* forwardTiles = p.dForwardTiles.as<int4>();
+    const int4* subtractions = p.dForwardSubtractions.as<int4>();
     const int2* triangle = p.dTriangleItems.as<int2>();
     double* W = p.solveW.as<double>();
+    double* partials = p.solvePartials.as<double>();
+    if (p.partialSlots > 0 && partials == nullptr)
+        fail(OOSG_INTERNAL, "solve: the sweeps' partial sums have no buffer");
     // The profile counts bytes as the work of the sweeps (they are bound by HBM): the values of L (or of an inverse)
     // a launch reads once, and the right-hand-side rows it reads or writes once.
     const double T = sizeof(TP), row = 8.0 * k;
@@ -1865,7 +1934,7 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
         // back into X), then every L_RF row tile of the level.
         timed(PROFILE_FORWARD, w.triangleValues * T + w.forwardRows * row, [&] {
             check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
-                                          p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, st),
+                                          p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, partials, st),
                   "solve: forward sweep");
         });
         const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
@@ -1880,10 +1949,14 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
                   "solve: forward sweep (triangles)");
         }
         const int tiles = static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]);
+        const int subtracted = static_cast<int>(p.forwardSubtractionStart[l + 1] - p.forwardSubtractionStart[l]);
         if (tiles > 0)
             timed(PROFILE_FORWARD_ROWS, w.rowValues * T + w.rowTileRows * row, [&] {
                 check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, nullptr, 0,
-                                              forwardTiles + p.forwardTileStart[l], tiles, st),
+                                              forwardTiles + p.forwardTileStart[l], tiles, partials, st),
+                      "solve: forward sweep (rows)");
+                check(oosg::launchSubtractPartials(X, U, k, subtractions + p.forwardSubtractionStart[l], subtracted,
+                                                   partials, st),
                       "solve: forward sweep (rows)");
             });
     }
@@ -1898,7 +1971,11 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
     const int32_t* levels = p.dLevelList.as<int32_t>();
-    const int2* backwardBlocks = p.dBackwardBlocks.as<int2>();
+    const int4* backwardBlocks = p.dBackwardBlocks.as<int4>();
+    const int4* subtractions = p.dBackwardSubtractions.as<int4>();
+    double* partials = p.solvePartials.as<double>();
+    if (p.partialSlots > 0 && partials == nullptr)
+        fail(OOSG_INTERNAL, "solve: the sweeps' partial sums have no buffer");
     const int2* triangle = p.dTriangleItems.as<int2>();
     double* W = p.solveW.as<double>();
     const double T = sizeof(TP), row = 8.0 * k;
@@ -1919,10 +1996,14 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
         // The L_RFᵀ products of every supernode first (column blocks), then the wide supernodes' triangles (with
         // their whole inverses), then the other triangles.
         const int blocks = static_cast<int>(p.backwardBlockStart[l + 1] - p.backwardBlockStart[l]);
+        const int subtracted = static_cast<int>(p.backwardSubtractionStart[l + 1] - p.backwardSubtractionStart[l]);
         if (blocks > 0)
             timed(PROFILE_BACKWARD_COLUMNS, w.rowValues * T + w.columnRows * row, [&] {
                 check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0,
-                                               backwardBlocks + p.backwardBlockStart[l], blocks, st),
+                                               backwardBlocks + p.backwardBlockStart[l], blocks, partials, st),
+                      "solve: backward sweep (columns)");
+                check(oosg::launchSubtractPartials(X, nullptr, k, subtractions + p.backwardSubtractionStart[l],
+                                                   subtracted, partials, st),
                       "solve: backward sweep (columns)");
             });
         const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
@@ -1938,7 +2019,7 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
         }
         timed(PROFILE_BACKWARD, w.triangleValues * T + w.backwardRows * row, [&] {
             check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
-                                           p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, st),
+                                           p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, partials, st),
                   "solve: backward sweep");
         });
     }
@@ -1964,10 +2045,7 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
         // In pivot order: X permuted into the structure's own X, swept, and permuted back.
         oosg_plan& d = *f.pivoted->structure;
         const int64_t n = p.n;
-        ensure(d.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
-        ensure(d.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
-        ensure(d.solveU, static_cast<size_t>(std::max<int64_t>(d.rowStart[d.S], 1) * 16) * sizeof(double),
-               "solve: update vectors");
+        ensureSolveBuffers(d);
         const int32_t* perm = f.pivoted->perm.as<int32_t>();
         check(oosg::launchPermuteRows(p.solveX.as<double>(), d.solveX.as<double>(), perm, n, k, false, p.stream),
               "solve: pivot order");
@@ -1991,7 +2069,7 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
     }
     p.sweepGraph.run(p.stream,
                      {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr, p.solveW.ptr,
-                      reinterpret_cast<const void*>(static_cast<intptr_t>(k * 2 + f.precision))},
+                      p.solvePartials.ptr, reinterpret_cast<const void*>(static_cast<intptr_t>(k * 2 + f.precision))},
                      body);
 }
 
@@ -2003,10 +2081,7 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     const int64_t width = std::min<int64_t>(nrhs, 16);
     (void)width;
     // Sized for 16 columns whatever the call: the buffers (and the graphs replaying the sweeps on them) stay put.
-    ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
-    ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
-    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
-           "solve: update vectors");
+    ensureSolveBuffers(p);
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
     for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
     {
@@ -2106,6 +2181,8 @@ void ensureSolveBuffers(oosg_plan& p)
     ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
     ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
            "solve: update vectors");
+    ensure(p.solvePartials, static_cast<size_t>(std::max<int64_t>(p.partialSlots, 1) * 64 * 16) * sizeof(double),
+           "solve: partial sums");
 }
 
 // out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides. With
@@ -2348,8 +2425,9 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                                         p->smallList.size() + p->stepFronts.size() + p->levelList.size()) *
                                            4 +
                                        p->panelOff.size() * 24 +
-                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->triangleItems.size() +
-                                        p->inverseItems.size()) * 8 +
+                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->forwardSubtractions.size() +
+                                        p->backwardSubtractions.size()) * 16 +
+                                       (p->triangleItems.size() + p->inverseItems.size()) * 8 +
                                        p->blocks.size() * sizeof(oosg::AssemblyBlock) +
                                        p->items.size() * sizeof(oosg::AssemblyItem) +
                                        p->tiles.size() * sizeof(oosg::UpdateTile);
@@ -2368,9 +2446,6 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                 check(hipStreamCreateWithFlags(&p->lanes[lane], hipStreamNonBlocking), "hipStreamCreate");
             for (hipEvent_t& e : p->laneEvents)
                 check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            check(hipStreamCreateWithFlags(&p->sweepStream, hipStreamNonBlocking), "hipStreamCreate");
-            for (hipEvent_t& e : p->aheadEvents)
-                check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             check(oosg::configureKernels(), "plan: kernel limits");
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
@@ -2383,10 +2458,7 @@ void oosg_plan_destroy(oosg_plan* plan)
 {
     if (plan == nullptr)
         return;
-    (void)hipSetDevice(plan->device);
-    if (plan->stream)
-        (void)hipStreamSynchronize(plan->stream);
-    delete plan;
+    releasePlan(plan);
 }
 
 oosg_status oosg_plan_stats(const oosg_plan* plan, int64_t* stats)
@@ -3106,6 +3178,7 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             }
             m->dCsrGroups = upload(m->csrGroups, st, "matrix: row groups");
             check(hipStreamSynchronize(st), "matrix: upload");
+            holdPlan(plan);
             *out = m.release();
         });
 }
@@ -3169,8 +3242,11 @@ void oosg_matrix_destroy(oosg_matrix* m)
 {
     if (m == nullptr)
         return;
-    (void)hipSetDevice(m->plan->device);
+    oosg_plan* plan = m->plan;
+    (void)hipSetDevice(plan->device);
     delete m;
+    releasePlan(plan);
+    (void)hipGetLastError();
 }
 
 oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options,
@@ -3220,6 +3296,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 f->info.flops = plan->flops;
                 f->info.precision = OOSG_FP64;
                 finishPivoted(*f);
+                holdPlan(plan);
                 *out = f.release();
                 return;
             }
@@ -3311,6 +3388,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             f->info.precision = opts.precision;
             // FP32: the pivots of S·A·S divided by s_j² are those of A (same signs: the inertia is kept).
             finish(*plan, *matrix, *f, zeroAt, stats, opts);
+            holdPlan(plan);
             *out = f.release();
         }
     }
@@ -3324,8 +3402,11 @@ void oosg_factor_destroy(oosg_factor* f)
 {
     if (f == nullptr)
         return;
-    (void)hipSetDevice(f->plan->device);
-    delete f;
+    oosg_plan* plan = f->plan;
+    (void)hipSetDevice(plan->device);
+    delete f; // its buffers go back to the plan's caches
+    releasePlan(plan);
+    (void)hipGetLastError();
 }
 
 oosg_status oosg_factor_info_get(const oosg_factor* f, oosg_factor_info* info)
@@ -3579,6 +3660,13 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
             const int64_t n = plan->n;
             if (ahead)
             {
+                // The sweep's stream and events, made on first use.
+                if (plan->sweepStream == nullptr)
+                {
+                    check(hipStreamCreateWithFlags(&plan->sweepStream, hipStreamNonBlocking), "hipStreamCreate");
+                    for (hipEvent_t& e : plan->aheadEvents)
+                        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                }
                 // b into the refinement's row-major copy, and scaled (FP32) into the sweeps' vectors.
                 ensureSolveBuffers(*plan);
                 ensure(plan->cgB, static_cast<size_t>(n) * 16 * sizeof(double), "solve: refinement");

@@ -459,11 +459,11 @@ def test_factorise_and_solve_equals_the_two_calls(gpu_device, precision):
     plan.launch_mode(precision, "graph")
 
 
-def test_factorise_and_solve_reports_what_the_two_calls_report(gpu_device):
-    # A refinement that runs out of solves raises NotConvergedError with the factor attached (which the caller then
-    # uses or drops); a singular matrix raises SingularityError and returns no factor; a wrong launch mode is refused.
+def test_factorise_and_solve_reports_not_converged_with_the_factor(gpu_device):
+    # A refinement that runs out of solves raises NotConvergedError with the factor attached, which the caller then
+    # uses or drops; a launch mode out of range is refused.
     import torch
-    print("[gpu] factorise and solve errors", flush=True)
+    print("[gpu] factorise and solve: not converged", flush=True)
     w = CASES["plate 40"]()
     sym = w.analyse()
     plan = sym.plan()
@@ -477,11 +477,17 @@ def test_factorise_and_solve_reports_what_the_two_calls_report(gpu_device):
     assert factor is not None and factor.inertia.positive == w.n
     x, rep = FactorSolve(factor).solve_refined(b, target=1e-10, max_steps=20)
     assert rep.converged and rep.backward_error <= 1e-10
-    del factor, caught
-    singular = SymmetricCsc(w.matrix.n, w.matrix.offsets, w.matrix.rows, w.matrix.values.copy())
-    first = singular.offsets[:-1]
-    singular.values[first[w.n // 2]] = 0.0  # a zero diagonal: the ratio test or a zero pivot fails there
-    with pytest.raises(oos.SingularityError):
-        SupernodalFactor.factorise_and_solve(sym, singular, b, target=1e-10, max_steps=4)
     assert plan._lib.oosg_plan_launch_mode(plan.handle, int(Precision.FP64), 3) == 1  # OOSG_INVALID_ARGUMENT
     plan.launch_mode(Precision.FP32, "graph")
+
+
+def test_factorise_and_solve_reports_a_singular_matrix(gpu_device):
+    import torch
+    print("[gpu] factorise and solve: singular", flush=True)
+    w = CASES["plate 40"]()
+    sym = w.analyse()
+    b = torch.tensor(w.loads("random", nrhs=2, seed=6), device="cuda:0")
+    singular = SymmetricCsc(w.matrix.n, w.matrix.offsets, w.matrix.rows, w.matrix.values.copy())
+    singular.values[singular.offsets[w.n // 2]] = 0.0  # a zero diagonal: the ratio test or a zero pivot fails
+    with pytest.raises(oos.SingularityError):
+        SupernodalFactor.factorise_and_solve(sym, singular, b, target=1e-10, max_steps=4)
