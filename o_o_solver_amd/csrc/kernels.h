@@ -159,13 +159,12 @@ hipError_t launchDiagonalBlocks(const DevicePlan& p, T* panels, T* linv, const i
 template <typename T>
 hipError_t launchTriangleInverse(const DevicePlan& p, const T* panels, T* linv, const int2* items, int count, int step,
                                  hipStream_t stream);
-// The triangles of the wide supernodes in a sweep: items (s, I) write W(rows of block I) = Σ_{J<=I} X_IJ·X(block J)
-// (forward) or Σ_{J>=I} X_JIᵀ·X(block J) (transpose), then copyRows sets X ← W on those rows.
+// The triangles of the wide supernodes in a sweep: items int4 {s, I, chunk, slot} write into their slot of partial
+// sums the chunk's part (SWEEP_CHUNK blocks J) of Σ_{J<=I} X_IJ·X(block J) (forward) or Σ_{J>=I} X_JIᵀ·X(block J)
+// (transpose); the subtraction launch's items with bit 29 (assign) then set X's rows of block I to the sum.
 template <typename TP>
-hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, double* W, int k,
-                                  const int2* items, int count, int transpose, hipStream_t stream);
-hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
-                                  hipStream_t stream);
+hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, int k, const int4* items,
+                                  int count, int transpose, double* partials, hipStream_t stream);
 
 // The update matrices: C = −A·diag(d)·Bᵀ + Σ children's contributions, on TS × TS tiles (64 or 128; the same bits
 // for both sizes).

@@ -1450,13 +1450,14 @@ __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP*
     }
 }
 
-// X or U rows −= the sum of an item's chunks of partial sums, in chunk order; one workgroup per item.
+// X or U rows −= the sum of an item's chunks of partial sums, in chunk order (or = with bit 29 of .w); one workgroup
+// per item.
 __global__ __launch_bounds__(256) void subtractPartialsKernel(double* X, double* U, int k, const int4* items,
                                                               const double* partials)
 {
     const int4 item = items[blockIdx.x];
-    const bool toU = (item.w >> 30) & 1;
-    const int chunks = item.w & ((1 << 30) - 1);
+    const bool toU = (item.w >> 30) & 1, assign = (item.w >> 29) & 1;
+    const int chunks = item.w & ((1 << 29) - 1);
     double* dst = (toU ? U : X) + int64_t(item.x) * k;
     for (int idx = threadIdx.x; idx < item.y * KMAX; idx += blockDim.x)
     {
@@ -1466,7 +1467,8 @@ __global__ __launch_bounds__(256) void subtractPartialsKernel(double* X, double*
         double sum = 0.0;
         for (int ch = 0; ch < chunks; ++ch)
             sum += partials[int64_t(item.z + ch) * SB * KMAX + idx];
-        dst[int64_t(i) * k + h] -= sum;
+        double& at = dst[int64_t(i) * k + h];
+        at = assign ? sum : at - sum;
     }
 }
 
@@ -1607,18 +1609,21 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
 // The triangle of a wide supernode in a sweep, by the whole inverse X = L_FF⁻¹: item (s, I) writes rows of block I
 // of W: Σ_{J<=I} X_IJ·x_J (forward) or Σ_{J>=I} X_JIᵀ·x_J (transpose), x = X(cols of s); every item reads x only.
 template <typename TP>
-__global__ __launch_bounds__(256) void triangleProductsKernel(DevicePlan p, const TP* linv, const double* X, double* W,
-                                                              int k, const int2* items, int transpose)
+__global__ __launch_bounds__(256) void triangleProductsKernel(DevicePlan p, const TP* linv, const double* X, int k,
+                                                              const int4* items, int transpose, double* partials)
 {
     __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
-    const int2 item = items[blockIdx.x];
+    const int4 item = items[blockIdx.x];
     const int s = item.x, I = item.y;
-    const int c = p.ncol[s], first = p.first[s], nb = (c + SB - 1) / SB;
+    const int c = p.ncol[s], nb = (c + SB - 1) / SB;
+    const int first = p.first[s];
     const TP* Xinv = linv + p.linvOff[s] + blockInverseValues(c);
     const int ib = min(SB, c - SB * I);
     double4_t acc = {0, 0, 0, 0};
-    const int jFirst = transpose ? I : 0, jEnd = transpose ? nb : I + 1;
+    // The blocks J of this chunk (SWEEP_CHUNK of them) of the item's range, J <= I (forward) or J >= I (transpose).
+    const int rangeFirst = transpose ? I : 0, rangeEnd = transpose ? nb : I + 1;
+    const int jFirst = rangeFirst + item.z * SWEEP_CHUNK, jEnd = min(rangeEnd, jFirst + SWEEP_CHUNK);
     // Block J + 1 is loaded while block J is multiplied.
     StagedBlock<TP> nextB;
     StagedRows nextY;
@@ -1642,32 +1647,12 @@ __global__ __launch_bounds__(256) void triangleProductsKernel(DevicePlan p, cons
             loadJ(J + 1);
         acc = transpose ? columnsTimesVectors(blk, ys, acc) : rowsTimesVectors(blk, ys, acc);
     }
+    // Into the item's slot of partial sums: the chunks' sums become the block's rows of X once every item has read X.
     const int h = accColumn();
+    double* out = partials + int64_t(item.w) * SB * KMAX;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-    {
-        const int i = accRow(v);
-        if (i < ib && h < k)
-            W[int64_t(first + SB * I + i) * k + h] = acc[v];
-    }
-}
-
-// X ← W on the rows of the items' blocks.
-__global__ __launch_bounds__(256) void copyTriangleRowsKernel(DevicePlan p, const double* W, double* X, int k,
-                                                              const int2* items)
-{
-    const int2 item = items[blockIdx.x];
-    const int c = p.ncol[item.x], first = p.first[item.x];
-    const int ib = min(SB, c - SB * item.y);
-    for (int idx = threadIdx.x; idx < SB * KMAX; idx += blockDim.x)
-    {
-        const int i = idx >> 4, h = idx & (KMAX - 1);
-        if (i < ib && h < k)
-        {
-            const int64_t at = int64_t(first + SB * item.y + i) * k + h;
-            X[at] = W[at];
-        }
-    }
+        out[accRow(v) * KMAX + h] = acc[v];
 }
 
 // ---- vectors and the sparse product --------------------------------------------------------------------------------
@@ -2064,20 +2049,12 @@ hipError_t launchTriangleInverse(const DevicePlan& p, const T* panels, T* linv, 
 }
 
 template <typename TP>
-hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, double* W, int k,
-                                  const int2* items, int count, int transpose, hipStream_t stream)
+hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const double* X, int k, const int4* items,
+                                  int count, int transpose, double* partials, hipStream_t stream)
 {
     if (count > 0)
-        hipLaunchKernelGGL(triangleProductsKernel<TP>, dim3(count), dim3(256), 0, stream, p, linv, X, W, k, items,
-                           transpose);
-    return hipGetLastError();
-}
-
-hipError_t launchCopyTriangleRows(const DevicePlan& p, const double* W, double* X, int k, const int2* items, int count,
-                                  hipStream_t stream)
-{
-    if (count > 0)
-        hipLaunchKernelGGL(copyTriangleRowsKernel, dim3(count), dim3(256), 0, stream, p, W, X, k, items);
+        hipLaunchKernelGGL(triangleProductsKernel<TP>, dim3(count), dim3(256), 0, stream, p, linv, X, k, items,
+                           transpose, partials);
     return hipGetLastError();
 }
 
@@ -2352,8 +2329,8 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int4*, int, double*, hipStream_t);                                     \
     template hipError_t launchTriangleInverse<T>(const DevicePlan&, const T*, T*, const int2*, int, int, hipStream_t); \
-    template hipError_t launchTriangleProducts<T>(const DevicePlan&, const T*, const double*, double*, int,            \
-                                                  const int2*, int, int, hipStream_t);                                 \
+    template hipError_t launchTriangleProducts<T>(const DevicePlan&, const T*, const double*, int, const int4*, int,   \
+                                                  int, double*, hipStream_t);                                          \
     template size_t smallFrontBytes<T>(int);
 
 OOSG_INSTANTIATE(double)

@@ -336,10 +336,11 @@ struct oosg_plan
     std::vector<int4> forwardTiles, backwardBlocks, forwardSubtractions, backwardSubtractions;
     std::vector<int64_t> forwardTileStart, backwardBlockStart, forwardSubtractionStart, backwardSubtractionStart;
     int64_t partialSlots = 0;
-    // The wide supernodes' triangle blocks (s, I), by level (the sweeps' products with the whole inverse), and per
-    // level the rounds of the factorisation that build the inverses.
-    std::vector<int2> triangleItems;
-    std::vector<int64_t> triangleStart;
+    // The wide supernodes' triangle blocks (s, I), by level (the sweeps' products with the whole inverse), as chunk
+    // items for each sweep {s, I, chunk, slot} with the assignments of their sums to X; and per level the rounds of
+    // the factorisation that build the inverses.
+    std::vector<int4> forwardTriangles, backwardTriangles, forwardTriangleSums, backwardTriangleSums;
+    std::vector<int64_t> forwardTriangleStart, backwardTriangleStart, triangleSumStart;
     struct InverseRound
     {
         int step; // −1: the last launch
@@ -374,7 +375,7 @@ struct oosg_plan
         dLinvOff;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dForwardSubtractions, dBackwardSubtractions,
-        dTriangleItems, dInverseItems;
+        dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
     oosg::DevicePlan dp{};
 
     DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
@@ -398,7 +399,6 @@ struct oosg_plan
         pivotMaps, pivotRanges, pivotInverseItems;
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
     DeviceBuffer solvePartials; // the sweeps' slots of partial sums (partialSlots × 64 × 16)
-    DeviceBuffer solveW; // the triangle products' results (rows of the wide supernodes, n × k row-major)
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
     DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
     uint64_t budget = 0;
@@ -685,7 +685,9 @@ void deriveLevels(oosg_plan& p)
     p.backwardBlockStart.assign(p.levels + 1, 0);
     p.forwardSubtractionStart.assign(p.levels + 1, 0);
     p.backwardSubtractionStart.assign(p.levels + 1, 0);
-    p.triangleStart.assign(p.levels + 1, 0);
+    p.forwardTriangleStart.assign(p.levels + 1, 0);
+    p.backwardTriangleStart.assign(p.levels + 1, 0);
+    p.triangleSumStart.assign(p.levels + 1, 0);
     p.partialSlots = 0;
     if (p.rowStart[p.S] > INT32_MAX)
         fail(OOSG_INVALID_ARGUMENT, "plan: more than 2^31 rows below the supernodes");
@@ -746,15 +748,35 @@ void deriveLevels(oosg_plan& p)
         p.backwardSubtractionStart[l + 1] = static_cast<int64_t>(p.backwardSubtractions.size());
         // Wide supernodes: their triangle's blocks (s, I) for the sweeps' products with the whole inverse, and the
         // rounds d of the factorisation that build that inverse (items (s, I), I >= d, in round d).
+        int32_t forwardTriangleSlots = 0, backwardTriangleSlots = 0;
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
             const int32_t s = p.levelList[q];
             if (!oosg::keepsTriangleInverse(p.ncol[s]))
                 continue;
-            for (int32_t I = 0; I * 64 < p.ncol[s]; ++I)
-                p.triangleItems.push_back(make_int2(s, I));
+            const int32_t nb = (p.ncol[s] + 63) / 64;
+            for (int32_t I = 0; I < nb; ++I)
+            {
+                const int32_t rows = std::min(64, p.ncol[s] - 64 * I);
+                // Forward: blocks J <= I; backward: J >= I; in chunks of SWEEP_CHUNK blocks.
+                const int32_t forwardChunks = (I + 1 + oosg::SWEEP_CHUNK - 1) / oosg::SWEEP_CHUNK;
+                const int32_t backwardChunks = (nb - I + oosg::SWEEP_CHUNK - 1) / oosg::SWEEP_CHUNK;
+                for (int32_t ch = 0; ch < forwardChunks; ++ch)
+                    p.forwardTriangles.push_back(make_int4(s, I, ch, forwardTriangleSlots + ch));
+                for (int32_t ch = 0; ch < backwardChunks; ++ch)
+                    p.backwardTriangles.push_back(make_int4(s, I, ch, backwardTriangleSlots + ch));
+                p.forwardTriangleSums.push_back(
+                    make_int4(p.first[s] + 64 * I, rows, forwardTriangleSlots, forwardChunks | (1 << 29)));
+                p.backwardTriangleSums.push_back(
+                    make_int4(p.first[s] + 64 * I, rows, backwardTriangleSlots, backwardChunks | (1 << 29)));
+                forwardTriangleSlots += forwardChunks;
+                backwardTriangleSlots += backwardChunks;
+            }
         }
-        p.triangleStart[l + 1] = static_cast<int64_t>(p.triangleItems.size());
+        p.partialSlots = std::max<int64_t>(p.partialSlots, std::max(forwardTriangleSlots, backwardTriangleSlots));
+        p.forwardTriangleStart[l + 1] = static_cast<int64_t>(p.forwardTriangles.size());
+        p.backwardTriangleStart[l + 1] = static_cast<int64_t>(p.backwardTriangles.size());
+        p.triangleSumStart[l + 1] = static_cast<int64_t>(p.forwardTriangleSums.size());
         std::vector<oosg_plan::InverseRound> rounds;
         // Steps K = 0 .. maxBlocks − 2 (items (s, I, J), I > K >= J), then the last launch (items I >= J, step −1).
         for (int K = 0; K <= maxBlocks - 1; ++K)
@@ -1171,7 +1193,10 @@ void uploadSweepPlan(oosg_plan& p)
     p.dBackwardBlocks = upload(p.backwardBlocks, st, "plan: backward column blocks");
     p.dForwardSubtractions = upload(p.forwardSubtractions, st, "plan: forward subtractions");
     p.dBackwardSubtractions = upload(p.backwardSubtractions, st, "plan: backward subtractions");
-    p.dTriangleItems = upload(p.triangleItems, st, "plan: triangle blocks");
+    p.dForwardTriangles = upload(p.forwardTriangles, st, "plan: triangle blocks");
+    p.dBackwardTriangles = upload(p.backwardTriangles, st, "plan: triangle blocks");
+    p.dForwardTriangleSums = upload(p.forwardTriangleSums, st, "plan: triangle blocks");
+    p.dBackwardTriangleSums = upload(p.backwardTriangleSums, st, "plan: triangle blocks");
     p.dInverseItems = upload(p.inverseItems, st, "plan: triangle inverse blocks");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
     p.dp.first = p.dFirst.as<int32_t>();
@@ -1913,8 +1938,8 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
     const int32_t* levels = p.dLevelList.as<int32_t>();
     const int4* forwardTiles = p.dForwardTiles.as<int4>();
     const int4* subtractions = p.dForwardSubtractions.as<int4>();
-    const int2* triangle = p.dTriangleItems.as<int2>();
-    double* W = p.solveW.as<double>();
+    const int4* triangles = p.dForwardTriangles.as<int4>();
+    const int4* triangleSums = p.dForwardTriangleSums.as<int4>();
     double* partials = p.solvePartials.as<double>();
     if (p.partialSlots > 0 && partials == nullptr)
         fail(OOSG_INTERNAL, "solve: the sweeps' partial sums have no buffer");
@@ -1930,22 +1955,23 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
     for (int32_t l = l0; l < l1; ++l)
     {
         const oosg_plan::SweepWork& w = p.sweepWork[l];
-        // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses, into W, then
-        // back into X), then every L_RF row tile of the level.
+        // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses in chunks of
+        // blocks, whose sums replace the rows of X once every product has read X), then every L_RF row tile.
         timed(PROFILE_FORWARD, w.triangleValues * T + w.forwardRows * row, [&] {
             check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
                                           p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, partials, st),
                   "solve: forward sweep");
         });
-        const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
-        if (triangles > 0)
+        const int items = static_cast<int>(p.forwardTriangleStart[l + 1] - p.forwardTriangleStart[l]);
+        const int sums = static_cast<int>(p.triangleSumStart[l + 1] - p.triangleSumStart[l]);
+        if (items > 0)
         {
             timed(PROFILE_TRIANGLE_PRODUCTS, w.wideValues * T + w.wideRows * row, [&] {
-                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles,
-                                                       0, st),
+                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, k, triangles + p.forwardTriangleStart[l], items,
+                                                       0, partials, st),
                       "solve: forward sweep (triangles)");
             });
-            check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+            check(oosg::launchSubtractPartials(X, nullptr, k, triangleSums + p.triangleSumStart[l], sums, partials, st),
                   "solve: forward sweep (triangles)");
         }
         const int tiles = static_cast<int>(p.forwardTileStart[l + 1] - p.forwardTileStart[l]);
@@ -1976,8 +2002,8 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     double* partials = p.solvePartials.as<double>();
     if (p.partialSlots > 0 && partials == nullptr)
         fail(OOSG_INTERNAL, "solve: the sweeps' partial sums have no buffer");
-    const int2* triangle = p.dTriangleItems.as<int2>();
-    double* W = p.solveW.as<double>();
+    const int4* triangles = p.dBackwardTriangles.as<int4>();
+    const int4* triangleSums = p.dBackwardTriangleSums.as<int4>();
     const double T = sizeof(TP), row = 8.0 * k;
     const auto timed = [&](int kind, double work, auto&& launch)
     {
@@ -2006,15 +2032,16 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
                                                    subtracted, partials, st),
                       "solve: backward sweep (columns)");
             });
-        const int triangles = static_cast<int>(p.triangleStart[l + 1] - p.triangleStart[l]);
-        if (triangles > 0)
+        const int items = static_cast<int>(p.backwardTriangleStart[l + 1] - p.backwardTriangleStart[l]);
+        const int sums = static_cast<int>(p.triangleSumStart[l + 1] - p.triangleSumStart[l]);
+        if (items > 0)
         {
             timed(PROFILE_TRIANGLE_PRODUCTS, w.wideValues * T + w.wideRows * row, [&] {
-                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, W, k, triangle + p.triangleStart[l], triangles,
-                                                       1, st),
+                check(oosg::launchTriangleProducts<TP>(p.dp, linv, X, k, triangles + p.backwardTriangleStart[l],
+                                                       items, 1, partials, st),
                       "solve: backward sweep (triangles)");
             });
-            check(oosg::launchCopyTriangleRows(p.dp, W, X, k, triangle + p.triangleStart[l], triangles, st),
+            check(oosg::launchSubtractPartials(X, nullptr, k, triangleSums + p.triangleSumStart[l], sums, partials, st),
                   "solve: backward sweep (triangles)");
         }
         timed(PROFILE_BACKWARD, w.triangleValues * T + w.backwardRows * row, [&] {
@@ -2068,7 +2095,7 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
         return;
     }
     p.sweepGraph.run(p.stream,
-                     {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr, p.solveW.ptr,
+                     {f.panels.ptr, f.linv.ptr, f.dDiag.ptr, p.solveX.ptr, p.solveU.ptr,
                       p.solvePartials.ptr, reinterpret_cast<const void*>(static_cast<intptr_t>(k * 2 + f.precision))},
                      body);
 }
@@ -2178,7 +2205,6 @@ void ensureSolveBuffers(oosg_plan& p)
 {
     const int64_t n = p.n;
     ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
-    ensure(p.solveW, static_cast<size_t>(n * 16) * sizeof(double), "solve: triangle products");
     ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
            "solve: update vectors");
     ensure(p.solvePartials, static_cast<size_t>(std::max<int64_t>(p.partialSlots, 1) * 64 * 16) * sizeof(double),
@@ -2427,7 +2453,9 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
                                        p->panelOff.size() * 24 +
                                        (p->forwardTiles.size() + p->backwardBlocks.size() + p->forwardSubtractions.size() +
                                         p->backwardSubtractions.size()) * 16 +
-                                       (p->triangleItems.size() + p->inverseItems.size()) * 8 +
+                                       (p->forwardTriangles.size() + p->backwardTriangles.size() +
+                                        2 * p->forwardTriangleSums.size()) * 16 +
+                                       p->inverseItems.size() * 8 +
                                        p->blocks.size() * sizeof(oosg::AssemblyBlock) +
                                        p->items.size() * sizeof(oosg::AssemblyItem) +
                                        p->tiles.size() * sizeof(oosg::UpdateTile);
