@@ -1303,40 +1303,37 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
         const int rc = p.nrow[ch];
         const int32_t* rel = p.rel + p.rowStart[ch];
         const double* uc = U + p.rowStart[ch] * k;
-        // Eight elements per thread per pass (distinct targets: a child's rows map to distinct rows of s): the child's
-        // values and positions are loaded first (from positions clamped into its rows), then the eight targets, then
-        // the eight sums are stored.
+        // Eight elements (i, h) of the child's rc × k values per thread per pass (distinct targets: a child's rows
+        // map to distinct rows of s): the child's values and positions are loaded first (from elements clamped into
+        // its values), then the eight targets, then the eight sums are stored.
         constexpr int UNROLL = 8;
-        for (int base = tid; base < rc * KMAX; base += UNROLL * nt)
+        const int total = rc * k;
+        for (int base = tid; base < total; base += UNROLL * nt)
         {
-            int pos[UNROLL];
+            int pos[UNROLL], col[UNROLL];
             double v[UNROLL];
 #pragma unroll
             for (int t = 0; t < UNROLL; ++t)
             {
-                const int idx = base + t * nt;
-                const int i = min(idx >> 4, rc - 1), h = min(idx & (KMAX - 1), k - 1);
+                const int idx = min(base + t * nt, total - 1);
+                const int i = idx / k;
+                col[t] = idx - i * k;
                 pos[t] = rel[i];
-                v[t] = uc[int64_t(i) * k + h];
+                v[t] = uc[idx];
             }
             double* y[UNROLL];
             double old[UNROLL];
 #pragma unroll
             for (int t = 0; t < UNROLL; ++t)
             {
-                const int idx = base + t * nt;
-                const int h = min(idx & (KMAX - 1), k - 1);
                 const int at = pinned(pos[t]);
-                y[t] = at < c ? X + int64_t(first + at) * k + h : u + int64_t(at - c) * k + h;
+                y[t] = at < c ? X + int64_t(first + at) * k + col[t] : u + int64_t(at - c) * k + col[t];
                 old[t] = *y[t];
             }
 #pragma unroll
             for (int t = 0; t < UNROLL; ++t)
-            {
-                const int idx = base + t * nt;
-                if (idx < rc * KMAX && (idx & (KMAX - 1)) < k)
+                if (base + t * nt < total)
                     *y[t] = pinned(old[t]) + pinned(v[t]);
-            }
         }
     }
     __syncthreads();
