@@ -1469,14 +1469,37 @@ __global__ __launch_bounds__(256) void subtractPartialsKernel(double* X, double*
     }
 }
 
-// X ← D⁻¹·X, X row-major n × k: a row per 16 lanes, lane h its column h.
+// Lanes of a row of a row-major n × k array (k <= 16) in the vector kernels: the smallest power of two >= k, so a
+// workgroup of 256 threads covers 256 >> vectorShift(k) rows a pass (a thread per row for one right-hand side).
+__host__ __device__ constexpr int vectorShift(int k)
+{
+    return k <= 1 ? 0 : k <= 2 ? 1 : k <= 4 ? 2 : k <= 8 ? 3 : 4;
+}
+
+// Thread threadIdx.x of a vector kernel: column h (idle when h >= k) of the rows slot, slot + rows, ... of its
+// workgroup's passes.
+struct VectorLanes
+{
+    int h, slot, rows;
+    __device__ explicit VectorLanes(int k)
+    {
+        const int shift = vectorShift(k);
+        h = threadIdx.x & ((1 << shift) - 1);
+        slot = threadIdx.x >> shift;
+        rows = 256 >> shift;
+    }
+    __device__ int64_t firstRow() const { return blockIdx.x * int64_t(rows) + slot; }
+    __device__ int64_t stride() const { return int64_t(gridDim.x) * rows; }
+};
+
+// X ← D⁻¹·X, X row-major n × k.
 __global__ __launch_bounds__(256) void diagonalSolveKernel(double* X, const double* d, int64_t n, int k)
 {
-    const int h = threadIdx.x & 15;
-    if (h >= k)
+    const VectorLanes v(k);
+    if (v.h >= k)
         return;
-    for (int64_t i = blockIdx.x * int64_t(16) + (threadIdx.x >> 4); i < n; i += int64_t(gridDim.x) * 16)
-        X[i * k + h] /= d[i];
+    for (int64_t i = v.firstRow(); i < n; i += v.stride())
+        X[i * k + v.h] /= d[i];
 }
 
 __device__ inline void gatherRows(double* xs, const double* X, const int32_t* rows, int first, int c, int i0, int ib,
@@ -1710,12 +1733,11 @@ __global__ void fillDoubleKernel(double* data, int64_t ld, int64_t n, int k, dou
         data[(idx % n) + (idx / n) * ld] = value;
 }
 
-// ---- Krylov vectors: n × k row-major (k <= 16), a row per group of 16 lanes ---------------------------------------
+// ---- Krylov vectors: n × k row-major (k <= 16), a row per 1, 2, 4, 8 or 16 lanes (VectorLanes) --------------------
 //
-// Thread t of a workgroup of 256 handles column h = t mod 16 (idle when h >= k) of the rows 16·g + t / 16 + ...,
-// g the workgroup: no integer division, and the k values of a row are one contiguous run of loads.
+// No integer division, the k values of a row are one contiguous run of loads, and no lane idles for k a power of two.
 
-constexpr int VECTOR_ROWS = 16; // rows of a workgroup's pass
+constexpr int VECTOR_ROWS = 16; // row groups of a workgroup's pass in the sparse product
 
 // out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC), by row
 // groups: runs of up to ROW_GROUP consecutive rows with the same columns (the degrees of freedom of one node). A
@@ -1723,11 +1745,59 @@ constexpr int VECTOR_ROWS = 16; // rows of a workgroup's pass
 // group, and each row's products are summed in its columns' order.
 constexpr int ROW_GROUP = 8;
 
+// multiplyRowsKernel for one right-hand side: the 16 lanes of a group share its entries (lane j the entries j, j + 16,
+// ...), and the lanes' sums of each row are added by a fixed butterfly over the 16 lanes.
+__device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t* cols, const double* values,
+                                          const int32_t* groups, int64_t groupCount, const double* x,
+                                          const double* b, double* out)
+{
+    const int lane = threadIdx.x & 15;
+    for (int64_t g = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); g < groupCount;
+         g += int64_t(gridDim.x) * VECTOR_ROWS)
+    {
+        const int32_t r0 = groups[g], rows = groups[g + 1] - r0;
+        const int64_t e0 = offsets[r0], degree = offsets[r0 + 1] - e0;
+        int64_t start[ROW_GROUP];
+        double sum[ROW_GROUP];
+#pragma unroll
+        for (int t = 0; t < ROW_GROUP; ++t)
+        {
+            start[t] = offsets[r0 + min(t, rows - 1)];
+            sum[t] = 0.0;
+        }
+        for (int64_t e = lane; e < degree; e += 16)
+        {
+            const double xv = x[cols[e0 + e]];
+#pragma unroll
+            for (int t = 0; t < ROW_GROUP; ++t)
+                if (t < rows)
+                    sum[t] += values[start[t] + e] * xv;
+        }
+        // Every lane of the group runs the butterfly (the group's lanes share g, so all of them are here).
+        double mine = 0.0;
+#pragma unroll
+        for (int t = 0; t < ROW_GROUP; ++t)
+        {
+#pragma unroll
+            for (int offset = 8; offset > 0; offset >>= 1)
+                sum[t] += __shfl_xor(sum[t], offset, 16);
+            mine = lane == t ? sum[t] : mine;
+        }
+        if (lane < rows)
+            out[r0 + lane] = b ? b[r0 + lane] - mine : mine;
+    }
+}
+
 __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets, const int32_t* cols,
                                                          const double* values, const int32_t* groups,
                                                          int64_t groupCount, const double* x, const double* b,
                                                          double* out, int k)
 {
+    if (k == 1)
+    {
+        multiplyRowsSingle(offsets, cols, values, groups, groupCount, x, b, out);
+        return;
+    }
     const int h = threadIdx.x & 15;
     if (h >= k)
         return;
@@ -1791,21 +1861,22 @@ __global__ __launch_bounds__(256) void columnDotsKernel(const double* a, const d
                                                        double* partial)
 {
     __shared__ double sums[256];
-    const int h = threadIdx.x & 15;
+    const VectorLanes v(k);
     double sum = 0.0;
-    if (h < k)
-        for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-             i += int64_t(gridDim.x) * VECTOR_ROWS)
-            sum += a[i * k + h] * b[i * k + h];
+    if (v.h < k)
+        for (int64_t i = v.firstRow(); i < n; i += v.stride())
+            sum += a[i * k + v.h] * b[i * k + v.h];
     sums[threadIdx.x] = sum;
+    // The row slots' sums by a fixed tree (slot s adds slot s + half), column h ending in sums[h].
+    for (int half = v.rows >> 1; half > 0; half >>= 1)
+    {
+        __syncthreads();
+        if (v.slot < half)
+            sums[threadIdx.x] += sums[threadIdx.x + (half << vectorShift(k))];
+    }
     __syncthreads();
     if (threadIdx.x < 16)
-    {
-        double total = 0.0;
-        for (int r = 0; r < 16; ++r)
-            total += sums[r * 16 + threadIdx.x];
-        partial[blockIdx.x * 16 + threadIdx.x] = total;
-    }
+        partial[blockIdx.x * 16 + threadIdx.x] = threadIdx.x < (256 / v.rows) ? sums[threadIdx.x] : 0.0;
 }
 
 // out[h] = Σ over the workgroups' partial sums: 16 threads per column sum runs of `groups / 16` workgroups, then the
@@ -1834,38 +1905,41 @@ __global__ __launch_bounds__(256) void sumPartialsKernel(const double* partial, 
 __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int64_t n, int k, unsigned long long* out)
 {
     __shared__ double slots[256];
-    const int h = threadIdx.x & 15;
+    const VectorLanes lanes(k);
     double m = 0.0;
-    if (h < k)
-        for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-             i += int64_t(gridDim.x) * VECTOR_ROWS)
+    if (lanes.h < k)
+        for (int64_t i = lanes.firstRow(); i < n; i += lanes.stride())
         {
-            const double a = fabs(v[i * k + h]);
+            const double a = fabs(v[i * k + lanes.h]);
             m = a > m || a != a ? a : m;
         }
     slots[threadIdx.x] = m;
+    // The row slots' maxima by a fixed tree, column h ending in slots[h].
+    for (int half = lanes.rows >> 1; half > 0; half >>= 1)
+    {
+        __syncthreads();
+        if (lanes.slot < half)
+        {
+            const double a = slots[threadIdx.x + (half << vectorShift(k))];
+            const double b = slots[threadIdx.x];
+            slots[threadIdx.x] = a > b || a != a ? a : b;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < k)
-    {
-        for (int r = 1; r < 16; ++r)
-        {
-            const double a = slots[r * 16 + threadIdx.x];
-            m = a > m || a != a ? a : m;
-        }
-        atomicMax(&out[threadIdx.x], static_cast<unsigned long long>(__double_as_longlong(m)));
-    }
+        atomicMax(&out[threadIdx.x], static_cast<unsigned long long>(__double_as_longlong(slots[threadIdx.x])));
 }
 
 // x += α_h·p, r −= α_h·q.
 __global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const double* p, const double* q,
                                                    ColumnCoefficients alpha, int64_t n, int k)
 {
-    const int h = threadIdx.x & 15;
+    const VectorLanes v(k);
+    const int h = v.h;
     if (h >= k)
         return;
     const double a = alpha.v[h];
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    for (int64_t i = v.firstRow(); i < n; i += v.stride())
     {
         x[i * k + h] += a * p[i * k + h];
         r[i * k + h] -= a * q[i * k + h];
@@ -1875,11 +1949,11 @@ __global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const 
 // x += z (row-major n × k).
 __global__ __launch_bounds__(256) void addRowsKernel(double* x, const double* z, int64_t n, int k)
 {
-    const int h = threadIdx.x & 15;
+    const VectorLanes v(k);
+    const int h = v.h;
     if (h >= k)
         return;
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    for (int64_t i = v.firstRow(); i < n; i += v.stride())
         x[i * k + h] += z[i * k + h];
 }
 
@@ -1887,12 +1961,12 @@ __global__ __launch_bounds__(256) void addRowsKernel(double* x, const double* z,
 __global__ __launch_bounds__(256) void cgDirectionKernel(double* p, const double* z, ColumnCoefficients beta,
                                                         int64_t n, int k)
 {
-    const int h = threadIdx.x & 15;
+    const VectorLanes v(k);
+    const int h = v.h;
     if (h >= k)
         return;
     const double b = beta.v[h];
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    for (int64_t i = v.firstRow(); i < n; i += v.stride())
         p[i * k + h] = z[i * k + h] + b * p[i * k + h];
 }
 
@@ -1900,11 +1974,11 @@ __global__ __launch_bounds__(256) void cgDirectionKernel(double* p, const double
 __global__ __launch_bounds__(256) void scaleRowsKernel(const double* src, double* dst, const double* scale, int64_t n,
                                                       int k)
 {
-    const int h = threadIdx.x & 15;
+    const VectorLanes v(k);
+    const int h = v.h;
     if (h >= k)
         return;
-    for (int64_t i = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); i < n;
-         i += int64_t(gridDim.x) * VECTOR_ROWS)
+    for (int64_t i = v.firstRow(); i < n; i += v.stride())
         dst[i * k + h] = scale ? src[i * k + h] * scale[i] : src[i * k + h];
 }
 
@@ -1916,6 +1990,14 @@ int blocksFor(int64_t work, int threads)
 {
     const int64_t b = (work + threads - 1) / threads;
     return int(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+// Workgroups of a vector kernel over n rows of k values (over n row groups of the sparse product: k = 16).
+int vectorGroups(int64_t n, int k)
+{
+    const int64_t rows = 256 >> vectorShift(k);
+    const int64_t g = (n + rows - 1) / rows;
+    return int(g < 1 ? 1 : (g > 4096 ? 4096 : g));
 }
 } // namespace
 
@@ -2057,8 +2139,7 @@ hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const dou
 
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(diagonalSolveKernel, dim3(std::max<int64_t>(1, std::min<int64_t>((n + 15) / 16, 8192))),
-                       dim3(256), 0, stream, X, d, n, k);
+    hipLaunchKernelGGL(diagonalSolveKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, X, d, n, k);
     return hipGetLastError();
 }
 
@@ -2118,17 +2199,11 @@ hipError_t configureKernels()
     return hipSuccess;
 }
 
-static int vectorGroups(int64_t n)
-{
-    const int64_t g = (n + VECTOR_ROWS - 1) / VECTOR_ROWS;
-    return int(g < 1 ? 1 : (g > 4096 ? 4096 : g));
-}
-
 hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values,
                               const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
                               int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(groupCount)), dim3(256), 0, stream, offsets, cols, values,
+    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(groupCount, 16)), dim3(256), 0, stream, offsets, cols, values,
                        groups, groupCount, x, b, out, k);
     return hipGetLastError();
 }
@@ -2144,7 +2219,7 @@ hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, 
 
 hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream)
 {
-    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(std::min(vectorGroups(n), 1024)), dim3(256), 0, stream, v, n, k,
+    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(std::min(vectorGroups(n, k), 1024)), dim3(256), 0, stream, v, n, k,
                        out);
     return hipGetLastError();
 }
@@ -2152,26 +2227,26 @@ hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long 
 hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, const ColumnCoefficients& alpha,
                         int64_t n, int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(cgStepKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, x, r, p, q, alpha, n, k);
+    hipLaunchKernelGGL(cgStepKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, x, r, p, q, alpha, n, k);
     return hipGetLastError();
 }
 
 hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(addRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, x, z, n, k);
+    hipLaunchKernelGGL(addRowsKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, x, z, n, k);
     return hipGetLastError();
 }
 
 hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
                              hipStream_t stream)
 {
-    hipLaunchKernelGGL(cgDirectionKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, p, z, beta, n, k);
+    hipLaunchKernelGGL(cgDirectionKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, p, z, beta, n, k);
     return hipGetLastError();
 }
 
 hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(scaleRowsKernel, dim3(vectorGroups(n)), dim3(256), 0, stream, src, dst, scale, n, k);
+    hipLaunchKernelGGL(scaleRowsKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, src, dst, scale, n, k);
     return hipGetLastError();
 }
 
