@@ -1003,7 +1003,6 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
 {
     __shared__ T As[32][TILE + 4];
     __shared__ T Bs[32][TILE + 4];
-    __shared__ T Ps[TILE][TILE + 4]; // the first product, as the second one's B operand: Ps[t][j]
     const int2 item = items[blockIdx.x];
     const int s = item.x, I = item.y >> 16, J = item.y & 0xFFFF;
     const int c = p.ncol[s], n = c + p.nrow[s];
@@ -1015,7 +1014,8 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
     typename Mfma<T>::Acc acc[2][2];
     if (step < 0)
     {
-        // The last launch: X_II from the diagonal block's inverse, or X_IJ = L_II⁻¹·T_IJ.
+        // A row block's final values: X_II from the diagonal block's inverse, or X_IJ = L_II⁻¹·T_IJ (T_IJ complete:
+        // before step I, or in the last launch).
         const T* inv = linv + p.linvOff[s] + int64_t(i0) * TILE; // ib × ib, leading dimension ib
         if (I == J)
         {
@@ -1036,19 +1036,8 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
         const T* L = panels + p.panelOff[s];
         if (J < K)
         {
-            // P = L_KK⁻¹·T_KJ into LDS.
-            zeroTile<T>(acc);
-            tileProduct<T>(acc, As, Bs, TILE, TILE, [&](int i, int t) { return invK[i + t * TILE]; },
-                           [&](int t, int j) { return X[(k0 + t) + int64_t(j0 + j) * c]; });
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        Ps[tileRow(x, v, lane, wave, fp64)][wn + y * 16 + (lane & 15)] = acc[x][y][v];
-            __syncthreads(); // P complete before any wave reads it
-            // T_IJ −= L_IK·P: the accumulators start from T_IJ, L enters negated.
+            // T_IJ −= L_IK·X_KJ (row block K is final before step K): the accumulators start from T_IJ, L enters
+            // negated.
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1058,7 +1047,7 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
                         acc[x][y][v] = X[(i0 + min(tileRow(x, v, lane, wave, fp64), ib - 1)) +
                                          int64_t(j0 + wn + y * 16 + (lane & 15)) * c];
             tileProduct<T>(acc, As, Bs, ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(k0 + t) * n]; },
-                           [&](int t, int j) { return Ps[t][j]; });
+                           [&](int t, int j) { return X[(k0 + t) + int64_t(j0 + j) * c]; });
         }
         else
         {

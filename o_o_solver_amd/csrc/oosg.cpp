@@ -347,6 +347,10 @@ struct oosg_plan
         int64_t offset;
         int count;
         double flops;
+        // Before step K: row block K's final values (items (s, K, J < K), launched as step −1).
+        int64_t rowOffset = 0;
+        int rowCount = 0;
+        double rowFlops = 0.0;
     };
     std::vector<int2> inverseItems;
     std::vector<std::vector<InverseRound>> inverseRounds;
@@ -778,11 +782,25 @@ void deriveLevels(oosg_plan& p)
         p.backwardTriangleStart[l + 1] = static_cast<int64_t>(p.backwardTriangles.size());
         p.triangleSumStart[l + 1] = static_cast<int64_t>(p.forwardTriangleSums.size());
         std::vector<oosg_plan::InverseRound> rounds;
-        // Steps K = 0 .. maxBlocks − 2 (items (s, I, J), I > K >= J), then the last launch (items I >= J, step −1).
+        // Steps K = 0 .. maxBlocks − 2 (items (s, I, J), I > K >= J), each after row block K's final values (items
+        // (s, K, J < K), for 1 <= K <= blocks − 2), then the last launch (the diagonal blocks and the last row block,
+        // step −1).
+        constexpr double ITEM_FLOPS = 2.0 * 64.0 * 64.0 * 64.0;
         for (int K = 0; K <= maxBlocks - 1; ++K)
         {
             const bool last = K == maxBlocks - 1;
-            oosg_plan::InverseRound round{last ? -1 : K, static_cast<int64_t>(p.inverseItems.size()), 0, 0.0};
+            oosg_plan::InverseRound round{last ? -1 : K, 0, 0, 0.0};
+            round.rowOffset = static_cast<int64_t>(p.inverseItems.size());
+            for (int32_t q = p.levelStart[l]; !last && K >= 1 && q < p.levelStart[l + 1]; ++q)
+            {
+                const int32_t s = p.levelList[q];
+                if (oosg::keepsTriangleInverse(p.ncol[s]) && K <= (p.ncol[s] + 63) / 64 - 2)
+                    for (int J = 0; J < K; ++J)
+                        p.inverseItems.push_back(make_int2(s, (K << 16) | J));
+            }
+            round.rowCount = static_cast<int>(p.inverseItems.size() - round.rowOffset);
+            round.rowFlops = round.rowCount * ITEM_FLOPS;
+            round.offset = static_cast<int64_t>(p.inverseItems.size());
             for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
             {
                 const int32_t s = p.levelList[q];
@@ -791,10 +809,10 @@ void deriveLevels(oosg_plan& p)
                     continue;
                 for (int I = 0; I < blocks; ++I)
                     for (int J = 0; J <= I; ++J)
-                        if (last ? true : (I > K && J <= K))
+                        if (last ? (I == J || I == blocks - 1) : (I > K && J <= K))
                         {
                             p.inverseItems.push_back(make_int2(s, (I << 16) | J));
-                            round.flops += 2.0 * 64.0 * 64.0 * 64.0 * (last || J == K ? 1.0 : 2.0);
+                            round.flops += ITEM_FLOPS;
                         }
             }
             round.count = static_cast<int>(p.inverseItems.size() - round.offset);
@@ -823,6 +841,19 @@ Launch syncLaunch(int lane, int wait, int record)
 }
 
 // A launch of one step of a level's whole inverses, on the inverses' stream after the event `wait`.
+// The launch of a step's row block finals (before the step's own launch).
+Launch inverseRowsLaunch(const oosg_plan::InverseRound& round, int wait)
+{
+    Launch rows{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
+    rows.offset = round.rowOffset;
+    rows.count = round.rowCount;
+    rows.k0 = -1;
+    rows.flops = round.rowFlops;
+    rows.lane = LANE_INVERSES;
+    rows.wait = wait;
+    return rows;
+}
+
 Launch inverseLaunch(const oosg_plan::InverseRound& round, int wait)
 {
     Launch inverse{LaunchKind::Inverse, PROFILE_TRIANGLE_INVERSES};
@@ -840,6 +871,8 @@ void buildSchedule(oosg_plan& p)
     using oosg::PANEL_BLOCK;
     // Columns of a super-block: the trailing update beyond it is one product with K = SUPER_BLOCK.
     constexpr int SUPER_BLOCK = 4 * PANEL_BLOCK;
+    // Panel steps between two records of EVENT_SOLVED (when the bulk update needs none sooner).
+    constexpr int SOLVED_RECORD_STEPS = 4;
     // Every event is recorded once on the critical stream first, so that each wait refers to a record of the same
     // sequence (a captured graph has no outside dependencies).
     p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_SOLVED));
@@ -958,6 +991,21 @@ void buildSchedule(oosg_plan& p)
         assemble.count = static_cast<int>(p.blocks.size() - blockStart);
         p.schedule.push_back(assemble);
         // Panel steps.
+        // A record of EVENT_SOLVED on the critical stream delays its next launch by several microseconds, so a step
+        // records only when a step of the whole inverses (which bound the top levels) or the bulk update needs it,
+        // every SOLVED_RECORD_STEPS steps, and at the level's last step. pendingInverses: the inverse steps whose
+        // panel solve is done but not yet recorded.
+        std::vector<size_t> pendingInverses;
+        const auto launchPendingInverses = [&]
+        {
+            for (const size_t K : pendingInverses)
+            {
+                if (p.inverseRounds[l][K].rowCount > 0)
+                    p.schedule.push_back(inverseRowsLaunch(p.inverseRounds[l][K], EVENT_SOLVED));
+                p.schedule.push_back(inverseLaunch(p.inverseRounds[l][K], EVENT_SOLVED));
+            }
+            pendingInverses.clear();
+        };
         for (int k0 = 0; k0 < maxC; k0 += PANEL_BLOCK)
         {
             Launch diagonal{LaunchKind::Diagonal, PROFILE_DIAGONAL};
@@ -1053,7 +1101,13 @@ void buildSchedule(oosg_plan& p)
             }
             diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
             diagonal.lane = solve.lane = inner.lane = next.lane = LANE_CRITICAL;
-            (solve.count > 0 ? solve : diagonal).record = EVENT_SOLVED;
+            const size_t stepK = static_cast<size_t>(k0 / PANEL_BLOCK);
+            const bool inverseStep = stepK + 1 < p.inverseRounds[l].size() && p.inverseRounds[l][stepK].count > 0;
+            const bool records = rest.count > 0 || inverseStep || k0 + PANEL_BLOCK >= maxC ||
+                                 (k0 / PANEL_BLOCK) % SOLVED_RECORD_STEPS == SOLVED_RECORD_STEPS - 1;
+            Launch& solvedAt = solve.count > 0 ? solve : diagonal;
+            if (records)
+                solvedAt.record = EVENT_SOLVED;
             // From the second step on, the previous step's update factorised this step's diagonal blocks.
             if (k0 == 0)
                 p.schedule.push_back(diagonal);
@@ -1081,8 +1135,16 @@ void buildSchedule(oosg_plan& p)
             {
                 if (solve.count == 0)
                     throw std::logic_error("plan: an inverse step without a panel solve");
-                p.schedule.push_back(inverseLaunch(rounds[K], EVENT_SOLVED));
+                pendingInverses.push_back(K);
             }
+            // A recorded step's solve covers the earlier steps' (one stream, in order).
+            if (records && (solve.count > 0 || k0 == 0))
+                launchPendingInverses();
+        }
+        if (!pendingInverses.empty())
+        {
+            p.schedule.push_back(syncLaunch(LANE_CRITICAL, -1, EVENT_SOLVED));
+            launchPendingInverses();
         }
         // The level's panels are final once the bulk stream is done with them.
         p.schedule.push_back(syncLaunch(LANE_CRITICAL, EVENT_RESTED, EVENT_PANELS));
@@ -1912,10 +1974,16 @@ void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
           "factor: inverses");
     for (int32_t l = 0; l < d->levels; ++l)
         for (const oosg_plan::InverseRound& round : d->inverseRounds[l])
+        {
+            check(oosg::launchTriangleInverse<double>(d->dp, f.panels.as<double>(), f.linv.as<double>(),
+                                                      d->dInverseItems.as<int2>() + round.rowOffset, round.rowCount,
+                                                      -1, st),
+                  "factor: triangle inverses");
             check(oosg::launchTriangleInverse<double>(d->dp, f.panels.as<double>(), f.linv.as<double>(),
                                                       d->dInverseItems.as<int2>() + round.offset, round.count,
                                                       round.step, st),
                   "factor: triangle inverses");
+        }
     f.dDiag = upload(pf->diag, st, "factor: pivots");
     pf->sub = upload(pf->sub_, st, "factor: pivots");
     pf->perm = upload(permOf, st, "factor: pivot order");
