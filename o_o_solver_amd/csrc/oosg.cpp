@@ -992,9 +992,9 @@ void buildSchedule(oosg_plan& p)
         p.schedule.push_back(assemble);
         // Panel steps.
         // A record of EVENT_SOLVED on the critical stream delays its next launch by several microseconds, so a step
-        // records only when a step of the whole inverses (which bound the top levels) or the bulk update needs it,
-        // every SOLVED_RECORD_STEPS steps, and at the level's last step. pendingInverses: the inverse steps whose
-        // panel solve is done but not yet recorded.
+        // records only when the bulk update needs it, every SOLVED_RECORD_STEPS steps, and at the level's last step.
+        // pendingInverses: the whole inverses' steps whose panel solve is done but not yet recorded (they wait for
+        // the next record).
         std::vector<size_t> pendingInverses;
         const auto launchPendingInverses = [&]
         {
@@ -1101,9 +1101,7 @@ void buildSchedule(oosg_plan& p)
             }
             diagonal.count = static_cast<int>(p.stepFronts.size() - diagonal.offset);
             diagonal.lane = solve.lane = inner.lane = next.lane = LANE_CRITICAL;
-            const size_t stepK = static_cast<size_t>(k0 / PANEL_BLOCK);
-            const bool inverseStep = stepK + 1 < p.inverseRounds[l].size() && p.inverseRounds[l][stepK].count > 0;
-            const bool records = rest.count > 0 || inverseStep || k0 + PANEL_BLOCK >= maxC ||
+            const bool records = rest.count > 0 || k0 + PANEL_BLOCK >= maxC ||
                                  (k0 / PANEL_BLOCK) % SOLVED_RECORD_STEPS == SOLVED_RECORD_STEPS - 1;
             Launch& solvedAt = solve.count > 0 ? solve : diagonal;
             if (records)
