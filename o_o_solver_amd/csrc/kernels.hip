@@ -1581,12 +1581,25 @@ __global__ __launch_bounds__(256) void backwardKernel(DevicePlan p, const TP* pa
         double4_t acc = {0, 0, 0, 0};
         // A larger supernode's rows of L_RF are the column blocks of backwardColumnsKernel.
         const int rowsEnd = sweepRowsInline(c, r) ? n : c;
-        for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
+        // Tile i0 + SB's rows of X and block of L are loaded into registers while tile i0 is multiplied (the first
+        // with the inverse and the block's own rows).
+        StagedRows nextRows;
+        StagedBlock<TP> nextBlock;
+        const auto loadTile = [&](int i0)
         {
             const int ib = min(SB, rowsEnd - i0);
-            gatherRows(xs, X, rows, first, c, i0, ib, k);
-            stageBlock(blk, L, n, i0, ib, q0, qb);
+            nextRows.gather(X, rows, first, c, i0, ib, k);
+            nextBlock.load(L, n, i0, ib, q0, qb);
+        };
+        if (q0 + qb < rowsEnd)
+            loadTile(q0 + qb);
+        for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
+        {
+            nextRows.store(xs);
+            nextBlock.store(blk);
             __syncthreads();
+            if (i0 + SB < rowsEnd)
+                loadTile(i0 + SB);
             acc = columnsTimesVectors(blk, xs, acc);
             __syncthreads();
         }
