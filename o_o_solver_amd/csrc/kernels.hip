@@ -1747,6 +1747,15 @@ constexpr int VECTOR_ROWS = 16; // row groups of a workgroup's pass in the spars
 // group, and each row's products are summed in its columns' order.
 constexpr int ROW_GROUP = 8;
 
+// The sparse product's workgroup index, XCD by XCD: the dispatcher hands workgroup b to XCD b mod 8, so XCD x takes
+// the contiguous range [x·G/8, (x + 1)·G/8) of row groups (G, the grid, a multiple of 8), and rows that share columns
+// read their x rows through the same L2.
+__device__ inline int64_t xcdOrderedBlock()
+{
+    const int perXcd = gridDim.x >> 3;
+    return int64_t(blockIdx.x & 7) * perXcd + (blockIdx.x >> 3);
+}
+
 // multiplyRowsKernel for one right-hand side: the 16 lanes of a group share its entries (lane j the entries j, j + 16,
 // ...), and the lanes' sums of each row are added by a fixed butterfly over the 16 lanes.
 __device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t* cols, const double* values,
@@ -1754,7 +1763,7 @@ __device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t*
                                           const double* b, double* out)
 {
     const int lane = threadIdx.x & 15;
-    for (int64_t g = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); g < groupCount;
+    for (int64_t g = xcdOrderedBlock() * VECTOR_ROWS + (threadIdx.x >> 4); g < groupCount;
          g += int64_t(gridDim.x) * VECTOR_ROWS)
     {
         const int32_t r0 = groups[g], rows = groups[g + 1] - r0;
@@ -1803,7 +1812,7 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
     const int h = threadIdx.x & 15;
     if (h >= k)
         return;
-    for (int64_t g = blockIdx.x * int64_t(VECTOR_ROWS) + (threadIdx.x >> 4); g < groupCount;
+    for (int64_t g = xcdOrderedBlock() * VECTOR_ROWS + (threadIdx.x >> 4); g < groupCount;
          g += int64_t(gridDim.x) * VECTOR_ROWS)
     {
         const int32_t r0 = groups[g], rows = groups[g + 1] - r0;
@@ -2205,8 +2214,11 @@ hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const
                               const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
                               int k, hipStream_t stream)
 {
-    hipLaunchKernelGGL(multiplyRowsKernel, dim3(vectorGroups(groupCount, 16)), dim3(256), 0, stream, offsets, cols, values,
-                       groups, groupCount, x, b, out, k);
+    // A workgroup per VECTOR_ROWS row groups (one pass), the grid a multiple of the 8 XCDs.
+    const int64_t blocks = std::min<int64_t>((groupCount + VECTOR_ROWS - 1) / VECTOR_ROWS, int64_t(1) << 24);
+    const int grid = static_cast<int>(std::max<int64_t>(8, (blocks + 7) / 8 * 8));
+    hipLaunchKernelGGL(multiplyRowsKernel, dim3(grid), dim3(256), 0, stream, offsets, cols, values, groups,
+                       groupCount, x, b, out, k);
     return hipGetLastError();
 }
 
