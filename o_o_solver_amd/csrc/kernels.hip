@@ -1355,14 +1355,13 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
             __syncthreads();
         }
         storeBlockRows(ys, X, first + q0, qb, k);
-        // The rows below the block, 64 at a time.
+        // The rows below the block, 64 at a time, the next 64 loaded into registers while these are multiplied.
         for (int i0 = q0 + qb; i0 < rowsEnd; i0 += SB)
         {
-            if (i0 == q0 + qb)
-                below.store(blk);
-            else
-                stageBlock(blk, L, n, i0, min(SB, rowsEnd - i0), q0, qb);
+            below.store(blk);
             __syncthreads();
+            if (i0 + SB < rowsEnd)
+                below.load(L, n, i0 + SB, min(SB, rowsEnd - i0 - SB), q0, qb);
             const double4_t acc = rowsTimesVectors(blk, ys, double4_t{0, 0, 0, 0});
             const int h = accColumn();
 #pragma unroll
