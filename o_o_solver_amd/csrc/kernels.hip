@@ -1808,9 +1808,8 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
         multiplyRowsSingle(offsets, cols, values, groups, groupCount, x, b, out);
         return;
     }
-    const int h = threadIdx.x & 15;
-    if (h >= k)
-        return;
+    // Lanes h >= k load and broadcast like the others (their x column clamped) and store nothing.
+    const int lane = threadIdx.x & 15, h = min(lane, k - 1);
     for (int64_t g = xcdOrderedBlock() * VECTOR_ROWS + (threadIdx.x >> 4); g < groupCount;
          g += int64_t(gridDim.x) * VECTOR_ROWS)
     {
@@ -1824,38 +1823,41 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
             start[t] = offsets[r0 + min(t, rows - 1)];
             sum[t] = 0.0;
         }
-        // Four entries at a time: their columns, x rows and values all loaded before the first product.
-        constexpr int BATCH = 4;
-        int64_t e = 0;
-        for (; e + BATCH <= degree; e += BATCH)
+        // Runs of 16 entries: lane j loads entry j's column and the group's values in it (coalesced, positions
+        // clamped into the run); each entry's column and values are then broadcast from its lane, in the entries'
+        // order, so every row's products are summed in its columns' order.
+        for (int64_t e = 0; e < degree; e += 16)
         {
-            int c[BATCH];
-            double xv[BATCH], v[BATCH][ROW_GROUP];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u)
-                c[u] = cols[e0 + e + u];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u)
-            {
-                xv[u] = x[int64_t(c[u]) * k + h];
-#pragma unroll
-                for (int t = 0; t < ROW_GROUP; ++t)
-                    v[u][t] = t < rows ? values[start[t] + e + u] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u)
-#pragma unroll
-                for (int t = 0; t < ROW_GROUP; ++t)
-                    sum[t] += v[u][t] * xv[u];
-        }
-        for (; e < degree; ++e)
-        {
-            const double xv = x[int64_t(cols[e0 + e]) * k + h];
+            const int run = int(min(int64_t(16), degree - e));
+            const int j = min(lane, run - 1);
+            const int cj = cols[e0 + e + j];
+            double vj[ROW_GROUP];
 #pragma unroll
             for (int t = 0; t < ROW_GROUP; ++t)
-                if (t < rows)
-                    sum[t] += values[start[t] + e] * xv;
+                vj[t] = t < rows ? values[start[t] + e + j] : 0.0;
+            if (run == 16)
+            {
+                double xv[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    xv[u] = x[int64_t(__shfl(cj, u, 16)) * k + h];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int t = 0; t < ROW_GROUP; ++t)
+                        sum[t] += __shfl(vj[t], u, 16) * xv[u];
+            }
+            else
+                for (int u = 0; u < run; ++u)
+                {
+                    const double xu = x[int64_t(__shfl(cj, u, 16)) * k + h];
+#pragma unroll
+                    for (int t = 0; t < ROW_GROUP; ++t)
+                        sum[t] += __shfl(vj[t], u, 16) * xu;
+                }
         }
+        if (lane >= k)
+            continue;
 #pragma unroll
         for (int t = 0; t < ROW_GROUP; ++t)
             if (t < rows)
