@@ -1031,13 +1031,13 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
     }
     else
     {
-        const int K = step, k0 = TILE * K;
-        const T* invK = linv + p.linvOff[s] + int64_t(k0) * TILE; // 64 × 64 (block K is full)
+        // The terms of steps K0 .. K, in that order (K0 = K − 1 for a paired step, flag bit 16: step K − 1 did only
+        // the items of row block K): T_IJ −= L_I,Kt·X_Kt,J for J < Kt (row block Kt is final before step Kt), or
+        // T_IKt = −L_I,Kt·L_KtKt⁻¹ for J = Kt. The accumulators start from T_IJ (J < K0) or zero, L enters negated.
+        const int K = step & 0xFFFF, K0 = ((step >> 16) & 1) ? K - 1 : K;
         const T* L = panels + p.panelOff[s];
-        if (J < K)
+        if (J < K0)
         {
-            // T_IJ −= L_IK·X_KJ (row block K is final before step K): the accumulators start from T_IJ, L enters
-            // negated.
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1046,15 +1046,25 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
                     for (int v = 0; v < 4; ++v)
                         acc[x][y][v] = X[(i0 + min(tileRow(x, v, lane, wave, fp64), ib - 1)) +
                                          int64_t(j0 + wn + y * 16 + (lane & 15)) * c];
-            tileProduct<T>(acc, As, Bs, ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(k0 + t) * n]; },
-                           [&](int t, int j) { return X[(k0 + t) + int64_t(j0 + j) * c]; });
         }
         else
-        {
-            // T_IK = −L_IK·L_KK⁻¹.
             zeroTile<T>(acc);
-            tileProduct<T>(acc, As, Bs, ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(k0 + t) * n]; },
-                           [&](int t, int j) { return invK[t + j * TILE]; });
+        for (int Kt = K0; Kt <= K; ++Kt)
+        {
+            if (J > Kt)
+                continue; // no term of step Kt (the same for the whole workgroup)
+            if (Kt > K0)
+                __syncthreads(); // the previous product has read its staged operands
+            const int kt0 = TILE * Kt;
+            const T* invKt = linv + p.linvOff[s] + int64_t(kt0) * TILE; // 64 × 64 (block Kt is full)
+            if (J < Kt)
+                tileProduct<T>(acc, As, Bs, ib, TILE,
+                               [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
+                               [&](int t, int j) { return X[(kt0 + t) + int64_t(j0 + j) * c]; });
+            else
+                tileProduct<T>(acc, As, Bs, ib, TILE,
+                               [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
+                               [&](int t, int j) { return invKt[t + j * TILE]; });
         }
     }
 #pragma unroll

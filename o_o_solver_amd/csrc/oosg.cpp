@@ -789,7 +789,7 @@ void deriveLevels(oosg_plan& p)
         for (int K = 0; K <= maxBlocks - 1; ++K)
         {
             const bool last = K == maxBlocks - 1;
-            oosg_plan::InverseRound round{last ? -1 : K, 0, 0, 0.0};
+            oosg_plan::InverseRound round{last ? -1 : (K % 2 == 1 ? K | (1 << 16) : K), 0, 0, 0.0};
             round.rowOffset = static_cast<int64_t>(p.inverseItems.size());
             for (int32_t q = p.levelStart[l]; !last && K >= 1 && q < p.levelStart[l + 1]; ++q)
             {
@@ -807,12 +807,14 @@ void deriveLevels(oosg_plan& p)
                 const int blocks = (p.ncol[s] + 63) / 64;
                 if (!oosg::keepsTriangleInverse(p.ncol[s]))
                     continue;
+                // Steps in pairs: an even step K does the items of row block K + 1 only, the odd step K + 1 every
+                // item below with the terms of both steps (each T_IJ read and written once for two steps).
                 for (int I = 0; I < blocks; ++I)
                     for (int J = 0; J <= I; ++J)
-                        if (last ? (I == J || I == blocks - 1) : (I > K && J <= K))
+                        if (last ? (I == J || I == blocks - 1) : (K % 2 == 0 ? I == K + 1 && J <= K : I > K && J <= K))
                         {
                             p.inverseItems.push_back(make_int2(s, (I << 16) | J));
-                            round.flops += ITEM_FLOPS;
+                            round.flops += ITEM_FLOPS * (!last && K % 2 == 1 && J < K ? 2.0 : 1.0);
                         }
             }
             round.count = static_cast<int>(p.inverseItems.size() - round.offset);
