@@ -87,7 +87,8 @@ def cpu_baseline(workload, b, threads, keep_solution=False):
             "gflops_factor": round(flops / times[0] / 1e9, 1), "cpu": model,
             "hardware_concurrency": int(_oracle.reference().ref_hardware_threads()),
             "sample": "the whole workload: one FP64 factorisation (SupernodalFactor::factorise, avx2 family, "
-                      f"{threads} threads) and one solve of the {b.shape[1]} right-hand sides (FactorSolve::solve)"}
+                      f"{threads} threads) and one solve of the {b.shape[1]} right-hand sides (FactorSolve::solve)"
+                      + ("; value is the median of 3 runs" if keep_solution else "")}
 
 
 def cylinder_main(args):
@@ -272,7 +273,8 @@ def main():
 
     import o_o_solver_amd as oos
     from o_o_solver_amd import profiling
-    from o_o_solver_amd import FactorOptions, FactorSolve, NotConvergedError, Precision, SupernodalFactor, models
+    from o_o_solver_amd import (FactorOptions, FactorSolve, NotConvergedError, Precision, PrecisionInsufficientError,
+                                SupernodalFactor, models)
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU", file=sys.stderr)
@@ -309,7 +311,10 @@ def main():
     choice = {"probability": None, "precision": "fp32"}
 
     # A step is one call of the combined factorisation and refined solve (the first solve's forward sweep of the
-    # lower levels runs beside the factorisation of the top ones; the same results as the two calls).
+    # lower levels runs beside the factorisation of the top ones; the same results as the two calls). The last
+    # solution of each path is kept for its error against the reference's.
+    last_x = {}
+
     def fp32_step():
         try:
             f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd,
@@ -317,12 +322,14 @@ def main():
                                                              device_matrix=dm, target=args.target,
                                                              max_steps=args.max_steps)
             fallback = False
-        except NotConvergedError as e:
-            # The FP32 factor is not accurate enough: the FP64 factor decides.
+        except (NotConvergedError, PrecisionInsufficientError) as e:
+            # The FP32 factor is not accurate enough (its refinement stalls, or its pivots fail the tests): the FP64
+            # factor decides.
             e.factor = None
             f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd, device_matrix=dm,
                                                              target=args.target, max_steps=4)
             fallback = True
+        last_x["fp32"] = x
         return f.factor_seconds, rep, fallback
 
     def mixed_step():
@@ -331,11 +338,14 @@ def main():
             p = float(selector.predict(PrecisionSelector.features(plan, dm))[0])
             choice["probability"] = p
             choice["precision"] = "fp32" if p >= 0.5 else "fp64"
-        return fp32_step() if choice["precision"] == "fp32" else fp64_step()
+        result = fp32_step() if choice["precision"] == "fp32" else fp64_step()
+        last_x["mixed"] = last_x[choice["precision"]]
+        return result
 
     def fp64_step():
         f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd, device_matrix=dm,
                                                          target=args.target, max_steps=4)
+        last_x["fp64"] = x
         return f.factor_seconds, rep, False
 
     def run(step, label, profile):
@@ -408,13 +418,28 @@ def main():
                 for k, v in profile.items()}
     kernels = breakdown(fp32_prof)
 
+    # The solutions on the host, for their error against the reference's FP64 solution.
+    solutions = {k: v.cpu().numpy() for k, v in last_x.items()}
     baseline = None
+    errors = {}
     if args.cpu_baseline == "auto":
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        log(f"cpu baseline: the reference's avx2 factor + solve on {threads} threads")
-        baseline = cpu_baseline(workload, b, threads)
-        if baseline:
-            log(f"cpu baseline: {baseline['value']} s ({baseline['gflops_factor']} GFLOP/s factor)")
+        log(f"cpu baseline: the reference's avx2 factor + solve on {threads} threads, median of 3")
+        runs = [cpu_baseline(workload, b, threads, keep_solution=True) for _ in range(3)]
+        if runs[0]:
+            runs.sort(key=lambda r: r["value"])
+            baseline = runs[1]
+            baseline["runs_s"] = sorted(r["value"] for r in runs)
+            reference_x = baseline.pop("x")
+            for r in runs:
+                r.pop("x", None)
+            scale = float(np.max(np.abs(reference_x)))
+            errors = {k: float(np.max(np.abs(v - reference_x)) / scale) for k, v in solutions.items()}
+            log(f"cpu baseline: {baseline['value']} s ({baseline['gflops_factor']} GFLOP/s factor); errors vs the "
+                f"reference's solution {errors}")
+            single = cpu_baseline(workload, b, 1)
+            baseline["one_thread"] = {k: single[k] for k in ("value", "factor_s", "solve_s", "gflops_factor")}
+            log(f"cpu baseline, 1 thread: {single['value']} s ({single['gflops_factor']} GFLOP/s factor)")
 
     flops = sym.flop_count
     out = {
@@ -454,6 +479,8 @@ def main():
                            "factor_tflops": round(flops / fp32_factor / 1e12, 2),
                            "mfma_frac": round(flops / fp32_factor / 1e12 / PEAK_FP32, 4)},
         "speedup_mixed_vs_fp64": round(fp64_s / mixed_s, 3),
+        # ‖x − x_ref‖∞ / ‖x_ref‖∞ against the reference's FP64 CPU factor and solve of the same matrix and loads.
+        "error_vs_reference": {k: float(f"{v:.3e}") for k, v in errors.items()} or None,
         "roofline": roofline,
         "autotune": tuning,
         "kernels_fp32_fronts_cg": kernels,

@@ -18,6 +18,9 @@
  * Ownership: the caller owns every array it passes; the library copies what it keeps. Handles own device memory.
  * Threading: a plan and the matrices and factors made with it belong to one device and one stream; calls on one plan
  * are serialised by the caller (distinct plans may be used from distinct threads). A factor is immutable once made.
+ * Streams: the library works on streams of its own. A call that reads device memory the caller gives first waits for
+ * the work queued on the caller's stream (oosg_plan_set_stream; the null stream by default), and every call returns
+ * only when its own device work is done, so its outputs may be used on any stream at once.
  */
 #ifndef OOSG_H
 #define OOSG_H
@@ -36,7 +39,10 @@ typedef enum oosg_status {
     OOSG_CANCELLED = 4,
     OOSG_INTERNAL = 5,
     OOSG_HIP_ERROR = 6,
-    OOSG_NOT_CONVERGED = 7
+    OOSG_NOT_CONVERGED = 7,
+    /* An FP32 factor met a zero or non-finite pivot, or failed the ratio test, where single precision may be the cause:
+     * the matrix is not shown singular; factorise it in FP64 (which decides, with OOSG_SINGULAR). */
+    OOSG_PRECISION_INSUFFICIENT = 8
 } oosg_status;
 
 /* The symbolic factor, factor order (SymbolicFactor, include/oos/linalg/symbolic/SymbolicFactor.hpp:37-125):
@@ -116,6 +122,21 @@ typedef struct oosg_factor oosg_factor;
 
 /* The message of the last failure on this thread. */
 const char *oosg_last_error(void);
+/* The equations the last OOSG_SINGULAR (or OOSG_PRECISION_INSUFFICIENT) on this thread names, in elimination order:
+ * the zero pivots, or the pivots failing the ratio test; at most OOSG_RECORDED_ISSUE_COUNT are kept, *count receives
+ * how many (the DOFs of SingularityError::dofs(), include/oos/base/errors/SingularityError.hpp:43-50; the cap of
+ * PivotDiagnostics::RECORDED_ISSUE_COUNT, include/oos/linalg/supernodal/PivotDiagnostics.hpp:70-75). Up to
+ * `capacity` of them are written to `equations` (may be null). */
+#define OOSG_RECORDED_ISSUE_COUNT 1000
+oosg_status oosg_last_error_equations(int32_t *equations, int64_t capacity, int64_t *count);
+/* What the last OOSG_OUT_OF_MEMORY on this thread asked for (MemoryShortfall of MemoryBudgetError,
+ * include/oos/base/errors/MemoryBudgetError.hpp:24-36): the bytes requested, those left in the budget, the budget. */
+typedef struct oosg_memory_shortfall {
+    uint64_t requested_bytes;
+    uint64_t available_bytes;
+    uint64_t limit_bytes;
+} oosg_memory_shortfall;
+oosg_status oosg_last_error_shortfall(oosg_memory_shortfall *shortfall);
 
 /* The plan of the numeric phase for one symbolic factor on `device`: the supernode tree by levels, the relative
  * indices of the extend-add, the schedule of the kernels and the device memory (panels, update matrices), all
@@ -124,6 +145,23 @@ const char *oosg_last_error(void);
  * matrices of its structure. */
 oosg_status oosg_plan_create(const oosg_symbolic_desc *symbolic, int32_t device, uint64_t hbm_budget_bytes,
                              oosg_plan **out);
+/* The device memory oosg_plan_create reserves for `symbolic` (SupernodalFactor::estimateMemoryBytes,
+ * SupernodalFactor.hpp:185-189): the structure, the update matrices, one factor's panels, inverses and pivots, and the
+ * solves' and refinements' vectors for 16 right-hand sides. In indefinite mode delayed pivots grow fronts and panels
+ * beyond it (reserved when they happen). No device is touched. */
+oosg_status oosg_estimate_memory_bytes(const oosg_symbolic_desc *symbolic, int32_t pivoting, uint64_t *bytes);
+/* The bytes of the plan's budget in use now (the plan's own memory and that of its live factors) and the budget. */
+oosg_status oosg_plan_memory(const oosg_plan *plan, uint64_t *reserved_bytes, uint64_t *budget_bytes);
+/* The caller's stream (a hipStream_t; null: the null stream): calls that read the caller's device memory wait for the
+ * work queued on it before their first launch. */
+oosg_status oosg_plan_set_stream(oosg_plan *plan, void *stream);
+/* Cancellation (FactorOptions::cancellation, SupernodalFactor.hpp:78-80; base::CancellationToken): `cancelled(context)`
+ * is polled before a factorisation, between its levels where it is launched one launch at a time, before it is
+ * finished, before each solve of a refinement and between the 16-column tiles of a solve; a non-zero answer stops the
+ * call with OOSG_CANCELLED once the work already queued is done (nothing is left half written: no factor is
+ * returned). Null: no polling. */
+typedef int32_t (*oosg_cancel_fn)(void *context);
+oosg_status oosg_plan_set_cancellation(oosg_plan *plan, oosg_cancel_fn cancelled, void *context);
 void oosg_plan_destroy(oosg_plan *plan);
 /* {supernodes, levels, panel values, update values, small fronts, large fronts, kernel launches per factorisation,
  * MFMA update tiles}: 8 values. */
@@ -143,6 +181,19 @@ oosg_status oosg_factorise(oosg_plan *plan, const oosg_matrix *matrix, const oos
                            oosg_factor **out);
 void oosg_factor_destroy(oosg_factor *factor);
 oosg_status oosg_factor_info_get(const oosg_factor *factor, oosg_factor_info *info);
+/* One pivot that fails the ratio test (PivotIssue, include/oos/linalg/supernodal/PivotDiagnostics.hpp:54-62). */
+typedef struct oosg_pivot_issue {
+    int32_t equation;
+    int32_t kind; /* 0 RatioAboveMaximum, 1 NegativePivot (PivotIssueKind) */
+    double matrix_diagonal; /* the diagonal of the test: the matrix's, or the reference given for the equation */
+    double factor_diagonal;
+    double ratio;
+} oosg_pivot_issue;
+/* The issues of a factor kept despite them (bailout < 0), in elimination order: the first OOSG_RECORDED_ISSUE_COUNT
+ * (PivotDiagnostics::issues(), PivotDiagnostics.hpp:78-81; info.issue_count counts all). Up to `capacity` are
+ * written; *recorded receives how many are kept. */
+oosg_status oosg_factor_issues(const oosg_factor *factor, oosg_pivot_issue *issues, int64_t capacity,
+                               int64_t *recorded);
 /* The panel of supernode s in the reference's layout (FactorPanel, SupernodalFactor.hpp:110-125): values
  * (c + r) × e column-major, L below the diagonal, D on it, zero above; rows[k] the equation of row k. Either output
  * may be null; *eliminated receives e. For tests and diagnostics. */
@@ -159,14 +210,17 @@ oosg_status oosg_factor_subdiagonal(const oosg_factor *factor, int32_t supernode
 oosg_status oosg_factor_diagonal(const oosg_factor *factor, double *d);
 
 /* X ← A⁻¹·B in place (FactorSolve::solve, FactorSolve.hpp:77-81): b is n × nrhs column-major, leading dimension
- * ldb >= n, host memory. With an FP32 factor the solve applies the factor once (no refinement). */
+ * ldb >= n, host memory, any nrhs (tiles of 16 columns through the plan's own vectors: nothing is allocated). With an
+ * FP32 factor the solve applies the factor once (no refinement). */
 oosg_status oosg_solve(const oosg_factor *factor, double *b, int64_t ldb, int64_t nrhs);
 /* The same on device memory: b_device is n × nrhs column-major (leading dimension ldb) in the plan's device memory. */
 oosg_status oosg_solve_device(const oosg_factor *factor, double *b_device, int64_t ldb, int64_t nrhs);
 /* x = A⁻¹·b by iterative refinement in double precision until the backward error is at most `target` (or
  * max_steps solves): x_device ← 0, then d = factor⁻¹·r, x += d, r = b − A·x with `matrix` (the matrix factorised).
- * b_device and x_device are n × nrhs column-major device arrays (leading dimension ld). OOSG_NOT_CONVERGED when the
- * target is not met (the report says how far it got). */
+ * b_device and x_device are n × nrhs column-major device arrays (leading dimension ld), any nrhs: tiles of 16 columns
+ * (FactorSolve.hpp:62-67 tiles its right-hand sides likewise), refined one after the other in the plan's vectors; the
+ * report gives the most solves of a tile and the largest backward error. OOSG_NOT_CONVERGED when the target is not
+ * met (the report says how far it got). */
 oosg_status oosg_solve_refined_device(const oosg_factor *factor, const oosg_matrix *matrix, const double *b_device,
                                       double *x_device, int64_t ld, int64_t nrhs, double target, int32_t max_steps,
                                       oosg_solve_report *report);

@@ -22,16 +22,51 @@ import numpy as np
 from . import _native
 
 __all__ = ["SymmetricCsc", "SymbolicFactor", "SupernodalFactor", "FactorSolve", "FactorOptions", "PivotingMode",
-           "Precision", "PivotPolicy", "Inertia", "PivotDiagnostics", "SolveReport", "Plan", "MemoryBudgetError",
-           "SingularityError", "InternalError", "HipError", "NotConvergedError", "nested_dissection"]
+           "Precision", "PivotPolicy", "Inertia", "PivotDiagnostics", "PivotIssue", "SolveReport", "Plan",
+           "MemoryBudgetError", "MemoryShortfall", "SingularityError", "PrecisionInsufficientError",
+           "OperationCancelledError", "InternalError", "HipError", "NotConvergedError", "nested_dissection",
+           "estimate_memory_bytes"]
+
+
+@dataclass(frozen=True)
+class MemoryShortfall:
+    """What a refused reservation asked for (MemoryShortfall, include/oos/base/errors/MemoryBudgetError.hpp:24-29)."""
+    requested_bytes: int
+    available_bytes: int
+    limit_bytes: int
 
 
 class MemoryBudgetError(MemoryError):
-    """The factorisation does not fit the device memory (refused before any numeric work)."""
+    """The factorisation does not fit the device memory (refused before any numeric work); `shortfall` says by how
+    much (MemoryBudgetError.hpp:31-36)."""
+
+    def __init__(self, message, shortfall=None):
+        super().__init__(message)
+        self.shortfall = shortfall
 
 
 class SingularityError(RuntimeError):
-    """A zero or non-finite pivot in positive definite mode, or a failed ratio test that stops the run."""
+    """A zero or non-finite pivot in positive definite mode, or a failed ratio test that stops the run; `dofs` lists
+    the equations (or the names an equation namer gives them) in elimination order, at most 1000
+    (SingularityError::dofs, include/oos/base/errors/SingularityError.hpp:43-50)."""
+
+    def __init__(self, message, dofs=()):
+        super().__init__(message)
+        self.dofs = list(dofs)
+
+
+class PrecisionInsufficientError(RuntimeError):
+    """An FP32 factor met a zero pivot or failed the ratio test: the matrix is not shown singular (factorise it in
+    FP64, which decides). `dofs` as SingularityError's."""
+
+    def __init__(self, message, dofs=()):
+        super().__init__(message)
+        self.dofs = list(dofs)
+
+
+class OperationCancelledError(RuntimeError):
+    """The cancellation callback asked to stop (OperationCancelledError,
+    include/oos/base/errors/OperationCancelledError.hpp)."""
 
 
 class InternalError(RuntimeError):
@@ -46,14 +81,49 @@ class NotConvergedError(RuntimeError):
     pass
 
 
-_ERRORS = {1: ValueError, 2: MemoryBudgetError, 3: SingularityError, 4: RuntimeError, 5: InternalError, 6: HipError,
-           7: NotConvergedError}
+_ERRORS = {1: ValueError, 2: MemoryBudgetError, 3: SingularityError, 4: OperationCancelledError, 5: InternalError,
+           6: HipError, 7: NotConvergedError, 8: PrecisionInsufficientError}
 
 
-def _check(status):
+def _last_equations(lib):
+    count = ctypes.c_int64()
+    lib.oosg_last_error_equations(None, 0, ctypes.byref(count))
+    out = np.zeros(count.value, np.int32)
+    lib.oosg_last_error_equations(_ptr(out), count.value, ctypes.byref(count))
+    return out.tolist()
+
+
+def _error(status, namer=None):
+    """The exception of a failed call, with the details the library kept for it."""
+    lib = _native.lib()
+    message = lib.oosg_last_error().decode(errors="replace")
+    kind = _ERRORS.get(status, RuntimeError)
+    if kind in (SingularityError, PrecisionInsufficientError):
+        equations = _last_equations(lib)
+        if namer is not None:
+            # The reference names equations through FactorOptions::equationNamer (describeEquation,
+            # PivotDiagnostics.hpp:122-127): "equation 12" becomes the namer's name of equation 12.
+            for e in equations[:20]:
+                message = message.replace(f"  equation {e}\n", f"  {namer(e)}\n")
+            return kind(message, [namer(e) for e in equations])
+        return kind(message, equations)
+    if kind is MemoryBudgetError:
+        sf = _native.MemoryShortfallC()
+        lib.oosg_last_error_shortfall(ctypes.byref(sf))
+        return kind(message, MemoryShortfall(int(sf.requested_bytes), int(sf.available_bytes), int(sf.limit_bytes)))
+    return kind(message)
+
+
+def _check(status, namer=None):
     if status != 0:
-        message = _native.lib().oosg_last_error().decode(errors="replace")
-        raise _ERRORS.get(status, RuntimeError)(message)
+        raise _error(status, namer)
+
+
+def _follow_torch(plan, device_tensor):
+    """The library's calls wait for the work torch queued on its current stream before reading `device_tensor`."""
+    import torch
+    stream = torch.cuda.current_stream(device_tensor.device)
+    _check(_native.lib().oosg_plan_set_stream(plan.handle, ctypes.c_void_p(stream.cuda_stream)))
 
 
 def _ptr(array):
@@ -93,6 +163,11 @@ class FactorOptions:
     # from which it picks FP32.
     selector: object = None
     fp32_threshold: float = 0.5
+    # Names the equations of the diagnostics (FactorOptions::equationNamer, SupernodalFactor.hpp:76-77): a callable
+    # equation -> str; None names them by number.
+    equation_namer: object = None
+    # Polled between the parts of a call (FactorOptions::cancellation): a callable returning True to stop, or None.
+    cancellation: object = None
 
 
 @dataclass(frozen=True)
@@ -103,12 +178,23 @@ class Inertia:
 
 
 @dataclass(frozen=True)
+class PivotIssue:
+    """One pivot failing the ratio test (PivotIssue, include/oos/linalg/supernodal/PivotDiagnostics.hpp:54-62)."""
+    equation: int
+    matrix_diagonal: float
+    factor_diagonal: float
+    ratio: float
+    negative_pivot: bool
+
+
+@dataclass(frozen=True)
 class PivotDiagnostics:
     issue_count: int
     first_issue_equation: int
     first_issue_negative: bool
     largest_ratio: float
     largest_ratio_equation: int
+    issues: tuple = ()  # the first 1000 PivotIssue, in elimination order
 
 
 @dataclass(frozen=True)
@@ -245,6 +331,10 @@ class SymbolicFactor:
         return _native.SymbolicDesc(self.n, self.supernode_count, _ptr(self.first_column), _ptr(self.column_count),
                                     _ptr(self.parent), _ptr(self.row_start), _ptr(self.rows_all))
 
+    def estimate_memory_bytes(self, pivoting=0):
+        """The device bytes a plan of this structure reserves (SupernodalFactor::estimateMemoryBytes)."""
+        return estimate_memory_bytes(self, pivoting)
+
     def plan(self, device=0, hbm_budget_bytes=0):
         """The device plan of this structure (kept: one per device and budget)."""
         key = (device, hbm_budget_bytes)
@@ -298,6 +388,22 @@ class Plan:
 
     def synchronize(self):
         _check(self._lib.oosg_plan_synchronize(self.handle))
+
+    def memory(self):
+        """(bytes of the budget in use: the plan's own memory and its live factors', the budget)."""
+        reserved, budget = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self._lib.oosg_plan_memory(self.handle, ctypes.byref(reserved), ctypes.byref(budget)))
+        return int(reserved.value), int(budget.value)
+
+    def set_cancellation(self, cancelled):
+        """Polls `cancelled()` between the parts of the plan's calls (None: no polling); True stops the call with
+        OperationCancelledError once the work queued is done."""
+        if cancelled is None:
+            self._cancel_fn = None
+            _check(self._lib.oosg_plan_set_cancellation(self.handle, _native.CANCEL_FN(), None))
+            return
+        self._cancel_fn = _native.CANCEL_FN(lambda _context: 1 if cancelled() else 0)
+        _check(self._lib.oosg_plan_set_cancellation(self.handle, self._cancel_fn, None))
 
     def use_graphs(self, enable=True):
         """Replays of the factorisation's and the sweeps' launch sequences as HIP graphs (default on)."""
@@ -377,6 +483,7 @@ class DeviceMatrix:
         import torch
         xs = _as_device_columns(x)
         y = torch.empty_like(xs)
+        _follow_torch(self.plan, xs)
         _check(self._lib.oosg_matrix_multiply_device(self.handle, xs.data_ptr(), y.data_ptr(), xs.shape[1],
                                                      xs.shape[0]))
         return y.T if x.dim() == 2 else y.reshape(-1)
@@ -403,6 +510,9 @@ class SupernodalFactor:
         self.plan = plan
         self.device_matrix = device_matrix
         self.options = options
+        self.selected_probability = None
+        # Precision.LEARNED: whether the FP32 factor it chose failed and FP64 took over.
+        self.fell_back = False
         info = _native.FactorInfoC()
         _check(self._lib.oosg_factor_info_get(handle, ctypes.byref(info)))
         self._info = info
@@ -432,54 +542,78 @@ class SupernodalFactor:
                 raise ValueError("factorise: Precision.LEARNED needs a selector (no committed weights found)")
             selected = float(selector.predict(selector.features(plan, device_matrix))[0])
             precision = Precision.FP32 if selected >= options.fp32_threshold else Precision.FP64
-        opts = _native.FactorOptionsC(int(options.pivoting), int(precision),
-                                      float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
-                                      _ptr(reference))
+        opts = cls._options_c(options, precision, reference)
+        plan.set_cancellation(options.cancellation)
         return options, plan, device_matrix, opts, reference, selected
+
+    @staticmethod
+    def _options_c(options, precision, reference):
+        return _native.FactorOptionsC(int(options.pivoting), int(precision), float(options.pivot_policy.max_ratio),
+                                      int(options.pivot_policy.bailout), _ptr(reference))
 
     @classmethod
     def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None):
         """SupernodalFactor::factorise (SupernodalFactor.hpp:176-183). `matrix` is a SymmetricCsc in the factor's
-        order; `device_matrix` (optional) is its device copy, reused across factorisations."""
+        order; `device_matrix` (optional) is its device copy, reused across factorisations. With Precision.LEARNED,
+        an FP32 factor that fails its pivot tests is replaced by the FP64 factor (`fell_back`)."""
         options, plan, device_matrix, opts, reference, selected = cls._call(symbolic, matrix, options, device,
                                                                             hbm_budget_bytes, device_matrix)
         handle = ctypes.c_void_p()
         lib = _native.lib()
-        _check(lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle)))
+        status = lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle))
+        fell_back = False
+        if status == 8 and options.precision == Precision.LEARNED:
+            opts = cls._options_c(options, Precision.FP64, reference)
+            status = lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle))
+            fell_back = True
+        _check(status, options.equation_namer)
         factor = cls(handle, plan, device_matrix, options)
         factor.selected_probability = selected
+        factor.fell_back = fell_back
         return factor
 
     @classmethod
     def factorise_and_solve(cls, symbolic, matrix, b, options=None, device=0, hbm_budget_bytes=0,
                             device_matrix=None, target=1e-10, max_steps=10):
-        """factorise, then FactorSolve.solve_refined of the device tensor b (n × k, k <= 16), as one call
+        """factorise, then FactorSolve.solve_refined of the device tensor b (n × k), as one call
         (oosg_factorise_solve_device): the first solve's forward sweep of the lower levels runs beside the
         factorisation of the top levels. Returns (factor, x, SolveReport); NotConvergedError carries the factor as
-        its `factor` attribute."""
+        its `factor` attribute. With Precision.LEARNED, an FP32 factor that fails its pivot tests or whose refinement
+        misses the target is replaced by the FP64 call (`fell_back` on the factor)."""
         import torch
         options, plan, device_matrix, opts, reference, selected = cls._call(symbolic, matrix, options, device,
                                                                             hbm_budget_bytes, device_matrix)
         bc = _as_device_columns(b)
         xc = torch.empty_like(bc)
-        handle = ctypes.c_void_p()
-        report = _native.SolveReportC()
         lib = _native.lib()
-        status = lib.oosg_factorise_solve_device(plan.handle, device_matrix.handle, ctypes.byref(opts),
-                                                 bc.data_ptr(), xc.data_ptr(), bc.shape[1], bc.shape[0],
-                                                 float(target), int(max_steps), ctypes.byref(handle),
-                                                 ctypes.byref(report))
-        factor = None
-        if handle.value:
-            factor = cls(handle, plan, device_matrix, options)
+
+        def run(c_options):
+            handle = ctypes.c_void_p()
+            report = _native.SolveReportC()
+            _follow_torch(plan, bc)
+            status = lib.oosg_factorise_solve_device(plan.handle, device_matrix.handle, ctypes.byref(c_options),
+                                                     bc.data_ptr(), xc.data_ptr(), bc.shape[1], bc.shape[0],
+                                                     float(target), int(max_steps), ctypes.byref(handle),
+                                                     ctypes.byref(report))
+            factor = cls(handle, plan, device_matrix, options) if handle.value else None
+            rep = SolveReport(int(report.steps), bool(report.converged), float(report.backward_error),
+                              float(report.seconds))
+            return status, factor, rep
+
+        status, factor, rep = run(opts)
+        fell_back = False
+        if status in (7, 8) and options.precision == Precision.LEARNED and opts.precision == Precision.FP32:
+            factor = None
+            status, factor, rep = run(cls._options_c(options, Precision.FP64, reference))
+            fell_back = True
+        if factor is not None:
             factor.selected_probability = selected
-        rep = SolveReport(int(report.steps), bool(report.converged), float(report.backward_error),
-                          float(report.seconds))
+            factor.fell_back = fell_back
         if status == 7:
             error = NotConvergedError(f"{lib.oosg_last_error().decode()} ({rep})")
             error.factor = factor
             raise error
-        _check(status)
+        _check(status, options.equation_namer)
         return factor, (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
 
     @property
@@ -505,8 +639,14 @@ class SupernodalFactor:
     @property
     def pivot_diagnostics(self):
         i = self._info
+        recorded = ctypes.c_int64()
+        _check(self._lib.oosg_factor_issues(self.handle, None, 0, ctypes.byref(recorded)))
+        raw = (_native.PivotIssueC * max(recorded.value, 1))()
+        _check(self._lib.oosg_factor_issues(self.handle, raw, recorded.value, ctypes.byref(recorded)))
+        issues = tuple(PivotIssue(int(r.equation), float(r.matrix_diagonal), float(r.factor_diagonal), float(r.ratio),
+                                  bool(r.kind == 1)) for r in raw[:recorded.value])
         return PivotDiagnostics(int(i.issue_count), int(i.first_issue_equation), bool(i.first_issue_negative),
-                                float(i.largest_ratio), int(i.largest_ratio_equation))
+                                float(i.largest_ratio), int(i.largest_ratio_equation), issues)
 
     @property
     def factor_seconds(self):
@@ -567,6 +707,7 @@ class FactorSolve:
             cols = _as_device_columns(b)
             if cols.shape[1] != n:
                 raise ValueError(f"FactorSolve::solve: the right-hand sides have {cols.shape[1]} rows, the factor {n}")
+            _follow_torch(self.factor.plan, cols)
             _check(self._lib.oosg_solve_device(self.factor.handle, cols.data_ptr(), n, cols.shape[0]))
             return cols.T if b.dim() == 2 else cols.reshape(-1)
         arr = np.asarray(b)
@@ -579,12 +720,12 @@ class FactorSolve:
 
     def solve_refined(self, b, target=1e-10, max_steps=10, x=None):
         """x = A⁻¹·b by refinement in FP64 with the factorised matrix, to the backward error `target`. `b`: torch
-        tensor n × k (k <= 16) on the device. Returns (x, SolveReport); raises NotConvergedError when the target is
-        missed."""
+        tensor n × k on the device. Returns (x, SolveReport); raises NotConvergedError when the target is missed."""
         import torch
         bc = _as_device_columns(b)
         xc = torch.empty_like(bc) if x is None else x
         report = _native.SolveReportC()
+        _follow_torch(self.factor.plan, bc)
         status = self._lib.oosg_solve_refined_device(self.factor.handle, self.factor.device_matrix.handle,
                                                      bc.data_ptr(), xc.data_ptr(), bc.shape[1], bc.shape[0],
                                                      float(target), int(max_steps), ctypes.byref(report))
@@ -594,6 +735,14 @@ class FactorSolve:
             raise NotConvergedError(f"{_native.lib().oosg_last_error().decode()} ({rep})")
         _check(status)
         return (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
+
+
+def estimate_memory_bytes(symbolic, pivoting=0):
+    """The device bytes oosg_plan_create reserves for `symbolic` (no device is touched)."""
+    out = ctypes.c_uint64()
+    desc = symbolic.desc()
+    _check(_native.lib().oosg_estimate_memory_bytes(ctypes.byref(desc), int(pivoting), ctypes.byref(out)))
+    return int(out.value)
 
 
 def nested_dissection(graph_offsets, adjacency, node_weights, coordinates, leaf_nodes=64):

@@ -42,8 +42,28 @@ class SolveReportC(ctypes.Structure):
     _fields_ = [("steps", c_int32), ("converged", c_int32), ("backward_error", c_double), ("seconds", c_double)]
 
 
+class PivotIssueC(ctypes.Structure):
+    _fields_ = [("equation", c_int32), ("kind", c_int32), ("matrix_diagonal", c_double),
+                ("factor_diagonal", c_double), ("ratio", c_double)]
+
+
+class MemoryShortfallC(ctypes.Structure):
+    _fields_ = [("requested_bytes", ctypes.c_uint64), ("available_bytes", ctypes.c_uint64),
+                ("limit_bytes", ctypes.c_uint64)]
+
+
+CANCEL_FN = ctypes.CFUNCTYPE(c_int32, c_void_p)
+
+
 _SIGNATURES = {
     "oosg_last_error": (ctypes.c_char_p, []),
+    "oosg_last_error_equations": (c_int, [c_void_p, c_int64, ctypes.POINTER(c_int64)]),
+    "oosg_last_error_shortfall": (c_int, [ctypes.POINTER(MemoryShortfallC)]),
+    "oosg_estimate_memory_bytes": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, ctypes.POINTER(ctypes.c_uint64)]),
+    "oosg_plan_memory": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "oosg_plan_set_stream": (c_int, [c_void_p, c_void_p]),
+    "oosg_plan_set_cancellation": (c_int, [c_void_p, CANCEL_FN, c_void_p]),
+    "oosg_factor_issues": (c_int, [c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)]),
     "oosg_plan_create": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
     "oosg_plan_destroy": (None, [c_void_p]),
     "oosg_plan_stats": (c_int, [c_void_p, c_void_p]),

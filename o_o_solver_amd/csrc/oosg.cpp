@@ -27,16 +27,27 @@ namespace
 {
 
 thread_local std::string g_lastError;
+// What the last failure names besides its message: the equations of a SingularityError, the shortfall of a
+// MemoryBudgetError.
+thread_local std::vector<int32_t> g_lastEquations;
+thread_local oosg_memory_shortfall g_lastShortfall{};
 
 struct Failure
 {
     oosg_status status;
     std::string message;
+    std::vector<int32_t> equations;
+    oosg_memory_shortfall shortfall{};
 };
 
 [[noreturn]] void fail(oosg_status status, const std::string& message)
 {
-    throw Failure{status, message};
+    throw Failure{status, message, {}, {}};
+}
+
+[[noreturn]] void failNaming(oosg_status status, const std::string& message, std::vector<int32_t> equations)
+{
+    throw Failure{status, message, std::move(equations), {}};
 }
 
 void check(hipError_t e, const char* what)
@@ -55,40 +66,51 @@ template <typename F> oosg_status guarded(F&& body)
     catch (const Failure& f)
     {
         g_lastError = f.message;
+        g_lastEquations = f.equations;
+        g_lastShortfall = f.shortfall;
         return f.status;
     }
     catch (const std::bad_alloc&)
     {
         g_lastError = "host memory exhausted";
+        g_lastEquations.clear();
+        g_lastShortfall = {};
         return OOSG_OUT_OF_MEMORY;
     }
     catch (const std::exception& e)
     {
         g_lastError = e.what();
+        g_lastEquations.clear();
+        g_lastShortfall = {};
         return OOSG_INTERNAL;
     }
 }
 
-// Device memory owned by a handle.
+// Device memory owned by a handle. A buffer counted in a plan's budget holds the plan's count (`account`) and gives
+// its bytes back when it is freed.
 struct DeviceBuffer
 {
     void* ptr = nullptr;
     size_t bytes = 0;
+    uint64_t* account = nullptr;
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes)
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes), account(o.account)
     {
         o.ptr = nullptr;
         o.bytes = 0;
+        o.account = nullptr;
     }
     DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
     {
         reset();
         ptr = o.ptr;
         bytes = o.bytes;
+        account = o.account;
         o.ptr = nullptr;
         o.bytes = 0;
+        o.account = nullptr;
         return *this;
     }
     ~DeviceBuffer() { reset(); }
@@ -96,8 +118,11 @@ struct DeviceBuffer
     {
         if (ptr != nullptr)
             (void)hipFree(ptr);
+        if (account != nullptr)
+            *account -= bytes;
         ptr = nullptr;
         bytes = 0;
+        account = nullptr;
     }
     void allocate(size_t size, const char* what)
     {
@@ -310,9 +335,26 @@ double frontFlops(int64_t c, int64_t r)
 
 } // namespace
 
+// The device memory of one factor: panels, inverses of the diagonal blocks, pivots, and (FP32) the equations' scales.
+// Sized for FP64 values, so that either precision can use it.
+struct FactorStorage
+{
+    DeviceBuffer panels, linv, diag, scale;
+};
+
 struct oosg_plan
 {
     int device = 0;
+    // The device memory budget and what the plan and its live factors hold of it (declared first: the buffers below
+    // give their bytes back to it when they are freed).
+    uint64_t budget = 0;
+    uint64_t reserved = 0;
+    // The caller's stream (calls that read the caller's device memory wait for its work first) and its event.
+    hipStream_t callerStream = nullptr;
+    hipEvent_t callerEvent = nullptr;
+    // Cancellation: polled between the parts of a call (null: none).
+    oosg_cancel_fn cancelled = nullptr;
+    void* cancelContext = nullptr;
     // Its holders: the caller that made it, and every matrix and factor made from it (the last release deletes it, so
     // that a matrix or factor released after the plan still finds it).
     int32_t references = 1;
@@ -382,10 +424,10 @@ struct oosg_plan
         dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
     oosg::DevicePlan dp{};
 
-    DeviceBuffer updates;       // the update matrices of a factorisation (FP64 size; FP32 uses half)
-    DeviceBuffer panelCache[2]; // panels given back by destroyed factors, per precision
-    DeviceBuffer linvCache[2];  // and their inverses of the diagonal blocks
-    DeviceBuffer diagCache;     // and their pivots
+    DeviceBuffer updates; // the update matrices of a factorisation (FP64 size; FP32 uses half)
+    // Factor storage not in use: reserved with the plan, and given back by destroyed factors for the next
+    // factorisation (the role of SupernodalFactor::releaseStorage and FactorOptions::storage, SupernodalFactor.hpp:91-96).
+    std::vector<FactorStorage> storagePool;
     DeviceBuffer pivotStatus;   // per supernode the first zero pivot of a factorisation (INT32_MAX: none)
     DeviceBuffer pivotStats;    // the pivots' statistics of the last factorisation (oosg::PivotStatistics)
     // A combined factorisation and solve: from the first launch of level aheadLevel on (-1: off), the forward sweep of
@@ -405,7 +447,6 @@ struct oosg_plan
     DeviceBuffer solvePartials; // the sweeps' slots of partial sums (partialSlots × 64 × 16)
     // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
     DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
-    uint64_t budget = 0;
     // Launch sequences replayed as graphs (off while the profiler times single launches): the factorisation per
     // precision, and the sweeps.
     bool graphs = true;
@@ -443,6 +484,8 @@ struct oosg_plan
                 (void)hipEventDestroy(e);
         if (sweepStream)
             (void)hipStreamDestroy(sweepStream);
+        if (callerEvent)
+            (void)hipEventDestroy(callerEvent);
         if (stream && ownsStream)
             (void)hipStreamDestroy(stream);
     }
@@ -498,16 +541,19 @@ struct oosg_factor
     // The pivots on the host (FP32: of the matrix, d_j / s_j²): filled on first use (pivoted factors: at once).
     mutable std::vector<double> diag;
     oosg_factor_info info{};
+    // The first OOSG_RECORDED_ISSUE_COUNT pivots failing the ratio test, when the factor was kept despite them.
+    std::vector<oosg_pivot_issue> issues;
     ~oosg_factor()
     {
-        if (pivoted)
-            return; // sized for its own structure
-        if (plan != nullptr && panels.ptr != nullptr && plan->panelCache[precision].ptr == nullptr)
-            plan->panelCache[precision] = std::move(panels);
-        if (plan != nullptr && linv.ptr != nullptr && plan->linvCache[precision].ptr == nullptr)
-            plan->linvCache[precision] = std::move(linv);
-        if (plan != nullptr && dDiag.ptr != nullptr && plan->diagCache.ptr == nullptr)
-            plan->diagCache = std::move(dDiag);
+        if (pivoted || plan == nullptr || panels.ptr == nullptr)
+            return; // a pivoted factor is sized for its own structure
+        FactorStorage back;
+        back.panels = std::move(panels);
+        back.linv = std::move(linv);
+        back.diag = std::move(dDiag);
+        back.scale = std::move(scale);
+        if (plan->storagePool.size() < 2)
+            plan->storagePool.push_back(std::move(back));
     }
 };
 
@@ -522,6 +568,97 @@ size_t valueBytes(int precision)
 void setDevice(const oosg_plan* plan)
 {
     check(hipSetDevice(plan->device), "hipSetDevice");
+}
+
+std::string mebibytes(uint64_t bytes)
+{
+    return std::to_string((bytes + (1u << 20) - 1) >> 20) + " MiB";
+}
+
+// MemoryBudgetError: `bytes` more do not fit the plan's budget (raised before the allocation, and before any numeric
+// work of the call that asked).
+[[noreturn]] void failBudget(const oosg_plan& p, uint64_t bytes, const std::string& what)
+{
+    Failure f{OOSG_OUT_OF_MEMORY,
+              what + " needs " + mebibytes(bytes) + " of device memory; the budget of " + mebibytes(p.budget) +
+                  " has " + mebibytes(p.budget > p.reserved ? p.budget - p.reserved : 0) + " left",
+              {},
+              {}};
+    f.shortfall.requested_bytes = bytes;
+    f.shortfall.available_bytes = p.budget > p.reserved ? p.budget - p.reserved : 0;
+    f.shortfall.limit_bytes = p.budget;
+    throw f;
+}
+
+// An allocation counted in the plan's budget (the buffer's previous memory is given back first).
+void allocateCounted(oosg_plan& p, DeviceBuffer& b, size_t bytes, const char* what)
+{
+    b.reset();
+    if (bytes == 0)
+        bytes = 8;
+    if (p.reserved + bytes > p.budget)
+        failBudget(p, bytes, what);
+    b.allocate(bytes, what);
+    b.account = &p.reserved;
+    p.reserved += bytes;
+}
+
+void ensureCounted(oosg_plan& p, DeviceBuffer& b, size_t bytes, const char* what)
+{
+    if (b.bytes < bytes)
+        allocateCounted(p, b, bytes, what);
+}
+
+// The bytes of one factor's storage (FactorStorage), sized for FP64 values.
+uint64_t factorStorageBytes(const oosg_plan& p);
+
+// Storage for a factor: one given back to the plan, or a new allocation counted in the budget.
+FactorStorage takeStorage(oosg_plan& p)
+{
+    const uint64_t panelBytes = static_cast<uint64_t>(std::max<int64_t>(p.panelOff[p.S], 1)) * sizeof(double);
+    const uint64_t linvBytes = static_cast<uint64_t>(std::max<int64_t>(p.linvOff[p.S], 1)) * sizeof(double);
+    const uint64_t vectorBytes = static_cast<uint64_t>(std::max(p.n, 1)) * sizeof(double);
+    for (size_t k = 0; k < p.storagePool.size(); ++k)
+    {
+        FactorStorage& st = p.storagePool[k];
+        if (st.panels.bytes >= panelBytes && st.linv.bytes >= linvBytes && st.diag.bytes >= vectorBytes &&
+            st.scale.bytes >= vectorBytes)
+        {
+            FactorStorage out = std::move(st);
+            p.storagePool.erase(p.storagePool.begin() + static_cast<std::ptrdiff_t>(k));
+            return out;
+        }
+    }
+    if (p.reserved + factorStorageBytes(p) > p.budget)
+        failBudget(p, factorStorageBytes(p), "a factor (panels, inverses, pivots)");
+    FactorStorage out;
+    allocateCounted(p, out.panels, panelBytes, "factor: panels");
+    allocateCounted(p, out.linv, linvBytes, "factor: inverses of the diagonal blocks");
+    allocateCounted(p, out.diag, vectorBytes, "factor: pivots");
+    allocateCounted(p, out.scale, vectorBytes, "factor: scales");
+    return out;
+}
+
+// The plan's stream waits for the work queued on the caller's stream (before a call reads the caller's memory).
+void followCaller(oosg_plan& p)
+{
+    if (p.callerEvent == nullptr)
+        check(hipEventCreateWithFlags(&p.callerEvent, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(p.callerEvent, p.callerStream), "caller stream");
+    check(hipStreamWaitEvent(p.stream, p.callerEvent, 0), "caller stream");
+}
+
+// OperationCancelledError (include/oos/base/errors/OperationCancelledError.hpp): the work queued is finished first.
+void pollCancellation(oosg_plan& p)
+{
+    if (p.cancelled == nullptr || p.cancelled(p.cancelContext) == 0)
+        return;
+    for (hipStream_t s : p.lanes)
+        if (s != nullptr)
+            (void)hipStreamSynchronize(s);
+    if (p.sweepStream != nullptr)
+        (void)hipStreamSynchronize(p.sweepStream);
+    fail(OOSG_CANCELLED, "the operation was cancelled (by an interruption or a progress callback)");
 }
 
 // ---- plan --------------------------------------------------------------------------------------------------------
@@ -1286,8 +1423,68 @@ void uploadPlan(oosg_plan& p)
     p.dBigTiles = upload(p.bigTiles, st, "plan: big update tiles");
     p.dItemStarts = upload(p.itemStarts, st, "plan: update tile items");
     p.dExtendItems = upload(p.extendItems, st, "plan: update tile items");
-    p.updates.allocate(static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
+    allocateCounted(p, p.updates, static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
+}
+
+// The plan's own arrays on the device (structure, launch lists), in bytes.
+uint64_t structureBytes(const oosg_plan& p)
+{
+    return (p.first.size() * 3 + p.rows.size() * 2 + p.children.size() + p.smallList.size() + p.stepFronts.size() +
+            p.levelList.size()) *
+               4 +
+           p.panelOff.size() * 24 +
+           (p.forwardTiles.size() + p.backwardBlocks.size() + p.forwardSubtractions.size() +
+            p.backwardSubtractions.size()) *
+               16 +
+           (p.forwardTriangles.size() + p.backwardTriangles.size() + 2 * p.forwardTriangleSums.size()) * 16 +
+           p.inverseItems.size() * 8 + p.blocks.size() * sizeof(oosg::AssemblyBlock) +
+           p.items.size() * sizeof(oosg::AssemblyItem) +
+           (p.tiles.size() + p.bigTiles.size()) * sizeof(oosg::UpdateTile) + p.itemStarts.size() * 4 +
+           p.extendItems.size() * sizeof(oosg::AssemblyItem);
+}
+
+uint64_t factorStorageBytes(const oosg_plan& p)
+{
+    return (static_cast<uint64_t>(std::max<int64_t>(p.panelOff[p.S], 1)) +
+            static_cast<uint64_t>(std::max<int64_t>(p.linvOff[p.S], 1)) + 2 * static_cast<uint64_t>(std::max(p.n, 1))) *
+           sizeof(double);
+}
+
+// The sizes of the vectors of the solves and refinements (16 right-hand sides), in the order reserveSolveBuffers
+// allocates them.
+std::vector<uint64_t> solveBufferBytes(const oosg_plan& p)
+{
+    const uint64_t vector = static_cast<uint64_t>(std::max(p.n, 1)) * 16 * sizeof(double);
+    return {vector,                                                                         // solveX
+            static_cast<uint64_t>(std::max<int64_t>(p.rowStart[p.S], 1)) * 16 * sizeof(double), // solveU
+            static_cast<uint64_t>(std::max<int64_t>(p.partialSlots, 1)) * 64 * 16 * sizeof(double), // solvePartials
+            3 * 16 * sizeof(unsigned long long),                                            // solveNorms
+            vector, vector, vector, vector, vector, vector,                                 // cgB .. cgQ
+            static_cast<uint64_t>(oosg::DOT_GROUPS) * 16 * sizeof(double),                  // cgPartial
+            16 * sizeof(double),                                                            // cgDots
+            static_cast<uint64_t>(std::max(p.S, 1)) * sizeof(int32_t),                      // pivotStatus
+            sizeof(oosg::PivotStatistics)};                                                 // pivotStats
+}
+
+// Everything a plan reserves: its arrays, the update matrices, one factor's storage, the solves' vectors.
+uint64_t planBytes(const oosg_plan& p)
+{
+    uint64_t bytes = structureBytes(p) + static_cast<uint64_t>(std::max<int64_t>(p.updateOff[p.S], 1)) * 8 +
+                     factorStorageBytes(p);
+    for (const uint64_t b : solveBufferBytes(p))
+        bytes += b;
+    return bytes;
+}
+
+void reserveSolveBuffers(oosg_plan& p)
+{
+    const std::vector<uint64_t> bytes = solveBufferBytes(p);
+    DeviceBuffer* buffers[] = {&p.solveX, &p.solveU, &p.solvePartials, &p.solveNorms, &p.cgB, &p.cgX,
+                               &p.cgR,    &p.cgZ,    &p.cgP,          &p.cgQ,         &p.cgPartial, &p.cgDots,
+                               &p.pivotStatus, &p.pivotStats};
+    for (size_t k = 0; k < bytes.size(); ++k)
+        ensureCounted(p, *buffers[k], bytes[k], "plan: solve vectors");
 }
 
 // ---- factorisation -------------------------------------------------------------------------------------------------
@@ -1296,8 +1493,10 @@ template <typename TP>
 void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t l0, int32_t l1, hipStream_t st);
 void ensureSolveBuffers(oosg_plan& p);
 
+// cancellable: launched one launch at a time (not captured), so the cancellation is polled between levels.
 template <typename T>
-void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled)
+void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled,
+               bool cancellable = false)
 {
     const hipStream_t st = p.stream;
     oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
@@ -1322,6 +1521,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     for (size_t li = 0; li < p.schedule.size(); ++li)
     {
         const Launch& l = p.schedule[li];
+        if (cancellable && li > 0 && l.level != p.schedule[li - 1].level)
+            pollCancellation(p);
         const bool concurrent = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] && !p.profiler.enabled;
         if (p.aheadLevel > 0 && !aheadIssued && l.level >= p.aheadLevel)
         {
@@ -1493,13 +1694,20 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
         if ((failedHere || failedBelow[s]) && p.parent[s] >= 0)
             failedBelow[p.parent[s]] = 1;
     }
+    // An FP32 factor's failures do not show the matrix singular: single precision may cause them.
+    const bool fp32 = f.precision == OOSG_FP32;
+    const std::string fp32Note = "\nthe factor was computed in FP32: the matrix is not shown singular; factorise it in "
+                                 "FP64";
     if (zeroCount > 0)
-        fail(OOSG_SINGULAR, "the stiffness matrix is singular: " + std::to_string(zeroCount) +
-                                " zero (or not finite) pivots met in the factorisation, a mechanism or an "
-                                "unconstrained DOF at:\n" +
-                                equationList(zeroPivots, zeroCount) +
-                                "constrain these DOFs (single-point constraints) or connect them to the structure; "
-                                "with a matrix that is meant to be indefinite, factorise it in indefinite mode");
+        failNaming(fp32 ? OOSG_PRECISION_INSUFFICIENT : OOSG_SINGULAR,
+                   "the stiffness matrix is singular: " + std::to_string(zeroCount) +
+                       " zero (or not finite) pivots met in the factorisation, a mechanism or an "
+                       "unconstrained DOF at:\n" +
+                       equationList(zeroPivots, zeroCount) +
+                       "constrain these DOFs (single-point constraints) or connect them to the structure; "
+                       "with a matrix that is meant to be indefinite, factorise it in indefinite mode" +
+                       (fp32 ? fp32Note : std::string()),
+                   zeroPivots);
     oosg_factor_info& info = f.info;
     const double maxRatio = options.max_ratio > 0 ? options.max_ratio : 1.0e7;
     info.positive = static_cast<int64_t>(stats.positive);
@@ -1519,26 +1727,35 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
         info.largest_ratio = p.n > 0 ? NAN : 0.0;
         info.largest_ratio_equation = p.n > 0 ? 0 : -1;
     }
-    if (info.issue_count > 0 && options.bailout >= 0)
+    if (info.issue_count == 0)
+        return;
+    // The failing pivots, in elimination order (the equations in order: a supernode's columns are consecutive), from
+    // the pivots on the host: the first OOSG_RECORDED_ISSUE_COUNT with their values (PivotIssue), as
+    // MultifrontalDriver::finish records them.
+    const std::vector<double>& d = hostDiagonal(f);
+    const double* reference = options.reference_diagonal;
+    std::vector<int32_t> equations;
+    for (int32_t j = 0; j < p.n && f.issues.size() < OOSG_RECORDED_ISSUE_COUNT; ++j)
     {
-        // The failing equations, for the message, from the pivots on the host.
-        const std::vector<double>& d = hostDiagonal(f);
-        const double* reference = options.reference_diagonal;
-        std::vector<int32_t> issues;
-        for (int32_t j = 0; j < p.n && issues.size() < 1000; ++j)
+        const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
+        const double ratio = diagonal / d[j];
+        const bool negative = ratio < 0.0 || d[j] < 0.0;
+        if (ratio > maxRatio || negative)
         {
-            const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
-            const double ratio = diagonal / d[j];
-            if (ratio > maxRatio || ratio < 0.0 || d[j] < 0.0)
-                issues.push_back(j);
+            f.issues.push_back({j, negative ? 1 : 0, diagonal, d[j], ratio});
+            equations.push_back(j);
         }
-        fail(OOSG_SINGULAR, "the stiffness matrix is nearly singular or not positive definite: " +
-                                std::to_string(info.issue_count) +
-                                " pivots fail the factor diagonal ratio test (ratio of the matrix diagonal to the "
-                                "factor diagonal above " +
-                                std::to_string(maxRatio) + ", or negative):\n" +
-                                equationList(issues, static_cast<size_t>(info.issue_count)));
     }
+    if (options.bailout >= 0)
+        failNaming(fp32 ? OOSG_PRECISION_INSUFFICIENT : OOSG_SINGULAR,
+                   "the stiffness matrix is nearly singular or not positive definite: " +
+                       std::to_string(info.issue_count) +
+                       " pivots fail the factor diagonal ratio test (ratio of the matrix diagonal to the "
+                       "factor diagonal above " +
+                       std::to_string(maxRatio) + ", or negative):\n" +
+                       equationList(equations, static_cast<size_t>(info.issue_count)) +
+                       (fp32 ? fp32Note : std::string()),
+                   equations);
 }
 
 // ---- solves --------------------------------------------------------------------------------------------------------
@@ -1670,12 +1887,12 @@ void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
             cOff[q] = cTotal;
             cTotal += cand[s];
         }
-        ensure(p.pivotFronts[l], static_cast<size_t>(std::max<int64_t>(fTotal, 1)) * sizeof(double),
+        ensureCounted(p, p.pivotFronts[l], static_cast<size_t>(std::max<int64_t>(fTotal, 1)) * sizeof(double),
                "factor: pivoted fronts");
-        ensure(p.pivotW, static_cast<size_t>(std::max<int64_t>(wTotal, 1)) * sizeof(double), "factor: W");
-        ensure(p.pivotPerm, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(int32_t), "factor: pivots");
-        ensure(p.pivotDiag, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
-        ensure(p.pivotSub, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
+        ensureCounted(p, p.pivotW, static_cast<size_t>(std::max<int64_t>(wTotal, 1)) * sizeof(double), "factor: W");
+        ensureCounted(p, p.pivotPerm, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(int32_t), "factor: pivots");
+        ensureCounted(p, p.pivotDiag, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
+        ensureCounted(p, p.pivotSub, static_cast<size_t>(std::max<int64_t>(cTotal, 1)) * sizeof(double), "factor: pivots");
         double* F0 = p.pivotFronts[l].as<double>();
         double* W0 = p.pivotW.as<double>();
         std::vector<oosg::PivotFront> fronts(count);
@@ -1873,7 +2090,8 @@ void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
     std::vector<int64_t> panelStart(S + 1, 0);
     for (int32_t s = 0; s < S; ++s)
         panelStart[s + 1] = panelStart[s] + int64_t(pf->order[s]) * pf->eliminated[s];
-    f.panels.allocate(static_cast<size_t>(std::max<int64_t>(panelStart[S], 1)) * sizeof(double), "factor: panels");
+    allocateCounted(p, f.panels, static_cast<size_t>(std::max<int64_t>(panelStart[S], 1)) * sizeof(double),
+                    "factor: panels");
     std::vector<oosg::CopyRange> ranges;
     for (int32_t s = 0; s < S; ++s)
         for (int64_t q0 = 0; q0 < panelStart[s + 1] - panelStart[s]; q0 += 65536)
@@ -1963,7 +2181,15 @@ void factorisePivoted(oosg_plan& p, const oosg_matrix& m, oosg_factor& f)
     deriveLevels(*d);
     uploadSweepPlan(*d);
     // The inverses of the unit lower triangles: the diagonal blocks, then the wide triangles' whole inverses.
-    f.linv.allocate(static_cast<size_t>(std::max<int64_t>(d->linvOff[S], 1)) * sizeof(double), "factor: inverses");
+    allocateCounted(p, f.linv, static_cast<size_t>(std::max<int64_t>(d->linvOff[S], 1)) * sizeof(double),
+                    "factor: inverses");
+    // The sweeps' vectors of the pivoted structure, counted in the plan's budget.
+    ensureCounted(p, d->solveX, static_cast<size_t>(std::max(n, 1)) * 16 * sizeof(double), "factor: solve vectors");
+    ensureCounted(p, d->solveU, static_cast<size_t>(std::max<int64_t>(d->rowStart[S], 1)) * 16 * sizeof(double),
+                  "factor: solve vectors");
+    ensureCounted(p, d->solvePartials, static_cast<size_t>(std::max<int64_t>(d->partialSlots, 1)) * 64 * 16 *
+                                           sizeof(double),
+                  "factor: solve vectors");
     std::vector<int2> blockItems;
     for (int32_t s = 0; s < S; ++s)
         for (int32_t J = 0; J * oosg::PANEL_BLOCK < d->ncol[s]; ++J)
@@ -2140,7 +2366,6 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
         // In pivot order: X permuted into the structure's own X, swept, and permuted back.
         oosg_plan& d = *f.pivoted->structure;
         const int64_t n = p.n;
-        ensureSolveBuffers(d);
         const int32_t* perm = f.pivoted->perm.as<int32_t>();
         check(oosg::launchPermuteRows(p.solveX.as<double>(), d.solveX.as<double>(), perm, n, k, false, p.stream),
               "solve: pivot order");
@@ -2180,6 +2405,8 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
     for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
     {
+        if (c0 > 0)
+            pollCancellation(p);
         const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
         double* block = b + c0 * ld;
         check(oosg::launchToRowMajor(block, ld, p.solveX.as<double>(), n, k, scale, p.stream), "solve: layout");
@@ -2226,8 +2453,6 @@ void valueStatistics(oosg_matrix& m)
 // Row-major n × k vectors (k <= 16) of the conjugate-gradient solve.
 std::vector<double> columnDots(oosg_plan& p, const double* a, const double* b, int k)
 {
-    ensure(p.cgPartial, static_cast<size_t>(oosg::DOT_GROUPS) * 16 * sizeof(double), "solve: partial sums");
-    ensure(p.cgDots, 16 * sizeof(double), "solve: dot products");
     check(oosg::launchColumnDots(a, b, p.n, k, p.cgPartial.as<double>(), p.cgDots.as<double>(), p.stream),
           "solve: dot products");
     std::vector<double> out(16);
@@ -2241,7 +2466,6 @@ std::vector<double> columnDots(oosg_plan& p, const double* a, const double* b, i
 // Column maxima |v| of row-major n × k vectors into slot `slot` (of 3) of the plan's norms, on the stream.
 void startColumnMax(oosg_plan& p, const double* v, int k, int slot)
 {
-    ensure(p.solveNorms, 3 * 16 * sizeof(unsigned long long), "solve: norms");
     unsigned long long* out = p.solveNorms.as<unsigned long long>() + 16 * slot;
     check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(out), 32, 0, p.stream), "solve: norms");
     check(oosg::launchColumnMaxRows(v, p.n, k, out, p.stream), "solve: norms");
@@ -2269,14 +2493,10 @@ std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
 }
 
 // The solve buffers of a plan, sized for 16 right-hand sides.
+// (Reserved with the plan: nothing is allocated here unless a buffer was given back.)
 void ensureSolveBuffers(oosg_plan& p)
 {
-    const int64_t n = p.n;
-    ensure(p.solveX, static_cast<size_t>(n * 16) * sizeof(double), "solve: right-hand sides");
-    ensure(p.solveU, static_cast<size_t>(std::max<int64_t>(p.rowStart[p.S], 1) * 16) * sizeof(double),
-           "solve: update vectors");
-    ensure(p.solvePartials, static_cast<size_t>(std::max<int64_t>(p.partialSlots, 1) * 64 * 16) * sizeof(double),
-           "solve: partial sums");
+    reserveSolveBuffers(p);
 }
 
 // out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides. With
@@ -2310,13 +2530,6 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
     oosg_plan& p = *f.plan;
     const hipStream_t st = p.stream;
     const int64_t n = p.n;
-    const size_t bytes = static_cast<size_t>(n) * 16 * sizeof(double);
-    ensure(p.cgB, bytes, "solve: conjugate gradients");
-    ensure(p.cgX, bytes, "solve: conjugate gradients");
-    ensure(p.cgR, bytes, "solve: conjugate gradients");
-    ensure(p.cgZ, bytes, "solve: conjugate gradients");
-    ensure(p.cgP, bytes, "solve: conjugate gradients");
-    ensure(p.cgQ, bytes, "solve: conjugate gradients");
     double* B = p.cgB.as<double>();
     double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
@@ -2424,11 +2637,6 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     oosg_plan& p = *f.plan;
     const hipStream_t st = p.stream;
     const int64_t n = p.n;
-    const size_t bytes = static_cast<size_t>(n) * 16 * sizeof(double);
-    ensure(p.cgB, bytes, "solve: refinement");
-    ensure(p.cgX, bytes, "solve: refinement");
-    ensure(p.cgR, bytes, "solve: refinement");
-    ensure(p.cgZ, bytes, "solve: refinement");
     double* B = p.cgB.as<double>();
     double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
@@ -2493,6 +2701,79 @@ const char* oosg_last_error(void)
     return g_lastError.c_str();
 }
 
+oosg_status oosg_last_error_equations(int32_t* equations, int64_t capacity, int64_t* count)
+{
+    const std::vector<int32_t> kept = g_lastEquations; // the calls below must not replace them
+    if (count != nullptr)
+        *count = static_cast<int64_t>(kept.size());
+    if (equations != nullptr && capacity > 0)
+        std::copy(kept.begin(), kept.begin() + std::min<int64_t>(capacity, static_cast<int64_t>(kept.size())),
+                  equations);
+    return OOSG_OK;
+}
+
+oosg_status oosg_last_error_shortfall(oosg_memory_shortfall* shortfall)
+{
+    if (shortfall == nullptr)
+        return OOSG_INVALID_ARGUMENT;
+    *shortfall = g_lastShortfall;
+    return OOSG_OK;
+}
+
+oosg_status oosg_estimate_memory_bytes(const oosg_symbolic_desc* symbolic, int32_t pivoting, uint64_t* bytes)
+{
+    return guarded(
+        [&]
+        {
+            if (symbolic == nullptr || bytes == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "estimate: null argument");
+            if (pivoting != OOSG_POSITIVE_DEFINITE && pivoting != OOSG_INDEFINITE)
+                fail(OOSG_INVALID_ARGUMENT, "estimate: unknown pivoting mode");
+            // The plan's host part only: the structure and the schedule need no device.
+            oosg_plan p;
+            buildStructure(p, *symbolic);
+            buildSchedule(p);
+            *bytes = planBytes(p);
+        });
+}
+
+oosg_status oosg_plan_memory(const oosg_plan* plan, uint64_t* reserved_bytes, uint64_t* budget_bytes)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            if (reserved_bytes != nullptr)
+                *reserved_bytes = plan->reserved;
+            if (budget_bytes != nullptr)
+                *budget_bytes = plan->budget;
+        });
+}
+
+oosg_status oosg_plan_set_stream(oosg_plan* plan, void* stream)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            plan->callerStream = static_cast<hipStream_t>(stream);
+        });
+}
+
+oosg_status oosg_plan_set_cancellation(oosg_plan* plan, oosg_cancel_fn cancelled, void* context)
+{
+    return guarded(
+        [&]
+        {
+            if (plan == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
+            plan->cancelled = cancelled;
+            plan->cancelContext = context;
+        });
+}
+
 oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device, uint64_t hbm_budget_bytes,
                              oosg_plan** out)
 {
@@ -2511,31 +2792,28 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
             setDevice(p.get());
             buildStructure(*p, *symbolic);
             buildSchedule(*p);
-            // The memory of every factorisation of this plan, reserved before any numeric work (MemoryBudgetError).
+            p->reserved = structureBytes(*p); // uploaded uncounted below, held while the plan lives
+            // The memory of a factorisation and of its solves, reserved before any numeric work (MemoryBudgetError).
             size_t freeBytes = 0, totalBytes = 0;
             check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
             p->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : freeBytes;
-            const uint64_t structure = (p->first.size() * 3 + p->rows.size() * 2 + p->children.size() +
-                                        p->smallList.size() + p->stepFronts.size() + p->levelList.size()) *
-                                           4 +
-                                       p->panelOff.size() * 24 +
-                                       (p->forwardTiles.size() + p->backwardBlocks.size() + p->forwardSubtractions.size() +
-                                        p->backwardSubtractions.size()) * 16 +
-                                       (p->forwardTriangles.size() + p->backwardTriangles.size() +
-                                        2 * p->forwardTriangleSums.size()) * 16 +
-                                       p->inverseItems.size() * 8 +
-                                       p->blocks.size() * sizeof(oosg::AssemblyBlock) +
-                                       p->items.size() * sizeof(oosg::AssemblyItem) +
-                                       p->tiles.size() * sizeof(oosg::UpdateTile);
-            const uint64_t values =
-                static_cast<uint64_t>(p->updateOff[p->S] + p->panelOff[p->S] + p->linvOff[p->S]) * sizeof(double);
-            if (structure + values > p->budget)
-                fail(OOSG_OUT_OF_MEMORY,
-                     "plan: the factorisation needs " + std::to_string((structure + values) >> 20) +
-                         " MiB of device memory (panels " +
-                         std::to_string((static_cast<uint64_t>(p->panelOff[p->S]) * 8) >> 20) + " MiB, update matrices " +
-                         std::to_string((static_cast<uint64_t>(p->updateOff[p->S]) * 8) >> 20) + " MiB), " +
-                         std::to_string(p->budget >> 20) + " MiB are available");
+            const uint64_t need = planBytes(*p);
+            if (need > p->budget)
+            {
+                const uint64_t panels = static_cast<uint64_t>(p->panelOff[p->S]) * 8;
+                const uint64_t updates = static_cast<uint64_t>(p->updateOff[p->S]) * 8;
+                Failure f{OOSG_OUT_OF_MEMORY,
+                          "plan: the factorisation and its solves need " + mebibytes(need) +
+                              " of device memory (panels " + mebibytes(panels) + ", update matrices " +
+                              mebibytes(updates) + ", inverses, pivots and the solves' vectors " +
+                              mebibytes(need - panels - updates) + "); the budget is " + mebibytes(p->budget),
+                          {},
+                          {}};
+                f.shortfall.requested_bytes = need;
+                f.shortfall.available_bytes = p->budget;
+                f.shortfall.limit_bytes = p->budget;
+                throw f;
+            }
             check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
             p->lanes[LANE_CRITICAL] = p->stream;
             for (int lane = 1; lane < LANES; ++lane)
@@ -2546,6 +2824,8 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
             uploadPlan(*p);
+            reserveSolveBuffers(*p);
+            p->storagePool.push_back(takeStorage(*p));
             *out = p.release();
         });
 }
@@ -2978,12 +3258,21 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
                 fail(OOSG_INVALID_ARGUMENT, "autotune: unknown precision");
             setDevice(plan);
             oosg_plan& p = *plan;
-            const size_t bytes = valueBytes(precision);
-            DeviceBuffer panels, linv, diag, zero;
-            panels.allocate(static_cast<size_t>(std::max<int64_t>(p.panelOff[p.S], 1)) * bytes, "autotune: panels");
-            linv.allocate(static_cast<size_t>(std::max<int64_t>(p.linvOff[p.S], 1)) * bytes, "autotune: inverses");
-            diag.allocate(static_cast<size_t>(std::max(p.n, 1)) * sizeof(double), "autotune: pivots");
-            zero.allocate(static_cast<size_t>(std::max(p.S, 1)) * sizeof(int32_t), "autotune: pivot status");
+            // A factor's storage of the plan's (given back at the end), and the plan's pivot status.
+            struct Lent
+            {
+                oosg_plan& p;
+                FactorStorage storage;
+                ~Lent()
+                {
+                    if (p.storagePool.size() < 2)
+                        p.storagePool.push_back(std::move(storage));
+                }
+            } lent{p, takeStorage(p)};
+            DeviceBuffer& panels = lent.storage.panels;
+            DeviceBuffer& linv = lent.storage.linv;
+            DeviceBuffer& diag = lent.storage.diag;
+            DeviceBuffer& zero = p.pivotStatus;
             const int32_t keep = p.tilePolicy;
             const bool profiling = p.profiler.enabled;
             p.profiler.enabled = false;
@@ -3246,6 +3535,13 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
                 m->normInf = std::max(m->normInf, sum);
             }
             const hipStream_t st = plan->stream;
+            // The matrix's device arrays are counted in the plan's budget (MemoryBudgetError before the upload).
+            const uint64_t matrixBytes = m->offsets.size() * 8 + m->rows.size() * 4 + values.size() * 8 +
+                                         position.size() * 4 + column.size() * 4 + m->scale.size() * 8 +
+                                         m->diagonal.size() * 8 + rowCount.size() * 8 + csrCols.size() * 4 +
+                                         csrValues.size() * 8 + (static_cast<uint64_t>(n) + 1) * 4;
+            if (plan->reserved + matrixBytes > plan->budget)
+                failBudget(*plan, matrixBytes, "the matrix (its lower and full forms, front positions)");
             m->dOffsets = upload(m->offsets, st, "matrix: offsets");
             m->dRows = upload(m->rows, st, "matrix: rows");
             m->dValues = upload(values, st, "matrix: values");
@@ -3274,6 +3570,12 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             }
             m->dCsrGroups = upload(m->csrGroups, st, "matrix: row groups");
             check(hipStreamSynchronize(st), "matrix: upload");
+            for (DeviceBuffer* b : {&m->dOffsets, &m->dRows, &m->dValues, &m->dPosition, &m->dColumn, &m->dScale,
+                                    &m->dDiagonal, &m->dCsrOffsets, &m->dCsrCols, &m->dCsrValues, &m->dCsrGroups})
+            {
+                b->account = &plan->reserved;
+                plan->reserved += b->bytes;
+            }
             holdPlan(plan);
             *out = m.release();
         });
@@ -3399,39 +3701,30 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: unknown precision");
             setDevice(plan);
+            pollCancellation(*plan);
             auto f = std::make_unique<oosg_factor>();
             f->plan = plan;
             f->precision = opts.precision;
-            const size_t panelBytes = static_cast<size_t>(plan->panelOff[plan->S]) * valueBytes(opts.precision);
-            if (plan->panelCache[opts.precision].ptr != nullptr && plan->panelCache[opts.precision].bytes >= panelBytes)
-                f->panels = std::move(plan->panelCache[opts.precision]);
-            else
-                f->panels.allocate(panelBytes, "factor: panels");
-            const size_t linvBytes = static_cast<size_t>(plan->linvOff[plan->S]) * valueBytes(opts.precision);
-            if (plan->linvCache[opts.precision].ptr != nullptr && plan->linvCache[opts.precision].bytes >= linvBytes)
-                f->linv = std::move(plan->linvCache[opts.precision]);
-            else
-                f->linv.allocate(linvBytes, "factor: inverses of the diagonal blocks");
+            {
+                FactorStorage storage = takeStorage(*plan);
+                f->panels = std::move(storage.panels);
+                f->linv = std::move(storage.linv);
+                f->dDiag = std::move(storage.diag);
+                f->scale = std::move(storage.scale);
+            }
             DeviceBuffer& diag = f->dDiag;
-            const size_t diagBytes = static_cast<size_t>(plan->n) * sizeof(double);
-            if (plan->diagCache.ptr != nullptr && plan->diagCache.bytes >= diagBytes)
-                diag = std::move(plan->diagCache);
-            else
-                diag.allocate(diagBytes, "factor: pivots");
-            ensure(plan->pivotStatus, static_cast<size_t>(std::max(plan->S, 1)) * sizeof(int32_t),
-                   "factor: pivot status");
             DeviceBuffer& zero = plan->pivotStatus;
             const hipStream_t st = plan->stream;
             check(hipEventRecord(plan->start, st), "hipEventRecord");
-            const auto body = [&]
+            const auto body = [&](bool cancellable)
             {
                 check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
                 if (opts.precision == OOSG_FP32)
                     runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
-                                     zero.as<int32_t>(), true);
+                                     zero.as<int32_t>(), true, cancellable);
                 else
                     runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(),
-                                      diag.as<double>(), zero.as<int32_t>(), false);
+                                      diag.as<double>(), zero.as<int32_t>(), false, cancellable);
             };
             if (plan->graphs && !plan->profiler.enabled && !plan->eager[opts.precision])
                 plan->factorGraph[opts.precision].run(
@@ -3439,20 +3732,17 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                     {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,
                      matrix->dColumn.ptr, matrix->dOffsets.ptr, matrix->dRows.ptr, matrix->dScale.ptr,
                      plan->updates.ptr, reinterpret_cast<const void*>(static_cast<intptr_t>(plan->concurrent[opts.precision]))},
-                    body);
+                    [&] { body(false); });
             else
-                body();
+                body(true);
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
             if (plan->aheadLevel > 0)
                 check(hipStreamWaitEvent(st, plan->aheadEvents[LANES], 0), "factor: sweep ahead"); // its join
             if (opts.precision == OOSG_FP32)
-            {
                 // The equations' scales the pivots are divided by, kept with the factor.
-                f->scale.allocate(static_cast<size_t>(std::max(plan->n, 1)) * sizeof(double), "factor: scales");
                 check(hipMemcpyAsync(f->scale.ptr, matrix->dScale.ptr, static_cast<size_t>(plan->n) * sizeof(double),
                                      hipMemcpyDeviceToDevice, st),
                       "factor: scales");
-            }
             // The ratio test's statistics over the pivots, on the device.
             std::vector<double> referenceHost;
             DeviceBuffer reference;
@@ -3461,7 +3751,6 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 referenceHost.assign(opts.reference_diagonal, opts.reference_diagonal + plan->n);
                 reference = upload(referenceHost, st, "factor: reference diagonal");
             }
-            ensure(plan->pivotStats, sizeof(oosg::PivotStatistics), "factor: pivot statistics");
             check(oosg::launchPivotStatistics(diag.as<double>(),
                                               opts.precision == OOSG_FP32 ? f->scale.as<double>() : nullptr,
                                               matrix->dDiagonal.as<double>(),
@@ -3477,6 +3766,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                   "factor: pivot statistics");
             check(hipStreamSynchronize(st), "factor");
             plan->profiler.collect();
+            pollCancellation(*plan);
             float ms = 0.0f;
             check(hipEventElapsedTime(&ms, plan->start, plan->stop), "hipEventElapsedTime");
             f->info.factor_seconds = ms * 1e-3;
@@ -3503,6 +3793,22 @@ void oosg_factor_destroy(oosg_factor* f)
     delete f; // its buffers go back to the plan's caches
     releasePlan(plan);
     (void)hipGetLastError();
+}
+
+oosg_status oosg_factor_issues(const oosg_factor* f, oosg_pivot_issue* issues, int64_t capacity, int64_t* recorded)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "issues: null factor");
+            if (recorded != nullptr)
+                *recorded = static_cast<int64_t>(f->issues.size());
+            if (issues != nullptr && capacity > 0)
+                std::copy(f->issues.begin(),
+                          f->issues.begin() + std::min<int64_t>(capacity, static_cast<int64_t>(f->issues.size())),
+                          issues);
+        });
 }
 
 oosg_status oosg_factor_info_get(const oosg_factor* f, oosg_factor_info* info)
@@ -3638,6 +3944,7 @@ oosg_status oosg_solve_device(const oosg_factor* f, double* b, int64_t ld, int64
             if (nrhs == 0 || f->plan->n == 0)
                 return;
             setDevice(f->plan);
+            followCaller(*f->plan);
             solveDevice(*f, b, ld, nrhs);
             check(hipStreamSynchronize(f->plan->stream), "solve");
             f->plan->profiler.collect();
@@ -3657,16 +3964,22 @@ oosg_status oosg_solve(const oosg_factor* f, double* b, int64_t ld, int64_t nrhs
             if (nrhs == 0 || p.n == 0)
                 return;
             setDevice(&p);
-            DeviceBuffer db;
-            db.allocate(static_cast<size_t>(p.n) * nrhs * sizeof(double), "solve: right-hand sides");
-            check(hipMemcpy2DAsync(db.ptr, p.n * sizeof(double), b, ld * sizeof(double), p.n * sizeof(double), nrhs,
-                                   hipMemcpyHostToDevice, p.stream),
-                  "solve: upload");
-            solveDevice(*f, db.as<double>(), p.n, nrhs);
-            check(hipMemcpy2DAsync(b, ld * sizeof(double), db.ptr, p.n * sizeof(double), p.n * sizeof(double), nrhs,
-                                   hipMemcpyDeviceToHost, p.stream),
-                  "solve: download");
-            check(hipStreamSynchronize(p.stream), "solve");
+            // Tiles of 16 columns through the plan's refinement vector (column-major n × 16 there): no allocation.
+            double* staging = p.cgB.as<double>();
+            for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
+            {
+                if (c0 > 0)
+                    pollCancellation(p);
+                const int64_t k = std::min<int64_t>(16, nrhs - c0);
+                check(hipMemcpy2DAsync(staging, p.n * sizeof(double), b + c0 * ld, ld * sizeof(double),
+                                       p.n * sizeof(double), k, hipMemcpyHostToDevice, p.stream),
+                      "solve: upload");
+                solveDevice(*f, staging, p.n, k);
+                check(hipMemcpy2DAsync(b + c0 * ld, ld * sizeof(double), staging, p.n * sizeof(double),
+                                       p.n * sizeof(double), k, hipMemcpyDeviceToHost, p.stream),
+                      "solve: download");
+                check(hipStreamSynchronize(p.stream), "solve");
+            }
             p.profiler.collect();
         });
 }
@@ -3678,9 +3991,10 @@ oosg_status oosg_matrix_multiply_device(const oosg_matrix* m, const double* x, d
         {
             if (m == nullptr || x == nullptr || y == nullptr)
                 fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::multiply: null argument");
-            if (ld < std::max<int64_t>(m->n, 1))
+            if (ld < std::max<int64_t>(m->n, 1) || nrhs < 0)
                 fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::multiply: the leading dimension is below the row count");
             setDevice(m->plan);
+            followCaller(*m->plan);
             check(oosg::launchResidual(m->dCsrOffsets.as<int64_t>(), m->dCsrCols.as<int32_t>(),
                                        m->dCsrValues.as<double>(), x, nullptr, y, ld, m->n, static_cast<int>(nrhs),
                                        m->plan->stream),
@@ -3688,6 +4002,36 @@ oosg_status oosg_matrix_multiply_device(const oosg_matrix* m, const double* x, d
             check(hipStreamSynchronize(m->plan->stream), "multiply");
         });
 }
+
+namespace
+{
+
+// The refined solve of columns [c0, nrhs) by tiles of 16 (the first tile's first forward sweep possibly done below
+// `from`), the report merged over the tiles: the most solves, the largest backward error, all converged.
+void refineTiles(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int64_t nrhs,
+                 double target, int32_t maxSteps, oosg_solve_report& report, int32_t from)
+{
+    oosg_plan& p = *f.plan;
+    report = {};
+    report.converged = 1;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
+    {
+        if (c0 > 0)
+            pollCancellation(p);
+        const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
+        oosg_solve_report tile{};
+        if (f.precision == OOSG_FP32)
+            solvePcg(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile, c0 == 0 ? from : 0);
+        else
+            solveRefinement(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile,
+                            c0 == 0 ? from : 0);
+        report.steps = std::max(report.steps, tile.steps);
+        report.backward_error = std::max(report.backward_error, tile.backward_error);
+        report.converged = report.converged && tile.converged;
+    }
+}
+
+} // namespace
 
 oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m, const double* b, double* x,
                                       int64_t ld, int64_t nrhs, double target, int32_t max_steps,
@@ -3702,16 +4046,13 @@ oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m
             oosg_plan& p = *f->plan;
             if (m->plan != &p)
                 fail(OOSG_INVALID_ARGUMENT, "refined solve: the matrix was made for another plan");
-            if (ld < std::max<int64_t>(p.n, 1) || nrhs < 1 || nrhs > 16)
-                fail(OOSG_INVALID_ARGUMENT, "refined solve: 1 to 16 right-hand sides, leading dimension >= n");
+            if (ld < std::max<int64_t>(p.n, 1) || nrhs < 1)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: at least one right-hand side, leading dimension >= n");
             setDevice(&p);
+            followCaller(p);
             const hipStream_t st = p.stream;
-            const int k = static_cast<int>(nrhs);
             const auto t0 = std::chrono::steady_clock::now();
-            if (f->precision == OOSG_FP32)
-                solvePcg(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
-            else
-                solveRefinement(*f, *m, b, x, ld, k, target, std::max(1, max_steps), local);
+            refineTiles(*f, *m, b, x, ld, nrhs, target, max_steps, local, 0);
             check(hipStreamSynchronize(st), "refined solve");
             p.profiler.collect();
             local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -3738,9 +4079,9 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
             *factor = nullptr;
             if (matrix->plan != plan)
                 fail(OOSG_INVALID_ARGUMENT, "factorise and solve: the matrix was made for another plan");
-            if (ld < std::max<int64_t>(plan->n, 1) || nrhs < 1 || nrhs > 16)
+            if (ld < std::max<int64_t>(plan->n, 1) || nrhs < 1)
                 fail(OOSG_INVALID_ARGUMENT,
-                     "factorise and solve: 1 to 16 right-hand sides, leading dimension >= n");
+                     "factorise and solve: at least one right-hand side, leading dimension >= n");
             oosg_factor_options opts{};
             opts.max_ratio = 1.0e7;
             if (options != nullptr)
@@ -3751,8 +4092,9 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
                                (opts.precision == OOSG_FP64 || fp32) && plan->overlapLevel > 0 &&
                                !plan->profiler.enabled && (!plan->graphs || plan->eager[opts.precision]);
             setDevice(plan);
+            followCaller(*plan);
             const hipStream_t st = plan->stream;
-            const int k = static_cast<int>(nrhs);
+            const int k = static_cast<int>(std::min<int64_t>(nrhs, 16));
             const int64_t n = plan->n;
             if (ahead)
             {
@@ -3763,9 +4105,8 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
                     for (hipEvent_t& e : plan->aheadEvents)
                         check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
                 }
-                // b into the refinement's row-major copy, and scaled (FP32) into the sweeps' vectors.
-                ensureSolveBuffers(*plan);
-                ensure(plan->cgB, static_cast<size_t>(n) * 16 * sizeof(double), "solve: refinement");
+                // The first tile of b into the refinement's row-major copy, and scaled (FP32) into the sweeps'
+                // vectors.
                 check(oosg::launchToRowMajor(b, ld, plan->cgB.as<double>(), n, k, nullptr, st), "solve: layout");
                 check(oosg::launchScaleRows(plan->cgB.as<double>(), plan->solveX.as<double>(),
                                             fp32 ? matrix->dScale.as<double>() : nullptr, n, k, st),
@@ -3781,12 +4122,8 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
             factoriseInto(plan, matrix, &opts, factor);
             plan->aheadLevel = -1;
             const oosg_factor& f = **factor;
-            const int32_t from = ahead ? plan->overlapLevel : 0;
             const auto t0 = std::chrono::steady_clock::now();
-            if (fp32)
-                solvePcg(f, *matrix, b, x, ld, k, target, std::max(1, max_steps), local, from);
-            else
-                solveRefinement(f, *matrix, b, x, ld, k, target, std::max(1, max_steps), local, from);
+            refineTiles(f, *matrix, b, x, ld, nrhs, target, max_steps, local, ahead ? plan->overlapLevel : 0);
             check(hipStreamSynchronize(st), "factorise and solve");
             plan->profiler.collect();
             local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

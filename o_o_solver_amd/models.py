@@ -35,9 +35,13 @@ class Workload:
     matrix: SymmetricCsc = None
 
     def __del__(self):
-        if getattr(self, "handle", None):
-            _native.models().oosm_free(self.handle)
-            self.handle = None
+        handle = getattr(self, "handle", None)
+        self.handle = None
+        if handle and _native is not None and getattr(_native, "models", None) is not None:
+            try:
+                _native.models().oosm_free(handle)
+            except (AttributeError, TypeError, ImportError):
+                pass  # interpreter shutdown: the library's module state is already gone
 
     @property
     def n(self):

@@ -153,3 +153,17 @@ def ref_factor_solve(handle, matrix, b=None, kernel=0, threads=1, pivoting=0, ma
                               pivoting, max_ratio, bailout, P(b), nrhs, P(inertia), P(counts), P(times), P(panels),
                               P(diag), err, 4096)
     return rc, err.value.decode(errors="replace"), inertia, counts, times, panels, diag
+
+
+def backward_error(matrix, x, b):
+    """max over columns of ‖b − A·x‖∞ / (‖A‖∞·‖x‖∞ + ‖b‖∞), the products by the oracle's SpMV (the definition the
+    refined solves stop on, computed on the host, independent of the device's own figure)."""
+    x = np.asarray(x, dtype=np.float64).reshape(matrix.n, -1)
+    b = np.asarray(b, dtype=np.float64).reshape(matrix.n, -1)
+    norm = matrix.norm_inf()
+    eta = 0.0
+    for h in range(x.shape[1]):
+        r = b[:, h] - oracle_spmv(matrix, x[:, h])
+        denominator = norm * np.max(np.abs(x[:, h])) + np.max(np.abs(b[:, h]))
+        eta = max(eta, float(np.max(np.abs(r)) / denominator) if denominator > 0 else float(np.max(np.abs(r))))
+    return eta

@@ -8,6 +8,10 @@ Fixtures (numpy .npz, no pickles):
   plate8.npz     8 × 8 clamped shell plate (jitter 0.2, thickness log-uniform [1, 4] mm, seed 11), the order of this
                  project's nested dissection; the symbolic factor, the CSC in factor order, two N(0, 1) load columns,
                  the reference's solution (reference kernel family), D per equation and the inertia.
+  c1plate24.npz  24 × 24 clamped shell plate of the statics configuration's kind (1 m square, t = 2 mm constant, no
+                 jitter, E = 70 GPa, ν = 0.3, uniform pressure 1 kPa as consistent nodal forces, one load case), the
+                 order of this project's nested dissection: the symbolic factor, the CSC, the load, the reference's
+                 solution, D and inertia, and the equation of the centre node's deflection w.
   grid20.npz     20 × 20 scalar grid, −1 couplings, the reference's minimum-degree order; diagonals 4.5 (positive
                  definite) and 3.3 (indefinite): the reference's inertia of each (positive definite and Bunch-Kaufman
                  factorisations) and its solutions of the right-hand sides (7i + 3j) mod 11 − 5
@@ -49,6 +53,31 @@ def plate8():
                         values=m.values, b=b, x=x, diag=diag, inertia=inertia, flops=flops, **symbolic_arrays(sym))
 
 
+def centre_equation(w, sym):
+    """The equation of the w DOF (the third of six) of the node nearest the plate's centre."""
+    xy = w.coordinates.reshape(-1, 3)[:, :2]
+    node = int(np.argmin(np.sum((xy - xy.mean(axis=0)) ** 2, axis=1)))
+    position = int(np.flatnonzero(sym.node_order == node)[0])
+    return position * w.dof_per_node + 2
+
+
+def c1plate24():
+    w = models.plate(24, thickness=(2e-3, 2e-3), thickness_mode="constant", pressure=1e3)
+    sym = w.analyse()
+    m = w.matrix
+    b = w.loads("pressure", nrhs=1)
+    x = b.copy(order="F")
+    h, sizes, flops, arr = _oracle.ref_symbolic(w.graph_offsets, w.adjacency, w.weights, order=sym.node_order.copy())
+    try:
+        rc, msg, inertia, counts, times, panels, diag = _oracle.ref_factor_solve(h, m, b=x, kernel=0)
+    finally:
+        _oracle.reference().ref_sym_free(h)
+    assert rc == 0, msg
+    np.savez_compressed(os.path.join(HERE, "c1plate24.npz"), n=m.n, offsets=m.offsets, csc_rows=m.rows,
+                        values=m.values, b=b, x=x, diag=diag, inertia=inertia, flops=flops,
+                        centre=centre_equation(w, sym), **symbolic_arrays(sym))
+
+
 def grid20():
     g = grid_graph(20)
     h, sizes, flops, arr = _oracle.ref_symbolic(g[0], g[1], g[2], method=1)
@@ -80,6 +109,7 @@ if __name__ == "__main__":
     if _oracle.reference() is None:
         sys.exit("the compiled reference is not built: python oracle/build_ref.py")
     plate8()
+    c1plate24()
     grid20()
     for name in sorted(os.listdir(HERE)):
         if name.endswith(".npz"):

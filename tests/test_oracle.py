@@ -2,6 +2,7 @@
 and checked against closed forms, reconstructions and the known answers of the reference's own checks."""
 
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -9,7 +10,9 @@ import pytest
 import _oracle
 from _oracle import P
 from _problems import chain_graph, graph_matrix, grid_graph, random_spd
-from o_o_solver_amd import SymbolicFactor, models, nested_dissection
+from o_o_solver_amd import SymbolicFactor, SymmetricCsc, models, nested_dissection
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _split(front, c):
@@ -232,3 +235,20 @@ def test_solve_manufactured_solution(plate_problem):
     r = b - dense @ x
     eta = np.max(np.abs(r)) / (np.abs(dense).sum(1).max() * np.abs(x).max() + np.abs(b).max())
     assert eta < 1e-15
+
+
+def test_oracle_solves_the_c1_fixture_and_meets_the_closed_form():
+    # The clamped 24 × 24 plate of constant thickness under 1 kPa (tests/golden/c1plate24.npz, the compiled
+    # reference's solution): the oracle's factor and solve give the reference's solution, and the centre deflection is
+    # the clamped-plate closed form w_c = 0.00126532·q·a⁴/D_b within 0.5 % (a coarse mesh of the model).
+    g = np.load(os.path.join(GOLDEN, "c1plate24.npz"))
+    sym = SymbolicFactor(int(g["n"]), g["first_column"], g["column_count"], g["parent"], g["row_start"], g["rows"],
+                         node_order=g["node_order"])
+    m = SymmetricCsc(int(g["n"]), g["offsets"], g["csc_rows"], g["values"])
+    panels, diag, res = _oracle.multifrontal(sym, m)
+    assert res.status == 0 and list(res.inertia) == g["inertia"].tolist()
+    assert np.array_equal(diag, g["diag"])
+    x = _oracle.oracle_solve(sym, panels, g["b"])
+    assert np.max(np.abs(x - g["x"])) <= 1e-12 * np.max(np.abs(g["x"]))
+    closed = 0.00126532 * 1e3 / (70e9 * 2e-3 ** 3 / (12 * (1 - 0.3 ** 2)))
+    assert abs(abs(float(g["x"][int(g["centre"]), 0])) - closed) <= 5e-3 * closed
