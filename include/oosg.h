@@ -224,6 +224,19 @@ oosg_status oosg_solve_device(const oosg_factor *factor, double *b_device, int64
 oosg_status oosg_solve_refined_device(const oosg_factor *factor, const oosg_matrix *matrix, const double *b_device,
                                       double *x_device, int64_t ld, int64_t nrhs, double target, int32_t max_steps,
                                       oosg_solve_report *report);
+/* No counterpart in the reference: oosg_solve_refined_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The same refinement, the same
+ * kernels in the same order and the same results, but its working vectors (the solve's and the refinement's, and a
+ * pivoted factor's sweep vectors) are laid out in a file-scope device array of OOSG_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each device from the first plan made there until the process ends; the call allocates and frees
+ * nothing and captures no graph. OOSG_INVALID_ARGUMENT when the factor's vectors for 16 right-hand-side columns do
+ * not fit the array: 16 × (7·n + the rows below all supernodes) doubles and the sweeps' partial sums, so n up to
+ * about 200,000 equations for a shell plate. Every call on a device shares that device's array: calls that overlap on one device (from
+ * several threads or plans) race; the array keeps the last call's values, and each call clears what it uses. */
+#define OOSG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+oosg_status oosg_solve_refined_device_noalloc(const oosg_factor *factor, const oosg_matrix *matrix,
+                                              const double *b_device, double *x_device, int64_t ld, int64_t nrhs,
+                                              double target, int32_t max_steps, oosg_solve_report *report);
 /* oosg_factorise followed by oosg_solve_refined_device, as the reference's caller runs them back to back
  * (FsetSolver.cpp:136 and :171); no counterpart of its own in the reference. Where the factorisation is launched one
  * launch at a time (OOSG_LAUNCH_EAGER_STREAMS, or graphs off) and the tree ends in levels of one supernode (the top

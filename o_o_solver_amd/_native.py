@@ -91,6 +91,8 @@ _SIGNATURES = {
     "oosg_solve_device": (c_int, [c_void_p, c_void_p, c_int64, c_int64]),
     "oosg_solve_refined_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_double,
                                           c_int32, ctypes.POINTER(SolveReportC)]),
+    "oosg_solve_refined_device_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                                  c_double, c_int32, ctypes.POINTER(SolveReportC)]),
     "oosg_factorise_solve_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                             c_double, c_int32, c_void_p, ctypes.POINTER(SolveReportC)]),
     "oosg_matrix_multiply_device": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64]),

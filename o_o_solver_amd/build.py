@@ -45,6 +45,7 @@ def build_native(verbose=False):
         (os.path.join(CSRC, "kernels.hip"), os.path.join(BUILD, "kernels.o"), HIP_FLAGS, HIPCC),
         (os.path.join(CSRC, "selector.hip"), os.path.join(BUILD, "selector.o"), HIP_FLAGS, HIPCC),
         (os.path.join(CSRC, "pivoting.hip"), os.path.join(BUILD, "pivoting.o"), HIP_FLAGS, HIPCC),
+        (os.path.join(CSRC, "scratch.hip"), os.path.join(BUILD, "scratch.o"), HIP_FLAGS, HIPCC),
         (os.path.join(CSRC, "oosg.cpp"), os.path.join(BUILD, "oosg.o"), HOST_FLAGS + ["-D__HIP_PLATFORM_AMD__"],
          HIPCC),
         (os.path.join(CSRC, "analysis.cpp"), os.path.join(BUILD, "analysis.o"), HOST_FLAGS, "g++"),
@@ -53,10 +54,10 @@ def build_native(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(len(jobs)) as pool:
         objects = list(pool.map(_compile, jobs))
     lib = os.path.join(PACKAGE, "liboosg.so")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib + ".tmp", *objects[:5]])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib + ".tmp", *objects[:6]])
     os.replace(lib + ".tmp", lib)
     models = os.path.join(PACKAGE, "liboosm.so")
-    _run(["g++", "-shared", "-fPIC", "-o", models + ".tmp", objects[5]])
+    _run(["g++", "-shared", "-fPIC", "-o", models + ".tmp", objects[6]])
     os.replace(models + ".tmp", models)
     if verbose:
         print(lib)

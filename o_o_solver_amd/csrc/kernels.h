@@ -255,4 +255,10 @@ hipError_t launchPivotStatistics(const double* diag, const double* scale, const 
                                  const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
                                  hipStream_t stream);
 
+// The scratch of the refined solve that allocates nothing (oosg_solve_refined_device_noalloc): a file-scope device
+// array of SCRATCH_BYTES in scratch.hip, one copy per device, held until the process ends. scratchAddress gives its
+// address on `device` (which must be current), taken once per device with hipGetSymbolAddress and kept.
+constexpr size_t SCRATCH_BYTES = size_t(256) << 20;
+hipError_t scratchAddress(int device, void** out);
+
 } // namespace oosg

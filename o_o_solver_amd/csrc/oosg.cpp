@@ -93,14 +93,16 @@ struct DeviceBuffer
     void* ptr = nullptr;
     size_t bytes = 0;
     uint64_t* account = nullptr;
+    bool owned = true; // false: a view of memory held elsewhere (never freed here)
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes), account(o.account)
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes), account(o.account), owned(o.owned)
     {
         o.ptr = nullptr;
         o.bytes = 0;
         o.account = nullptr;
+        o.owned = true;
     }
     DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
     {
@@ -108,21 +110,33 @@ struct DeviceBuffer
         ptr = o.ptr;
         bytes = o.bytes;
         account = o.account;
+        owned = o.owned;
         o.ptr = nullptr;
         o.bytes = 0;
         o.account = nullptr;
+        o.owned = true;
         return *this;
     }
     ~DeviceBuffer() { reset(); }
+    // A view of `size` bytes at `at` (not owned, not counted).
+    static DeviceBuffer view(void* at, size_t size)
+    {
+        DeviceBuffer b;
+        b.ptr = at;
+        b.bytes = size;
+        b.owned = false;
+        return b;
+    }
     void reset()
     {
-        if (ptr != nullptr)
+        if (ptr != nullptr && owned)
             (void)hipFree(ptr);
         if (account != nullptr)
             *account -= bytes;
         ptr = nullptr;
         bytes = 0;
         account = nullptr;
+        owned = true;
     }
     void allocate(size_t size, const char* what)
     {
@@ -175,6 +189,7 @@ enum ProfileKind
     PROFILE_KINDS
 };
 static_assert(PROFILE_KINDS == OOSG_PROFILE_KINDS, "the profile kinds of the header");
+static_assert(oosg::SCRATCH_BYTES == OOSG_NOALLOC_SCRATCH_BYTES, "the scratch array of the header");
 
 struct Launch
 {
@@ -355,6 +370,8 @@ struct oosg_plan
     // Cancellation: polled between the parts of a call (null: none).
     oosg_cancel_fn cancelled = nullptr;
     void* cancelContext = nullptr;
+    // This device's copy of the no-allocation solve's scratch array (oosg::scratchAddress), taken at plan creation.
+    void* scratch = nullptr;
     // Its holders: the caller that made it, and every matrix and factor made from it (the last release deletes it, so
     // that a matrix or factor released after the plan still finds it).
     int32_t references = 1;
@@ -2821,6 +2838,8 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
             for (hipEvent_t& e : p->laneEvents)
                 check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             check(oosg::configureKernels(), "plan: kernel limits");
+            check(hipEventCreateWithFlags(&p->callerEvent, hipEventDisableTiming), "hipEventCreate");
+            check(oosg::scratchAddress(device, &p->scratch), "plan: the scratch array's address");
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
             uploadPlan(*p);
@@ -4055,6 +4074,98 @@ oosg_status oosg_solve_refined_device(const oosg_factor* f, const oosg_matrix* m
             refineTiles(*f, *m, b, x, ld, nrhs, target, max_steps, local, 0);
             check(hipStreamSynchronize(st), "refined solve");
             p.profiler.collect();
+            local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (!local.converged)
+                fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +
+                                             " did not reach " + std::to_string(target) + " in " +
+                                             std::to_string(local.steps) + " solves");
+        });
+    if (report != nullptr)
+        *report = local;
+    return status;
+}
+
+oosg_status oosg_solve_refined_device_noalloc(const oosg_factor* f, const oosg_matrix* m, const double* b, double* x,
+                                              int64_t ld, int64_t nrhs, double target, int32_t max_steps,
+                                              oosg_solve_report* report)
+{
+    oosg_solve_report local{};
+    const oosg_status status = guarded(
+        [&]
+        {
+            if (f == nullptr || m == nullptr || b == nullptr || x == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: null argument");
+            oosg_plan& p = *f->plan;
+            if (m->plan != &p)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: the matrix was made for another plan");
+            if (ld < std::max<int64_t>(p.n, 1) || nrhs < 1)
+                fail(OOSG_INVALID_ARGUMENT, "refined solve: at least one right-hand side, leading dimension >= n");
+            if (p.scratch == nullptr)
+                fail(OOSG_INTERNAL, "refined solve: no scratch array on this device");
+            setDevice(&p);
+            // The vectors the refined solve works in (sized for 16 right-hand sides, as the plan's own), laid out in
+            // this device's scratch array: the plan's solve and refinement vectors, and a pivoted factor's sweep
+            // vectors of its structure.
+            oosg_plan* d = f->pivoted ? f->pivoted->structure.get() : nullptr;
+            std::vector<DeviceBuffer*> targets = {&p.solveX, &p.solveU, &p.solvePartials, &p.solveNorms, &p.cgB,
+                                                  &p.cgX,    &p.cgR,    &p.cgZ,          &p.cgP,         &p.cgQ,
+                                                  &p.cgPartial, &p.cgDots};
+            std::vector<uint64_t> sizes = solveBufferBytes(p);
+            sizes.resize(targets.size());
+            if (d != nullptr)
+            {
+                targets.insert(targets.end(), {&d->solveX, &d->solveU, &d->solvePartials});
+                sizes.push_back(static_cast<uint64_t>(std::max(d->n, 1)) * 16 * sizeof(double));
+                sizes.push_back(static_cast<uint64_t>(std::max<int64_t>(d->rowStart[d->S], 1)) * 16 * sizeof(double));
+                sizes.push_back(static_cast<uint64_t>(std::max<int64_t>(d->partialSlots, 1)) * 64 * 16 *
+                                sizeof(double));
+            }
+            std::vector<uint64_t> offsets(sizes.size());
+            uint64_t total = 0;
+            for (size_t k = 0; k < sizes.size(); ++k)
+            {
+                offsets[k] = total;
+                total += (sizes[k] + 255) / 256 * 256;
+            }
+            if (total > oosg::SCRATCH_BYTES)
+                fail(OOSG_INVALID_ARGUMENT,
+                     "refined solve without allocation: the " + std::to_string(p.n) + " equations need " +
+                         mebibytes(total) + " of scratch (16 right-hand-side columns), more than the " +
+                         mebibytes(oosg::SCRATCH_BYTES) +
+                         " of the library's scratch array; use oosg_solve_refined_device");
+            // The views replace the plan's vectors for the call (launched one launch at a time: no graph capture, no
+            // profiler events), and the plan's own come back at the end.
+            struct Borrow
+            {
+                oosg_plan& p;
+                std::vector<DeviceBuffer*> targets;
+                std::vector<DeviceBuffer> saved;
+                bool graphs, profiling;
+                ~Borrow()
+                {
+                    for (size_t k = 0; k < targets.size(); ++k)
+                        *targets[k] = std::move(saved[k]);
+                    p.graphs = graphs;
+                    p.profiler.enabled = profiling;
+                }
+            } borrow{p, targets, {}, p.graphs, p.profiler.enabled};
+            unsigned char* base = static_cast<unsigned char*>(p.scratch);
+            for (size_t k = 0; k < targets.size(); ++k)
+            {
+                borrow.saved.push_back(std::move(*targets[k]));
+                *targets[k] = DeviceBuffer::view(base + offsets[k], sizes[k]);
+            }
+            p.graphs = false;
+            p.profiler.enabled = false;
+            followCaller(p);
+            const hipStream_t st = p.stream;
+            // The array keeps the last call's values: what is used is cleared first (a fill kernel).
+            check(oosg::launchFillDouble(reinterpret_cast<double*>(base), static_cast<int64_t>(total / 8),
+                                         static_cast<int64_t>(total / 8), 1, 0.0, st),
+                  "refined solve: scratch");
+            const auto t0 = std::chrono::steady_clock::now();
+            refineTiles(*f, *m, b, x, ld, nrhs, target, max_steps, local, 0);
+            check(hipStreamSynchronize(st), "refined solve");
             local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (!local.converged)
                 fail(OOSG_NOT_CONVERGED, "refined solve: the backward error " + std::to_string(local.backward_error) +

@@ -447,10 +447,18 @@ def test_ratio_test_statistics_match_the_pivots(gpu_device, precision):
     assert diagnostics.first_issue_equation == fails[0] and not diagnostics.first_issue_negative
     assert diagnostics.largest_ratio == np.max(ratio)
     assert diagnostics.largest_ratio_equation == int(np.argmax(ratio))
-    with pytest.raises(oos.SingularityError, match=f"equation {fails[0]}"):
+    # The issue list: the first 1000 failing pivots in elimination order, with their values (PivotIssue).
+    issues = diagnostics.issues
+    assert [i.equation for i in issues] == fails[:1000].tolist()
+    assert all(i.factor_diagonal == d[i.equation] and i.matrix_diagonal == a[i.equation] and
+               i.ratio == ratio[i.equation] for i in issues)
+    # A failure of an FP32 factor does not show the matrix singular: its own status.
+    expected = oos.SingularityError if precision == Precision.FP64 else oos.PrecisionInsufficientError
+    with pytest.raises(expected, match=f"equation {fails[0]}") as error:
         SupernodalFactor.factorise(sym, w.matrix, FactorOptions(precision=precision,
                                                                 pivot_policy=oos.PivotPolicy(max_ratio=threshold)),
                                    device_matrix=plain.device_matrix)
+    assert error.value.dofs == fails[:1000].tolist()
 
 
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32])
