@@ -205,8 +205,31 @@ struct ColumnCoefficients
 {
     double v[16];
 };
-// Workgroups of the column dot products (the partial sums array holds DOT_GROUPS × 16 values).
+// Workgroups of the column dot products (the partial sums array holds DOT_GROUPS × 16 values) and of the Gram
+// products (DOT_GROUPS × GRAM_PAIRS × 256 values).
 constexpr int DOT_GROUPS = 1024;
+constexpr int GRAM_PAIRS = 4;
+// Gram products of up to GRAM_PAIRS pairs of row-major n × k vectors (k <= 16): out[q·256 + 16a + b] =
+// Σ_i U_q(i, a)·V_q(i, b), zero beyond k; a fixed order of summation (deterministic).
+struct GramPairs
+{
+    const double* u[GRAM_PAIRS];
+    const double* v[GRAM_PAIRS];
+    int pairs;
+};
+hipError_t launchBlockGram(const GramPairs& pairs, int64_t n, int k, double* partial, double* out,
+                           hipStream_t stream);
+// Y(i, :) = c0·Y(i, :) + U(i, :)·C for row-major n × k vectors (k <= 16) and a 16 × 16 coefficient block C (row a,
+// column b): given by value, or read from device memory (the result of a Gram product) times `scale`. Y must not be U.
+struct BlockTerm
+{
+    const double* u;
+    const double* deviceC; // non-null: C = scale·deviceC (16 × 16 on the device)
+    double scale;
+    double c0;
+    double c[256]; // C by value where deviceC is null
+};
+hipError_t launchBlockCombine(double* y, const BlockTerm& term, int64_t n, int k, hipStream_t stream);
 
 // Row-major n × k vectors (k <= 16): out = b − A·x (b null: A·x) with the CSR copy of A, by its row groups (rows
 // [groups[g], groups[g + 1]), at most 8, with the same columns); column dot products (into

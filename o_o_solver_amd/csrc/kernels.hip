@@ -1953,6 +1953,80 @@ __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int6
 }
 
 // x += α_h·p, r −= α_h·q.
+// ---- block conjugate gradients: Gram products and block combinations -------------------------------------------
+//
+// Gram products U_qᵀ·V_q of n × k row-major vectors by 16 × 16 × 4 FP64 MFMA, four rows at a time: lane l holds
+// U(row + l/16, l mod 16) as A(a = l mod 16, i = l/16) and V(row + l/16, l mod 16) as B(i, b = l mod 16) (each lane
+// loads one value of each operand, the 4 rows × k values coalesced). A workgroup takes a fixed contiguous range of
+// rows, its waves rows 4w, 4w + 16, ...; the waves' sums are added in wave order and the workgroups' in index order.
+__global__ __launch_bounds__(256) void blockGramKernel(GramPairs g, int64_t n, int k, double* partial)
+{
+    __shared__ double sums[4][GRAM_PAIRS][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + 3) / 4 * 4;
+    const int64_t r0 = min(n, int64_t(blockIdx.x) * per), r1 = min(n, r0 + per);
+    const int col = lane & 15;
+    double4_t acc[GRAM_PAIRS];
+#pragma unroll
+    for (int q = 0; q < GRAM_PAIRS; ++q)
+        acc[q] = double4_t{0, 0, 0, 0};
+    for (int64_t row = r0 + 4 * wave; row < r1; row += 16)
+    {
+        const int64_t i = row + (lane >> 4);
+        const bool ok = i < r1 && col < k;
+        const int64_t at = ok ? i * k + col : 0;
+#pragma unroll
+        for (int q = 0; q < GRAM_PAIRS; ++q)
+            if (q < g.pairs)
+            {
+                const double a = ok ? g.u[q][at] : 0.0;
+                const double b = ok ? g.v[q][at] : 0.0;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < GRAM_PAIRS; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            sums[wave][q][((lane >> 4) + 4 * v) * 16 + col] = acc[q][v];
+    __syncthreads();
+    for (int q = 0; q < g.pairs; ++q)
+    {
+        const int e = threadIdx.x;
+        partial[(int64_t(blockIdx.x) * GRAM_PAIRS + q) * 256 + e] =
+            ((sums[0][q][e] + sums[1][q][e]) + sums[2][q][e]) + sums[3][q][e];
+    }
+}
+
+// out[q·256 + e] = Σ_g partial[(g·GRAM_PAIRS + q)·256 + e], the workgroups in index order (workgroup q, thread e).
+__global__ __launch_bounds__(256) void sumGramPartialsKernel(const double* partial, int groups, double* out)
+{
+    const int q = blockIdx.x, e = threadIdx.x;
+    double sum = 0.0;
+    for (int g = 0; g < groups; ++g)
+        sum += partial[(int64_t(g) * GRAM_PAIRS + q) * 256 + e];
+    out[q * 256 + e] = sum;
+}
+
+// Thread (i, b): Y(i, b) = c0·Y(i, b) + Σ_a U(i, a)·C(a, b), a increasing.
+__global__ __launch_bounds__(256) void blockCombineKernel(double* y, BlockTerm m, int64_t n, int k)
+{
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t i = t >> 4;
+    const int b = int(t & 15);
+    if (i >= n || b >= k)
+        return;
+    double sum = m.c0 == 0.0 ? 0.0 : m.c0 * y[i * k + b];
+    const double* u = m.u + i * k;
+    if (m.deviceC != nullptr)
+        for (int a = 0; a < k; ++a)
+            sum += u[a] * (m.scale * m.deviceC[a * 16 + b]);
+    else
+        for (int a = 0; a < k; ++a)
+            sum += u[a] * m.c[a * 16 + b];
+    y[i * k + b] = sum;
+}
+
 __global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const double* p, const double* q,
                                                    ColumnCoefficients alpha, int64_t n, int k)
 {
@@ -2239,6 +2313,22 @@ hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, 
     hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
     static_assert(DOT_GROUPS % 16 == 0, "runs of the partial sums");
     hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+    return hipGetLastError();
+}
+
+hipError_t launchBlockGram(const GramPairs& pairs, int64_t n, int k, double* partial, double* out,
+                           hipStream_t stream)
+{
+    hipLaunchKernelGGL(blockGramKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, pairs, n, k, partial);
+    hipLaunchKernelGGL(sumGramPartialsKernel, dim3(pairs.pairs), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+    return hipGetLastError();
+}
+
+hipError_t launchBlockCombine(double* y, const BlockTerm& term, int64_t n, int k, hipStream_t stream)
+{
+    const int64_t threads = n * 16;
+    hipLaunchKernelGGL(blockCombineKernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, (threads + 255) / 256))),
+                       dim3(256), 0, stream, y, term, n, k);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -763,6 +764,86 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
     deriveLevels(p);
 }
 
+// The update matrices' places in one arena, by liveness: U(s) is written at the level of s and read at its parent's
+// level, and its place is free again after that level (the role of the reference's stack of update matrices and the
+// bound of its workspace, MultifrontalDriver.cpp:257-284, FactorSchedule.cpp:156; here a whole level is live at
+// once, so the arena holds the update matrices of the levels in flight). Places are taken first fit, lowest address
+// first, a level's largest matrices first, and freed places are merged. Every launch that reads or writes an update matrix runs on the critical stream
+// in level order, so a place is reused only after the launches that read its previous matrix. updateOff[S] is the
+// arena's size (elements).
+void placeUpdateMatrices(oosg_plan& p, const std::vector<int32_t>& height)
+{
+    constexpr int64_t ALIGN = 32; // elements: 256 bytes in FP64
+    const int32_t S = p.S;
+    std::map<int64_t, int64_t> freeSpace; // start → length
+    int64_t top = 0;
+    std::vector<std::vector<int32_t>> releaseAfter(static_cast<size_t>(std::max(p.levels, 1)));
+    const auto sizeOf = [&](int32_t s)
+    { return (static_cast<int64_t>(p.nrow[s]) * p.nrow[s] + ALIGN - 1) / ALIGN * ALIGN; };
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        // A level's matrices placed largest first (less fragmentation than in supernode order).
+        std::vector<int32_t> order(p.levelList.begin() + p.levelStart[l], p.levelList.begin() + p.levelStart[l + 1]);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return sizeOf(a) > sizeOf(b); });
+        for (const int32_t s : order)
+        {
+            const int64_t size = sizeOf(s);
+            if (size == 0)
+            {
+                p.updateOff[s] = 0;
+                continue;
+            }
+            auto it = freeSpace.begin();
+            while (it != freeSpace.end() && it->second < size)
+                ++it;
+            if (it != freeSpace.end())
+            {
+                p.updateOff[s] = it->first;
+                const int64_t rest = it->second - size, start = it->first + size;
+                freeSpace.erase(it);
+                if (rest > 0)
+                    freeSpace[start] = rest;
+            }
+            else
+            {
+                p.updateOff[s] = top;
+                top += size;
+            }
+            if (p.parent[s] >= 0)
+                releaseAfter[height[p.parent[s]]].push_back(s);
+        }
+        for (const int32_t s : releaseAfter[l])
+        {
+            int64_t start = p.updateOff[s], length = sizeOf(s);
+            auto next = freeSpace.lower_bound(start);
+            if (next != freeSpace.end() && next->first == start + length)
+            {
+                length += next->second;
+                next = freeSpace.erase(next);
+            }
+            if (next != freeSpace.begin())
+            {
+                auto previous = std::prev(next);
+                if (previous->first + previous->second == start)
+                {
+                    start = previous->first;
+                    length += previous->second;
+                    freeSpace.erase(previous);
+                }
+            }
+            if (start + length == top)
+                top = start; // the end of the arena is free again
+            else
+                freeSpace[start] = length;
+        }
+    }
+    // The arena's size: the highest end of any place.
+    int64_t end = 0;
+    for (int32_t s = 0; s < S; ++s)
+        end = std::max(end, p.updateOff[s] + sizeOf(s));
+    p.updateOff[S] = end;
+}
+
 // From the structure (first, ncol, nrow, parent, children): the offsets of the panels, update matrices and inverses,
 // the levels by height, the flops, and the sweeps' work lists by level.
 void deriveLevels(oosg_plan& p)
@@ -776,7 +857,6 @@ void deriveLevels(oosg_plan& p)
     {
         const int64_t c = p.ncol[k], r = p.nrow[k];
         p.panelOff[k + 1] = p.panelOff[k] + c * (c + r);
-        p.updateOff[k + 1] = p.updateOff[k] + r * r;
         // The diagonal blocks' inverses, then (wide supernodes) the whole inverse of the triangle.
         const int64_t inverses = oosg::blockInverseValues(c) + (oosg::keepsTriangleInverse(static_cast<int>(c)) ? c * c : 0);
         p.linvOff[k + 1] = p.linvOff[k] + inverses;
@@ -976,6 +1056,7 @@ void deriveLevels(oosg_plan& p)
         }
         p.inverseRounds.push_back(std::move(rounds));
     }
+    placeUpdateMatrices(p, height);
 }
 
 // The algorithmic flops of a tile: 2·k per element it updates (the lower part only on the diagonal).
@@ -1478,8 +1559,8 @@ std::vector<uint64_t> solveBufferBytes(const oosg_plan& p)
             static_cast<uint64_t>(std::max<int64_t>(p.partialSlots, 1)) * 64 * 16 * sizeof(double), // solvePartials
             3 * 16 * sizeof(unsigned long long),                                            // solveNorms
             vector, vector, vector, vector, vector, vector,                                 // cgB .. cgQ
-            static_cast<uint64_t>(oosg::DOT_GROUPS) * 16 * sizeof(double),                  // cgPartial
-            16 * sizeof(double),                                                            // cgDots
+            static_cast<uint64_t>(oosg::DOT_GROUPS) * oosg::GRAM_PAIRS * 256 * sizeof(double), // cgPartial
+            static_cast<uint64_t>(oosg::GRAM_PAIRS) * 256 * sizeof(double),                  // cgDots
             static_cast<uint64_t>(std::max(p.S, 1)) * sizeof(int32_t),                      // pivotStatus
             sizeof(oosg::PivotStatistics)};                                                 // pivotStats
 }
@@ -2636,6 +2717,237 @@ void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, doubl
                 rho[h] = rhoNew[h];
         }
         check(oosg::launchCgDirection(P, Z, beta, n, k, st), "solve: direction");
+    }
+    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
+    report.steps = solves;
+    report.converged = converged ? 1 : 0;
+    report.backward_error = 0.0;
+    for (int h = 0; h < k; ++h)
+        report.backward_error = std::max(report.backward_error, eta[h]);
+}
+
+// ---- block conjugate gradients ------------------------------------------------------------------------------------
+
+// The eigenvalues w and eigenvectors (columns of v, row-major k × k) of a symmetric k × k matrix `a` (row-major), by
+// cyclic Jacobi rotations (k <= 16: a few sweeps).
+void symmetricEigen(int k, std::vector<double> a, std::vector<double>& w, std::vector<double>& v)
+{
+    v.assign(static_cast<size_t>(k * k), 0.0);
+    for (int i = 0; i < k; ++i)
+        v[i * k + i] = 1.0;
+    for (int sweep = 0; sweep < 60; ++sweep)
+    {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j)
+                (i == j ? diag : off) += a[i * k + j] * a[i * k + j];
+        if (off <= 1e-32 * diag || off == 0.0)
+            break;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q)
+            {
+                const double apq = a[p * k + q];
+                if (apq == 0.0)
+                    continue;
+                const double theta = (a[q * k + q] - a[p * k + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int r = 0; r < k; ++r)
+                {
+                    const double arp = a[r * k + p], arq = a[r * k + q];
+                    a[r * k + p] = c * arp - sn * arq;
+                    a[r * k + q] = sn * arp + c * arq;
+                }
+                for (int r = 0; r < k; ++r)
+                {
+                    const double apr = a[p * k + r], aqr = a[q * k + r];
+                    a[p * k + r] = c * apr - sn * aqr;
+                    a[q * k + r] = sn * apr + c * aqr;
+                }
+                for (int r = 0; r < k; ++r)
+                {
+                    const double vrp = v[r * k + p], vrq = v[r * k + q];
+                    v[r * k + p] = c * vrp - sn * vrq;
+                    v[r * k + q] = sn * vrp + c * vrq;
+                }
+            }
+    }
+    w.resize(static_cast<size_t>(k));
+    for (int i = 0; i < k; ++i)
+        w[i] = a[i * k + i];
+}
+
+// T (16 × 16, row a, column j) such that Tᵀ·G·T is the identity on the directions kept and T's other columns are zero,
+// for the Gram matrix G = Wᵀ·A·W of a block of directions W (16 × 16 with k used): G scaled to a unit diagonal,
+// its eigenvectors with eigenvalues below 1e-12 of the largest (linearly dependent directions, or zero ones)
+// dropped.
+std::vector<double> orthonormalizer(const double* G, int k)
+{
+    std::vector<double> d(static_cast<size_t>(k)), g(static_cast<size_t>(k * k)), w, v;
+    for (int a = 0; a < k; ++a)
+        d[a] = G[a * 16 + a] > 0.0 && std::isfinite(G[a * 16 + a]) ? 1.0 / std::sqrt(G[a * 16 + a]) : 0.0;
+    for (int a = 0; a < k; ++a)
+        for (int b = 0; b < k; ++b)
+            g[a * k + b] = d[a] * 0.5 * (G[a * 16 + b] + G[b * 16 + a]) * d[b];
+    symmetricEigen(k, g, w, v);
+    double largest = 0.0;
+    for (const double x : w)
+        largest = std::max(largest, x);
+    std::vector<double> T(256, 0.0);
+    for (int j = 0; j < k; ++j)
+    {
+        if (!(w[j] > 1e-12 * largest))
+            continue;
+        const double s = 1.0 / std::sqrt(w[j]);
+        for (int a = 0; a < k; ++a)
+            T[a * 16 + j] = d[a] * v[a * k + j] * s;
+    }
+    return T;
+}
+
+oosg::BlockTerm blockTerm(const double* u, double c0, const std::vector<double>& c)
+{
+    oosg::BlockTerm t{};
+    t.u = u;
+    t.deviceC = nullptr;
+    t.scale = 1.0;
+    t.c0 = c0;
+    std::copy(c.begin(), c.end(), t.c);
+    return t;
+}
+
+// Block conjugate gradients on A (symmetric positive definite) preconditioned by the factor, over the k columns at
+// once (O'Leary, "The block conjugate gradient algorithm", Linear Algebra Appl. 29, 1980): the search block P is
+// kept A-orthonormal (Pᵀ·A·P = I), so α = Pᵀ·R and β = −(A·P)ᵀ·Z. With Z = M⁻¹·R:
+//   W = Z − P·((A·P)ᵀ·Z),  G = Wᵀ·A·W,  T with Tᵀ·G·T = I (dependent directions dropped),
+//   P = W·T, A·P = (A·W)·T,  α = Pᵀ·R,  X += P·α,  R −= (A·P)·α.
+// The k columns share one Krylov space: the few directions where the FP32 factor is least accurate (the smooth
+// modes of a thin plate) serve every right-hand side, so it needs fewer solves than k separate recurrences. The
+// stopping rule is solvePcg's: the recurrence's residual first, then the true residual b − A·x, which replaces the
+// recurrence's when it has drifted. Buffers: Z holds W, P holds A·W before it holds P, Q holds A·P.
+void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
+                   double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
+{
+    oosg_plan& p = *f.plan;
+    const hipStream_t st = p.stream;
+    const int64_t n = p.n;
+    double* B = p.cgB.as<double>();
+    double* X = p.cgX.as<double>();
+    double* R = p.cgR.as<double>();
+    double* Z = p.cgZ.as<double>();
+    double* P = p.cgP.as<double>();
+    double* Q = p.cgQ.as<double>();
+    double* grams = p.cgDots.as<double>();
+    double* partial = p.cgPartial.as<double>();
+    const int64_t* offsets = m.dCsrOffsets.as<int64_t>();
+    const int32_t* cols = m.dCsrCols.as<int32_t>();
+    const double* values = m.dCsrValues.as<double>();
+    const int32_t* groups = m.dCsrGroups.as<int32_t>();
+    const int64_t groupCount = static_cast<int64_t>(m.csrGroups.size()) - 1;
+    const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
+    const auto product = [&](const double* in, const double* minus, double* out)
+    {
+        const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
+        check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, in, minus, out, k, st),
+              "solve: product");
+        p.profiler.end(slot, st);
+    };
+    const auto gram = [&](std::initializer_list<std::pair<const double*, const double*>> pairs)
+    {
+        oosg::GramPairs g{};
+        g.pairs = 0;
+        for (const auto& pr : pairs)
+        {
+            g.u[g.pairs] = pr.first;
+            g.v[g.pairs] = pr.second;
+            ++g.pairs;
+        }
+        check(oosg::launchBlockGram(g, n, k, partial, grams, st), "solve: Gram products");
+    };
+
+    if (aheadLevel == 0)
+        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+    const std::vector<double> bNorm = columnMaxRows(p, B, k);
+    check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
+          "solve: zero");
+    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+    const auto backwardErrors = [&](const double* residual)
+    {
+        startColumnMax(p, residual, k, 1);
+        startColumnMax(p, X, k, 2);
+        const std::vector<std::vector<double>> norms = finishColumnMax(p, 3, k);
+        std::vector<double> e(static_cast<size_t>(k));
+        for (int h = 0; h < k; ++h)
+        {
+            const double denominator = m.normInf * norms[2][h] + bNorm[h];
+            const double v = denominator > 0 ? norms[1][h] / denominator : norms[1][h];
+            e[h] = v != v ? INFINITY : v;
+        }
+        return e;
+    };
+    std::vector<double> eta(static_cast<size_t>(k), INFINITY);
+    int32_t solves = 0;
+    bool converged = false;
+    for (int32_t iteration = 0;; ++iteration)
+    {
+        // Z = M⁻¹·R, then W = Z − P·((A·P)ᵀ·Z) in Z.
+        applyFactorRows(f, R, Z, k, iteration == 0 ? aheadLevel : 0);
+        ++solves;
+        if (iteration > 0)
+        {
+            gram({{Q, Z}});
+            oosg::BlockTerm t{};
+            t.u = P;
+            t.deviceC = grams;
+            t.scale = -1.0;
+            t.c0 = 1.0;
+            check(oosg::launchBlockCombine(Z, t, n, k, st), "solve: directions");
+        }
+        // A·W (in P), G = Wᵀ·A·W and F = Wᵀ·R, read back.
+        product(Z, nullptr, P);
+        gram({{Z, P}, {Z, R}});
+        std::vector<double> G(512);
+        check(hipMemcpyAsync(G.data(), grams, G.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+              "solve: Gram products");
+        check(hipStreamSynchronize(st), "solve: Gram products");
+        const std::vector<double> T = orthonormalizer(G.data(), k);
+        // α = Tᵀ·F (16 × 16, row j, column h).
+        std::vector<double> alpha(256, 0.0), minusAlpha(256, 0.0);
+        for (int j = 0; j < k; ++j)
+            for (int h = 0; h < k; ++h)
+            {
+                double sum = 0.0;
+                for (int a = 0; a < k; ++a)
+                    sum += T[a * 16 + j] * G[256 + a * 16 + h];
+                alpha[j * 16 + h] = sum;
+                minusAlpha[j * 16 + h] = -sum;
+            }
+        // A·P = (A·W)·T into Q, P = W·T, then X += P·α, R −= (A·P)·α.
+        check(oosg::launchBlockCombine(Q, blockTerm(P, 0.0, T), n, k, st), "solve: directions");
+        check(oosg::launchBlockCombine(P, blockTerm(Z, 0.0, T), n, k, st), "solve: directions");
+        check(oosg::launchBlockCombine(X, blockTerm(P, 1.0, alpha), n, k, st), "solve: step");
+        check(oosg::launchBlockCombine(R, blockTerm(Q, 1.0, minusAlpha), n, k, st), "solve: step");
+        const std::vector<double> estimate = backwardErrors(R);
+        bool met = true;
+        for (int h = 0; h < k; ++h)
+            met = met && estimate[h] <= target;
+        if (met || solves >= maxSteps)
+        {
+            // The true residual, into Z.
+            product(X, B, Z);
+            const std::vector<double> trueEta = backwardErrors(Z);
+            converged = true;
+            for (int h = 0; h < k; ++h)
+            {
+                eta[h] = trueEta[h];
+                converged = converged && trueEta[h] <= target;
+            }
+            if (converged || solves >= maxSteps)
+                break;
+            // Residual replacement: the recurrence drifted from b − A·x.
+            check(hipMemcpyAsync(R, Z, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+        }
+        pollCancellation(p);
     }
     check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
     report.steps = solves;
@@ -4040,7 +4352,8 @@ void refineTiles(const oosg_factor& f, const oosg_matrix& m, const double* b, do
         const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
         oosg_solve_report tile{};
         if (f.precision == OOSG_FP32)
-            solvePcg(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile, c0 == 0 ? from : 0);
+            solveBlockPcg(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile,
+                          c0 == 0 ? from : 0);
         else
             solveRefinement(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile,
                             c0 == 0 ? from : 0);

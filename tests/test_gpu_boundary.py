@@ -188,3 +188,37 @@ def test_refined_solves_of_more_than_16_right_hand_sides(gpu_device, precision):
     _, x2, r2 = SupernodalFactor.factorise_and_solve(sym, w.matrix, torch.tensor(b, device=gpu_device),
                                                      FactorOptions(precision=precision), target=1e-10, max_steps=40)
     assert r2.converged and _oracle.backward_error(w.matrix, x2.cpu().numpy(), b) <= 1e-10
+
+
+def _liveness_bound(sym):
+    """The most update-matrix values live at once when a level's update matrices are written while their children's
+    are read: max over levels L of Σ r_s² over the supernodes s with level(s) <= L <= level(parent(s)) (places
+    rounded to 32 values)."""
+    S = sym.supernode_count
+    height = np.zeros(S, np.int64)
+    for s in range(S):
+        if sym.parent[s] >= 0:
+            height[sym.parent[s]] = max(height[sym.parent[s]], height[s] + 1)
+    r = np.diff(sym.row_start)
+    size = (r * r + 31) // 32 * 32
+    last = np.where(sym.parent >= 0, height[np.maximum(sym.parent, 0)], height)
+    levels = int(height.max()) + 1
+    return max(int(size[(height <= lv) & (last >= lv)].sum()) for lv in range(levels))
+
+
+@pytest.mark.parametrize("case", ["plate 60", "hex 10"])
+def test_update_matrices_held_by_liveness(gpu_device, case):
+    # The update matrices share one arena by liveness (a matrix's place is free once its parent's level has read
+    # it): the arena is within 35 % of the most values live at once (first-fit fragmentation), below one place per
+    # supernode, and the factor is the oracle's.
+    w = (models.plate(60, jitter=0.2, seed=5, thickness=(1e-3, 4e-3), thickness_mode="log-uniform")
+         if case.startswith("plate") else models.hex_block(10, seed=2, sigma=1.0))
+    sym = w.analyse()
+    arena = sym.plan().stats()["update_values"]
+    bound = _liveness_bound(sym)
+    every = int(np.sum(np.diff(sym.row_start) ** 2))
+    print(f"[gpu] {case}: update arena {arena} values, live bound {bound}, one place each {every}", flush=True)
+    assert bound <= arena <= 1.35 * bound and arena < every
+    factor = SupernodalFactor.factorise(sym, w.matrix)
+    _, diag, _ = _oracle.multifrontal(sym, w.matrix)
+    assert np.max(np.abs(factor.diagonal() - diag) / np.abs(diag)) <= 1e-9
