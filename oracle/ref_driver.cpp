@@ -10,6 +10,9 @@
 //                        (include/oos/linalg/supernodal/SupernodalFactor.hpp:181-183,
 //                         include/oos/linalg/supernodal/FactorSolve.hpp:77-81)
 //   ref_spmv             SymmetricCsc::multiply (include/oos/linalg/sparse/SymmetricCsc.hpp:86-90)
+//   ref_combine          SymmetricCsc::combine (src/linalg/sparse/SymmetricCsc.cpp:209-236)
+//   ref_scale_symmetric  SymmetricCsc::scaleSymmetric (src/linalg/sparse/SymmetricCsc.cpp:239-252)
+//   ref_pencil_scale_of  pencilScaleOf (src/linalg/eigen/PencilScaling.cpp:63-78)
 //   ref_dense_*          the reference kernel family's LDLᵀ entry points
 //                        (include/oos/linalg/kernels/DenseKernelEntryPoints.hpp:253-264)
 // Errors come back as status codes: 1 std::invalid_argument, 2 MemoryBudgetError, 3 SingularityError, 4 other.
@@ -20,6 +23,7 @@
 #include "oos/base/memory/MemoryBudget.hpp"
 #include "oos/base/threading/ThreadPool.hpp"
 #include "oos/linalg/dense/DenseKernels.hpp"
+#include "oos/linalg/eigen/PencilScaling.hpp"
 #include "oos/linalg/kernels/DenseKernelEntryPoints.hpp"
 #include "oos/linalg/ordering/ConstrainedOrdering.hpp"
 #include "oos/linalg/ordering/NodeGraph.hpp"
@@ -281,6 +285,69 @@ int ref_spmv(std::int32_t n, const std::int64_t* offsets, const std::int32_t* ro
                 std::span<const double>(values, static_cast<std::size_t>(nnz)), budget);
             matrix.multiply(std::span<const double>(x, static_cast<std::size_t>(n)),
                             std::span<double>(y, static_cast<std::size_t>(n)));
+        },
+        err, errlen);
+}
+
+namespace
+{
+
+linalg::SymmetricCsc cscOf(std::int32_t n, const std::int64_t* offsets, const std::int32_t* rows, const double* values,
+                           base::MemoryBudget& budget)
+{
+    const std::int64_t nnz = offsets[n];
+    return linalg::SymmetricCsc::fromCompressedColumns(
+        n, std::span<const std::int64_t>(offsets, static_cast<std::size_t>(n) + 1),
+        std::span<const std::int32_t>(rows, static_cast<std::size_t>(nnz)),
+        std::span<const double>(values, static_cast<std::size_t>(nnz)), budget);
+}
+
+} // namespace
+
+// out = alpha·A + beta·B (one structure, the values of A and B given).
+int ref_combine(std::int32_t n, const std::int64_t* offsets, const std::int32_t* rows, double alpha,
+                const double* a_values, double beta, const double* b_values, double* out, char* err, int errlen)
+{
+    return guarded(
+        [&]
+        {
+            base::MemoryBudget budget(BUDGET);
+            const linalg::SymmetricCsc a = cscOf(n, offsets, rows, a_values, budget);
+            const linalg::SymmetricCsc b = cscOf(n, offsets, rows, b_values, budget);
+            const linalg::SymmetricCsc c = linalg::SymmetricCsc::combine(alpha, a, beta, b, budget);
+            const auto v = c.values();
+            std::copy(v.begin(), v.end(), out);
+        },
+        err, errlen);
+}
+
+// values ← S·A·S in place.
+int ref_scale_symmetric(std::int32_t n, const std::int64_t* offsets, const std::int32_t* rows, double* values,
+                        const double* scale, char* err, int errlen)
+{
+    return guarded(
+        [&]
+        {
+            base::MemoryBudget budget(BUDGET);
+            linalg::SymmetricCsc a = cscOf(n, offsets, rows, values, budget);
+            a.scaleSymmetric(std::span<const double>(scale, static_cast<std::size_t>(n)));
+            const auto v = a.values();
+            std::copy(v.begin(), v.end(), values);
+        },
+        err, errlen);
+}
+
+// scale = pencilScaleOf(K).
+int ref_pencil_scale_of(std::int32_t n, const std::int64_t* offsets, const std::int32_t* rows, const double* values,
+                        double* scale, char* err, int errlen)
+{
+    return guarded(
+        [&]
+        {
+            base::MemoryBudget budget(BUDGET);
+            const linalg::SymmetricCsc k = cscOf(n, offsets, rows, values, budget);
+            const base::BudgetedBuffer<double> s = linalg::pencilScaleOf(k, budget);
+            std::copy(s.data(), s.data() + n, scale);
         },
         err, errlen);
 }

@@ -79,6 +79,12 @@ def reference():
         lib.ref_dense_bunch_kaufman.restype = i64
         lib.ref_dense_bunch_kaufman.argtypes = [vp, i64, i64, i64, vp, vp]
         lib.ref_hardware_threads.restype = ctypes.c_int
+        lib.ref_combine.restype = ctypes.c_int
+        lib.ref_combine.argtypes = [i32, vp, vp, dbl, vp, dbl, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_scale_symmetric.restype = ctypes.c_int
+        lib.ref_scale_symmetric.argtypes = [i32, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_pencil_scale_of.restype = ctypes.c_int
+        lib.ref_pencil_scale_of.argtypes = [i32, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
         _ref = lib
     return _ref
 
@@ -167,3 +173,34 @@ def backward_error(matrix, x, b):
         denominator = norm * np.max(np.abs(x[:, h])) + np.max(np.abs(b[:, h]))
         eta = max(eta, float(np.max(np.abs(r)) / denominator) if denominator > 0 else float(np.max(np.abs(r))))
     return eta
+
+
+def ref_combine(alpha, a, beta, b):
+    """The reference's SymmetricCsc::combine values of alpha·A + beta·B (A and B of one structure)."""
+    out = np.zeros_like(a.values)
+    err = ctypes.create_string_buffer(1024)
+    rc = reference().ref_combine(a.n, P(a.offsets), P(a.rows), alpha, P(a.values), beta, P(b.values), P(out), err,
+                                 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return out
+
+
+def ref_scale_symmetric(matrix, scale):
+    values = np.array(matrix.values, dtype=np.float64, copy=True)
+    err = ctypes.create_string_buffer(1024)
+    rc = reference().ref_scale_symmetric(matrix.n, P(matrix.offsets), P(matrix.rows), P(values),
+                                         P(np.ascontiguousarray(scale, dtype=np.float64)), err, 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return values
+
+
+def ref_pencil_scale_of(matrix):
+    scale = np.zeros(matrix.n)
+    err = ctypes.create_string_buffer(1024)
+    rc = reference().ref_pencil_scale_of(matrix.n, P(matrix.offsets), P(matrix.rows), P(matrix.values), P(scale),
+                                         err, 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return scale

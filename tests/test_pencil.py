@@ -76,3 +76,36 @@ def test_sturm_check_orders_verifies_and_refuses_contradictions():
     assert not v.has_mismatch and v.verified_up_to == 2.0 and v.verified_count == 4
     v = check.verify([0.6, 1.2, 1.5])
     assert v.has_mismatch and v.verified_up_to == 0.5 and v.mismatch == (0.5, 1.0, 2, 1)
+
+
+def _cylinder_pencil():
+    from o_o_solver_amd import models
+    w = models.cylinder(48, 20)
+    w.analyse()
+    return w.matrix, w.geometric()
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, -3.7), (0.3, 1e-9), (-1.0, 2.5e6)])
+def test_pencil_work_equals_the_compiled_reference_bit_for_bit(reference_lib, alpha, beta):
+    # combine, pencilScaleOf and scaleSymmetric against the reference's own (oracle/_ref), on a stiffened cylinder's
+    # stiffness and geometric stiffness and on a grid with extreme diagonals (zero, negative, subnormal, infinite,
+    # missing): every value's bits.
+    import _oracle
+    k, kg = _cylinder_pencil()
+    combined = combine(alpha, k, beta, kg)
+    assert np.array_equal(combined.values.view(np.uint64), _oracle.ref_combine(alpha, k, beta, kg).view(np.uint64))
+    s = pencil_scale_of(k)
+    assert np.array_equal(s.view(np.uint64), _oracle.ref_pencil_scale_of(k).view(np.uint64))
+    scaled = scale_symmetric(combined, s)
+    assert np.array_equal(scaled.values.view(np.uint64),
+                          _oracle.ref_scale_symmetric(combined, s).view(np.uint64))
+    _, g = _grid(4.0)
+    values = g.values.copy()
+    diag_at = g.offsets[:-1]
+    values[diag_at[:6]] = [0.0, -1.0, 5e-320, np.inf, 3.0 * 2.0 ** 700, 2.0 ** -1000]
+    odd = SymmetricCsc(g.n, g.offsets, g.rows, values)
+    assert np.array_equal(pencil_scale_of(odd), _oracle.ref_pencil_scale_of(odd))
+    # A column without its diagonal term (column 1 holds row 2 only): the scale is 1.
+    missing = SymmetricCsc(3, np.array([0, 2, 3, 4]), np.array([0, 1, 2, 2], np.int32), np.array([4.0, 1.0, 1.0, 9.0]))
+    assert np.array_equal(pencil_scale_of(missing), _oracle.ref_pencil_scale_of(missing))
+    assert pencil_scale_of(missing)[1] == 1.0
