@@ -173,6 +173,15 @@ oosg_status oosg_plan_stats(const oosg_plan *plan, int64_t *stats);
 oosg_status oosg_matrix_create(oosg_plan *plan, const oosg_csc_desc *matrix, oosg_matrix **out);
 /* New values for the same structure (the shifted matrices K − σB of an eigenvalue solver). */
 oosg_status oosg_matrix_set_values(oosg_matrix *matrix, const double *values);
+/* New values for `matrix` from two matrices of its plan and structure, on the device: SymmetricCsc::combine then
+ * scaleSymmetric (src/linalg/sparse/SymmetricCsc.cpp:209-252), the shifted pencil S·(K − σ·M)·S of an eigenvalue
+ * solver (BucklingPencil.cpp:74-97): each value is scale[row]·(alpha·a + beta·b)·scale[column], the two products,
+ * the sum and the two scalings each rounded in that order (no fused multiply-add): the reference's bits. scale: n
+ * values (host memory) or null for none. matrix may be a or b. The derived arrays (the full rows of the residual
+ * products, the diagonal, ‖A‖∞, the FP32 scales) are recomputed on the device; nothing passes through the host but
+ * the diagonal. OOSG_INVALID_ARGUMENT when the structures differ (the message of SymmetricCsc::combine). */
+oosg_status oosg_matrix_set_combination(oosg_matrix *matrix, double alpha, const oosg_matrix *a, double beta,
+                                        const oosg_matrix *b, const double *scale);
 void oosg_matrix_destroy(oosg_matrix *matrix);
 
 /* The numeric factorisation (SupernodalFactor::factorise). OOSG_SINGULAR for a zero or non-finite pivot in positive

@@ -466,6 +466,7 @@ class DeviceMatrix:
         self._lib = _native.lib()
         self.plan = plan
         self.matrix = matrix
+        self.structure = matrix  # the offsets and rows (the values of `matrix` may be superseded)
         handle = ctypes.c_void_p()
         desc = matrix.desc()
         _check(self._lib.oosg_matrix_create(plan.handle, ctypes.byref(desc), ctypes.byref(handle)))
@@ -473,10 +474,21 @@ class DeviceMatrix:
 
     def set_values(self, values):
         values = np.ascontiguousarray(values, dtype=np.float64)
-        if values.shape != self.matrix.values.shape:
+        if values.shape != self.structure.values.shape:
             raise ValueError("DeviceMatrix.set_values: one value per stored entry expected")
         _check(self._lib.oosg_matrix_set_values(self.handle, _ptr(values)))
-        self.matrix = SymmetricCsc(self.matrix.n, self.matrix.offsets, self.matrix.rows, values)
+        self.matrix = SymmetricCsc(self.structure.n, self.structure.offsets, self.structure.rows, values)
+
+    def set_combination(self, alpha, a, beta, b, scale=None):
+        """Values S·(alpha·A + beta·B)·S from two device matrices of this plan and structure, on the device
+        (SymmetricCsc::combine then scaleSymmetric, the reference's rounding). The host copy `matrix` is then None:
+        the values live on the device."""
+        scale_arr = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+        if scale_arr is not None and scale_arr.shape != (self.plan.symbolic.n,):
+            raise ValueError("SymmetricCsc::scaleSymmetric: one scale per equation expected")
+        _check(self._lib.oosg_matrix_set_combination(self.handle, float(alpha), a.handle, float(beta), b.handle,
+                                                     _ptr(scale_arr)))
+        self.matrix = None
 
     def multiply(self, x):
         """y = A·x for a torch tensor on the plan's device, n × k (column-major storage: x.T contiguous)."""

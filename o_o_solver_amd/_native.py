@@ -79,6 +79,7 @@ _SIGNATURES = {
     "oosg_plan_profile_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "oosg_matrix_create": (c_int, [c_void_p, ctypes.POINTER(CscDesc), ctypes.POINTER(c_void_p)]),
     "oosg_matrix_set_values": (c_int, [c_void_p, c_void_p]),
+    "oosg_matrix_set_combination": (c_int, [c_void_p, c_double, c_void_p, c_double, c_void_p, c_void_p]),
     "oosg_matrix_destroy": (None, [c_void_p]),
     "oosg_factorise": (c_int, [c_void_p, c_void_p, ctypes.POINTER(FactorOptionsC), ctypes.POINTER(c_void_p)]),
     "oosg_factor_destroy": (None, [c_void_p]),

@@ -37,7 +37,7 @@ class BucklingShifts:
         geometric = workload.geometric()
         # G = −K_G, positive semi-definite under compression.
         self.load = type(geometric)(geometric.n, geometric.offsets, geometric.rows, -geometric.values)
-        self.pencil = ShiftedPencil(self.symbolic, self.stiffness, self.load)
+        self.pencil = ShiftedPencil(self.symbolic, self.stiffness, self.load, transform="buckling")
         self.counts = {}
         self.seconds = []
 

@@ -278,6 +278,18 @@ hipError_t launchPivotStatistics(const double* diag, const double* scale, const 
                                  const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
                                  hipStream_t stream);
 
+// The values of a matrix from two of its structure (SymmetricCsc::combine, then ::scaleSymmetric when scale is not
+// null): out[k] = scale[row]·(alpha·a[k] + beta·b[k])·scale[column], every product and the sum rounded in that
+// order; the diagonal of each column (0 where not stored) and the power-of-two scale of the FP32 factor. out may be
+// a or b.
+hipError_t launchCombineValues(int32_t n, const int64_t* offsets, const int32_t* rows, double alpha, const double* a,
+                               double beta, const double* b, const double* scale, double* out, double* diagonal,
+                               double* fp32Scale, hipStream_t stream);
+// The full rows of the residual product from the lower values (csrValues[e] = values[map[e]]) and ‖A‖∞: each row's
+// |values| summed in its column order, the largest sum as an ordered key (atomicMax; *normKey cleared first).
+hipError_t launchRowValues(int32_t n, const int64_t* csrOffsets, const int64_t* map, const double* values,
+                           double* csrValues, unsigned long long* normKey, hipStream_t stream);
+
 // The scratch of the refined solve that allocates nothing (oosg_solve_refined_device_noalloc): a file-scope device
 // array of SCRATCH_BYTES in scratch.hip, one copy per device, held until the process ends. scratchAddress gives its
 // address on `device` (which must be current), taken once per device with hipGetSymbolAddress and kept.

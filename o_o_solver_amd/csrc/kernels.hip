@@ -1953,6 +1953,69 @@ __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int6
 }
 
 // x += α_h·p, r −= α_h·q.
+// ---- matrix values on the device ----------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void combineValuesKernel(int32_t n, const int64_t* offsets, const int32_t* rows,
+                                                           double alpha, const double* a, double beta,
+                                                           const double* b, const double* scale, double* out,
+                                                           double* diagonal, double* fp32Scale)
+{
+#pragma clang fp contract(off)
+    const int32_t j = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    if (j >= n)
+        return;
+    const int64_t k0 = offsets[j], k1 = offsets[j + 1];
+    for (int64_t k = k0; k < k1; ++k)
+    {
+        // Two products and a sum, each rounded; then S·A·S as scale[i]·a_ij·scale[j], left to right.
+        const double left = alpha * a[k];
+        const double right = beta * b[k];
+        double v = left + right;
+        if (scale != nullptr)
+        {
+            const double rowScaled = scale[rows[k]] * v;
+            v = rowScaled * scale[j];
+        }
+        out[k] = v;
+    }
+    const double d = (k0 < k1 && rows[k0] == j) ? out[k0] : 0.0;
+    diagonal[j] = d;
+    double s = 1.0;
+    if (d > 0.0 && isfinite(d))
+    {
+        int e = 0;
+        (void)frexp(d, &e);
+        s = ldexp(1.0, -(e / 2));
+    }
+    fp32Scale[j] = s;
+}
+
+__global__ __launch_bounds__(256) void rowValuesKernel(int32_t n, const int64_t* csrOffsets, const int64_t* map,
+                                                       const double* values, double* csrValues,
+                                                       unsigned long long* normKey)
+{
+    const int32_t i = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    double sum = 0.0;
+    if (i < n)
+        for (int64_t e = csrOffsets[i]; e < csrOffsets[i + 1]; ++e)
+        {
+            const double v = values[map[e]];
+            csrValues[e] = v;
+            sum += fabs(v);
+        }
+    // Non-negative doubles order as their bit patterns; a row whose sum is not a number is passed over, as the
+    // host's std::max passes it over.
+    const unsigned long long key = sum == sum ? __double_as_longlong(sum) : 0ull;
+    unsigned long long top = key;
+    for (int offset = 32; offset > 0; offset >>= 1)
+    {
+        const unsigned long long other = __shfl_xor(top, offset);
+        top = other > top ? other : top;
+    }
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(normKey, top);
+}
+
 // ---- block conjugate gradients: Gram products and block combinations -------------------------------------------
 //
 // Gram products U_qᵀ·V_q of n × k row-major vectors by 16 × 16 × 4 FP64 MFMA, four rows at a time: lane l holds
@@ -2313,6 +2376,24 @@ hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, 
     hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
     static_assert(DOT_GROUPS % 16 == 0, "runs of the partial sums");
     hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+    return hipGetLastError();
+}
+
+hipError_t launchCombineValues(int32_t n, const int64_t* offsets, const int32_t* rows, double alpha, const double* a,
+                               double beta, const double* b, const double* scale, double* out, double* diagonal,
+                               double* fp32Scale, hipStream_t stream)
+{
+    hipLaunchKernelGGL(combineValuesKernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, n,
+                       offsets, rows, alpha, a, beta, b, scale, out, diagonal, fp32Scale);
+    return hipGetLastError();
+}
+
+hipError_t launchRowValues(int32_t n, const int64_t* csrOffsets, const int64_t* map, const double* values,
+                           double* csrValues, unsigned long long* normKey, hipStream_t stream)
+{
+    hipLaunchKernelGGL(fillInt32Kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<int32_t*>(normKey), 2, 0);
+    hipLaunchKernelGGL(rowValuesKernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, n,
+                       csrOffsets, map, values, csrValues, normKey);
     return hipGetLastError();
 }
 

@@ -530,6 +530,11 @@ struct oosg_matrix
     DeviceBuffer dCsrGroups;
     DeviceBuffer dDiagonal; // a_jj on the device (the ratio test)
     std::vector<double> scale;
+    // The CSR entries' CSC entries on the device (for values combined on the device), made on first use, and the
+    // key of ‖A‖∞ they are summed into.
+    DeviceBuffer dCsrMap, dNormKey;
+    // Whether hostValues (and the selector's statistics) lag behind values set on the device.
+    bool hostStale = false;
 };
 
 // A factor with Bunch–Kaufman pivoting: its structure after the interchanges and delays, as a plan of its own (for
@@ -3366,6 +3371,20 @@ oosg_status oosg_selector_set_parameters(oosg_selector* s, const float* params)
         });
 }
 
+// hostValues and the selector's statistics of a matrix whose values were last set on the device.
+static void refreshHostValues(const oosg_matrix& cm)
+{
+    oosg_matrix& m = const_cast<oosg_matrix&>(cm);
+    if (!m.hostStale)
+        return;
+    setDevice(m.plan);
+    m.hostValues.resize(static_cast<size_t>(m.nnz));
+    check(hipMemcpy(m.hostValues.data(), m.dValues.ptr, m.hostValues.size() * sizeof(double), hipMemcpyDeviceToHost),
+          "matrix: values");
+    valueStatistics(m);
+    m.hostStale = false;
+}
+
 oosg_status oosg_selector_features(const oosg_plan* plan, const oosg_matrix* m, float* features)
 {
     return guarded(
@@ -3375,6 +3394,7 @@ oosg_status oosg_selector_features(const oosg_plan* plan, const oosg_matrix* m, 
                 fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
             if (m->plan != plan)
                 fail(OOSG_INVALID_ARGUMENT, "selector: the matrix was made for another plan");
+            refreshHostValues(*m);
             int64_t largestFront = 1;
             double entries = 0.0; // Σ c(c + 1)/2 + c·r: the factor's stored values (its solves read them)
             for (int32_t s = 0; s < plan->S; ++s)
@@ -3964,6 +3984,66 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
             check(hipStreamSynchronize(st), "matrix: values");
             m->hostValues = std::move(v);
             valueStatistics(*m);
+            m->hostStale = false;
+        });
+}
+
+oosg_status oosg_matrix_set_combination(oosg_matrix* m, double alpha, const oosg_matrix* a, double beta,
+                                        const oosg_matrix* b, const double* scale)
+{
+    return guarded(
+        [&]
+        {
+            if (m == nullptr || a == nullptr || b == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            if (a->plan != m->plan || b->plan != m->plan)
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::combine: the matrices were made for other plans");
+            // The same structure: the plan fixes the positions, but not which of them hold entries.
+            if (a->offsets != m->offsets || b->offsets != m->offsets || a->rows != m->rows || b->rows != m->rows)
+                fail(OOSG_INVALID_ARGUMENT,
+                     "SymmetricCsc::combine: the matrices must have the same structure, found " +
+                         std::to_string(a->nnz) + " and " + std::to_string(b->nnz) +
+                         " terms or other rows; assemble both on one sparse pattern and convert them with one DOF "
+                         "selection");
+            oosg_plan& p = *m->plan;
+            setDevice(&p);
+            followCaller(p);
+            const hipStream_t st = p.stream;
+            if (m->dCsrMap.ptr == nullptr)
+            {
+                const uint64_t bytes = m->csrFromCsc.size() * sizeof(int64_t) + sizeof(unsigned long long);
+                if (p.reserved + bytes > p.budget)
+                    failBudget(p, bytes, "the matrix's row map (values combined on the device)");
+                m->dCsrMap = upload(m->csrFromCsc, st, "matrix: row map");
+                m->dCsrMap.account = &p.reserved;
+                p.reserved += m->dCsrMap.bytes;
+                allocateCounted(p, m->dNormKey, sizeof(unsigned long long), "matrix: norm");
+            }
+            DeviceBuffer dScaleIn;
+            if (scale != nullptr)
+            {
+                std::vector<double> host(scale, scale + m->n);
+                dScaleIn = upload(host, st, "matrix: scales");
+                check(hipStreamSynchronize(st), "matrix: scales");
+            }
+            check(oosg::launchCombineValues(m->n, m->dOffsets.as<int64_t>(), m->dRows.as<int32_t>(), alpha,
+                                            a->dValues.as<double>(), beta, b->dValues.as<double>(),
+                                            dScaleIn.ptr != nullptr ? dScaleIn.as<double>() : nullptr,
+                                            m->dValues.as<double>(), m->dDiagonal.as<double>(),
+                                            m->dScale.as<double>(), st),
+                  "matrix: combination");
+            check(oosg::launchRowValues(m->n, m->dCsrOffsets.as<int64_t>(), m->dCsrMap.as<int64_t>(),
+                                        m->dValues.as<double>(), m->dCsrValues.as<double>(),
+                                        m->dNormKey.as<unsigned long long>(), st),
+                  "matrix: rows");
+            unsigned long long key = 0;
+            check(hipMemcpyAsync(&key, m->dNormKey.ptr, sizeof(key), hipMemcpyDeviceToHost, st), "matrix: norm");
+            check(hipMemcpyAsync(m->diagonal.data(), m->dDiagonal.ptr, m->diagonal.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, st),
+                  "matrix: diagonal");
+            check(hipStreamSynchronize(st), "matrix: combination");
+            std::memcpy(&m->normInf, &key, sizeof(double));
+            m->hostStale = true;
         });
 }
 

@@ -4,9 +4,9 @@ The host side of the reference's eigenvalue pencils, as far as they feed the num
   combine / scale_symmetric   SymmetricCsc::combine and ::scaleSymmetric (src/linalg/sparse/SymmetricCsc.cpp:209-252)
   pencil_scale_of             pencilScaleOf (src/linalg/eigen/PencilScaling.cpp:7-60): powers of two near 1/sqrt(K_ii)
   SturmCheck                  include/oos/linalg/eigen/SturmCheck.hpp:53-72, src/linalg/eigen/SturmCheck.cpp:24-54
-  ShiftedPencil               the factorise(σ) / solve(block) / multiplyMass part of ShiftedPencil
-                              (include/oos/linalg/eigen/ShiftedPencil.hpp:34-88): S·(K − σM)·S factorised in indefinite
-                              mode on the device; its negative pivots are ν(σ), the eigenvalues of Kφ = λMφ below σ.
+  ShiftedPencil               ShiftedPencil (include/oos/linalg/eigen/ShiftedPencil.hpp:34-88) held on the device:
+                              S·(K − σM)·S formed and factorised in indefinite mode there, its negative pivots ν(σ) the
+                              eigenvalues of Kφ = λMφ below σ; the solves, M- and inner products on device blocks.
 """
 
 from dataclasses import dataclass
@@ -135,43 +135,96 @@ class ShiftFactorisation:
 
 
 class ShiftedPencil:
-    """K − σM on one symbolic structure (K and M assembled on one pattern, factor order). Each factorise(σ) forms
-    S·(K − σM)·S on the host (O(nnz)), copies its values into the plan's device matrix and factorises it in indefinite
-    mode on the GPU; solve(block) returns (K − σM)⁻¹·block = S·(S·(K − σM)·S)⁻¹·S·block."""
+    """The pencil (K, M) of K φ = λ M φ on one symbolic structure (K and M assembled on one pattern, factor order), as
+    the reference's eigenvalue solvers see it (ShiftedPencil, include/oos/linalg/eigen/ShiftedPencil.hpp:55-88), held
+    on the device: K and M are uploaded once; factorise(σ) forms S·(K − σM)·S on the device
+    (oosg_matrix_set_combination: SymmetricCsc::combine and scaleSymmetric with the reference's rounding) and
+    factorises it in indefinite mode; the factor stays resident for the solves of its blocks; the products with M and
+    with the inner-product matrix run on the device. Blocks are torch tensors on the device (or host arrays, copied).
 
-    def __init__(self, symbolic, stiffness, mass, device=0):
+    transform: "vibration" (the inner product of M) or "buckling" (M = B = −K_G, the inner product of K)."""
+
+    def __init__(self, symbolic, stiffness, mass, device=0, transform="vibration"):
         if not (np.array_equal(stiffness.offsets, mass.offsets) and np.array_equal(stiffness.rows, mass.rows)):
             raise ValueError("ShiftedPencil: K and M must have one structure")
+        if transform not in ("vibration", "buckling"):
+            raise ValueError("ShiftedPencil: transform is 'vibration' or 'buckling'")
         self.symbolic = symbolic
         self.stiffness = stiffness
         self.mass = mass
+        self._transform = transform
         self.scale = pencil_scale_of(stiffness)
         self.plan = symbolic.plan(device)
-        self.device_matrix = None
+        self.stiffness_device = self.plan.matrix(stiffness)
+        self.mass_device = self.plan.matrix(mass)
+        self.device_matrix = self.plan.matrix(stiffness)  # S·(K − σM)·S of the last shift
+        self.last = None
         self.sturm = SturmCheck()
+        self._scale_device = None
 
     @property
     def dimension(self):
         return self.stiffness.n
 
+    def transform(self):
+        return self._transform
+
     def factorise(self, shift):
-        shifted = scale_symmetric(combine(1.0, self.stiffness, -float(shift), self.mass), self.scale)
-        if self.device_matrix is None:
-            self.device_matrix = self.plan.matrix(shifted)
-        else:
-            self.device_matrix.set_values(shifted.values)
-        factor = SupernodalFactor.factorise(self.symbolic, shifted, FactorOptions(pivoting=PivotingMode.INDEFINITE),
+        """ShiftedPencil::factorise(σ): the factor of S·(K − σM)·S (formed on the device), kept for the solves; a
+        singular factor is returned, not raised (is_singular); its Sturm count is added to the check."""
+        self.device_matrix.set_combination(1.0, self.stiffness_device, -float(shift), self.mass_device, self.scale)
+        factor = SupernodalFactor.factorise(self.symbolic, None, FactorOptions(pivoting=PivotingMode.INDEFINITE),
                                             device_matrix=self.device_matrix)
         result = ShiftFactorisation(float(shift), factor, self.scale)
+        self.last = result
         self.sturm.add_point(SturmPoint(float(shift), result.count_below))
         return result
 
+    def _scale_on(self, device):
+        import torch
+        if self._scale_device is None or self._scale_device.device != device:
+            self._scale_device = torch.tensor(self.scale, device=device)
+        return self._scale_device
+
     def solve(self, factorisation, block):
-        """(K − σM)⁻¹·block for an n × k host block."""
+        """(K − σM)⁻¹·block = S·(S·(K − σM)·S)⁻¹·S·block with `factorisation` (the last one: ShiftedPencil::solve),
+        block n × k: a device tensor (solved on the device) or a host array."""
+        if hasattr(block, "data_ptr"):
+            s = self._scale_on(block.device)
+            s = s.reshape(-1, 1) if block.dim() == 2 else s
+            return s * FactorSolve(factorisation.factor).solve((s * block).contiguous())
         b = np.asarray(block, dtype=np.float64)
         s = self.scale.reshape(-1, 1) if b.ndim == 2 else self.scale
         return s * FactorSolve(factorisation.factor).solve(s * b)
 
     def multiply_mass(self, block):
-        """M·block (host arrays; the symmetric product from the lower CSC)."""
+        """M·block (ShiftedPencil::multiplyMass): on the device for a device tensor, on the host otherwise."""
+        if hasattr(block, "data_ptr"):
+            return self.mass_device.multiply(block)
         return self.mass.multiply(block)
+
+    def multiply_inner_product(self, block):
+        """W·block with W the inner product's matrix: M (vibration) or K (buckling) (multiplyInnerProduct)."""
+        if self._transform == "buckling":
+            if hasattr(block, "data_ptr"):
+                return self.stiffness_device.multiply(block)
+            return self.stiffness.multiply(block)
+        return self.multiply_mass(block)
+
+    def diagonal_rayleigh_bound(self):
+        """min K_ii / M_ii over M_ii > 0, K_ii > 0 (vibration), min K_ii / |M_ii| over M_ii ≠ 0 (buckling); +∞ when
+        there is none (diagonalRayleighBound)."""
+        k = _diagonal(self.stiffness)
+        m = _diagonal(self.mass)
+        ok = (k > 0) & ((m != 0) if self._transform == "buckling" else (m > 0))
+        return float(np.min(k[ok] / np.abs(m[ok]))) if np.any(ok) else float("inf")
+
+
+def _diagonal(matrix):
+    starts = matrix.offsets[:-1]
+    has = matrix.offsets[1:] > starts
+    first = np.where(has, starts, 0)
+    is_diag = has & (matrix.rows[first] == np.arange(matrix.n))
+    d = np.zeros(matrix.n)
+    d[is_diag] = matrix.values[first[is_diag]]
+    return d
