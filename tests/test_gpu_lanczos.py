@@ -27,7 +27,7 @@ def test_device_pencil_serves_a_block_lanczos_loop(gpu_device):
     k_dense, g_dense = b.stiffness.to_dense(), b.load.to_dense()
     mu = scipy.linalg.eigh(g_dense, k_dense, eigvals_only=True)
     lam = np.sort(1.0 / mu[mu > 1e-12 * np.max(mu)])
-    sigma = 0.5 * lam[0]
+    sigma = 0.97 * lam[0]  # below the first factor, near it: the wanted values lead the transformed spectrum
     f = pencil.factorise(sigma)
     assert f.count_below == 0 and not f.factor.is_singular
     # The shifted matrix formed on the device has the host's bits: its factor equals, bit for bit, the factor of the
@@ -47,7 +47,7 @@ def test_device_pencil_serves_a_block_lanczos_loop(gpu_device):
         r = torch.linalg.cholesky(0.5 * (g + g.T))
         return torch.linalg.solve_triangular(r, block.T, upper=False).T  # block·R⁻ᵀ: Kᵀ-orthonormal columns
 
-    p, steps = 4, 6
+    p, steps = 6, 10
     q = k_orthonormal(torch.tensor(np.random.default_rng(1).standard_normal((w.n, p)), device=gpu_device))
     basis, alphas, betas = [q], [], []
     previous, beta = None, None
@@ -80,4 +80,4 @@ def test_device_pencil_serves_a_block_lanczos_loop(gpu_device):
     ritz = np.sort(sigma * nu / (nu - 1.0))
     ritz = ritz[ritz > 0]
     print(f"[gpu] Ritz values {ritz[:4]}, dense {lam[:4]}", flush=True)
-    assert np.allclose(ritz[:3], lam[:3], rtol=1e-8)
+    assert np.allclose(ritz[:4], lam[:4], rtol=1e-9)
