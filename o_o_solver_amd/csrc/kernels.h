@@ -135,6 +135,13 @@ constexpr int SWEEP_CHUNK = 8;
 // products in the sweeps inside the workgroup of their triangle, with no row tiles or column blocks of their own: the
 // low levels' thousands of small supernodes then take one launch and one workgroup each per sweep.
 constexpr int SWEEP_INLINE_COLUMNS = 64, SWEEP_INLINE_ROWS = 128;
+// Supernodes swept by one wave each (tinyForwardKernel, tinyBackwardKernel): at most TINY_COLUMNS columns and
+// TINY_ROWS rows below them.
+constexpr int TINY_COLUMNS = 64, TINY_ROWS = 128;
+__host__ __device__ inline bool isTinySupernode(int c, int r)
+{
+    return c > 0 && c <= TINY_COLUMNS && r <= TINY_ROWS;
+}
 __host__ __device__ inline bool sweepRowsInline(int c, int r)
 {
     return c <= SWEEP_INLINE_COLUMNS && r <= SWEEP_INLINE_ROWS;
@@ -182,6 +189,13 @@ hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, 
                          const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
                          hipStream_t stream);
 // One level of the backward sweep: the column blocks (supernode, block) of its big supernodes, then all of `list`.
+// The sweeps of tiny supernodes (isTinySupernode): one wave each; maxOrder bounds c + r over the list (LDS).
+template <typename TP>
+hipError_t launchTinyForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                             const int32_t* list, int count, int maxOrder, hipStream_t stream);
+template <typename TP>
+hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                              const int32_t* list, int count, hipStream_t stream);
 template <typename TP>
 hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k, const int32_t* list,
                           int count, const int4* columnBlocks, int blockCount, double* partials, hipStream_t stream);
@@ -209,6 +223,7 @@ struct ColumnCoefficients
 // products (DOT_GROUPS × GRAM_PAIRS × 256 values).
 constexpr int DOT_GROUPS = 1024;
 constexpr int GRAM_PAIRS = 4;
+constexpr int GRAM_GROUPS = 256; // workgroups of a Gram product (a contiguous range of rows each)
 // Gram products of up to GRAM_PAIRS pairs of row-major n × k vectors (k <= 16): out[q·256 + 16a + b] =
 // Σ_i U_q(i, a)·V_q(i, b), zero beyond k; a fixed order of summation (deterministic).
 struct GramPairs

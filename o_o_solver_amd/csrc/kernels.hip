@@ -1390,6 +1390,153 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
     }
 }
 
+// ---- the sweeps of tiny supernodes: a wave each ---------------------------------------------------------------------
+//
+// Supernodes with at most TINY_COLUMNS columns and TINY_ROWS rows below them (the many small supernodes of the
+// lowest levels) are swept by one workgroup of one wave each, their values read straight from HBM into registers
+// (a lane per row), the right-hand sides of the front in LDS (n × KMAX doubles): many supernodes in flight per CU and
+// no LDS staging of L. The arithmetic is FP64 FMA (the panels' values widened).
+
+// Forward: y = [X(cols); 0] + the children's update vectors (child order), y_F ← L_FF⁻¹·y_F (the inverse of the
+// unit lower triangle, c × c column-major), X(cols) = y_F, U(s) = y_R − L_RF·y_F.
+template <typename TP>
+__global__ __launch_bounds__(64) void tinyForwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                        double* U, int k, const int32_t* list)
+{
+    extern __shared__ double y[]; // n × KMAX
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int lane = threadIdx.x;
+    for (int idx = lane; idx < n * KMAX; idx += 64)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        y[idx] = (i < c && h < k) ? X[int64_t(first + i) * k + h] : 0.0;
+    }
+    __syncthreads();
+    for (int q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+    {
+        const int ch = p.children[q];
+        const int total = p.nrow[ch] * k;
+        const int32_t* rel = p.rel + p.rowStart[ch];
+        const double* uc = U + p.rowStart[ch] * k;
+        // A child's rows land on distinct rows of the front: no two lanes add to one element.
+        constexpr int UNROLL = 8;
+        for (int base = lane; base < total; base += UNROLL * 64)
+        {
+            int at[UNROLL];
+            double v[UNROLL];
+#pragma unroll
+            for (int t = 0; t < UNROLL; ++t)
+            {
+                const int idx = min(base + t * 64, total - 1);
+                const int i = idx / k;
+                at[t] = rel[i] * KMAX + (idx - i * k);
+                v[t] = uc[idx];
+            }
+#pragma unroll
+            for (int t = 0; t < UNROLL; ++t)
+                if (base + t * 64 < total)
+                    y[pinned(at[t])] += pinned(v[t]);
+        }
+        __syncthreads();
+    }
+    // y_F ← L_FF⁻¹·y_F: lane i forms row i.
+    double out[KMAX];
+    const TP* inv = linv + p.linvOff[s];
+    if (lane < c)
+    {
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            out[h] = 0.0;
+        for (int j = 0; j <= lane; ++j)
+        {
+            const double a = double(inv[lane + j * c]);
+#pragma unroll
+            for (int h = 0; h < KMAX; ++h)
+                out[h] += a * y[j * KMAX + h];
+        }
+    }
+    __syncthreads();
+    if (lane < c)
+    {
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            y[lane * KMAX + h] = out[h];
+        for (int h = 0; h < k; ++h)
+            X[int64_t(first + lane) * k + h] = out[h];
+    }
+    __syncthreads();
+    // U(s) = y_R − L_RF·y_F: lane ρ forms rows ρ, ρ + 64 of the update vector.
+    const TP* L = panels + p.panelOff[s];
+    double* u = U + p.rowStart[s] * k;
+    for (int rho = lane; rho < r; rho += 64)
+    {
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            out[h] = y[(c + rho) * KMAX + h];
+        const TP* row = L + c + rho;
+        for (int j = 0; j < c; ++j)
+        {
+            const double l = double(row[int64_t(j) * n]);
+#pragma unroll
+            for (int h = 0; h < KMAX; ++h)
+                out[h] -= l * y[j * KMAX + h];
+        }
+        for (int h = 0; h < k; ++h)
+            u[int64_t(rho) * k + h] = out[h];
+    }
+}
+
+// Backward: x_F = X(cols) − L_RFᵀ·X(rows), then X(cols) = L_FF⁻ᵀ·x_F (lane j forms column j's row of x).
+template <typename TP>
+__global__ __launch_bounds__(64) void tinyBackwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                         int k, const int32_t* list)
+{
+    __shared__ double x[TINY_COLUMNS * KMAX];
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int lane = threadIdx.x;
+    const TP* L = panels + p.panelOff[s];
+    const int32_t* rows = p.rows + p.rowStart[s];
+    double acc[KMAX];
+    if (lane < c)
+    {
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            acc[h] = h < k ? X[int64_t(first + lane) * k + h] : 0.0;
+        const TP* column = L + int64_t(lane) * n + c;
+        for (int rho = 0; rho < r; ++rho)
+        {
+            const double* xr = X + int64_t(rows[rho]) * k; // the same row on every lane
+            const double l = double(column[rho]);
+#pragma unroll
+            for (int h = 0; h < KMAX; ++h)
+                if (h < k)
+                    acc[h] -= l * xr[h];
+        }
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            x[lane * KMAX + h] = acc[h];
+    }
+    __syncthreads();
+    if (lane < c)
+    {
+        const TP* inv = linv + p.linvOff[s] + int64_t(lane) * c; // column j of the c × c inverse
+#pragma unroll
+        for (int h = 0; h < KMAX; ++h)
+            acc[h] = 0.0;
+        for (int i = lane; i < c; ++i)
+        {
+            const double a = double(inv[i]);
+#pragma unroll
+            for (int h = 0; h < KMAX; ++h)
+                acc[h] += a * x[i * KMAX + h];
+        }
+        for (int h = 0; h < k; ++h)
+            X[int64_t(first + lane) * k + h] = acc[h];
+    }
+}
+
 // The update vectors of the big supernodes of a level: U(s)(rows of the tile) −= L_RF(tile, :)·X(cols(s)).
 template <typename TP>
 __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP* panels, const double* X, double* U,
@@ -2061,33 +2208,57 @@ __global__ __launch_bounds__(256) void blockGramKernel(GramPairs g, int64_t n, i
     }
 }
 
-// out[q·256 + e] = Σ_g partial[(g·GRAM_PAIRS + q)·256 + e], the workgroups in index order (workgroup q, thread e).
+// out[q·256 + e] = Σ_g partial[(g·GRAM_PAIRS + q)·256 + e], the workgroups in index order (workgroup q, thread e),
+// eight loads in flight.
 __global__ __launch_bounds__(256) void sumGramPartialsKernel(const double* partial, int groups, double* out)
 {
     const int q = blockIdx.x, e = threadIdx.x;
     double sum = 0.0;
-    for (int g = 0; g < groups; ++g)
-        sum += partial[(int64_t(g) * GRAM_PAIRS + q) * 256 + e];
+    for (int g = 0; g < groups; g += 8)
+    {
+        double v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            v[t] = g + t < groups ? partial[(int64_t(g + t) * GRAM_PAIRS + q) * 256 + e] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            sum += v[t];
+    }
     out[q * 256 + e] = sum;
 }
 
-// Thread (i, b): Y(i, b) = c0·Y(i, b) + Σ_a U(i, a)·C(a, b), a increasing.
+// Y(rows, :) = c0·Y(rows, :) + U(rows, :)·C, a wave per 16 rows: four 16 × 16 × 4 FP64 MFMA steps over the k
+// columns of U (lane l: A(l mod 16, l / 16) = U(row0 + l mod 16, 4s + l / 16), B(l / 16, l mod 16) = C(4s + l / 16,
+// l mod 16)), then each lane adds c0·Y to its four results and stores them.
 __global__ __launch_bounds__(256) void blockCombineKernel(double* y, BlockTerm m, int64_t n, int k)
 {
-    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    const int64_t i = t >> 4;
-    const int b = int(t & 15);
-    if (i >= n || b >= k)
-        return;
-    double sum = m.c0 == 0.0 ? 0.0 : m.c0 * y[i * k + b];
-    const double* u = m.u + i * k;
-    if (m.deviceC != nullptr)
-        for (int a = 0; a < k; ++a)
-            sum += u[a] * (m.scale * m.deviceC[a * 16 + b]);
-    else
-        for (int a = 0; a < k; ++a)
-            sum += u[a] * m.c[a * 16 + b];
-    y[i * k + b] = sum;
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6) * 16;
+    if (row0 >= n)
+        return; // the whole wave
+    const int r = lane & 15, q = lane >> 4;
+    const int64_t i = row0 + r;
+    double4_t acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int step = 0; step < 4; ++step)
+    {
+        const int a = 4 * step + q;
+        const double av = (i < n && a < k) ? m.u[i * k + a] : 0.0;
+        double bv = 0.0;
+        if (a < k && r < k)
+            bv = m.deviceC != nullptr ? m.scale * m.deviceC[a * 16 + r] : m.c[a * 16 + r];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int64_t row = row0 + q + 4 * v;
+        if (row < n && r < k)
+        {
+            double* at = y + row * k + r;
+            *at = (m.c0 == 0.0 ? 0.0 : m.c0 * *at) + acc[v];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const double* p, const double* q,
@@ -2241,6 +2412,25 @@ hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateT
         return hipSuccess;
     hipLaunchKernelGGL((updateTilesKernel<double, true>), dim3(count), dim3(256), 0, stream, DevicePlan{}, fronts,
                        nullptr, const_cast<double*>(w), tiles, count, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+template <typename TP>
+hipError_t launchTinyForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                             const int32_t* list, int count, int maxOrder, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(tinyForwardKernel<TP>, dim3(count), dim3(64), size_t(maxOrder) * KMAX * sizeof(double),
+                           stream, p, panels, linv, X, U, k, list);
+    return hipGetLastError();
+}
+
+template <typename TP>
+hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                              const int32_t* list, int count, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(tinyBackwardKernel<TP>, dim3(count), dim3(64), 0, stream, p, panels, linv, X, k, list);
     return hipGetLastError();
 }
 
@@ -2400,15 +2590,15 @@ hipError_t launchRowValues(int32_t n, const int64_t* csrOffsets, const int64_t* 
 hipError_t launchBlockGram(const GramPairs& pairs, int64_t n, int k, double* partial, double* out,
                            hipStream_t stream)
 {
-    hipLaunchKernelGGL(blockGramKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, pairs, n, k, partial);
-    hipLaunchKernelGGL(sumGramPartialsKernel, dim3(pairs.pairs), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+    hipLaunchKernelGGL(blockGramKernel, dim3(GRAM_GROUPS), dim3(256), 0, stream, pairs, n, k, partial);
+    hipLaunchKernelGGL(sumGramPartialsKernel, dim3(pairs.pairs), dim3(256), 0, stream, partial, GRAM_GROUPS, out);
     return hipGetLastError();
 }
 
 hipError_t launchBlockCombine(double* y, const BlockTerm& term, int64_t n, int k, hipStream_t stream)
 {
-    const int64_t threads = n * 16;
-    hipLaunchKernelGGL(blockCombineKernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, (threads + 255) / 256))),
+    const int64_t waves = (n + 15) / 16;
+    hipLaunchKernelGGL(blockCombineKernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, (waves + 3) / 4))),
                        dim3(256), 0, stream, y, term, n, k);
     return hipGetLastError();
 }
@@ -2596,6 +2786,10 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
                                          int, const int4*, int, double*, hipStream_t);                                 \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int4*, int, double*, hipStream_t);                                     \
+    template hipError_t launchTinyForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \
+                                             const int32_t*, int, int, hipStream_t);                                   \
+    template hipError_t launchTinyBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int, \
+                                              hipStream_t);                                                            \
     template hipError_t launchTriangleInverse<T>(const DevicePlan&, const T*, T*, const int2*, int, int, hipStream_t); \
     template hipError_t launchTriangleProducts<T>(const DevicePlan&, const T*, const double*, int, const int4*, int,   \
                                                   int, double*, hipStream_t);                                          \

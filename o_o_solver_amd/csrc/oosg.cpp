@@ -389,6 +389,9 @@ struct oosg_plan
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
+    // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others, with per level the
+    // largest order c + r of its tiny ones.
+    std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, tinyOrder;
     std::vector<int64_t> rowStart, panelOff, updateOff, linvOff;
     // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward), in chunks where
     // their products are long, with the subtractions of the chunks' partial sums; the most slots of partial sums a
@@ -437,6 +440,7 @@ struct oosg_plan
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
+    DeviceBuffer dTinyList, dOtherList;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dForwardSubtractions, dBackwardSubtractions,
         dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
@@ -1062,6 +1066,27 @@ void deriveLevels(oosg_plan& p)
         p.inverseRounds.push_back(std::move(rounds));
     }
     placeUpdateMatrices(p, height);
+    p.tinyList.clear();
+    p.otherList.clear();
+    p.tinyStart.assign(p.levels + 1, 0);
+    p.otherStart.assign(p.levels + 1, 0);
+    p.tinyOrder.assign(p.levels, 0);
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+        {
+            const int32_t s = p.levelList[q];
+            if (oosg::isTinySupernode(p.ncol[s], p.nrow[s]))
+            {
+                p.tinyList.push_back(s);
+                p.tinyOrder[l] = std::max(p.tinyOrder[l], p.ncol[s] + p.nrow[s]);
+            }
+            else
+                p.otherList.push_back(s);
+        }
+        p.tinyStart[l + 1] = static_cast<int32_t>(p.tinyList.size());
+        p.otherStart[l + 1] = static_cast<int32_t>(p.otherList.size());
+    }
 }
 
 // The algorithmic flops of a tile: 2·k per element it updates (the lower part only on the diagonal).
@@ -1501,6 +1526,8 @@ void uploadSweepPlan(oosg_plan& p)
     p.dBackwardTriangleSums = upload(p.backwardTriangleSums, st, "plan: triangle blocks");
     p.dInverseItems = upload(p.inverseItems, st, "plan: triangle inverse blocks");
     p.dLevelList = upload(p.levelList, st, "plan: levels");
+    p.dTinyList = upload(p.tinyList, st, "plan: tiny supernodes");
+    p.dOtherList = upload(p.otherList, st, "plan: sweep supernodes");
     p.dp.first = p.dFirst.as<int32_t>();
     p.dp.ncol = p.dNcol.as<int32_t>();
     p.dp.nrow = p.dNrow.as<int32_t>();
@@ -1564,7 +1591,8 @@ std::vector<uint64_t> solveBufferBytes(const oosg_plan& p)
             static_cast<uint64_t>(std::max<int64_t>(p.partialSlots, 1)) * 64 * 16 * sizeof(double), // solvePartials
             3 * 16 * sizeof(unsigned long long),                                            // solveNorms
             vector, vector, vector, vector, vector, vector,                                 // cgB .. cgQ
-            static_cast<uint64_t>(oosg::DOT_GROUPS) * oosg::GRAM_PAIRS * 256 * sizeof(double), // cgPartial
+            static_cast<uint64_t>(std::max(oosg::DOT_GROUPS * 16, oosg::GRAM_GROUPS * oosg::GRAM_PAIRS * 256)) *
+                sizeof(double), // cgPartial
             static_cast<uint64_t>(oosg::GRAM_PAIRS) * 256 * sizeof(double),                  // cgDots
             static_cast<uint64_t>(std::max(p.S, 1)) * sizeof(int32_t),                      // pivotStatus
             sizeof(oosg::PivotStatistics)};                                                 // pivotStats
@@ -2332,7 +2360,8 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
 {
     double* X = p.solveX.as<double>();
     double* U = p.solveU.as<double>();
-    const int32_t* levels = p.dLevelList.as<int32_t>();
+    const int32_t* tiny = p.dTinyList.as<int32_t>();
+    const int32_t* others = p.dOtherList.as<int32_t>();
     const int4* forwardTiles = p.dForwardTiles.as<int4>();
     const int4* subtractions = p.dForwardSubtractions.as<int4>();
     const int4* triangles = p.dForwardTriangles.as<int4>();
@@ -2355,8 +2384,11 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
         // Gathers and triangles, the wide supernodes' triangles (products with their whole inverses in chunks of
         // blocks, whose sums replace the rows of X once every product has read X), then every L_RF row tile.
         timed(PROFILE_FORWARD, w.triangleValues * T + w.forwardRows * row, [&] {
-            check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, levels + p.levelStart[l],
-                                          p.levelStart[l + 1] - p.levelStart[l], nullptr, 0, partials, st),
+            check(oosg::launchTinyForward<TP>(p.dp, panels, linv, X, U, k, tiny + p.tinyStart[l],
+                                              p.tinyStart[l + 1] - p.tinyStart[l], p.tinyOrder[l], st),
+                  "solve: forward sweep (tiny supernodes)");
+            check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, others + p.otherStart[l],
+                                          p.otherStart[l + 1] - p.otherStart[l], nullptr, 0, partials, st),
                   "solve: forward sweep");
         });
         const int items = static_cast<int>(p.forwardTriangleStart[l + 1] - p.forwardTriangleStart[l]);
@@ -2393,7 +2425,8 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
 {
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
-    const int32_t* levels = p.dLevelList.as<int32_t>();
+    const int32_t* tiny = p.dTinyList.as<int32_t>();
+    const int32_t* others = p.dOtherList.as<int32_t>();
     const int4* backwardBlocks = p.dBackwardBlocks.as<int4>();
     const int4* subtractions = p.dBackwardSubtractions.as<int4>();
     double* partials = p.solvePartials.as<double>();
@@ -2422,7 +2455,7 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
         const int subtracted = static_cast<int>(p.backwardSubtractionStart[l + 1] - p.backwardSubtractionStart[l]);
         if (blocks > 0)
             timed(PROFILE_BACKWARD_COLUMNS, w.rowValues * T + w.columnRows * row, [&] {
-                check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels, 0,
+                check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, others, 0,
                                                backwardBlocks + p.backwardBlockStart[l], blocks, partials, st),
                       "solve: backward sweep (columns)");
                 check(oosg::launchSubtractPartials(X, nullptr, k, subtractions + p.backwardSubtractionStart[l],
@@ -2442,8 +2475,11 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
                   "solve: backward sweep (triangles)");
         }
         timed(PROFILE_BACKWARD, w.triangleValues * T + w.backwardRows * row, [&] {
-            check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, levels + p.levelStart[l],
-                                           p.levelStart[l + 1] - p.levelStart[l], backwardBlocks, 0, partials, st),
+            check(oosg::launchTinyBackward<TP>(p.dp, panels, linv, X, k, tiny + p.tinyStart[l],
+                                               p.tinyStart[l + 1] - p.tinyStart[l], st),
+                  "solve: backward sweep (tiny supernodes)");
+            check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, others + p.otherStart[l],
+                                           p.otherStart[l + 1] - p.otherStart[l], backwardBlocks, 0, partials, st),
                   "solve: backward sweep");
         });
     }
