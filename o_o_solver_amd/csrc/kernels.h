@@ -32,6 +32,12 @@ struct DevicePlan
     const int32_t* childStart;  // [S + 1]
     const int32_t* children;    // children in increasing order
     const int64_t* linvOff;     // [S] offsets of the inverses of the diagonal blocks (PANEL_BLOCK columns each)
+    // The pull table of the tiny supernodes' forward sweep: position q of the front of s is hit by the update rows
+    // (indices into the rows of all supernodes) pulls[pullBase[s] + q], in child order, -1 past the last, at most
+    // pullWidth <= TINY_PULLS of them anywhere; pullBase[s] is -1 for a supernode without children or not tiny.
+    const int32_t* pullBase;    // [S]
+    const int4* pulls;
+    int32_t pullWidth;
 };
 
 // A matrix in device memory: its lower CSC, the front position of each entry (within the front of the supernode
@@ -138,6 +144,9 @@ constexpr int SWEEP_INLINE_COLUMNS = 64, SWEEP_INLINE_ROWS = 128;
 // Supernodes swept by one wave each (tinyForwardKernel, tinyBackwardKernel): at most TINY_COLUMNS columns and
 // TINY_ROWS rows below them.
 constexpr int TINY_COLUMNS = 64, TINY_ROWS = 128;
+// At most this many children's update rows may hit one position of a tiny supernode's front (else it is swept by the
+// kernels of the other supernodes).
+constexpr int TINY_PULLS = 4;
 __host__ __device__ inline bool isTinySupernode(int c, int r)
 {
     return c > 0 && c <= TINY_COLUMNS && r <= TINY_ROWS;
@@ -189,10 +198,10 @@ hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, 
                          const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
                          hipStream_t stream);
 // One level of the backward sweep: the column blocks (supernode, block) of its big supernodes, then all of `list`.
-// The sweeps of tiny supernodes (isTinySupernode): one wave each; maxOrder bounds c + r over the list (LDS).
+// The sweeps of tiny supernodes (isTinySupernode and a pull table entry when they have children): one wave each.
 template <typename TP>
 hipError_t launchTinyForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                             const int32_t* list, int count, int maxOrder, hipStream_t stream);
+                             const int32_t* list, int count, hipStream_t stream);
 template <typename TP>
 hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
                               const int32_t* list, int count, hipStream_t stream);
@@ -223,7 +232,7 @@ struct ColumnCoefficients
 // products (DOT_GROUPS × GRAM_PAIRS × 256 values).
 constexpr int DOT_GROUPS = 1024;
 constexpr int GRAM_PAIRS = 4;
-constexpr int GRAM_GROUPS = 256; // workgroups of a Gram product (a contiguous range of rows each)
+constexpr int GRAM_GROUPS = 1024; // workgroups of a Gram product (a contiguous range of rows each)
 // Gram products of up to GRAM_PAIRS pairs of row-major n × k vectors (k <= 16): out[q·256 + 16a + b] =
 // Σ_i U_q(i, a)·V_q(i, b), zero beyond k; a fixed order of summation (deterministic).
 struct GramPairs

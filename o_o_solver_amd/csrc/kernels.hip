@@ -1393,148 +1393,249 @@ __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* pan
 // ---- the sweeps of tiny supernodes: a wave each ---------------------------------------------------------------------
 //
 // Supernodes with at most TINY_COLUMNS columns and TINY_ROWS rows below them (the many small supernodes of the
-// lowest levels) are swept by one workgroup of one wave each, their values read straight from HBM into registers
-// (a lane per row), the right-hand sides of the front in LDS (n × KMAX doubles): many supernodes in flight per CU and
-// no LDS staging of L. The arithmetic is FP64 FMA (the panels' values widened).
+// lowest levels) are swept by one wave each, TINY_WAVES waves a workgroup, with FP64 MFMA products (16 × 16 × 4) on
+// values read straight from HBM into registers and no LDS, so that occupancy is bounded by registers alone. Lane
+// l = 16·lq + lr holds a 16-row tile of a block of vectors as the MFMA result layout: its register v holds row lq + 4v,
+// column (right-hand side) lr. That is also the B-operand layout of k-step v of a product whose inner dimension runs
+// over those 16 rows, so one product's result is the next one's operand in place.
+//
+// The children's update vectors reach a front by the pull table: position q of the front of s (a column for q < c,
+// else row q − c below) is hit by the update rows pulls[pullBase[s] + q] (at most TINY_PULLS, in child order, -1 past
+// the last); they are added to the front in that order, as the assembly of the factorisation does.
+constexpr int TINY_WAVES = 4;
 
-// Forward: y = [X(cols); 0] + the children's update vectors (child order), y_F ← L_FF⁻¹·y_F (the inverse of the
-// unit lower triangle, c × c column-major), X(cols) = y_F, U(s) = y_R − L_RF·y_F.
-template <typename TP>
-__global__ __launch_bounds__(64) void tinyForwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
-                                                        double* U, int k, const int32_t* list)
+// The update rows that land on position q of the front of s (base = pullBase[s] >= 0).
+__device__ inline int4 pullsOf(const DevicePlan& p, int base, int q)
 {
-    extern __shared__ double y[]; // n × KMAX
-    const int s = list[blockIdx.x];
-    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
-    const int lane = threadIdx.x;
-    for (int idx = lane; idx < n * KMAX; idx += 64)
-    {
-        const int i = idx >> 4, h = idx & (KMAX - 1);
-        y[idx] = (i < c && h < k) ? X[int64_t(first + i) * k + h] : 0.0;
-    }
-    __syncthreads();
-    for (int q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
-    {
-        const int ch = p.children[q];
-        const int total = p.nrow[ch] * k;
-        const int32_t* rel = p.rel + p.rowStart[ch];
-        const double* uc = U + p.rowStart[ch] * k;
-        // A child's rows land on distinct rows of the front: no two lanes add to one element.
-        constexpr int UNROLL = 8;
-        for (int base = lane; base < total; base += UNROLL * 64)
-        {
-            int at[UNROLL];
-            double v[UNROLL];
+    return p.pulls[base + q];
+}
+
+// y + the values of column h of the update rows e (in order; -1: none; row 0 stands in for none as an address). The
+// loads are issued together, the sums formed after them.
+__device__ inline double addPulled(const DevicePlan& p, int4 e, double y, const double* U, int k, int h)
+{
+    const double u0 = U[int64_t(max(e.x, 0)) * k + h];
+    const double u1 = p.pullWidth > 1 ? U[int64_t(max(e.y, 0)) * k + h] : 0.0;
+    const double u2 = p.pullWidth > 2 ? U[int64_t(max(e.z, 0)) * k + h] : 0.0;
+    const double u3 = p.pullWidth > 3 ? U[int64_t(max(e.w, 0)) * k + h] : 0.0;
+    y = e.x >= 0 ? y + u0 : y;
+    y = e.y >= 0 ? y + u1 : y;
+    y = e.z >= 0 ? y + u2 : y;
+    y = e.w >= 0 ? y + u3 : y;
+    return y;
+}
+
+// Forward: y_F = X(cols) + pulls, y_F ← L_FF⁻¹·y_F (the inverse of the unit lower triangle, c × c column-major),
+// X(cols) = y_F, U(s) = pulls(rows) − L_RF·y_F.
+template <typename TP>
+__global__ __launch_bounds__(64 * TINY_WAVES) void tinyForwardKernel(DevicePlan p, const TP* panels, const TP* linv,
+                                                                     double* X, double* U, int k,
+                                                                     const int32_t* list, int count)
+{
+    const int at = blockIdx.x * TINY_WAVES + (threadIdx.x >> 6);
+    if (at >= count)
+        return; // the whole wave: no barrier follows
+    const int s = list[at];
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s], base = p.pullBase[s];
+    const int ct = (c + 15) >> 4, rt = (r + 15) >> 4;
+    const bool column = lr < k;
+    double4_t y[4];
 #pragma unroll
-            for (int t = 0; t < UNROLL; ++t)
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            const bool valid = t < ct && i < c && column;
+            const double x = X[valid ? int64_t(first + i) * k + lr : 0];
+            y[t][v] = valid ? x : 0.0;
+        }
+    if (base >= 0)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t < ct)
             {
-                const int idx = min(base + t * 64, total - 1);
-                const int i = idx / k;
-                at[t] = rel[i] * KMAX + (idx - i * k);
-                v[t] = uc[idx];
+                int4 e[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    e[v] = pullsOf(p, base, min(16 * t + lq + 4 * v, c - 1));
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int i = 16 * t + lq + 4 * v;
+                    if (i < c && column)
+                        y[t][v] = addPulled(p, e[v], y[t][v], U, k, lr);
+                }
             }
-#pragma unroll
-            for (int t = 0; t < UNROLL; ++t)
-                if (base + t * 64 < total)
-                    y[pinned(at[t])] += pinned(v[t]);
-        }
-        __syncthreads();
-    }
-    // y_F ← L_FF⁻¹·y_F: lane i forms row i.
-    double out[KMAX];
+    // y_F ← L_FF⁻¹·y_F, tiles from the last: tile t reads tiles 0..t only.
     const TP* inv = linv + p.linvOff[s];
-    if (lane < c)
-    {
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            out[h] = 0.0;
-        for (int j = 0; j <= lane; ++j)
+    for (int t = 3; t >= 0; --t)
+        if (t < ct)
         {
-            const double a = double(inv[lane + j * c]);
+            const int i = 16 * t + lr;
+            double a[16];
 #pragma unroll
-            for (int h = 0; h < KMAX; ++h)
-                out[h] += a * y[j * KMAX + h];
+            for (int q = 0; q < 16; ++q)
+            {
+                const int j = 4 * q + lq; // column 16·kt + 4m + lq, kt = q / 4, m = q % 4
+                const bool valid = (q >> 2) <= t && i < c && j <= i;
+                const double value = double(inv[valid ? i + j * c : 0]);
+                a[q] = valid ? value : 0.0;
+            }
+            double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if ((q >> 2) <= t)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], y[q >> 2][q & 3], acc, 0, 0, 0);
+            y[t] = acc;
         }
-    }
-    __syncthreads();
-    if (lane < c)
-    {
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            y[lane * KMAX + h] = out[h];
-        for (int h = 0; h < k; ++h)
-            X[int64_t(first + lane) * k + h] = out[h];
-    }
-    __syncthreads();
-    // U(s) = y_R − L_RF·y_F: lane ρ forms rows ρ, ρ + 64 of the update vector.
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            if (t < ct && i < c && column)
+                X[int64_t(first + i) * k + lr] = y[t][v];
+        }
+    // U(s) = pulls(rows) − L_RF·y_F, a 16-row tile at a time; the pull entries of the next tile are loaded with this
+    // tile's values.
     const TP* L = panels + p.panelOff[s];
     double* u = U + p.rowStart[s] * k;
-    for (int rho = lane; rho < r; rho += 64)
+    int4 e[4];
+    const int4 none = {-1, -1, -1, -1};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        e[v] = base >= 0 && r > 0 ? pullsOf(p, base, c + min(lq + 4 * v, r - 1)) : none;
+    for (int t = 0; t < rt; ++t)
     {
+        const int rho = 16 * t + lr; // the A operand's row
+        TP a[16];
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            out[h] = y[(c + rho) * KMAX + h];
-        const TP* row = L + c + rho;
-        for (int j = 0; j < c; ++j)
+        for (int q = 0; q < 16; ++q)
         {
-            const double l = double(row[int64_t(j) * n]);
-#pragma unroll
-            for (int h = 0; h < KMAX; ++h)
-                out[h] -= l * y[j * KMAX + h];
+            const int j = 4 * q + lq;
+            const bool valid = (q >> 2) < ct && rho < r && j < c;
+            const TP value = L[valid ? (c + rho) + int64_t(j) * n : 0];
+            a[q] = valid ? value : TP(0);
         }
-        for (int h = 0; h < k; ++h)
-            u[int64_t(rho) * k + h] = out[h];
+        double4_t acc = {0.0, 0.0, 0.0, 0.0};
+        int4 next[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            if (base >= 0 && i < r && column)
+                acc[v] = addPulled(p, e[v], 0.0, U, k, lr);
+            next[v] = base >= 0 && t + 1 < rt ? pullsOf(p, base, c + min(i + 16, r - 1)) : none;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if ((q >> 2) < ct)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-double(a[q]), y[q >> 2][q & 3], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            if (i < r && column)
+                u[int64_t(i) * k + lr] = acc[v];
+            e[v] = next[v];
+        }
     }
 }
 
-// Backward: x_F = X(cols) − L_RFᵀ·X(rows), then X(cols) = L_FF⁻ᵀ·x_F (lane j forms column j's row of x).
+// Backward: x_F = X(cols) − L_RFᵀ·X(rows), then X(cols) = L_FF⁻ᵀ·x_F.
 template <typename TP>
-__global__ __launch_bounds__(64) void tinyBackwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
-                                                         int k, const int32_t* list)
+__global__ __launch_bounds__(64 * TINY_WAVES) void tinyBackwardKernel(DevicePlan p, const TP* panels, const TP* linv,
+                                                                      double* X, int k, const int32_t* list, int count)
 {
-    __shared__ double x[TINY_COLUMNS * KMAX];
-    const int s = list[blockIdx.x];
+    const int at = blockIdx.x * TINY_WAVES + (threadIdx.x >> 6);
+    if (at >= count)
+        return; // the whole wave: no barrier follows
+    const int s = list[at];
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
-    const int lane = threadIdx.x;
+    const int ct = (c + 15) >> 4;
+    const bool column = lr < k;
+    double4_t x[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            const bool valid = t < ct && i < c && column;
+            const double value = X[valid ? int64_t(first + i) * k + lr : 0];
+            x[t][v] = valid ? value : 0.0;
+        }
+    // x_F −= L_RFᵀ·X(rows): k-steps of 4 rows below, 8 at a time (the loads of a batch all in flight).
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
-    double acc[KMAX];
-    if (lane < c)
+    for (int r0 = 0; r0 < r; r0 += 32)
     {
+        double b[8];
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            acc[h] = h < k ? X[int64_t(first + lane) * k + h] : 0.0;
-        const TP* column = L + int64_t(lane) * n + c;
-        for (int rho = 0; rho < r; ++rho)
+        for (int m = 0; m < 8; ++m)
         {
-            const double* xr = X + int64_t(rows[rho]) * k; // the same row on every lane
-            const double l = double(column[rho]);
-#pragma unroll
-            for (int h = 0; h < KMAX; ++h)
-                if (h < k)
-                    acc[h] -= l * xr[h];
+            const int rho = r0 + 4 * m + lq;
+            const bool valid = rho < r;
+            const int32_t row = rows[valid ? rho : 0];
+            const double value = X[valid && column ? int64_t(row) * k + lr : 0];
+            b[m] = valid && column ? value : 0.0;
         }
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            x[lane * KMAX + h] = acc[h];
-    }
-    __syncthreads();
-    if (lane < c)
-    {
-        const TP* inv = linv + p.linvOff[s] + int64_t(lane) * c; // column j of the c × c inverse
+        for (int t = 0; t < 4; ++t)
+            if (t < ct)
+            {
+                const int j = 16 * t + lr; // the A operand's row: column j of L
+                double a[8];
 #pragma unroll
-        for (int h = 0; h < KMAX; ++h)
-            acc[h] = 0.0;
-        for (int i = lane; i < c; ++i)
+                for (int m = 0; m < 8; ++m)
+                {
+                    const int rho = r0 + 4 * m + lq;
+                    const bool valid = rho < r && j < c;
+                    const double value = double(L[valid ? (c + rho) + int64_t(j) * n : 0]);
+                    a[m] = valid ? -value : 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                    x[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[m], x[t], 0, 0, 0);
+            }
+    }
+    // X(cols) = L_FF⁻ᵀ·x_F, tiles from the first: tile t reads tiles t..3 only.
+    const TP* inv = linv + p.linvOff[s];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (t < ct)
         {
-            const double a = double(inv[i]);
+            const int i = 16 * t + lr; // row i of the transposed inverse: column i of the inverse
+            double a[16];
 #pragma unroll
-            for (int h = 0; h < KMAX; ++h)
-                acc[h] += a * x[i * KMAX + h];
+            for (int q = 0; q < 16; ++q)
+            {
+                const int j = 4 * q + lq;
+                const bool valid = (q >> 2) >= t && (q >> 2) < ct && j < c && j >= i;
+                const double value = double(inv[valid ? j + i * c : 0]);
+                a[q] = valid ? value : 0.0;
+            }
+            double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if ((q >> 2) >= t && (q >> 2) < ct)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], x[q >> 2][q & 3], acc, 0, 0, 0);
+            x[t] = acc;
         }
-        for (int h = 0; h < k; ++h)
-            X[int64_t(first + lane) * k + h] = acc[h];
-    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            if (t < ct && i < c && column)
+                X[int64_t(first + i) * k + lr] = x[t][v];
+        }
 }
 
 // The update vectors of the big supernodes of a level: U(s)(rows of the tile) −= L_RF(tile, :)·X(cols(s)).
@@ -2417,11 +2518,11 @@ hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateT
 
 template <typename TP>
 hipError_t launchTinyForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
-                             const int32_t* list, int count, int maxOrder, hipStream_t stream)
+                             const int32_t* list, int count, hipStream_t stream)
 {
     if (count > 0)
-        hipLaunchKernelGGL(tinyForwardKernel<TP>, dim3(count), dim3(64), size_t(maxOrder) * KMAX * sizeof(double),
-                           stream, p, panels, linv, X, U, k, list);
+        hipLaunchKernelGGL(tinyForwardKernel<TP>, dim3((count + TINY_WAVES - 1) / TINY_WAVES), dim3(64 * TINY_WAVES), 0,
+                           stream, p, panels, linv, X, U, k, list, count);
     return hipGetLastError();
 }
 
@@ -2430,7 +2531,8 @@ hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* l
                               const int32_t* list, int count, hipStream_t stream)
 {
     if (count > 0)
-        hipLaunchKernelGGL(tinyBackwardKernel<TP>, dim3(count), dim3(64), 0, stream, p, panels, linv, X, k, list);
+        hipLaunchKernelGGL(tinyBackwardKernel<TP>, dim3((count + TINY_WAVES - 1) / TINY_WAVES), dim3(64 * TINY_WAVES),
+                           0, stream, p, panels, linv, X, k, list, count);
     return hipGetLastError();
 }
 
@@ -2787,7 +2889,7 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int4*, int, double*, hipStream_t);                                     \
     template hipError_t launchTinyForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \
-                                             const int32_t*, int, int, hipStream_t);                                   \
+                                             const int32_t*, int, hipStream_t);                                        \
     template hipError_t launchTinyBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int, \
                                               hipStream_t);                                                            \
     template hipError_t launchTriangleInverse<T>(const DevicePlan&, const T*, T*, const int2*, int, int, hipStream_t); \

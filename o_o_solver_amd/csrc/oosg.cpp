@@ -389,9 +389,10 @@ struct oosg_plan
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
-    // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others, with per level the
-    // largest order c + r of its tiny ones.
-    std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, tinyOrder;
+    // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others; the tiny ones' pull
+    // table (oosg::DevicePlan::pulls) and its width.
+    std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, pullBase, pulls;
+    int32_t pullWidth = 0;
     std::vector<int64_t> rowStart, panelOff, updateOff, linvOff;
     // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward), in chunks where
     // their products are long, with the subtractions of the chunks' partial sums; the most slots of partial sums a
@@ -440,7 +441,7 @@ struct oosg_plan
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dTinyList, dOtherList;
+    DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dForwardSubtractions, dBackwardSubtractions,
         dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
@@ -773,6 +774,86 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
     deriveLevels(p);
 }
 
+// Splits each level for the sweeps into its tiny supernodes (isTinySupernode, and no front position hit by more than
+// TINY_PULLS children's update rows) and the others, and builds the tiny ones' pull table: for each position of the
+// front of a tiny supernode with children, the update rows that land there, in child order (the order in which the
+// assembly adds them), padded with -1 to TINY_PULLS entries.
+void splitTinySupernodes(oosg_plan& p)
+{
+    p.tinyList.clear();
+    p.otherList.clear();
+    p.pulls.clear();
+    p.pullBase.assign(p.S, -1);
+    p.tinyStart.assign(p.levels + 1, 0);
+    p.otherStart.assign(p.levels + 1, 0);
+    // Update rows are indexed by int32 in the table.
+    const bool indexable = p.rowStart[p.S] < std::numeric_limits<int32_t>::max();
+    std::vector<int32_t> hits;
+    std::vector<char> tiny(p.S, 0);
+    int32_t width = 0;
+    int64_t positions = 0;
+    for (int32_t s = 0; s < p.S; ++s)
+    {
+        if (!indexable || !oosg::isTinySupernode(p.ncol[s], p.nrow[s]))
+            continue;
+        const int32_t order = p.ncol[s] + p.nrow[s];
+        hits.assign(order, 0);
+        int32_t most = 0;
+        for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+        {
+            const int32_t ch = p.children[q];
+            for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+                most = std::max(most, ++hits[p.rel[e]]);
+        }
+        if (most > oosg::TINY_PULLS)
+            continue;
+        tiny[s] = 1;
+        width = std::max(width, most);
+        if (p.childStart[s + 1] > p.childStart[s])
+        {
+            p.pullBase[s] = static_cast<int32_t>(positions);
+            positions += order;
+        }
+    }
+    if (positions * oosg::TINY_PULLS >= std::numeric_limits<int32_t>::max())
+    {
+        // A table too large for int32 offsets: no supernode is swept as tiny.
+        std::fill(tiny.begin(), tiny.end(), 0);
+        std::fill(p.pullBase.begin(), p.pullBase.end(), -1);
+        width = 0;
+        positions = 0;
+    }
+    p.pullWidth = width;
+    p.pulls.assign(static_cast<size_t>(positions) * oosg::TINY_PULLS, -1);
+    for (int32_t s = 0; s < p.S; ++s)
+    {
+        if (p.pullBase[s] < 0)
+            continue;
+        const int32_t order = p.ncol[s] + p.nrow[s];
+        hits.assign(order, 0);
+        for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+        {
+            const int32_t ch = p.children[q];
+            for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+            {
+                const int32_t at = p.rel[e];
+                p.pulls[(static_cast<size_t>(p.pullBase[s]) + at) * oosg::TINY_PULLS + hits[at]++] =
+                    static_cast<int32_t>(e);
+            }
+        }
+    }
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
+        {
+            const int32_t s = p.levelList[q];
+            (tiny[s] ? p.tinyList : p.otherList).push_back(s);
+        }
+        p.tinyStart[l + 1] = static_cast<int32_t>(p.tinyList.size());
+        p.otherStart[l + 1] = static_cast<int32_t>(p.otherList.size());
+    }
+}
+
 // The update matrices' places in one arena, by liveness: U(s) is written at the level of s and read at its parent's
 // level, and its place is free again after that level (the role of the reference's stack of update matrices and the
 // bound of its workspace, MultifrontalDriver.cpp:257-284, FactorSchedule.cpp:156; here a whole level is live at
@@ -1066,27 +1147,7 @@ void deriveLevels(oosg_plan& p)
         p.inverseRounds.push_back(std::move(rounds));
     }
     placeUpdateMatrices(p, height);
-    p.tinyList.clear();
-    p.otherList.clear();
-    p.tinyStart.assign(p.levels + 1, 0);
-    p.otherStart.assign(p.levels + 1, 0);
-    p.tinyOrder.assign(p.levels, 0);
-    for (int32_t l = 0; l < p.levels; ++l)
-    {
-        for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
-        {
-            const int32_t s = p.levelList[q];
-            if (oosg::isTinySupernode(p.ncol[s], p.nrow[s]))
-            {
-                p.tinyList.push_back(s);
-                p.tinyOrder[l] = std::max(p.tinyOrder[l], p.ncol[s] + p.nrow[s]);
-            }
-            else
-                p.otherList.push_back(s);
-        }
-        p.tinyStart[l + 1] = static_cast<int32_t>(p.tinyList.size());
-        p.otherStart[l + 1] = static_cast<int32_t>(p.otherList.size());
-    }
+    splitTinySupernodes(p);
 }
 
 // The algorithmic flops of a tile: 2·k per element it updates (the lower part only on the diagonal).
@@ -1528,6 +1589,8 @@ void uploadSweepPlan(oosg_plan& p)
     p.dLevelList = upload(p.levelList, st, "plan: levels");
     p.dTinyList = upload(p.tinyList, st, "plan: tiny supernodes");
     p.dOtherList = upload(p.otherList, st, "plan: sweep supernodes");
+    p.dPullBase = upload(p.pullBase, st, "plan: pull table");
+    p.dPulls = upload(p.pulls, st, "plan: pull table");
     p.dp.first = p.dFirst.as<int32_t>();
     p.dp.ncol = p.dNcol.as<int32_t>();
     p.dp.nrow = p.dNrow.as<int32_t>();
@@ -1539,6 +1602,9 @@ void uploadSweepPlan(oosg_plan& p)
     p.dp.childStart = p.dChildStart.as<int32_t>();
     p.dp.children = p.dChildren.as<int32_t>();
     p.dp.linvOff = p.dLinvOff.as<int64_t>();
+    p.dp.pullBase = p.dPullBase.as<int32_t>();
+    p.dp.pulls = p.dPulls.as<int4>();
+    p.dp.pullWidth = p.pullWidth;
 }
 
 void uploadPlan(oosg_plan& p)
@@ -1561,7 +1627,7 @@ void uploadPlan(oosg_plan& p)
 uint64_t structureBytes(const oosg_plan& p)
 {
     return (p.first.size() * 3 + p.rows.size() * 2 + p.children.size() + p.smallList.size() + p.stepFronts.size() +
-            p.levelList.size()) *
+            p.levelList.size() * 2 + p.pullBase.size() + p.pulls.size()) *
                4 +
            p.panelOff.size() * 24 +
            (p.forwardTiles.size() + p.backwardBlocks.size() + p.forwardSubtractions.size() +
@@ -2385,7 +2451,7 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
         // blocks, whose sums replace the rows of X once every product has read X), then every L_RF row tile.
         timed(PROFILE_FORWARD, w.triangleValues * T + w.forwardRows * row, [&] {
             check(oosg::launchTinyForward<TP>(p.dp, panels, linv, X, U, k, tiny + p.tinyStart[l],
-                                              p.tinyStart[l + 1] - p.tinyStart[l], p.tinyOrder[l], st),
+                                              p.tinyStart[l + 1] - p.tinyStart[l], st),
                   "solve: forward sweep (tiny supernodes)");
             check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, others + p.otherStart[l],
                                           p.otherStart[l + 1] - p.otherStart[l], nullptr, 0, partials, st),
