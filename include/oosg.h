@@ -189,6 +189,35 @@ void oosg_matrix_destroy(oosg_matrix *matrix);
 oosg_status oosg_factorise(oosg_plan *plan, const oosg_matrix *matrix, const oosg_factor_options *options,
                            oosg_factor **out);
 void oosg_factor_destroy(oosg_factor *factor);
+
+/* ---- partial factorisation (PartialFactor, include/oos/linalg/supernodal/PartialFactor.hpp:47-75) ---------------
+ *
+ * The plan of a symbolic factor analysed with a trailing set (the boundary of a unit, ordered last: its last supernode,
+ * of trailing_dof_count columns and no rows below): oosg_factorise on it eliminates the interior only, in positive
+ * definite mode and FP64 (PartialFactor::factorise, PartialFactor.cpp:65-83: OOSG_INVALID_ARGUMENT otherwise, with the
+ * reference's message), and assembles the trailing front, the Schur complement
+ *     K* = K_bb − K_bi·K_ii⁻¹·K_ib
+ * (the matrix entries of its columns, then its children's update matrices in child order). The factor's inertia,
+ * ratio test and issues cover the interior equations. Its solves are the two calls below (the whole-matrix solves
+ * return OOSG_INVALID_ARGUMENT). trailing_dof_count = 0 gives a whole plan (oosg_plan_create). */
+oosg_status oosg_plan_create_partial(const oosg_symbolic_desc *symbolic, int32_t trailing_dof_count, int32_t device,
+                                     uint64_t hbm_budget_bytes, oosg_plan **out);
+/* K* (PartialFactor::schurComplement, PartialFactor.hpp:85-86): t × t, both triangles, into host `out` (leading
+ * dimension ld >= t), in the order of the trailing equations. */
+oosg_status oosg_partial_schur_complement(const oosg_factor *factor, double *out, int64_t ld);
+/* f* = f_b − K_bi·K_ii⁻¹·f_i for each column of the host `loads` (n rows, leading dimension ld, nrhs columns), into the
+ * host `condensed` (t × nrhs, leading dimension ldc) (PartialFactor::condenseLoads, PartialFactor.cpp:117-133). */
+oosg_status oosg_partial_condense_loads(const oosg_factor *factor, const double *loads, int64_t ld, int64_t nrhs,
+                                        double *condensed, int64_t ldc);
+/* Recovery (PartialFactor::recover, PartialFactor.cpp:135-158): on entry the interior rows of the host
+ * `loads_to_solution` (n × nrhs, leading dimension ld) hold f_i, and `trailing_values` (t × nrhs, leading dimension
+ * ldt) holds u_b; on return every row holds u: u_i = K_ii⁻¹·(f_i − K_ib·u_b), u_b copied. */
+oosg_status oosg_partial_recover(const oosg_factor *factor, double *loads_to_solution, int64_t ld, int64_t nrhs,
+                                 const double *trailing_values, int64_t ldt);
+/* The checksum of a matrix's structure and values (host), by which a recomputed interior factor is refused for another
+ * matrix (PartialFactor::isFactorisationOf and recomputeInteriorFactor, PartialFactor.cpp:40-60): 64-bit FNV-1a over
+ * the dimension, the n + 1 column offsets, the row indices (32 bits each) and the values' bits. */
+oosg_status oosg_csc_checksum(const oosg_csc_desc *matrix, uint64_t *checksum);
 oosg_status oosg_factor_info_get(const oosg_factor *factor, oosg_factor_info *info);
 /* One pivot that fails the ratio test (PivotIssue, include/oos/linalg/supernodal/PivotDiagnostics.hpp:54-62). */
 typedef struct oosg_pivot_issue {
@@ -361,7 +390,8 @@ int oosa_order_dissection(int32_t n, const int64_t *offsets, const int32_t *adja
 void *oosa_analyse(int32_t n, const int64_t *offsets, const int32_t *adjacency, const int32_t *weights,
                    const int32_t *order, int32_t trailing, int amalgamate);
 void oosa_free(void *symbolic);
-/* {nodes, DOFs, supernodes, row total, panel values, factor entries, exact entries}, and the flop count. */
+/* {nodes, DOFs, supernodes, row total, panel values, factor entries, exact entries, trailing DOFs}, and the flop
+ * count. */
 void oosa_sizes(void *symbolic, int64_t *sizes, double *flops);
 void oosa_export(void *symbolic, int32_t *node_order, int32_t *first_column, int32_t *column_count, int32_t *parent,
                  int64_t *row_start, int32_t *rows);

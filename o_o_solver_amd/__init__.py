@@ -25,7 +25,7 @@ __all__ = ["SymmetricCsc", "SymbolicFactor", "SupernodalFactor", "FactorSolve", 
            "Precision", "PivotPolicy", "Inertia", "PivotDiagnostics", "PivotIssue", "SolveReport", "Plan",
            "MemoryBudgetError", "MemoryShortfall", "SingularityError", "PrecisionInsufficientError",
            "OperationCancelledError", "InternalError", "HipError", "NotConvergedError", "nested_dissection",
-           "estimate_memory_bytes"]
+           "estimate_memory_bytes", "PartialFactor", "matrix_checksum"]
 
 
 @dataclass(frozen=True)
@@ -264,8 +264,10 @@ class SymbolicFactor:
     the reference's structure for the same graph, weights and order."""
 
     def __init__(self, n, first_column, column_count, parent, row_start, rows, node_order=None, flop_count=None,
-                 panel_value_count=None, factor_entry_count=None):
+                 panel_value_count=None, factor_entry_count=None, trailing_dof_count=0):
         self.n = int(n)
+        # The equations of the trailing set (the last supernode when > 0; SymbolicFactor::trailingDofCount).
+        self.trailing_dof_count = int(trailing_dof_count)
         self.first_column = np.ascontiguousarray(first_column, dtype=np.int32)
         self.column_count = np.ascontiguousarray(column_count, dtype=np.int32)
         self.parent = np.ascontiguousarray(parent, dtype=np.int32)
@@ -300,7 +302,7 @@ class SymbolicFactor:
         if not handle:
             raise ValueError(lib.oosa_last_error().decode())
         try:
-            sizes = np.zeros(7, np.int64)
+            sizes = np.zeros(8, np.int64)
             flops = ctypes.c_double()
             lib.oosa_sizes(handle, _ptr(sizes), ctypes.byref(flops))
             S = int(sizes[2])
@@ -311,7 +313,8 @@ class SymbolicFactor:
         finally:
             lib.oosa_free(handle)
         return cls(int(sizes[1]), fc, cc, pa, rs, rows, node_order=node_order, flop_count=flops.value,
-                   panel_value_count=int(sizes[4]), factor_entry_count=int(sizes[5]))
+                   panel_value_count=int(sizes[4]), factor_entry_count=int(sizes[5]),
+                   trailing_dof_count=int(sizes[7]))
 
     @property
     def supernode_count(self):
@@ -335,24 +338,27 @@ class SymbolicFactor:
         """The device bytes a plan of this structure reserves (SupernodalFactor::estimateMemoryBytes)."""
         return estimate_memory_bytes(self, pivoting)
 
-    def plan(self, device=0, hbm_budget_bytes=0):
-        """The device plan of this structure (kept: one per device and budget)."""
-        key = (device, hbm_budget_bytes)
+    def plan(self, device=0, hbm_budget_bytes=0, partial=False):
+        """The device plan of this structure (kept: one per device, budget and kind). partial: the plan of a
+        PartialFactor, whose trailing supernode is assembled (K*) and never eliminated."""
+        key = (device, hbm_budget_bytes, bool(partial))
         if key not in self._plans:
-            self._plans[key] = Plan(self, device, hbm_budget_bytes)
+            self._plans[key] = Plan(self, device, hbm_budget_bytes, partial)
         return self._plans[key]
 
 
 class Plan:
     """oosg_plan: levels, relative indices, the kernel schedule and the device memory of one symbolic factor."""
 
-    def __init__(self, symbolic, device=0, hbm_budget_bytes=0):
+    def __init__(self, symbolic, device=0, hbm_budget_bytes=0, partial=False):
         self._lib = _native.lib()
         self.symbolic = symbolic
         handle = ctypes.c_void_p()
         desc = symbolic.desc()
-        _check(self._lib.oosg_plan_create(ctypes.byref(desc), int(device), int(hbm_budget_bytes),
-                                          ctypes.byref(handle)))
+        trailing = symbolic.trailing_dof_count if partial else 0
+        _check(self._lib.oosg_plan_create_partial(ctypes.byref(desc), int(trailing), int(device),
+                                                  int(hbm_budget_bytes), ctypes.byref(handle)))
+        self.partial = bool(partial)
         self.handle = handle
         self._matrices = {}
 
@@ -747,6 +753,187 @@ class FactorSolve:
             raise NotConvergedError(f"{_native.lib().oosg_last_error().decode()} ({rep})")
         _check(status)
         return (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
+
+
+def matrix_checksum(matrix):
+    """The 64-bit FNV-1a checksum of a SymmetricCsc's structure and values (oosg_csc_checksum), by which a recomputed
+    interior factor is refused for another matrix (PartialFactor.cpp:40-60)."""
+    out = ctypes.c_uint64()
+    desc = matrix.desc()
+    _check(_native.lib().oosg_csc_checksum(ctypes.byref(desc), ctypes.byref(out)))
+    return int(out.value)
+
+
+class PartialFactor:
+    """PartialFactor (include/oos/linalg/supernodal/PartialFactor.hpp:47-150) on the GPU: the interior equations of a
+    unit eliminated, the trailing set (its boundary, ordered last by the symbolic factor: SymbolicFactor.analyse with
+    trailing_node_count) left as the exact Schur complement
+
+        K* = K_bb − K_bi·K_ii⁻¹·K_ib,
+
+    with the same elimination applied to loads (condense_loads: f* = f_b − K_bi·K_ii⁻¹·f_i) and the back-substitution
+    of the interior from given boundary values (recover: u_i = K_ii⁻¹·(f_i − K_ib·u_b)). The interior is factorised in
+    positive definite mode and FP64 on the partial plan of the symbolic factor (oosg_plan_create_partial), whose
+    trailing front is assembled into K* after the last level. The interior factor stays in device memory while kept;
+    release_interior_factor frees it and recompute_interior_factor computes it again, to the same bits (the
+    factorisation is deterministic)."""
+
+    @classmethod
+    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0):
+        """PartialFactor::factorise (PartialFactor.cpp:65-83). ValueError when options.pivoting is not positive
+        definite; MemoryBudgetError before any work; SingularityError for a mechanism of the interior."""
+        options = options or FactorOptions()
+        cls._require_positive_definite(options)
+        self = cls.__new__(cls)
+        self._dimension = symbolic.dof_count
+        self._trailing = symbolic.trailing_dof_count
+        self._device = device
+        self._budget = hbm_budget_bytes
+        self._interior_supernodes = symbolic.supernode_count - (1 if self._trailing > 0 else 0)
+        self._interior = self._factorise_interior(symbolic, matrix, options)
+        self._pivot_policy = PivotPolicy(options.pivot_policy.max_ratio, options.pivot_policy.bailout)
+        self._checksum = matrix_checksum(matrix)
+        t = self._trailing
+        self._schur = np.zeros((t, t), order="F")
+        if t > 0:
+            _check(_native.lib().oosg_partial_schur_complement(self._interior.handle, _ptr(self._schur), t))
+        return self
+
+    @staticmethod
+    def _require_positive_definite(options):
+        if options.pivoting != PivotingMode.POSITIVE_DEFINITE:
+            raise ValueError("PartialFactor: the interior of a unit is factorised in positive definite mode (a "
+                             "delayed pivot would move an interior equation into the boundary); pass "
+                             "PivotingMode::PositiveDefinite")
+
+    def _factorise_interior(self, symbolic, matrix, options):
+        plan = symbolic.plan(self._device, self._budget, partial=self._trailing > 0)
+        device_matrix = plan.matrix(matrix)
+        reference = None
+        if options.reference_diagonal is not None:
+            reference = np.ascontiguousarray(options.reference_diagonal, dtype=np.float64)
+        opts = SupernodalFactor._options_c(options, Precision.FP64, reference)
+        plan.set_cancellation(options.cancellation)
+        handle = ctypes.c_void_p()
+        _check(_native.lib().oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts),
+                                            ctypes.byref(handle)), options.equation_namer)
+        return SupernodalFactor(handle, plan, device_matrix, options)
+
+    @property
+    def dimension(self):
+        return self._dimension
+
+    @property
+    def trailing_dof_count(self):
+        return self._trailing
+
+    @property
+    def interior_dof_count(self):
+        return self._dimension - self._trailing
+
+    def schur_complement(self):
+        """K*, trailing_dof_count square, both triangles, in the order of the trailing equations."""
+        return self._schur
+
+    @property
+    def is_interior_factor_kept(self):
+        return self._interior is not None
+
+    @property
+    def pivot_policy(self):
+        return self._pivot_policy
+
+    def _require_kept(self, operation):
+        if self._interior is None:
+            raise RuntimeError(f"PartialFactor::{operation}: the interior factor was released; call "
+                               "recomputeInteriorFactor() first (the same bits as the first factorisation)")
+
+    def _columns(self, operation, array, expected, what):
+        a = np.asarray(array)
+        rows = a.shape[0] if a.ndim >= 1 else 0
+        if rows != expected:
+            raise ValueError(f"PartialFactor::{operation}: {what} have {rows} rows, expected {expected}")
+        return a.reshape(expected, -1) if a.ndim == 1 else a
+
+    @property
+    def interior_factor(self):
+        """The interior factor (its inertia, its ratio test)."""
+        self._require_kept("interiorFactor")
+        return self._interior
+
+    def condense_loads(self, loads):
+        """f* = f_b − K_bi·K_ii⁻¹·f_i for each column of `loads` (dimension rows, in the matrix's numbering):
+        trailing_dof_count × columns (PartialFactor::condenseLoads, PartialFactor.cpp:117-133)."""
+        self._require_kept("condenseLoads")
+        f = np.asfortranarray(self._columns("condenseLoads", loads, self._dimension, "the loads"), dtype=np.float64)
+        t, k = self._trailing, f.shape[1]
+        if t == 0:
+            return np.zeros((0, k), order="F")
+        out = np.zeros((t, k), order="F")
+        _check(_native.lib().oosg_partial_condense_loads(self._interior.handle, _ptr(f), self._dimension, k,
+                                                          _ptr(out), t))
+        return out
+
+    def recover(self, loads_to_solution, trailing_values):
+        """Recovery (PartialFactor::recover, PartialFactor.cpp:135-158): the interior rows of `loads_to_solution`
+        hold f_i (its trailing rows are ignored) and trailing_values holds u_b (trailing_dof_count × the same
+        columns); returns u in every row (in place when `loads_to_solution` is a Fortran-ordered float64 array)."""
+        self._require_kept("recover")
+        x = self._columns("recover", loads_to_solution, self._dimension, "the loads")
+        ub = self._columns("recover", trailing_values, self._trailing, "the trailing values")
+        if ub.shape[1] != x.shape[1]:
+            raise ValueError(f"PartialFactor::recover: {ub.shape[1]} columns of trailing values for {x.shape[1]} "
+                             "load columns; give one per load column")
+        inplace = isinstance(loads_to_solution, np.ndarray) and loads_to_solution.dtype == np.float64 and \
+            loads_to_solution.flags.f_contiguous and loads_to_solution.ndim == 2
+        x = x if inplace else np.array(x, dtype=np.float64, order="F", copy=True)
+        if self._trailing == 0:
+            x[:] = FactorSolve(self._interior).solve(x)
+            return x
+        ub = np.asfortranarray(ub, dtype=np.float64)
+        _check(_native.lib().oosg_partial_recover(self._interior.handle, _ptr(x), self._dimension, x.shape[1],
+                                                   _ptr(ub), self._trailing))
+        return x
+
+    def is_factorisation_of(self, symbolic, matrix, policy=None):
+        """Whether the interior factor is kept and is the factorisation of `matrix` on `symbolic` with the pivot policy
+        `policy` (default: this factor's): the shapes, the policy and the matrix checksum (PartialFactor.cpp:86-92)."""
+        policy = policy or self._pivot_policy
+        return (self._interior is not None and self._dimension == symbolic.dof_count == matrix.n and
+                self._trailing == symbolic.trailing_dof_count and
+                self._pivot_policy.max_ratio == policy.max_ratio and self._pivot_policy.bailout == policy.bailout and
+                self._checksum == matrix_checksum(matrix))
+
+    def release_interior_factor(self):
+        """Frees the interior factor (K* stays): recovery needs recompute_interior_factor first."""
+        self._interior = None
+
+    def recompute_interior_factor(self, symbolic, matrix, options=None):
+        """Computes the interior factor again from the same symbolic factor and matrix, with the same pivot policy:
+        the same bits as the first time. ValueError when they do not have the shape of the first factorisation, when
+        the matrix differs (its checksum) or the pivot policy differs (PartialFactor.cpp:111-150)."""
+        options = options or FactorOptions()
+        self._require_positive_definite(options)
+        if symbolic.dof_count != self._dimension or symbolic.trailing_dof_count != self._trailing:
+            raise ValueError(f"PartialFactor::recomputeInteriorFactor: the symbolic factor has {symbolic.dof_count} "
+                             f"equations and {symbolic.trailing_dof_count} trailing ones, the partial factor "
+                             f"{self._dimension} and {self._trailing}; pass the symbolic factor and the matrix of the "
+                             "first factorisation")
+        policy = options.pivot_policy
+        if policy.max_ratio != self._pivot_policy.max_ratio or policy.bailout != self._pivot_policy.bailout:
+            raise ValueError(f"PartialFactor::recomputeInteriorFactor: the pivot policy (ratio limit "
+                             f"{policy.max_ratio}, bail-out {policy.bailout}) differs from the first factorisation's "
+                             f"({self._pivot_policy.max_ratio}, {self._pivot_policy.bailout}); pass the options of "
+                             "the first factorisation")
+        if matrix_checksum(matrix) != self._checksum:
+            raise ValueError("PartialFactor::recomputeInteriorFactor: the matrix differs from the one first factorised "
+                             "(its checksum of structure and values changed); pass the same matrix, or factorise the "
+                             "unit again")
+        count = symbolic.supernode_count - (1 if self._trailing > 0 else 0)
+        if count != self._interior_supernodes:
+            raise ValueError("PartialFactor::recomputeInteriorFactor: the symbolic factor has another supernode "
+                             "structure than the first factorisation; pass the same symbolic factor")
+        self._interior = self._factorise_interior(symbolic, matrix, options)
 
 
 def estimate_memory_bytes(symbolic, pivoting=0):

@@ -65,6 +65,12 @@ _SIGNATURES = {
     "oosg_plan_set_cancellation": (c_int, [c_void_p, CANCEL_FN, c_void_p]),
     "oosg_factor_issues": (c_int, [c_void_p, c_void_p, c_int64, ctypes.POINTER(c_int64)]),
     "oosg_plan_create": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
+    "oosg_plan_create_partial": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, c_int32, ctypes.c_uint64,
+                                         ctypes.POINTER(c_void_p)]),
+    "oosg_partial_schur_complement": (c_int, [c_void_p, c_void_p, c_int64]),
+    "oosg_partial_condense_loads": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
+    "oosg_partial_recover": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
+    "oosg_csc_checksum": (c_int, [ctypes.POINTER(CscDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "oosg_plan_destroy": (None, [c_void_p]),
     "oosg_plan_stats": (c_int, [c_void_p, c_void_p]),
     "oosg_plan_synchronize": (c_int, [c_void_p]),

@@ -722,6 +722,7 @@ void oosa_sizes(void* handle, std::int64_t* sizes, double* flops)
     sizes[4] = s.panelValueCount;
     sizes[5] = s.factorEntryCount;
     sizes[6] = s.exactEntryCount;
+    sizes[7] = s.trailingDofCount;
     *flops = s.flopCount;
 }
 

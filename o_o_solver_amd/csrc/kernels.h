@@ -212,6 +212,9 @@ hipError_t launchBackward(const DevicePlan& p, const TP* panels, const TP* linv,
 hipError_t launchSubtractPartials(double* X, double* U, int k, const int4* items, int count, const double* partials,
                                   hipStream_t stream);
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream);
+// A partial factor's condensation: row-major X's trailing columns [first, first + c) += their children's update rows.
+hipError_t launchTrailingGather(double* X, const double* U, int k, int first, int c, const int32_t* pullStart,
+                                const int32_t* pulls, hipStream_t stream);
 
 hipError_t launchFillInt32(int32_t* data, int64_t count, int32_t value, hipStream_t stream);
 hipError_t launchToRowMajor(const double* src, int64_t ld, double* dst, int64_t n, int k, const double* scale,

@@ -2588,6 +2588,31 @@ hipError_t launchTriangleProducts(const DevicePlan& p, const TP* linv, const dou
     return hipGetLastError();
 }
 
+// The condensation of loads into the trailing columns of a partial factor (row-major X, n × k): X(first + j) += the
+// update rows that land on trailing column j, U(pulls[pullStart[j] .. pullStart[j + 1])), in child order.
+__global__ __launch_bounds__(256) void trailingGatherKernel(double* X, const double* U, int k, int first, int c,
+                                                            const int32_t* pullStart, const int32_t* pulls)
+{
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= int64_t(c) * k)
+        return;
+    const int j = int(idx / k), h = int(idx - int64_t(j) * k);
+    double y = X[int64_t(first + j) * k + h];
+    for (int e = pullStart[j]; e < pullStart[j + 1]; ++e)
+        y += U[int64_t(pulls[e]) * k + h];
+    X[int64_t(first + j) * k + h] = y;
+}
+
+hipError_t launchTrailingGather(double* X, const double* U, int k, int first, int c, const int32_t* pullStart,
+                                const int32_t* pulls, hipStream_t stream)
+{
+    const int64_t total = int64_t(c) * k;
+    if (total > 0)
+        hipLaunchKernelGGL(trailingGatherKernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, X, U, k,
+                           first, c, pullStart, pulls);
+    return hipGetLastError();
+}
+
 hipError_t launchDiagonalSolve(double* X, const double* d, int64_t n, int k, hipStream_t stream)
 {
     hipLaunchKernelGGL(diagonalSolveKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, X, d, n, k);

@@ -388,6 +388,14 @@ struct oosg_plan
     bool eager[2] = {false, false};
     hipEvent_t start = nullptr, stop = nullptr;
     int32_t n = 0, S = 0;
+    // A partial plan (PartialFactor): its last supernode, the trailing set (trailingDofs columns, no rows below), is
+    // assembled into K* and never eliminated nor swept (-1: a whole plan); eliminated: the equations factorised. The
+    // trailing front's assembly blocks, and per trailing column the children's update rows that land on it (child
+    // order) for the condensation of loads.
+    int32_t trailing = -1, trailingDofs = 0, eliminated = 0;
+    std::vector<oosg::AssemblyBlock> trailingBlocks;
+    std::vector<oosg::AssemblyItem> trailingItems;
+    std::vector<int32_t> trailingPullStart, trailingPulls;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
     // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others; the tiny ones' pull
     // table (oosg::DevicePlan::pulls) and its width.
@@ -442,6 +450,7 @@ struct oosg_plan
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
     DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls;
+    DeviceBuffer dTrailingBlocks, dTrailingItems, dTrailingPullStart, dTrailingPulls;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dForwardSubtractions, dBackwardSubtractions,
         dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
@@ -733,6 +742,18 @@ void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
     }
     if (expected != s.n)
         fail(OOSG_INVALID_ARGUMENT, "plan: the supernodes do not cover the n columns");
+    p.eliminated = p.n;
+    if (p.trailingDofs > 0)
+    {
+        const int32_t T = S - 1;
+        if (S < 1 || p.ncol[T] != p.trailingDofs || p.nrow[T] != 0)
+            fail(OOSG_INVALID_ARGUMENT,
+                 "plan: the trailing set of " + std::to_string(p.trailingDofs) +
+                     " equations is not the last supernode (with no rows below it); analyse the symbolic factor "
+                     "with the trailing nodes last");
+        p.trailing = T;
+        p.eliminated = p.n - p.trailingDofs;
+    }
     // Children in increasing order.
     p.childStart.assign(S + 1, 0);
     for (int32_t k = 0; k < S; ++k)
@@ -794,7 +815,7 @@ void splitTinySupernodes(oosg_plan& p)
     int64_t positions = 0;
     for (int32_t s = 0; s < p.S; ++s)
     {
-        if (!indexable || !oosg::isTinySupernode(p.ncol[s], p.nrow[s]))
+        if (!indexable || s == p.trailing || !oosg::isTinySupernode(p.ncol[s], p.nrow[s]))
             continue;
         const int32_t order = p.ncol[s] + p.nrow[s];
         hits.assign(order, 0);
@@ -899,7 +920,9 @@ void placeUpdateMatrices(oosg_plan& p, const std::vector<int32_t>& height)
                 p.updateOff[s] = top;
                 top += size;
             }
-            if (p.parent[s] >= 0)
+            // Released once the parent's level has assembled it; the trailing front is assembled after every level
+            // (its children's matrices stay to the end).
+            if (p.parent[s] >= 0 && p.parent[s] != p.trailing)
                 releaseAfter[height[p.parent[s]]].push_back(s);
         }
         for (const int32_t s : releaseAfter[l])
@@ -946,30 +969,40 @@ void deriveLevels(oosg_plan& p)
     for (int32_t k = 0; k < S; ++k)
     {
         const int64_t c = p.ncol[k], r = p.nrow[k];
-        p.panelOff[k + 1] = p.panelOff[k] + c * (c + r);
-        // The diagonal blocks' inverses, then (wide supernodes) the whole inverse of the triangle.
-        const int64_t inverses = oosg::blockInverseValues(c) + (oosg::keepsTriangleInverse(static_cast<int>(c)) ? c * c : 0);
+        p.panelOff[k + 1] = p.panelOff[k] + c * (c + r); // the trailing supernode's panel holds K*
+        // The diagonal blocks' inverses, then (wide supernodes) the whole inverse of the triangle; none for the
+        // trailing supernode (not eliminated).
+        const int64_t inverses =
+            k == p.trailing
+                ? 0
+                : oosg::blockInverseValues(c) + (oosg::keepsTriangleInverse(static_cast<int>(c)) ? c * c : 0);
         p.linvOff[k + 1] = p.linvOff[k] + inverses;
-        for (int64_t m = r; m < r + c; ++m)
-            p.flops += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
+        if (k != p.trailing)
+            for (int64_t m = r; m < r + c; ++m)
+                p.flops += static_cast<double>(m) * static_cast<double>(m + 1) + 2.0 * static_cast<double>(m);
         if (p.parent[k] >= 0)
             height[p.parent[k]] = std::max(height[p.parent[k]], height[k] + 1);
     }
+    // The levels hold the supernodes eliminated (the trailing one is in none).
     p.levels = 0;
     for (int32_t k = 0; k < S; ++k)
-        p.levels = std::max(p.levels, height[k] + 1);
+        if (k != p.trailing)
+            p.levels = std::max(p.levels, height[k] + 1);
     p.levelStart.assign(p.levels + 1, 0);
     for (int32_t k = 0; k < S; ++k)
-        ++p.levelStart[height[k] + 1];
+        if (k != p.trailing)
+            ++p.levelStart[height[k] + 1];
     for (int32_t l = 0; l < p.levels; ++l)
         p.levelStart[l + 1] += p.levelStart[l];
-    p.levelList.resize(S);
+    p.levelList.resize(p.levelStart[p.levels]);
     p.levelPanelValues.assign(p.levels, 0.0);
     p.sweepWork.assign(p.levels, oosg_plan::SweepWork{});
     {
         std::vector<int32_t> fill(p.levels, 0);
         for (int32_t k = 0; k < S; ++k)
         {
+            if (k == p.trailing)
+                continue;
             p.levelList[p.levelStart[height[k]] + fill[height[k]]++] = k;
             p.levelPanelValues[height[k]] += static_cast<double>(p.panelOff[k + 1] - p.panelOff[k]);
             oosg_plan::SweepWork& w = p.sweepWork[height[k]];
@@ -1194,6 +1227,86 @@ Launch inverseLaunch(const oosg_plan::InverseRound& round, int wait)
     return inverse;
 }
 
+// The assembly blocks of the panel part (columns < c) of the front of s, each with the parts of its children's update
+// matrices that land in it (items appended to `items`, indexed from its start); the update matrices take their
+// children's parts in the update-matrix tiles (extendTilesKernel).
+void appendPanelBlocks(const oosg_plan& p, int32_t s, std::vector<oosg::AssemblyBlock>& blocks,
+                       std::vector<oosg::AssemblyItem>& items)
+{
+    const auto addBlock = [&](int j0, int j1, int i0, int i1)
+    {
+        oosg::AssemblyBlock blk{s, j0, j1, i0, i1, static_cast<int32_t>(items.size()), 0, 0};
+        for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+        {
+            const int32_t ch = p.children[q];
+            const int32_t* rel = p.rel.data() + p.rowStart[ch];
+            const int32_t rc = p.nrow[ch];
+            const auto at = [&](int value)
+            { return static_cast<int32_t>(std::lower_bound(rel, rel + rc, value) - rel); };
+            const oosg::AssemblyItem it{ch, at(j0), at(j1), at(i0), at(i1), 0};
+            if (it.q0 < it.q1 && std::max(it.q0, it.ia) < it.ib)
+                items.push_back(it);
+        }
+        blk.itemEnd = static_cast<int32_t>(items.size());
+        blocks.push_back(blk);
+    };
+    const int c = p.ncol[s], n = c + p.nrow[s];
+    int j0 = 0;
+    while (j0 < c)
+    {
+        // Rows from the block's first column down: the panel's values above the diagonal are never read.
+        const int i0 = j0;
+        const int height = n - i0;
+        if (height <= oosg::ASSEMBLY_BLOCK_VALUES)
+        {
+            int j1 = std::min(n, j0 + std::max(1, oosg::ASSEMBLY_BLOCK_VALUES / height));
+            if (j0 < c && j1 > c)
+                j1 = c;
+            addBlock(j0, j1, i0, n);
+            j0 = j1;
+        }
+        else
+        {
+            for (int i = i0; i < n; i += oosg::ASSEMBLY_BLOCK_VALUES)
+                addBlock(j0, j0 + 1, i, std::min(n, i + oosg::ASSEMBLY_BLOCK_VALUES));
+            j0 += 1;
+        }
+    }
+}
+
+// A partial plan's trailing front: its assembly blocks (K* = the matrix entries of its columns + its children's
+// update matrices, in child order, as a large front's panel is assembled), and per trailing column the children's
+// update rows that land on it, in child order (the condensation of loads adds them so).
+void buildTrailing(oosg_plan& p)
+{
+    p.trailingBlocks.clear();
+    p.trailingItems.clear();
+    p.trailingPullStart.clear();
+    p.trailingPulls.clear();
+    if (p.trailing < 0)
+        return;
+    const int32_t T = p.trailing;
+    appendPanelBlocks(p, T, p.trailingBlocks, p.trailingItems);
+    const int32_t c = p.ncol[T];
+    std::vector<int32_t> count(static_cast<size_t>(c) + 1, 0);
+    for (int32_t q = p.childStart[T]; q < p.childStart[T + 1]; ++q)
+    {
+        const int32_t ch = p.children[q];
+        for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+            ++count[p.rel[e] + 1];
+    }
+    for (int32_t j = 0; j < c; ++j)
+        count[j + 1] += count[j];
+    p.trailingPullStart = count;
+    p.trailingPulls.assign(static_cast<size_t>(count[c]), 0);
+    for (int32_t q = p.childStart[T]; q < p.childStart[T + 1]; ++q)
+    {
+        const int32_t ch = p.children[q];
+        for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+            p.trailingPulls[count[p.rel[e]]++] = static_cast<int32_t>(e);
+    }
+}
+
 void buildSchedule(oosg_plan& p)
 {
     using oosg::PANEL_BLOCK;
@@ -1269,50 +1382,10 @@ void buildSchedule(oosg_plan& p)
         // Assembly blocks, each with the parts of the children's update matrices that land in it.
         const int64_t blockStart = static_cast<int64_t>(p.blocks.size());
         int maxC = 0;
-        const auto addBlock = [&p](int32_t s, int j0, int j1, int i0, int i1)
-        {
-            oosg::AssemblyBlock blk{s, j0, j1, i0, i1, static_cast<int32_t>(p.items.size()), 0, 0};
-            for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
-            {
-                const int32_t ch = p.children[q];
-                const int32_t* rel = p.rel.data() + p.rowStart[ch];
-                const int32_t rc = p.nrow[ch];
-                const auto at = [&](int value)
-                { return static_cast<int32_t>(std::lower_bound(rel, rel + rc, value) - rel); };
-                const oosg::AssemblyItem it{ch, at(j0), at(j1), at(i0), at(i1), 0};
-                if (it.q0 < it.q1 && std::max(it.q0, it.ia) < it.ib)
-                    p.items.push_back(it);
-            }
-            blk.itemEnd = static_cast<int32_t>(p.items.size());
-            p.blocks.push_back(blk);
-        };
-        // The panel part only (columns < c): the update matrices take their children's parts in the update-matrix
-        // tiles (extendTilesKernel).
         for (const int32_t s : large)
         {
-            const int c = p.ncol[s], n = c + p.nrow[s];
-            maxC = std::max(maxC, c);
-            int j0 = 0;
-            while (j0 < c)
-            {
-                // Rows from the block's first column down: the panel's values above the diagonal are never read.
-                const int i0 = j0;
-                const int height = n - i0;
-                if (height <= oosg::ASSEMBLY_BLOCK_VALUES)
-                {
-                    int j1 = std::min(n, j0 + std::max(1, oosg::ASSEMBLY_BLOCK_VALUES / height));
-                    if (j0 < c && j1 > c)
-                        j1 = c;
-                    addBlock(s, j0, j1, i0, n);
-                    j0 = j1;
-                }
-                else
-                {
-                    for (int i = i0; i < n; i += oosg::ASSEMBLY_BLOCK_VALUES)
-                        addBlock(s, j0, j0 + 1, i, std::min(n, i + oosg::ASSEMBLY_BLOCK_VALUES));
-                    j0 += 1;
-                }
-            }
+            appendPanelBlocks(p, s, p.blocks, p.items);
+            maxC = std::max(maxC, p.ncol[s]);
         }
         Launch assemble{LaunchKind::Assemble, PROFILE_ASSEMBLE};
         assemble.offset = blockStart;
@@ -1619,6 +1692,10 @@ void uploadPlan(oosg_plan& p)
     p.dBigTiles = upload(p.bigTiles, st, "plan: big update tiles");
     p.dItemStarts = upload(p.itemStarts, st, "plan: update tile items");
     p.dExtendItems = upload(p.extendItems, st, "plan: update tile items");
+    p.dTrailingBlocks = upload(p.trailingBlocks, st, "plan: trailing front");
+    p.dTrailingItems = upload(p.trailingItems, st, "plan: trailing front");
+    p.dTrailingPullStart = upload(p.trailingPullStart, st, "plan: trailing front");
+    p.dTrailingPulls = upload(p.trailingPulls, st, "plan: trailing front");
     allocateCounted(p, p.updates, static_cast<size_t>(p.updateOff[p.S]) * sizeof(double), "plan: update matrices");
     check(hipStreamSynchronize(st), "plan: upload");
 }
@@ -1637,7 +1714,10 @@ uint64_t structureBytes(const oosg_plan& p)
            p.inverseItems.size() * 8 + p.blocks.size() * sizeof(oosg::AssemblyBlock) +
            p.items.size() * sizeof(oosg::AssemblyItem) +
            (p.tiles.size() + p.bigTiles.size()) * sizeof(oosg::UpdateTile) + p.itemStarts.size() * 4 +
-           p.extendItems.size() * sizeof(oosg::AssemblyItem);
+           p.extendItems.size() * sizeof(oosg::AssemblyItem) +
+           p.trailingBlocks.size() * sizeof(oosg::AssemblyBlock) +
+           p.trailingItems.size() * sizeof(oosg::AssemblyItem) +
+           (p.trailingPullStart.size() + p.trailingPulls.size()) * 4;
 }
 
 uint64_t factorStorageBytes(const oosg_plan& p)
@@ -1921,8 +2001,8 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
     else
     {
         // No ratio is a number (or no equation).
-        info.largest_ratio = p.n > 0 ? NAN : 0.0;
-        info.largest_ratio_equation = p.n > 0 ? 0 : -1;
+        info.largest_ratio = p.eliminated > 0 ? NAN : 0.0;
+        info.largest_ratio_equation = p.eliminated > 0 ? 0 : -1;
     }
     if (info.issue_count == 0)
         return;
@@ -1932,7 +2012,7 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
     const std::vector<double>& d = hostDiagonal(f);
     const double* reference = options.reference_diagonal;
     std::vector<int32_t> equations;
-    for (int32_t j = 0; j < p.n && f.issues.size() < OOSG_RECORDED_ISSUE_COUNT; ++j)
+    for (int32_t j = 0; j < p.eliminated && f.issues.size() < OOSG_RECORDED_ISSUE_COUNT; ++j)
     {
         const double diagonal = reference != nullptr && reference[j] >= 0.0 ? reference[j] : m.diagonal[j];
         const double ratio = diagonal / d[j];
@@ -2483,11 +2563,9 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
     }
 }
 
-// p.solveX ← factor⁻¹·p.solveX: the forward sweep from level `fromLevel` (the levels below it already swept), D⁻¹,
-// the backward sweep.
-template <typename TP>
-void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k, const double* sub = nullptr,
-               int32_t fromLevel = 0)
+// p.solveX ← L⁻ᵀ·p.solveX: the backward sweep, levels from the top (rows outside the levels, a partial factor's
+// trailing ones, are read and kept).
+template <typename TP> void backwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k)
 {
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
@@ -2507,11 +2585,6 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
         launch();
         p.profiler.end(slot, st);
     };
-    forwardSweep<TP>(p, panels, linv, k, fromLevel, p.levels, st);
-    if (sub != nullptr)
-        check(oosg::launchBlockDiagonalSolve(X, diag, sub, p.n, k, st), "solve: block diagonal");
-    else
-        check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
     for (int32_t l = p.levels - 1; l >= 0; --l)
     {
         const oosg_plan::SweepWork& w = p.sweepWork[l];
@@ -2551,12 +2624,31 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     }
 }
 
+// p.solveX ← factor⁻¹·p.solveX: the forward sweep from level `fromLevel` (the levels below it already swept), D⁻¹,
+// the backward sweep.
+template <typename TP>
+void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* diag, int k, const double* sub = nullptr,
+               int32_t fromLevel = 0)
+{
+    const hipStream_t st = p.stream;
+    double* X = p.solveX.as<double>();
+    forwardSweep<TP>(p, panels, linv, k, fromLevel, p.levels, st);
+    if (sub != nullptr)
+        check(oosg::launchBlockDiagonalSolve(X, diag, sub, p.n, k, st), "solve: block diagonal");
+    else
+        check(oosg::launchDiagonalSolve(X, diag, p.n, k, st), "solve: diagonal");
+    backwardSweep<TP>(p, panels, linv, k);
+}
+
 // p.solveX ← factor⁻¹·p.solveX (row-major n × k), as a graph replay unless the profiler is on; with fromLevel > 0
 // the forward sweep of the levels below it is done already (a combined factorisation and solve), and the rest runs
 // launch by launch.
 void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
 {
     oosg_plan& p = *f.plan;
+    if (p.trailing >= 0)
+        fail(OOSG_INVALID_ARGUMENT, "solve: a partial factor solves by oosg_partial_condense_loads and "
+                                    "oosg_partial_recover (its trailing equations are not eliminated)");
     if (fromLevel > 0)
     {
         if (f.precision == OOSG_FP32)
@@ -3156,6 +3248,40 @@ oosg_status oosg_last_error_shortfall(oosg_memory_shortfall* shortfall)
     return OOSG_OK;
 }
 
+oosg_status oosg_csc_checksum(const oosg_csc_desc* matrix, uint64_t* checksum)
+{
+    return guarded(
+        [&]
+        {
+            if (matrix == nullptr || checksum == nullptr || matrix->n < 0 ||
+                (matrix->n > 0 && (matrix->offsets == nullptr || matrix->rows == nullptr || matrix->values == nullptr)))
+                fail(OOSG_INVALID_ARGUMENT, "checksum: null argument");
+            // FNV-1a (Fowler, Noll, Vo), 64 bits, one word at a time.
+            constexpr uint64_t OFFSET_BASIS = 14695981039346656037ULL, PRIME = 1099511628211ULL;
+            uint64_t hash = OFFSET_BASIS;
+            const auto mix = [&hash](uint64_t word)
+            {
+                hash ^= word;
+                hash *= PRIME;
+            };
+            const int64_t n = matrix->n, nnz = n > 0 ? matrix->offsets[n] : 0;
+            mix(static_cast<uint64_t>(n));
+            for (int64_t j = 0; j <= n && n > 0; ++j)
+                mix(static_cast<uint64_t>(matrix->offsets[j]));
+            if (n == 0)
+                mix(0); // the one offset of an empty matrix
+            for (int64_t e = 0; e < nnz; ++e)
+                mix(static_cast<uint32_t>(matrix->rows[e]));
+            for (int64_t e = 0; e < nnz; ++e)
+            {
+                uint64_t bits = 0;
+                std::memcpy(&bits, &matrix->values[e], sizeof(bits));
+                mix(bits);
+            }
+            *checksum = hash;
+        });
+}
+
 oosg_status oosg_estimate_memory_bytes(const oosg_symbolic_desc* symbolic, int32_t pivoting, uint64_t* bytes)
 {
     return guarded(
@@ -3213,21 +3339,26 @@ oosg_status oosg_plan_set_cancellation(oosg_plan* plan, oosg_cancel_fn cancelled
 oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device, uint64_t hbm_budget_bytes,
                              oosg_plan** out)
 {
+    return oosg_plan_create_partial(symbolic, 0, device, hbm_budget_bytes, out);
+}
+
+oosg_status oosg_plan_create_partial(const oosg_symbolic_desc* symbolic, int32_t trailing_dof_count, int32_t device,
+                                     uint64_t hbm_budget_bytes, oosg_plan** out)
+{
     return guarded(
         [&]
         {
             if (symbolic == nullptr || out == nullptr)
                 fail(OOSG_INVALID_ARGUMENT, "plan: null argument");
             *out = nullptr;
+            if (trailing_dof_count < 0)
+                fail(OOSG_INVALID_ARGUMENT, "plan: a negative trailing count");
             auto p = std::make_unique<oosg_plan>();
             p->device = device;
-            int count = 0;
-            check(hipGetDeviceCount(&count), "hipGetDeviceCount");
-            if (device < 0 || device >= count)
-                fail(OOSG_INVALID_ARGUMENT, "plan: no device " + std::to_string(device));
-            setDevice(p.get());
+            p->trailingDofs = trailing_dof_count;            setDevice(p.get());
             buildStructure(*p, *symbolic);
             buildSchedule(*p);
+            buildTrailing(*p);
             p->reserved = structureBytes(*p); // uploaded uncounted below, held while the plan lives
             // The memory of a factorisation and of its solves, reserved before any numeric work (MemoryBudgetError).
             size_t freeBytes = 0, totalBytes = 0;
@@ -4187,6 +4318,14 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 opts = *options;
             if (opts.pivoting != OOSG_POSITIVE_DEFINITE && opts.pivoting != OOSG_INDEFINITE)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: unknown pivoting mode");
+            if (plan->trailing >= 0 && opts.pivoting != OOSG_POSITIVE_DEFINITE)
+                // requirePositiveDefinite (src/linalg/supernodal/PartialFactor.cpp:24-30)
+                fail(OOSG_INVALID_ARGUMENT,
+                     "PartialFactor: the interior of a unit is factorised in positive definite mode (a delayed pivot "
+                     "would move an interior equation into the boundary); pass PivotingMode::PositiveDefinite");
+            if (plan->trailing >= 0 && opts.precision != OOSG_FP64)
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor: the interior factor and K* are computed in FP64; pass "
+                                            "Precision.FP64");
             if (opts.pivoting == OOSG_INDEFINITE && opts.precision != OOSG_FP64)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: indefinite factors are FP64 (their inertia decides Sturm "
                                             "counts)");
@@ -4248,6 +4387,19 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                     [&] { body(false); });
             else
                 body(true);
+            if (plan->trailing >= 0)
+                // K*: the trailing front assembled once every level is done (MultifrontalDriver::assembleTrailing,
+                // MultifrontalDriver.cpp:567-599), into the trailing supernode's panel (its lower triangle).
+                check(oosg::launchAssembly<double>(
+                          plan->dp,
+                          oosg::DeviceMatrix{matrix->dOffsets.as<int64_t>(), matrix->dRows.as<int32_t>(),
+                                             matrix->dValues.as<double>(), matrix->dPosition.as<int32_t>(),
+                                             matrix->dColumn.as<int32_t>(), nullptr},
+                          f->panels.as<double>(), plan->updates.as<double>(),
+                          plan->dTrailingBlocks.as<oosg::AssemblyBlock>(),
+                          plan->dTrailingItems.as<oosg::AssemblyItem>(),
+                          static_cast<int>(plan->trailingBlocks.size()), st),
+                      "factor: the trailing front (K*)");
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
             if (plan->aheadLevel > 0)
                 check(hipStreamWaitEvent(st, plan->aheadEvents[LANES], 0), "factor: sweep ahead"); // its join
@@ -4267,7 +4419,8 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             check(oosg::launchPivotStatistics(diag.as<double>(),
                                               opts.precision == OOSG_FP32 ? f->scale.as<double>() : nullptr,
                                               matrix->dDiagonal.as<double>(),
-                                              reference.ptr != nullptr ? reference.as<double>() : nullptr, plan->n,
+                                              reference.ptr != nullptr ? reference.as<double>() : nullptr,
+                                              plan->eliminated,
                                               opts.max_ratio > 0 ? opts.max_ratio : 1.0e7,
                                               plan->pivotStats.as<oosg::PivotStatistics>(), st),
                   "factor: pivot statistics");
@@ -4306,6 +4459,152 @@ void oosg_factor_destroy(oosg_factor* f)
     delete f; // its buffers go back to the plan's caches
     releasePlan(plan);
     (void)hipGetLastError();
+}
+
+} // extern "C"
+
+namespace
+{
+
+// A partial factor's tiles of 16 loads through the plan's vectors: each column block of `loads` (host, n rows) is
+// uploaded, laid out row-major in p.solveX and swept forward over the interior; `after` then works on p.solveX.
+template <typename After>
+void partialTiles(const oosg_factor& f, const double* loads, int64_t ld, int64_t nrhs, After&& after)
+{
+    oosg_plan& p = *f.plan;
+    setDevice(&p);
+    ensureSolveBuffers(p);
+    double* staging = p.cgB.as<double>();
+    for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
+    {
+        if (c0 > 0)
+            pollCancellation(p);
+        const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
+        check(hipMemcpy2DAsync(staging, p.n * sizeof(double), loads + c0 * ld, ld * sizeof(double),
+                               p.n * sizeof(double), k, hipMemcpyHostToDevice, p.stream),
+              "partial factor: upload");
+        check(oosg::launchToRowMajor(staging, p.n, p.solveX.as<double>(), p.n, k, nullptr, p.stream),
+              "partial factor: layout");
+        forwardSweep<double>(p, f.panels.as<double>(), f.linv.as<double>(), k, 0, p.levels, p.stream);
+        after(c0, k, staging);
+        check(hipStreamSynchronize(p.stream), "partial factor");
+    }
+    p.profiler.collect();
+}
+
+const oosg_plan& partialPlan(const oosg_factor* f, const char* operation)
+{
+    if (f == nullptr)
+        fail(OOSG_INVALID_ARGUMENT, std::string("PartialFactor::") + operation + ": null factor");
+    if (f->plan->trailing < 0)
+        fail(OOSG_INVALID_ARGUMENT, std::string("PartialFactor::") + operation +
+                                        ": the factor is not a partial one (make its plan with "
+                                        "oosg_plan_create_partial)");
+    return *f->plan;
+}
+
+} // namespace
+
+extern "C" {
+
+oosg_status oosg_partial_schur_complement(const oosg_factor* f, double* out, int64_t ld)
+{
+    return guarded(
+        [&]
+        {
+            const oosg_plan& p = partialPlan(f, "schurComplement");
+            const int64_t t = p.trailingDofs;
+            if (out == nullptr || ld < std::max<int64_t>(t, 1))
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor::schurComplement: null output or leading dimension below "
+                                            "the trailing count");
+            setDevice(&p);
+            const double* kstar = f->panels.as<double>() + p.panelOff[p.trailing];
+            check(hipMemcpy2DAsync(out, ld * sizeof(double), kstar, t * sizeof(double), t * sizeof(double), t,
+                                   hipMemcpyDeviceToHost, p.stream),
+                  "PartialFactor::schurComplement");
+            check(hipStreamSynchronize(p.stream), "PartialFactor::schurComplement");
+            // The lower triangle was assembled: both triangles handed over (MultifrontalDriver.cpp:593-596).
+            for (int64_t j = 0; j < t; ++j)
+                for (int64_t i = j + 1; i < t; ++i)
+                    out[j + i * ld] = out[i + j * ld];
+        });
+}
+
+oosg_status oosg_partial_condense_loads(const oosg_factor* f, const double* loads, int64_t ld, int64_t nrhs,
+                                        double* condensed, int64_t ldc)
+{
+    return guarded(
+        [&]
+        {
+            const oosg_plan& p = partialPlan(f, "condenseLoads");
+            const int64_t t = p.trailingDofs;
+            if ((loads == nullptr || condensed == nullptr) && nrhs > 0)
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor::condenseLoads: null argument");
+            if (ld < std::max<int64_t>(p.n, 1) || ldc < std::max<int64_t>(t, 1) || nrhs < 0)
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor::condenseLoads: a leading dimension below the row count");
+            if (nrhs == 0)
+                return;
+            oosg_plan& plan = *f->plan;
+            const int32_t T = plan.trailing;
+            partialTiles(*f, loads, ld, nrhs,
+                         [&](int64_t c0, int k, double* staging)
+                         {
+                             double* X = plan.solveX.as<double>();
+                             check(oosg::launchTrailingGather(X, plan.solveU.as<double>(), k, plan.first[T],
+                                                              plan.ncol[T], plan.dTrailingPullStart.as<int32_t>(),
+                                                              plan.dTrailingPulls.as<int32_t>(), plan.stream),
+                                   "PartialFactor::condenseLoads");
+                             check(oosg::launchFromRowMajor(X, staging, plan.n, plan.n, k, nullptr, 0, plan.stream),
+                                   "PartialFactor::condenseLoads");
+                             check(hipMemcpy2DAsync(condensed + c0 * ldc, ldc * sizeof(double),
+                                                    staging + plan.eliminated, plan.n * sizeof(double),
+                                                    t * sizeof(double), k, hipMemcpyDeviceToHost, plan.stream),
+                                   "PartialFactor::condenseLoads: download");
+                         });
+        });
+}
+
+oosg_status oosg_partial_recover(const oosg_factor* f, double* loads_to_solution, int64_t ld, int64_t nrhs,
+                                 const double* trailing_values, int64_t ldt)
+{
+    return guarded(
+        [&]
+        {
+            const oosg_plan& p = partialPlan(f, "recover");
+            const int64_t t = p.trailingDofs;
+            if ((loads_to_solution == nullptr || (trailing_values == nullptr && t > 0)) && nrhs > 0)
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor::recover: null argument");
+            if (ld < std::max<int64_t>(p.n, 1) || ldt < std::max<int64_t>(t, 1) || nrhs < 0)
+                fail(OOSG_INVALID_ARGUMENT, "PartialFactor::recover: a leading dimension below the row count");
+            if (nrhs == 0)
+                return;
+            oosg_plan& plan = *f->plan;
+            partialTiles(*f, loads_to_solution, ld, nrhs,
+                         [&](int64_t c0, int k, double* staging)
+                         {
+                             double* X = plan.solveX.as<double>();
+                             // The trailing rows (f* after the forward sweep) replaced by u_b, then D⁻¹ and the
+                             // backward sweep over the interior.
+                             check(hipMemcpy2DAsync(staging + plan.eliminated, plan.n * sizeof(double),
+                                                    trailing_values + c0 * ldt, ldt * sizeof(double),
+                                                    t * sizeof(double), k, hipMemcpyHostToDevice, plan.stream),
+                                   "PartialFactor::recover: upload");
+                             check(oosg::launchToRowMajor(staging + plan.eliminated, plan.n,
+                                                          X + static_cast<int64_t>(plan.eliminated) * k, t, k,
+                                                          nullptr, plan.stream),
+                                   "PartialFactor::recover: layout");
+                             check(oosg::launchDiagonalSolve(X, f->dDiag.as<double>(), plan.eliminated, k,
+                                                             plan.stream),
+                                   "PartialFactor::recover: diagonal");
+                             backwardSweep<double>(plan, f->panels.as<double>(), f->linv.as<double>(), k);
+                             check(oosg::launchFromRowMajor(X, staging, plan.n, plan.n, k, nullptr, 0, plan.stream),
+                                   "PartialFactor::recover: layout");
+                             check(hipMemcpy2DAsync(loads_to_solution + c0 * ld, ld * sizeof(double), staging,
+                                                    plan.n * sizeof(double), plan.n * sizeof(double), k,
+                                                    hipMemcpyDeviceToHost, plan.stream),
+                                   "PartialFactor::recover: download");
+                         });
+        });
 }
 
 oosg_status oosg_factor_issues(const oosg_factor* f, oosg_pivot_issue* issues, int64_t capacity, int64_t* recorded)
@@ -4685,6 +4984,9 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
             *factor = nullptr;
             if (matrix->plan != plan)
                 fail(OOSG_INVALID_ARGUMENT, "factorise and solve: the matrix was made for another plan");
+            if (plan->trailing >= 0)
+                fail(OOSG_INVALID_ARGUMENT, "factorise and solve: a partial plan's factor solves by "
+                                            "oosg_partial_condense_loads and oosg_partial_recover");
             if (ld < std::max<int64_t>(plan->n, 1) || nrhs < 1)
                 fail(OOSG_INVALID_ARGUMENT,
                      "factorise and solve: at least one right-hand side, leading dimension >= n");

@@ -17,6 +17,10 @@
  *                                         Schur complements on integer matrices.
  *   orc_solve                             residual of manufactured solutions; within 1e-12 of the reference's solve.
  *   orc_spmv                              bit-equal to the reference's SymmetricCsc::multiply; dense product.
+ *   orc_multifrontal_partial              K* and the interior panels bit-equal to the reference's PartialFactor;
+ *                                         K* equal to the dense K_bb − K_bi·K_ii⁻¹·K_ib.
+ *   orc_partial_condense, _recover        within 1e-12 of the reference's condenseLoads and recover; the dense
+ *                                         f_b − K_bi·K_ii⁻¹·f_i and K_ii⁻¹·(f_i − K_ib·u_b).
  */
 
 #include "oracle.h"
@@ -301,8 +305,9 @@ static int64_t front_position(const orc_symbolic *sym, int32_t s, int32_t row)
  * out->status: 0 done; 1 a matrix entry outside the structure (std::invalid_argument); 3 a zero or non-finite pivot
  * (SingularityError), out->zero_pivot_count zero pivots, the first at out->first_zero_pivot.
  * diag (n values): d of each equation. */
-void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows, const double *values,
-                         double max_ratio, double *panels, double *diag, orc_result *out)
+static void multifrontal(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows, const double *values,
+                         double max_ratio, double *panels, double *diag, orc_result *out, int32_t trailing,
+                         double *schur)
 {
     const int32_t S = sym->supernode_count;
     memset(out, 0, sizeof(*out));
@@ -389,6 +394,17 @@ void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const 
             free(updates[child]);
             updates[child] = NULL;
         }
+        if (s == trailing) {
+            /* The trailing front is K* itself (MultifrontalDriver::assembleTrailing, MultifrontalDriver.cpp:567-599):
+             * handed over with both triangles, never eliminated. */
+            for (int32_t j = 0; j < c; ++j)
+                for (int32_t i = j; i < c; ++i) {
+                    schur[i + (int64_t)j * c] = panel[i + (int64_t)j * n];
+                    schur[j + (int64_t)i * c] = panel[i + (int64_t)j * n];
+                }
+            status[s] = 1;
+            continue;
+        }
         const int64_t eliminated = orc_front_nopivot(panel, n, update, r, c, r);
         if (eliminated < c) {
             status[s] = 2;
@@ -419,6 +435,8 @@ void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const 
         double largest = 0.0;
         int32_t largestAt = -1;
         for (int32_t s = 0; s < S; ++s) {
+            if (s == trailing)
+                continue; /* not eliminated: the boundary's equations */
             const int32_t c = sym->column_count[s], first = sym->first_column[s];
             const int64_t n = c + sym->row_start[s + 1] - sym->row_start[s];
             for (int32_t k = 0; k < c; ++k) {
@@ -459,6 +477,24 @@ void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const 
     free(fill);
 }
 
+void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows, const double *values,
+                         double max_ratio, double *panels, double *diag, orc_result *out)
+{
+    multifrontal(sym, offsets, rows, values, max_ratio, panels, diag, out, -1, NULL);
+}
+
+/* PartialFactor::factorise (src/linalg/supernodal/PartialFactor.cpp:65-83): the same factorisation with the last
+ * supernode (the trailing set: trailing_dofs columns, no rows below) assembled but not eliminated; its front, matrix
+ * entries then the children's update matrices in increasing child order, is K* (trailing_dofs square, both
+ * triangles, into schur). The inertia and the ratio test cover the interior equations only. */
+void orc_multifrontal_partial(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows,
+                              const double *values, double max_ratio, double *panels, double *diag, double *schur,
+                              orc_result *out)
+{
+    const int32_t S = sym->supernode_count;
+    multifrontal(sym, offsets, rows, values, max_ratio, panels, diag, out, S - 1, schur);
+}
+
 /* ---- solve ----------------------------------------------------------------------------------------------------- */
 
 /* X ← A⁻¹X with the factor of orc_multifrontal_pd (positive definite, symbolic shape): the forward sweep
@@ -466,7 +502,62 @@ void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const 
  * (X_F −= L_RFᵀ·X_R, X_F ← L_FF⁻ᵀX_F, supernodes in decreasing order): src/linalg/supernodal/FactorSolve.cpp:176-190,
  * SolveSweeps.cpp:272-294. x is n × nrhs column-major (leading dimension ldx). The chains differ from the reference's
  * (its subtree accumulators regroup the terms), so the solve agrees with it to rounding, not to the bit. */
-void orc_solve(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs)
+static void sweep_forward(const orc_symbolic *sym, const double *panels, const int64_t *panelOffset, int32_t skip,
+                          double *b)
+{
+    for (int32_t s = 0; s < sym->supernode_count; ++s) {
+        if (s == skip)
+            continue;
+        const int32_t c = sym->column_count[s], first = sym->first_column[s];
+        const int64_t r = sym->row_start[s + 1] - sym->row_start[s], n = c + r;
+        const double *p = panels + panelOffset[s];
+        const int32_t *rws = sym->rows + sym->row_start[s];
+        for (int32_t q = 0; q < c; ++q) {
+            const double xq = b[first + q];
+            for (int32_t i = q + 1; i < c; ++i)
+                b[first + i] -= p[i + (int64_t)q * n] * xq;
+            for (int64_t i = 0; i < r; ++i)
+                b[rws[i]] -= p[c + i + (int64_t)q * n] * xq;
+        }
+    }
+}
+
+static void sweep_diagonal(const orc_symbolic *sym, const double *panels, const int64_t *panelOffset, int32_t skip,
+                           double *b)
+{
+    for (int32_t s = 0; s < sym->supernode_count; ++s) {
+        if (s == skip)
+            continue;
+        const int32_t c = sym->column_count[s], first = sym->first_column[s];
+        const int64_t n = c + sym->row_start[s + 1] - sym->row_start[s];
+        const double *p = panels + panelOffset[s];
+        for (int32_t q = 0; q < c; ++q)
+            b[first + q] = b[first + q] / p[q + (int64_t)q * n];
+    }
+}
+
+static void sweep_backward(const orc_symbolic *sym, const double *panels, const int64_t *panelOffset, int32_t skip,
+                           double *b)
+{
+    for (int32_t s = sym->supernode_count - 1; s >= 0; --s) {
+        if (s == skip)
+            continue;
+        const int32_t c = sym->column_count[s], first = sym->first_column[s];
+        const int64_t r = sym->row_start[s + 1] - sym->row_start[s], n = c + r;
+        const double *p = panels + panelOffset[s];
+        const int32_t *rws = sym->rows + sym->row_start[s];
+        for (int32_t q = c - 1; q >= 0; --q) {
+            double sum = b[first + q];
+            for (int64_t i = r - 1; i >= 0; --i)
+                sum -= p[c + i + (int64_t)q * n] * b[rws[i]];
+            for (int32_t i = c - 1; i > q; --i)
+                sum -= p[i + (int64_t)q * n] * b[first + i];
+            b[first + q] = sum;
+        }
+    }
+}
+
+static int64_t *panel_offsets(const orc_symbolic *sym)
 {
     const int32_t S = sym->supernode_count;
     int64_t *panelOffset = (int64_t *)calloc((size_t)S + 1, sizeof(int64_t));
@@ -474,37 +565,46 @@ void orc_solve(const orc_symbolic *sym, const double *panels, double *x, int64_t
         const int64_t c = sym->column_count[s], r = sym->row_start[s + 1] - sym->row_start[s];
         panelOffset[s + 1] = panelOffset[s] + c * (c + r);
     }
+    return panelOffset;
+}
+
+void orc_solve(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs)
+{
+    int64_t *panelOffset = panel_offsets(sym);
     for (int64_t col = 0; col < nrhs; ++col) {
         double *b = x + col * ldx;
-        for (int32_t s = 0; s < S; ++s) {
-            const int32_t c = sym->column_count[s], first = sym->first_column[s];
-            const int64_t r = sym->row_start[s + 1] - sym->row_start[s], n = c + r;
-            const double *p = panels + panelOffset[s];
-            const int32_t *rws = sym->rows + sym->row_start[s];
-            for (int32_t q = 0; q < c; ++q) {
-                const double xq = b[first + q];
-                for (int32_t i = q + 1; i < c; ++i)
-                    b[first + i] -= p[i + (int64_t)q * n] * xq;
-                for (int64_t i = 0; i < r; ++i)
-                    b[rws[i]] -= p[c + i + (int64_t)q * n] * xq;
-            }
-            for (int32_t q = 0; q < c; ++q)
-                b[first + q] = b[first + q] / p[q + (int64_t)q * n];
-        }
-        for (int32_t s = S - 1; s >= 0; --s) {
-            const int32_t c = sym->column_count[s], first = sym->first_column[s];
-            const int64_t r = sym->row_start[s + 1] - sym->row_start[s], n = c + r;
-            const double *p = panels + panelOffset[s];
-            const int32_t *rws = sym->rows + sym->row_start[s];
-            for (int32_t q = c - 1; q >= 0; --q) {
-                double sum = b[first + q];
-                for (int64_t i = r - 1; i >= 0; --i)
-                    sum -= p[c + i + (int64_t)q * n] * b[rws[i]];
-                for (int32_t i = c - 1; i > q; --i)
-                    sum -= p[i + (int64_t)q * n] * b[first + i];
-                b[first + q] = sum;
-            }
-        }
+        sweep_forward(sym, panels, panelOffset, -1, b);
+        sweep_diagonal(sym, panels, panelOffset, -1, b);
+        sweep_backward(sym, panels, panelOffset, -1, b);
+    }
+    free(panelOffset);
+}
+
+/* PartialFactor::condenseLoads and recover (src/linalg/supernodal/PartialFactor.cpp:117-158) with the factor of
+ * orc_multifrontal_partial: the forward sweep over the interior supernodes leaves f* = f_b − K_bi·K_ii⁻¹·f_i in the
+ * trailing rows (condense: stop there). Recover: the trailing rows then replaced by u_b (trailing_values, trailing ×
+ * nrhs, leading dimension ldt), the interior's diagonal and backward sweeps give u_i = K_ii⁻¹·(f_i − K_ib·u_b). */
+void orc_partial_condense(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs)
+{
+    int64_t *panelOffset = panel_offsets(sym);
+    for (int64_t col = 0; col < nrhs; ++col)
+        sweep_forward(sym, panels, panelOffset, sym->supernode_count - 1, x + col * ldx);
+    free(panelOffset);
+}
+
+void orc_partial_recover(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs,
+                         const double *trailing_values, int64_t ldt)
+{
+    const int32_t T = sym->supernode_count - 1;
+    const int32_t first = sym->first_column[T], c = sym->column_count[T];
+    int64_t *panelOffset = panel_offsets(sym);
+    for (int64_t col = 0; col < nrhs; ++col) {
+        double *b = x + col * ldx;
+        sweep_forward(sym, panels, panelOffset, T, b);
+        for (int32_t i = 0; i < c; ++i)
+            b[first + i] = trailing_values[i + col * ldt];
+        sweep_diagonal(sym, panels, panelOffset, T, b);
+        sweep_backward(sym, panels, panelOffset, T, b);
     }
     free(panelOffset);
 }

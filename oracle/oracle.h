@@ -43,6 +43,12 @@ void orc_spmv(int32_t n, const int64_t *offsets, const int32_t *rows, const doub
 void orc_multifrontal_pd(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows, const double *values,
                          double max_ratio, double *panels, double *diag, orc_result *out);
 void orc_solve(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs);
+void orc_multifrontal_partial(const orc_symbolic *sym, const int64_t *offsets, const int32_t *rows,
+                              const double *values, double max_ratio, double *panels, double *diag, double *schur,
+                              orc_result *out);
+void orc_partial_condense(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs);
+void orc_partial_recover(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs,
+                         const double *trailing_values, int64_t ldt);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,9 @@
 //                        (include/oos/linalg/supernodal/SupernodalFactor.hpp:181-183,
 //                         include/oos/linalg/supernodal/FactorSolve.hpp:77-81)
 //   ref_spmv             SymmetricCsc::multiply (include/oos/linalg/sparse/SymmetricCsc.hpp:86-90)
+//   ref_analyse_trailing SymbolicFactor::analyse of a given order with a trailing set (SymbolicFactor.hpp:41-50)
+//   ref_partial          PartialFactor::factorise, schurComplement, condenseLoads, recover
+//                        (include/oos/linalg/supernodal/PartialFactor.hpp:75-101)
 //   ref_combine          SymmetricCsc::combine (src/linalg/sparse/SymmetricCsc.cpp:209-236)
 //   ref_scale_symmetric  SymmetricCsc::scaleSymmetric (src/linalg/sparse/SymmetricCsc.cpp:239-252)
 //   ref_pencil_scale_of  pencilScaleOf (src/linalg/eigen/PencilScaling.cpp:63-78)
@@ -30,6 +33,7 @@
 #include "oos/linalg/ordering/NodePermutation.hpp"
 #include "oos/linalg/sparse/SymmetricCsc.hpp"
 #include "oos/linalg/supernodal/FactorSolve.hpp"
+#include "oos/linalg/supernodal/PartialFactor.hpp"
 #include "oos/linalg/supernodal/SupernodalFactor.hpp"
 #include "oos/linalg/symbolic/SymbolicFactor.hpp"
 
@@ -141,6 +145,36 @@ void* ref_analyse(std::int32_t node_count, const std::int64_t* adj_offsets, cons
                 amalgamate != 0 ? linalg::AmalgamationThresholds{} : linalg::AmalgamationThresholds::none();
             analysis->symbolic =
                 linalg::SymbolicFactor::analyse(*analysis->graph, analysis->weights, order, 0, thresholds, budget);
+        },
+        err, errlen);
+    if (*status != 0)
+        return nullptr;
+    return analysis.release();
+}
+
+// The symbolic factor of the order given in `given_order` (node at each position) whose last `trailing` positions
+// are the trailing set (kept last, forming the last supernode).
+void* ref_analyse_trailing(std::int32_t node_count, const std::int64_t* adj_offsets, const std::int32_t* adj,
+                           const std::int32_t* weights, const std::int32_t* given_order, std::int32_t trailing,
+                           int amalgamate, int* status, char* err, int errlen)
+{
+    auto analysis = std::make_unique<Analysis>();
+    *status = guarded(
+        [&]
+        {
+            analysis->budget = std::make_unique<base::MemoryBudget>(BUDGET);
+            base::MemoryBudget& budget = *analysis->budget;
+            const std::int64_t adjacency = adj_offsets[node_count];
+            analysis->graph = std::make_unique<linalg::NodeGraph>(linalg::NodeGraph::fromAdjacency(
+                node_count, std::span<const std::int64_t>(adj_offsets, static_cast<std::size_t>(node_count) + 1),
+                std::span<const std::int32_t>(adj, static_cast<std::size_t>(adjacency)), budget));
+            analysis->weights.assign(weights, weights + node_count);
+            const linalg::NodePermutation order(
+                std::span<const std::int32_t>(given_order, static_cast<std::size_t>(node_count)), budget);
+            const linalg::AmalgamationThresholds thresholds =
+                amalgamate != 0 ? linalg::AmalgamationThresholds{} : linalg::AmalgamationThresholds::none();
+            analysis->symbolic = linalg::SymbolicFactor::analyse(*analysis->graph, analysis->weights, order,
+                                                                 trailing, thresholds, budget);
         },
         err, errlen);
     if (*status != 0)
@@ -266,6 +300,83 @@ int ref_factor_solve(void* handle, std::int32_t n, const std::int64_t* offsets, 
             }
             times[0] = std::chrono::duration<double>(t1 - t0).count();
             times[1] = solveSeconds;
+        },
+        err, errlen);
+}
+
+// The reference's PartialFactor of the matrix on the symbolic factor of `handle` (a trailing set), reference family
+// unless kernel is 1 (avx2) or 2 (avx512). schur (t × t, t = trailing DOFs) receives K*; panels (optional) the
+// interior panels in the symbolic shape, diag (optional, n values) the interior pivots, inertia[3] the interior's.
+// With loads (n × nrhs column-major): condensed (t × nrhs) receives condenseLoads(loads); with trailing_values
+// (t × nrhs) too, recovered (n × nrhs) receives recover(loads, trailing_values).
+int ref_partial(void* handle, std::int32_t n, const std::int64_t* offsets, const std::int32_t* rows,
+                const double* values, int kernel, double max_ratio, double* schur, double* panels, double* diag,
+                std::int64_t* inertia, const double* loads, std::int64_t nrhs, double* condensed,
+                const double* trailing_values, double* recovered, char* err, int errlen)
+{
+    Analysis& analysis = *static_cast<Analysis*>(handle);
+    return guarded(
+        [&]
+        {
+            base::MemoryBudget budget(BUDGET);
+            const std::int64_t nnz = offsets[n];
+            const linalg::SymmetricCsc matrix = linalg::SymmetricCsc::fromCompressedColumns(
+                n, std::span<const std::int64_t>(offsets, static_cast<std::size_t>(n) + 1),
+                std::span<const std::int32_t>(rows, static_cast<std::size_t>(nnz)),
+                std::span<const double>(values, static_cast<std::size_t>(nnz)), budget);
+            const linalg::KernelKind kind = kernel == 1   ? linalg::KernelKind::Avx2
+                                            : kernel == 2 ? linalg::KernelKind::Avx512
+                                                          : linalg::KernelKind::Reference;
+            const linalg::DenseKernels kernels(kind, linalg::DenseKernelOptions{});
+            linalg::FactorOptions options;
+            options.pivoting = linalg::PivotingMode::PositiveDefinite;
+            options.pivotPolicy.maxRatio = max_ratio;
+            const linalg::PartialFactor partial =
+                linalg::PartialFactor::factorise(analysis.symbolic, matrix, kernels, options, budget);
+            const auto t = static_cast<std::size_t>(partial.trailingDofCount());
+            const linalg::DenseMatrix& kstar = partial.schurComplement();
+            for (std::size_t j = 0; j < t; ++j)
+                for (std::size_t i = 0; i < t; ++i)
+                    schur[i + j * t] = kstar(i, j);
+            const linalg::SupernodalFactor& interior = partial.interiorFactor();
+            inertia[0] = static_cast<std::int64_t>(interior.inertia().positive);
+            inertia[1] = static_cast<std::int64_t>(interior.inertia().negative);
+            inertia[2] = static_cast<std::int64_t>(interior.inertia().zero);
+            if (panels != nullptr || diag != nullptr)
+            {
+                std::int64_t at = 0;
+                for (std::int32_t s = 0; s < interior.supernodeCount(); ++s)
+                {
+                    const linalg::FactorPanel panel = interior.panel(s);
+                    const std::size_t m = panel.rows.size();
+                    const std::size_t e = panel.eliminatedCount();
+                    if (panels != nullptr)
+                        for (std::size_t j = 0; j < e; ++j)
+                            for (std::size_t i = 0; i < m; ++i)
+                                panels[at++] = panel.values(i, j);
+                    if (diag != nullptr)
+                        for (std::size_t k = 0; k < e; ++k)
+                            diag[panel.rows[k]] = panel.values(k, k);
+                }
+            }
+            if (loads == nullptr || nrhs <= 0)
+                return;
+            const auto rowsN = static_cast<std::size_t>(n), columns = static_cast<std::size_t>(nrhs);
+            const linalg::ConstDenseMatrixView given{loads, rowsN, columns, rowsN};
+            if (condensed != nullptr)
+            {
+                const linalg::DenseMatrix out = partial.condenseLoads(given, kernels, budget);
+                for (std::size_t j = 0; j < columns; ++j)
+                    for (std::size_t i = 0; i < t; ++i)
+                        condensed[i + j * t] = out(i, j);
+            }
+            if (trailing_values != nullptr && recovered != nullptr)
+            {
+                std::memcpy(recovered, loads, sizeof(double) * rowsN * columns);
+                partial.recover(linalg::DenseMatrixView{recovered, rowsN, columns, rowsN},
+                                linalg::ConstDenseMatrixView{trailing_values, t, columns, std::max<std::size_t>(t, 1)},
+                                kernels);
+            }
         },
         err, errlen);
 }

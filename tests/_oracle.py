@@ -52,6 +52,13 @@ def oracle():
                                             ctypes.POINTER(OrcResult)]
         lib.orc_solve.restype = None
         lib.orc_solve.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64]
+        lib.orc_multifrontal_partial.restype = None
+        lib.orc_multifrontal_partial.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, vp, dbl, vp, vp, vp,
+                                                 ctypes.POINTER(OrcResult)]
+        lib.orc_partial_condense.restype = None
+        lib.orc_partial_condense.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64]
+        lib.orc_partial_recover.restype = None
+        lib.orc_partial_recover.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64, vp, i64]
         _orc = lib
     return _orc
 
@@ -85,6 +92,12 @@ def reference():
         lib.ref_scale_symmetric.argtypes = [i32, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
         lib.ref_pencil_scale_of.restype = ctypes.c_int
         lib.ref_pencil_scale_of.argtypes = [i32, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_analyse_trailing.restype = vp
+        lib.ref_analyse_trailing.argtypes = [i32, vp, vp, vp, vp, i32, ctypes.c_int, vp, ctypes.c_char_p,
+                                             ctypes.c_int]
+        lib.ref_partial.restype = ctypes.c_int
+        lib.ref_partial.argtypes = [vp, i32, vp, vp, vp, ctypes.c_int, dbl, vp, vp, vp, vp, vp, i64, vp, vp, vp,
+                                    ctypes.c_char_p, ctypes.c_int]
         _ref = lib
     return _ref
 
@@ -204,3 +217,81 @@ def ref_pencil_scale_of(matrix):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return scale
+
+
+def multifrontal_partial(sym, matrix, max_ratio=1e7):
+    """(panels, D per equation, K*, OrcResult) of the oracle's partial factorisation (the last supernode is the
+    trailing set: assembled, not eliminated)."""
+    lib = oracle()
+    t = int(sym.column_count[-1])
+    panels = np.zeros(sym.panel_value_count)
+    diag = np.zeros(sym.n)
+    schur = np.zeros((t, t), order="F")
+    res = OrcResult()
+    s = orc_symbolic(sym)
+    lib.orc_multifrontal_partial(ctypes.byref(s), P(matrix.offsets), P(matrix.rows), P(matrix.values), max_ratio,
+                                 P(panels), P(diag), P(schur), ctypes.byref(res))
+    return panels, diag, schur, res
+
+
+def oracle_condense(sym, panels, loads):
+    """The trailing rows of the forward sweep over the interior (f*), t × k."""
+    x = np.array(loads, dtype=np.float64, order="F", copy=True).reshape(sym.n, -1, order="F")
+    s = orc_symbolic(sym)
+    oracle().orc_partial_condense(ctypes.byref(s), P(panels), P(x), sym.n, x.shape[1])
+    t = int(sym.column_count[-1])
+    return np.asfortranarray(x[sym.n - t:, :])
+
+
+def oracle_recover(sym, panels, loads, trailing_values):
+    x = np.array(loads, dtype=np.float64, order="F", copy=True).reshape(sym.n, -1, order="F")
+    ub = np.asfortranarray(trailing_values, dtype=np.float64).reshape(-1, x.shape[1], order="F")
+    s = orc_symbolic(sym)
+    oracle().orc_partial_recover(ctypes.byref(s), P(panels), P(x), sym.n, x.shape[1], P(ub), max(ub.shape[0], 1))
+    return x
+
+
+def ref_symbolic_trailing(graph_offsets, adjacency, weights, order, trailing, amalgamate=1):
+    """The reference's SymbolicFactor::analyse of `order` with its last `trailing` positions the trailing set, as
+    (handle, sizes, flops, arrays) like ref_symbolic."""
+    lib = reference()
+    n = int(len(graph_offsets) - 1)
+    status = ctypes.c_int()
+    err = ctypes.create_string_buffer(1024)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    h = lib.ref_analyse_trailing(n, P(graph_offsets), P(adjacency), P(weights), P(order), int(trailing), amalgamate,
+                                 ctypes.byref(status), err, 1024)
+    if status.value != 0:
+        raise RuntimeError(err.value.decode())
+    sizes = np.zeros(7, np.int64)
+    flops = dbl()
+    lib.ref_sym_sizes(h, P(sizes), ctypes.byref(flops))
+    S = int(sizes[2])
+    arrays = dict(node_order=np.zeros(n, np.int32), first_column=np.zeros(S, np.int32),
+                  column_count=np.zeros(S, np.int32), parent=np.zeros(S, np.int32),
+                  row_start=np.zeros(S + 1, np.int64), rows=np.zeros(int(sizes[3]), np.int32))
+    lib.ref_sym_export(h, *[P(arrays[k]) for k in ["node_order", "first_column", "column_count", "parent",
+                                                     "row_start", "rows"]])
+    return h, sizes, flops.value, arrays
+
+
+def ref_partial(handle, matrix, trailing, interior_panel_values, loads=None, trailing_values=None, max_ratio=1e7,
+                kernel=0):
+    """The reference's PartialFactor: (K*, interior panels, D, inertia, condensed, recovered)."""
+    lib = reference()
+    n = matrix.n
+    schur = np.zeros((trailing, trailing), order="F")
+    panels = np.zeros(interior_panel_values)
+    diag = np.zeros(n)
+    inertia = np.zeros(3, np.int64)
+    nrhs = 0 if loads is None else loads.shape[1]
+    f = None if loads is None else np.asfortranarray(loads, dtype=np.float64)
+    condensed = np.zeros((trailing, nrhs), order="F") if nrhs else None
+    ub = None if trailing_values is None else np.asfortranarray(trailing_values, dtype=np.float64)
+    recovered = np.zeros((n, nrhs), order="F") if ub is not None else None
+    err = ctypes.create_string_buffer(4096)
+    rc = lib.ref_partial(handle, n, P(matrix.offsets), P(matrix.rows), P(matrix.values), kernel, max_ratio, P(schur),
+                         P(panels), P(diag), P(inertia), P(f), nrhs, P(condensed), P(ub), P(recovered), err, 4096)
+    if rc != 0:
+        raise RuntimeError(f"ref_partial {rc}: {err.value.decode(errors='replace')}")
+    return schur, panels, diag, inertia, condensed, recovered

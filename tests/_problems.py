@@ -83,3 +83,23 @@ def nested_chain_order(count):
 
     visit(0, count)
     return np.array(order, np.int32)
+
+
+def partial_plate(nx, jitter=0.0, seed=0, thickness_mode="constant"):
+    """A clamped plate cut by the line of nodes nearest x = 0.4 (its unit's boundary, the trailing set, ordered last
+    in the order of their node numbers), the rest in nested dissection order: (workload, symbolic factor with its
+    trailing set, matrix in its order, the trailing node count)."""
+    from o_o_solver_amd import SymbolicFactor, models, nested_dissection
+    w = models.plate(nx, jitter=jitter, seed=seed, thickness=(1e-3, 5e-3) if thickness_mode != "constant" else
+                     (2e-3, 2e-3), thickness_mode=thickness_mode)
+    x = np.asarray(w.coordinates).reshape(-1, 3)[:, 0]
+    line = np.abs(x - 0.4)
+    trailing = np.flatnonzero(line <= line.min() + 0.5 / nx)
+    order = nested_dissection(w.graph_offsets, w.adjacency, w.weights, w.coordinates)
+    keep = np.ones(w.node_count, bool)
+    keep[trailing] = False
+    order = np.concatenate([order[keep[order]], trailing]).astype(np.int32)
+    sym = SymbolicFactor.analyse(w.graph_offsets, w.adjacency, w.weights, order, trailing_node_count=len(trailing))
+    w.symbolic = sym
+    w.matrix = w.csc(sym.node_order)
+    return w, sym, w.matrix, len(trailing), order
