@@ -184,6 +184,38 @@ oosg_status oosg_matrix_set_combination(oosg_matrix *matrix, double alpha, const
                                         const oosg_matrix *b, const double *scale);
 void oosg_matrix_destroy(oosg_matrix *matrix);
 
+/* ---- the CSC of a block matrix on the device (SymmetricCsc::fromBlocks) ------------------------------------------ */
+
+/* A matrix assembled at node granularity (SparsePattern + BlockSparseMatrix, include/oos/linalg/sparse/
+ * SparsePattern.hpp:13-22, BlockSparseMatrix.hpp:14-36), in device memory. */
+typedef struct oosg_block_matrix_desc {
+    int32_t node_count;
+    const int64_t *first_dof;           /* node_count + 1: node n owns the DOFs [first_dof[n], first_dof[n + 1]), 1-6 */
+    const int64_t *column_start;        /* node_count + 1: the blocks of node column j */
+    const int32_t *block_rows;          /* per block, its row node: the diagonal block first, then rows below ascending */
+    const int64_t *block_value_offsets; /* blocks + 1: block b column-major, dofs(its row) × dofs(column), in values */
+    const double *values;
+} oosg_block_matrix_desc;
+
+/* SymmetricCsc::fromBlocks (SymmetricCsc.hpp:40-42, SymmetricCsc.cpp:41-200) on the device: the lower CSC of the kept
+ * DOFs in equation order (equation_of_dof: per node-major DOF its equation, -1 when eliminated:
+ * DofSelection::equationOfDof; dof_count = first_dof[node_count]), each column's rows ascending, the values copied
+ * from the blocks (a diagonal block read on and below its diagonal): the reference's bits. Every pointer is device
+ * memory; the work is queued on `stream` (a hipStream_t; null: the null stream) and finished on return. Call once with
+ * rows == NULL for offsets (kept + 1) and *nnz, then with rows and values of *nnz terms. OOSG_INVALID_ARGUMENT for a
+ * block matrix or a selection that is not well formed (DOF counts, block order, value offsets, equations that are not
+ * a numbering of the kept DOFs). */
+oosg_status oosg_csc_from_blocks_device(int32_t device, const oosg_block_matrix_desc *blocks,
+                                        const int32_t *equation_of_dof, int64_t dof_count, int32_t kept,
+                                        int64_t *offsets, int32_t *rows, double *values, int64_t *nnz, void *stream);
+/* A matrix of the plan from a CSC in device memory (oosg_csc_from_blocks_device's): its structure is read once to the
+ * host for the plan's positions, as oosg_matrix_create does. */
+oosg_status oosg_matrix_create_device(oosg_plan *plan, int32_t n, const int64_t *offsets, const int32_t *rows,
+                                      const double *values, oosg_matrix **out);
+/* New values of a matrix from device memory, in the order of its CSC (a matrix of the same structure converted on the
+ * device): its diagonal, FP32 scales and rows' copy follow on the device, no host copy of the values. */
+oosg_status oosg_matrix_set_values_device(oosg_matrix *matrix, const double *values);
+
 /* The numeric factorisation (SupernodalFactor::factorise). OOSG_SINGULAR for a zero or non-finite pivot in positive
  * definite mode, or when the ratio test fails with bailout >= 0; the message names the equations. */
 oosg_status oosg_factorise(oosg_plan *plan, const oosg_matrix *matrix, const oosg_factor_options *options,

@@ -21,11 +21,12 @@ import numpy as np
 
 from . import _native
 
-__all__ = ["SymmetricCsc", "SymbolicFactor", "SupernodalFactor", "FactorSolve", "FactorOptions", "PivotingMode",
+__all__ = ["DeviceMatrix", "SymmetricCsc", "SymbolicFactor", "SupernodalFactor", "FactorSolve", "FactorOptions", "PivotingMode",
            "Precision", "PivotPolicy", "Inertia", "PivotDiagnostics", "PivotIssue", "SolveReport", "Plan",
            "MemoryBudgetError", "MemoryShortfall", "SingularityError", "PrecisionInsufficientError",
            "OperationCancelledError", "InternalError", "HipError", "NotConvergedError", "nested_dissection",
-           "estimate_memory_bytes", "PartialFactor", "matrix_checksum"]
+           "estimate_memory_bytes", "PartialFactor", "matrix_checksum", "SparsePattern", "DofSelection",
+           "BlockSparseMatrix", "csc_from_blocks_device"]
 
 
 @dataclass(frozen=True)
@@ -216,6 +217,12 @@ class SymmetricCsc:
         if self.offsets.shape != (self.n + 1,) or self.rows.shape != self.values.shape or \
                 self.offsets[-1] != self.rows.shape[0]:
             raise ValueError("SymmetricCsc: offsets of n + 1 values and one row per value expected")
+
+    @classmethod
+    def from_blocks(cls, matrix, selection, device=0):
+        """SymmetricCsc::fromBlocks (SymmetricCsc.hpp:40-42), converted on the GPU and copied to the host."""
+        offsets, rows, values = csc_from_blocks_device(matrix, selection, device)
+        return cls(selection.kept_dof_count, offsets.cpu().numpy(), rows.cpu().numpy(), values.cpu().numpy())
 
     @classmethod
     def from_compressed_columns(cls, n, offsets, rows, values):
@@ -478,6 +485,23 @@ class DeviceMatrix:
         _check(self._lib.oosg_matrix_create(plan.handle, ctypes.byref(desc), ctypes.byref(handle)))
         self.handle = handle
 
+    @classmethod
+    def from_device_csc(cls, plan, offsets, rows, values):
+        """oosg_matrix_create_device: a matrix of the plan from a CSC in device memory (device tensors, as
+        csc_from_blocks_device makes them); its structure is read once to the host for the plan's positions."""
+        self = cls.__new__(cls)
+        self._lib = _native.lib()
+        self.plan = plan
+        n = int(offsets.numel()) - 1
+        handle = ctypes.c_void_p()
+        _follow_torch(plan, values)
+        _check(self._lib.oosg_matrix_create_device(plan.handle, n, offsets.data_ptr(), rows.data_ptr(),
+                                                   values.data_ptr(), ctypes.byref(handle)))
+        self.handle = handle
+        self.structure = SymmetricCsc(n, offsets.cpu().numpy(), rows.cpu().numpy(), values.cpu().numpy())
+        self.matrix = None
+        return self
+
     def set_values(self, values):
         values = np.ascontiguousarray(values, dtype=np.float64)
         if values.shape != self.structure.values.shape:
@@ -494,6 +518,18 @@ class DeviceMatrix:
             raise ValueError("SymmetricCsc::scaleSymmetric: one scale per equation expected")
         _check(self._lib.oosg_matrix_set_combination(self.handle, float(alpha), a.handle, float(beta), b.handle,
                                                      _ptr(scale_arr)))
+        self.matrix = None
+
+    def set_values_device(self, values):
+        """New values from a device tensor in the order of this structure's CSC (a matrix of the same pattern and
+        selection converted on the GPU: csc_from_blocks_device), with no host copy (oosg_matrix_set_values_device).
+        The host copy `matrix` is then None."""
+        if values.numel() != self.structure.values.shape[0]:
+            raise ValueError("DeviceMatrix.set_values_device: one value per stored entry expected")
+        import torch
+        values = values.to(torch.float64).contiguous()
+        _follow_torch(self.plan, values)
+        _check(self._lib.oosg_matrix_set_values_device(self.handle, values.data_ptr()))
         self.matrix = None
 
     def multiply(self, x):
@@ -753,6 +789,136 @@ class FactorSolve:
             raise NotConvergedError(f"{_native.lib().oosg_last_error().decode()} ({rep})")
         _check(status)
         return (xc.T if b.dim() == 2 else xc.reshape(-1)), rep
+
+
+class SparsePattern:
+    """The node-block pattern of a symmetric matrix (SparsePattern, include/oos/linalg/sparse/SparsePattern.hpp:13-68):
+    node n owns the DOFs [first_dof[n], first_dof[n + 1]); block column j is [column_start[j], column_start[j + 1]),
+    its diagonal block first, then the neighbours i > j ascending (block_rows); block b is dofs(i) × dofs(j),
+    column-major at block_value_offsets[b]. Built as the reference builds it from a node graph
+    (src/linalg/sparse/SparsePattern.cpp:10-60)."""
+
+    def __init__(self, first_dof, column_start, block_rows, block_value_offsets):
+        self.first_dof = np.ascontiguousarray(first_dof, dtype=np.int64)
+        self.column_start = np.ascontiguousarray(column_start, dtype=np.int64)
+        self.block_rows = np.ascontiguousarray(block_rows, dtype=np.int32)
+        self.block_value_offsets = np.ascontiguousarray(block_value_offsets, dtype=np.int64)
+
+    @classmethod
+    def from_graph(cls, graph_offsets, adjacency, node_dof_counts):
+        counts = np.asarray(node_dof_counts, dtype=np.int64)
+        nodes = counts.shape[0]
+        if np.any((counts < 1) | (counts > 6)):
+            raise ValueError("SparsePattern: expected 1 to 6 DOFs per node")
+        first_dof = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        column_start = [0]
+        rows, offsets = [], [0]
+        for j in range(nodes):
+            neighbours = np.sort(np.asarray(adjacency[graph_offsets[j]:graph_offsets[j + 1]]))
+            below = neighbours[neighbours > j]
+            for i in [j] + below.tolist():
+                rows.append(i)
+                offsets.append(offsets[-1] + int(counts[j]) * int(counts[i]))
+            column_start.append(len(rows))
+        return cls(first_dof, column_start, rows, offsets)
+
+    @property
+    def node_count(self):
+        return int(self.first_dof.shape[0] - 1)
+
+    @property
+    def dof_count(self):
+        return int(self.first_dof[-1])
+
+    @property
+    def value_count(self):
+        return int(self.block_value_offsets[-1])
+
+
+class DofSelection:
+    """The equations of the kept DOFs (DofSelection, include/oos/linalg/sparse/DofSelection.hpp:13-60): numbered node by
+    node in `node_order` (empty: natural), by increasing component within a node, eliminated DOFs removed
+    (src/linalg/sparse/DofSelection.cpp:22-95). equation_of_dof: per node-major DOF, -1 when eliminated."""
+
+    ELIMINATED = -1
+
+    def __init__(self, node_dof_counts, is_eliminated=None, node_order=None):
+        counts = np.asarray(node_dof_counts, dtype=np.int64)
+        if np.any((counts < 1) | (counts > 6)):
+            raise ValueError("DofSelection: expected 1 to 6 DOFs per node; give the DOF counts of the pattern "
+                             "(SparsePattern)")
+        first = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        total = int(first[-1])
+        kept_flag = np.ones(total, bool) if is_eliminated is None or len(is_eliminated) == 0 else \
+            (np.asarray(is_eliminated) == 0)
+        if kept_flag.shape[0] != total:
+            raise ValueError(f"DofSelection: expected one elimination flag per DOF ({total}), found "
+                             f"{kept_flag.shape[0]}")
+        nodes = counts.shape[0]
+        order = np.arange(nodes) if node_order is None or len(node_order) == 0 else np.asarray(node_order)
+        if len(order) > nodes or len(np.unique(order)) != len(order) or np.any((order < 0) | (order >= nodes)):
+            raise ValueError(f"DofSelection: expected distinct nodes in [0, {nodes}) in the node order")
+        equation = np.full(total, -1, np.int32)
+        nxt = 0
+        for node in order.tolist():
+            dofs = np.arange(first[node], first[node + 1])
+            kept = dofs[kept_flag[dofs]]
+            equation[kept] = np.arange(nxt, nxt + kept.shape[0], dtype=np.int32)
+            nxt += kept.shape[0]
+        if nxt != int(np.count_nonzero(kept_flag)):
+            raise ValueError("DofSelection: the node order leaves out nodes that keep DOFs")
+        self.first_dof = first
+        self.equation_of_dof = equation
+        self.kept_dof_count = nxt
+
+
+class BlockSparseMatrix:
+    """Values on a SparsePattern (BlockSparseMatrix, include/oos/linalg/sparse/BlockSparseMatrix.hpp:14-36): a numpy
+    array or a device tensor of pattern.value_count values."""
+
+    def __init__(self, pattern, values):
+        self.pattern = pattern
+        self.values = values
+
+
+def _device_blocks(matrix, selection, device):
+    import torch
+    dev = torch.device("cuda", device)
+    pat = matrix.pattern
+    arrays = dict(first_dof=torch.as_tensor(pat.first_dof, device=dev),
+                  column_start=torch.as_tensor(pat.column_start, device=dev),
+                  block_rows=torch.as_tensor(pat.block_rows, device=dev),
+                  block_value_offsets=torch.as_tensor(pat.block_value_offsets, device=dev),
+                  values=torch.as_tensor(matrix.values, dtype=torch.float64, device=dev).contiguous(),
+                  equation_of_dof=torch.as_tensor(selection.equation_of_dof, device=dev))
+    desc = _native.BlockMatrixDesc(pat.node_count, arrays["first_dof"].data_ptr(), arrays["column_start"].data_ptr(),
+                                   arrays["block_rows"].data_ptr(), arrays["block_value_offsets"].data_ptr(),
+                                   arrays["values"].data_ptr())
+    return arrays, desc
+
+
+def csc_from_blocks_device(matrix, selection, device=0):
+    """SymmetricCsc::fromBlocks (SymmetricCsc.hpp:40-42) on the GPU (oosg_csc_from_blocks_device): the lower CSC of the
+    kept DOFs in equation order as device tensors (offsets int64, rows int32, values float64), bit-identical to the
+    reference's conversion."""
+    import torch
+    arrays, desc = _device_blocks(matrix, selection, device)
+    dev = torch.device("cuda", device)
+    lib = _native.lib()
+    kept = selection.kept_dof_count
+    offsets = torch.empty(kept + 1, dtype=torch.int64, device=dev)
+    nnz = ctypes.c_int64()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    dofs = int(selection.equation_of_dof.shape[0])
+    _check(lib.oosg_csc_from_blocks_device(device, ctypes.byref(desc), arrays["equation_of_dof"].data_ptr(), dofs,
+                                           kept, offsets.data_ptr(), None, None, ctypes.byref(nnz),
+                                           ctypes.c_void_p(stream)))
+    rows = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    values = torch.empty(max(nnz.value, 1), dtype=torch.float64, device=dev)
+    _check(lib.oosg_csc_from_blocks_device(device, ctypes.byref(desc), arrays["equation_of_dof"].data_ptr(), dofs,
+                                           kept, offsets.data_ptr(), rows.data_ptr(), values.data_ptr(),
+                                           ctypes.byref(nnz), ctypes.c_void_p(stream)))
+    return offsets, rows[:nnz.value], values[:nnz.value]
 
 
 def matrix_checksum(matrix):

@@ -22,6 +22,11 @@ class SymbolicDesc(ctypes.Structure):
                 ("column_count", c_void_p), ("parent", c_void_p), ("row_start", c_void_p), ("rows", c_void_p)]
 
 
+class BlockMatrixDesc(ctypes.Structure):
+    _fields_ = [("node_count", c_int32), ("first_dof", c_void_p), ("column_start", c_void_p),
+                ("block_rows", c_void_p), ("block_value_offsets", c_void_p), ("values", c_void_p)]
+
+
 class CscDesc(ctypes.Structure):
     _fields_ = [("n", c_int32), ("offsets", c_void_p), ("rows", c_void_p), ("values", c_void_p)]
 
@@ -71,6 +76,11 @@ _SIGNATURES = {
     "oosg_partial_condense_loads": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
     "oosg_partial_recover": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
     "oosg_csc_checksum": (c_int, [ctypes.POINTER(CscDesc), ctypes.POINTER(ctypes.c_uint64)]),
+    "oosg_csc_from_blocks_device": (c_int, [c_int32, ctypes.POINTER(BlockMatrixDesc), c_void_p, c_int64, c_int32,
+                                            c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_int64), c_void_p]),
+    "oosg_matrix_create_device": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                          ctypes.POINTER(c_void_p)]),
+    "oosg_matrix_set_values_device": (c_int, [c_void_p, c_void_p]),
     "oosg_plan_destroy": (None, [c_void_p]),
     "oosg_plan_stats": (c_int, [c_void_p, c_void_p]),
     "oosg_plan_synchronize": (c_int, [c_void_p]),

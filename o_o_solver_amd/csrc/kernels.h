@@ -305,9 +305,38 @@ hipError_t launchPivotStatistics(const double* diag, const double* scale, const 
                                  const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
                                  hipStream_t stream);
 
+// A block matrix on a node pattern (SparsePattern + BlockSparseMatrix): node j owns the DOFs [firstDof[j],
+// firstDof[j + 1]); its block column is [columnStart[j], columnStart[j + 1]), the diagonal block first, then the blocks
+// of the rows blockRows[b] > j ascending, each column-major (dofs(row) × dofs(j)) at values + blockValueOffsets[b].
+struct BlockMatrixView
+{
+    int32_t nodes;
+    const int64_t* firstDof;
+    const int64_t* columnStart;
+    const int32_t* blockRows;
+    const int64_t* blockValueOffsets;
+    const double* values;
+};
+
+// SymmetricCsc::fromBlocks on the device. Validation (flag bits: DOF counts, block columns, block rows, value offsets,
+// equations; hits: per equation, zeroed by the caller), term counts per column (cursor null) or their scatter to the
+// slots cursor[column]++, the exclusive scan of the counts (pieceSums: scanPieces(n) values), the columns put in row
+// order.
+hipError_t launchBlocksValidate(const BlockMatrixView& m, const int32_t* equationOfDof, int64_t dofCount,
+                                int32_t kept, int64_t blockCount, int64_t valueCount, int32_t* hits, int32_t* flag,
+                                hipStream_t stream);
+hipError_t launchBlockTerms(const BlockMatrixView& m, const int32_t* equationOfDof, int32_t* counts,
+                            unsigned long long* cursor, int32_t* rows, double* values, hipStream_t stream);
+int64_t scanPieces(int64_t n);
+hipError_t launchExclusiveScan(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* pieceSums,
+                               hipStream_t stream);
+hipError_t launchSortColumns(const int64_t* offsets, int32_t n, const int32_t* rowsIn, const double* valuesIn,
+                             int32_t* rowsOut, double* valuesOut, hipStream_t stream);
+
 // The values of a matrix from two of its structure (SymmetricCsc::combine, then ::scaleSymmetric when scale is not
 // null): out[k] = scale[row]·(alpha·a[k] + beta·b[k])·scale[column], every product and the sum rounded in that
-// order; the diagonal of each column (0 where not stored) and the power-of-two scale of the FP32 factor. out may be
+// order (b null: scale[row]·a[k]·scale[column]); the diagonal of each column (0 where not stored) and the
+// power-of-two scale of the FP32 factor. out may be
 // a or b.
 hipError_t launchCombineValues(int32_t n, const int64_t* offsets, const int32_t* rows, double alpha, const double* a,
                                double beta, const double* b, const double* scale, double* out, double* diagonal,

@@ -2215,10 +2215,15 @@ __global__ __launch_bounds__(256) void combineValuesKernel(int32_t n, const int6
     const int64_t k0 = offsets[j], k1 = offsets[j + 1];
     for (int64_t k = k0; k < k1; ++k)
     {
-        // Two products and a sum, each rounded; then S·A·S as scale[i]·a_ij·scale[j], left to right.
-        const double left = alpha * a[k];
-        const double right = beta * b[k];
-        double v = left + right;
+        // Two products and a sum, each rounded (b null: the values of a as they are); then S·A·S as
+        // scale[i]·a_ij·scale[j], left to right.
+        double v = a[k];
+        if (b != nullptr)
+        {
+            const double left = alpha * a[k];
+            const double right = beta * b[k];
+            v = left + right;
+        }
         if (scale != nullptr)
         {
             const double rowScaled = scale[rows[k]] * v;
@@ -2693,6 +2698,285 @@ hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, 
     hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
     static_assert(DOT_GROUPS % 16 == 0, "runs of the partial sums");
     hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+    return hipGetLastError();
+}
+
+// ---- the CSC of a block matrix (SymmetricCsc::fromBlocks) ----------------------------------------------------------
+//
+// A thread per column node j walks its blocks (the diagonal block first, then the blocks of the rows below it) and
+// their lower-triangle terms of kept DOFs: a term of equations p, q lands in column min(p, q) at row max(p, q). The
+// counts per column are atomic integer sums (exact); the terms are scattered to atomic slots of their column, then
+// each column is put in row order by ranks (every row appears once in a column), so the result does not depend on the
+// order of the atomics.
+
+// Bits of the validation flag.
+constexpr int BLOCKS_BAD_DOFS = 1, BLOCKS_BAD_COLUMNS = 2, BLOCKS_BAD_ROWS = 4, BLOCKS_BAD_VALUES = 8,
+              BLOCKS_BAD_EQUATIONS = 16;
+
+__global__ __launch_bounds__(256) void blocksValidateKernel(BlockMatrixView m, const int32_t* equationOfDof,
+                                                            int64_t dofCount, int32_t kept, int64_t blockCount,
+                                                            int64_t valueCount, int32_t* hits, int32_t* flag)
+{
+    const int32_t j = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    if (j >= m.nodes)
+        return;
+    int bad = 0;
+    const int64_t fj = m.firstDof[j];
+    const int64_t dj = m.firstDof[j + 1] - fj;
+    if (dj < 1 || dj > 6 || fj < 0 || fj + dj > dofCount)
+        bad |= BLOCKS_BAD_DOFS;
+    else
+        for (int c = 0; c < int(dj); ++c)
+        {
+            const int32_t q = equationOfDof[fj + c];
+            if (q < -1 || q >= kept)
+                bad |= BLOCKS_BAD_EQUATIONS;
+            else if (q >= 0)
+                atomicAdd(&hits[q], 1);
+        }
+    const int64_t b0 = m.columnStart[j], b1 = m.columnStart[j + 1];
+    if (b0 < 0 || b1 < b0 || b1 > blockCount)
+        bad |= BLOCKS_BAD_COLUMNS;
+    else if (!(bad & BLOCKS_BAD_DOFS))
+        for (int64_t b = b0; b < b1; ++b)
+        {
+            const int32_t row = m.blockRows[b];
+            // The diagonal block first, then the rows below ascending (each node pair once).
+            if (b == b0 ? row != j : (row <= m.blockRows[b - 1] || row >= m.nodes))
+            {
+                bad |= BLOCKS_BAD_ROWS;
+                break;
+            }
+            const int64_t dr = m.firstDof[row + 1] - m.firstDof[row];
+            const int64_t v0 = m.blockValueOffsets[b], v1 = m.blockValueOffsets[b + 1];
+            if (dr < 1 || dr > 6 || v0 < 0 || v1 > valueCount || v1 - v0 != dr * dj)
+            {
+                bad |= BLOCKS_BAD_VALUES;
+                break;
+            }
+        }
+    if (bad)
+        atomicOr(flag, bad);
+}
+
+// Every kept equation is the equation of exactly one DOF.
+__global__ __launch_bounds__(256) void equationHitsKernel(const int32_t* hits, int32_t kept, int32_t* flag)
+{
+    const int32_t q = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    if (q < kept && hits[q] != 1)
+        atomicOr(flag, BLOCKS_BAD_EQUATIONS);
+}
+
+// Counts the terms of each column (scatter = false), or writes them to the slots cursor[column]++ (scatter = true).
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void blockTermsKernel(BlockMatrixView m, const int32_t* equationOfDof,
+                                                        int32_t* counts, unsigned long long* cursor, int32_t* rows,
+                                                        double* values)
+{
+    const int32_t j = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    if (j >= m.nodes)
+        return;
+    const int64_t fj = m.firstDof[j];
+    const int dj = int(m.firstDof[j + 1] - fj);
+    for (int64_t b = m.columnStart[j]; b < m.columnStart[j + 1]; ++b)
+    {
+        const int32_t node = m.blockRows[b];
+        const int64_t fr = m.firstDof[node];
+        const int dr = int(m.firstDof[node + 1] - fr);
+        const double* v = m.values + m.blockValueOffsets[b];
+        for (int c = 0; c < dj; ++c)
+        {
+            const int32_t q = equationOfDof[fj + c];
+            if (q < 0)
+                continue;
+            for (int r = node == j ? c : 0; r < dr; ++r)
+            {
+                const int32_t p = equationOfDof[fr + r];
+                if (p < 0)
+                    continue;
+                const int32_t column = min(p, q);
+                if (SCATTER)
+                {
+                    const unsigned long long at = atomicAdd(&cursor[column], 1ull);
+                    rows[at] = max(p, q);
+                    values[at] = v[int64_t(c) * dr + r];
+                }
+                else
+                    atomicAdd(&counts[column], 1);
+            }
+        }
+    }
+}
+
+// Exclusive scan of n int32 counts into int64 offsets (n + 1), three launches: sums of SCAN_ITEMS-element pieces,
+// their exclusive scan (one workgroup, a carry across passes), the pieces' scans plus their bases.
+constexpr int SCAN_ITEMS = 1024;
+
+__device__ inline int64_t blockInclusiveScan(int64_t v, int64_t* lds)
+{
+    // Hillis–Steele over the 256 threads' values.
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1)
+    {
+        const int64_t add = t >= d ? lds[t - d] : 0;
+        __syncthreads();
+        lds[t] += add;
+        __syncthreads();
+    }
+    const int64_t out = lds[t];
+    __syncthreads();
+    return out;
+}
+
+__global__ __launch_bounds__(256) void scanPieceSumsKernel(const int32_t* counts, int64_t n, int64_t* sums)
+{
+    __shared__ int64_t lds[256];
+    const int64_t base = int64_t(blockIdx.x) * SCAN_ITEMS;
+    int64_t v = 0;
+    for (int u = 0; u < SCAN_ITEMS / 256; ++u)
+    {
+        const int64_t i = base + threadIdx.x * (SCAN_ITEMS / 256) + u;
+        v += i < n ? counts[i] : 0;
+    }
+    v = blockInclusiveScan(v, lds);
+    if (threadIdx.x == 255)
+        sums[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void scanSumsKernel(int64_t* sums, int64_t pieces)
+{
+    __shared__ int64_t lds[256];
+    __shared__ int64_t total;
+    int64_t carry = 0;
+    for (int64_t p0 = 0; p0 < pieces; p0 += 256)
+    {
+        const int64_t i = p0 + threadIdx.x;
+        const int64_t v = i < pieces ? sums[i] : 0;
+        const int64_t inclusive = blockInclusiveScan(v, lds);
+        if (i < pieces)
+            sums[i] = carry + inclusive - v; // exclusive
+        if (threadIdx.x == 255)
+            total = inclusive;
+        __syncthreads();
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void scanPiecesKernel(const int32_t* counts, int64_t n, const int64_t* bases,
+                                                        int64_t* offsets)
+{
+    __shared__ int64_t lds[256];
+    constexpr int PER = SCAN_ITEMS / 256;
+    const int64_t base = int64_t(blockIdx.x) * SCAN_ITEMS + threadIdx.x * PER;
+    int64_t local[PER];
+    int64_t v = 0;
+    for (int u = 0; u < PER; ++u)
+    {
+        local[u] = base + u < n ? counts[base + u] : 0;
+        v += local[u];
+    }
+    int64_t before = bases[blockIdx.x] + blockInclusiveScan(v, lds) - v;
+    for (int u = 0; u < PER; ++u)
+    {
+        if (base + u < n)
+            offsets[base + u] = before;
+        before += local[u];
+        if (base + u == n - 1)
+            offsets[n] = before;
+    }
+}
+
+// Each column in row order: a wave per column ranks its terms (the rows below it in the column), then writes each at
+// its rank. Columns of at most RANK_LDS terms are ranked from LDS, longer ones from memory.
+constexpr int RANK_LDS = 256;
+
+__global__ __launch_bounds__(256) void sortColumnsKernel(const int64_t* offsets, int32_t n, const int32_t* rowsIn,
+                                                         const double* valuesIn, int32_t* rowsOut, double* valuesOut)
+{
+    __shared__ int32_t lds[4][RANK_LDS];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t column = int32_t(blockIdx.x) * 4 + w;
+    const bool active = column < n;
+    const int64_t k0 = active ? offsets[column] : 0;
+    const int m = active ? int(offsets[column + 1] - k0) : 0;
+    const int32_t* rows = rowsIn + k0;
+    const bool inLds = m <= RANK_LDS;
+    if (inLds)
+        for (int e = lane; e < m; e += 64)
+            lds[w][e] = rows[e];
+    __syncthreads();
+    for (int e = lane; e < m; e += 64)
+    {
+        const int32_t row = rows[e];
+        int rank = 0;
+        if (inLds)
+            for (int f = 0; f < m; ++f)
+                rank += lds[w][f] < row ? 1 : 0;
+        else
+            for (int f = 0; f < m; ++f)
+                rank += rows[f] < row ? 1 : 0;
+        rowsOut[k0 + rank] = row;
+        valuesOut[k0 + rank] = valuesIn[k0 + e];
+    }
+}
+
+hipError_t launchBlocksValidate(const BlockMatrixView& m, const int32_t* equationOfDof, int64_t dofCount,
+                                int32_t kept, int64_t blockCount, int64_t valueCount, int32_t* hits, int32_t* flag,
+                                hipStream_t stream)
+{
+    if (m.nodes > 0)
+        hipLaunchKernelGGL(blocksValidateKernel, dim3(unsigned((m.nodes + 255) / 256)), dim3(256), 0, stream, m,
+                           equationOfDof, dofCount, kept, blockCount, valueCount, hits, flag);
+    if (kept > 0)
+        hipLaunchKernelGGL(equationHitsKernel, dim3(unsigned((kept + 255) / 256)), dim3(256), 0, stream, hits, kept,
+                           flag);
+    return hipGetLastError();
+}
+
+hipError_t launchBlockTerms(const BlockMatrixView& m, const int32_t* equationOfDof, int32_t* counts,
+                            unsigned long long* cursor, int32_t* rows, double* values, hipStream_t stream)
+{
+    if (m.nodes == 0)
+        return hipSuccess;
+    const dim3 grid(unsigned((m.nodes + 255) / 256));
+    if (cursor == nullptr)
+        hipLaunchKernelGGL(blockTermsKernel<false>, grid, dim3(256), 0, stream, m, equationOfDof, counts, nullptr,
+                           nullptr, nullptr);
+    else
+        hipLaunchKernelGGL(blockTermsKernel<true>, grid, dim3(256), 0, stream, m, equationOfDof, nullptr, cursor,
+                           rows, values);
+    return hipGetLastError();
+}
+
+int64_t scanPieces(int64_t n)
+{
+    return (n + SCAN_ITEMS - 1) / SCAN_ITEMS;
+}
+
+hipError_t launchExclusiveScan(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* pieceSums,
+                               hipStream_t stream)
+{
+    if (n == 0)
+    {
+        hipLaunchKernelGGL(fillInt32Kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<int32_t*>(offsets), 2, 0);
+        return hipGetLastError();
+    }
+    const int64_t pieces = scanPieces(n);
+    hipLaunchKernelGGL(scanPieceSumsKernel, dim3(unsigned(pieces)), dim3(256), 0, stream, counts, n, pieceSums);
+    hipLaunchKernelGGL(scanSumsKernel, dim3(1), dim3(256), 0, stream, pieceSums, pieces);
+    hipLaunchKernelGGL(scanPiecesKernel, dim3(unsigned(pieces)), dim3(256), 0, stream, counts, n, pieceSums, offsets);
+    return hipGetLastError();
+}
+
+hipError_t launchSortColumns(const int64_t* offsets, int32_t n, const int32_t* rowsIn, const double* valuesIn,
+                             int32_t* rowsOut, double* valuesOut, hipStream_t stream)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(sortColumnsKernel, dim3(unsigned((n + 3) / 4)), dim3(256), 0, stream, offsets, n, rowsIn,
+                           valuesIn, rowsOut, valuesOut);
     return hipGetLastError();
 }
 

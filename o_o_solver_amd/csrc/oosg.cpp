@@ -4000,6 +4000,146 @@ oosg_status oosg_plan_synchronize(oosg_plan* plan)
         });
 }
 
+oosg_status oosg_csc_from_blocks_device(int32_t device, const oosg_block_matrix_desc* blocks,
+                                        const int32_t* equation_of_dof, int64_t dof_count, int32_t kept,
+                                        int64_t* offsets, int32_t* rows, double* values, int64_t* nnz, void* stream)
+{
+    return guarded(
+        [&]
+        {
+            if (blocks == nullptr || nnz == nullptr || offsets == nullptr || kept < 0 || blocks->node_count < 0 ||
+                (blocks->node_count > 0 && (blocks->first_dof == nullptr || blocks->column_start == nullptr ||
+                                            blocks->block_value_offsets == nullptr)) ||
+                (dof_count > 0 && equation_of_dof == nullptr))
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: null argument");
+            int count = 0;
+            check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+            if (device < 0 || device >= count)
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: no device " + std::to_string(device));
+            check(hipSetDevice(device), "hipSetDevice");
+            const hipStream_t st = static_cast<hipStream_t>(stream);
+            const int32_t N = blocks->node_count;
+            // The sizes the arrays end with: the DOFs, the blocks, the values.
+            int64_t totalDofs = 0, blockCount = 0, valueCount = 0;
+            if (N > 0)
+            {
+                check(hipMemcpyAsync(&totalDofs, blocks->first_dof + N, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+                      "fromBlocks: sizes");
+                check(hipMemcpyAsync(&blockCount, blocks->column_start + N, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                     st),
+                      "fromBlocks: sizes");
+                check(hipStreamSynchronize(st), "fromBlocks: sizes");
+                if (blockCount < 0)
+                    fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: a negative block count");
+                if (blockCount > 0 && blocks->block_rows == nullptr)
+                    fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: null argument");
+                check(hipMemcpyAsync(&valueCount, blocks->block_value_offsets + blockCount, sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, st),
+                      "fromBlocks: sizes");
+                check(hipStreamSynchronize(st), "fromBlocks: sizes");
+            }
+            if (totalDofs != dof_count)
+                // requireMatchingSelection (SymmetricCsc.cpp:25-37)
+                fail(OOSG_INVALID_ARGUMENT,
+                     "SymmetricCsc::fromBlocks: the DOF selection (" + std::to_string(dof_count) +
+                         " DOFs) was not made for the pattern of the matrix (" + std::to_string(totalDofs) +
+                         " DOFs); build it from the same node DOF counts");
+            if (valueCount > 0 && blocks->values == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: null argument");
+            const oosg::BlockMatrixView view{N, blocks->first_dof, blocks->column_start, blocks->block_rows,
+                                             blocks->block_value_offsets, blocks->values};
+            const auto scratch = [](DeviceBuffer& b, size_t bytes, const char* what)
+            { b.allocate(std::max<size_t>(bytes, 8), what); };
+            DeviceBuffer flag, hits, counts, pieceSums;
+            scratch(flag, sizeof(int32_t), "fromBlocks: flag");
+            scratch(hits, static_cast<size_t>(kept) * sizeof(int32_t), "fromBlocks: equations");
+            scratch(counts, static_cast<size_t>(kept) * sizeof(int32_t), "fromBlocks: counts");
+            scratch(pieceSums, static_cast<size_t>(oosg::scanPieces(kept)) * sizeof(int64_t), "fromBlocks: scan");
+            check(oosg::launchFillInt32(flag.as<int32_t>(), 1, 0, st), "fromBlocks");
+            check(oosg::launchFillInt32(hits.as<int32_t>(), kept, 0, st), "fromBlocks");
+            check(oosg::launchFillInt32(counts.as<int32_t>(), kept, 0, st), "fromBlocks");
+            check(oosg::launchBlocksValidate(view, equation_of_dof, dof_count, kept, blockCount, valueCount,
+                                             hits.as<int32_t>(), flag.as<int32_t>(), st),
+                  "fromBlocks: validation");
+            int32_t bad = 0;
+            check(hipMemcpyAsync(&bad, flag.ptr, sizeof(bad), hipMemcpyDeviceToHost, st), "fromBlocks: validation");
+            check(hipStreamSynchronize(st), "fromBlocks: validation");
+            if (bad != 0)
+                fail(OOSG_INVALID_ARGUMENT,
+                     std::string("SymmetricCsc::fromBlocks: the block matrix or the selection is not well formed:") +
+                         ((bad & 1) ? " a node's DOF count is outside 1 to 6;" : "") +
+                         ((bad & 2) ? " the block columns' starts do not ascend within the blocks;" : "") +
+                         ((bad & 4) ? " a block column does not start with its diagonal block and go down by "
+                                      "ascending rows;"
+                                    : "") +
+                         ((bad & 8) ? " a block's values do not span dofs(row) × dofs(column) within the values;"
+                                    : "") +
+                         ((bad & 16) ? " the equations are not a numbering of the kept DOFs;" : ""));
+            check(oosg::launchBlockTerms(view, equation_of_dof, counts.as<int32_t>(), nullptr, nullptr, nullptr, st),
+                  "fromBlocks: counts");
+            check(oosg::launchExclusiveScan(counts.as<int32_t>(), kept, offsets, pieceSums.as<int64_t>(), st),
+                  "fromBlocks: offsets");
+            check(hipMemcpyAsync(nnz, offsets + kept, sizeof(int64_t), hipMemcpyDeviceToHost, st), "fromBlocks");
+            check(hipStreamSynchronize(st), "fromBlocks: offsets");
+            if (rows == nullptr)
+                return;
+            if (*nnz > 0 && values == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "SymmetricCsc::fromBlocks: null argument");
+            DeviceBuffer cursor, scatteredRows, scatteredValues;
+            scratch(cursor, static_cast<size_t>(kept) * sizeof(int64_t), "fromBlocks: slots");
+            scratch(scatteredRows, static_cast<size_t>(*nnz) * sizeof(int32_t), "fromBlocks: terms");
+            scratch(scatteredValues, static_cast<size_t>(*nnz) * sizeof(double), "fromBlocks: terms");
+            if (kept > 0)
+                check(hipMemcpyAsync(cursor.ptr, offsets, static_cast<size_t>(kept) * sizeof(int64_t),
+                                     hipMemcpyDeviceToDevice, st),
+                      "fromBlocks: slots");
+            check(oosg::launchBlockTerms(view, equation_of_dof, nullptr, cursor.as<unsigned long long>(),
+                                         scatteredRows.as<int32_t>(), scatteredValues.as<double>(), st),
+                  "fromBlocks: terms");
+            check(oosg::launchSortColumns(offsets, kept, scatteredRows.as<int32_t>(), scatteredValues.as<double>(),
+                                          rows, values, st),
+                  "fromBlocks: row order");
+            check(hipStreamSynchronize(st), "fromBlocks");
+        });
+}
+
+oosg_status oosg_matrix_create_device(oosg_plan* plan, int32_t n, const int64_t* offsets, const int32_t* rows,
+                                      const double* values, oosg_matrix** out)
+{
+    std::vector<int64_t> hostOffsets;
+    std::vector<int32_t> hostRows;
+    std::vector<double> hostValues;
+    const oosg_status copied = guarded(
+        [&]
+        {
+            if (plan == nullptr || offsets == nullptr || out == nullptr || n < 0)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            setDevice(plan);
+            hostOffsets.resize(static_cast<size_t>(n) + 1);
+            check(hipMemcpy(hostOffsets.data(), offsets, hostOffsets.size() * sizeof(int64_t), hipMemcpyDeviceToHost),
+                  "matrix: structure");
+            const int64_t nnz = hostOffsets[n];
+            if (hostOffsets[0] != 0 || nnz < 0)
+                fail(OOSG_INVALID_ARGUMENT, "matrix: offsets[0] must be 0");
+            if (nnz > 0 && (rows == nullptr || values == nullptr))
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            hostRows.resize(static_cast<size_t>(nnz));
+            hostValues.resize(static_cast<size_t>(nnz));
+            if (nnz > 0)
+            {
+                check(hipMemcpy(hostRows.data(), rows, hostRows.size() * sizeof(int32_t), hipMemcpyDeviceToHost),
+                      "matrix: structure");
+                check(hipMemcpy(hostValues.data(), values, hostValues.size() * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "matrix: values");
+            }
+        });
+    if (copied != OOSG_OK)
+        return copied;
+    const oosg_csc_desc desc{n, hostOffsets.data(), hostRows.data(), hostValues.data()};
+    return oosg_matrix_create(plan, &desc, out);
+}
+
 oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_matrix** out)
 {
     return guarded(
@@ -4221,6 +4361,49 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
         });
 }
 
+} // extern "C"
+
+namespace
+{
+
+// The row map and the norm's slot of a matrix whose values change on the device (counted in the plan's budget).
+void prepareDeviceValues(oosg_matrix& m)
+{
+    oosg_plan& p = *m.plan;
+    if (m.dCsrMap.ptr != nullptr)
+        return;
+    const uint64_t bytes = m.csrFromCsc.size() * sizeof(int64_t) + sizeof(unsigned long long);
+    if (p.reserved + bytes > p.budget)
+        failBudget(p, bytes, "the matrix's row map (values set on the device)");
+    m.dCsrMap = upload(m.csrFromCsc, p.stream, "matrix: row map");
+    m.dCsrMap.account = &p.reserved;
+    p.reserved += m.dCsrMap.bytes;
+    allocateCounted(p, m.dNormKey, sizeof(unsigned long long), "matrix: norm");
+}
+
+// After new values in m.dValues (and its diagonal and scales): the rows' copy, the norm and the diagonal on the host;
+// the host values are stale from here.
+void valuesChangedOnDevice(oosg_matrix& m)
+{
+    oosg_plan& p = *m.plan;
+    const hipStream_t st = p.stream;
+    check(oosg::launchRowValues(m.n, m.dCsrOffsets.as<int64_t>(), m.dCsrMap.as<int64_t>(), m.dValues.as<double>(),
+                                m.dCsrValues.as<double>(), m.dNormKey.as<unsigned long long>(), st),
+          "matrix: rows");
+    unsigned long long key = 0;
+    check(hipMemcpyAsync(&key, m.dNormKey.ptr, sizeof(key), hipMemcpyDeviceToHost, st), "matrix: norm");
+    check(hipMemcpyAsync(m.diagonal.data(), m.dDiagonal.ptr, m.diagonal.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st),
+          "matrix: diagonal");
+    check(hipStreamSynchronize(st), "matrix: values");
+    std::memcpy(&m.normInf, &key, sizeof(double));
+    m.hostStale = true;
+}
+
+} // namespace
+
+extern "C" {
+
 oosg_status oosg_matrix_set_combination(oosg_matrix* m, double alpha, const oosg_matrix* a, double beta,
                                         const oosg_matrix* b, const double* scale)
 {
@@ -4242,16 +4425,7 @@ oosg_status oosg_matrix_set_combination(oosg_matrix* m, double alpha, const oosg
             setDevice(&p);
             followCaller(p);
             const hipStream_t st = p.stream;
-            if (m->dCsrMap.ptr == nullptr)
-            {
-                const uint64_t bytes = m->csrFromCsc.size() * sizeof(int64_t) + sizeof(unsigned long long);
-                if (p.reserved + bytes > p.budget)
-                    failBudget(p, bytes, "the matrix's row map (values combined on the device)");
-                m->dCsrMap = upload(m->csrFromCsc, st, "matrix: row map");
-                m->dCsrMap.account = &p.reserved;
-                p.reserved += m->dCsrMap.bytes;
-                allocateCounted(p, m->dNormKey, sizeof(unsigned long long), "matrix: norm");
-            }
+            prepareDeviceValues(*m);
             DeviceBuffer dScaleIn;
             if (scale != nullptr)
             {
@@ -4265,18 +4439,26 @@ oosg_status oosg_matrix_set_combination(oosg_matrix* m, double alpha, const oosg
                                             m->dValues.as<double>(), m->dDiagonal.as<double>(),
                                             m->dScale.as<double>(), st),
                   "matrix: combination");
-            check(oosg::launchRowValues(m->n, m->dCsrOffsets.as<int64_t>(), m->dCsrMap.as<int64_t>(),
-                                        m->dValues.as<double>(), m->dCsrValues.as<double>(),
-                                        m->dNormKey.as<unsigned long long>(), st),
-                  "matrix: rows");
-            unsigned long long key = 0;
-            check(hipMemcpyAsync(&key, m->dNormKey.ptr, sizeof(key), hipMemcpyDeviceToHost, st), "matrix: norm");
-            check(hipMemcpyAsync(m->diagonal.data(), m->dDiagonal.ptr, m->diagonal.size() * sizeof(double),
-                                 hipMemcpyDeviceToHost, st),
-                  "matrix: diagonal");
-            check(hipStreamSynchronize(st), "matrix: combination");
-            std::memcpy(&m->normInf, &key, sizeof(double));
-            m->hostStale = true;
+            valuesChangedOnDevice(*m);
+        });
+}
+
+oosg_status oosg_matrix_set_values_device(oosg_matrix* m, const double* values)
+{
+    return guarded(
+        [&]
+        {
+            if (m == nullptr || (values == nullptr && m->nnz > 0))
+                fail(OOSG_INVALID_ARGUMENT, "matrix: null argument");
+            oosg_plan& p = *m->plan;
+            setDevice(&p);
+            followCaller(p);
+            prepareDeviceValues(*m);
+            check(oosg::launchCombineValues(m->n, m->dOffsets.as<int64_t>(), m->dRows.as<int32_t>(), 1.0, values, 0.0,
+                                            nullptr, nullptr, m->dValues.as<double>(), m->dDiagonal.as<double>(),
+                                            m->dScale.as<double>(), p.stream),
+                  "matrix: values");
+            valuesChangedOnDevice(*m);
         });
 }
 

@@ -21,6 +21,8 @@
  *                                         K* equal to the dense K_bb − K_bi·K_ii⁻¹·K_ib.
  *   orc_partial_condense, _recover        within 1e-12 of the reference's condenseLoads and recover; the dense
  *                                         f_b − K_bi·K_ii⁻¹·f_i and K_ii⁻¹·(f_i − K_ib·u_b).
+ *   orc_csc_from_blocks                   bit-equal to the reference's SymmetricCsc::fromBlocks (random graphs, DOF
+ *                                         counts, eliminations and orders); the dense matrix of the blocks.
  */
 
 #include "oracle.h"
@@ -607,4 +609,76 @@ void orc_partial_recover(const orc_symbolic *sym, const double *panels, double *
         sweep_backward(sym, panels, panelOffset, T, b);
     }
     free(panelOffset);
+}
+
+/* ---- CSC conversion ------------------------------------------------------------------------------------------ */
+
+/* SymmetricCsc::fromBlocks (src/linalg/sparse/SymmetricCsc.cpp:41-200): the lower-triangle terms of the node blocks
+ * (block b of column node j couples node block_rows[b] >= j; its values column-major at block_value_offsets[b],
+ * dofs(row node) × dofs(j); a diagonal block read on and below its diagonal), the kept DOFs only
+ * (equation_of_dof[d] >= 0), each term at (max(p, q), min(p, q)) of its equations; every column's rows ascending.
+ * rows == NULL: offsets (kept + 1) only. Returns the terms. */
+int64_t orc_csc_from_blocks(int32_t nodes, const int64_t *first_dof, const int64_t *column_start,
+                            const int32_t *block_rows, const int64_t *block_value_offsets, const double *values,
+                            const int32_t *equation_of_dof, int32_t kept, int64_t *offsets, int32_t *rows,
+                            double *csc_values)
+{
+    for (int32_t q = 0; q <= kept; ++q)
+        offsets[q] = 0;
+    for (int pass = 0; pass < (rows != NULL ? 2 : 1); ++pass) {
+        int64_t *cursor = NULL;
+        if (pass == 1) {
+            cursor = (int64_t *)malloc(sizeof(int64_t) * (size_t)(kept > 0 ? kept : 1));
+            for (int32_t q = 0; q < kept; ++q)
+                cursor[q] = offsets[q];
+        }
+        for (int32_t j = 0; j < nodes; ++j) {
+            const int64_t fj = first_dof[j];
+            const int cj = (int)(first_dof[j + 1] - fj);
+            for (int64_t b = column_start[j]; b < column_start[j + 1]; ++b) {
+                const int32_t node = block_rows[b];
+                const int64_t fr = first_dof[node];
+                const int cr = (int)(first_dof[node + 1] - fr);
+                const double *v = values + block_value_offsets[b];
+                for (int c = 0; c < cj; ++c) {
+                    const int32_t q = equation_of_dof[fj + c];
+                    if (q < 0)
+                        continue;
+                    for (int r = node == j ? c : 0; r < cr; ++r) {
+                        const int32_t p = equation_of_dof[fr + r];
+                        if (p < 0)
+                            continue;
+                        const int32_t column = p < q ? p : q, row = p < q ? q : p;
+                        if (pass == 0)
+                            ++offsets[column + 1];
+                        else {
+                            const int64_t at = cursor[column]++;
+                            rows[at] = row;
+                            csc_values[at] = v[(int64_t)c * cr + r];
+                        }
+                    }
+                }
+            }
+        }
+        if (pass == 0)
+            for (int32_t q = 0; q < kept; ++q)
+                offsets[q + 1] += offsets[q];
+        free(cursor);
+    }
+    if (rows != NULL)
+        for (int32_t q = 0; q < kept; ++q)
+            /* insertion sort of the column by row (each row once) */
+            for (int64_t a = offsets[q] + 1; a < offsets[q + 1]; ++a) {
+                const int32_t row = rows[a];
+                const double value = csc_values[a];
+                int64_t b = a - 1;
+                while (b >= offsets[q] && rows[b] > row) {
+                    rows[b + 1] = rows[b];
+                    csc_values[b + 1] = csc_values[b];
+                    --b;
+                }
+                rows[b + 1] = row;
+                csc_values[b + 1] = value;
+            }
+    return offsets[kept];
 }

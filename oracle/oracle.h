@@ -47,6 +47,10 @@ void orc_multifrontal_partial(const orc_symbolic *sym, const int64_t *offsets, c
                               const double *values, double max_ratio, double *panels, double *diag, double *schur,
                               orc_result *out);
 void orc_partial_condense(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs);
+int64_t orc_csc_from_blocks(int32_t nodes, const int64_t *first_dof, const int64_t *column_start,
+                            const int32_t *block_rows, const int64_t *block_value_offsets, const double *values,
+                            const int32_t *equation_of_dof, int32_t kept, int64_t *offsets, int32_t *rows,
+                            double *csc_values);
 void orc_partial_recover(const orc_symbolic *sym, const double *panels, double *x, int64_t ldx, int64_t nrhs,
                          const double *trailing_values, int64_t ldt);
 

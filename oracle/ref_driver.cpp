@@ -13,6 +13,10 @@
 //   ref_analyse_trailing SymbolicFactor::analyse of a given order with a trailing set (SymbolicFactor.hpp:41-50)
 //   ref_partial          PartialFactor::factorise, schurComplement, condenseLoads, recover
 //                        (include/oos/linalg/supernodal/PartialFactor.hpp:75-101)
+//   ref_pattern_*, ref_selection, ref_from_blocks
+//                        SparsePattern of a node graph, DofSelection, SymmetricCsc::fromBlocks
+//                        (include/oos/linalg/sparse/SparsePattern.hpp:24-27, DofSelection.hpp:24-27,
+//                         SymmetricCsc.hpp:40-42)
 //   ref_combine          SymmetricCsc::combine (src/linalg/sparse/SymmetricCsc.cpp:209-236)
 //   ref_scale_symmetric  SymmetricCsc::scaleSymmetric (src/linalg/sparse/SymmetricCsc.cpp:239-252)
 //   ref_pencil_scale_of  pencilScaleOf (src/linalg/eigen/PencilScaling.cpp:63-78)
@@ -31,6 +35,9 @@
 #include "oos/linalg/ordering/ConstrainedOrdering.hpp"
 #include "oos/linalg/ordering/NodeGraph.hpp"
 #include "oos/linalg/ordering/NodePermutation.hpp"
+#include "oos/linalg/sparse/BlockSparseMatrix.hpp"
+#include "oos/linalg/sparse/DofSelection.hpp"
+#include "oos/linalg/sparse/SparsePattern.hpp"
 #include "oos/linalg/sparse/SymmetricCsc.hpp"
 #include "oos/linalg/supernodal/FactorSolve.hpp"
 #include "oos/linalg/supernodal/PartialFactor.hpp"
@@ -377,6 +384,115 @@ int ref_partial(void* handle, std::int32_t n, const std::int64_t* offsets, const
                                 linalg::ConstDenseMatrixView{trailing_values, t, columns, std::max<std::size_t>(t, 1)},
                                 kernels);
             }
+        },
+        err, errlen);
+}
+
+namespace
+{
+
+// The node graph and pattern of ref_pattern_* and ref_from_blocks.
+struct PatternOf
+{
+    base::MemoryBudget budget{BUDGET};
+    std::unique_ptr<linalg::NodeGraph> graph;
+    std::unique_ptr<linalg::SparsePattern> pattern;
+
+    PatternOf(std::int32_t nodes, const std::int64_t* offsets, const std::int32_t* adjacency,
+              const std::uint8_t* dofCounts)
+    {
+        graph = std::make_unique<linalg::NodeGraph>(linalg::NodeGraph::fromAdjacency(
+            nodes, std::span<const std::int64_t>(offsets, static_cast<std::size_t>(nodes) + 1),
+            std::span<const std::int32_t>(adjacency, static_cast<std::size_t>(offsets[nodes])), budget));
+        pattern = std::make_unique<linalg::SparsePattern>(
+            *graph, std::span<const std::uint8_t>(dofCounts, static_cast<std::size_t>(nodes)), budget);
+    }
+};
+
+} // namespace
+
+// The reference's SparsePattern of a node graph: sizes[0] = blocks, sizes[1] = values; then its arrays (first_dof and
+// column_start nodes + 1, block_rows blocks, block_value_offsets blocks + 1).
+int ref_pattern_export(std::int32_t nodes, const std::int64_t* offsets, const std::int32_t* adjacency,
+                       const std::uint8_t* dof_counts, std::int64_t* sizes, std::int64_t* first_dof,
+                       std::int64_t* column_start, std::int32_t* block_rows, std::int64_t* block_value_offsets,
+                       char* err, int errlen)
+{
+    return guarded(
+        [&]
+        {
+            const PatternOf p(nodes, offsets, adjacency, dof_counts);
+            const linalg::SparsePattern& pattern = *p.pattern;
+            sizes[0] = pattern.blockCount();
+            sizes[1] = pattern.valueCount();
+            if (first_dof == nullptr)
+                return;
+            for (std::int32_t v = 0; v <= nodes; ++v)
+            {
+                first_dof[v] = v < nodes ? pattern.firstDof(v) : pattern.dofCount();
+                column_start[v] = v < nodes ? pattern.columnBegin(v) : pattern.blockCount();
+            }
+            for (std::int64_t b = 0; b < pattern.blockCount(); ++b)
+            {
+                block_rows[b] = pattern.blockRow(b);
+                block_value_offsets[b] = pattern.blockValueOffset(b);
+            }
+            block_value_offsets[pattern.blockCount()] = pattern.valueCount();
+        },
+        err, errlen);
+}
+
+// The reference's DofSelection: equation_of_dof (one per node-major DOF), *kept the equations.
+int ref_selection(std::int32_t nodes, const std::uint8_t* dof_counts, const std::uint8_t* eliminated,
+                  std::int64_t total_dofs, const std::int32_t* node_order, std::int32_t order_count,
+                  std::int32_t* equation_of_dof, std::int32_t* kept, char* err, int errlen)
+{
+    return guarded(
+        [&]
+        {
+            base::MemoryBudget budget(BUDGET);
+            const linalg::DofSelection selection(
+                std::span<const std::uint8_t>(dof_counts, static_cast<std::size_t>(nodes)),
+                std::span<const std::uint8_t>(eliminated, eliminated != nullptr ? static_cast<std::size_t>(total_dofs)
+                                                                                 : 0),
+                std::span<const std::int32_t>(node_order, static_cast<std::size_t>(order_count)), budget);
+            for (std::int64_t d = 0; d < selection.totalDofCount(); ++d)
+                equation_of_dof[d] = selection.equationOfDof(d);
+            *kept = selection.keptDofCount();
+        },
+        err, errlen);
+}
+
+// SymmetricCsc::fromBlocks of the block matrix with `values` on the pattern of the graph, under the selection of
+// ref_selection's arguments: *nnz, then (when rows is not null) offsets (kept + 1), rows, csc_values.
+int ref_from_blocks(std::int32_t nodes, const std::int64_t* offsets, const std::int32_t* adjacency,
+                    const std::uint8_t* dof_counts, const double* values, const std::uint8_t* eliminated,
+                    std::int64_t total_dofs, const std::int32_t* node_order, std::int32_t order_count, int threads,
+                    std::int64_t* nnz, std::int64_t* csc_offsets, std::int32_t* rows, double* csc_values, char* err,
+                    int errlen)
+{
+    return guarded(
+        [&]
+        {
+            const PatternOf p(nodes, offsets, adjacency, dof_counts);
+            base::MemoryBudget budget(BUDGET);
+            linalg::BlockSparseMatrix matrix(*p.pattern, budget, "blocks");
+            std::copy_n(values, static_cast<std::size_t>(p.pattern->valueCount()), matrix.values().data());
+            const linalg::DofSelection selection(
+                std::span<const std::uint8_t>(dof_counts, static_cast<std::size_t>(nodes)),
+                std::span<const std::uint8_t>(eliminated, eliminated != nullptr ? static_cast<std::size_t>(total_dofs)
+                                                                                 : 0),
+                std::span<const std::int32_t>(node_order, static_cast<std::size_t>(order_count)), budget);
+            std::unique_ptr<base::ThreadPool> pool;
+            if (threads > 1)
+                pool = std::make_unique<base::ThreadPool>(static_cast<std::size_t>(threads));
+            const linalg::SymmetricCsc csc = linalg::SymmetricCsc::fromBlocks(matrix, selection, budget, pool.get());
+            *nnz = csc.nonZeroCount();
+            if (rows == nullptr)
+                return;
+            std::copy(csc.columnOffsets().begin(), csc.columnOffsets().end(), csc_offsets);
+            std::copy(csc.rowIndices().begin(), csc.rowIndices().end(), rows);
+            std::copy(csc.values().begin(), csc.values().end(), csc_values);
         },
         err, errlen);
 }

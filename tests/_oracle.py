@@ -57,6 +57,8 @@ def oracle():
                                                  ctypes.POINTER(OrcResult)]
         lib.orc_partial_condense.restype = None
         lib.orc_partial_condense.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64]
+        lib.orc_csc_from_blocks.restype = i64
+        lib.orc_csc_from_blocks.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]
         lib.orc_partial_recover.restype = None
         lib.orc_partial_recover.argtypes = [ctypes.POINTER(OrcSymbolic), vp, vp, i64, i64, vp, i64]
         _orc = lib
@@ -95,6 +97,13 @@ def reference():
         lib.ref_analyse_trailing.restype = vp
         lib.ref_analyse_trailing.argtypes = [i32, vp, vp, vp, vp, i32, ctypes.c_int, vp, ctypes.c_char_p,
                                              ctypes.c_int]
+        lib.ref_pattern_export.restype = ctypes.c_int
+        lib.ref_pattern_export.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_selection.restype = ctypes.c_int
+        lib.ref_selection.argtypes = [i32, vp, vp, i64, vp, i32, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        lib.ref_from_blocks.restype = ctypes.c_int
+        lib.ref_from_blocks.argtypes = [i32, vp, vp, vp, vp, vp, i64, vp, i32, ctypes.c_int, vp, vp, vp, vp,
+                                        ctypes.c_char_p, ctypes.c_int]
         lib.ref_partial.restype = ctypes.c_int
         lib.ref_partial.argtypes = [vp, i32, vp, vp, vp, ctypes.c_int, dbl, vp, vp, vp, vp, vp, i64, vp, vp, vp,
                                     ctypes.c_char_p, ctypes.c_int]
@@ -295,3 +304,80 @@ def ref_partial(handle, matrix, trailing, interior_panel_values, loads=None, tra
     if rc != 0:
         raise RuntimeError(f"ref_partial {rc}: {err.value.decode(errors='replace')}")
     return schur, panels, diag, inertia, condensed, recovered
+
+
+def oracle_csc_from_blocks(pattern, values, equation_of_dof, kept):
+    """(offsets, rows, values) of the oracle's restatement of SymmetricCsc::fromBlocks."""
+    lib = oracle()
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    eq = np.ascontiguousarray(equation_of_dof, dtype=np.int32)
+    offsets = np.zeros(kept + 1, np.int64)
+    args = (pattern.node_count, P(pattern.first_dof), P(pattern.column_start), P(pattern.block_rows),
+            P(pattern.block_value_offsets), P(values), P(eq), kept)
+    nnz = lib.orc_csc_from_blocks(*args, P(offsets), None, None)
+    rows = np.zeros(max(nnz, 1), np.int32)
+    out = np.zeros(max(nnz, 1))
+    lib.orc_csc_from_blocks(*args, P(offsets), P(rows), P(out))
+    return offsets, rows[:nnz], out[:nnz]
+
+
+def _check_ref(rc, err):
+    if rc != 0:
+        raise RuntimeError(f"reference {rc}: {err.value.decode(errors='replace')}")
+
+
+def ref_pattern(graph_offsets, adjacency, dof_counts):
+    """The reference's SparsePattern arrays (first_dof, column_start, block_rows, block_value_offsets)."""
+    lib = reference()
+    nodes = int(len(graph_offsets) - 1)
+    goff = np.ascontiguousarray(graph_offsets, dtype=np.int64)
+    adj = np.ascontiguousarray(adjacency, dtype=np.int32)
+    counts = np.ascontiguousarray(dof_counts, dtype=np.uint8)
+    sizes = np.zeros(2, np.int64)
+    err = ctypes.create_string_buffer(1024)
+    _check_ref(lib.ref_pattern_export(nodes, P(goff), P(adj), P(counts), P(sizes), None, None, None, None, err, 1024),
+               err)
+    first = np.zeros(nodes + 1, np.int64)
+    cols = np.zeros(nodes + 1, np.int64)
+    rows = np.zeros(int(sizes[0]), np.int32)
+    voff = np.zeros(int(sizes[0]) + 1, np.int64)
+    _check_ref(lib.ref_pattern_export(nodes, P(goff), P(adj), P(counts), P(sizes), P(first), P(cols), P(rows),
+                                      P(voff), err, 1024), err)
+    return first, cols, rows, voff
+
+
+def ref_selection(dof_counts, eliminated, node_order):
+    lib = reference()
+    counts = np.ascontiguousarray(dof_counts, dtype=np.uint8)
+    total = int(np.sum(counts.astype(np.int64)))
+    elim = None if eliminated is None else np.ascontiguousarray(eliminated, dtype=np.uint8)
+    order = np.ascontiguousarray(node_order if node_order is not None else [], dtype=np.int32)
+    eq = np.zeros(total, np.int32)
+    kept = ctypes.c_int32()
+    err = ctypes.create_string_buffer(1024)
+    _check_ref(lib.ref_selection(len(counts), P(counts), P(elim), total, P(order), len(order), P(eq),
+                                 ctypes.byref(kept), err, 1024), err)
+    return eq, kept.value
+
+
+def ref_from_blocks(graph_offsets, adjacency, dof_counts, values, eliminated, node_order, threads=1):
+    lib = reference()
+    nodes = int(len(graph_offsets) - 1)
+    goff = np.ascontiguousarray(graph_offsets, dtype=np.int64)
+    adj = np.ascontiguousarray(adjacency, dtype=np.int32)
+    counts = np.ascontiguousarray(dof_counts, dtype=np.uint8)
+    total = int(np.sum(counts.astype(np.int64)))
+    vals = np.ascontiguousarray(values, dtype=np.float64)
+    elim = None if eliminated is None else np.ascontiguousarray(eliminated, dtype=np.uint8)
+    order = np.ascontiguousarray(node_order if node_order is not None else [], dtype=np.int32)
+    nnz = ctypes.c_int64()
+    err = ctypes.create_string_buffer(1024)
+    args = (nodes, P(goff), P(adj), P(counts), P(vals), P(elim), total, P(order), len(order), threads,
+            ctypes.byref(nnz))
+    _check_ref(lib.ref_from_blocks(*args, None, None, None, err, 1024), err)
+    kept = int(ref_selection(dof_counts, eliminated, node_order)[1])
+    offsets = np.zeros(kept + 1, np.int64)
+    rows = np.zeros(max(nnz.value, 1), np.int32)
+    out = np.zeros(max(nnz.value, 1))
+    _check_ref(lib.ref_from_blocks(*args, P(offsets), P(rows), P(out), err, 1024), err)
+    return offsets, rows[:nnz.value], out[:nnz.value]

@@ -103,3 +103,24 @@ def partial_plate(nx, jitter=0.0, seed=0, thickness_mode="constant"):
     w.symbolic = sym
     w.matrix = w.csc(sym.node_order)
     return w, sym, w.matrix, len(trailing), order
+
+
+def block_problem(graph_offsets, adjacency, seed, max_dofs=6, eliminate=0.1, ordered=True):
+    """A block matrix on the pattern of a node graph: random DOF counts (1 to max_dofs per node), random values (the
+    diagonal blocks symmetric), a random fraction of DOFs eliminated (each node keeps one at least when ordered) and
+    a random node order: (dof_counts, pattern, values, eliminated flags, node order)."""
+    from o_o_solver_amd import SparsePattern
+    rng = np.random.default_rng(seed)
+    nodes = len(graph_offsets) - 1
+    counts = rng.integers(1, max_dofs + 1, nodes).astype(np.uint8)
+    pattern = SparsePattern.from_graph(graph_offsets, adjacency, counts)
+    values = rng.standard_normal(pattern.value_count)
+    # A diagonal block is stored complete (both triangles): make it symmetric.
+    for j in range(nodes):
+        b = pattern.column_start[j]
+        d = int(counts[j])
+        v = values[pattern.block_value_offsets[b]:pattern.block_value_offsets[b + 1]].reshape(d, d)
+        v[:] = (v + v.T) / 2
+    eliminated = (rng.random(pattern.dof_count) < eliminate).astype(np.uint8)
+    order = rng.permutation(nodes).astype(np.int32) if ordered else None
+    return counts, pattern, values, eliminated, order
