@@ -222,6 +222,25 @@ oosg_status oosg_factorise(oosg_plan *plan, const oosg_matrix *matrix, const oos
                            oosg_factor **out);
 void oosg_factor_destroy(oosg_factor *factor);
 
+/* ---- out-of-core factorisation (OutOfCoreFactor, include/oos/linalg/supernodal/OutOfCoreFactor.hpp:14-95) -------
+ *
+ * A plan whose factorisations keep their panels in pinned host memory (the spool, ScratchSpool's in-memory mode,
+ * ScratchSpool.hpp:44-60): each level's panels and inverses are made in one of two device windows and copied to the
+ * spool once the level is done, so the device holds two levels' panels, not the factor's. The solves (every solve
+ * entry point) copy the levels back into the windows, the next level's copy beside this level's sweep. The factor and
+ * the solves give the in-core plan's bits. Positive definite mode (FP64 or FP32 fronts). */
+oosg_status oosg_plan_create_out_of_core(const oosg_symbolic_desc *symbolic, int32_t device,
+                                         uint64_t hbm_budget_bytes, oosg_plan **out);
+/* What the spool holds and what the factorisation wrote and the solves read back (OutOfCoreTraffic,
+ * OutOfCoreFactor.hpp:14-20). */
+typedef struct oosg_out_of_core_traffic {
+    uint64_t spooled_bytes;
+    uint64_t written_bytes;
+    uint64_t read_bytes;
+    uint64_t read_requests;
+} oosg_out_of_core_traffic;
+oosg_status oosg_factor_out_of_core_traffic(const oosg_factor *factor, oosg_out_of_core_traffic *traffic);
+
 /* ---- partial factorisation (PartialFactor, include/oos/linalg/supernodal/PartialFactor.hpp:47-75) ---------------
  *
  * The plan of a symbolic factor analysed with a trailing set (the boundary of a unit, ordered last: its last supernode,

@@ -26,7 +26,7 @@ __all__ = ["DeviceMatrix", "SymmetricCsc", "SymbolicFactor", "SupernodalFactor",
            "MemoryBudgetError", "MemoryShortfall", "SingularityError", "PrecisionInsufficientError",
            "OperationCancelledError", "InternalError", "HipError", "NotConvergedError", "nested_dissection",
            "estimate_memory_bytes", "PartialFactor", "matrix_checksum", "SparsePattern", "DofSelection",
-           "BlockSparseMatrix", "csc_from_blocks_device"]
+           "BlockSparseMatrix", "csc_from_blocks_device", "OutOfCoreFactor", "OutOfCoreTraffic"]
 
 
 @dataclass(frozen=True)
@@ -345,27 +345,35 @@ class SymbolicFactor:
         """The device bytes a plan of this structure reserves (SupernodalFactor::estimateMemoryBytes)."""
         return estimate_memory_bytes(self, pivoting)
 
-    def plan(self, device=0, hbm_budget_bytes=0, partial=False):
+    def plan(self, device=0, hbm_budget_bytes=0, partial=False, out_of_core=False):
         """The device plan of this structure (kept: one per device, budget and kind). partial: the plan of a
-        PartialFactor, whose trailing supernode is assembled (K*) and never eliminated."""
-        key = (device, hbm_budget_bytes, bool(partial))
+        PartialFactor, whose trailing supernode is assembled (K*) and never eliminated; out_of_core: the plan of an
+        OutOfCoreFactor, whose panels are spooled to pinned host memory level by level."""
+        key = (device, hbm_budget_bytes, bool(partial), bool(out_of_core))
         if key not in self._plans:
-            self._plans[key] = Plan(self, device, hbm_budget_bytes, partial)
+            self._plans[key] = Plan(self, device, hbm_budget_bytes, partial, out_of_core)
         return self._plans[key]
 
 
 class Plan:
     """oosg_plan: levels, relative indices, the kernel schedule and the device memory of one symbolic factor."""
 
-    def __init__(self, symbolic, device=0, hbm_budget_bytes=0, partial=False):
+    def __init__(self, symbolic, device=0, hbm_budget_bytes=0, partial=False, out_of_core=False):
         self._lib = _native.lib()
         self.symbolic = symbolic
         handle = ctypes.c_void_p()
         desc = symbolic.desc()
-        trailing = symbolic.trailing_dof_count if partial else 0
-        _check(self._lib.oosg_plan_create_partial(ctypes.byref(desc), int(trailing), int(device),
-                                                  int(hbm_budget_bytes), ctypes.byref(handle)))
+        if partial and out_of_core:
+            raise ValueError("plan: a partial plan is made in core")
+        if out_of_core:
+            _check(self._lib.oosg_plan_create_out_of_core(ctypes.byref(desc), int(device), int(hbm_budget_bytes),
+                                                          ctypes.byref(handle)))
+        else:
+            trailing = symbolic.trailing_dof_count if partial else 0
+            _check(self._lib.oosg_plan_create_partial(ctypes.byref(desc), int(trailing), int(device),
+                                                      int(hbm_budget_bytes), ctypes.byref(handle)))
         self.partial = bool(partial)
+        self.out_of_core = bool(out_of_core)
         self.handle = handle
         self._matrices = {}
 
@@ -572,11 +580,11 @@ class SupernodalFactor:
         self._info = info
 
     @classmethod
-    def _call(cls, symbolic, matrix, options, device, hbm_budget_bytes, device_matrix):
+    def _call(cls, symbolic, matrix, options, device, hbm_budget_bytes, device_matrix, plan=None):
         """The plan, the device matrix, the C options (with the precision the selector picks, for LEARNED), the
         reference diagonal they point to, and the selector's probability."""
         options = options or FactorOptions()
-        plan = symbolic.plan(device, hbm_budget_bytes)
+        plan = plan or (device_matrix.plan if device_matrix is not None else symbolic.plan(device, hbm_budget_bytes))
         if device_matrix is None:
             device_matrix = plan.matrix(matrix)
         reference = None
@@ -606,12 +614,12 @@ class SupernodalFactor:
                                       int(options.pivot_policy.bailout), _ptr(reference))
 
     @classmethod
-    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None):
+    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None, plan=None):
         """SupernodalFactor::factorise (SupernodalFactor.hpp:176-183). `matrix` is a SymmetricCsc in the factor's
         order; `device_matrix` (optional) is its device copy, reused across factorisations. With Precision.LEARNED,
         an FP32 factor that fails its pivot tests is replaced by the FP64 factor (`fell_back`)."""
         options, plan, device_matrix, opts, reference, selected = cls._call(symbolic, matrix, options, device,
-                                                                            hbm_budget_bytes, device_matrix)
+                                                                            hbm_budget_bytes, device_matrix, plan)
         handle = ctypes.c_void_p()
         lib = _native.lib()
         status = lib.oosg_factorise(plan.handle, device_matrix.handle, ctypes.byref(opts), ctypes.byref(handle))
@@ -919,6 +927,63 @@ def csc_from_blocks_device(matrix, selection, device=0):
                                            kept, offsets.data_ptr(), rows.data_ptr(), values.data_ptr(),
                                            ctypes.byref(nnz), ctypes.c_void_p(stream)))
     return offsets, rows[:nnz.value], values[:nnz.value]
+
+
+@dataclass(frozen=True)
+class OutOfCoreTraffic:
+    """What the spool holds and what the factorisation wrote and the solves read back (OutOfCoreTraffic,
+    include/oos/linalg/supernodal/OutOfCoreFactor.hpp:14-20)."""
+    spooled_bytes: int
+    written_bytes: int
+    read_bytes: int
+    read_requests: int
+
+
+class OutOfCoreFactor:
+    """OutOfCoreFactor (include/oos/linalg/supernodal/OutOfCoreFactor.hpp:22-95) on the GPU: the factorisation on an
+    out-of-core plan (oosg_plan_create_out_of_core) keeps each level's panels and inverses in one of two device windows
+    and spools them to pinned host memory once the level is done; the solves copy the levels back, the next level's
+    copy beside this level's sweep. The factor and its solves are the in-core factor's bits. Positive definite mode
+    (FP64 or FP32 fronts): the reference's indefinite mode of this class is not built here."""
+
+    @classmethod
+    def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0):
+        """The factorisation with its panels spooled. ValueError for indefinite pivoting; otherwise what
+        SupernodalFactor.factorise raises."""
+        options = options or FactorOptions()
+        if options.pivoting != PivotingMode.POSITIVE_DEFINITE:
+            raise ValueError("OutOfCoreFactor: the out-of-core factorisation runs in positive definite mode here")
+        self = cls.__new__(cls)
+        plan = symbolic.plan(device, hbm_budget_bytes, out_of_core=True)
+        self._factor = SupernodalFactor.factorise(symbolic, matrix, options, device, hbm_budget_bytes,
+                                                  device_matrix=plan.matrix(matrix), plan=plan)
+        return self
+
+    @property
+    def factor(self):
+        """The factor (its panels are read from the spool: panel(s), diagonal(), inertia, diagnostics)."""
+        return self._factor
+
+    @property
+    def inertia(self):
+        return self._factor.inertia
+
+    @property
+    def is_singular(self):
+        return self._factor.is_singular
+
+    def traffic(self):
+        t = _native.OutOfCoreTrafficC()
+        _check(_native.lib().oosg_factor_out_of_core_traffic(self._factor.handle, ctypes.byref(t)))
+        return OutOfCoreTraffic(int(t.spooled_bytes), int(t.written_bytes), int(t.read_bytes), int(t.read_requests))
+
+    @property
+    def spooled_bytes(self):
+        return self.traffic().spooled_bytes
+
+    def solve(self, b):
+        """B ← A⁻¹·B (FactorSolve::solve's bits), as FactorSolve.solve."""
+        return FactorSolve(self._factor).solve(b)
 
 
 def matrix_checksum(matrix):

@@ -27,6 +27,11 @@ class BlockMatrixDesc(ctypes.Structure):
                 ("block_rows", c_void_p), ("block_value_offsets", c_void_p), ("values", c_void_p)]
 
 
+class OutOfCoreTrafficC(ctypes.Structure):
+    _fields_ = [("spooled_bytes", ctypes.c_uint64), ("written_bytes", ctypes.c_uint64),
+                ("read_bytes", ctypes.c_uint64), ("read_requests", ctypes.c_uint64)]
+
+
 class CscDesc(ctypes.Structure):
     _fields_ = [("n", c_int32), ("offsets", c_void_p), ("rows", c_void_p), ("values", c_void_p)]
 
@@ -73,6 +78,9 @@ _SIGNATURES = {
     "oosg_plan_create_partial": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, c_int32, ctypes.c_uint64,
                                          ctypes.POINTER(c_void_p)]),
     "oosg_partial_schur_complement": (c_int, [c_void_p, c_void_p, c_int64]),
+    "oosg_plan_create_out_of_core": (c_int, [ctypes.POINTER(SymbolicDesc), c_int32, ctypes.c_uint64,
+                                             ctypes.POINTER(c_void_p)]),
+    "oosg_factor_out_of_core_traffic": (c_int, [c_void_p, ctypes.POINTER(OutOfCoreTrafficC)]),
     "oosg_partial_condense_loads": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
     "oosg_partial_recover": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64]),
     "oosg_csc_checksum": (c_int, [ctypes.POINTER(CscDesc), ctypes.POINTER(ctypes.c_uint64)]),

@@ -393,6 +393,17 @@ struct oosg_plan
     // trailing front's assembly blocks, and per trailing column the children's update rows that land on it (child
     // order) for the condensation of loads.
     int32_t trailing = -1, trailingDofs = 0, eliminated = 0;
+    // An out-of-core plan (OutOfCoreFactor): each level's panels and inverses are made in one of two device windows
+    // (level l in window l mod 2; panelOff and linvOff are offsets within the level's window), spooled to pinned host
+    // memory once the level is done (level l at spoolPanelBase[l], spoolLinvBase[l]) and streamed back level by level
+    // by the solves. The copies run on copyStream; oocReady[w] is recorded once window w holds what its next user
+    // needs, oocFree[w] once its last user is done with it.
+    bool outOfCore = false;
+    int64_t windowPanels = 0, windowLinv = 0;
+    std::vector<int32_t> levelOf;
+    std::vector<int64_t> spoolPanelBase, spoolLinvBase;
+    hipStream_t copyStream = nullptr;
+    hipEvent_t oocReady[2] = {nullptr, nullptr}, oocFree[2] = {nullptr, nullptr};
     std::vector<oosg::AssemblyBlock> trailingBlocks;
     std::vector<oosg::AssemblyItem> trailingItems;
     std::vector<int32_t> trailingPullStart, trailingPulls;
@@ -516,6 +527,14 @@ struct oosg_plan
                 (void)hipEventDestroy(e);
         if (sweepStream)
             (void)hipStreamDestroy(sweepStream);
+        if (copyStream)
+            (void)hipStreamDestroy(copyStream);
+        for (hipEvent_t e : oocReady)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (hipEvent_t e : oocFree)
+            if (e)
+                (void)hipEventDestroy(e);
         if (callerEvent)
             (void)hipEventDestroy(callerEvent);
         if (stream && ownsStream)
@@ -580,8 +599,15 @@ struct oosg_factor
     oosg_factor_info info{};
     // The first OOSG_RECORDED_ISSUE_COUNT pivots failing the ratio test, when the factor was kept despite them.
     std::vector<oosg_pivot_issue> issues;
+    // Out of core: the panels, then the inverses, of every level in pinned host memory (the spool), and what the
+    // factorisation wrote to it and the solves read back.
+    void* spool = nullptr;
+    size_t spoolBytes = 0;
+    mutable oosg_out_of_core_traffic traffic{};
     ~oosg_factor()
     {
+        if (spool != nullptr)
+            (void)hipHostFree(spool);
         if (pivoted || plan == nullptr || panels.ptr == nullptr)
             return; // a pivoted factor is sized for its own structure
         FactorStorage back;
@@ -701,6 +727,8 @@ void pollCancellation(oosg_plan& p)
 // ---- plan --------------------------------------------------------------------------------------------------------
 
 void deriveLevels(oosg_plan& p);
+oosg_status createPlan(const oosg_symbolic_desc* symbolic, int32_t trailing_dof_count, bool outOfCore, int32_t device,
+                       uint64_t hbm_budget_bytes, oosg_plan** out);
 
 void buildStructure(oosg_plan& p, const oosg_symbolic_desc& s)
 {
@@ -957,6 +985,46 @@ void placeUpdateMatrices(oosg_plan& p, const std::vector<int32_t>& height)
     p.updateOff[S] = end;
 }
 
+// An out-of-core plan's layout: each level's panels (and inverses) from the start of its window, in supernode order;
+// the two windows one after the other (panelOff[S], linvOff[S]: both windows); the spool's copy of each level.
+void windowLayout(oosg_plan& p, const std::vector<int32_t>& height)
+{
+    const int32_t S = p.S;
+    std::vector<int64_t> panelValues(S), linvValues(S);
+    for (int32_t k = 0; k < S; ++k)
+    {
+        panelValues[k] = p.panelOff[k + 1] - p.panelOff[k];
+        linvValues[k] = p.linvOff[k + 1] - p.linvOff[k];
+    }
+    std::vector<int64_t> panelFill(p.levels, 0), linvFill(p.levels, 0);
+    p.levelOf.assign(S, -1);
+    for (int32_t k = 0; k < S; ++k)
+    {
+        if (k == p.trailing)
+            continue;
+        const int32_t l = height[k];
+        p.levelOf[k] = l;
+        p.panelOff[k] = panelFill[l];
+        p.linvOff[k] = linvFill[l];
+        panelFill[l] += panelValues[k];
+        linvFill[l] += linvValues[k];
+    }
+    p.windowPanels = 0;
+    p.windowLinv = 0;
+    p.spoolPanelBase.assign(p.levels + 1, 0);
+    p.spoolLinvBase.assign(p.levels + 1, 0);
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        // Windows of a multiple of 64 values (the second starts on a 512-byte boundary in FP64).
+        p.windowPanels = std::max(p.windowPanels, (panelFill[l] + 63) / 64 * 64);
+        p.windowLinv = std::max(p.windowLinv, (linvFill[l] + 63) / 64 * 64);
+        p.spoolPanelBase[l + 1] = p.spoolPanelBase[l] + panelFill[l];
+        p.spoolLinvBase[l + 1] = p.spoolLinvBase[l] + linvFill[l];
+    }
+    p.panelOff[S] = 2 * p.windowPanels;
+    p.linvOff[S] = 2 * p.windowLinv;
+}
+
 // From the structure (first, ncol, nrow, parent, children): the offsets of the panels, update matrices and inverses,
 // the levels by height, the flops, and the sweeps' work lists by level.
 void deriveLevels(oosg_plan& p)
@@ -1181,6 +1249,8 @@ void deriveLevels(oosg_plan& p)
     }
     placeUpdateMatrices(p, height);
     splitTinySupernodes(p);
+    if (p.outOfCore)
+        windowLayout(p, height);
 }
 
 // The algorithmic flops of a tile: 2·k per element it updates (the lower part only on the diagonal).
@@ -1325,7 +1395,8 @@ void buildSchedule(oosg_plan& p)
     // matrices more than it gains), 42.7 ms without the overlap.
     constexpr int32_t OVERLAP_FRONTS = 1;
     p.overlapLevel = -1;
-    for (int32_t l = p.levels - 1; l >= 1 && p.levelStart[l + 1] - p.levelStart[l] <= OVERLAP_FRONTS; --l)
+    for (int32_t l = p.levels - 1; l >= 1 && !p.outOfCore && p.levelStart[l + 1] - p.levelStart[l] <= OVERLAP_FRONTS;
+         --l)
         p.overlapLevel = l;
     for (int32_t l = 0; l < p.levels; ++l)
     {
@@ -1773,8 +1844,10 @@ void ensureSolveBuffers(oosg_plan& p);
 // cancellable: launched one launch at a time (not captured), so the cancellation is polled between levels.
 template <typename T>
 void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled,
-               bool cancellable = false)
+               bool cancellable = false, void* spool = nullptr)
 {
+    if (p.outOfCore && spool == nullptr)
+        fail(OOSG_INTERNAL, "factor: an out-of-core factorisation without its spool");
     const hipStream_t st = p.stream;
     oosg::DeviceMatrix a{m.dOffsets.as<int64_t>(), m.dRows.as<int32_t>(), m.dValues.as<double>(),
                          m.dPosition.as<int32_t>(), m.dColumn.as<int32_t>(),
@@ -1795,12 +1868,49 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
             check(hipEventCreate(&e), "hipEventCreate");
             p.launchEvents.push_back(e);
         }
+    // Out of core: the launches of level l write window l mod 2 of the panels and inverses; once a level's launches
+    // are queued, its windows are copied to the spool on the copy stream, and a window is written again only after
+    // that copy (oocFree).
+    T* const panelBase = panels;
+    T* const linvBase = linv;
+    int32_t current = -1;
+    const auto spoolLevel = [&](int32_t level)
+    {
+        const int w = level % 2;
+        check(hipEventRecord(p.oocReady[w], st), "factor: spool");
+        check(hipStreamWaitEvent(p.copyStream, p.oocReady[w], 0), "factor: spool");
+        const int64_t values = p.spoolPanelBase[level + 1] - p.spoolPanelBase[level];
+        const int64_t inverses = p.spoolLinvBase[level + 1] - p.spoolLinvBase[level];
+        if (values > 0)
+            check(hipMemcpyAsync(static_cast<T*>(spool) + p.spoolPanelBase[level], panelBase + w * p.windowPanels,
+                                 static_cast<size_t>(values) * sizeof(T), hipMemcpyDeviceToHost, p.copyStream),
+                  "factor: spool");
+        if (inverses > 0)
+            check(hipMemcpyAsync(static_cast<T*>(spool) + p.spoolPanelBase[p.levels] + p.spoolLinvBase[level],
+                                 linvBase + w * p.windowLinv, static_cast<size_t>(inverses) * sizeof(T),
+                                 hipMemcpyDeviceToHost, p.copyStream),
+                  "factor: spool");
+        check(hipEventRecord(p.oocFree[w], p.copyStream), "factor: spool");
+    };
     for (size_t li = 0; li < p.schedule.size(); ++li)
     {
         const Launch& l = p.schedule[li];
         if (cancellable && li > 0 && l.level != p.schedule[li - 1].level)
             pollCancellation(p);
-        const bool concurrent = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] && !p.profiler.enabled;
+        if (p.outOfCore && l.kind != LaunchKind::Sync && l.level != current)
+        {
+            if (l.level < current)
+                fail(OOSG_INTERNAL, "factor: the launches of an out-of-core plan go back a level");
+            if (current >= 0)
+                spoolLevel(current);
+            current = l.level;
+            // The window's previous level copied out first (no wait before its first record).
+            check(hipStreamWaitEvent(st, p.oocFree[current % 2], 0), "factor: spool");
+            panels = panelBase + (current % 2) * p.windowPanels;
+            linv = linvBase + (current % 2) * p.windowLinv;
+        }
+        const bool concurrent = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] &&
+                                !p.profiler.enabled && !p.outOfCore;
         if (p.aheadLevel > 0 && !aheadIssued && l.level >= p.aheadLevel)
         {
             // Everything launched so far is the factorisation of the levels below aheadLevel, on whichever streams
@@ -1879,6 +1989,12 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
             check(hipEventRecord(p.laneEvents[l.record], ls), "factor: stream order");
         if (timing)
             check(hipEventRecord(p.launchEvents[2 * li + 1], ls), "hipEventRecord");
+    }
+    if (p.outOfCore && current >= 0)
+    {
+        spoolLevel(current);
+        // The caller's later work on the plan's stream follows the last copy.
+        check(hipStreamWaitEvent(st, p.oocFree[current % 2], 0), "factor: spool");
     }
     if (timing)
     {
@@ -2565,8 +2681,11 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
 
 // p.solveX ← L⁻ᵀ·p.solveX: the backward sweep, levels from the top (rows outside the levels, a partial factor's
 // trailing ones, are read and kept).
-template <typename TP> void backwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k)
+template <typename TP>
+void backwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t l0 = 0, int32_t l1 = -1)
 {
+    if (l1 < 0)
+        l1 = p.levels;
     const hipStream_t st = p.stream;
     double* X = p.solveX.as<double>();
     const int32_t* tiny = p.dTinyList.as<int32_t>();
@@ -2585,7 +2704,7 @@ template <typename TP> void backwardSweep(oosg_plan& p, const TP* panels, const 
         launch();
         p.profiler.end(slot, st);
     };
-    for (int32_t l = p.levels - 1; l >= 0; --l)
+    for (int32_t l = l1 - 1; l >= l0; --l)
     {
         const oosg_plan::SweepWork& w = p.sweepWork[l];
         // The L_RFᵀ products of every supernode first (column blocks), then the wide supernodes' triangles (with
@@ -2640,6 +2759,62 @@ void runSweeps(oosg_plan& p, const TP* panels, const TP* linv, const double* dia
     backwardSweep<TP>(p, panels, linv, k);
 }
 
+// An out-of-core factor's solve: p.solveX ← factor⁻¹·p.solveX with each level's panels and inverses copied from the
+// spool into window l mod 2 on the copy stream, the next level's copy beside this level's sweep; after the forward
+// sweep the windows hold the two top levels, so the backward sweep starts without a copy.
+template <typename TP> void runSweepsOutOfCore(const oosg_factor& f, int k)
+{
+    oosg_plan& p = *f.plan;
+    const hipStream_t st = p.stream;
+    TP* panels = f.panels.as<TP>();
+    TP* linv = f.linv.as<TP>();
+    const TP* spool = static_cast<const TP*>(f.spool);
+    const int32_t L = p.levels;
+    const auto load = [&](int32_t level)
+    {
+        const int w = level % 2;
+        check(hipStreamWaitEvent(p.copyStream, p.oocFree[w], 0), "solve: spool");
+        const int64_t values = p.spoolPanelBase[level + 1] - p.spoolPanelBase[level];
+        const int64_t inverses = p.spoolLinvBase[level + 1] - p.spoolLinvBase[level];
+        if (values > 0)
+            check(hipMemcpyAsync(panels + w * p.windowPanels, spool + p.spoolPanelBase[level],
+                                 static_cast<size_t>(values) * sizeof(TP), hipMemcpyHostToDevice, p.copyStream),
+                  "solve: spool");
+        if (inverses > 0)
+            check(hipMemcpyAsync(linv + w * p.windowLinv, spool + p.spoolPanelBase[L] + p.spoolLinvBase[level],
+                                 static_cast<size_t>(inverses) * sizeof(TP), hipMemcpyHostToDevice, p.copyStream),
+                  "solve: spool");
+        check(hipEventRecord(p.oocReady[w], p.copyStream), "solve: spool");
+        f.traffic.read_bytes += static_cast<uint64_t>(values + inverses) * sizeof(TP);
+        f.traffic.read_requests += (values > 0 ? 1 : 0) + (inverses > 0 ? 1 : 0);
+    };
+    const auto sweep = [&](int32_t level, bool forward)
+    {
+        const int w = level % 2;
+        check(hipStreamWaitEvent(st, p.oocReady[w], 0), "solve: spool");
+        if (forward)
+            forwardSweep<TP>(p, panels + w * p.windowPanels, linv + w * p.windowLinv, k, level, level + 1, st);
+        else
+            backwardSweep<TP>(p, panels + w * p.windowPanels, linv + w * p.windowLinv, k, level, level + 1);
+        check(hipEventRecord(p.oocFree[w], st), "solve: spool");
+    };
+    if (L > 0)
+        load(0);
+    for (int32_t l = 0; l < L; ++l)
+    {
+        if (l + 1 < L)
+            load(l + 1);
+        sweep(l, true);
+    }
+    check(oosg::launchDiagonalSolve(p.solveX.as<double>(), f.dDiag.as<double>(), p.n, k, st), "solve: diagonal");
+    for (int32_t l = L - 1; l >= 0; --l)
+    {
+        if (l - 1 >= 0 && l - 1 <= L - 3)
+            load(l - 1); // into the window of level l + 1, swept already
+        sweep(l, false);
+    }
+}
+
 // p.solveX ← factor⁻¹·p.solveX (row-major n × k), as a graph replay unless the profiler is on; with fromLevel > 0
 // the forward sweep of the levels below it is done already (a combined factorisation and solve), and the rest runs
 // launch by launch.
@@ -2649,6 +2824,14 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
     if (p.trailing >= 0)
         fail(OOSG_INVALID_ARGUMENT, "solve: a partial factor solves by oosg_partial_condense_loads and "
                                     "oosg_partial_recover (its trailing equations are not eliminated)");
+    if (p.outOfCore)
+    {
+        if (f.precision == OOSG_FP32)
+            runSweepsOutOfCore<float>(f, k);
+        else
+            runSweepsOutOfCore<double>(f, k);
+        return;
+    }
     if (fromLevel > 0)
     {
         if (f.precision == OOSG_FP32)
@@ -3345,6 +3528,23 @@ oosg_status oosg_plan_create(const oosg_symbolic_desc* symbolic, int32_t device,
 oosg_status oosg_plan_create_partial(const oosg_symbolic_desc* symbolic, int32_t trailing_dof_count, int32_t device,
                                      uint64_t hbm_budget_bytes, oosg_plan** out)
 {
+    return createPlan(symbolic, trailing_dof_count, false, device, hbm_budget_bytes, out);
+}
+
+oosg_status oosg_plan_create_out_of_core(const oosg_symbolic_desc* symbolic, int32_t device, uint64_t hbm_budget_bytes,
+                                         oosg_plan** out)
+{
+    return createPlan(symbolic, 0, true, device, hbm_budget_bytes, out);
+}
+
+} // extern "C"
+
+namespace
+{
+
+oosg_status createPlan(const oosg_symbolic_desc* symbolic, int32_t trailing_dof_count, bool outOfCore, int32_t device,
+                       uint64_t hbm_budget_bytes, oosg_plan** out)
+{
     return guarded(
         [&]
         {
@@ -3355,7 +3555,13 @@ oosg_status oosg_plan_create_partial(const oosg_symbolic_desc* symbolic, int32_t
                 fail(OOSG_INVALID_ARGUMENT, "plan: a negative trailing count");
             auto p = std::make_unique<oosg_plan>();
             p->device = device;
-            p->trailingDofs = trailing_dof_count;            setDevice(p.get());
+            p->trailingDofs = trailing_dof_count;
+            p->outOfCore = outOfCore;
+            int count = 0;
+            check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+            if (device < 0 || device >= count)
+                fail(OOSG_INVALID_ARGUMENT, "plan: no device " + std::to_string(device));
+            setDevice(p.get());
             buildStructure(*p, *symbolic);
             buildSchedule(*p);
             buildTrailing(*p);
@@ -3392,12 +3598,27 @@ oosg_status oosg_plan_create_partial(const oosg_symbolic_desc* symbolic, int32_t
             check(oosg::scratchAddress(device, &p->scratch), "plan: the scratch array's address");
             check(hipEventCreate(&p->start), "hipEventCreate");
             check(hipEventCreate(&p->stop), "hipEventCreate");
+            if (p->outOfCore)
+            {
+                check(hipStreamCreateWithFlags(&p->copyStream, hipStreamNonBlocking), "hipStreamCreate");
+                for (int w = 0; w < 2; ++w)
+                {
+                    check(hipEventCreateWithFlags(&p->oocReady[w], hipEventDisableTiming), "hipEventCreate");
+                    check(hipEventCreateWithFlags(&p->oocFree[w], hipEventDisableTiming), "hipEventCreate");
+                }
+                // The windows' launches run one after the other on the plan's stream, launch by launch.
+                p->graphs = false;
+            }
             uploadPlan(*p);
             reserveSolveBuffers(*p);
             p->storagePool.push_back(takeStorage(*p));
             *out = p.release();
         });
 }
+
+} // namespace
+
+extern "C" {
 
 void oosg_plan_destroy(oosg_plan* plan)
 {
@@ -3840,6 +4061,9 @@ oosg_status oosg_plan_autotune(oosg_plan* plan, const oosg_matrix* matrix, int32
                 fail(OOSG_INVALID_ARGUMENT, "autotune: the matrix was made for another plan");
             if (precision != OOSG_FP64 && precision != OOSG_FP32)
                 fail(OOSG_INVALID_ARGUMENT, "autotune: unknown precision");
+            if (plan->outOfCore)
+                fail(OOSG_INVALID_ARGUMENT, "autotune: an out-of-core plan launches one level at a time on one "
+                                            "stream; there is nothing to tune");
             setDevice(plan);
             oosg_plan& p = *plan;
             // A factor's storage of the plan's (given back at the end), and the plan's pivot status.
@@ -4505,6 +4729,8 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 fail(OOSG_INVALID_ARGUMENT,
                      "PartialFactor: the interior of a unit is factorised in positive definite mode (a delayed pivot "
                      "would move an interior equation into the boundary); pass PivotingMode::PositiveDefinite");
+            if (plan->outOfCore && opts.pivoting != OOSG_POSITIVE_DEFINITE)
+                fail(OOSG_INVALID_ARGUMENT, "factorise: an out-of-core plan factorises in positive definite mode");
             if (plan->trailing >= 0 && opts.precision != OOSG_FP64)
                 fail(OOSG_INVALID_ARGUMENT, "PartialFactor: the interior factor and K* are computed in FP64; pass "
                                             "Precision.FP64");
@@ -4546,6 +4772,16 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 f->dDiag = std::move(storage.diag);
                 f->scale = std::move(storage.scale);
             }
+            if (plan->outOfCore)
+            {
+                const size_t values = static_cast<size_t>(plan->spoolPanelBase[plan->levels] +
+                                                          plan->spoolLinvBase[plan->levels]);
+                f->spoolBytes = values * valueBytes(opts.precision);
+                check(hipHostMalloc(&f->spool, std::max<size_t>(f->spoolBytes, 8), hipHostMallocDefault),
+                      "factor: the spool (pinned host memory)");
+                f->traffic.spooled_bytes = f->spoolBytes;
+                f->traffic.written_bytes = f->spoolBytes;
+            }
             DeviceBuffer& diag = f->dDiag;
             DeviceBuffer& zero = plan->pivotStatus;
             const hipStream_t st = plan->stream;
@@ -4555,12 +4791,12 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
                 if (opts.precision == OOSG_FP32)
                     runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
-                                     zero.as<int32_t>(), true, cancellable);
+                                     zero.as<int32_t>(), true, cancellable, f->spool);
                 else
                     runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(),
-                                      diag.as<double>(), zero.as<int32_t>(), false, cancellable);
+                                      diag.as<double>(), zero.as<int32_t>(), false, cancellable, f->spool);
             };
-            if (plan->graphs && !plan->profiler.enabled && !plan->eager[opts.precision])
+            if (plan->graphs && !plan->profiler.enabled && !plan->eager[opts.precision] && !plan->outOfCore)
                 plan->factorGraph[opts.precision].run(
                     st,
                     {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,
@@ -4789,6 +5025,17 @@ oosg_status oosg_partial_recover(const oosg_factor* f, double* loads_to_solution
         });
 }
 
+oosg_status oosg_factor_out_of_core_traffic(const oosg_factor* f, oosg_out_of_core_traffic* traffic)
+{
+    return guarded(
+        [&]
+        {
+            if (f == nullptr || traffic == nullptr)
+                fail(OOSG_INVALID_ARGUMENT, "out of core: null argument");
+            *traffic = f->traffic;
+        });
+}
+
 oosg_status oosg_factor_issues(const oosg_factor* f, oosg_pivot_issue* issues, int64_t capacity, int64_t* recorded)
 {
     return guarded(
@@ -4856,7 +5103,16 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
             if (values == nullptr)
                 return;
             setDevice(&p);
-            if (f->precision == OOSG_FP32)
+            if (p.outOfCore)
+            {
+                // From the spool: the level's panels, the supernode's within them.
+                const int64_t at = p.spoolPanelBase[p.levelOf[s]] + p.panelOff[s];
+                if (f->precision == OOSG_FP32)
+                    std::copy_n(static_cast<const float*>(f->spool) + at, count, values);
+                else
+                    std::copy_n(static_cast<const double*>(f->spool) + at, count, values);
+            }
+            else if (f->precision == OOSG_FP32)
             {
                 std::vector<float> tmp(count);
                 check(hipMemcpy(tmp.data(), f->panels.as<float>() + p.panelOff[s], count * sizeof(float),
@@ -4868,7 +5124,8 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
                 check(hipMemcpy(values, f->panels.as<double>() + p.panelOff[s], count * sizeof(double),
                                 hipMemcpyDeviceToHost),
                       "panel: copy");
-            // The device never reads or writes the values above the diagonal: they are zeros here.
+            // The device never reads or writes the values above the diagonal (their memory may hold other values, an
+            // out-of-core window another level's): they are zeros here, as in the reference's panels.
             const int64_t n = c + r;
             for (int64_t j = 1; j < c; ++j)
                 std::fill(values + j * n, values + j * n + j, 0.0);
