@@ -9,6 +9,10 @@
         Per kernel, the average FETCH_SIZE and WRITE_SIZE per dispatch (KiB, as the counters read), each scaled by
         copy bytes / counter reading of the calibration copy, and their sum: the HBM bytes per launch.
 
+    python -m o_o_solver_amd.profiling mfma COUNTERS_CSV OUT [--command CMD] [--stats STATS_SUMMARY]
+        Per kernel, the MFMA flops per dispatch from SQ_INSTS_VALU_MFMA_MOPS_F64 and _F32 (units of 512 flops), and
+        with a statistics summary their rate over the kernel's average duration and its fraction of the MFMA peak.
+
 read_traffic(path, kernel) returns the calibrated bytes per launch of one kernel from a traffic summary.
 """
 
@@ -93,6 +97,52 @@ def traffic(fetch, write, copy_fetch, copy_write, out, commands):
         fo.write("\n".join(lines) + "\n")
 
 
+MFMA_COUNTERS = ["SQ_INSTS_VALU_MFMA_MOPS_F64", "SQ_INSTS_VALU_MFMA_MOPS_F32"]
+# MI355X dense matrix-core peaks (FLOP/s).
+MFMA_PEAK = {"f64": 78.6e12, "f32": 157.3e12}
+
+
+def read_stats(path):
+    """{kernel: average duration in seconds} of a kernel statistics summary (stats())."""
+    out = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            parts = line.split()
+            if len(parts) == 7:
+                out[parts[0]] = float(parts[3]) * 1e-6
+    return out
+
+
+def mfma(csv_path, out, command, stats_path=None):
+    per = {c: counter_per_kernel(csv_path, c) for c in MFMA_COUNTERS}
+    durations = read_stats(stats_path) if stats_path else {}
+    kernels = sorted(set().union(*[set(v) for v in per.values()]))
+    lines = ["# Matrix-core work per dispatch (rocprofv3 --pmc SQ_INSTS_VALU_MFMA_MOPS_F64 / _F32: MFMA operations in",
+             "# units of 512 flops, all padding included), and its rate over the kernel's average duration in the",
+             f"# statistics summary {os.path.basename(stats_path) if stats_path else '(none)'}, against the dense MFMA",
+             "# peak of its precision (FP64 78.6, FP32 157.3 TFLOP/s).",
+             f"# command: {command}", "#",
+             f"# {'kernel':<40} {'dispatches':>10} {'mfma flops':>12} {'avg us':>9} {'TFLOP/s':>8} {'of peak':>8}"]
+    for k in kernels:
+        f64 = per["SQ_INSTS_VALU_MFMA_MOPS_F64"].get(k, [])
+        f32 = per["SQ_INSTS_VALU_MFMA_MOPS_F32"].get(k, [])
+        count = max(len(f64), len(f32))
+        a64 = sum(f64) / len(f64) if f64 else 0.0
+        a32 = sum(f32) / len(f32) if f32 else 0.0
+        flops = 512.0 * (a64 + a32)
+        if flops == 0.0:
+            continue
+        peak = MFMA_PEAK["f32" if a32 > a64 else "f64"]
+        t = durations.get(k)
+        rate = flops / t if t else float("nan")
+        lines.append(f"  {k:<40} {count:>10} {flops:>12.4g} {t * 1e6 if t else float('nan'):>9.2f} "
+                     f"{rate / 1e12:>8.2f} {rate / peak:>8.3f}")
+    with open(out, "w") as fo:
+        fo.write("\n".join(lines) + "\n")
+
+
 def read_traffic(path, kernel):
     """The calibrated HBM bytes per launch of `kernel` in a traffic summary, or None."""
     if not os.path.exists(path):
@@ -131,11 +181,18 @@ def main(argv=None):
         t.add_argument("--" + name, required=True)
     t.add_argument("--command", action="append", default=[])
     t.add_argument("out")
+    m = sub.add_parser("mfma")
+    m.add_argument("csv")
+    m.add_argument("out")
+    m.add_argument("--command", default="")
+    m.add_argument("--stats", default=None)
     args = ap.parse_args(argv)
     if args.what == "copy":
         copy()
     elif args.what == "stats":
         stats(args.csv, args.out, args.command)
+    elif args.what == "mfma":
+        mfma(args.csv, args.out, args.command, args.stats)
     else:
         traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.out, args.command)
 
