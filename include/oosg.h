@@ -78,7 +78,9 @@ typedef enum oosg_pivoting {
 
 typedef enum oosg_precision {
     OOSG_FP64 = 0, /* every front in double precision */
-    OOSG_FP32 = 1  /* fronts in single precision on the diagonally scaled matrix; solves refine in double */
+    OOSG_FP32 = 1, /* fronts in single precision on the diagonally scaled matrix; solves refine in double */
+    OOSG_MIXED = 2 /* the fronts of the lowest fp32_levels levels (the subtrees below a cut) in single precision, the
+                      others in double, all on the diagonally scaled matrix; solves refine in double */
 } oosg_precision;
 
 typedef struct oosg_factor_options {
@@ -90,6 +92,9 @@ typedef struct oosg_factor_options {
      * matrix diagonal of that equation) or null for the matrix diagonal everywhere (FactorOptions::referenceDiagonal,
      * SupernodalFactor.hpp:75; ratioTestDiagonal, PivotDiagnostics.hpp:108-120). */
     const double *reference_diagonal;
+    /* OOSG_MIXED: the levels (heights in the supernode tree, from the leaves) factorised in FP32; clamped to the
+     * tree's levels. */
+    int32_t fp32_levels;
 } oosg_factor_options;
 
 /* The inertia, pivot counts and ratio test of a factor (SupernodalFactor::inertia, pivotDiagnostics,
@@ -106,6 +111,7 @@ typedef struct oosg_factor_info {
     int32_t precision;
     double factor_seconds;        /* device time of the numeric factorisation */
     double flops;                 /* SymbolicFactor::flopCount() of the structure */
+    int32_t fp32_levels;          /* OOSG_MIXED: the levels in FP32 */
 } oosg_factor_info;
 
 /* What a refined solve did. */

@@ -139,8 +139,15 @@ class PivotingMode(enum.IntEnum):
 class Precision(enum.IntEnum):
     FP64 = 0
     FP32 = 1
-    # The learned selector chooses FP32 or FP64 per matrix (FactorOptions.selector, or the committed weights).
+    # The learned selector chooses the precision of the fronts (FactorOptions.selector, or the committed weights).
     LEARNED = 2
+    # FP32 fronts in the lowest FactorOptions.fp32_levels levels of the supernode tree (the subtrees below a cut),
+    # FP64 above, on the diagonally scaled matrix; refined by conjugate gradients as FP32 (OOSG_MIXED).
+    MIXED = 3
+
+
+# The C ABI's oosg_precision of each precision a factor is made in.
+_C_PRECISION = {Precision.FP64: 0, Precision.FP32: 1, Precision.MIXED: 2}
 
 
 @dataclass
@@ -169,6 +176,8 @@ class FactorOptions:
     equation_namer: object = None
     # Polled between the parts of a call (FactorOptions::cancellation): a callable returning True to stop, or None.
     cancellation: object = None
+    # Precision.MIXED: the levels (heights in the supernode tree, from the leaves) factorised in FP32.
+    fp32_levels: int = 0
 
 
 @dataclass(frozen=True)
@@ -610,8 +619,9 @@ class SupernodalFactor:
 
     @staticmethod
     def _options_c(options, precision, reference):
-        return _native.FactorOptionsC(int(options.pivoting), int(precision), float(options.pivot_policy.max_ratio),
-                                      int(options.pivot_policy.bailout), _ptr(reference))
+        return _native.FactorOptionsC(int(options.pivoting), _C_PRECISION[Precision(precision)],
+                                      float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
+                                      _ptr(reference), int(options.fp32_levels))
 
     @classmethod
     def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None, plan=None):
@@ -720,7 +730,12 @@ class SupernodalFactor:
 
     @property
     def precision(self):
-        return Precision(int(self._info.precision))
+        return {0: Precision.FP64, 1: Precision.FP32, 2: Precision.MIXED}[int(self._info.precision)]
+
+    @property
+    def fp32_levels(self):
+        """Precision.MIXED: the levels factorised in FP32."""
+        return int(self._info.fp32_levels)
 
     def panel(self, s):
         """(values m × e, rows) of supernode s, in the reference's FactorPanel layout: m = c + r rows, or c + d + r when

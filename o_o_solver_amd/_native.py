@@ -38,14 +38,14 @@ class CscDesc(ctypes.Structure):
 
 class FactorOptionsC(ctypes.Structure):
     _fields_ = [("pivoting", c_int32), ("precision", c_int32), ("max_ratio", c_double), ("bailout", c_int32),
-                ("reference_diagonal", c_void_p)]
+                ("reference_diagonal", c_void_p), ("fp32_levels", c_int32)]
 
 
 class FactorInfoC(ctypes.Structure):
     _fields_ = [("positive", c_int64), ("negative", c_int64), ("zero", c_int64), ("delayed_pivots", c_int64),
                 ("two_by_two_pivots", c_int64), ("issue_count", c_int64), ("first_issue_equation", c_int32),
                 ("first_issue_negative", c_int32), ("largest_ratio", c_double), ("largest_ratio_equation", c_int32),
-                ("precision", c_int32), ("factor_seconds", c_double), ("flops", c_double)]
+                ("precision", c_int32), ("factor_seconds", c_double), ("flops", c_double), ("fp32_levels", c_int32)]
 
 
 class SolveReportC(ctypes.Structure):

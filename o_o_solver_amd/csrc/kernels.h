@@ -281,6 +281,17 @@ hipError_t launchBunchKaufman(const PivotFront* fronts, PivotState* states, cons
 hipError_t launchPivotMatrix(const DeviceMatrix& a, const PivotFront* fronts, int count, hipStream_t stream);
 hipError_t launchPivotExtendAdd(const ExtendItem* items, int count, hipStream_t stream);
 hipError_t launchCopyRanges(const CopyRange* ranges, int count, hipStream_t stream);
+// A mixed factor's boundary: update matrices written in FP32 by the levels below the cut, read in FP64 above it. Each
+// range (at most 65536 values) is staged through tmp: phase 0 copies src to tmp for every range, phase 1 widens tmp
+// into dst (dst may overlap the sources of other ranges, never a range's own tmp).
+struct WidenRange
+{
+    const float* src;
+    float* tmp;
+    double* dst;
+    int64_t count;
+};
+hipError_t launchWidenRanges(const WidenRange* ranges, int count, hipStream_t stream);
 hipError_t launchUnitBlockInverses(const DevicePlan& p, const double* panels, double* linv, const int2* items,
                                    int count, hipStream_t stream);
 hipError_t launchBlockDiagonalSolve(double* X, const double* d, const double* sub, int64_t n, int k,

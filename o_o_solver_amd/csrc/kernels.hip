@@ -2980,6 +2980,27 @@ hipError_t launchSortColumns(const int64_t* offsets, int32_t n, const int32_t* r
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void widenRangesKernel(const WidenRange* ranges, int phase)
+{
+    const WidenRange r = ranges[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < r.count; i += 256)
+    {
+        if (phase == 0)
+            r.tmp[i] = r.src[i];
+        else
+            r.dst[i] = double(r.tmp[i]);
+    }
+}
+
+hipError_t launchWidenRanges(const WidenRange* ranges, int count, hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(widenRangesKernel, dim3(count), dim3(256), 0, stream, ranges, 0);
+    hipLaunchKernelGGL(widenRangesKernel, dim3(count), dim3(256), 0, stream, ranges, 1);
+    return hipGetLastError();
+}
+
 hipError_t launchCombineValues(int32_t n, const int64_t* offsets, const int32_t* rows, double alpha, const double* a,
                                double beta, const double* b, const double* scale, double* out, double* diagonal,
                                double* fp32Scale, hipStream_t stream)

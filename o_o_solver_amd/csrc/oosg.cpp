@@ -400,7 +400,6 @@ struct oosg_plan
     // needs, oocFree[w] once its last user is done with it.
     bool outOfCore = false;
     int64_t windowPanels = 0, windowLinv = 0;
-    std::vector<int32_t> levelOf;
     std::vector<int64_t> spoolPanelBase, spoolLinvBase;
     hipStream_t copyStream = nullptr;
     hipEvent_t oocReady[2] = {nullptr, nullptr}, oocFree[2] = {nullptr, nullptr};
@@ -408,6 +407,7 @@ struct oosg_plan
     std::vector<oosg::AssemblyItem> trailingItems;
     std::vector<int32_t> trailingPullStart, trailingPulls;
     std::vector<int32_t> first, ncol, nrow, parent, rows, childStart, children, rel, levelStart, levelList;
+    std::vector<int32_t> levelOf; // the level (height) of each supernode (-1: the trailing one)
     // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others; the tiny ones' pull
     // table (oosg::DevicePlan::pulls) and its width.
     std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, pullBase, pulls;
@@ -462,6 +462,8 @@ struct oosg_plan
         dLinvOff;
     DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls;
     DeviceBuffer dTrailingBlocks, dTrailingItems, dTrailingPullStart, dTrailingPulls;
+    // A mixed factor's boundary update matrices staged for their widening, and the ranges of that copy.
+    DeviceBuffer widenTemp, widenRanges;
     DeviceBuffer dSmallList, dStepFronts, dBlocks, dItems, dTiles, dBigTiles, dItemStarts, dExtendItems, dLevelList, dForwardTiles, dBackwardBlocks,
         dForwardSubtractions, dBackwardSubtractions,
         dForwardTriangles, dBackwardTriangles, dForwardTriangleSums, dBackwardTriangleSums, dInverseItems;
@@ -593,6 +595,10 @@ struct oosg_factor
     DeviceBuffer panels;
     DeviceBuffer linv;  // the inverses of the diagonal blocks of PANEL_BLOCK columns (L_JJ⁻¹)
     DeviceBuffer dDiag; // D as the solves divide by it (FP32: of the scaled matrix)
+    // A mixed factor (OOSG_MIXED): the levels below fp32Levels in FP32 (these buffers, panelOff / linvOff in floats),
+    // the others in FP64 (panels, linv), all on the scaled matrix.
+    DeviceBuffer panels32, linv32;
+    int32_t fp32Levels = 0;
     DeviceBuffer scale; // FP32: the equations' scales of the factorised matrix
     // The pivots on the host (FP32: of the matrix, d_j / s_j²): filled on first use (pivoted factors: at once).
     mutable std::vector<double> diag;
@@ -622,6 +628,13 @@ struct oosg_factor
 
 namespace
 {
+
+// Whether a factor is of the diagonally scaled matrix and its solves refine by conjugate gradients (FP32 fronts, all
+// of them or below a cut).
+bool scaledPrecision(int precision)
+{
+    return precision != OOSG_FP64;
+}
 
 size_t valueBytes(int precision)
 {
@@ -997,13 +1010,11 @@ void windowLayout(oosg_plan& p, const std::vector<int32_t>& height)
         linvValues[k] = p.linvOff[k + 1] - p.linvOff[k];
     }
     std::vector<int64_t> panelFill(p.levels, 0), linvFill(p.levels, 0);
-    p.levelOf.assign(S, -1);
     for (int32_t k = 0; k < S; ++k)
     {
         if (k == p.trailing)
             continue;
         const int32_t l = height[k];
-        p.levelOf[k] = l;
         p.panelOff[k] = panelFill[l];
         p.linvOff[k] = linvFill[l];
         panelFill[l] += panelValues[k];
@@ -1063,6 +1074,7 @@ void deriveLevels(oosg_plan& p)
     for (int32_t l = 0; l < p.levels; ++l)
         p.levelStart[l + 1] += p.levelStart[l];
     p.levelList.resize(p.levelStart[p.levels]);
+    p.levelOf.assign(S, -1);
     p.levelPanelValues.assign(p.levels, 0.0);
     p.sweepWork.assign(p.levels, oosg_plan::SweepWork{});
     {
@@ -1071,6 +1083,7 @@ void deriveLevels(oosg_plan& p)
         {
             if (k == p.trailing)
                 continue;
+            p.levelOf[k] = height[k];
             p.levelList[p.levelStart[height[k]] + fill[height[k]]++] = k;
             p.levelPanelValues[height[k]] += static_cast<double>(p.panelOff[k + 1] - p.panelOff[k]);
             oosg_plan::SweepWork& w = p.sweepWork[height[k]];
@@ -1844,8 +1857,11 @@ void ensureSolveBuffers(oosg_plan& p);
 // cancellable: launched one launch at a time (not captured), so the cancellation is polled between levels.
 template <typename T>
 void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* diagOut, int32_t* zeroAt, bool scaled,
-               bool cancellable = false, void* spool = nullptr)
+               bool cancellable = false, void* spool = nullptr, int32_t levelFrom = 0,
+               int32_t levelTo = std::numeric_limits<int32_t>::max())
 {
+    // A part of the levels (a mixed factor's) runs on the plan's stream alone, launch by launch.
+    const bool part = levelFrom > 0 || levelTo < p.levels;
     if (p.outOfCore && spool == nullptr)
         fail(OOSG_INTERNAL, "factor: an out-of-core factorisation without its spool");
     const hipStream_t st = p.stream;
@@ -1895,6 +1911,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     for (size_t li = 0; li < p.schedule.size(); ++li)
     {
         const Launch& l = p.schedule[li];
+        if (part && (l.kind == LaunchKind::Sync || l.level < levelFrom || l.level >= levelTo))
+            continue;
         if (cancellable && li > 0 && l.level != p.schedule[li - 1].level)
             pollCancellation(p);
         if (p.outOfCore && l.kind != LaunchKind::Sync && l.level != current)
@@ -1910,7 +1928,7 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
             linv = linvBase + (current % 2) * p.windowLinv;
         }
         const bool concurrent = p.concurrent[sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64] &&
-                                !p.profiler.enabled && !p.outOfCore;
+                                !p.profiler.enabled && !p.outOfCore && !part;
         if (p.aheadLevel > 0 && !aheadIssued && l.level >= p.aheadLevel)
         {
             // Everything launched so far is the factorisation of the levels below aheadLevel, on whichever streams
@@ -2047,7 +2065,7 @@ const std::vector<double>& hostDiagonal(const oosg_factor& f)
     const size_t n = static_cast<size_t>(p.n);
     std::vector<double> d(n);
     check(hipMemcpy(d.data(), f.dDiag.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "factor: pivots");
-    if (f.precision == OOSG_FP32)
+    if (scaledPrecision(f.precision))
     {
         std::vector<double> scale(n);
         check(hipMemcpy(scale.data(), f.scale.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "factor: scales");
@@ -2088,7 +2106,7 @@ void finish(const oosg_plan& p, const oosg_matrix& m, oosg_factor& f, const std:
             failedBelow[p.parent[s]] = 1;
     }
     // An FP32 factor's failures do not show the matrix singular: single precision may cause them.
-    const bool fp32 = f.precision == OOSG_FP32;
+    const bool fp32 = scaledPrecision(f.precision);
     const std::string fp32Note = "\nthe factor was computed in FP32: the matrix is not shown singular; factorise it in "
                                  "FP64";
     if (zeroCount > 0)
@@ -2841,6 +2859,29 @@ void sweeps(const oosg_factor& f, int k, int32_t fromLevel = 0)
                               fromLevel);
         return;
     }
+    if (f.precision == OOSG_MIXED)
+    {
+        // FP32 levels below the cut, FP64 above: forward up, D⁻¹, backward down.
+        const int32_t cut = f.fp32Levels;
+        const auto body = [&]
+        {
+            forwardSweep<float>(p, f.panels32.as<float>(), f.linv32.as<float>(), k, 0, cut, p.stream);
+            forwardSweep<double>(p, f.panels.as<double>(), f.linv.as<double>(), k, cut, p.levels, p.stream);
+            check(oosg::launchDiagonalSolve(p.solveX.as<double>(), f.dDiag.as<double>(), p.n, k, p.stream),
+                  "solve: diagonal");
+            backwardSweep<double>(p, f.panels.as<double>(), f.linv.as<double>(), k, cut, p.levels);
+            backwardSweep<float>(p, f.panels32.as<float>(), f.linv32.as<float>(), k, 0, cut);
+        };
+        if (!p.graphs || p.profiler.enabled)
+            body();
+        else
+            p.sweepGraph.run(p.stream,
+                             {f.panels.ptr, f.linv.ptr, f.panels32.ptr, f.linv32.ptr, f.dDiag.ptr, p.solveX.ptr,
+                              p.solveU.ptr, p.solvePartials.ptr,
+                              reinterpret_cast<const void*>(static_cast<intptr_t>(k * 4096 + 2048 + cut))},
+                             body);
+        return;
+    }
     if (f.pivoted)
     {
         // In pivot order: X permuted into the structure's own X, swept, and permuted back.
@@ -2882,7 +2923,7 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     (void)width;
     // Sized for 16 columns whatever the call: the buffers (and the graphs replaying the sweeps on them) stay put.
     ensureSolveBuffers(p);
-    const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
+    const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
     for (int64_t c0 = 0; c0 < nrhs; c0 += 16)
     {
         if (c0 > 0)
@@ -2986,7 +3027,7 @@ void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k,
     oosg_plan& p = *f.plan;
     const int64_t n = p.n;
     ensureSolveBuffers(p);
-    const double* scale = f.precision == OOSG_FP32 ? f.scale.as<double>() : nullptr;
+    const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
     if (fromLevel == 0)
         check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
     sweeps(f, k, fromLevel);
@@ -4708,6 +4749,48 @@ oosg_status oosg_factorise(oosg_plan* plan, const oosg_matrix* matrix, const oos
 namespace
 {
 
+// A mixed factor's cut: the update matrices of the supernodes below it whose parents are above it, written in FP32
+// and read in FP64, each widened in place through a staging buffer (the plan's, grown as needed): their ranges
+// uploaded to the plan's widenRanges (whose bytes give their count).
+void prepareWidening(oosg_plan& p, int32_t cut)
+{
+    std::vector<oosg::WidenRange> ranges;
+    int64_t staged = 0;
+    for (int32_t s = 0; s < p.S; ++s)
+    {
+        const int32_t q = p.parent[s];
+        if (q < 0 || p.levelOf[s] >= cut || p.levelOf[q] < cut)
+            continue;
+        const int64_t values = static_cast<int64_t>(p.nrow[s]) * p.nrow[s];
+        for (int64_t a = 0; a < values; a += 65536)
+        {
+            oosg::WidenRange r{};
+            r.src = reinterpret_cast<const float*>(static_cast<uintptr_t>(p.updateOff[s] + a));     // offsets for now
+            r.tmp = reinterpret_cast<float*>(static_cast<uintptr_t>(staged));
+            r.dst = reinterpret_cast<double*>(static_cast<uintptr_t>(p.updateOff[s] + a));
+            r.count = std::min<int64_t>(65536, values - a);
+            staged += r.count;
+            ranges.push_back(r);
+        }
+    }
+    ensureCounted(p, p.widenTemp, static_cast<size_t>(std::max<int64_t>(staged, 1)) * sizeof(float),
+                  "factor: the cut's update matrices");
+    float* arena32 = p.updates.as<float>();
+    double* arena64 = p.updates.as<double>();
+    float* temp = p.widenTemp.as<float>();
+    for (oosg::WidenRange& r : ranges)
+    {
+        r.src = arena32 + reinterpret_cast<uintptr_t>(r.src);
+        r.tmp = temp + reinterpret_cast<uintptr_t>(r.tmp);
+        r.dst = arena64 + reinterpret_cast<uintptr_t>(r.dst);
+    }
+    p.widenRanges.reset();
+    if (!ranges.empty())
+        p.widenRanges = upload(ranges, p.stream, "factor: the cut's update matrices");
+    else
+        p.widenRanges.bytes = 0;
+}
+
 // The numeric factorisation behind oosg_factorise (and the combined factorisation and solve).
 void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor_options* options, oosg_factor** out)
 {
@@ -4729,6 +4812,9 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 fail(OOSG_INVALID_ARGUMENT,
                      "PartialFactor: the interior of a unit is factorised in positive definite mode (a delayed pivot "
                      "would move an interior equation into the boundary); pass PivotingMode::PositiveDefinite");
+            if (opts.precision == OOSG_MIXED && (plan->outOfCore || plan->trailing >= 0 ||
+                                                 opts.pivoting != OOSG_POSITIVE_DEFINITE))
+                fail(OOSG_INVALID_ARGUMENT, "factorise: a mixed factor is positive definite, in core and whole");
             if (plan->outOfCore && opts.pivoting != OOSG_POSITIVE_DEFINITE)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: an out-of-core plan factorises in positive definite mode");
             if (plan->trailing >= 0 && opts.precision != OOSG_FP64)
@@ -4758,7 +4844,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 *out = f.release();
                 return;
             }
-            if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32)
+            if (opts.precision != OOSG_FP64 && opts.precision != OOSG_FP32 && opts.precision != OOSG_MIXED)
                 fail(OOSG_INVALID_ARGUMENT, "factorise: unknown precision");
             setDevice(plan);
             pollCancellation(*plan);
@@ -4782,6 +4868,18 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 f->traffic.spooled_bytes = f->spoolBytes;
                 f->traffic.written_bytes = f->spoolBytes;
             }
+            const bool mixed = opts.precision == OOSG_MIXED;
+            if (mixed)
+            {
+                f->fp32Levels = std::clamp(opts.fp32_levels, 0, plan->levels);
+                allocateCounted(*plan, f->panels32,
+                                static_cast<size_t>(std::max<int64_t>(plan->panelOff[plan->S], 1)) * sizeof(float),
+                                "factor: FP32 panels (mixed)");
+                allocateCounted(*plan, f->linv32,
+                                static_cast<size_t>(std::max<int64_t>(plan->linvOff[plan->S], 1)) * sizeof(float),
+                                "factor: FP32 inverses (mixed)");
+                prepareWidening(*plan, f->fp32Levels);
+            }
             DeviceBuffer& diag = f->dDiag;
             DeviceBuffer& zero = plan->pivotStatus;
             const hipStream_t st = plan->stream;
@@ -4789,14 +4887,26 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             const auto body = [&](bool cancellable)
             {
                 check(oosg::launchFillInt32(zero.as<int32_t>(), plan->S, INT32_MAX, st), "factor: status");
-                if (opts.precision == OOSG_FP32)
+                if (mixed)
+                {
+                    const int32_t cut = f->fp32Levels;
+                    runFactor<float>(*plan, *matrix, f->panels32.as<float>(), f->linv32.as<float>(),
+                                     diag.as<double>(), zero.as<int32_t>(), true, cancellable, nullptr, 0, cut);
+                    check(oosg::launchWidenRanges(plan->widenRanges.as<oosg::WidenRange>(),
+                                                  static_cast<int>(plan->widenRanges.bytes / sizeof(oosg::WidenRange)),
+                                                  st),
+                          "factor: the cut's update matrices");
+                    runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(), diag.as<double>(),
+                                      zero.as<int32_t>(), true, cancellable, nullptr, cut);
+                }
+                else if (opts.precision == OOSG_FP32)
                     runFactor<float>(*plan, *matrix, f->panels.as<float>(), f->linv.as<float>(), diag.as<double>(),
                                      zero.as<int32_t>(), true, cancellable, f->spool);
                 else
                     runFactor<double>(*plan, *matrix, f->panels.as<double>(), f->linv.as<double>(),
                                       diag.as<double>(), zero.as<int32_t>(), false, cancellable, f->spool);
             };
-            if (plan->graphs && !plan->profiler.enabled && !plan->eager[opts.precision] && !plan->outOfCore)
+            if (plan->graphs && !plan->profiler.enabled && !mixed && !plan->eager[opts.precision] && !plan->outOfCore)
                 plan->factorGraph[opts.precision].run(
                     st,
                     {f->panels.ptr, f->linv.ptr, diag.ptr, zero.ptr, matrix->dValues.ptr, matrix->dPosition.ptr,
@@ -4821,7 +4931,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             check(hipEventRecord(plan->stop, st), "hipEventRecord");
             if (plan->aheadLevel > 0)
                 check(hipStreamWaitEvent(st, plan->aheadEvents[LANES], 0), "factor: sweep ahead"); // its join
-            if (opts.precision == OOSG_FP32)
+            if (scaledPrecision(opts.precision))
                 // The equations' scales the pivots are divided by, kept with the factor.
                 check(hipMemcpyAsync(f->scale.ptr, matrix->dScale.ptr, static_cast<size_t>(plan->n) * sizeof(double),
                                      hipMemcpyDeviceToDevice, st),
@@ -4835,7 +4945,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 reference = upload(referenceHost, st, "factor: reference diagonal");
             }
             check(oosg::launchPivotStatistics(diag.as<double>(),
-                                              opts.precision == OOSG_FP32 ? f->scale.as<double>() : nullptr,
+                                              scaledPrecision(opts.precision) ? f->scale.as<double>() : nullptr,
                                               matrix->dDiagonal.as<double>(),
                                               reference.ptr != nullptr ? reference.as<double>() : nullptr,
                                               plan->eliminated,
@@ -4856,6 +4966,7 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             f->info.factor_seconds = ms * 1e-3;
             f->info.flops = plan->flops;
             f->info.precision = opts.precision;
+            f->info.fp32_levels = f->fp32Levels;
             // FP32: the pivots of S·A·S divided by s_j² are those of A (same signs: the inertia is kept).
             finish(*plan, *matrix, *f, zeroAt, stats, opts);
             holdPlan(plan);
@@ -5112,11 +5223,12 @@ oosg_status oosg_factor_panel(const oosg_factor* f, int32_t s, double* values, i
                 else
                     std::copy_n(static_cast<const double*>(f->spool) + at, count, values);
             }
-            else if (f->precision == OOSG_FP32)
+            else if (f->precision == OOSG_FP32 ||
+                     (f->precision == OOSG_MIXED && p.levelOf[s] >= 0 && p.levelOf[s] < f->fp32Levels))
             {
                 std::vector<float> tmp(count);
-                check(hipMemcpy(tmp.data(), f->panels.as<float>() + p.panelOff[s], count * sizeof(float),
-                                hipMemcpyDeviceToHost),
+                const float* from = (f->precision == OOSG_MIXED ? f->panels32 : f->panels).as<float>();
+                check(hipMemcpy(tmp.data(), from + p.panelOff[s], count * sizeof(float), hipMemcpyDeviceToHost),
                       "panel: copy");
                 std::copy(tmp.begin(), tmp.end(), values);
             }
@@ -5271,7 +5383,7 @@ void refineTiles(const oosg_factor& f, const oosg_matrix& m, const double* b, do
             pollCancellation(p);
         const int k = static_cast<int>(std::min<int64_t>(16, nrhs - c0));
         oosg_solve_report tile{};
-        if (f.precision == OOSG_FP32)
+        if (scaledPrecision(f.precision))
             solveBlockPcg(f, m, b + c0 * ld, x + c0 * ld, ld, k, target, std::max(1, maxSteps), tile,
                           c0 == 0 ? from : 0);
         else
