@@ -196,7 +196,17 @@ hipError_t launchUpdateTiles(const DevicePlan& p, T* panels, T* updates, T* linv
 template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
                          const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
-                         hipStream_t stream);
+                         hipStream_t stream, bool gathered = false);
+// The gathers of a level's other supernodes, by chunks of GATHER_CHUNK front positions: X(cols) += and U(rows) = the
+// children's update rows, before launchForward(gathered). A chunk: its first position q0 in the front of its supernode
+// (order c + r, c columns from `first`, update rows from `rowStart`), the front's first position in the gather lists.
+constexpr int GATHER_CHUNK = 16;
+struct GatherChunk
+{
+    int32_t q0, order, columns, first, rowStart, positions, pad0, pad1;
+};
+hipError_t launchGather(const GatherChunk* chunks, int chunkCount, const int32_t* gatherStart,
+                        const int32_t* gatherRows, double* X, double* U, int k, hipStream_t stream);
 // One level of the backward sweep: the column blocks (supernode, block) of its big supernodes, then all of `list`.
 // The sweeps of tiny supernodes (isTinySupernode and a pull table entry when they have children): one wave each.
 template <typename TP>

@@ -1284,18 +1284,21 @@ __device__ inline void storeBlockRows(const double* ys, double* X, int64_t row0,
 
 template <typename TP>
 __global__ __launch_bounds__(256) void forwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
-                                                     double* U, int k, const int32_t* list)
+                                                     double* U, int k, const int32_t* list, int gathered)
 {
     __shared__ TP blk[SB * SBP];
     __shared__ double ys[SB * KMAX];
     const int s = list[blockIdx.x];
     const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    if (gathered && isWideSupernode(c))
+        return; // its gather is done (gatherKernel), its triangle is a product with the whole inverse
     const TP* L = panels + p.panelOff[s];
     double* u = U + p.rowStart[s] * k;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < r * k; idx += nt)
-        u[idx] = 0.0;
-    for (int q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+    if (!gathered)
+        for (int idx = tid; idx < r * k; idx += nt)
+            u[idx] = 0.0;
+    for (int q = p.childStart[s]; q < p.childStart[s + 1] && !gathered; ++q)
     {
         __syncthreads();
         const int ch = p.children[q];
@@ -2541,13 +2544,55 @@ hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* l
     return hipGetLastError();
 }
 
+// The forward sweep's gathers of a level's other supernodes, every front position at once: a workgroup per chunk of
+// GATHER_CHUNK positions of one front (the chunk's record holds what the threads need of its supernode), 16 threads
+// (the right-hand sides) per position. A column position adds its children's update rows to X (child order); a row
+// position sets its row of U(s) to their sum (0 without children).
+__global__ __launch_bounds__(256) void gatherKernel(const GatherChunk* chunks, const int32_t* gatherStart,
+                                                    const int32_t* gatherRows, double* X, double* U, int k)
+{
+    const GatherChunk chunk = chunks[blockIdx.x];
+    const int q = chunk.q0 + int(threadIdx.x >> 4), h = int(threadIdx.x & (KMAX - 1));
+    if (q >= chunk.order || h >= k)
+        return;
+    const int64_t gp = int64_t(chunk.positions) + q;
+    const int32_t e0 = gatherStart[gp], e1 = gatherStart[gp + 1];
+    if (q < chunk.columns)
+    {
+        if (e1 == e0)
+            return;
+        double* x = X + int64_t(chunk.first + q) * k + h;
+        double y = *x;
+        for (int32_t e = e0; e < e1; ++e)
+            y += U[int64_t(gatherRows[e]) * k + h];
+        *x = y;
+    }
+    else
+    {
+        double y = 0.0;
+        for (int32_t e = e0; e < e1; ++e)
+            y += U[int64_t(gatherRows[e]) * k + h];
+        U[(int64_t(chunk.rowStart) + q - chunk.columns) * k + h] = y;
+    }
+}
+
+hipError_t launchGather(const GatherChunk* chunks, int chunkCount, const int32_t* gatherStart,
+                        const int32_t* gatherRows, double* X, double* U, int k, hipStream_t stream)
+{
+    if (chunkCount > 0)
+        hipLaunchKernelGGL(gatherKernel, dim3(unsigned(chunkCount)), dim3(256), 0, stream, chunks, gatherStart,
+                           gatherRows, X, U, k);
+    return hipGetLastError();
+}
+
 template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
                          const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
-                         hipStream_t stream)
+                         hipStream_t stream, bool gathered)
 {
     if (count > 0)
-        hipLaunchKernelGGL(forwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, list);
+        hipLaunchKernelGGL(forwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, list,
+                           gathered ? 1 : 0);
     if (tileCount > 0)
         hipLaunchKernelGGL(forwardRowsKernel<TP>, dim3(tileCount), dim3(256), 0, stream, p, panels, X, U, k,
                            rowTiles, partials);
@@ -3215,7 +3260,7 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
     template hipError_t launchExtendTiles<T>(const DevicePlan&, T*, T*, const UpdateTile*, int, const int32_t*,        \
                                              const AssemblyItem*, int, hipStream_t);                                   \
     template hipError_t launchForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int, const int32_t*, \
-                                         int, const int4*, int, double*, hipStream_t);                                 \
+                                         int, const int4*, int, double*, hipStream_t, bool);                           \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int4*, int, double*, hipStream_t);                                     \
     template hipError_t launchTinyForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \

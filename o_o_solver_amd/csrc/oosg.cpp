@@ -411,6 +411,12 @@ struct oosg_plan
     // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others; the tiny ones' pull
     // table (oosg::DevicePlan::pulls) and its width.
     std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, pullBase, pulls;
+    // The forward sweep's gathers of the other supernodes: their positions in otherList order (otherPosStart, one
+    // more value), and per position the children's update rows that land there (gatherStart, gatherRows; child
+    // order).
+    std::vector<int64_t> otherPosStart; // per supernode (those not swept as tiny: -1)
+    std::vector<int32_t> gatherStart, gatherRows, gatherChunkStart;
+    std::vector<oosg::GatherChunk> gatherChunks; // by level: chunks of GATHER_CHUNK positions
     int32_t pullWidth = 0;
     std::vector<int64_t> rowStart, panelOff, updateOff, linvOff;
     // The sweeps' work on big supernodes, by level: row tiles (forward) and column blocks (backward), in chunks where
@@ -460,7 +466,7 @@ struct oosg_plan
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls;
+    DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls, dOtherPosStart, dGatherStart, dGatherRows, dGatherChunks;
     DeviceBuffer dTrailingBlocks, dTrailingItems, dTrailingPullStart, dTrailingPulls;
     // A mixed factor's boundary update matrices staged for their widening, and the ranges of that copy.
     DeviceBuffer widenTemp, widenRanges;
@@ -913,6 +919,54 @@ void splitTinySupernodes(oosg_plan& p)
         }
         p.tinyStart[l + 1] = static_cast<int32_t>(p.tinyList.size());
         p.otherStart[l + 1] = static_cast<int32_t>(p.otherList.size());
+    }
+    // The other supernodes' gather lists (positions in otherList order) and their chunks by level.
+    const size_t others = p.otherList.size();
+    p.otherPosStart.assign(p.S, -1);
+    int64_t gathered = 0;
+    p.gatherChunks.clear();
+    p.gatherChunkStart.assign(p.levels + 1, 0);
+    for (int32_t l = 0; l < p.levels; ++l)
+    {
+        for (int32_t i = p.otherStart[l]; i < p.otherStart[l + 1]; ++i)
+        {
+            const int32_t s = p.otherList[i];
+            const int32_t order = p.ncol[s] + p.nrow[s];
+            p.otherPosStart[s] = gathered;
+            gathered += order;
+            for (int32_t q0 = 0; q0 < order; q0 += oosg::GATHER_CHUNK)
+                p.gatherChunks.push_back(oosg::GatherChunk{q0, order, p.ncol[s], p.first[s],
+                                                           static_cast<int32_t>(p.rowStart[s]),
+                                                           static_cast<int32_t>(p.otherPosStart[s]), 0, 0});
+        }
+        p.gatherChunkStart[l + 1] = static_cast<int32_t>(p.gatherChunks.size());
+    }
+    if (gathered >= std::numeric_limits<int32_t>::max() || p.rowStart[p.S] >= std::numeric_limits<int32_t>::max())
+        fail(OOSG_INVALID_ARGUMENT, "plan: more front positions than the sweeps' gather lists hold");
+    std::vector<int32_t> count(static_cast<size_t>(gathered) + 1, 0);
+    for (size_t i = 0; i < others; ++i)
+    {
+        const int32_t s = p.otherList[i];
+        for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+        {
+            const int32_t ch = p.children[q];
+            for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+                ++count[static_cast<size_t>(p.otherPosStart[s] + p.rel[e]) + 1];
+        }
+    }
+    for (size_t q = 1; q < count.size(); ++q)
+        count[q] += count[q - 1];
+    p.gatherStart = count;
+    p.gatherRows.assign(static_cast<size_t>(count.back()), 0);
+    for (size_t i = 0; i < others; ++i)
+    {
+        const int32_t s = p.otherList[i];
+        for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
+        {
+            const int32_t ch = p.children[q];
+            for (int64_t e = p.rowStart[ch]; e < p.rowStart[ch + 1]; ++e)
+                p.gatherRows[count[static_cast<size_t>(p.otherPosStart[s] + p.rel[e])]++] = static_cast<int32_t>(e);
+        }
     }
 }
 
@@ -1747,6 +1801,10 @@ void uploadSweepPlan(oosg_plan& p)
     p.dTinyList = upload(p.tinyList, st, "plan: tiny supernodes");
     p.dOtherList = upload(p.otherList, st, "plan: sweep supernodes");
     p.dPullBase = upload(p.pullBase, st, "plan: pull table");
+    p.dOtherPosStart = upload(p.otherPosStart, st, "plan: gather lists");
+    p.dGatherStart = upload(p.gatherStart, st, "plan: gather lists");
+    p.dGatherRows = upload(p.gatherRows, st, "plan: gather lists");
+    p.dGatherChunks = upload(p.gatherChunks, st, "plan: gather lists");
     p.dPulls = upload(p.pulls, st, "plan: pull table");
     p.dp.first = p.dFirst.as<int32_t>();
     p.dp.ncol = p.dNcol.as<int32_t>();
@@ -1788,7 +1846,8 @@ void uploadPlan(oosg_plan& p)
 uint64_t structureBytes(const oosg_plan& p)
 {
     return (p.first.size() * 3 + p.rows.size() * 2 + p.children.size() + p.smallList.size() + p.stepFronts.size() +
-            p.levelList.size() * 2 + p.pullBase.size() + p.pulls.size()) *
+            p.levelList.size() * 2 + p.pullBase.size() + p.pulls.size() + p.gatherStart.size() +
+            p.gatherRows.size() + p.otherPosStart.size() * 2 + p.gatherChunks.size() * 8) *
                4 +
            p.panelOff.size() * 24 +
            (p.forwardTiles.size() + p.backwardBlocks.size() + p.forwardSubtractions.size() +
@@ -2667,8 +2726,13 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
             check(oosg::launchTinyForward<TP>(p.dp, panels, linv, X, U, k, tiny + p.tinyStart[l],
                                               p.tinyStart[l + 1] - p.tinyStart[l], st),
                   "solve: forward sweep (tiny supernodes)");
-            check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, others + p.otherStart[l],
-                                          p.otherStart[l + 1] - p.otherStart[l], nullptr, 0, partials, st),
+            const int count = p.otherStart[l + 1] - p.otherStart[l];
+            check(oosg::launchGather(p.dGatherChunks.as<oosg::GatherChunk>() + p.gatherChunkStart[l],
+                                     p.gatherChunkStart[l + 1] - p.gatherChunkStart[l], p.dGatherStart.as<int32_t>(),
+                                     p.dGatherRows.as<int32_t>(), X, U, k, st),
+                  "solve: forward sweep (gathers)");
+            check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, others + p.otherStart[l], count, nullptr, 0,
+                                          partials, st, true),
                   "solve: forward sweep");
         });
         const int items = static_cast<int>(p.forwardTriangleStart[l + 1] - p.forwardTriangleStart[l]);
