@@ -147,6 +147,13 @@ constexpr int TINY_COLUMNS = 64, TINY_ROWS = 128;
 // At most this many children's update rows may hit one position of a tiny supernode's front (else it is swept by the
 // kernels of the other supernodes).
 constexpr int TINY_PULLS = 4;
+// The other supernodes of at most MEDIUM_COLUMNS columns are swept by a workgroup each (mediumForwardKernel,
+// mediumBackwardKernel), after their gathers; they have no row tiles or column blocks.
+constexpr int MEDIUM_COLUMNS = 128;
+__host__ __device__ inline bool isMediumSupernode(int c)
+{
+    return c > 0 && c <= MEDIUM_COLUMNS;
+}
 __host__ __device__ inline bool isTinySupernode(int c, int r)
 {
     return c > 0 && c <= TINY_COLUMNS && r <= TINY_ROWS;
@@ -212,6 +219,12 @@ hipError_t launchGather(const GatherChunk* chunks, int chunkCount, const int32_t
 template <typename TP>
 hipError_t launchTinyForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
                              const int32_t* list, int count, hipStream_t stream);
+template <typename TP>
+hipError_t launchMediumForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                               const int32_t* list, int count, hipStream_t stream);
+template <typename TP>
+hipError_t launchMediumBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                                const int32_t* list, int count, hipStream_t stream);
 template <typename TP>
 hipError_t launchTinyBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
                               const int32_t* list, int count, hipStream_t stream);

@@ -1641,6 +1641,269 @@ __global__ __launch_bounds__(64 * TINY_WAVES) void tinyBackwardKernel(DevicePlan
         }
 }
 
+// ---- the sweeps of medium supernodes: a workgroup each ---------------------------------------------------------------
+//
+// Supernodes of at most MEDIUM_COLUMNS columns that are not tiny are swept by one workgroup of four waves each, with
+// FP64 MFMA products: the right-hand sides of the front's columns in LDS (MEDIUM_COLUMNS × KMAX doubles, zero beyond
+// its rows and columns), the triangle by the inverses of its diagonal blocks of PANEL_BLOCK columns (two at most), a
+// wave per 16-row tile of a block; the rows below in 16-row tiles spread over the waves (forward), the columns' tiles
+// two per wave over every row below (backward). Their gathers are gatherKernel's, before them.
+
+// Wave-level: a 16-row tile (rows 16·tile within the block) of Inv·Y (TRANSPOSE: Invᵀ·Y) for the unit lower inverse of
+// the diagonal block of qb columns from row q0 (qb × qb column-major at inv), Y the block's rows of ys; the result in
+// the MFMA result layout.
+template <typename TP, bool TRANSPOSE>
+__device__ inline double4_t blockInverseTile(const TP* inv, int qb, int q0, const double* ys, int tile, int lr, int lq)
+{
+    const int i = 16 * tile + lr;
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+        const int j = 4 * q + lq;
+        const bool valid = i < qb && j < qb && (TRANSPOSE ? j >= i : j <= i);
+        const double value = double(inv[valid ? (TRANSPOSE ? j + i * qb : i + j * qb) : 0]);
+        a[q] = valid ? value : 0.0;
+    }
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], ys[(q0 + 4 * q + lq) * KMAX + lr], acc, 0, 0, 0);
+    return acc;
+}
+
+// Stores a tile in the result layout to the rows row0 + lq + 4v of ys.
+__device__ inline void storeTile(double* ys, int row0, double4_t acc, int lr, int lq)
+{
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        ys[(row0 + lq + 4 * v) * KMAX + lr] = acc[v];
+}
+
+__device__ inline double4_t loadTile(const double* ys, int row0, int lr, int lq)
+{
+    double4_t t;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        t[v] = ys[(row0 + lq + 4 * v) * KMAX + lr];
+    return t;
+}
+
+// Forward: y_F = X(cols) (gathered), y_F ← L_FF⁻¹·y_F block by block, X(cols) = y_F, U(s) −= L_RF·y_F.
+template <typename TP>
+__global__ __launch_bounds__(256) void mediumForwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                           double* U, int k, const int32_t* list)
+{
+    __shared__ double ys[MEDIUM_COLUMNS * KMAX];
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    const int ct = (c + 15) >> 4;
+    for (int idx = tid; idx < MEDIUM_COLUMNS * KMAX; idx += 256)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        const bool valid = i < c && h < k;
+        const double x = X[valid ? int64_t(first + i) * k + h : 0];
+        ys[idx] = valid ? x : 0.0;
+    }
+    __syncthreads();
+    const TP* L = panels + p.panelOff[s];
+    const TP* inv = linv + p.linvOff[s];
+    const int qb0 = min(c, PANEL_BLOCK);
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    if (16 * w < qb0)
+        acc = blockInverseTile<TP, false>(inv, qb0, 0, ys, w, lr, lq);
+    __syncthreads();
+    if (16 * w < qb0)
+        storeTile(ys, 16 * w, acc, lr, lq);
+    __syncthreads();
+    if (c > PANEL_BLOCK)
+    {
+        const int qb1 = c - PANEL_BLOCK;
+        // y_1 −= L_10·z_0.
+        if (16 * w < qb1)
+        {
+            const int i = 16 * w + lr;
+            acc = loadTile(ys, PANEL_BLOCK + 16 * w, lr, lq);
+            double a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+            {
+                const int j = 4 * q + lq;
+                const bool valid = i < qb1;
+                const double value = double(L[valid ? (PANEL_BLOCK + i) + int64_t(j) * n : 0]);
+                a[q] = valid ? -value : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], ys[(4 * q + lq) * KMAX + lr], acc, 0, 0, 0);
+        }
+        __syncthreads();
+        if (16 * w < qb1)
+            storeTile(ys, PANEL_BLOCK + 16 * w, acc, lr, lq);
+        __syncthreads();
+        if (16 * w < qb1)
+            acc = blockInverseTile<TP, false>(inv + PANEL_BLOCK * PANEL_BLOCK, qb1, PANEL_BLOCK, ys, w, lr, lq);
+        __syncthreads();
+        if (16 * w < qb1)
+            storeTile(ys, PANEL_BLOCK + 16 * w, acc, lr, lq);
+        __syncthreads();
+    }
+    for (int idx = tid; idx < c * KMAX; idx += 256)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        if (h < k)
+            X[int64_t(first + i) * k + h] = ys[idx];
+    }
+    // U(s) −= L_RF·y_F, 16-row tiles over the waves, the columns' k-steps 16 at a time.
+    double* u = U + p.rowStart[s] * k;
+    const int rt = (r + 15) >> 4, steps = 4 * ct;
+    for (int t = w; t < rt; t += 4)
+    {
+        const int rho = 16 * t + lr; // the A operand's row
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            const bool valid = i < r && lr < k;
+            const double value = u[valid ? int64_t(i) * k + lr : 0];
+            acc[v] = valid ? value : 0.0;
+        }
+        for (int q0 = 0; q0 < steps; q0 += 16)
+        {
+            double a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+            {
+                const int j = 4 * (q0 + q) + lq;
+                const bool valid = rho < r && j < c;
+                const double value = double(L[valid ? (c + rho) + int64_t(j) * n : 0]);
+                a[q] = valid ? -value : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (q0 + q < steps)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], ys[(4 * (q0 + q) + lq) * KMAX + lr], acc, 0, 0,
+                                                               0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * t + lq + 4 * v;
+            if (i < r && lr < k)
+                u[int64_t(i) * k + lr] = acc[v];
+        }
+    }
+}
+
+// Backward: x_F = X(cols) − L_RFᵀ·X(rows) (wave w: the columns' tiles w and w + 4 over every row below), then
+// X(cols) = L_FF⁻ᵀ·x_F block by block from the last.
+template <typename TP>
+__global__ __launch_bounds__(256) void mediumBackwardKernel(DevicePlan p, const TP* panels, const TP* linv, double* X,
+                                                            int k, const int32_t* list)
+{
+    __shared__ double xs[MEDIUM_COLUMNS * KMAX];
+    const int s = list[blockIdx.x];
+    const int c = p.ncol[s], r = p.nrow[s], n = c + r, first = p.first[s];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    const int ct = (c + 15) >> 4;
+    const TP* L = panels + p.panelOff[s];
+    const int32_t* rows = p.rows + p.rowStart[s];
+    double4_t acc[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int i = 16 * (w + 4 * u) + lq + 4 * v;
+            const bool valid = i < c && lr < k;
+            const double value = X[valid ? int64_t(first + i) * k + lr : 0];
+            acc[u][v] = valid ? value : 0.0;
+        }
+    for (int r0 = 0; r0 < r; r0 += 32)
+    {
+        double b[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+        {
+            const int rho = r0 + 4 * m + lq;
+            const bool valid = rho < r && lr < k;
+            const int32_t row = rows[rho < r ? rho : 0];
+            const double value = X[valid ? int64_t(row) * k + lr : 0];
+            b[m] = valid ? value : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (w + 4 * u < ct)
+            {
+                const int j = 16 * (w + 4 * u) + lr; // the A operand's row: column j of L
+                double a[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                {
+                    const int rho = r0 + 4 * m + lq;
+                    const bool valid = rho < r && j < c;
+                    const double value = double(L[valid ? (c + rho) + int64_t(j) * n : 0]);
+                    a[m] = valid ? -value : 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[m], acc[u], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if (16 * (w + 4 * u) < MEDIUM_COLUMNS)
+            storeTile(xs, 16 * (w + 4 * u), acc[u], lr, lq);
+    __syncthreads();
+    const TP* inv = linv + p.linvOff[s];
+    const int qb0 = min(c, PANEL_BLOCK);
+    double4_t t0 = {0.0, 0.0, 0.0, 0.0};
+    if (c > PANEL_BLOCK)
+    {
+        const int qb1 = c - PANEL_BLOCK;
+        if (16 * w < qb1)
+            t0 = blockInverseTile<TP, true>(inv + PANEL_BLOCK * PANEL_BLOCK, qb1, PANEL_BLOCK, xs, w, lr, lq);
+        __syncthreads();
+        if (16 * w < qb1)
+            storeTile(xs, PANEL_BLOCK + 16 * w, t0, lr, lq);
+        __syncthreads();
+        // x_0 −= L_10ᵀ·z_1.
+        {
+            const int i = 16 * w + lr; // row i of L_10ᵀ: column i of L
+            t0 = loadTile(xs, 16 * w, lr, lq);
+            double a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+            {
+                const int j = 4 * q + lq; // row PANEL_BLOCK + j of L
+                const bool valid = j < qb1;
+                const double value = double(L[valid ? (PANEL_BLOCK + j) + int64_t(i) * n : 0]);
+                a[q] = valid ? -value : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                t0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], xs[(PANEL_BLOCK + 4 * q + lq) * KMAX + lr], t0, 0, 0,
+                                                          0);
+        }
+        __syncthreads();
+        storeTile(xs, 16 * w, t0, lr, lq);
+        __syncthreads();
+    }
+    if (16 * w < qb0)
+        t0 = blockInverseTile<TP, true>(inv, qb0, 0, xs, w, lr, lq);
+    __syncthreads();
+    if (16 * w < qb0)
+        storeTile(xs, 16 * w, t0, lr, lq);
+    __syncthreads();
+    for (int idx = tid; idx < c * KMAX; idx += 256)
+    {
+        const int i = idx >> 4, h = idx & (KMAX - 1);
+        if (h < k)
+            X[int64_t(first + i) * k + h] = xs[idx];
+    }
+}
+
 // The update vectors of the big supernodes of a level: U(s)(rows of the tile) −= L_RF(tile, :)·X(cols(s)).
 template <typename TP>
 __global__ __launch_bounds__(256) void forwardRowsKernel(DevicePlan p, const TP* panels, const double* X, double* U,
@@ -2586,6 +2849,24 @@ hipError_t launchGather(const GatherChunk* chunks, int chunkCount, const int32_t
 }
 
 template <typename TP>
+hipError_t launchMediumForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
+                               const int32_t* list, int count, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(mediumForwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, U, k, list);
+    return hipGetLastError();
+}
+
+template <typename TP>
+hipError_t launchMediumBackward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, int k,
+                                const int32_t* list, int count, hipStream_t stream)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(mediumBackwardKernel<TP>, dim3(count), dim3(256), 0, stream, p, panels, linv, X, k, list);
+    return hipGetLastError();
+}
+
+template <typename TP>
 hipError_t launchForward(const DevicePlan& p, const TP* panels, const TP* linv, double* X, double* U, int k,
                          const int32_t* list, int count, const int4* rowTiles, int tileCount, double* partials,
                          hipStream_t stream, bool gathered)
@@ -3263,6 +3544,10 @@ hipError_t launchMfmaProbe(const double* a, const double* b, double* d, const fl
                                          int, const int4*, int, double*, hipStream_t, bool);                           \
     template hipError_t launchBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int,     \
                                           const int4*, int, double*, hipStream_t);                                     \
+    template hipError_t launchMediumForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int,           \
+                                               const int32_t*, int, hipStream_t);                                      \
+    template hipError_t launchMediumBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*,   \
+                                                int, hipStream_t);                                                     \
     template hipError_t launchTinyForward<T>(const DevicePlan&, const T*, const T*, double*, double*, int,             \
                                              const int32_t*, int, hipStream_t);                                        \
     template hipError_t launchTinyBackward<T>(const DevicePlan&, const T*, const T*, double*, int, const int32_t*, int, \

@@ -410,7 +410,7 @@ struct oosg_plan
     std::vector<int32_t> levelOf; // the level (height) of each supernode (-1: the trailing one)
     // Each level's supernodes split for the sweeps: the tiny ones (a wave each) and the others; the tiny ones' pull
     // table (oosg::DevicePlan::pulls) and its width.
-    std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, pullBase, pulls;
+    std::vector<int32_t> tinyList, tinyStart, otherList, otherStart, pullBase, pulls, mediumList, mediumStart;
     // The forward sweep's gathers of the other supernodes: their positions in otherList order (otherPosStart, one
     // more value), and per position the children's update rows that land there (gatherStart, gatherRows; child
     // order).
@@ -466,7 +466,8 @@ struct oosg_plan
 
     DeviceBuffer dFirst, dNcol, dNrow, dPanelOff, dUpdateOff, dRowStart, dRows, dRel, dChildStart, dChildren,
         dLinvOff;
-    DeviceBuffer dTinyList, dOtherList, dPullBase, dPulls, dOtherPosStart, dGatherStart, dGatherRows, dGatherChunks;
+    DeviceBuffer dTinyList, dOtherList, dMediumList, dPullBase, dPulls, dOtherPosStart, dGatherStart, dGatherRows,
+        dGatherChunks;
     DeviceBuffer dTrailingBlocks, dTrailingItems, dTrailingPullStart, dTrailingPulls;
     // A mixed factor's boundary update matrices staged for their widening, and the ranges of that copy.
     DeviceBuffer widenTemp, widenRanges;
@@ -850,6 +851,8 @@ void splitTinySupernodes(oosg_plan& p)
 {
     p.tinyList.clear();
     p.otherList.clear();
+    p.mediumList.clear();
+    p.mediumStart.assign(p.levels + 1, 0);
     p.pulls.clear();
     p.pullBase.assign(p.S, -1);
     p.tinyStart.assign(p.levels + 1, 0);
@@ -915,25 +918,28 @@ void splitTinySupernodes(oosg_plan& p)
         for (int32_t q = p.levelStart[l]; q < p.levelStart[l + 1]; ++q)
         {
             const int32_t s = p.levelList[q];
-            (tiny[s] ? p.tinyList : p.otherList).push_back(s);
+            (tiny[s] ? p.tinyList : oosg::isMediumSupernode(p.ncol[s]) ? p.mediumList : p.otherList).push_back(s);
         }
         p.tinyStart[l + 1] = static_cast<int32_t>(p.tinyList.size());
         p.otherStart[l + 1] = static_cast<int32_t>(p.otherList.size());
+        p.mediumStart[l + 1] = static_cast<int32_t>(p.mediumList.size());
     }
-    // The other supernodes' gather lists (positions in otherList order) and their chunks by level.
-    const size_t others = p.otherList.size();
+    // The gather lists of the medium and other supernodes (positions level by level) and their chunks by level.
     p.otherPosStart.assign(p.S, -1);
     int64_t gathered = 0;
     p.gatherChunks.clear();
     p.gatherChunkStart.assign(p.levels + 1, 0);
+    std::vector<int32_t> gatheredList;
     for (int32_t l = 0; l < p.levels; ++l)
     {
-        for (int32_t i = p.otherStart[l]; i < p.otherStart[l + 1]; ++i)
+        std::vector<int32_t> level(p.mediumList.begin() + p.mediumStart[l], p.mediumList.begin() + p.mediumStart[l + 1]);
+        level.insert(level.end(), p.otherList.begin() + p.otherStart[l], p.otherList.begin() + p.otherStart[l + 1]);
+        for (const int32_t s : level)
         {
-            const int32_t s = p.otherList[i];
             const int32_t order = p.ncol[s] + p.nrow[s];
             p.otherPosStart[s] = gathered;
             gathered += order;
+            gatheredList.push_back(s);
             for (int32_t q0 = 0; q0 < order; q0 += oosg::GATHER_CHUNK)
                 p.gatherChunks.push_back(oosg::GatherChunk{q0, order, p.ncol[s], p.first[s],
                                                            static_cast<int32_t>(p.rowStart[s]),
@@ -944,9 +950,8 @@ void splitTinySupernodes(oosg_plan& p)
     if (gathered >= std::numeric_limits<int32_t>::max() || p.rowStart[p.S] >= std::numeric_limits<int32_t>::max())
         fail(OOSG_INVALID_ARGUMENT, "plan: more front positions than the sweeps' gather lists hold");
     std::vector<int32_t> count(static_cast<size_t>(gathered) + 1, 0);
-    for (size_t i = 0; i < others; ++i)
+    for (const int32_t s : gatheredList)
     {
-        const int32_t s = p.otherList[i];
         for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
         {
             const int32_t ch = p.children[q];
@@ -958,9 +963,8 @@ void splitTinySupernodes(oosg_plan& p)
         count[q] += count[q - 1];
     p.gatherStart = count;
     p.gatherRows.assign(static_cast<size_t>(count.back()), 0);
-    for (size_t i = 0; i < others; ++i)
+    for (const int32_t s : gatheredList)
     {
-        const int32_t s = p.otherList[i];
         for (int32_t q = p.childStart[s]; q < p.childStart[s + 1]; ++q)
         {
             const int32_t ch = p.children[q];
@@ -1200,7 +1204,7 @@ void deriveLevels(oosg_plan& p)
             // The L_RF products of every supernode with rows below but the small ones (those do them in the
             // workgroup of their triangle): row tiles (forward), column blocks (backward). A supernode of a pivoted
             // factor whose columns were all delayed has none (c = 0).
-            if (oosg::sweepRowsInline(p.ncol[s], p.nrow[s]))
+            if (oosg::sweepRowsInline(p.ncol[s], p.nrow[s]) || oosg::isMediumSupernode(p.ncol[s]))
                 continue;
             const int32_t c = p.ncol[s], r = p.nrow[s];
             const int32_t columnBlocks = (c + 63) / 64, rowTiles = (r + 63) / 64;
@@ -1800,6 +1804,7 @@ void uploadSweepPlan(oosg_plan& p)
     p.dLevelList = upload(p.levelList, st, "plan: levels");
     p.dTinyList = upload(p.tinyList, st, "plan: tiny supernodes");
     p.dOtherList = upload(p.otherList, st, "plan: sweep supernodes");
+    p.dMediumList = upload(p.mediumList, st, "plan: sweep supernodes");
     p.dPullBase = upload(p.pullBase, st, "plan: pull table");
     p.dOtherPosStart = upload(p.otherPosStart, st, "plan: gather lists");
     p.dGatherStart = upload(p.gatherStart, st, "plan: gather lists");
@@ -1846,7 +1851,7 @@ void uploadPlan(oosg_plan& p)
 uint64_t structureBytes(const oosg_plan& p)
 {
     return (p.first.size() * 3 + p.rows.size() * 2 + p.children.size() + p.smallList.size() + p.stepFronts.size() +
-            p.levelList.size() * 2 + p.pullBase.size() + p.pulls.size() + p.gatherStart.size() +
+            p.levelList.size() * 2 + p.mediumList.size() + p.pullBase.size() + p.pulls.size() + p.gatherStart.size() +
             p.gatherRows.size() + p.otherPosStart.size() * 2 + p.gatherChunks.size() * 8) *
                4 +
            p.panelOff.size() * 24 +
@@ -2731,6 +2736,10 @@ void forwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_t
                                      p.gatherChunkStart[l + 1] - p.gatherChunkStart[l], p.dGatherStart.as<int32_t>(),
                                      p.dGatherRows.as<int32_t>(), X, U, k, st),
                   "solve: forward sweep (gathers)");
+            check(oosg::launchMediumForward<TP>(p.dp, panels, linv, X, U, k,
+                                                p.dMediumList.as<int32_t>() + p.mediumStart[l],
+                                                p.mediumStart[l + 1] - p.mediumStart[l], st),
+                  "solve: forward sweep (medium supernodes)");
             check(oosg::launchForward<TP>(p.dp, panels, linv, X, U, k, others + p.otherStart[l], count, nullptr, 0,
                                           partials, st, true),
                   "solve: forward sweep");
@@ -2818,6 +2827,9 @@ void backwardSweep(oosg_plan& p, const TP* panels, const TP* linv, int k, int32_
             check(oosg::launchTinyBackward<TP>(p.dp, panels, linv, X, k, tiny + p.tinyStart[l],
                                                p.tinyStart[l + 1] - p.tinyStart[l], st),
                   "solve: backward sweep (tiny supernodes)");
+            check(oosg::launchMediumBackward<TP>(p.dp, panels, linv, X, k, p.dMediumList.as<int32_t>() + p.mediumStart[l],
+                                                 p.mediumStart[l + 1] - p.mediumStart[l], st),
+                  "solve: backward sweep (medium supernodes)");
             check(oosg::launchBackward<TP>(p.dp, panels, linv, X, k, others + p.otherStart[l],
                                            p.otherStart[l + 1] - p.otherStart[l], backwardBlocks, 0, partials, st),
                   "solve: backward sweep");
