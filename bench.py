@@ -2,9 +2,12 @@
 """Benchmark of the numeric multifrontal LDLᵀ on one MI355X: factor + solve to a backward error of 1e-10, mixed
 precision against the plain FP64 GPU factor.
 
-The mixed path is the learned precision selector's choice per matrix (FP32 fronts on MFMA refined in FP64 by
-conjugate gradients, or FP64 fronts when it predicts the FP32 way would be slower; FP32 when no trained weights are
-present). The FP32-fronts path is also timed on its own ("fp32_fronts_cg") and its kernels give the roofline.
+The mixed path is the learned precision selector's choice of a cut of the supernode tree's levels for this matrix
+(made once, untimed, after the FP64 leg whose pivot growth it reads): FP32 fronts on MFMA below the cut and FP64
+above, refined in FP64 by conjugate gradients; FP64 fronts throughout when it predicts no cut pays (also when no
+trained weights are present), FP32 throughout when the best cut is the top. The FP32-fronts path is also timed on its
+own ("fp32_fronts_cg"). The roofline is that of the dominant kernel of the FP32 leg when the choice is FP32 fronts,
+else of the FP64 leg.
 
 Workload: the 500 × 500 quad4 shell plate (1,494,006 DOF), interior nodes jittered ±20 % of the element size, element
 thickness log-uniform in [1, 4] mm, E = 70 GPa, ν = 0.3, clamped edges, 16 load cases of independent N(0, 1) forces on
@@ -308,7 +311,21 @@ def main():
 
     from o_o_solver_amd.selector import PrecisionSelector
     selector = PrecisionSelector.trained()
-    choice = {"probability": None, "precision": "fp32"}
+    choice = {"probability": None, "precision": "fp64", "fp32_levels": 0, "levels": stats["levels"],
+              "selector_ms": None}
+
+    def choose():
+        # The learned selector's cut for this matrix, made once (it reads the pivot growth of the plan's last
+        # factorisation, the FP64 leg's): the levels below it in FP32, the rest in FP64.
+        if selector is None:
+            return
+        t = time.perf_counter()
+        cut, p = selector.choose(plan, dm)
+        choice["selector_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        choice["probability"] = float(p.max()) if p.size else None
+        choice["fp32_levels"] = cut
+        choice["precision"] = "fp64" if cut == 0 else ("fp32" if cut >= p.size else "mixed")
+        choice["probabilities"] = [round(float(v), 4) for v in p]
 
     # A step is one call of the combined factorisation and refined solve (the first solve's forward sweep of the
     # lower levels runs beside the factorisation of the top ones; the same results as the two calls). The last
@@ -332,14 +349,26 @@ def main():
         last_x["fp32"] = x
         return f.factor_seconds, rep, fallback
 
+    def cut_step():
+        # FP32 fronts below the selector's cut, FP64 above, refined by conjugate gradients.
+        try:
+            f, x, rep = SupernodalFactor.factorise_and_solve(
+                sym, workload.matrix, bd, FactorOptions(precision=Precision.MIXED, fp32_levels=choice["fp32_levels"]),
+                device_matrix=dm, target=args.target, max_steps=args.max_steps)
+            fallback = False
+        except (NotConvergedError, PrecisionInsufficientError) as e:
+            e.factor = None
+            f, x, rep = SupernodalFactor.factorise_and_solve(sym, workload.matrix, bd, device_matrix=dm,
+                                                             target=args.target, max_steps=4)
+            fallback = True
+        last_x["cut"] = x
+        return f.factor_seconds, rep, fallback
+
     def mixed_step():
-        # The learned selector decides between FP32 fronts (refined by conjugate gradients) and FP64 fronts.
-        if selector is not None:
-            p = float(selector.predict(PrecisionSelector.features(plan, dm))[0])
-            choice["probability"] = p
-            choice["precision"] = "fp32" if p >= 0.5 else "fp64"
-        result = fp32_step() if choice["precision"] == "fp32" else fp64_step()
-        last_x["mixed"] = last_x[choice["precision"]]
+        # The learned selector's choice: FP64 fronts, FP32 fronts, or FP32 below a cut of the levels.
+        step = {"fp64": fp64_step, "fp32": fp32_step, "mixed": cut_step}[choice["precision"]]
+        result = step()
+        last_x["mixed"] = last_x[{"fp64": "fp64", "fp32": "fp32", "mixed": "cut"}[choice["precision"]]]
         return result
 
     def fp64_step():
@@ -374,9 +403,11 @@ def main():
         return elapsed / args.steps, float(np.mean(fs)), reps, any(r[2] for r in results), prof
 
     fp64_s, fp64_factor, fp64_reps, _, fp64_prof = run(fp64_step, "fp64", True)
+    choose()
+    log(f"selector ({choice['selector_ms']} ms): p(cut pays) = {choice.get('probabilities')}, chose "
+        f"{choice['precision']} ({choice['fp32_levels']} of {choice['levels']} levels in FP32)")
     fp32_s, fp32_factor, fp32_reps, fp32_fallback, fp32_prof = run(fp32_step, "fp32 fronts + cg", True)
     mixed_s, mixed_factor, mixed_reps, fallback, _ = run(mixed_step, "mixed", False)
-    log(f"selector: p(fp32 pays) = {choice['probability']}, chose {choice['precision']}")
 
     # The roofline of the HIP kernel that takes the most device time in the path the mixed step ran (the profile
     # kinds summed per kernel: the three tile kinds are one kernel), with its HBM traffic per launch from the
@@ -469,8 +500,9 @@ def main():
                   "solve_s": round(float(np.mean([r.seconds for r in mixed_reps])), 5),
                   "solves": max(r.steps for r in mixed_reps),
                   "backward_error": max(r.backward_error for r in mixed_reps),
-                  "precision_chosen": choice["precision"], "selector_probability": choice["probability"],
-                  "fp64_fallback": fallback},
+                  "precision_chosen": choice["precision"], "fp32_levels": choice["fp32_levels"],
+                  "levels": choice["levels"], "selector_probability": choice["probability"],
+                  "selector_ms": choice["selector_ms"], "fp64_fallback": fallback},
         "fp32_fronts_cg": {"seconds": round(fp32_s, 5), "factor_s": round(fp32_factor, 5),
                            "solve_s": round(float(np.mean([r.seconds for r in fp32_reps])), 5),
                            "solves": max(r.steps for r in fp32_reps),

@@ -410,9 +410,10 @@ oosg_status oosg_plan_synchronize(oosg_plan *plan);
 
 /* ---- the learned precision selector (no counterpart in the reference) -------------------------------------- */
 
-/* A small MLP (8 features → 32 → 32 → 1, tanh hidden layers, logistic output) that gives p(FP32 fronts with FP64
- * refinement reach the target backward error) for a matrix of a plan; FP32 on the device, forward, backward and Adam
- * steps as HIP kernels with deterministic reductions. Arrays are host memory. */
+/* A small MLP (8 features → 32 → 32 → 1, tanh hidden layers, logistic output) that gives, for a cut of a plan's
+ * levels (oosg_selector_cut_features), p(factorising the subtrees below the cut in FP32 and the rest in FP64, refined
+ * by conjugate gradients, reaches the target backward error faster than FP64 fronts throughout); FP32 on the device,
+ * forward, backward and Adam steps as HIP kernels with deterministic reductions. Arrays are host memory. */
 #define OOSG_SELECTOR_FEATURES 8
 #define OOSG_SELECTOR_PARAMETERS 1377
 typedef struct oosg_selector oosg_selector;
@@ -422,13 +423,20 @@ void oosg_selector_destroy(oosg_selector *selector);
 /* The flat parameter array: W1 (32 × 8 row-major), b1 (32), W2 (32 × 32), b2 (32), W3 (32), b3 (1). */
 oosg_status oosg_selector_parameters(const oosg_selector *selector, float *params);
 oosg_status oosg_selector_set_parameters(oosg_selector *selector, const float *params);
-/* The 8 features of a matrix of a plan (normalised to about unit range): log10 of the factorisation's flops per
- * stored factor value, log10 flops, log10 of the largest front order, log10 of the spread of the positive diagonal, the mean and the minimum of log10(a_jj / Σ_i≠j |a_ij|),
- * the fraction of equations whose diagonal does not dominate, log10(‖A‖∞ / max a_jj). */
-oosg_status oosg_selector_features(const oosg_plan *plan, const oosg_matrix *matrix, float *features);
+/* The 8 features of each cut of a plan's levels, for the per-subtree choice of OOSG_MIXED: cut l (1 <= l <= L, the
+ * plan's levels) factorises the subtrees below level l (the levels 0 .. l - 1) in FP32 and the rest in FP64; row l - 1
+ * of `features` holds: l / L; the share of the flops below the cut; log10 of the largest front order (c + r) below
+ * the cut − 2.5 and at or above it − 2.5; log10 n − 5; the mean and the minimum over the equations below the cut of
+ * log10 of the scaled matrix's diagonal dominance |a_jj| / Σ_i≠j |a_ij| (clamped to [−3, 3]); log10(1 + g) / 4 for the
+ * largest pivot growth g = |a_jj / d_j| of those equations in the plan's last factorisation (0 before the first).
+ * *count receives L; with features null nothing else is done, otherwise capacity must be at least L. No counterpart
+ * in the reference (the learned precision choice is this library's). */
+oosg_status oosg_selector_cut_features(const oosg_plan *plan, const oosg_matrix *matrix, float *features,
+                                       int32_t capacity, int32_t *count);
 /* p for count × 8 features. */
 oosg_status oosg_selector_predict(oosg_selector *selector, const float *features, int64_t count, float *probability);
-/* The mean binary cross-entropy over count samples (labels 0 or 1) and its gradient by the parameters. */
+/* The mean binary cross-entropy over count samples (labels in [0, 1]: y·log p + (1 − y)·log(1 − p), negated) and its
+ * gradient by the parameters. */
 oosg_status oosg_selector_gradient(oosg_selector *selector, const float *features, const float *labels, int64_t count,
                                    float *loss, float *gradient);
 /* `steps` full-batch Adam steps (learning rate lr); *loss receives the loss before the last step. */

@@ -168,7 +168,7 @@ class FactorOptions:
     # keeps the matrix diagonal there), or None (FactorOptions::referenceDiagonal, SupernodalFactor.hpp:75).
     reference_diagonal: object = None
     # Precision.LEARNED: the selector (a selector.PrecisionSelector; None: the committed weights) and the probability
-    # from which it picks FP32.
+    # from which it takes a cut (the levels below it in FP32) rather than FP64 fronts throughout.
     selector: object = None
     fp32_threshold: float = 0.5
     # Names the equations of the diagnostics (FactorOptions::equationNamer, SupernodalFactor.hpp:76-77): a callable
@@ -385,6 +385,11 @@ class Plan:
         self.out_of_core = bool(out_of_core)
         self.handle = handle
         self._matrices = {}
+
+    @property
+    def levels(self):
+        """The plan's levels (heights of the supernode tree)."""
+        return int(self.stats()["levels"])
 
     def stats(self):
         s = np.zeros(8, np.int64)
@@ -606,22 +611,31 @@ class SupernodalFactor:
                                  "(a negative value where the matrix diagonal is kept) or none")
         precision = options.precision
         selected = None
+        fp32_levels = options.fp32_levels
         if precision == Precision.LEARNED:
+            # The selector's cut: the levels below it in FP32 (all of them: FP32 fronts; none: FP64 fronts).
             from .selector import PrecisionSelector
             selector = options.selector or PrecisionSelector.trained(device)
             if selector is None:
                 raise ValueError("factorise: Precision.LEARNED needs a selector (no committed weights found)")
-            selected = float(selector.predict(selector.features(plan, device_matrix))[0])
-            precision = Precision.FP32 if selected >= options.fp32_threshold else Precision.FP64
-        opts = cls._options_c(options, precision, reference)
+            cut, probabilities = selector.choose(plan, device_matrix, options.fp32_threshold)
+            selected = float(probabilities.max()) if probabilities.size else 0.0
+            if cut == 0:
+                precision = Precision.FP64
+            elif cut >= probabilities.size:
+                precision = Precision.FP32
+            else:
+                precision, fp32_levels = Precision.MIXED, cut
+        opts = cls._options_c(options, precision, reference, fp32_levels)
         plan.set_cancellation(options.cancellation)
         return options, plan, device_matrix, opts, reference, selected
 
     @staticmethod
-    def _options_c(options, precision, reference):
+    def _options_c(options, precision, reference, fp32_levels=None):
+        levels = options.fp32_levels if fp32_levels is None else fp32_levels
         return _native.FactorOptionsC(int(options.pivoting), _C_PRECISION[Precision(precision)],
                                       float(options.pivot_policy.max_ratio), int(options.pivot_policy.bailout),
-                                      _ptr(reference), int(options.fp32_levels))
+                                      _ptr(reference), int(levels))
 
     @classmethod
     def factorise(cls, symbolic, matrix, options=None, device=0, hbm_budget_bytes=0, device_matrix=None, plan=None):
@@ -674,7 +688,8 @@ class SupernodalFactor:
 
         status, factor, rep = run(opts)
         fell_back = False
-        if status in (7, 8) and options.precision == Precision.LEARNED and opts.precision == Precision.FP32:
+        scaled = opts.precision != _C_PRECISION[Precision.FP64]
+        if status in (7, 8) and options.precision == Precision.LEARNED and scaled:
             factor = None
             status, factor, rep = run(cls._options_c(options, Precision.FP64, reference))
             fell_back = True

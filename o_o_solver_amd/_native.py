@@ -126,7 +126,7 @@ _SIGNATURES = {
     "oosg_selector_destroy": (None, [c_void_p]),
     "oosg_selector_parameters": (c_int, [c_void_p, c_void_p]),
     "oosg_selector_set_parameters": (c_int, [c_void_p, c_void_p]),
-    "oosg_selector_features": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "oosg_selector_cut_features": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "oosg_selector_predict": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oosg_selector_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oosg_selector_train": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, ctypes.c_float, c_void_p]),

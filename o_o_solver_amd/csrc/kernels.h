@@ -150,6 +150,7 @@ constexpr int TINY_PULLS = 4;
 // The other supernodes of at most MEDIUM_COLUMNS columns are swept by a workgroup each (mediumForwardKernel,
 // mediumBackwardKernel), after their gathers; they have no row tiles or column blocks.
 constexpr int MEDIUM_COLUMNS = 128;
+constexpr int MEDIUM_ROW_INDICES = 2048; // row indices a medium backward sweep stages in LDS
 __host__ __device__ inline bool isMediumSupernode(int c)
 {
     return c > 0 && c <= MEDIUM_COLUMNS;
@@ -249,14 +250,8 @@ hipError_t launchResidual(const int64_t* offsets, const int32_t* cols, const dou
 
 hipError_t launchFillDouble(double* data, int64_t ld, int64_t n, int k, double value, hipStream_t stream);
 
-// Per-column coefficients of the Krylov updates (k <= 16), passed by value.
-struct ColumnCoefficients
-{
-    double v[16];
-};
-// Workgroups of the column dot products (the partial sums array holds DOT_GROUPS × 16 values) and of the Gram
-// products (DOT_GROUPS × GRAM_PAIRS × 256 values).
-constexpr int DOT_GROUPS = 1024;
+// Gram products: their partial sums array holds (GRAM_GROUPS + 16) × GRAM_PAIRS × 256 values (the workgroups' sums,
+// then the sums of 16 runs of them).
 constexpr int GRAM_PAIRS = 4;
 constexpr int GRAM_GROUPS = 1024; // workgroups of a Gram product (a contiguous range of rows each)
 // Gram products of up to GRAM_PAIRS pairs of row-major n × k vectors (k <= 16): out[q·256 + 16a + b] =
@@ -280,21 +275,22 @@ struct BlockTerm
     double c[256]; // C by value where deviceC is null
 };
 hipError_t launchBlockCombine(double* y, const BlockTerm& term, int64_t n, int k, hipStream_t stream);
+// One block conjugate-gradient step over row-major n × k vectors, from W (in z), A·W (in p) and T, U = T·α (16 × 16
+// each at coef and coef + 256 on the device): Q = (A·W)·T, P = W·T (over A·W), X += W·U, R −= (A·W)·U, Z = R (over
+// W); the column maxima of |S⁻¹·R| and |S·X| into norms[0, 16) and norms[16, 32) (bits; the caller zeroes them;
+// scale null: S = I).
+hipError_t launchBlockStep(double* z, double* p, double* q, double* x, double* r, const double* coef,
+                           const double* scale, int64_t n, int k, unsigned long long* norms, hipStream_t stream);
 
 // Row-major n × k vectors (k <= 16): out = b − A·x (b null: A·x) with the CSR copy of A, by its row groups (rows
-// [groups[g], groups[g + 1]), at most 8, with the same columns); column dot products (into
-// out[0, k), through `partial`); column max |v| (bits into out, which the caller zeroes); x += α p, r −= α q;
-// p = z + β p; dst = src·diag(scale) by rows.
+// [groups[g], groups[g + 1]), at most 8, with the same columns), or with a row scaling S (powers of two) the scaled
+// matrix's out = S·(b − A·S·x); column max |v| (bits into out, which the caller zeroes), of S·v or S⁻¹·v (divide)
+// with a scaling; dst = src·diag(scale) by rows; x += z.
 hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values,
                               const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
-                              int k, hipStream_t stream);
-hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, double* partial, double* out,
-                            hipStream_t stream);
-hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream);
-hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, const ColumnCoefficients& alpha,
-                        int64_t n, int k, hipStream_t stream);
-hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
-                             hipStream_t stream);
+                              int k, hipStream_t stream, const double* scale = nullptr);
+hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream,
+                               const double* scale = nullptr, int divide = 0);
 hipError_t launchScaleRows(const double* src, double* dst, const double* scale, int64_t n, int k, hipStream_t stream);
 hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream);
 hipError_t launchPlainUpdateTiles(double* fronts, const double* w, const UpdateTile* tiles, int count,
@@ -337,6 +333,11 @@ struct PivotStatistics
 };
 hipError_t launchPivotStatistics(const double* diag, const double* scale, const double* diagonal,
                                  const double* reference, int64_t n, double maxRatio, PivotStatistics* out,
+                                 hipStream_t stream);
+// Per supernode s < count the largest |a_jj / d_j| over its columns (its pivot growth), 0 for a supernode whose
+// columns are not all among the first `eliminated` equations.
+hipError_t launchSupernodeGrowth(const double* diag, const double* scale, const double* diagonal, const int32_t* first,
+                                 const int32_t* ncol, int32_t count, int64_t eliminated, float* out,
                                  hipStream_t stream);
 
 // A block matrix on a node pattern (SparsePattern + BlockSparseMatrix): node j owns the DOFs [firstDof[j],

@@ -1573,21 +1573,36 @@ __global__ __launch_bounds__(64 * TINY_WAVES) void tinyBackwardKernel(DevicePlan
             const double value = X[valid ? int64_t(first + i) * k + lr : 0];
             x[t][v] = valid ? value : 0.0;
         }
-    // x_F −= L_RFᵀ·X(rows): k-steps of 4 rows below, 8 at a time (the loads of a batch all in flight).
+    // x_F −= L_RFᵀ·X(rows): k-steps of 4 rows below, 8 at a time. The row indices (r <= 128) are loaded once, two
+    // per lane (rows lane and 64 + lane), and handed to the lanes that gather by shuffles; each batch's X rows are
+    // gathered while the previous batch is multiplied.
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
+    const int32_t index0 = r > 0 ? rows[min(lane, r - 1)] : 0;
+    const int32_t index1 = r > 0 ? rows[min(64 + lane, r - 1)] : 0;
+    // Batch r0's X rows: row r0 + 4m + lq (r0 + 4m < 64 for every lane of a batch or for none).
+    const auto gatherBatch = [&](int r0, double (&b)[8])
+    {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+        {
+            const int rho = r0 + 4 * m + lq;
+            const bool valid = rho < r && column;
+            const int32_t row = __shfl(r0 + 4 * m < 64 ? index0 : index1, rho & 63);
+            const double value = X[valid ? int64_t(row) * k + lr : 0];
+            b[m] = valid ? value : 0.0;
+        }
+    };
+    double next[8];
+    gatherBatch(0, next);
     for (int r0 = 0; r0 < r; r0 += 32)
     {
         double b[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-        {
-            const int rho = r0 + 4 * m + lq;
-            const bool valid = rho < r;
-            const int32_t row = rows[valid ? rho : 0];
-            const double value = X[valid && column ? int64_t(row) * k + lr : 0];
-            b[m] = valid && column ? value : 0.0;
-        }
+            b[m] = next[m];
+        if (r0 + 32 < r)
+            gatherBatch(r0 + 32, next);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
             if (t < ct)
@@ -1809,34 +1824,45 @@ __global__ __launch_bounds__(256) void mediumBackwardKernel(DevicePlan p, const 
     const int ct = (c + 15) >> 4;
     const TP* L = panels + p.panelOff[s];
     const int32_t* rows = p.rows + p.rowStart[s];
-    double4_t acc[2];
+    // The first MEDIUM_ROW_INDICES row indices through LDS (one coalesced load), the rest read where used.
+    __shared__ int32_t rowIndex[MEDIUM_ROW_INDICES];
+    for (int i = tid; i < min(r, MEDIUM_ROW_INDICES); i += 256)
+        rowIndex[i] = rows[i];
+    __syncthreads();
+    // x_F −= L_RFᵀ·X(rows): wave w takes the batches of 32 rows w, w + 4, ... over every column tile, its partial sums
+    // in registers; each batch's X rows (rows r0 + 4m + lq) are gathered while the previous batch is multiplied.
+    constexpr int TILES = MEDIUM_COLUMNS / 16;
+    double4_t part[TILES];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-        {
-            const int i = 16 * (w + 4 * u) + lq + 4 * v;
-            const bool valid = i < c && lr < k;
-            const double value = X[valid ? int64_t(first + i) * k + lr : 0];
-            acc[u][v] = valid ? value : 0.0;
-        }
-    for (int r0 = 0; r0 < r; r0 += 32)
+    for (int t = 0; t < TILES; ++t)
+        part[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const auto gatherBatch = [&](int r0, double (&b)[8])
     {
-        double b[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m)
         {
             const int rho = r0 + 4 * m + lq;
             const bool valid = rho < r && lr < k;
-            const int32_t row = rows[rho < r ? rho : 0];
+            const int32_t row = rho < MEDIUM_ROW_INDICES ? rowIndex[rho < r ? rho : 0] : rows[rho < r ? rho : 0];
             const double value = X[valid ? int64_t(row) * k + lr : 0];
             b[m] = valid ? value : 0.0;
         }
+    };
+    double next[8];
+    gatherBatch(32 * w, next);
+    for (int r0 = 32 * w; r0 < r; r0 += 128)
+    {
+        double b[8];
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (w + 4 * u < ct)
+        for (int m = 0; m < 8; ++m)
+            b[m] = next[m];
+        if (r0 + 128 < r)
+            gatherBatch(r0 + 128, next);
+#pragma unroll
+        for (int t = 0; t < TILES; ++t)
+            if (t < ct)
             {
-                const int j = 16 * (w + 4 * u) + lr; // the A operand's row: column j of L
+                const int j = 16 * t + lr; // the A operand's row: column j of L
                 double a[8];
 #pragma unroll
                 for (int m = 0; m < 8; ++m)
@@ -1848,14 +1874,39 @@ __global__ __launch_bounds__(256) void mediumBackwardKernel(DevicePlan p, const 
                 }
 #pragma unroll
                 for (int m = 0; m < 8; ++m)
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[m], acc[u], 0, 0, 0);
+                    part[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[m], part[t], 0, 0, 0);
             }
     }
+    // x_F = X(cols) + the waves' sums, added in wave order through LDS (every tile written by wave 0: zeros beyond
+    // the columns).
+    for (int q = 0; q < 4; ++q)
+    {
+        if (w == q)
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
-        if (16 * (w + 4 * u) < MEDIUM_COLUMNS)
-            storeTile(xs, 16 * (w + 4 * u), acc[u], lr, lq);
-    __syncthreads();
+            for (int t = 0; t < TILES; ++t)
+            {
+                double4_t base;
+                if (q == 0)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                    {
+                        const int i = 16 * t + lq + 4 * v;
+                        const bool valid = i < c && lr < k;
+                        const double value = X[valid ? int64_t(first + i) * k + lr : 0];
+                        base[v] = valid ? value : 0.0;
+                    }
+                else if (t < ct)
+                    base = loadTile(xs, 16 * t, lr, lq);
+                if (q == 0 || t < ct)
+                {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        base[v] += part[t][v];
+                    storeTile(xs, 16 * t, base, lr, lq);
+                }
+            }
+        __syncthreads();
+    }
     const TP* inv = linv + p.linvOff[s];
     const int qb0 = min(c, PANEL_BLOCK);
     double4_t t0 = {0.0, 0.0, 0.0, 0.0};
@@ -2267,7 +2318,8 @@ constexpr int VECTOR_ROWS = 16; // row groups of a workgroup's pass in the spars
 // out = b − A·x (b null: out = A·x), A the full symmetric matrix by rows (the CSR copy of the lower CSC), by row
 // groups: runs of up to ROW_GROUP consecutive rows with the same columns (the degrees of freedom of one node). A
 // group per 16 lanes (four groups in flight per wave), lane h its column h: each x row is read once for the whole
-// group, and each row's products are summed in its columns' order.
+// group, and each row's products are summed in its columns' order. With a row scaling S (powers of two, so every
+// scaling is exact) the product is the scaled matrix's: out = S·(b − A·S·x), each value a_ij taken as a_ij·s_j.
 constexpr int ROW_GROUP = 8;
 
 // The sparse product's workgroup index, XCD by XCD: the dispatcher hands workgroup b to XCD b mod 8, so XCD x takes
@@ -2283,7 +2335,7 @@ __device__ inline int64_t xcdOrderedBlock()
 // ...), and the lanes' sums of each row are added by a fixed butterfly over the 16 lanes.
 __device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t* cols, const double* values,
                                           const int32_t* groups, int64_t groupCount, const double* x,
-                                          const double* b, double* out)
+                                          const double* b, double* out, const double* scale)
 {
     const int lane = threadIdx.x & 15;
     for (int64_t g = xcdOrderedBlock() * VECTOR_ROWS + (threadIdx.x >> 4); g < groupCount;
@@ -2301,7 +2353,8 @@ __device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t*
         }
         for (int64_t e = lane; e < degree; e += 16)
         {
-            const double xv = x[cols[e0 + e]];
+            const int32_t c = cols[e0 + e];
+            const double xv = scale ? x[c] * scale[c] : x[c];
 #pragma unroll
             for (int t = 0; t < ROW_GROUP; ++t)
                 if (t < rows)
@@ -2318,18 +2371,21 @@ __device__ inline void multiplyRowsSingle(const int64_t* offsets, const int32_t*
             mine = lane == t ? sum[t] : mine;
         }
         if (lane < rows)
-            out[r0 + lane] = b ? b[r0 + lane] - mine : mine;
+        {
+            const double y = b ? b[r0 + lane] - mine : mine;
+            out[r0 + lane] = scale ? y * scale[r0 + lane] : y;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets, const int32_t* cols,
                                                          const double* values, const int32_t* groups,
                                                          int64_t groupCount, const double* x, const double* b,
-                                                         double* out, int k)
+                                                         double* out, int k, const double* scale)
 {
     if (k == 1)
     {
-        multiplyRowsSingle(offsets, cols, values, groups, groupCount, x, b, out);
+        multiplyRowsSingle(offsets, cols, values, groups, groupCount, x, b, out, scale);
         return;
     }
     // Lanes h >= k load and broadcast like the others (their x column clamped) and store nothing.
@@ -2355,10 +2411,11 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
             const int run = int(min(int64_t(16), degree - e));
             const int j = min(lane, run - 1);
             const int cj = cols[e0 + e + j];
+            const double sj = scale ? scale[cj] : 1.0;
             double vj[ROW_GROUP];
 #pragma unroll
             for (int t = 0; t < ROW_GROUP; ++t)
-                vj[t] = t < rows ? values[start[t] + e + j] : 0.0;
+                vj[t] = t < rows ? values[start[t] + e + j] * sj : 0.0;
             if (run == 16)
             {
                 double xv[16];
@@ -2387,58 +2444,17 @@ __global__ __launch_bounds__(256) void multiplyRowsKernel(const int64_t* offsets
             if (t < rows)
             {
                 const int64_t at = int64_t(r0 + t) * k + h;
-                out[at] = b ? b[at] - sum[t] : sum[t];
+                const double y = b ? b[at] - sum[t] : sum[t];
+                out[at] = scale ? y * scale[r0 + t] : y;
             }
     }
 }
 
-// Per column h the partial sums Σ_i a(i, h)·b(i, h) of this workgroup's rows, in a fixed order (deterministic).
-__global__ __launch_bounds__(256) void columnDotsKernel(const double* a, const double* b, int64_t n, int k,
-                                                       double* partial)
-{
-    __shared__ double sums[256];
-    const VectorLanes v(k);
-    double sum = 0.0;
-    if (v.h < k)
-        for (int64_t i = v.firstRow(); i < n; i += v.stride())
-            sum += a[i * k + v.h] * b[i * k + v.h];
-    sums[threadIdx.x] = sum;
-    // The row slots' sums by a fixed tree (slot s adds slot s + half), column h ending in sums[h].
-    for (int half = v.rows >> 1; half > 0; half >>= 1)
-    {
-        __syncthreads();
-        if (v.slot < half)
-            sums[threadIdx.x] += sums[threadIdx.x + (half << vectorShift(k))];
-    }
-    __syncthreads();
-    if (threadIdx.x < 16)
-        partial[blockIdx.x * 16 + threadIdx.x] = threadIdx.x < (256 / v.rows) ? sums[threadIdx.x] : 0.0;
-}
-
-// out[h] = Σ over the workgroups' partial sums: 16 threads per column sum runs of `groups / 16` workgroups, then the
-// 16 run sums are added in order (a fixed order: deterministic). groups is a multiple of 16.
-__global__ __launch_bounds__(256) void sumPartialsKernel(const double* partial, int groups, double* out)
-{
-    __shared__ double runs[256];
-    const int h = threadIdx.x & 15, run = threadIdx.x >> 4, length = groups >> 4;
-    double total = 0.0;
-#pragma unroll 8
-    for (int g = run * length; g < (run + 1) * length; ++g)
-        total += partial[g * 16 + h];
-    runs[threadIdx.x] = total;
-    __syncthreads();
-    if (threadIdx.x < 16)
-    {
-        double sum = 0.0;
-        for (int r = 0; r < 16; ++r)
-            sum += runs[r * 16 + threadIdx.x];
-        out[threadIdx.x] = sum;
-    }
-}
-
 // Per column the largest |v| (bits of a non-negative double, ordered as the values; NaN wins): the workgroup's 16
-// row slots combined in LDS, then one atomic per workgroup and column.
-__global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int64_t n, int k, unsigned long long* out)
+// row slots combined in LDS, then one atomic per workgroup and column. With a row scaling (powers of two) the
+// maximum is that of S·v (divide: S⁻¹·v).
+__global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int64_t n, int k, unsigned long long* out,
+                                                          const double* scale, int divide)
 {
     __shared__ double slots[256];
     const VectorLanes lanes(k);
@@ -2446,7 +2462,8 @@ __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int6
     if (lanes.h < k)
         for (int64_t i = lanes.firstRow(); i < n; i += lanes.stride())
         {
-            const double a = fabs(v[i * k + lanes.h]);
+            const double value = v[i * k + lanes.h];
+            const double a = fabs(scale ? (divide ? value / scale[i] : value * scale[i]) : value);
             m = a > m || a != a ? a : m;
         }
     slots[threadIdx.x] = m;
@@ -2466,7 +2483,6 @@ __global__ __launch_bounds__(256) void columnMaxRowsKernel(const double* v, int6
         atomicMax(&out[threadIdx.x], static_cast<unsigned long long>(__double_as_longlong(slots[threadIdx.x])));
 }
 
-// x += α_h·p, r −= α_h·q.
 // ---- matrix values on the device ----------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void combineValuesKernel(int32_t n, const int64_t* offsets, const int32_t* rows,
@@ -2557,11 +2573,13 @@ __global__ __launch_bounds__(256) void blockGramKernel(GramPairs g, int64_t n, i
         const int64_t i = row + (lane >> 4);
         const bool ok = i < r1 && col < k;
         const int64_t at = ok ? i * k + col : 0;
+        // A pair whose U is the first pair's takes the first pair's loaded value (one read of a shared vector).
+        const double a0 = ok ? g.u[0][at] : 0.0;
 #pragma unroll
         for (int q = 0; q < GRAM_PAIRS; ++q)
             if (q < g.pairs)
             {
-                const double a = ok ? g.u[q][at] : 0.0;
+                const double a = q == 0 || g.u[q] == g.u[0] ? a0 : (ok ? g.u[q][at] : 0.0);
                 const double b = ok ? g.v[q][at] : 0.0;
                 acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
             }
@@ -2580,22 +2598,40 @@ __global__ __launch_bounds__(256) void blockGramKernel(GramPairs g, int64_t n, i
     }
 }
 
-// out[q·256 + e] = Σ_g partial[(g·GRAM_PAIRS + q)·256 + e], the workgroups in index order (workgroup q, thread e),
-// eight loads in flight.
-__global__ __launch_bounds__(256) void sumGramPartialsKernel(const double* partial, int groups, double* out)
+// out[q·256 + e] = Σ_g partial[(g·GRAM_PAIRS + q)·256 + e] in two fixed-order stages: workgroup (q, run) adds the
+// GRAM_RUNS consecutive workgroups' partial sums of its run (all its loads in flight in four batches) into
+// runs[(q·GRAM_RUN_COUNT + run)·256 + e], then sumGramRunsKernel adds the runs in order.
+constexpr int GRAM_RUN_COUNT = 16, GRAM_RUNS = GRAM_GROUPS / GRAM_RUN_COUNT;
+static_assert(GRAM_GROUPS % (4 * GRAM_RUN_COUNT) == 0, "runs of whole batches");
+__global__ __launch_bounds__(256) void sumGramPartialsKernel(const double* partial, double* runs)
 {
-    const int q = blockIdx.x, e = threadIdx.x;
+    const int q = blockIdx.x / GRAM_RUN_COUNT, run = blockIdx.x % GRAM_RUN_COUNT, e = threadIdx.x;
+    const int g0 = run * GRAM_RUNS;
     double sum = 0.0;
-    for (int g = 0; g < groups; g += 8)
+    for (int g = 0; g < GRAM_RUNS; g += GRAM_RUNS / 4)
     {
-        double v[8];
+        double v[GRAM_RUNS / 4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-            v[t] = g + t < groups ? partial[(int64_t(g + t) * GRAM_PAIRS + q) * 256 + e] : 0.0;
+        for (int t = 0; t < GRAM_RUNS / 4; ++t)
+            v[t] = partial[(int64_t(g0 + g + t) * GRAM_PAIRS + q) * 256 + e];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int t = 0; t < GRAM_RUNS / 4; ++t)
             sum += v[t];
     }
+    runs[(q * GRAM_RUN_COUNT + run) * 256 + e] = sum;
+}
+
+__global__ __launch_bounds__(256) void sumGramRunsKernel(const double* runs, double* out)
+{
+    const int q = blockIdx.x, e = threadIdx.x;
+    double v[GRAM_RUN_COUNT];
+#pragma unroll
+    for (int t = 0; t < GRAM_RUN_COUNT; ++t)
+        v[t] = runs[(q * GRAM_RUN_COUNT + t) * 256 + e];
+    double sum = 0.0;
+#pragma unroll
+    for (int t = 0; t < GRAM_RUN_COUNT; ++t)
+        sum += v[t];
     out[q * 256 + e] = sum;
 }
 
@@ -2633,18 +2669,99 @@ __global__ __launch_bounds__(256) void blockCombineKernel(double* y, BlockTerm m
     }
 }
 
-__global__ __launch_bounds__(256) void cgStepKernel(double* x, double* r, const double* p, const double* q,
-                                                   ColumnCoefficients alpha, int64_t n, int k)
+// The step of block conjugate gradients in one pass, a wave per 16 rows (grid-stride): with W (in z) and A·W (in p)
+// and the coefficient blocks T and U = T·α (16 × 16 each, row a, column j, at coef and coef + 256 on the device),
+//   Q = (A·W)·T,  P = W·T,  X += W·U,  R −= (A·W)·U,  Z = R,
+// each product by four 16 × 16 × 4 FP64 MFMA steps over the k columns (the operands' lanes as in
+// blockCombineKernel). P overwrites A·W and Z overwrites W row block by row block, after the wave has read them.
+// The column maxima of |S⁻¹·R| and |S·X| (the backward error's norms of the unscaled vectors) go to norms[0, 16)
+// and norms[16, 32) (bits, atomicMax; zeroed by the caller) once per workgroup.
+__global__ __launch_bounds__(256) void blockStepKernel(double* z, double* p, double* q, double* x, double* r,
+                                                       const double* coef, const double* scale, int64_t n, int k,
+                                                       unsigned long long* norms)
 {
-    const VectorLanes v(k);
-    const int h = v.h;
-    if (h >= k)
-        return;
-    const double a = alpha.v[h];
-    for (int64_t i = v.firstRow(); i < n; i += v.stride())
+    __shared__ double slots[2][4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    // This lane's B operands: T(4s + lq, lr) and U(4s + lq, lr).
+    double bt[4], bu[4];
+#pragma unroll
+    for (int step = 0; step < 4; ++step)
     {
-        x[i * k + h] += a * p[i * k + h];
-        r[i * k + h] -= a * q[i * k + h];
+        const int a = 4 * step + lq;
+        const bool valid = a < k && lr < k;
+        bt[step] = valid ? coef[a * 16 + lr] : 0.0;
+        bu[step] = valid ? coef[256 + a * 16 + lr] : 0.0;
+    }
+    double maxR = 0.0, maxX = 0.0;
+    const int64_t blocks = (n + 15) / 16;
+    for (int64_t blk = int64_t(blockIdx.x) * 4 + wave; blk < blocks; blk += int64_t(gridDim.x) * 4)
+    {
+        const int64_t row0 = blk * 16, i = row0 + lr;
+        double aw[4], w[4];
+#pragma unroll
+        for (int step = 0; step < 4; ++step)
+        {
+            const int a = 4 * step + lq;
+            const bool valid = i < n && a < k;
+            aw[step] = valid ? p[i * k + a] : 0.0;
+            w[step] = valid ? z[i * k + a] : 0.0;
+        }
+        double4_t dq = {0, 0, 0, 0}, dp = {0, 0, 0, 0}, dx = {0, 0, 0, 0}, dr = {0, 0, 0, 0};
+#pragma unroll
+        for (int step = 0; step < 4; ++step)
+        {
+            dq = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[step], bt[step], dq, 0, 0, 0);
+            dp = __builtin_amdgcn_mfma_f64_16x16x4f64(w[step], bt[step], dp, 0, 0, 0);
+            dx = __builtin_amdgcn_mfma_f64_16x16x4f64(w[step], bu[step], dx, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[step], bu[step], dr, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int64_t row = row0 + lq + 4 * v;
+            if (row < n && lr < k)
+            {
+                const int64_t at = row * k + lr;
+                const double xn = x[at] + dx[v];
+                const double rn = r[at] - dr[v];
+                q[at] = dq[v];
+                p[at] = dp[v];
+                x[at] = xn;
+                r[at] = rn;
+                z[at] = rn;
+                const double s = scale ? scale[row] : 1.0;
+                const double ar = fabs(rn / s), ax = fabs(xn * s);
+                maxR = ar > maxR || ar != ar ? ar : maxR;
+                maxX = ax > maxX || ax != ax ? ax : maxX;
+            }
+        }
+    }
+    // Column lr's maxima over the wave's four row quarters, then over the four waves.
+#pragma unroll
+    for (int offset = 16; offset < 64; offset <<= 1)
+    {
+        const double or_ = __shfl_xor(maxR, offset), ox = __shfl_xor(maxX, offset);
+        maxR = or_ > maxR || or_ != or_ ? or_ : maxR;
+        maxX = ox > maxX || ox != ox ? ox : maxX;
+    }
+    if (lq == 0)
+    {
+        slots[0][wave][lr] = maxR;
+        slots[1][wave][lr] = maxX;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32)
+    {
+        const int which = threadIdx.x >> 4, h = threadIdx.x & 15;
+        double m = slots[which][0][h];
+        for (int w2 = 1; w2 < 4; ++w2)
+        {
+            const double a = slots[which][w2][h];
+            m = a > m || a != a ? a : m;
+        }
+        if (h < k)
+            atomicMax(&norms[16 * which + h], static_cast<unsigned long long>(__double_as_longlong(m)));
     }
 }
 
@@ -2657,19 +2774,6 @@ __global__ __launch_bounds__(256) void addRowsKernel(double* x, const double* z,
         return;
     for (int64_t i = v.firstRow(); i < n; i += v.stride())
         x[i * k + h] += z[i * k + h];
-}
-
-// p = z + β_h·p.
-__global__ __launch_bounds__(256) void cgDirectionKernel(double* p, const double* z, ColumnCoefficients beta,
-                                                        int64_t n, int k)
-{
-    const VectorLanes v(k);
-    const int h = v.h;
-    if (h >= k)
-        return;
-    const double b = beta.v[h];
-    for (int64_t i = v.firstRow(); i < n; i += v.stride())
-        p[i * k + h] = z[i * k + h] + b * p[i * k + h];
 }
 
 // dst(i, h) = src(i, h)·scale[i] (scale null: a copy).
@@ -3008,22 +3112,13 @@ hipError_t configureKernels()
 
 hipError_t launchMultiplyRows(const int64_t* offsets, const int32_t* cols, const double* values,
                               const int32_t* groups, int64_t groupCount, const double* x, const double* b, double* out,
-                              int k, hipStream_t stream)
+                              int k, hipStream_t stream, const double* scale)
 {
     // A workgroup per VECTOR_ROWS row groups (one pass), the grid a multiple of the 8 XCDs.
     const int64_t blocks = std::min<int64_t>((groupCount + VECTOR_ROWS - 1) / VECTOR_ROWS, int64_t(1) << 24);
     const int grid = static_cast<int>(std::max<int64_t>(8, (blocks + 7) / 8 * 8));
     hipLaunchKernelGGL(multiplyRowsKernel, dim3(grid), dim3(256), 0, stream, offsets, cols, values, groups,
-                       groupCount, x, b, out, k);
-    return hipGetLastError();
-}
-
-hipError_t launchColumnDots(const double* a, const double* b, int64_t n, int k, double* partial, double* out,
-                            hipStream_t stream)
-{
-    hipLaunchKernelGGL(columnDotsKernel, dim3(DOT_GROUPS), dim3(256), 0, stream, a, b, n, k, partial);
-    static_assert(DOT_GROUPS % 16 == 0, "runs of the partial sums");
-    hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(256), 0, stream, partial, DOT_GROUPS, out);
+                       groupCount, x, b, out, k, scale);
     return hipGetLastError();
 }
 
@@ -3349,7 +3444,9 @@ hipError_t launchBlockGram(const GramPairs& pairs, int64_t n, int k, double* par
                            hipStream_t stream)
 {
     hipLaunchKernelGGL(blockGramKernel, dim3(GRAM_GROUPS), dim3(256), 0, stream, pairs, n, k, partial);
-    hipLaunchKernelGGL(sumGramPartialsKernel, dim3(pairs.pairs), dim3(256), 0, stream, partial, GRAM_GROUPS, out);
+    double* runs = partial + int64_t(GRAM_GROUPS) * GRAM_PAIRS * 256;
+    hipLaunchKernelGGL(sumGramPartialsKernel, dim3(pairs.pairs * GRAM_RUN_COUNT), dim3(256), 0, stream, partial, runs);
+    hipLaunchKernelGGL(sumGramRunsKernel, dim3(pairs.pairs), dim3(256), 0, stream, runs, out);
     return hipGetLastError();
 }
 
@@ -3361,30 +3458,26 @@ hipError_t launchBlockCombine(double* y, const BlockTerm& term, int64_t n, int k
     return hipGetLastError();
 }
 
-hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream)
+hipError_t launchBlockStep(double* z, double* p, double* q, double* x, double* r, const double* coef,
+                           const double* scale, int64_t n, int k, unsigned long long* norms, hipStream_t stream)
 {
-    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(std::min(vectorGroups(n, k), 1024)), dim3(256), 0, stream, v, n, k,
-                       out);
+    const int64_t groups = ((n + 15) / 16 + 3) / 4;
+    const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(groups, 2048)));
+    hipLaunchKernelGGL(blockStepKernel, dim3(grid), dim3(256), 0, stream, z, p, q, x, r, coef, scale, n, k, norms);
     return hipGetLastError();
 }
 
-hipError_t launchCgStep(double* x, double* r, const double* p, const double* q, const ColumnCoefficients& alpha,
-                        int64_t n, int k, hipStream_t stream)
+hipError_t launchColumnMaxRows(const double* v, int64_t n, int k, unsigned long long* out, hipStream_t stream,
+                               const double* scale, int divide)
 {
-    hipLaunchKernelGGL(cgStepKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, x, r, p, q, alpha, n, k);
+    hipLaunchKernelGGL(columnMaxRowsKernel, dim3(std::min(vectorGroups(n, k), 1024)), dim3(256), 0, stream, v, n, k,
+                       out, scale, divide);
     return hipGetLastError();
 }
 
 hipError_t launchAddRows(double* x, const double* z, int64_t n, int k, hipStream_t stream)
 {
     hipLaunchKernelGGL(addRowsKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, x, z, n, k);
-    return hipGetLastError();
-}
-
-hipError_t launchCgDirection(double* p, const double* z, const ColumnCoefficients& beta, int64_t n, int k,
-                             hipStream_t stream)
-{
-    hipLaunchKernelGGL(cgDirectionKernel, dim3(vectorGroups(n, k)), dim3(256), 0, stream, p, z, beta, n, k);
     return hipGetLastError();
 }
 
@@ -3414,6 +3507,33 @@ __device__ inline PivotTerms pivotTerms(const double* diag, const double* scale,
     const double pivot = scale != nullptr ? diag[j] / (scale[j] * scale[j]) : diag[j];
     const double a = reference != nullptr && reference[j] >= 0.0 ? reference[j] : diagonal[j];
     return {pivot, a / pivot};
+}
+
+// The pivot growth of each supernode: the largest ratio |a_jj / d_j| (pivotTerms) over its columns, a wave per
+// supernode (0 for the supernodes whose columns are not all eliminated; infinity where a ratio is not a number).
+__global__ __launch_bounds__(256) void supernodeGrowthKernel(const double* diag, const double* scale,
+                                                             const double* diagonal, const int32_t* first,
+                                                             const int32_t* ncol, int32_t count, int64_t eliminated,
+                                                             float* out)
+{
+    const int32_t s = int32_t(blockIdx.x) * 4 + int32_t(threadIdx.x >> 6);
+    if (s >= count)
+        return; // the whole wave
+    const int lane = threadIdx.x & 63;
+    const int32_t f = first[s], c = ncol[s];
+    float m = 0.0f;
+    if (int64_t(f) + c <= eliminated)
+        for (int i = lane; i < c; i += 64)
+        {
+            const double ratio = fabs(pivotTerms(diag, scale, diagonal, nullptr, f + i).ratio);
+            const float g = isnan(ratio) ? INFINITY : float(fmin(ratio, 3.0e38));
+            m = fmaxf(m, g);
+        }
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1)
+        m = fmaxf(m, __shfl_xor(m, offset));
+    if (lane == 0)
+        out[s] = m;
 }
 
 __global__ void pivotStatisticsInitKernel(PivotStatistics* out)
@@ -3519,6 +3639,17 @@ hipError_t launchPivotStatistics(const double* diag, const double* scale, const 
         hipLaunchKernelGGL(pivotLargestKernel, dim3(groups), dim3(256), 0, stream, diag, scale, diagonal, reference,
                            n, out);
     }
+    return hipGetLastError();
+}
+
+hipError_t launchSupernodeGrowth(const double* diag, const double* scale, const double* diagonal, const int32_t* first,
+                                 const int32_t* ncol, int32_t count, int64_t eliminated, float* out,
+                                 hipStream_t stream)
+{
+    if (count <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(supernodeGrowthKernel, dim3((count + 3) / 4), dim3(256), 0, stream, diag, scale, diagonal, first,
+                       ncol, count, eliminated, out);
     return hipGetLastError();
 }
 

@@ -497,8 +497,13 @@ struct oosg_plan
         pivotMaps, pivotRanges, pivotInverseItems;
     DeviceBuffer solveX, solveU, solveD, solveR, solveNorms;
     DeviceBuffer solvePartials; // the sweeps' slots of partial sums (partialSlots × 64 × 16)
-    // The conjugate-gradient vectors (row-major n × k): b, x, r, z, p, q; the dot products' partial sums.
-    DeviceBuffer cgB, cgX, cgR, cgZ, cgP, cgQ, cgPartial, cgDots;
+    // The conjugate-gradient vectors (row-major n × k): b, x, r, p, q; the Gram products' partial sums, the Gram
+    // products and the step's coefficient blocks.
+    DeviceBuffer cgB, cgX, cgR, cgP, cgQ, cgPartial, cgDots;
+    // The pivot growth of each supernode in the last (positive definite) factorisation of this plan, once there was
+    // one (growthRecorded): the learned precision selector's history of the plan.
+    DeviceBuffer supernodeGrowth;
+    bool growthRecorded = false;
     // Launch sequences replayed as graphs (off while the profiler times single launches): the factorisation per
     // precision, and the sweeps.
     bool graphs = true;
@@ -560,9 +565,6 @@ struct oosg_matrix
     std::vector<int32_t> rows;
     std::vector<double> diagonal; // a_jj, 0 when not stored
     std::vector<double> hostValues; // the values as last given
-    // Statistics of the values for the precision selector: mean and minimum of log10(|a_jj| / Σ_i≠j |a_ij|) (clamped
-    // to [−3, 3]), the fraction of equations whose diagonal does not dominate, the smallest and largest positive a_jj.
-    double meanDominance = 0, minDominance = 3, weakFraction = 0, diagonalMin = INFINITY, diagonalMax = 0;
     std::vector<int64_t> csrFromCsc; // per CSR entry, its CSC entry
     double normInf = 0.0;
     DeviceBuffer dOffsets, dRows, dValues, dPosition, dColumn, dScale, dCsrOffsets, dCsrCols, dCsrValues;
@@ -1884,12 +1886,12 @@ std::vector<uint64_t> solveBufferBytes(const oosg_plan& p)
             static_cast<uint64_t>(std::max<int64_t>(p.rowStart[p.S], 1)) * 16 * sizeof(double), // solveU
             static_cast<uint64_t>(std::max<int64_t>(p.partialSlots, 1)) * 64 * 16 * sizeof(double), // solvePartials
             3 * 16 * sizeof(unsigned long long),                                            // solveNorms
-            vector, vector, vector, vector, vector, vector,                                 // cgB .. cgQ
-            static_cast<uint64_t>(std::max(oosg::DOT_GROUPS * 16, oosg::GRAM_GROUPS * oosg::GRAM_PAIRS * 256)) *
-                sizeof(double), // cgPartial
+            vector, vector, vector, vector, vector,                                         // cgB .. cgQ
+            static_cast<uint64_t>((oosg::GRAM_GROUPS + 16) * oosg::GRAM_PAIRS * 256) * sizeof(double), // cgPartial
             static_cast<uint64_t>(oosg::GRAM_PAIRS) * 256 * sizeof(double),                  // cgDots
             static_cast<uint64_t>(std::max(p.S, 1)) * sizeof(int32_t),                      // pivotStatus
-            sizeof(oosg::PivotStatistics)};                                                 // pivotStats
+            sizeof(oosg::PivotStatistics),                                                  // pivotStats
+            static_cast<uint64_t>(std::max(p.S, 1)) * sizeof(float)};                       // supernodeGrowth
 }
 
 // Everything a plan reserves: its arrays, the update matrices, one factor's storage, the solves' vectors.
@@ -1906,8 +1908,8 @@ void reserveSolveBuffers(oosg_plan& p)
 {
     const std::vector<uint64_t> bytes = solveBufferBytes(p);
     DeviceBuffer* buffers[] = {&p.solveX, &p.solveU, &p.solvePartials, &p.solveNorms, &p.cgB, &p.cgX,
-                               &p.cgR,    &p.cgZ,    &p.cgP,          &p.cgQ,         &p.cgPartial, &p.cgDots,
-                               &p.pivotStatus, &p.pivotStats};
+                               &p.cgR,    &p.cgP,    &p.cgQ,          &p.cgPartial,   &p.cgDots,
+                               &p.pivotStatus, &p.pivotStats, &p.supernodeGrowth};
     for (size_t k = 0; k < bytes.size(); ++k)
         ensureCounted(p, *buffers[k], bytes[k], "plan: solve vectors");
 }
@@ -1940,7 +1942,21 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
     const int2* inverseItems = p.dInverseItems.as<int2>();
     const oosg::UpdateTile* tiles = p.dTiles.as<oosg::UpdateTile>();
     const bool timing = p.timeLaunches;
+    // The sweep ahead (a combined factorisation and solve): its levels are launched one at a time between the
+    // factorisation's launches, AHEAD_SPREAD of them apart, so that the host's launches of the factorisation's top
+    // levels do not wait behind all of the sweep's (the GPU runs both: the sweep's stream waits for the levels below
+    // aheadLevel only).
+    constexpr int AHEAD_SPREAD = 4;
     bool aheadIssued = false;
+    int32_t aheadNext = 0; // the next level of the sweep to launch
+    int aheadWait = 0;     // factorisation launches before it
+    const auto aheadLevelLaunch = [&]()
+    {
+        forwardSweep<T>(p, panels, linv, p.aheadK, aheadNext, aheadNext + 1, p.sweepStream);
+        if (++aheadNext == p.aheadLevel)
+            check(hipEventRecord(p.aheadEvents[LANES], p.sweepStream), "factor: sweep ahead");
+        aheadWait = AHEAD_SPREAD;
+    };
     if (timing)
         while (p.launchEvents.size() < 2 * p.schedule.size())
         {
@@ -2004,9 +2020,10 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
                 check(hipEventRecord(p.aheadEvents[q], q == 0 ? st : p.lanes[q]), "factor: sweep ahead");
                 check(hipStreamWaitEvent(p.sweepStream, p.aheadEvents[q], 0), "factor: sweep ahead");
             }
-            forwardSweep<T>(p, panels, linv, p.aheadK, 0, p.aheadLevel, p.sweepStream);
-            check(hipEventRecord(p.aheadEvents[LANES], p.sweepStream), "factor: sweep ahead");
+            aheadLevelLaunch();
         }
+        else if (aheadIssued && aheadNext < p.aheadLevel && l.kind != LaunchKind::Sync && --aheadWait <= 0)
+            aheadLevelLaunch();
         const bool lanes = concurrent && p.lanes[l.lane] != nullptr;
         const hipStream_t ls = lanes ? p.lanes[l.lane] : st;
         if (lanes && l.wait >= 0)
@@ -2072,6 +2089,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
         if (timing)
             check(hipEventRecord(p.launchEvents[2 * li + 1], ls), "hipEventRecord");
     }
+    while (aheadIssued && aheadNext < p.aheadLevel)
+        aheadLevelLaunch();
     if (p.outOfCore && current >= 0)
     {
         spoolLevel(current);
@@ -3012,60 +3031,13 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     }
 }
 
-// The selector's statistics of a matrix's values (oosg_matrix::meanDominance and the following fields).
-void valueStatistics(oosg_matrix& m)
-{
-    const int32_t n = m.n;
-    std::vector<double> off(static_cast<size_t>(n), 0.0);
-    for (int32_t j = 0; j < n; ++j)
-        for (int64_t k = m.offsets[j]; k < m.offsets[j + 1]; ++k)
-            if (m.rows[k] != j)
-            {
-                off[m.rows[k]] += std::fabs(m.hostValues[k]);
-                off[j] += std::fabs(m.hostValues[k]);
-            }
-    m.meanDominance = 0;
-    m.minDominance = 3;
-    m.diagonalMin = INFINITY;
-    m.diagonalMax = 0;
-    int64_t weak = 0;
-    for (int32_t j = 0; j < n; ++j)
-    {
-        const double a = m.diagonal[j];
-        if (a > 0.0 && std::isfinite(a))
-        {
-            m.diagonalMin = std::min(m.diagonalMin, a);
-            m.diagonalMax = std::max(m.diagonalMax, a);
-        }
-        double dominance = off[j] > 0.0 ? std::log10(std::fabs(a) / off[j]) : 3.0;
-        dominance = std::min(3.0, std::max(-3.0, dominance != dominance ? -3.0 : dominance));
-        m.meanDominance += dominance;
-        m.minDominance = std::min(m.minDominance, dominance);
-        weak += std::fabs(a) < off[j] ? 1 : 0;
-    }
-    m.meanDominance /= std::max(n, 1);
-    m.weakFraction = static_cast<double>(weak) / std::max(n, 1);
-}
-
-// Row-major n × k vectors (k <= 16) of the conjugate-gradient solve.
-std::vector<double> columnDots(oosg_plan& p, const double* a, const double* b, int k)
-{
-    check(oosg::launchColumnDots(a, b, p.n, k, p.cgPartial.as<double>(), p.cgDots.as<double>(), p.stream),
-          "solve: dot products");
-    std::vector<double> out(16);
-    check(hipMemcpyAsync(out.data(), p.cgDots.ptr, 16 * sizeof(double), hipMemcpyDeviceToHost, p.stream),
-          "solve: dot products");
-    check(hipStreamSynchronize(p.stream), "solve: dot products");
-    out.resize(static_cast<size_t>(k));
-    return out;
-}
-
-// Column maxima |v| of row-major n × k vectors into slot `slot` (of 3) of the plan's norms, on the stream.
-void startColumnMax(oosg_plan& p, const double* v, int k, int slot)
+// Column maxima |v| of row-major n × k vectors into slot `slot` (of 3) of the plan's norms, on the stream; with a
+// row scaling S, of S·v (divide: S⁻¹·v).
+void startColumnMax(oosg_plan& p, const double* v, int k, int slot, const double* scale = nullptr, int divide = 0)
 {
     unsigned long long* out = p.solveNorms.as<unsigned long long>() + 16 * slot;
     check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(out), 32, 0, p.stream), "solve: norms");
-    check(oosg::launchColumnMaxRows(v, p.n, k, out, p.stream), "solve: norms");
+    check(oosg::launchColumnMaxRows(v, p.n, k, out, p.stream, scale, divide), "solve: norms");
 }
 
 // The first `slots` slots of column maxima, read back with one copy and one wait.
@@ -3096,133 +3068,11 @@ void ensureSolveBuffers(oosg_plan& p)
     reserveSolveBuffers(p);
 }
 
-// out ← factor⁻¹·in for row-major n × k vectors (k <= 16); the FP32 factor's scaling S applied on both sides. With
-// fromLevel > 0, p.solveX holds S·in already, swept forward through the levels below fromLevel.
-void applyFactorRows(const oosg_factor& f, const double* in, double* out, int k, int32_t fromLevel = 0)
-{
-    oosg_plan& p = *f.plan;
-    const int64_t n = p.n;
-    ensureSolveBuffers(p);
-    const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
-    if (fromLevel == 0)
-        check(oosg::launchScaleRows(in, p.solveX.as<double>(), scale, n, k, p.stream), "solve: scaling");
-    sweeps(f, k, fromLevel);
-    check(oosg::launchScaleRows(p.solveX.as<double>(), out, scale, n, k, p.stream), "solve: scaling");
-}
-
 // The residual product of a step: its bytes for the profile.
 double residualBytes(const oosg_matrix& m, int64_t n, int k)
 {
     return static_cast<double>(m.csrFromCsc.size()) * 12.0 + static_cast<double>(n + 1) * 8.0 +
            2.0 * static_cast<double>(n) * k * 8.0;
-}
-
-// Conjugate gradients on A (symmetric positive definite) preconditioned by the factor (FP32 fronts of the scaled
-// matrix): per column h, x_0 = 0, r_0 = b, z = M⁻¹r, p = z; α = (r·z)/(p·Ap), x += αp, r −= αAp; β = (r'·z')/(r·z),
-// p = z' + βp. When the recurrence's residual meets the target, the true residual b − A·x decides (and replaces the
-// recurrence's when it does not). A converged column stops moving (α = β = 0). Returns the solves with the factor.
-void solvePcg(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
-              double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
-{
-    oosg_plan& p = *f.plan;
-    const hipStream_t st = p.stream;
-    const int64_t n = p.n;
-    double* B = p.cgB.as<double>();
-    double* X = p.cgX.as<double>();
-    double* R = p.cgR.as<double>();
-    double* Z = p.cgZ.as<double>();
-    double* P = p.cgP.as<double>();
-    double* Q = p.cgQ.as<double>();
-    const int64_t* offsets = m.dCsrOffsets.as<int64_t>();
-    const int32_t* cols = m.dCsrCols.as<int32_t>();
-    const double* values = m.dCsrValues.as<double>();
-    const int32_t* groups = m.dCsrGroups.as<int32_t>();
-    const int64_t groupCount = static_cast<int64_t>(m.csrGroups.size()) - 1;
-    const size_t vectorBytes = static_cast<size_t>(n) * k * sizeof(double);
-
-    // With aheadLevel > 0, B holds b already and the first solve's forward sweep is done below that level.
-    if (aheadLevel == 0)
-        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
-    const std::vector<double> bNorm = columnMaxRows(p, B, k);
-    check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
-          "solve: zero");
-    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
-    applyFactorRows(f, R, Z, k, aheadLevel);
-    int32_t solves = 1;
-    check(hipMemcpyAsync(P, Z, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: direction");
-    std::vector<double> rho = columnDots(p, R, Z, k);
-    std::vector<char> active(static_cast<size_t>(k), 1);
-    std::vector<double> eta(static_cast<size_t>(k), INFINITY);
-    const auto backwardErrors = [&](const double* residual)
-    {
-        const std::vector<double> rN = columnMaxRows(p, residual, k);
-        const std::vector<double> xN = columnMaxRows(p, X, k);
-        std::vector<double> e(static_cast<size_t>(k));
-        for (int h = 0; h < k; ++h)
-        {
-            const double denominator = m.normInf * xN[h] + bNorm[h];
-            const double v = denominator > 0 ? rN[h] / denominator : rN[h];
-            e[h] = v != v ? INFINITY : v;
-        }
-        return e;
-    };
-    bool converged = false;
-    for (int32_t iteration = 0;; ++iteration)
-    {
-        const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
-        check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, P, nullptr, Q, k, st),
-              "solve: product");
-        p.profiler.end(slot, st);
-        const std::vector<double> sigma = columnDots(p, P, Q, k);
-        oosg::ColumnCoefficients alpha{};
-        for (int h = 0; h < k; ++h)
-            alpha.v[h] = active[h] && sigma[h] != 0.0 ? rho[h] / sigma[h] : 0.0;
-        check(oosg::launchCgStep(X, R, P, Q, alpha, n, k, st), "solve: step");
-        const std::vector<double> estimate = backwardErrors(R);
-        bool check_ = true;
-        for (int h = 0; h < k; ++h)
-            if (active[h] && !(estimate[h] <= target))
-                check_ = false;
-        if (check_ || solves >= maxSteps)
-        {
-            // The true residual, into Q.
-            const int slot2 = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
-            check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, X, B, Q, k, st),
-                  "solve: residual");
-            p.profiler.end(slot2, st);
-            const std::vector<double> trueEta = backwardErrors(Q);
-            converged = true;
-            for (int h = 0; h < k; ++h)
-            {
-                if (active[h])
-                    eta[h] = trueEta[h];
-                if (active[h] && trueEta[h] <= target)
-                    active[h] = 0;
-                converged = converged && !active[h];
-            }
-            if (converged || solves >= maxSteps)
-                break;
-            // Residual replacement: the recurrence drifted from b − A·x.
-            check(hipMemcpyAsync(R, Q, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
-        }
-        applyFactorRows(f, R, Z, k);
-        ++solves;
-        const std::vector<double> rhoNew = columnDots(p, R, Z, k);
-        oosg::ColumnCoefficients beta{};
-        for (int h = 0; h < k; ++h)
-        {
-            beta.v[h] = active[h] && rho[h] != 0.0 ? rhoNew[h] / rho[h] : 0.0;
-            if (active[h])
-                rho[h] = rhoNew[h];
-        }
-        check(oosg::launchCgDirection(P, Z, beta, n, k, st), "solve: direction");
-    }
-    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
-    report.steps = solves;
-    report.converged = converged ? 1 : 0;
-    report.backward_error = 0.0;
-    for (int h = 0; h < k; ++h)
-        report.backward_error = std::max(report.backward_error, eta[h]);
 }
 
 // ---- block conjugate gradients ------------------------------------------------------------------------------------
@@ -3304,26 +3154,19 @@ std::vector<double> orthonormalizer(const double* G, int k)
     return T;
 }
 
-oosg::BlockTerm blockTerm(const double* u, double c0, const std::vector<double>& c)
-{
-    oosg::BlockTerm t{};
-    t.u = u;
-    t.deviceC = nullptr;
-    t.scale = 1.0;
-    t.c0 = c0;
-    std::copy(c.begin(), c.end(), t.c);
-    return t;
-}
-
 // Block conjugate gradients on A (symmetric positive definite) preconditioned by the factor, over the k columns at
-// once (O'Leary, "The block conjugate gradient algorithm", Linear Algebra Appl. 29, 1980): the search block P is
-// kept A-orthonormal (Pᵀ·A·P = I), so α = Pᵀ·R and β = −(A·P)ᵀ·Z. With Z = M⁻¹·R:
-//   W = Z − P·((A·P)ᵀ·Z),  G = Wᵀ·A·W,  T with Tᵀ·G·T = I (dependent directions dropped),
-//   P = W·T, A·P = (A·W)·T,  α = Pᵀ·R,  X += P·α,  R −= (A·P)·α.
-// The k columns share one Krylov space: the few directions where the FP32 factor is least accurate (the smooth
-// modes of a thin plate) serve every right-hand side, so it needs fewer solves than k separate recurrences. The
-// stopping rule is solvePcg's: the recurrence's residual first, then the true residual b − A·x, which replaces the
-// recurrence's when it has drifted. Buffers: Z holds W, P holds A·W before it holds P, Q holds A·P.
+// once (O'Leary, "The block conjugate gradient algorithm", Linear Algebra Appl. 29, 1980), run on the scaled system
+// A_s·y = S·b, A_s = S·A·S, x = S·y, whose factor the FP32 fronts hold (S: powers of two, so every scaling is exact
+// and no vector is rescaled between the steps). The search block P is kept A_s-orthonormal (Pᵀ·A_s·P = I), so
+// α = Pᵀ·R and β = −(A_s·P)ᵀ·Z. With Z = A_s⁻¹·R (the sweeps, in place on the sweep vectors):
+//   W = Z − P·((A_s·P)ᵀ·Z),  G = Wᵀ·A_s·W,  T with Tᵀ·G·T = I (dependent directions dropped),
+//   P = W·T, A_s·P = (A_s·W)·T,  α = Pᵀ·R,  Y += P·α,  R −= (A_s·P)·α,
+// the last four and the copy of R into the sweep vectors in one pass (launchBlockStep), which also takes the column
+// maxima of the unscaled residual S⁻¹·R and solution S·Y. The k columns share one Krylov space: the few directions
+// where the FP32 factor is least accurate (the smooth modes of a thin plate) serve every right-hand side. Stopping
+// rule: the recurrence's backward error first; once it meets the target, the true residual S·(b − A·x) decides, and
+// replaces the recurrence's when it does not. Buffers: the sweep vectors hold W, P holds A_s·W before it holds P,
+// Q holds A_s·P.
 void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
                    double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
 {
@@ -3333,11 +3176,14 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
     double* B = p.cgB.as<double>();
     double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
-    double* Z = p.cgZ.as<double>();
+    double* Z = p.solveX.as<double>();
     double* P = p.cgP.as<double>();
     double* Q = p.cgQ.as<double>();
     double* grams = p.cgDots.as<double>();
+    double* coef = grams + 512; // T and U = T·α, after the two Gram products
     double* partial = p.cgPartial.as<double>();
+    unsigned long long* norms = p.solveNorms.as<unsigned long long>();
+    const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
     const int64_t* offsets = m.dCsrOffsets.as<int64_t>();
     const int32_t* cols = m.dCsrCols.as<int32_t>();
     const double* values = m.dCsrValues.as<double>();
@@ -3347,7 +3193,7 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
     const auto product = [&](const double* in, const double* minus, double* out)
     {
         const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
-        check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, in, minus, out, k, st),
+        check(oosg::launchMultiplyRows(offsets, cols, values, groups, groupCount, in, minus, out, k, st, scale),
               "solve: product");
         p.profiler.end(slot, st);
     };
@@ -3364,22 +3210,25 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
         check(oosg::launchBlockGram(g, n, k, partial, grams, st), "solve: Gram products");
     };
 
+    // With aheadLevel > 0, B holds b and the sweep vectors S·b, swept forward below that level.
     if (aheadLevel == 0)
+    {
         check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
+        check(oosg::launchScaleRows(B, Z, scale, n, k, st), "solve: scaling");
+    }
     const std::vector<double> bNorm = columnMaxRows(p, B, k);
     check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
           "solve: zero");
-    check(hipMemcpyAsync(R, B, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
-    const auto backwardErrors = [&](const double* residual)
+    check(oosg::launchScaleRows(B, R, scale, n, k, st), "solve: residual");
+    // The backward errors from the norms of slots 1 (the residual) and 2 (the solution).
+    const auto backwardErrors = [&]()
     {
-        startColumnMax(p, residual, k, 1);
-        startColumnMax(p, X, k, 2);
-        const std::vector<std::vector<double>> norms = finishColumnMax(p, 3, k);
+        const std::vector<std::vector<double>> norms3 = finishColumnMax(p, 3, k);
         std::vector<double> e(static_cast<size_t>(k));
         for (int h = 0; h < k; ++h)
         {
-            const double denominator = m.normInf * norms[2][h] + bNorm[h];
-            const double v = denominator > 0 ? norms[1][h] / denominator : norms[1][h];
+            const double denominator = m.normInf * norms3[2][h] + bNorm[h];
+            const double v = denominator > 0 ? norms3[1][h] / denominator : norms3[1][h];
             e[h] = v != v ? INFINITY : v;
         }
         return e;
@@ -3389,8 +3238,8 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
     bool converged = false;
     for (int32_t iteration = 0;; ++iteration)
     {
-        // Z = M⁻¹·R, then W = Z − P·((A·P)ᵀ·Z) in Z.
-        applyFactorRows(f, R, Z, k, iteration == 0 ? aheadLevel : 0);
+        // Z = A_s⁻¹·R, then W = Z − P·((A_s·P)ᵀ·Z) in Z.
+        sweeps(f, k, iteration == 0 ? aheadLevel : 0);
         ++solves;
         if (iteration > 0)
         {
@@ -3402,16 +3251,18 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
             t.c0 = 1.0;
             check(oosg::launchBlockCombine(Z, t, n, k, st), "solve: directions");
         }
-        // A·W (in P), G = Wᵀ·A·W and F = Wᵀ·R, read back.
+        // A_s·W (in P), G = Wᵀ·A_s·W and F = Wᵀ·R, read back.
         product(Z, nullptr, P);
         gram({{Z, P}, {Z, R}});
         std::vector<double> G(512);
         check(hipMemcpyAsync(G.data(), grams, G.size() * sizeof(double), hipMemcpyDeviceToHost, st),
               "solve: Gram products");
         check(hipStreamSynchronize(st), "solve: Gram products");
+        // T, then U = T·α with α = Tᵀ·F (16 × 16, row j, column h): U = T·Tᵀ·F.
+        std::vector<double> coefficients(512, 0.0);
         const std::vector<double> T = orthonormalizer(G.data(), k);
-        // α = Tᵀ·F (16 × 16, row j, column h).
-        std::vector<double> alpha(256, 0.0), minusAlpha(256, 0.0);
+        std::copy(T.begin(), T.end(), coefficients.begin());
+        std::vector<double> alpha(256, 0.0);
         for (int j = 0; j < k; ++j)
             for (int h = 0; h < k; ++h)
             {
@@ -3419,22 +3270,32 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
                 for (int a = 0; a < k; ++a)
                     sum += T[a * 16 + j] * G[256 + a * 16 + h];
                 alpha[j * 16 + h] = sum;
-                minusAlpha[j * 16 + h] = -sum;
             }
-        // A·P = (A·W)·T into Q, P = W·T, then X += P·α, R −= (A·P)·α.
-        check(oosg::launchBlockCombine(Q, blockTerm(P, 0.0, T), n, k, st), "solve: directions");
-        check(oosg::launchBlockCombine(P, blockTerm(Z, 0.0, T), n, k, st), "solve: directions");
-        check(oosg::launchBlockCombine(X, blockTerm(P, 1.0, alpha), n, k, st), "solve: step");
-        check(oosg::launchBlockCombine(R, blockTerm(Q, 1.0, minusAlpha), n, k, st), "solve: step");
-        const std::vector<double> estimate = backwardErrors(R);
+        for (int a = 0; a < k; ++a)
+            for (int h = 0; h < k; ++h)
+            {
+                double sum = 0.0;
+                for (int j = 0; j < k; ++j)
+                    sum += T[a * 16 + j] * alpha[j * 16 + h];
+                coefficients[256 + a * 16 + h] = sum;
+            }
+        check(hipMemcpyAsync(coef, coefficients.data(), coefficients.size() * sizeof(double), hipMemcpyHostToDevice,
+                             st),
+              "solve: coefficients");
+        // Q = (A_s·W)·T, P = W·T, Y += W·U, R −= (A_s·W)·U, Z = R, and the norms of slots 1 and 2.
+        check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(norms + 16), 64, 0, st), "solve: norms");
+        check(oosg::launchBlockStep(Z, P, Q, X, R, coef, scale, n, k, norms + 16, st), "solve: step");
+        const std::vector<double> estimate = backwardErrors();
         bool met = true;
         for (int h = 0; h < k; ++h)
             met = met && estimate[h] <= target;
         if (met || solves >= maxSteps)
         {
-            // The true residual, into Z.
-            product(X, B, Z);
-            const std::vector<double> trueEta = backwardErrors(Z);
+            // The true residual S·(b − A·x), into R.
+            product(X, B, R);
+            startColumnMax(p, R, k, 1, scale, 1);
+            startColumnMax(p, X, k, 2, scale, 0);
+            const std::vector<double> trueEta = backwardErrors();
             converged = true;
             for (int h = 0; h < k; ++h)
             {
@@ -3443,12 +3304,12 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
             }
             if (converged || solves >= maxSteps)
                 break;
-            // Residual replacement: the recurrence drifted from b − A·x.
-            check(hipMemcpyAsync(R, Z, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
+            // Residual replacement: the recurrence drifted from b − A·x; the sweeps start from the true residual.
+            check(hipMemcpyAsync(Z, R, vectorBytes, hipMemcpyDeviceToDevice, st), "solve: residual");
         }
         pollCancellation(p);
     }
-    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
+    check(oosg::launchFromRowMajor(X, x, ld, n, k, scale, 0, st), "solve: layout");
     report.steps = solves;
     report.converged = converged ? 1 : 0;
     report.backward_error = 0.0;
@@ -3458,7 +3319,8 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
 
 // Iterative refinement in double precision on row-major vectors: x_0 = 0, r_0 = b; d = factor⁻¹·r, x += d,
 // r = b − A·x; stops at the backward-error target, after max_steps solves, or when a step does not halve the
-// backward error (the factor is then not accurate enough for the matrix).
+// backward error (the factor is then not accurate enough for the matrix). The first solution stays in the sweeps'
+// vectors (no copy) unless a second step needs them.
 void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b, double* x, int64_t ld, int k,
                      double target, int32_t maxSteps, oosg_solve_report& report, int32_t aheadLevel = 0)
 {
@@ -3466,10 +3328,11 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     const hipStream_t st = p.stream;
     const int64_t n = p.n;
     double* B = p.cgB.as<double>();
-    double* X = p.cgX.as<double>();
     double* R = p.cgR.as<double>();
-    double* Z = p.cgZ.as<double>();
-    // With aheadLevel > 0, B holds b already and the first solve's forward sweep is done below that level.
+    double* sweepVectors = p.solveX.as<double>();
+    double* X = sweepVectors;
+    const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
+    // With aheadLevel > 0, B holds b already and the sweep vectors S·b, swept forward below that level.
     if (aheadLevel == 0)
         check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
     startColumnMax(p, B, k, 0); // read with the first step's norms
@@ -3480,11 +3343,27 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     {
         // x_0 = 0, r_0 = b: the first correction is the first solution.
         if (step == 1)
-            applyFactorRows(f, B, X, k, aheadLevel);
+        {
+            if (aheadLevel == 0)
+                check(oosg::launchScaleRows(B, sweepVectors, scale, n, k, st), "solve: scaling");
+            sweeps(f, k, aheadLevel);
+            if (scale != nullptr)
+                check(oosg::launchScaleRows(sweepVectors, sweepVectors, scale, n, k, st), "solve: scaling");
+        }
         else
         {
-            applyFactorRows(f, R, Z, k);
-            check(oosg::launchAddRows(X, Z, n, k, st), "solve: update");
+            if (X == sweepVectors)
+            {
+                X = p.cgX.as<double>();
+                check(hipMemcpyAsync(X, sweepVectors, static_cast<size_t>(n) * k * sizeof(double),
+                                     hipMemcpyDeviceToDevice, st),
+                      "solve: update");
+            }
+            check(oosg::launchScaleRows(R, sweepVectors, scale, n, k, st), "solve: scaling");
+            sweeps(f, k, 0);
+            if (scale != nullptr)
+                check(oosg::launchScaleRows(sweepVectors, sweepVectors, scale, n, k, st), "solve: scaling");
+            check(oosg::launchAddRows(X, sweepVectors, n, k, st), "solve: update");
         }
         const int slot = p.profiler.begin(PROFILE_RESIDUAL, residualBytes(m, n, k), st);
         check(oosg::launchMultiplyRows(m.dCsrOffsets.as<int64_t>(), m.dCsrCols.as<int32_t>(), m.dCsrValues.as<double>(),
@@ -3847,8 +3726,8 @@ void checkBatch(const oosg_selector* s, const float* features, const float* labe
         fail(OOSG_INVALID_ARGUMENT, "selector: null argument or empty batch");
     if (needLabels)
         for (int64_t b = 0; b < count; ++b)
-            if (!(labels[b] == 0.0f || labels[b] == 1.0f))
-                fail(OOSG_INVALID_ARGUMENT, "selector: labels are 0 or 1");
+            if (!(labels[b] >= 0.0f && labels[b] <= 1.0f))
+                fail(OOSG_INVALID_ARGUMENT, "selector: labels lie in [0, 1]");
 }
 
 } // namespace
@@ -3942,7 +3821,7 @@ oosg_status oosg_selector_set_parameters(oosg_selector* s, const float* params)
         });
 }
 
-// hostValues and the selector's statistics of a matrix whose values were last set on the device.
+// hostValues of a matrix whose values were last set on the device.
 static void refreshHostValues(const oosg_matrix& cm)
 {
     oosg_matrix& m = const_cast<oosg_matrix&>(cm);
@@ -3952,42 +3831,100 @@ static void refreshHostValues(const oosg_matrix& cm)
     m.hostValues.resize(static_cast<size_t>(m.nnz));
     check(hipMemcpy(m.hostValues.data(), m.dValues.ptr, m.hostValues.size() * sizeof(double), hipMemcpyDeviceToHost),
           "matrix: values");
-    valueStatistics(m);
     m.hostStale = false;
 }
 
-oosg_status oosg_selector_features(const oosg_plan* plan, const oosg_matrix* m, float* features)
+oosg_status oosg_selector_cut_features(const oosg_plan* plan, const oosg_matrix* m, float* features,
+                                       int32_t capacity, int32_t* count)
 {
     return guarded(
         [&]
         {
-            if (plan == nullptr || m == nullptr || features == nullptr)
+            if (plan == nullptr || m == nullptr || count == nullptr)
                 fail(OOSG_INVALID_ARGUMENT, "selector: null argument");
             if (m->plan != plan)
                 fail(OOSG_INVALID_ARGUMENT, "selector: the matrix was made for another plan");
+            const int32_t L = plan->levels;
+            *count = L;
+            if (features == nullptr)
+                return;
+            if (capacity < L)
+                fail(OOSG_INVALID_ARGUMENT, "selector: room for " + std::to_string(L) + " cuts (8 features each) needed");
             refreshHostValues(*m);
-            int64_t largestFront = 1;
-            double entries = 0.0; // Σ c(c + 1)/2 + c·r: the factor's stored values (its solves read them)
+            setDevice(plan);
+            const int32_t n = m->n;
+            // The scaled matrix's off-diagonal row sums and its diagonal dominance per equation.
+            std::vector<double> scale(static_cast<size_t>(n), 1.0);
+            if (n > 0 && m->dScale.ptr != nullptr)
+                check(hipMemcpy(scale.data(), m->dScale.ptr, scale.size() * sizeof(double), hipMemcpyDeviceToHost),
+                      "selector: scales");
+            std::vector<double> off(static_cast<size_t>(n), 0.0), diagonal(static_cast<size_t>(n), 0.0);
+            for (int32_t j = 0; j < n; ++j)
+                for (int64_t e = m->offsets[j]; e < m->offsets[j + 1]; ++e)
+                {
+                    const int32_t i = m->rows[e];
+                    const double a = std::fabs(m->hostValues[e]) * scale[i] * scale[j];
+                    if (i == j)
+                    {
+                        diagonal[j] = a;
+                        continue;
+                    }
+                    off[i] += a;
+                    off[j] += a;
+                }
+            std::vector<float> growth(static_cast<size_t>(plan->S), 0.0f);
+            if (plan->growthRecorded && plan->S > 0)
+                check(hipMemcpy(growth.data(), plan->supernodeGrowth.ptr, growth.size() * sizeof(float),
+                                hipMemcpyDeviceToHost),
+                      "selector: pivot growth");
+            // Per level: flops, the largest front order, the dominance sum, minimum and count, the largest growth.
+            std::vector<double> flops(L, 0.0), order(L, 0.0), domSum(L, 0.0), domMin(L, 3.0), growthMax(L, 0.0);
+            std::vector<int64_t> equations(L, 0);
+            double totalFlops = 0.0;
             for (int32_t s = 0; s < plan->S; ++s)
             {
+                const int32_t l = plan->levelOf[s];
+                if (l < 0 || l >= L)
+                    continue;
                 const double c = plan->ncol[s], r = plan->nrow[s];
-                largestFront = std::max<int64_t>(largestFront, plan->ncol[s] + plan->nrow[s]);
-                entries += c * (c + 1) / 2 + c * r;
+                const double work = c * c * c / 3.0 + c * c * r + c * r * r;
+                flops[l] += work;
+                totalFlops += work;
+                order[l] = std::max(order[l], c + r);
+                growthMax[l] = std::max(growthMax[l], static_cast<double>(growth[s]));
+                for (int32_t j = plan->first[s]; j < plan->first[s] + plan->ncol[s]; ++j)
+                {
+                    double d = off[j] > 0.0 ? std::log10(diagonal[j] / off[j]) : 3.0;
+                    d = d != d ? -3.0 : std::min(3.0, std::max(-3.0, d));
+                    domSum[l] += d;
+                    domMin[l] = std::min(domMin[l], d);
+                    ++equations[l];
+                }
             }
-            const double dmin = m->diagonalMin, dmax = m->diagonalMax;
-            const double meanDominance = m->meanDominance, minDominance = m->minDominance;
-            const double spread = dmax > 0.0 ? std::log10(dmax / dmin) : 0.0;
-            const double norm = dmax > 0.0 ? std::log10(m->normInf / dmax) : 0.0;
-            // The factorisation's flops per stored value (its work against a solve's), then its size.
-            const double raw[8] = {std::log10(std::max(plan->flops, 1.0) / std::max(entries, 1.0)),
-                                   std::log10(std::max(plan->flops, 1.0)),
-                                   std::log10(static_cast<double>(largestFront)), spread, meanDominance,
-                                   minDominance, m->weakFraction, norm};
-            // Fixed affine maps to about unit range.
-            const double center[8] = {3.0, 10.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0};
-            const double width[8] = {2.0, 3.0, 1.0, 4.0, 1.0, 1.0, 1.0, 1.0};
-            for (int q = 0; q < 8; ++q)
-                features[q] = static_cast<float>((raw[q] - center[q]) / width[q]);
+            std::vector<double> orderAbove(static_cast<size_t>(L) + 1, 0.0);
+            for (int32_t l = L - 1; l >= 0; --l)
+                orderAbove[l] = std::max(orderAbove[l + 1], order[l]);
+            double flopsBelow = 0.0, orderBelow = 1.0, sumBelow = 0.0, minBelow = 3.0, growthBelow = 0.0;
+            int64_t equationsBelow = 0;
+            for (int32_t cut = 1; cut <= L; ++cut)
+            {
+                const int32_t l = cut - 1; // the level the cut adds to the FP32 part
+                flopsBelow += flops[l];
+                orderBelow = std::max(orderBelow, order[l]);
+                sumBelow += domSum[l];
+                minBelow = std::min(minBelow, domMin[l]);
+                equationsBelow += equations[l];
+                growthBelow = std::max(growthBelow, growthMax[l]);
+                float* f = features + 8 * static_cast<size_t>(l);
+                f[0] = static_cast<float>(static_cast<double>(cut) / L);
+                f[1] = static_cast<float>(totalFlops > 0.0 ? flopsBelow / totalFlops : 0.0);
+                f[2] = static_cast<float>(std::log10(orderBelow) - 2.5);
+                f[3] = static_cast<float>(std::log10(std::max(orderAbove[cut], 1.0)) - 2.5);
+                f[4] = static_cast<float>(std::log10(std::max<double>(n, 1)) - 5.0);
+                f[5] = static_cast<float>(equationsBelow > 0 ? sumBelow / static_cast<double>(equationsBelow) : 0.0);
+                f[6] = static_cast<float>(minBelow);
+                f[7] = static_cast<float>(std::isfinite(growthBelow) ? std::log10(1.0 + growthBelow) / 4.0 : 10.0);
+            }
         });
 }
 
@@ -4611,7 +4548,6 @@ oosg_status oosg_matrix_create(oosg_plan* plan, const oosg_csc_desc* csc, oosg_m
             m->dRows = upload(m->rows, st, "matrix: rows");
             m->dValues = upload(values, st, "matrix: values");
             m->hostValues = values;
-            valueStatistics(*m);
             m->dPosition = upload(position, st, "matrix: positions");
             m->dColumn = upload(column, st, "matrix: columns");
             m->dScale = upload(m->scale, st, "matrix: scales");
@@ -4697,7 +4633,6 @@ oosg_status oosg_matrix_set_values(oosg_matrix* m, const double* values)
                   "matrix: row values");
             check(hipStreamSynchronize(st), "matrix: values");
             m->hostValues = std::move(v);
-            valueStatistics(*m);
             m->hostStale = false;
         });
 }
@@ -5028,6 +4963,12 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                                               opts.max_ratio > 0 ? opts.max_ratio : 1.0e7,
                                               plan->pivotStats.as<oosg::PivotStatistics>(), st),
                   "factor: pivot statistics");
+            check(oosg::launchSupernodeGrowth(diag.as<double>(),
+                                              scaledPrecision(opts.precision) ? f->scale.as<double>() : nullptr,
+                                              matrix->dDiagonal.as<double>(), plan->dp.first, plan->dp.ncol, plan->S,
+                                              plan->eliminated, plan->supernodeGrowth.as<float>(), st),
+                  "factor: pivot growth");
+            plan->growthRecorded = true;
             std::vector<int32_t> zeroAt(static_cast<size_t>(plan->S));
             oosg::PivotStatistics stats{};
             check(hipMemcpyAsync(zeroAt.data(), zero.ptr, zeroAt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st),
@@ -5529,8 +5470,8 @@ oosg_status oosg_solve_refined_device_noalloc(const oosg_factor* f, const oosg_m
             // vectors of its structure.
             oosg_plan* d = f->pivoted ? f->pivoted->structure.get() : nullptr;
             std::vector<DeviceBuffer*> targets = {&p.solveX, &p.solveU, &p.solvePartials, &p.solveNorms, &p.cgB,
-                                                  &p.cgX,    &p.cgR,    &p.cgZ,          &p.cgP,         &p.cgQ,
-                                                  &p.cgPartial, &p.cgDots};
+                                                  &p.cgX,    &p.cgR,    &p.cgP,          &p.cgQ,         &p.cgPartial,
+                                                  &p.cgDots};
             std::vector<uint64_t> sizes = solveBufferBytes(p);
             sizes.resize(targets.size());
             if (d != nullptr)

@@ -13,6 +13,13 @@
         Per kernel, the MFMA flops per dispatch from SQ_INSTS_VALU_MFMA_MOPS_F64 and _F32 (units of 512 flops), and
         with a statistics summary their rate over the kernel's average duration and its fraction of the MFMA peak.
 
+    python -m o_o_solver_amd.profiling levels KERNEL_TRACE_CSV OUT [--command CMD]
+        From a `rocprofv3 --kernel-trace -f csv` run of the benchmark: for the last combined factorisation and solve
+        of each precision in the trace (a factorisation with the solve's forward sweep beside it, as the benchmark's
+        timed steps run; the fastest factorisation when there is none), the time of each level (from its assembly
+        launch to the next level's, the small fronts before the first assembly apart), and the kernel time of the
+        solve that follows it.
+
 read_traffic(path, kernel) returns the calibrated bytes per launch of one kernel from a traffic summary.
 """
 
@@ -157,6 +164,70 @@ def read_traffic(path, kernel):
     return None
 
 
+def factorisations(csv_path):
+    """The factorisations of a kernel trace: per factorisation its precision, its kernels (name, start, end in ns)
+    from its first small-front or assembly launch to its pivot statistics, and the kernels of the solve after it."""
+    with open(csv_path, newline="") as f:
+        rows = [(kernel_name(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                for r in csv.DictReader(f)]
+    rows.sort(key=lambda r: r[1])
+    found, current, after = [], None, None
+    for r in rows:
+        name = r[0]
+        starts = name.startswith("smallFrontKernel") or name.startswith("assembleKernel")
+        if current is None and starts:
+            current, after = [], None
+        if current is not None:
+            current.append(r)
+            if name.startswith("pivotStatisticsKernel"):
+                precision = next((k.split("<")[1].split(">")[0] for k, _, _ in current if "<" in k
+                                  and (k.startswith("assembleKernel") or k.startswith("smallFrontKernel"))), "?")
+                after = []
+                found.append({"precision": precision, "kernels": current, "solve": after})
+                current = None
+        elif after is not None:
+            after.append(r)
+    return found
+
+
+def levels(csv_path, out, command):
+    lines = [f"# levels of the last combined factorisation and solve of each precision, from "
+             f"{os.path.basename(csv_path)}",
+             f"# command: {command}" if command else "#",
+             "# level: from its assembly launch to the next level's (the last: to the pivot statistics); busy: the",
+             "# kernel time started in that window (the streams' kernels overlap, so busy can exceed the span)"]
+    found = factorisations(csv_path)
+    for precision in sorted({f["precision"] for f in found}):
+        runs = [f for f in found if f["precision"] == precision]
+        span = lambda f: f["kernels"][-1][1] - f["kernels"][0][1]
+        combined = [f for f in runs if any(k[0].startswith("tinyForwardKernel") for k in f["kernels"])]
+        best = combined[-1] if combined else min(runs, key=span)
+        ks = best["kernels"]
+        t0 = ks[0][1]
+        marks = [k[1] for k in ks if k[0].startswith("assembleKernel")] + [ks[-1][1]]
+        lines.append(f"\n{precision}: {len(runs)} factorisations in the trace, {len(combined)} combined with the solve; "
+                     f"{'the last combined' if combined else 'the fastest'} {span(best) / 1e6:.3f} ms "
+                     f"(first launch to pivot statistics)")
+        lines.append(f"  {'level':>5s} {'span ms':>9s} {'busy ms':>9s}")
+        windows = [("small", t0, marks[0])] + [(str(i), marks[i], marks[i + 1]) for i in range(len(marks) - 1)]
+        for label, a, b in windows:
+            busy = sum(e - s for _, s, e in ks if a <= s < b)
+            lines.append(f"  {label:>5s} {(b - a) / 1e6:9.3f} {busy / 1e6:9.3f}")
+        solve = best["solve"]
+        if solve:
+            lines.append(f"  solve after it: {len(solve)} kernels, span {(solve[-1][2] - solve[0][1]) / 1e6:.3f} ms, "
+                         f"kernel time {sum(e - s for _, s, e in solve) / 1e6:.3f} ms")
+            per = collections.defaultdict(lambda: [0, 0])
+            for name, s0, e0 in solve:
+                per[name][0] += 1
+                per[name][1] += e0 - s0
+            for name, (count, ns) in sorted(per.items(), key=lambda x: -x[1][1])[:12]:
+                lines.append(f"    {ns / 1e6:8.3f} ms {count:4d}  {name}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def copy():
     import torch
     n = COPY_BYTES // 8
@@ -186,6 +257,10 @@ def main(argv=None):
     m.add_argument("out")
     m.add_argument("--command", default="")
     m.add_argument("--stats", default=None)
+    v = sub.add_parser("levels")
+    v.add_argument("csv")
+    v.add_argument("out")
+    v.add_argument("--command", default="")
     args = ap.parse_args(argv)
     if args.what == "copy":
         copy()
@@ -193,6 +268,8 @@ def main(argv=None):
         stats(args.csv, args.out, args.command)
     elif args.what == "mfma":
         mfma(args.csv, args.out, args.command, args.stats)
+    elif args.what == "levels":
+        levels(args.csv, args.out, args.command)
     else:
         traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.out, args.command)
 

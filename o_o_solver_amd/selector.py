@@ -1,10 +1,13 @@
 """The learned precision selector (no counterpart in the reference): an MLP of 8 → 32 → 32 → 1 on the device that
-gives p(FP32 fronts with FP64 refinement reach the target backward error) from features of a plan and a matrix, so
-that a caller can skip an FP32 factorisation that would not converge and go to FP64 at once.
+gives, for each cut of a plan's levels, p(the cut pays): factorising the subtrees below the cut in FP32 (the rest in
+FP64, refined by conjugate gradients to the target) reaches the target backward error in less time than FP64 fronts
+throughout. Its features per cut (oosg_selector_cut_features) are the cut's depth, the share of the flops below it,
+the front orders below and above it, log n, the scaled-diagonal dominance of the equations below it and their pivot
+growth in the plan's last factorisation. The selector takes the cut of the highest probability when it is at least
+the threshold, else FP64 fronts throughout.
 
-Training data are (features, outcome) pairs of real runs: o_o_solver_amd/train_selector.py factorises a family of
-synthetic models both ways on the GPU and records whether the FP32 factor with conjugate-gradient refinement met the
-target within the step budget.
+Training data are (features, outcome) pairs of real runs: o_o_solver_amd/train_selector.py times a family of
+synthetic models with FP64 fronts and with every cut on the GPU.
 """
 
 import ctypes
@@ -67,10 +70,25 @@ class PrecisionSelector:
         np.savez(path, parameters=self.parameters())
 
     @staticmethod
-    def features(plan, device_matrix):
-        out = np.zeros(FEATURES, np.float32)
-        _check(_native.lib().oosg_selector_features(plan.handle, device_matrix.handle, _ptr(out)))
+    def cut_features(plan, device_matrix):
+        """The features of each cut l = 1 .. L of the plan's levels (row l - 1): the levels below l in FP32."""
+        lib = _native.lib()
+        count = ctypes.c_int32()
+        _check(lib.oosg_selector_cut_features(plan.handle, device_matrix.handle, None, 0, ctypes.byref(count)))
+        out = np.zeros((count.value, FEATURES), np.float32)
+        if count.value > 0:
+            _check(lib.oosg_selector_cut_features(plan.handle, device_matrix.handle, _ptr(out), count.value,
+                                                  ctypes.byref(count)))
         return out
+
+    def choose(self, plan, device_matrix, threshold=0.5):
+        """The cut (the levels factorised in FP32: 0 for FP64 fronts throughout, L for FP32 throughout) and the
+        probability of each cut 1 .. L."""
+        p = self.predict(self.cut_features(plan, device_matrix)) if plan.levels > 0 else np.zeros(0, np.float32)
+        if p.size == 0:
+            return 0, p
+        best = int(np.argmax(p))
+        return (best + 1 if p[best] >= threshold else 0), p
 
     def predict(self, features):
         x = np.ascontiguousarray(np.atleast_2d(features), dtype=np.float32)
