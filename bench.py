@@ -94,6 +94,20 @@ def cpu_baseline(workload, b, threads, keep_solution=False):
                       + ("; value is the median of 3 runs" if keep_solution else "")}
 
 
+def reference_order_flops(workload, threads):
+    """The factorisation flops of the workload under the reference's own nested-dissection order
+    (GeometricDissection, through oracle/_ref), beside this library's order: the orders differ, so the structures do
+    too. None when the compiled reference is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    if _oracle.reference() is None:
+        return None
+    h, _, flops, _ = _oracle.ref_symbolic(workload.graph_offsets, workload.adjacency, workload.weights,
+                                          coords=workload.coordinates, method=2, threads=threads)
+    _oracle.reference().ref_sym_free(h)
+    return float(flops)
+
+
 def cylinder_main(args):
     """The buckling shifts of the stiffened cylinder (360 × 200 quad4 skin, 24 stringers, 429,840 DOF): the first 20
     buckling factors by bisection on GPU Sturm counts, then 8 shifts across them, each an indefinite FP64
@@ -176,14 +190,14 @@ def statics_main(args, kind):
         b = workload.loads("top", nrhs=4, seed=20261204)
         description = (f"hex8 block 64^3, {workload.n} DOF, E lognormal (median 200 GPa, sigma 1.5), base fixed; 4 "
                        "N(0,1) top-face load cases")
-        metric = "factor+solve s at 1e-10 backward err (64^3 hex block): FP32 fronts + FP64 CG vs FP64 GPU"
+        metric = "factor+solve s at 1e-10 backward err (64^3 hex block): learned precision vs FP64 GPU"
     else:
         workload = models.plate_batch_c4(512)
         sym = workload.analyse()
         b = workload.loads("random", nrhs=1, seed=20261205)
         description = (f"512 clamped quad4 plates, sides U{{20..80}} elements, t U[1,5] mm, {workload.n} DOF in one "
                        "block-diagonal factor; one N(0,1) load case each")
-        metric = "factor+solve s at 1e-10 backward err (512 batched plates): FP32 fronts + FP64 CG vs FP64 GPU"
+        metric = "factor+solve s at 1e-10 backward err (512 batched plates): learned precision vs FP64 GPU"
     log(f"{workload.name}: {workload.n} DOF, {sym.supernode_count} supernodes, {sym.flop_count:.3e} flops "
         f"({time.perf_counter() - t:.1f} s)")
     plan = sym.plan()
@@ -196,12 +210,13 @@ def statics_main(args, kind):
                                           "launch_mode": plan.launch_mode(precision)}
     log(f"autotune: {tuning}")
     results = {}
-    for name, precision, steps in (("fp64", Precision.FP64, 4), ("fp32_fronts_cg", Precision.FP32, args.max_steps)):
+
+    def leg(name, options, steps):
         times = []
         for k in range(args.warmup + args.steps):
             torch.cuda.synchronize()
             t = time.perf_counter()
-            f = SupernodalFactor.factorise(sym, workload.matrix, FactorOptions(precision=precision), device_matrix=dm)
+            f = SupernodalFactor.factorise(sym, workload.matrix, options, device_matrix=dm)
             try:
                 x, rep = FactorSolve(f).solve_refined(bd, target=args.target, max_steps=steps)
             except NotConvergedError as e:
@@ -221,6 +236,18 @@ def statics_main(args, kind):
         log(f"{name}: {results[name]['seconds'] * 1e3:.1f} ms/step, factor {factor_s * 1e3:.1f} ms "
             f"({results[name]['factor_tflops']} TFLOP/s), solves {results[name]['solves']}, eta "
             f"{results[name]['backward_error']}")
+
+    from o_o_solver_amd.selector import PrecisionSelector
+    leg("fp64", FactorOptions(precision=Precision.FP64), 4)
+    # The learned selector's cut for this matrix (after the FP64 leg, whose pivot growth it reads).
+    selector = PrecisionSelector.trained()
+    cut, p = selector.choose(plan, dm) if selector is not None else (0, np.zeros(0))
+    chosen = "fp64" if cut == 0 else ("fp32" if cut >= p.size else "mixed")
+    log(f"selector: p(cut pays) = {[round(float(v), 4) for v in p]}, chose {chosen} ({cut} of {plan.levels} levels)")
+    leg("fp32_fronts_cg", FactorOptions(precision=Precision.FP32), args.max_steps)
+    if chosen == "mixed":
+        leg("mixed_cut", FactorOptions(precision=Precision.MIXED, fp32_levels=cut), args.max_steps)
+    learned = results[{"fp64": "fp64", "fp32": "fp32_fronts_cg", "mixed": "mixed_cut"}[chosen]]
     baseline = None
     if args.cpu_baseline == "auto":
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -234,11 +261,14 @@ def statics_main(args, kind):
                 results[name]["error_vs_reference"] = float(np.max(np.abs(x - xr)) / np.max(np.abs(xr)))
     for name in results:
         results[name].pop("x")
-    best = min(r["seconds"] for r in results.values())
-    out = {"metric": metric, "value": best, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(best * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+    value = learned["seconds"]
+    out = {"metric": metric, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(value * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32+f64", "data": "synthetic",
            "config": {"workload": description, "flops": sym.flop_count, "supernodes": sym.supernode_count},
+           "learned": {"precision_chosen": chosen, "fp32_levels": int(cut), "levels": plan.levels,
+                       "selector_probability": float(p.max()) if p.size else None},
+           "speedup_learned_vs_fp64": round(results["fp64"]["seconds"] / value, 3),
            "autotune": tuning, **results, "cpu_baseline": baseline}
     print(json.dumps(out), flush=True)
 
@@ -452,6 +482,7 @@ def main():
     # The solutions on the host, for their error against the reference's FP64 solution.
     solutions = {k: v.cpu().numpy() for k, v in last_x.items()}
     baseline = None
+    reference_flops = None
     errors = {}
     if args.cpu_baseline == "auto":
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -471,6 +502,9 @@ def main():
             single = cpu_baseline(workload, b, 1)
             baseline["one_thread"] = {k: single[k] for k in ("value", "factor_s", "solve_s", "gflops_factor")}
             log(f"cpu baseline, 1 thread: {single['value']} s ({single['gflops_factor']} GFLOP/s factor)")
+            reference_flops = reference_order_flops(workload, threads)
+            log(f"flops under the reference's own order (GeometricDissection): {reference_flops:.4e} "
+                f"(this library's order: {sym.flop_count:.4e})")
 
     flops = sym.flop_count
     out = {
@@ -489,6 +523,7 @@ def main():
         "config": {"workload": f"shell plate {args.scale}x{args.scale} quad4, {workload.n} DOF, jitter 0.2, "
                                f"t log-uniform [1,4] mm, clamped; {args.nrhs} N(0,1) load cases",
                    "nrhs": args.nrhs, "target_backward_error": args.target, "flops": flops,
+                   "flops_reference_order": reference_flops,
                    "supernodes": sym.supernode_count, "levels": stats["levels"]},
         "fp64": {"seconds": round(fp64_s, 5), "factor_s": round(fp64_factor, 5),
                  "solve_s": round(float(np.mean([r.seconds for r in fp64_reps])), 5),

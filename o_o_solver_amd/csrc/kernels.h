@@ -243,8 +243,13 @@ hipError_t launchTrailingGather(double* X, const double* U, int k, int first, in
 hipError_t launchFillInt32(int32_t* data, int64_t count, int32_t value, hipStream_t stream);
 hipError_t launchToRowMajor(const double* src, int64_t ld, double* dst, int64_t n, int k, const double* scale,
                             hipStream_t stream);
+// With norm: the column maxima |S·src| into it too (bits, zeroed by the caller).
 hipError_t launchFromRowMajor(const double* src, double* dst, int64_t ld, int64_t n, int k, const double* scale,
-                              int accumulate, hipStream_t stream);
+                              int accumulate, hipStream_t stream, unsigned long long* norm = nullptr);
+// Right-hand sides src (column-major n × k, leading dimension ld) into B (row-major) and X1 = S·src (row-major; null:
+// none; scale null: a copy), with the column maxima |src| into norm (bits, zeroed by the caller; null: none): one pass.
+hipError_t launchRightHandSides(const double* src, int64_t ld, double* B, double* X1, const double* scale, int64_t n,
+                                int k, unsigned long long* norm, hipStream_t stream);
 hipError_t launchResidual(const int64_t* offsets, const int32_t* cols, const double* values, const double* x,
                           const double* b, double* out, int64_t ld, int64_t n, int k, hipStream_t stream);
 

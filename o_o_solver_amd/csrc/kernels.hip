@@ -2266,6 +2266,127 @@ __global__ void toRowMajorKernel(const double* src, int64_t ld, double* dst, int
     }
 }
 
+// The column maxima of a workgroup's rows (m[h] for its thread's rows) into norm[h] (bits, atomicMax; NaN wins),
+// through the wave's shuffles and LDS: every thread of the workgroup calls it.
+__device__ inline void columnMaxima(const double (&m)[16], int k, unsigned long long* norm)
+{
+    __shared__ double slots[4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 0; h < 16; ++h)
+    {
+        double v = m[h];
+#pragma unroll
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            const double o = __shfl_xor(v, offset);
+            v = o > v || o != o ? o : v;
+        }
+        if (lane == 0)
+            slots[wave][h] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < k)
+    {
+        double v = slots[0][threadIdx.x];
+        for (int w = 1; w < 4; ++w)
+        {
+            const double o = slots[w][threadIdx.x];
+            v = o > v || o != o ? o : v;
+        }
+        atomicMax(&norm[threadIdx.x], static_cast<unsigned long long>(__double_as_longlong(v)));
+    }
+}
+
+// Right-hand sides into the solve's vectors, 256 rows per workgroup pass through an LDS tile (the column-major loads
+// coalesced per column, the row-major stores coalesced over the tile's contiguous k·256 values): B = src row-major,
+// X1 = S·src row-major (X1 null: none; S null: a copy), and the column maxima |src| into norm (bits, zeroed by the
+// caller; null: none).
+__global__ __launch_bounds__(256) void rightHandSidesKernel(const double* src, int64_t ld, double* B, double* X1,
+                                                           const double* scale, int64_t n, int k,
+                                                           unsigned long long* norm)
+{
+    __shared__ double tile[256 * 17];
+    __shared__ double rowScale[256];
+    double m[16];
+#pragma unroll
+    for (int h = 0; h < 16; ++h)
+        m[h] = 0.0;
+    const int t = threadIdx.x;
+    for (int64_t r0 = blockIdx.x * int64_t(256); r0 < n; r0 += int64_t(gridDim.x) * 256)
+    {
+        const int rows = int(min(int64_t(256), n - r0));
+        const int64_t i = r0 + t;
+#pragma unroll
+        for (int h = 0; h < 16; ++h)
+            if (h < k && t < rows)
+            {
+                const double v = src[i + h * ld];
+                tile[t * 17 + h] = v;
+                const double a = fabs(v);
+                m[h] = a > m[h] || a != a ? a : m[h];
+            }
+        if (t < rows)
+            rowScale[t] = scale ? scale[i] : 1.0;
+        __syncthreads();
+        const int values = rows * k;
+        for (int e = t; e < values; e += 256)
+        {
+            const int row = e / k, h = e - row * k;
+            const double v = tile[row * 17 + h];
+            B[r0 * k + e] = v;
+            if (X1)
+                X1[r0 * k + e] = scale ? v * rowScale[row] : v;
+        }
+        __syncthreads(); // the tile is read before the next pass writes it
+    }
+    if (norm)
+        columnMaxima(m, k, norm);
+}
+
+// dst (column-major, leading dimension ld) = S·src (src row-major n × k; S null: the identity), or += with
+// accumulate; with norm, the column maxima |S·src| into it (bits, zeroed by the caller). 256 rows per workgroup pass
+// through an LDS tile (the row-major loads coalesced over the tile's contiguous values, the stores per column).
+__global__ __launch_bounds__(256) void columnsFromRowsKernel(const double* src, double* dst, int64_t ld, int64_t n,
+                                                            int k, const double* scale, int accumulate,
+                                                            unsigned long long* norm)
+{
+    __shared__ double tile[256 * 17];
+    double m[16];
+#pragma unroll
+    for (int h = 0; h < 16; ++h)
+        m[h] = 0.0;
+    const int t = threadIdx.x;
+    for (int64_t r0 = blockIdx.x * int64_t(256); r0 < n; r0 += int64_t(gridDim.x) * 256)
+    {
+        const int rows = int(min(int64_t(256), n - r0));
+        const int values = rows * k;
+        for (int e = t; e < values; e += 256)
+        {
+            const int row = e / k, h = e - row * k;
+            tile[row * 17 + h] = src[r0 * k + e];
+        }
+        __syncthreads();
+        const int64_t i = r0 + t;
+        const double s = scale && t < rows ? scale[i] : 1.0;
+#pragma unroll
+        for (int h = 0; h < 16; ++h)
+            if (h < k && t < rows)
+            {
+                const double v = scale ? tile[t * 17 + h] * s : tile[t * 17 + h];
+                if (accumulate)
+                    dst[i + h * ld] += v;
+                else
+                    dst[i + h * ld] = v;
+                const double a = fabs(v);
+                m[h] = a > m[h] || a != a ? a : m[h];
+            }
+        __syncthreads(); // the tile is read before the next pass writes it
+    }
+    if (norm)
+        columnMaxima(m, k, norm);
+}
+
 __global__ void fromRowMajorKernel(const double* src, double* dst, int64_t ld, int64_t n, int k, const double* scale,
                                    int accumulate)
 {
@@ -3068,10 +3189,25 @@ hipError_t launchToRowMajor(const double* src, int64_t ld, double* dst, int64_t 
 }
 
 hipError_t launchFromRowMajor(const double* src, double* dst, int64_t ld, int64_t n, int k, const double* scale,
-                              int accumulate, hipStream_t stream)
+                              int accumulate, hipStream_t stream, unsigned long long* norm)
 {
-    hipLaunchKernelGGL(fromRowMajorKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, src, dst, ld, n, k,
-                       scale, accumulate);
+    if (norm != nullptr)
+    {
+        const int groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256)));
+        hipLaunchKernelGGL(columnsFromRowsKernel, dim3(groups), dim3(256), 0, stream, src, dst, ld, n, k, scale,
+                           accumulate, norm);
+    }
+    else
+        hipLaunchKernelGGL(fromRowMajorKernel, dim3(blocksFor(n * k, 256)), dim3(256), 0, stream, src, dst, ld, n, k,
+                           scale, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launchRightHandSides(const double* src, int64_t ld, double* B, double* X1, const double* scale, int64_t n,
+                                int k, unsigned long long* norm, hipStream_t stream)
+{
+    const int groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256)));
+    hipLaunchKernelGGL(rightHandSidesKernel, dim3(groups), dim3(256), 0, stream, src, ld, B, X1, scale, n, k, norm);
     return hipGetLastError();
 }
 

@@ -352,10 +352,11 @@ double frontFlops(int64_t c, int64_t r)
 } // namespace
 
 // The device memory of one factor: panels, inverses of the diagonal blocks, pivots, and (FP32) the equations' scales.
-// Sized for FP64 values, so that either precision can use it.
+// Sized for FP64 values, so that either precision can use it; a mixed factor's FP32 panels and inverses besides, once
+// one was made (kept with the storage for the next mixed factor).
 struct FactorStorage
 {
-    DeviceBuffer panels, linv, diag, scale;
+    DeviceBuffer panels, linv, diag, scale, panels32, linv32;
 };
 
 struct oosg_plan
@@ -630,6 +631,8 @@ struct oosg_factor
         back.linv = std::move(linv);
         back.diag = std::move(dDiag);
         back.scale = std::move(scale);
+        back.panels32 = std::move(panels32);
+        back.linv32 = std::move(linv32);
         if (plan->storagePool.size() < 2)
             plan->storagePool.push_back(std::move(back));
     }
@@ -3031,6 +3034,16 @@ void solveDevice(const oosg_factor& f, double* b, int64_t ld, int64_t nrhs)
     }
 }
 
+// The right-hand sides b (column-major, leading dimension ld, k <= 16 columns) into the plan's cgB (row-major) and
+// X1 = S·b (row-major; scale null: a copy), their column maxima into slot 0 of the plan's norms: one pass, on the
+// plan's stream.
+void prepareRightHandSides(oosg_plan& p, const double* b, int64_t ld, double* X1, const double* scale, int k)
+{
+    unsigned long long* norm = p.solveNorms.as<unsigned long long>();
+    check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(norm), 32, 0, p.stream), "solve: norms");
+    check(oosg::launchRightHandSides(b, ld, p.cgB.as<double>(), X1, scale, p.n, k, norm, p.stream), "solve: layout");
+}
+
 // Column maxima |v| of row-major n × k vectors into slot `slot` (of 3) of the plan's norms, on the stream; with a
 // row scaling S, of S·v (divide: S⁻¹·v).
 void startColumnMax(oosg_plan& p, const double* v, int k, int slot, const double* scale = nullptr, int divide = 0)
@@ -3053,12 +3066,6 @@ std::vector<std::vector<double>> finishColumnMax(oosg_plan& p, int slots, int k)
         for (int h = 0; h < k; ++h)
             std::memcpy(&out[q][h], &bits[16 * q + h], sizeof(double));
     return out;
-}
-
-std::vector<double> columnMaxRows(oosg_plan& p, const double* v, int k)
-{
-    startColumnMax(p, v, k, 0);
-    return finishColumnMax(p, 1, k)[0];
 }
 
 // The solve buffers of a plan, sized for 16 right-hand sides.
@@ -3210,13 +3217,11 @@ void solveBlockPcg(const oosg_factor& f, const oosg_matrix& m, const double* b, 
         check(oosg::launchBlockGram(g, n, k, partial, grams, st), "solve: Gram products");
     };
 
-    // With aheadLevel > 0, B holds b and the sweep vectors S·b, swept forward below that level.
+    // With aheadLevel > 0, B holds b, the sweep vectors S·b, swept forward below that level, and the first slot of
+    // the norms ‖b‖∞ by column.
     if (aheadLevel == 0)
-    {
-        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
-        check(oosg::launchScaleRows(B, Z, scale, n, k, st), "solve: scaling");
-    }
-    const std::vector<double> bNorm = columnMaxRows(p, B, k);
+        prepareRightHandSides(p, b, ld, Z, scale, k);
+    const std::vector<double> bNorm = finishColumnMax(p, 1, k)[0];
     check(oosg::launchFillDouble(X, static_cast<int64_t>(n) * k, static_cast<int64_t>(n) * k, 1, 0.0, st),
           "solve: zero");
     check(oosg::launchScaleRows(B, R, scale, n, k, st), "solve: residual");
@@ -3332,11 +3337,12 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
     double* sweepVectors = p.solveX.as<double>();
     double* X = sweepVectors;
     const double* scale = scaledPrecision(f.precision) ? f.scale.as<double>() : nullptr;
-    // With aheadLevel > 0, B holds b already and the sweep vectors S·b, swept forward below that level.
+    // B holds b, the sweep vectors S·b (swept forward below aheadLevel when it is > 0) and the first slot of the norms
+    // ‖b‖∞ by column (read with the first step's norms).
     if (aheadLevel == 0)
-        check(oosg::launchToRowMajor(b, ld, B, n, k, nullptr, st), "solve: layout");
-    startColumnMax(p, B, k, 0); // read with the first step's norms
+        prepareRightHandSides(p, b, ld, sweepVectors, scale, k);
     std::vector<double> bNorm;
+    bool written = false; // x holds the first solution already
     double previous = INFINITY;
     report.converged = 0;
     for (int32_t step = 1; step <= std::max(1, maxSteps); ++step)
@@ -3344,8 +3350,6 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
         // x_0 = 0, r_0 = b: the first correction is the first solution.
         if (step == 1)
         {
-            if (aheadLevel == 0)
-                check(oosg::launchScaleRows(B, sweepVectors, scale, n, k, st), "solve: scaling");
             sweeps(f, k, aheadLevel);
             if (scale != nullptr)
                 check(oosg::launchScaleRows(sweepVectors, sweepVectors, scale, n, k, st), "solve: scaling");
@@ -3372,7 +3376,20 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
               "solve: residual");
         p.profiler.end(slot, st);
         startColumnMax(p, R, k, 1);
-        startColumnMax(p, X, k, 2);
+        if (step == 1)
+        {
+            // The first solution into x at once, its column maxima taken in the same pass (a later step writes x
+            // again).
+            unsigned long long* xNorm = p.solveNorms.as<unsigned long long>() + 32;
+            check(oosg::launchFillInt32(reinterpret_cast<int32_t*>(xNorm), 32, 0, st), "solve: norms");
+            check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st, xNorm), "solve: layout");
+            written = true;
+        }
+        else
+        {
+            startColumnMax(p, X, k, 2);
+            written = false;
+        }
         const std::vector<std::vector<double>> norms = finishColumnMax(p, 3, k);
         if (step == 1)
             bNorm = norms[0];
@@ -3396,7 +3413,8 @@ void solveRefinement(const oosg_factor& f, const oosg_matrix& m, const double* b
             break;
         previous = eta;
     }
-    check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
+    if (!written)
+        check(oosg::launchFromRowMajor(X, x, ld, n, k, nullptr, 0, st), "solve: layout");
 }
 
 } // namespace
@@ -4868,6 +4886,8 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
                 f->linv = std::move(storage.linv);
                 f->dDiag = std::move(storage.diag);
                 f->scale = std::move(storage.scale);
+                f->panels32 = std::move(storage.panels32);
+                f->linv32 = std::move(storage.linv32);
             }
             if (plan->outOfCore)
             {
@@ -4883,12 +4903,13 @@ void factoriseInto(oosg_plan* plan, const oosg_matrix* matrix, const oosg_factor
             if (mixed)
             {
                 f->fp32Levels = std::clamp(opts.fp32_levels, 0, plan->levels);
-                allocateCounted(*plan, f->panels32,
-                                static_cast<size_t>(std::max<int64_t>(plan->panelOff[plan->S], 1)) * sizeof(float),
-                                "factor: FP32 panels (mixed)");
-                allocateCounted(*plan, f->linv32,
-                                static_cast<size_t>(std::max<int64_t>(plan->linvOff[plan->S], 1)) * sizeof(float),
-                                "factor: FP32 inverses (mixed)");
+                // Kept with the factor storage for the next mixed factor (no allocation once one was made).
+                ensureCounted(*plan, f->panels32,
+                              static_cast<size_t>(std::max<int64_t>(plan->panelOff[plan->S], 1)) * sizeof(float),
+                              "factor: FP32 panels (mixed)");
+                ensureCounted(*plan, f->linv32,
+                              static_cast<size_t>(std::max<int64_t>(plan->linvOff[plan->S], 1)) * sizeof(float),
+                              "factor: FP32 inverses (mixed)");
                 prepareWidening(*plan, f->fp32Levels);
             }
             DeviceBuffer& diag = f->dDiag;
@@ -5582,11 +5603,9 @@ oosg_status oosg_factorise_solve_device(oosg_plan* plan, const oosg_matrix* matr
                         check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
                 }
                 // The first tile of b into the refinement's row-major copy, and scaled (FP32) into the sweeps'
-                // vectors.
-                check(oosg::launchToRowMajor(b, ld, plan->cgB.as<double>(), n, k, nullptr, st), "solve: layout");
-                check(oosg::launchScaleRows(plan->cgB.as<double>(), plan->solveX.as<double>(),
-                                            fp32 ? matrix->dScale.as<double>() : nullptr, n, k, st),
-                      "solve: scaling");
+                // vectors, with its column maxima in the first slot of the solve's norms: one pass.
+                prepareRightHandSides(*plan, b, ld, plan->solveX.as<double>(),
+                                      fp32 ? matrix->dScale.as<double>() : nullptr, k);
                 plan->aheadLevel = plan->overlapLevel;
                 plan->aheadK = k;
             }

@@ -459,6 +459,25 @@ def test_ratio_test_statistics_match_the_pivots(gpu_device, precision):
                                                                 pivot_policy=oos.PivotPolicy(max_ratio=threshold)),
                                    device_matrix=plain.device_matrix)
     assert error.value.dofs == fails[:1000].tolist()
+    if precision == Precision.FP64:
+        # Against the oracle's pivots (the reference's algorithm on the CPU): the same signs, the same failing
+        # equations at a threshold between the oracle's ratios (no ratio within 1e-9 of it, so the GPU's rounding
+        # cannot move one across), the same largest ratio to 1e-9 (the pivots' rounding grows with the ratio: a
+        # ratio of 65 is a pivot 65 times smaller than its diagonal).
+        _, oracle_d, _ = _oracle.multifrontal(sym, w.matrix)
+        oracle_ratio = a / oracle_d
+        ordered = np.sort(oracle_ratio)
+        gaps = np.flatnonzero(np.diff(ordered) > 1e-9 * np.abs(ordered[1:]))
+        middle = gaps[np.argmin(np.abs(gaps - len(ordered) // 2))]
+        cut = 0.5 * (ordered[middle] + ordered[middle + 1])
+        g = SupernodalFactor.factorise(sym, w.matrix, FactorOptions(pivot_policy=oos.PivotPolicy(max_ratio=cut,
+                                                                                                  bailout=-1)),
+                                       device_matrix=plain.device_matrix)
+        oracle_fails = np.flatnonzero((oracle_ratio > cut) | (oracle_ratio < 0) | (oracle_d < 0))
+        assert g.pivot_diagnostics.issue_count == len(oracle_fails) > 0
+        assert [i.equation for i in g.pivot_diagnostics.issues] == oracle_fails[:1000].tolist()
+        assert (g.inertia.positive, g.inertia.negative) == (int(np.sum(oracle_d > 0)), int(np.sum(oracle_d < 0)))
+        assert abs(g.pivot_diagnostics.largest_ratio - np.max(oracle_ratio)) <= 1e-9 * np.max(oracle_ratio)
 
 
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32])
