@@ -46,7 +46,7 @@ KIND_KERNELS = {"small_fronts": "smallFrontKernel<{T}>", "assembly": "assembleKe
                 "backward_columns": "backwardColumnsKernel<{T}>", "triangle_products": "triangleProductsKernel<{T}>",
                 "triangle_inverses": "triangleInverseKernel<{T}>"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_shell_plate_c2.txt")
-BYTE_KINDS = ("forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
+BYTE_KINDS = ("assembly", "forward_triangles", "backward_triangles", "residual", "forward_rows", "backward_columns",
               "triangle_products")
 
 
@@ -473,11 +473,18 @@ def main():
                      "work_per_launch": work / launches,
                      "share_of_device_time": round(seconds / sum(v[0] for v in prof.values()), 3),
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if traffic else None})
-    def breakdown(profile):
-        return {k: {"ms_per_step": round(v[0] * 1e3, 3), "launches_per_step": v[1],
-                    "rate": round(v[2] / v[0] / (1e9 if k in BYTE_KINDS else 1e12), 2) if v[0] > 0 else None}
-                for k, v in profile.items()}
-    kernels = breakdown(fp32_prof)
+    def breakdown(profile, peak_flops):
+        # Per kind: its time, launches and rate (GB/s for the HBM-bound kinds, TFLOP/s for the others) and the rate's
+        # fraction of its roofline (HBM 8 TB/s, or the MFMA peak of the precision).
+        out = {}
+        for k, v in profile.items():
+            rate = v[2] / v[0] / (1e9 if k in BYTE_KINDS else 1e12) if v[0] > 0 else None
+            peak = PEAK_HBM if k in BYTE_KINDS else peak_flops
+            out[k] = {"ms_per_step": round(v[0] * 1e3, 3), "launches_per_step": v[1],
+                      "rate": round(rate, 2) if rate is not None else None,
+                      "roofline_frac": round(rate / peak, 4) if rate is not None else None}
+        return out
+    kernels = breakdown(fp32_prof, PEAK_FP32)
 
     # The solutions on the host, for their error against the reference's FP64 solution.
     solutions = {k: v.cpu().numpy() for k, v in last_x.items()}
@@ -551,7 +558,7 @@ def main():
         "roofline": roofline,
         "autotune": tuning,
         "kernels_fp32_fronts_cg": kernels,
-        "kernels_fp64": breakdown(fp64_prof),
+        "kernels_fp64": breakdown(fp64_prof, PEAK_FP64),
         "cpu_baseline": baseline,
     }
     print(json.dumps(out), flush=True)

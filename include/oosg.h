@@ -358,10 +358,10 @@ oosg_status oosg_debug_mfma_probe(const double *a, const double *b, double *d, c
  * triangles, 7 backward sweep triangles, 8 residual products, 9 forward sweep row tiles (L_RF), 10 backward sweep
  * column blocks (L_RFᵀ), 11 products with the wide triangles' whole inverses (both sweeps), 12 the whole inverses
  * (factorisation), 13 update matrices on 128 x 128 tiles, 14 panel updates whose last tile factorises the next
- * diagonal block (fused). work[k] is the algorithmic work of the launches timed: flops for kinds 0, 2-5 and 12-14
- * (0 for kind 1, which only moves values); bytes for kinds 6-11 (the values of L or of an inverse read once, and the
- * right-hand-side rows read or written once; the matrix and the vectors of a residual). Arrays of OOSG_PROFILE_KINDS
- * values; any may be null. */
+ * diagonal block (fused). work[k] is the algorithmic work of the launches timed: flops for kinds 0, 2-5 and 12-14;
+ * bytes for kinds 1 (the panels' lower parts written and the children's update-matrix parts read, the matrix's own
+ * entries not counted) and 6-11 (the values of L or of an inverse read once, and the right-hand-side rows read or
+ * written once; the matrix and the vectors of a residual). Arrays of OOSG_PROFILE_KINDS values; any may be null. */
 #define OOSG_PROFILE_KINDS 15
 oosg_status oosg_plan_profile(oosg_plan *plan, int32_t enable);
 oosg_status oosg_plan_profile_read(const oosg_plan *plan, double *seconds, int64_t *launches, double *work);

@@ -201,6 +201,9 @@ struct Launch
     int k0 = 0;
     int maxOrder = 0;
     double flops = 0.0; // algorithmic flops of the launch
+    // Assembly launches: the values they write (the panels' lower parts) and read (the children's update-matrix parts
+    // landing there), the algorithmic bytes per value size; the matrix's entries they read are not counted.
+    double values = 0.0;
     // Update-matrix launches: the same products as 128 × 128 tiles (the big tile list), and whether each precision
     // runs them so (the tile policy and the autotuner set it).
     int64_t bigOffset = 0;
@@ -1537,6 +1540,18 @@ void buildSchedule(oosg_plan& p)
         Launch assemble{LaunchKind::Assemble, PROFILE_ASSEMBLE};
         assemble.offset = blockStart;
         assemble.count = static_cast<int>(p.blocks.size() - blockStart);
+        for (size_t x = static_cast<size_t>(blockStart); x < p.blocks.size(); ++x)
+        {
+            const oosg::AssemblyBlock& blk = p.blocks[x];
+            for (int32_t j = blk.j0; j < blk.j1; ++j)
+                assemble.values += std::max(0, blk.i1 - std::max(blk.i0, j));
+            for (int32_t it = blk.itemStart; it < blk.itemEnd; ++it)
+            {
+                const oosg::AssemblyItem& item = p.items[it];
+                for (int32_t jj = item.q0; jj < item.q1; ++jj)
+                    assemble.values += std::max(0, item.ib - std::max(jj, item.ia));
+            }
+        }
         p.schedule.push_back(assemble);
         // Panel steps.
         // A record of EVENT_SOLVED on the critical stream delays its next launch by several microseconds, so a step
@@ -2047,7 +2062,8 @@ void runFactor(oosg_plan& p, const oosg_matrix& m, T* panels, T* linv, double* d
         const int precision = sizeof(T) == sizeof(float) ? OOSG_FP32 : OOSG_FP64;
         const bool big = l.kind == LaunchKind::Tiles && l.bigCount > 0 &&
                          (p.tilePolicy == 1 || (p.tilePolicy == 2 && l.big[precision]));
-        const int slot = p.profiler.begin(big ? PROFILE_UPDATE_MATRIX_BIG : l.profile, l.flops, ls);
+        const int slot = p.profiler.begin(big ? PROFILE_UPDATE_MATRIX_BIG : l.profile,
+                                          l.kind == LaunchKind::Assemble ? l.values * sizeof(T) : l.flops, ls);
         switch (l.kind)
         {
         case LaunchKind::Sync:

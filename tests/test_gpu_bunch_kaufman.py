@@ -69,11 +69,11 @@ def test_plate_pencil_pivots_match_the_reference(gpu_device, reference_lib, nx, 
     print(f"plate {nx} σ-fraction {fraction}: inertia {got} (reference {inertia.tolist()}), delayed "
           f"{f.delayed_pivot_count} ({counts[0]}), 2x2 {f.two_by_two_pivot_count} ({counts[1]})", flush=True)
     assert got == inertia.tolist()
-    assert f.two_by_two_pivot_count > 0 or counts[1] == 0
-    if f.delayed_pivot_count == counts[0] and f.two_by_two_pivot_count == counts[1]:
-        # The same pivot sequence: D per equation agrees to rounding.
-        scale = np.maximum(np.abs(diag), 1e-300)
-        assert np.max(np.abs(f.diagonal() - diag) / scale) <= 1e-8
+    # The same pivot sequence as the reference's (the same delayed columns and 2 × 2 pivots), so D per equation
+    # agrees to rounding.
+    assert (f.delayed_pivot_count, f.two_by_two_pivot_count) == (int(counts[0]), int(counts[1]))
+    scale = np.maximum(np.abs(diag), 1e-300)
+    assert np.max(np.abs(f.diagonal() - diag) / scale) <= 1e-8
     x = FactorSolve(f).solve(b)
     assert backward_error(a, x, b) <= 1e-12
     assert np.max(np.abs(x - x_ref)) <= 1e-6 * np.max(np.abs(x_ref))
