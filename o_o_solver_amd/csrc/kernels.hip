@@ -940,36 +940,57 @@ __global__ __launch_bounds__(256) void extendTilesKernel(DevicePlan p, T* panels
 // be narrower (ib). Four waves of 32 × 32 over a 64 × 64 tile, 16 × 16 × 4 MFMA blocks; products of 64 terms staged
 // through LDS in chunks of 32.
 
-// acc += A·B over 64 terms for the tile: A(i, t) = a(i, t) for i < rowsA (zero beyond), B(t, j) = b(t, j) (terms beyond
-// `terms` read as zero); a, b give values at positions clamped into range.
-template <typename T, typename FA, typename FB>
-__device__ inline void tileProduct(typename Mfma<T>::Acc (&acc)[2][2], T (*As)[TILE + 4], T (*Bs)[TILE + 4], int rowsA,
-                                   int terms, FA a, FB b)
+// The operands of a product of 64 terms for a 64 × 64 tile, held in registers: a thread stages 16 values of A(i, t)
+// (row i = tid mod 64, terms kq + 4u of each half) and 16 of B(t, j) (term bk of each half, columns bj + 8u). All of
+// an item's operands are loaded before its first product, so its global loads wait once, not once per chunk.
+template <typename T> struct TileTerms
 {
-    constexpr int KC = 32;
+    T a[16], b[16];
+};
+
+// Loads A(i, t) = a(i, t) for i < rowsA and B(t, j) = b(t, j) for t < terms; a and b are given positions clamped into
+// range, and the values beyond are zeroed when staged.
+template <typename T, typename FA, typename FB>
+__device__ inline void loadTileTerms(TileTerms<T>& s, int rowsA, int terms, FA a, FB b)
+{
+    const int tid = threadIdx.x;
+    const int row = tid & (TILE - 1), kq = tid >> 6;
+    const int bk = tid & 31, bj = tid >> 5;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            s.a[8 * h + u] = a(min(row, rowsA - 1), min(32 * h + kq + 4 * u, terms - 1));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            s.b[8 * h + u] = b(min(32 * h + bk, terms - 1), bj + 8 * u);
+    }
+}
+
+// acc += A·B over the staged terms, through LDS in two chunks of 32. Four waves of 32 × 32 over the tile, 16 × 16 × 4
+// MFMA blocks.
+template <typename T>
+__device__ inline void multiplyTileTerms(typename Mfma<T>::Acc (&acc)[2][2], T (*As)[TILE + 4], T (*Bs)[TILE + 4],
+                                         const TileTerms<T>& s, int rowsA, int terms)
+{
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int row = tid & (TILE - 1), kq = tid >> 6; // A: row, terms kq + 4u
-    const int bk = tid & (KC - 1), bj = tid >> 5;    // B: term bk, columns bj + 8u
-    for (int p0 = 0; p0 < TILE; p0 += KC)
+    const int row = tid & (TILE - 1), kq = tid >> 6;
+    const int bk = tid & 31, bj = tid >> 5;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
     {
-        T av[KC / 4], bv[TILE / 8];
-#pragma unroll
-        for (int u = 0; u < KC / 4; ++u)
-            av[u] = a(min(row, rowsA - 1), min(p0 + kq + 4 * u, terms - 1));
-#pragma unroll
-        for (int u = 0; u < TILE / 8; ++u)
-            bv[u] = b(min(p0 + bk, terms - 1), bj + 8 * u);
         __syncthreads(); // the previous chunk's products have read the LDS
 #pragma unroll
-        for (int u = 0; u < KC / 4; ++u)
-            As[kq + 4 * u][row] = row < rowsA && p0 + kq + 4 * u < terms ? pinned(av[u]) : T(0);
+        for (int u = 0; u < 8; ++u)
+            As[kq + 4 * u][row] = row < rowsA && 32 * h + kq + 4 * u < terms ? pinned(s.a[8 * h + u]) : T(0);
 #pragma unroll
-        for (int u = 0; u < TILE / 8; ++u)
-            Bs[bk][bj + 8 * u] = p0 + bk < terms ? pinned(bv[u]) : T(0);
+        for (int u = 0; u < 8; ++u)
+            Bs[bk][bj + 8 * u] = 32 * h + bk < terms ? pinned(s.b[8 * h + u]) : T(0);
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < KC; kk += 4)
+        for (int kk = 0; kk < 32; kk += 4)
         {
             const int ka = kk + (lane >> 4);
 #pragma unroll
@@ -1024,9 +1045,11 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
                     X[(i0 + lane) + int64_t(i0 + j) * c] = inv[lane + j * ib];
             return;
         }
+        TileTerms<T> terms;
+        loadTileTerms<T>(terms, ib, ib, [&](int i, int t) { return inv[i + t * ib]; },
+                         [&](int t, int j) { return X[(i0 + t) + int64_t(j0 + j) * c]; });
         zeroTile<T>(acc);
-        tileProduct<T>(acc, As, Bs, ib, ib, [&](int i, int t) { return inv[i + t * ib]; },
-                       [&](int t, int j) { return X[(i0 + t) + int64_t(j0 + j) * c]; });
+        multiplyTileTerms<T>(acc, As, Bs, terms, ib, ib);
         __syncthreads(); // every wave has read T_IJ before any writes X_IJ over it
     }
     else
@@ -1049,23 +1072,27 @@ __global__ __launch_bounds__(256) void triangleInverseKernel(DevicePlan p, const
         }
         else
             zeroTile<T>(acc);
-        for (int Kt = K0; Kt <= K; ++Kt)
+        // The operands of both steps are loaded before the first product (step Kt has a term when J <= Kt; J <= K).
+        TileTerms<T> terms[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
         {
-            if (J > Kt)
-                continue; // no term of step Kt (the same for the whole workgroup)
-            if (Kt > K0)
-                __syncthreads(); // the previous product has read its staged operands
+            const int Kt = K0 + e;
+            if (Kt > K || J > Kt)
+                continue; // the same for the whole workgroup
             const int kt0 = TILE * Kt;
             const T* invKt = linv + p.linvOff[s] + int64_t(kt0) * TILE; // 64 × 64 (block Kt is full)
             if (J < Kt)
-                tileProduct<T>(acc, As, Bs, ib, TILE,
-                               [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
-                               [&](int t, int j) { return X[(kt0 + t) + int64_t(j0 + j) * c]; });
+                loadTileTerms<T>(terms[e], ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
+                                 [&](int t, int j) { return X[(kt0 + t) + int64_t(j0 + j) * c]; });
             else
-                tileProduct<T>(acc, As, Bs, ib, TILE,
-                               [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
-                               [&](int t, int j) { return invKt[t + j * TILE]; });
+                loadTileTerms<T>(terms[e], ib, TILE, [&](int i, int t) { return -L[(i0 + i) + int64_t(kt0 + t) * n]; },
+                                 [&](int t, int j) { return invKt[t + j * TILE]; });
         }
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+            if (K0 + e <= K && J <= K0 + e)
+                multiplyTileTerms<T>(acc, As, Bs, terms[e], ib, TILE);
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
